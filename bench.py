@@ -1,0 +1,193 @@
+"""PD14 real-time factor and synaptic events/s on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 1000] [--no-cpu]
+
+Workload: the full-scale Potjans & Diesmann (2014) microcircuit of naer2's test/networks/PD14 (77,169 LIF
+neurons with exponential current synapses, ~298.8 M synapses by the reference's fixed-total-number rule,
+Poisson background, dt = 0.1 ms), built with seed 0 from synthetic random draws.  A step is one fused
+kernel launch advancing the whole network by dt; `steps` of them are timed (default 10,000 = 1 s of
+biological time) after `warmup` untimed ones, replayed from captured hipGraphs with every input resident
+in HBM.
+
+value: real-time factor = simulated (bio) seconds / wall seconds.  Also reported: synaptic events/s
+(links delivered per wall second), the roofline of the step kernel (algorithmic bytes per launch over its
+average duration: HIP events recorded on the launch stream around the timed region, divided by the
+launches in it), and the CPU oracle's rate on this host's
+cores over a bounded sample of the same workload, from the same state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _events_and_spikes(rt, step_lo: int, step_hi: int, out_deg: torch.Tensor) -> tuple[int, int]:
+    """(links delivered, spikes) for the spikes of steps [step_lo, step_hi): each spike of source s is
+    delivered over its out-degree."""
+    words = rt.raster_bits(step_lo, step_hi)  # int64 [steps, G]
+    G = rt.n_groups
+    gf = rt.layout.group_first.to(torch.int64)
+    cnt = (rt.layout.group_info & 0xFFFF).to(torch.int64)
+    lane = torch.arange(64, device=words.device)
+    valid = lane[None, :] < cnt[:, None]
+    nid = (gf[:, None] + lane[None, :]).clamp(max=rt.n - 1)
+    deg = torch.where(valid, out_deg[nid], torch.zeros_like(nid))  # [G, 64]
+    ev = 0
+    sp = 0
+    for a in range(0, words.shape[0], 500):
+        w = words[a:a + 500]
+        bits = ((w[:, :, None] >> lane[None, None, :]) & 1).to(torch.int64)  # [k, G, 64]
+        ev += int((bits * deg[None]).sum())
+        sp += int(bits.sum())
+    return ev, sp
+
+
+def cpu_baseline(rt, state_snapshot, step0: int, n_steps: int) -> dict:
+    """The CPU oracle (single thread, this host) on n_steps of the same workload from the same state.
+    Returns its rate and whether its raster equals the GPU's for those steps."""
+    from oracle import oracle
+    from naer2_amd.params import params_bytes
+    c = rt.canon.to("cpu")
+    v, isyn, ref, ring, prev = state_snapshot
+    st = oracle.State(rt.n, rt.d_slots, v, isyn, ref, ring, prev)
+    t0 = time.perf_counter()
+    raster, events = oracle.run(oracle.LIF_EXP, rt.pop_first, params_bytes(rt.params), rt.d_slots, rt.net.seed,
+                                rt.net.instance, c.src_ptr.numpy(), c.tgt.numpy(), c.w.numpy(), c.d.numpy(), st,
+                                step0, n_steps, True)
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "events": events, "raster": raster, "v": st.v}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--window", type=int, default=100, help="steps per captured hipGraph")
+    ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build "
+              "(see DESIGN.md for how it would shard)", file=sys.stderr)
+        return 2
+    if args.window <= 0 or args.steps % args.window or args.warmup % args.window:
+        print("--steps and --warmup must be multiples of --window", file=sys.stderr)
+        return 2
+
+    from naer2_amd.networks import pd14
+
+    t_build = time.perf_counter()
+    net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
+    rt = net.build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    print(f"[bench] built PD14: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, "
+          f"{rt.d_slots} ring slots in {t_build:.1f} s", file=sys.stderr, flush=True)
+    net.record = True
+    rt._ensure_slots(args.warmup + args.steps + 400)
+
+    net.run(args.warmup, graph_window=args.window)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    s0 = rt.step
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    net.run(args.steps, graph_window=args.window)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    s1 = rt.step
+    print(f"[bench] timed {args.steps} steps: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream",
+          file=sys.stderr, flush=True)
+
+    out_deg = rt.canon.out_degree()
+    events, spikes_timed = _events_and_spikes(rt, s0 - 1, s1 - 1, out_deg)
+    bio_s = args.steps * rt.dt / 1000.0
+    rtf = bio_s / wall
+    ms_per_step = wall * 1000.0 / args.steps
+
+    # roofline of the step kernel: algorithmic bytes per launch (N (8 n_state + 8) + 8 E + 8 S, n_state = 3)
+    n_state = 3
+    bytes_per_step = rt.n * (8 * n_state + 8) + 8 * events / args.steps + 8 * spikes_timed / args.steps
+    k_ms = gpu_ms / args.steps  # HIP events on the launch stream over the timed region, per launch
+    achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "pd14_step_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+
+    rates = net.rates(t_skip_ms=(s0 * rt.dt))
+    rates_hz = {name: round(rates[p], 3) for name, p in pops.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        snap = (rt.v.cpu().numpy(), rt.isyn.cpu().numpy(), rt.ref.cpu().numpy(), rt.ring.cpu().numpy(),
+                rt.raster(rt.step - 1, rt.step)[0].astype(np.uint8))
+        c0 = rt.step
+        net.run(args.cpu_steps, graph_window=0)
+        torch.cuda.synchronize()
+        gpu_raster = rt.raster(c0, c0 + args.cpu_steps)
+        res = cpu_baseline(rt, snap, c0, args.cpu_steps)
+        same = bool(np.array_equal(gpu_raster, res["raster"])) and bool(
+            np.array_equal(rt.v.cpu().numpy().view(np.uint32), res["v"].view(np.uint32)))
+        cpu_bio = args.cpu_steps * rt.dt / 1000.0
+        cpu = {"value": round(cpu_bio / res["wall_s"], 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps ({cpu_bio * 1000:.0f} ms bio) of the same PD14 network from the "
+                         f"GPU state after step {c0}; single-thread C oracle (-O2); "
+                         f"{res['events'] / res['wall_s']:.3e} events/s; raster and V bit-exact vs GPU: {same}",
+               "events_per_s": round(res["events"] / res["wall_s"], 1), "bit_exact_vs_gpu": same}
+        print(f"[bench] cpu oracle {args.cpu_steps} steps: {res['wall_s']:.1f} s, bit-exact {same}",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "PD14 real-time factor + synaptic events/s",
+        "value": round(rtf, 4),
+        "unit": "bio_s/wall_s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pd14 full scale (config 1): 77169 LIF-exp neurons, 8 populations, "
+                               f"{rt.canon.n_links} synapses, Poisson background, dt 0.1 ms, "
+                               f"{args.steps} steps timed, seed {args.seed}",
+                   "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
+                   "graph_window": args.window, "parallelism": "single GPU"},
+        "events_per_s": round(events / wall, 1),
+        "spikes_per_step": round(spikes_timed / args.steps, 3),
+        "events_per_step": round(events / args.steps, 1),
+        "rates_hz": rates_hz,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel_avg_us": round(k_ms * 1000.0, 3),
+                     "algorithmic_bytes_per_launch": round(bytes_per_step, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,145 @@
+/*
+ * naer2_hip.h -- the C-ABI of libnaer2_hip.so, the MI355X (gfx950) execution backend of naer2's
+ * time-driven spiking-network step.
+ *
+ * One call of n2h_step advances a network by whole steps of dt.  Each step is ONE fused kernel launch:
+ *   1. pull intake + delivery of the previous step's spikes into the delayed-input ring
+ *      (each target wavefront gathers the group segments of the spiking sources, in source order),
+ *   2. the ring slot that is due is read and cleared,
+ *   3. the LIF update (delta synapse, or exponential current synapse with the exact propagator)
+ *      with an in-kernel Poisson background drawn from Philox4x32-10,
+ *   4. the threshold, and the spikes written as one 64-bit word per target group (the step's bitmap).
+ *
+ * What each entry point replaces in the reference (file:line in MatteoSchmider/naer2):
+ *   n2h_plan_create / n2h_plan_destroy  naer2/runtime.py:213-323 (PopulationRuntime.__init__ / connect:
+ *                                       state, ring, plan, history of one population) and
+ *                                       naer2/transpiler/templates/fused.py:121-172 (FusedPlan)
+ *   n2h_bind_state                      naer2/runtime.py:490-504 (_launch_args: one pointer per dynamic attr)
+ *   n2h_bind_ring                       naer2/ring.py:132-245 (Ring) and naer2/history.py:76-130
+ *                                       (EmissionHistory: the spikes of a run)
+ *   n2h_step                            naer2/runtime.py:506-526 (_launch) and naer2/fusion.py:137-141
+ *                                       (FusedRound.launch: every population of a round in one grid)
+ *   n2h_graph_create / _launch / _destroy
+ *                                       naer2/graph.py:35-136 (WindowGraph: a captured window of rounds)
+ *   n2h_set_step / n2h_get_step         naer2/schedule.py:30-36 (write_entry: the run's round and times)
+ *   n2h_status                          naer2/runtime.py:724-735 (check: flags raised by the kernel)
+ *   n2h_last_error                      (error text of the last failed call; the reference raises RuntimeError)
+ *
+ * Conventions.  Every function returns an int status: N2H_OK (0) or a negative N2H_E* code; the text of
+ * the last error is copied out by n2h_last_error.  No exception crosses the ABI.  Device buffers are owned
+ * by the caller (PyTorch-ROCm tensors) and passed as plain pointers; the library allocates only its small
+ * per-plan parameter block and step counter.  Calls that take a stream only enqueue work on it and never
+ * synchronize, so they can be captured into a hipGraph.  A plan is not safe for concurrent calls from two
+ * host threads.
+ */
+#ifndef NAER2_HIP_H
+#define NAER2_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define N2H_ABI_VERSION 1
+
+/* status codes */
+#define N2H_OK 0
+#define N2H_EINVAL (-1)   /* a bad argument (null pointer, size, layout) */
+#define N2H_EHIP (-2)     /* a HIP runtime call failed */
+#define N2H_ESTATE (-3)   /* the plan is missing a binding */
+#define N2H_ENOMEM (-4)   /* a host or device allocation failed */
+
+/* model families (one per plan) */
+#define N2H_MODEL_LIF_DELTA 1 /* delta-synapse LIF: V = V*decay + input (tools/brunel_reference.py:34-48) */
+#define N2H_MODEL_LIF_EXP 2   /* exp-synapse LIF, exact propagator (test/networks/PD14/neuron.py:37-60) */
+
+#define N2H_GROUP 64        /* neurons per target group: one wavefront */
+#define N2H_POISSON_K 32    /* entries of the Poisson inversion table */
+#define N2H_MAX_POPS 64
+#define N2H_MAX_SLOTS 128   /* ring slots = the largest delay in steps */
+
+/* Parameters of one population (fp32 constants precomputed on the host for the fixed dt). */
+typedef struct n2h_pop_params {
+    int32_t ref_steps;   /* steps a neuron stays refractory after a spike */
+    int32_t reserved;
+    float decay;         /* LIF_DELTA: exp(-dt/tau_m); LIF_EXP: P_V = exp(-dt/tau_m) */
+    float p_i;           /* LIF_EXP: P_I = exp(-dt/tau_syn) */
+    float p_vi;          /* LIF_EXP: R tau_syn/(tau_m - tau_syn) (P_V - P_I) */
+    float v_rest;        /* LIF_EXP: resting potential */
+    float v_th;          /* threshold: LIF_DELTA fires at V >= v_th, LIF_EXP at V > v_th */
+    float v_reset;       /* potential after a spike (and during refractoriness for LIF_EXP) */
+    float j_ext;         /* weight of one background Poisson event */
+    float reserved_f;
+    /* Poisson background: the count is the number of j with u >= poisson_thr[j], u a 32-bit Philox word;
+     * poisson_thr[j] = min(floor(2^32 CDF(j)), 2^32-1), non-decreasing (all 0xFFFFFFFF for no drive). */
+    uint32_t poisson_thr[N2H_POISSON_K];
+} n2h_pop_params;
+
+/* The network: neurons numbered 0..n_neurons-1, population p owning [pop_first[p], pop_first[p+1]). */
+typedef struct n2h_model_desc {
+    int32_t model;                 /* N2H_MODEL_* */
+    int32_t n_neurons;
+    int32_t n_pops;
+    int32_t d_slots;               /* ring slots: the largest delay in steps (every delay in [1, d_slots]) */
+    const int32_t* pop_first;      /* host [n_pops + 1] */
+    const n2h_pop_params* pops;    /* host [n_pops] */
+    uint64_t seed;                 /* Philox key */
+    uint32_t instance;             /* Philox counter word 3 (batched instances draw distinct streams) */
+    uint32_t reserved;
+} n2h_model_desc;
+
+/* Group-segment synapse tables, on the device.  Target group g covers neurons
+ * [group_first[g], group_first[g] + (group_info[g] & 0xffff)) of population group_info[g] >> 16;
+ * groups never straddle populations.  The links of source s into group g are
+ * links[seg_off[s * n_groups + g] .. seg_off[s * n_groups + g + 1]), in canonical link order, each
+ * packed as {x = delay_steps << 16 | target - group_first[g], y = fp32 weight bits}. */
+typedef struct n2h_tables {
+    int32_t n_groups;
+    int32_t reserved;
+    const int32_t* group_first;    /* device [n_groups] */
+    const int32_t* group_info;     /* device [n_groups] */
+    const int64_t* seg_off;        /* device [n_neurons * n_groups + 1] */
+    const uint32_t* links;         /* device [2 * n_links] */
+    int64_t n_links;
+} n2h_tables;
+
+typedef struct n2h_plan n2h_plan;
+typedef struct n2h_graph n2h_graph;
+
+int n2h_version(void);
+int n2h_last_error(char* buf, size_t len);
+
+/* Validates desc and tables, copies the population parameters to the device (current device). */
+int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_plan** out);
+int n2h_plan_destroy(n2h_plan* plan);
+
+/* State, fp32/int32 [n_neurons] device tensors.  i_syn is required for LIF_EXP and ignored for LIF_DELTA. */
+int n2h_bind_state(n2h_plan* plan, float* v, float* i_syn, int32_t* ref);
+
+/* ring: fp32 [d_slots][n_neurons], the input arriving at step n in slot n % d_slots.
+ * spike_bits: uint64 [bit_slots][n_groups], step n's spikes in slot n % bit_slots (bit_slots >= 2; slot
+ * (step0 - 1) % bit_slots must hold the spikes of step0 - 1, zero before the first step). */
+int n2h_bind_ring(n2h_plan* plan, float* ring, uint64_t* spike_bits, int64_t bit_slots);
+
+/* The device step counter: the next step n2h_step computes. */
+int n2h_set_step(n2h_plan* plan, int64_t step, void* stream);
+int n2h_get_step(n2h_plan* plan, int64_t* step);   /* synchronizes with the device */
+
+/* Enqueue n_steps fused step kernels on stream (a hipStream_t; NULL = the null stream) and advance the
+ * counter by n_steps. */
+int n2h_step(n2h_plan* plan, int32_t n_steps, void* stream);
+
+/* A captured hipGraph of n_steps steps (each replay continues from the device counter). */
+int n2h_graph_create(n2h_plan* plan, int32_t n_steps, void* stream, n2h_graph** out);
+int n2h_graph_launch(n2h_graph* graph, void* stream);
+int n2h_graph_destroy(n2h_graph* graph);
+
+/* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */
+int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NAER2_HIP_H */

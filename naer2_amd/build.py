@@ -1,0 +1,43 @@
+"""Builds libnaer2_hip.so in-tree with hipcc for gfx950.
+
+    python -m naer2_amd.build
+
+-ffp-contract=off keeps every fp32 multiply and add separately rounded (the summation and update order
+is part of the results contract); -packed-fp32-ops is disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+SRC = PKG / "csrc" / "naer2_hip.hip"
+OUT = PKG / "libnaer2_hip.so"
+ARCH = os.environ.get("NAER2_OFFLOAD_ARCH", "gfx950")
+
+FLAGS = [
+    "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{ROOT / 'include'}",
+]
+
+
+def build(verbose: bool = False) -> Path:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [hipcc, *FLAGS, str(SRC), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd)}")
+    if verbose:
+        sys.stderr.write(res.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

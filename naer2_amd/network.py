@@ -1,0 +1,373 @@
+"""Network: the populations and connections of one model, built into one plan of libnaer2_hip.so and
+stepped on the GPU.
+
+Reference: naer2/network.py:17-152 (Network, step), naer2/runtime.py:213-526 (PopulationRuntime: state,
+ring, history, launch), naer2/graph.py:35-136 (captured windows).  The reference schedules every
+population separately (one kernel and one stream per population and round, or one fused kernel per round,
+naer2/fusion.py).  Every network here is time-driven on one clock dt (`resolution`), so one step of the
+whole network is one fused kernel launch over all target groups of all populations; windows of steps are
+captured into hipGraphs and replayed.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GROUP, MODEL_LIF_DELTA, MODEL_LIF_EXP
+from .topology import Canonical, GroupLayout, canonical, delay_steps, group_layout, segments
+
+_FAMILY_MODEL = {"lif_delta": MODEL_LIF_DELTA, "lif_exp": MODEL_LIF_EXP}
+_EPS = 1e-9
+
+
+def default_device() -> torch.device:
+    return torch.device("cuda" if torch.cuda.is_available() else "cpu")
+
+
+class Network:
+    def __init__(self, population=None):
+        self.populations: list = [population] if population is not None else []
+        self.connections: list = []
+        self.resolution: float = 0.1      # dt, ms
+        self.seed: int = 0                # connectivity draws and the Philox key
+        self.instance: int = 0            # Philox counter word: distinct streams for batched copies
+        self.device: torch.device = default_device()
+        self.graph_window: int = 100      # steps per captured hipGraph in run() (0: no graphs)
+        self.record: bool = True          # keep every step's spike bitmap (the raster)
+        self.t: float = 0.0
+        self._generator = None
+        self._fuse_rounds = True
+        self._rt: Runtime | None = None
+
+    # -- composition (naer2/network.py:60-66)
+    def merge(self, other: "Network") -> "Network":
+        if other is self:
+            return self
+        for pop in other.populations:
+            object.__setattr__(pop, "network", self)
+            if pop not in self.populations:
+                self.populations.append(pop)
+        self.connections += [c for c in other.connections if c not in self.connections]
+        self._invalidate()
+        return self
+
+    __add__ = __iadd__ = merge
+
+    @property
+    def generator(self) -> torch.Generator:
+        if self._generator is None:
+            self._generator = torch.Generator(device=self.device).manual_seed(int(self.seed))
+        return self._generator
+
+    @property
+    def fuse_rounds(self) -> bool:
+        """Kept from naer2/network.py:42-51: here every step is already one launch for all populations."""
+        return self._fuse_rounds
+
+    @fuse_rounds.setter
+    def fuse_rounds(self, on: bool) -> None:
+        self._fuse_rounds = bool(on)
+
+    def _invalidate(self) -> None:
+        if self._rt is not None:
+            raise RuntimeError("the network is built: change populations and connections before its first step")
+
+    # -- build and run
+    @property
+    def runtime(self) -> "Runtime":
+        if self._rt is None:
+            self._rt = Runtime(self)
+        return self._rt
+
+    def build(self) -> "Runtime":
+        return self.runtime
+
+    @property
+    def step_index(self) -> int:
+        return self._rt.step if self._rt is not None else 0
+
+    def step(self, t_step: float) -> None:
+        """Advance the network by t_step ms, a whole number of steps of `resolution`."""
+        n = t_step / self.resolution
+        if abs(n - round(n)) > 1e-6 * max(1.0, abs(n)):
+            raise ValueError(f"t_step {t_step} is not a whole number of steps of {self.resolution} ms")
+        self.run(int(round(n)))
+
+    def run(self, n_steps: int, graph_window: int | None = None) -> None:
+        rt = self.runtime
+        rt.run(n_steps, self.graph_window if graph_window is None else graph_window)
+        self.t = rt.step * self.resolution
+
+    def spikes(self, pop=None) -> tuple[np.ndarray, np.ndarray]:
+        """(steps, neuron ids) of every recorded spike, ascending by step then id; neuron ids are the
+        population's own (local) indices when pop is given, the network's otherwise."""
+        return self.runtime.spikes(pop)
+
+    def rates(self, t_skip_ms: float = 0.0) -> dict:
+        """Mean rate in Hz of each population over the recorded steps after t_skip_ms."""
+        rt = self.runtime
+        steps, ids = rt.spikes(None)
+        first = int(round(t_skip_ms / self.resolution))
+        dur = (rt.step - first) * self.resolution / 1000.0
+        out = {}
+        keep = steps >= first
+        ids = ids[keep]
+        for pop, lo in zip(rt.pops, rt.pop_first[:-1]):
+            n = ((ids >= lo) & (ids < lo + pop.n)).sum()
+            out[pop] = float(n) / pop.n / dur if dur > 0 else float("nan")
+        return out
+
+
+@dataclass
+class Assembled:
+    """A network's host-side description for one device: populations in creation order, their
+    parameters, the canonical synapse table and the initial state."""
+    pops: list
+    family: str
+    model: int
+    dt: float
+    pop_first: np.ndarray
+    params: list
+    canon: Canonical
+    d_slots: int
+    v: torch.Tensor
+    isyn: torch.Tensor | None
+    ref: torch.Tensor
+
+    @property
+    def n(self) -> int:
+        return int(self.pop_first[-1])
+
+
+def assemble(net: Network, device=None) -> Assembled:
+    """Everything the plan needs, on `device` (default: the network's), without touching the GPU."""
+    pops = sorted(net.populations, key=lambda p: p._order)
+    fams = {p.family for p in pops}
+    if None in fams:
+        bad = [type(p.instance).__name__ for p in pops if p.family is None]
+        raise NotImplementedError(f"no hand-written kernel for node types {bad}: the reference's hook "
+                                  f"transpiler is not part of this backend (nodes.py lists the families)")
+    if len(fams) != 1:
+        raise NotImplementedError(f"one model family per network (got {sorted(fams)})")
+    family = fams.pop()
+    model = _FAMILY_MODEL[family]
+    if len(pops) > _lib.MAX_POPS:
+        raise ValueError(f"at most {_lib.MAX_POPS} populations")
+    dt = float(net.resolution)
+    dev = torch.device(device if device is not None else net.device)
+    sizes = [p.n for p in pops]
+    pop_first = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(pop_first[-1])
+    first_of = {p: int(f) for p, f in zip(pops, pop_first[:-1])}
+    params = [p.instance.params(dt) for p in pops]
+    srcs, tgts, ws, ds = [], [], [], []
+    for c in net.connections:
+        if c._weight is None or c._delay is None:
+            raise ValueError(f"{c!r}: weight and delay must be set")
+        srcs.append(c.src.to(dev) + first_of[c.source])
+        tgts.append(c.tgt.to(dev) + first_of[c.target])
+        ws.append(c.weight.to(dev))
+        ds.append(delay_steps(c.delay.to(dev), dt))
+    cat = (lambda xs, dt_: torch.cat(xs) if xs else torch.zeros(0, dtype=dt_, device=dev))
+    canon = canonical(n, cat(srcs, torch.int64), cat(tgts, torch.int64), cat(ws, torch.float32),
+                      cat(ds, torch.int32))
+    del srcs, tgts, ws, ds
+    v = torch.empty(n, dtype=torch.float32, device=dev)
+    isyn = torch.zeros(n, dtype=torch.float32, device=dev) if model == MODEL_LIF_EXP else None
+    ref = torch.zeros(n, dtype=torch.int32, device=dev)
+    vname = "V" if model == MODEL_LIF_DELTA else "potential"
+    for p, lo in zip(pops, pop_first[:-1]):
+        sl = slice(int(lo), int(lo) + p.n)
+        v[sl] = p._init.get(vname, torch.tensor(float(getattr(p.instance, vname)))).to(dev, torch.float32)
+        ref[sl] = p._init.get("ref", torch.tensor(0)).to(dev, torch.int32)
+        if isyn is not None:
+            isyn[sl] = p._init.get("current", torch.tensor(float(p.instance.current))).to(dev, torch.float32)
+    return Assembled(pops, family, model, dt, pop_first, params, canon, canon.max_delay, v, isyn, ref)
+
+
+class Runtime:
+    """The built network: device tensors, the C plan, captured graphs."""
+
+    def __init__(self, net: Network):
+        self.net = net
+        self.device = torch.device(net.device)
+        if self.device.type != "cuda":
+            raise RuntimeError("naer2_amd runs on the GPU: no CUDA/HIP device is available")
+        a = assemble(net, self.device)
+        self.pops, self.family, self.model, self.dt = a.pops, a.family, a.model, a.dt
+        self.pop_first, self.params, self.canon, self.d_slots = a.pop_first, a.params, a.canon, a.d_slots
+        self.n = a.n
+        self.v, self.isyn, self.ref = a.v, a.isyn, a.ref
+        self.layout: GroupLayout = group_layout([p.n for p in self.pops], self.device)
+        self.n_groups = self.layout.n_groups
+        seg = segments(self.canon, self.layout)
+        self.seg_off, self.links = seg.seg_off, seg.links
+        self.ring = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
+        self.bit_slots = 2
+        self.bits = torch.zeros(self.bit_slots, self.n_groups, dtype=torch.int64, device=self.device)
+        self.record_from = 0
+        self.step = 0
+        self._graphs: dict = {}
+        self._plan = None
+        self._create_plan()
+        for p, lo in zip(self.pops, self.pop_first[:-1]):
+            object.__setattr__(p, "_net_slice", (self, int(lo)))
+
+    # -- the C plan
+    def _create_plan(self) -> None:
+        lib = _lib.load()
+        pops_arr = (_lib.PopParams * len(self.params))(*self.params)
+        first_arr = (ctypes.c_int32 * (len(self.pops) + 1))(*[int(x) for x in self.pop_first])
+        desc = _lib.ModelDesc(model=self.model, n_neurons=self.n, n_pops=len(self.pops), d_slots=self.d_slots,
+                              pop_first=first_arr, pops=pops_arr, seed=int(self.net.seed) & (2**64 - 1),
+                              instance=int(self.net.instance) & 0xFFFFFFFF)
+        tab = _lib.Tables(n_groups=self.n_groups, group_first=self.layout.group_first.data_ptr(),
+                          group_info=self.layout.group_info.data_ptr(), seg_off=self.seg_off.data_ptr(),
+                          links=self.links.data_ptr(), n_links=self.canon.n_links)
+        plan = ctypes.c_void_p()
+        _lib.check(lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan)), "n2h_plan_create")
+        self._plan = plan
+        _lib.check(lib.n2h_bind_state(plan, self.v.data_ptr(), self.isyn.data_ptr() if self.isyn is not None else None,
+                                      self.ref.data_ptr()), "n2h_bind_state")
+        self._bind_ring()
+
+    def _bind_ring(self) -> None:
+        lib = _lib.load()
+        _lib.check(lib.n2h_bind_ring(self._plan, self.ring.data_ptr(), self.bits.data_ptr(), self.bit_slots),
+                   "n2h_bind_ring")
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            lib = _lib.load()
+            for g in self._graphs.values():
+                lib.n2h_graph_destroy(g)
+            if self._plan is not None:
+                lib.n2h_plan_destroy(self._plan)
+        except Exception:
+            pass
+
+    def _ensure_slots(self, n_more: int) -> None:
+        """With recording on, keep every step's bitmap since record_from (and the step before it, which
+        the first recorded step's delivery reads): grow the history to hold the steps to come."""
+        if not self.net.record:
+            return
+        need = self.step + n_more - self.record_from + 1
+        if need <= self.bit_slots:
+            return
+        new_slots = max(need, 2 * self.bit_slots)
+        new = torch.zeros(new_slots, self.n_groups, dtype=torch.int64, device=self.device)
+        keep = torch.arange(max(self.record_from - 1, 0), self.step, device=self.device)
+        if keep.numel():
+            new[keep % new_slots] = self.bits[keep % self.bit_slots]
+        self.bits, self.bit_slots = new, new_slots
+        self._bind_ring()
+
+    def run(self, n_steps: int, graph_window: int = 0) -> None:
+        if n_steps < 0:
+            raise ValueError("n_steps must be >= 0")
+        if n_steps == 0:
+            return
+        self._ensure_slots(n_steps)
+        lib = _lib.load()
+        sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        done = 0
+        if graph_window and n_steps >= graph_window:
+            g = self._graphs.get(graph_window)
+            if g is None:
+                gp = ctypes.c_void_p()
+                _lib.check(lib.n2h_graph_create(self._plan, graph_window, sv, ctypes.byref(gp)), "n2h_graph_create")
+                g = self._graphs[graph_window] = gp
+            while n_steps - done >= graph_window:
+                _lib.check(lib.n2h_graph_launch(g, sv), "n2h_graph_launch")
+                done += graph_window
+        if n_steps > done:
+            _lib.check(lib.n2h_step(self._plan, n_steps - done, sv), "n2h_step")
+        self.step += n_steps
+        if not self.net.record:
+            self.record_from = max(self.step - 1, 0)
+
+    def set_step(self, step: int) -> None:
+        """Move the step counter (the Philox counter and ring slot follow the absolute step)."""
+        lib = _lib.load()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(lib.n2h_set_step(self._plan, int(step), ctypes.c_void_p(stream)), "n2h_set_step")
+        self.step = int(step)
+        self.record_from = int(step)
+
+    # -- state access for populations
+    def _tensor(self, name: str):
+        if name in ("V", "potential"):
+            return self.v
+        if name == "current":
+            if self.isyn is None:
+                raise AttributeError("current: no synaptic current in the delta-synapse model")
+            return self.isyn
+        if name == "ref":
+            return self.ref
+        raise AttributeError(name)
+
+    def _read_state(self, pop, name):
+        lo = dict(zip(self.pops, self.pop_first[:-1]))[pop]
+        return self._tensor(name)[int(lo):int(lo) + pop.n]
+
+    def _write_state(self, pop, name, value):
+        self._read_state(pop, name).copy_(value.to(self.device))
+
+    # -- raster
+    def raster_bits(self, step_lo: int | None = None, step_hi: int | None = None) -> torch.Tensor:
+        """int64 [steps, n_groups] spike words of the recorded steps [step_lo, step_hi)."""
+        lo = self.record_from if step_lo is None else step_lo
+        hi = self.step if step_hi is None else step_hi
+        if lo < self.record_from or hi > self.step or (hi - lo) > self.bit_slots:
+            raise ValueError(f"steps [{lo}, {hi}) are not recorded")
+        idx = torch.arange(lo, hi, device=self.device) % self.bit_slots
+        return self.bits[idx]
+
+    def raster(self, step_lo: int | None = None, step_hi: int | None = None) -> np.ndarray:
+        """bool [steps, n] spikes per step and neuron."""
+        words = self.raster_bits(step_lo, step_hi).cpu().numpy()
+        return bits_to_raster(words, self.layout, self.n)
+
+    def spikes(self, pop=None):
+        lo = self.record_from
+        r = self.raster()
+        steps, ids = np.nonzero(r)
+        steps = steps + lo
+        if pop is not None:
+            first = int(dict(zip(self.pops, self.pop_first[:-1]))[pop])
+            m = (ids >= first) & (ids < first + pop.n)
+            return steps[m], ids[m] - first
+        return steps, ids
+
+
+def bits_to_raster(words: np.ndarray, layout: GroupLayout, n: int) -> np.ndarray:
+    """bool [steps, n] from spike words [steps, n_groups] (bit b of group g = neuron group_first[g] + b)."""
+    words = np.ascontiguousarray(words).astype(np.uint64)
+    bits = np.unpackbits(words.view(np.uint8).reshape(words.shape[0], -1), axis=1, bitorder="little")
+    bits = bits.reshape(words.shape[0], -1, GROUP)
+    gf = layout.group_first.cpu().numpy().astype(np.int64)
+    cnt = (layout.group_info.cpu().numpy() & 0xFFFF).astype(np.int64)
+    out = np.zeros((words.shape[0], n), dtype=bool)
+    for g in range(len(gf)):
+        out[:, gf[g]:gf[g] + cnt[g]] = bits[:, g, :cnt[g]]
+    return out
+
+
+def raster_to_bits(raster: np.ndarray, layout: GroupLayout) -> np.ndarray:
+    """Spike words [steps, n_groups] from a bool raster [steps, n]."""
+    raster = np.asarray(raster, dtype=bool)
+    gf = layout.group_first.cpu().numpy().astype(np.int64)
+    cnt = (layout.group_info.cpu().numpy() & 0xFFFF).astype(np.int64)
+    out = np.zeros((raster.shape[0], len(gf)), dtype=np.uint64)
+    weights = (np.uint64(1) << np.arange(GROUP, dtype=np.uint64))
+    for g in range(len(gf)):
+        blk = raster[:, gf[g]:gf[g] + cnt[g]].astype(np.uint64)
+        out[:, g] = (blk * weights[:cnt[g]]).sum(axis=1, dtype=np.uint64)
+    return out.view(np.int64)

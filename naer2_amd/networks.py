@@ -1,0 +1,108 @@
+"""The two plain workloads, built with the description API.
+
+brunel(): Brunel (2000) model A, time-driven (tools/brunel_reference.py:14-48), with the default
+  parameters N_E=2000, N_I=500, C_E=200, C_I=50, J=0.2 mV, g=5, eta=2, D=1.5 ms, tau=20 ms, theta=20 mV,
+  V_r=10 mV, t_ref=2 ms, dt=0.1 ms.  External drive: Poisson counts per step of mean
+  C_E eta nu_thr dt with nu_thr = theta / (J C_E tau) (test/networks/Brunel/net.py:83-91), weight J.
+  Initial potentials uniform in [0, theta).
+pd14(): the Potjans & Diesmann (2014) microcircuit of test/networks/PD14/net.py:25-108 without the thalamic
+  population, with a Poisson background of K_ext inputs at 8 Hz (J_ext = 87.81 pA) instead of the DC shift
+  of V_rest, 8 populations, fixed-total-number connectivity (connection.py:9-31), weights normal +-10% with
+  the sign kept (clamped), L4E -> L2/3E doubled, inhibitory -4 J; delays normal (E 1.5 +- 0.75 ms,
+  I 0.75 +- 0.375 ms) clipped at 0.1 ms; initial potentials normal per population.
+"""
+from __future__ import annotations
+
+import torch
+
+from .network import Network
+from .nodes import FixedInDegree, FixedTotalNumber, LIFDelta, LIFExp
+
+_J_E = 0.08781
+_J_I = -0.35124
+
+PD14 = {
+    "name": ["E_23", "I_23", "E_4", "I_4", "E_5", "I_5", "E_6", "I_6"],
+    "K_ext": [1600, 1500, 2100, 1900, 2000, 1900, 2900, 2100],
+    "size": [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948],
+    "V_mean": [-68.28, -63.16, -63.33, -63.45, -63.11, -61.66, -66.72, -61.45],
+    "V_std": [5.36, 4.57, 4.74, 4.94, 4.94, 4.55, 5.46, 4.48],
+    # rows = source population, columns = target population (test/networks/PD14/net.py:45-56)
+    "probabilities": [
+        [0.1009, 0.1346, 0.0077, 0.0691, 0.1004, 0.0548, 0.0156, 0.0364],
+        [0.1689, 0.1371, 0.0059, 0.0029, 0.0622, 0.0269, 0.0066, 0.0010],
+        [0.0437, 0.0316, 0.0497, 0.0794, 0.0505, 0.0257, 0.0211, 0.0034],
+        [0.0818, 0.0515, 0.1350, 0.1597, 0.0057, 0.0022, 0.0166, 0.0005],
+        [0.0323, 0.0755, 0.0067, 0.0033, 0.0831, 0.0600, 0.0572, 0.0277],
+        [0.0, 0.0, 0.0003, 0.0, 0.3726, 0.3158, 0.0197, 0.0080],
+        [0.0076, 0.0042, 0.0453, 0.1057, 0.0204, 0.0086, 0.0396, 0.0658],
+        [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2252, 0.1443],
+    ],
+    "weight_mean": [_J_E, _J_I, _J_E, _J_I, _J_E, _J_I, _J_E, _J_I],
+    "delay_mean": [1.5, 0.75, 1.5, 0.75, 1.5, 0.75, 1.5, 0.75],
+}
+
+
+def brunel(N_E: int = 2000, N_I: int = 500, C_E: int = 200, C_I: int = 50, J: float = 0.2, g: float = 5.0,
+           eta: float = 2.0, D: float = 1.5, tau: float = 20.0, theta: float = 20.0, V_r: float = 10.0,
+           t_ref: float = 2.0, dt: float = 0.1, seed: int = 0, device=None, instance: int = 0):
+    """(network, {"E": E, "I": I})."""
+    nu_thr = theta / (J * C_E * tau)              # per ms
+    rate_ext = C_E * eta * nu_thr                  # background events per ms per neuron
+    node = LIFDelta(tau=tau, theta=theta, V_r=V_r, t_ref=t_ref, J_ext=J, rate_ext=rate_ext)
+    E = node * N_E
+    I = LIFDelta(tau=tau, theta=theta, V_r=V_r, t_ref=t_ref, J_ext=J, rate_ext=rate_ext) * N_I
+    net = E.network
+    net.merge(I.network)
+    _configure(net, dt, seed, device, instance)
+    gen = net.generator
+    E.V = torch.rand(N_E, generator=gen, device=gen.device) * theta
+    I.V = torch.rand(N_I, generator=gen, device=gen.device) * theta
+    for target in (E, I):
+        ce = E >> FixedInDegree(C_E) >> target
+        ce.weight, ce.delay = J, D
+        ci = I >> FixedInDegree(C_I) >> target
+        ci.weight, ci.delay = -g * J, D
+    return net, {"E": E, "I": I}
+
+
+def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instance: int = 0,
+         k_ext_scale: float = 1.0):
+    """(network, {name: population}); scale shrinks every population (connectivity follows the rule)."""
+    pops = {}
+    net = None
+    for name, k_ext, size in zip(PD14["name"], PD14["K_ext"], PD14["size"]):
+        n = max(1, int(round(size * scale)))
+        pops[name] = LIFExp(K_ext=int(round(k_ext * k_ext_scale))) * n
+        if net is None:
+            net = pops[name].network
+        else:
+            net.merge(pops[name].network)
+    _configure(net, dt, seed, device, instance)
+    gen = net.generator
+    dev = gen.device
+    for name, vm, vs in zip(PD14["name"], PD14["V_mean"], PD14["V_std"]):
+        p = pops[name]
+        p.potential = vm + vs * torch.randn(p.n, generator=gen, device=dev)
+    for si, src in enumerate(PD14["name"]):
+        for ti, tgt in enumerate(PD14["name"]):
+            prob = PD14["probabilities"][si][ti]
+            if prob == 0.0:
+                continue
+            w_mean = 2 * _J_E if (src == "E_4" and tgt == "E_23") else PD14["weight_mean"][si]
+            conn = pops[src] >> FixedTotalNumber(prob) >> pops[tgt]
+            n_links = conn.shape[0]
+            d_mean = PD14["delay_mean"][si]
+            conn.delay = (d_mean + 0.5 * d_mean * torch.randn(n_links, generator=gen, device=dev)).clamp_(min=0.1)
+            w = w_mean + 0.1 * abs(w_mean) * torch.randn(n_links, generator=gen, device=dev)
+            conn.weight = w.clamp_(min=0.0) if w_mean >= 0 else w.clamp_(max=0.0)
+    return net, pops
+
+
+def _configure(net: Network, dt: float, seed: int, device, instance: int) -> None:
+    net.resolution = dt
+    net.seed = seed
+    net.instance = instance
+    if device is not None:
+        net.device = torch.device(device)
+    net._generator = None

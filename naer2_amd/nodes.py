@@ -1,0 +1,111 @@
+"""The node types and connection patterns this backend runs, with the reference models' names and
+parameters.
+
+Node families (one hand-written kernel each, naer2_amd/csrc/naer2_hip.hip):
+  LIFDelta  "lif_delta"  current-based delta-synapse LIF, the time-driven Brunel model A of
+                         tools/brunel_reference.py:14-48 (Brunel_neuron's parameters, test/networks/Brunel/
+                         neuron.py:23-30), with a Poisson drive of rate_ext events/ms of weight J_ext.
+  LIFExp    "lif_exp"    LIF with exponential current synapses and the exact propagator
+                         (test/networks/PD14/neuron.py:6-60), with a Poisson background of K_ext inputs at
+                         nu_ext of weight J_ext instead of the reference's DC shift of V_rest (neuron.py:28-29).
+Patterns:
+  FixedInDegree(K)          every target draws K sources uniformly with replacement
+                            (test/networks/Brunel/connection.py:7-33, tools/brunel_reference.py:20-21)
+  FixedTotalNumber(p)       every source draws round(Q / N_x) targets uniformly with replacement,
+                            Q = ln(1 - p) / ln(1 - 1/(N_x N_y)) (test/networks/PD14/connection.py:9-31)
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .dsl import node, pattern
+from .params import lif_delta_params, lif_exp_params
+
+
+@node
+class LIFDelta:
+    family = "lif_delta"
+    state = ("V", "ref")
+
+    def __init__(self, tau: float = 20.0, theta: float = 20.0, V_r: float = 10.0, t_ref: float = 2.0,
+                 J_ext: float = 0.0, rate_ext: float = 0.0):
+        self.tau, self.theta, self.V_r, self.t_ref = tau, theta, V_r, t_ref
+        self.J_ext = J_ext          # mV per background event
+        self.rate_ext = rate_ext    # background events per ms (all external inputs together)
+        self.V = 0.0
+        self.ref = 0
+
+    def params(self, dt: float):
+        return lif_delta_params(dt, self.tau, self.theta, self.V_r, self.t_ref, self.J_ext, self.rate_ext * dt)
+
+
+@node
+class LIFExp:
+    family = "lif_exp"
+    state = ("potential", "current", "ref")
+
+    def __init__(self, K_ext: int = 0, t_mem: float = 10.0, t_syn: float = 0.5, t_ref: float = 2.0,
+                 R_mem: float = 40.0, E_leak: float = -65.0, V_thresh: float = -50.0, V_reset: float = -65.0,
+                 nu_ext: float = 8e-3, J_ext: float = 0.08781):
+        self.K_ext = K_ext
+        self.t_mem, self.t_syn, self.t_ref, self.R_mem = t_mem, t_syn, t_ref, R_mem
+        self.E_leak, self.V_thresh, self.V_reset = E_leak, V_thresh, V_reset
+        self.nu_ext = nu_ext        # spikes/ms per background input
+        self.J_ext = J_ext          # nA per background spike
+        self.potential = V_reset
+        self.current = 0.0
+        self.ref = 0
+
+    def params(self, dt: float):
+        return lif_exp_params(dt, self.t_mem, self.t_syn, self.R_mem, self.t_ref, self.E_leak, self.V_thresh,
+                              self.V_reset, self.J_ext, self.K_ext * self.nu_ext * dt)
+
+
+@pattern
+class FixedInDegree:
+    def __init__(self, K: int):
+        if K < 0:
+            raise ValueError("K must be >= 0")
+        self.K = int(K)
+
+    def links(self, n_src: int, n_tgt: int, generator: torch.Generator):
+        dev = generator.device
+        if n_src == 0 or self.K == 0:
+            z = torch.zeros(0, dtype=torch.int64, device=dev)
+            return z, z
+        src = torch.randint(0, n_src, (n_tgt * self.K,), generator=generator, device=dev)
+        tgt = torch.arange(n_tgt, device=dev).repeat_interleave(self.K)
+        return src, tgt
+
+    def __call__(self, position, source_region, target_region=None):
+        raise NotImplementedError("FixedInDegree draws per target: use links()")
+
+
+@pattern
+class FixedTotalNumber:
+    def __init__(self, probability: float):
+        if not 0.0 <= probability < 1.0:
+            raise ValueError("probability must be in [0, 1)")
+        self.probability = probability
+
+    def per_source(self, n_src: int, n_tgt: int) -> int:
+        if self.probability == 0.0 or n_src == 0 or n_tgt == 0:
+            return 0
+        q = math.log(1 - self.probability) / math.log(1 - 1 / (n_src * n_tgt))
+        return round(q / n_src)
+
+    def links(self, n_src: int, n_tgt: int, generator: torch.Generator):
+        dev = generator.device
+        k = self.per_source(n_src, n_tgt)
+        src = torch.arange(n_src, device=dev).repeat_interleave(k)
+        tgt = torch.randint(0, max(n_tgt, 1), (n_src * k,), generator=generator, device=dev)
+        return src, tgt
+
+    def __call__(self, position, source_region, target_region=None):
+        from .dsl import Points
+        n_src = len(source_region)
+        n_tgt = len(target_region if target_region is not None else source_region)
+        k = self.per_source(n_src, n_tgt)
+        return Points(torch.randint(0, max(n_tgt, 1), (k, 1)))

@@ -1,0 +1,132 @@
+"""Synapse tables: the canonical CSR by source, the target-group layout, and the group segments the step
+kernel reads.
+
+Reference: naer2/topology.py:98-183 (Topology, canonical CSR by source) and :291-557
+(SimulationTopology: links sorted by (source, target group, delay bucket) into segments with a directory
+per source).  Here delays are whole steps fixed on the host (d_steps = max(1, round(d / dt))) instead of
+bucket means, a target group is one wavefront of 64 neurons, and the per-source directory is a dense
+(source, group) offset table -- 8 bytes per (source, group) pair, which HBM holds easily, and two adjacent
+loads per spiking source instead of the reference's three dependent ones (fused.py:252-267).
+
+Canonical link order: links sorted stably by source; for one source, the order in which its connections
+were made and, inside one connection, the order its pattern produced them.  The summation order of the
+step (ascending canonical link index per target) is defined on this order.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from ._lib import GROUP, MAX_SLOTS
+
+
+@dataclass
+class Canonical:
+    """Canonical CSR by source: the links of source s are [src_ptr[s], src_ptr[s+1])."""
+    n: int
+    src_ptr: torch.Tensor  # int64 [n + 1]
+    tgt: torch.Tensor      # int32 [E]
+    w: torch.Tensor        # float32 [E]
+    d: torch.Tensor        # int32 [E], delay in steps, >= 1
+
+    @property
+    def n_links(self) -> int:
+        return int(self.tgt.numel())
+
+    @property
+    def max_delay(self) -> int:
+        return int(self.d.max()) if self.n_links else 1
+
+    def out_degree(self) -> torch.Tensor:
+        return self.src_ptr[1:] - self.src_ptr[:-1]
+
+    def to(self, device) -> "Canonical":
+        return Canonical(self.n, self.src_ptr.to(device), self.tgt.to(device), self.w.to(device),
+                         self.d.to(device))
+
+
+def delay_steps(delay_ms: torch.Tensor, dt: float) -> torch.Tensor:
+    """int32 delay steps max(1, round(d / dt)), rounded half to even in float64."""
+    steps = torch.round(delay_ms.to(torch.float64) / dt).clamp_(min=1)
+    if steps.numel() and float(steps.max()) > MAX_SLOTS:
+        raise ValueError(f"a delay of {float(steps.max())} steps exceeds the ring's {MAX_SLOTS} slots")
+    return steps.to(torch.int32)
+
+
+def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: torch.Tensor) -> Canonical:
+    """The canonical CSR of links given in any source order (a stable sort by source keeps each source's
+    links in their given order)."""
+    src = src.to(torch.int64)
+    if src.numel() and (int(src.min()) < 0 or int(src.max()) >= n):
+        raise ValueError("link source out of range")
+    if tgt.numel() and (int(tgt.min()) < 0 or int(tgt.max()) >= n):
+        raise ValueError("link target out of range")
+    if d.numel() and int(d.min()) < 1:
+        raise ValueError("link delays must be at least one step")
+    order = torch.argsort(src, stable=True)
+    counts = torch.bincount(src, minlength=n)
+    src_ptr = torch.zeros(n + 1, dtype=torch.int64, device=src.device)
+    src_ptr[1:] = torch.cumsum(counts, 0)
+    return Canonical(n, src_ptr, tgt[order].to(torch.int32), w[order].to(torch.float32),
+                     d[order].to(torch.int32))
+
+
+@dataclass
+class GroupLayout:
+    """Target groups of at most 64 neurons of one population each."""
+    n: int
+    group_first: torch.Tensor  # int32 [G]
+    group_info: torch.Tensor   # int32 [G]: count | pop << 16
+    group_of: torch.Tensor     # int64 [n]
+    local_of: torch.Tensor     # int64 [n]
+
+    @property
+    def n_groups(self) -> int:
+        return int(self.group_first.numel())
+
+
+def group_layout(pop_sizes: list[int], device="cpu") -> GroupLayout:
+    firsts, infos = [], []
+    base = 0
+    for p, size in enumerate(pop_sizes):
+        for g0 in range(0, size, GROUP):
+            firsts.append(base + g0)
+            infos.append(min(GROUP, size - g0) | (p << 16))
+        base += size
+    group_first = torch.tensor(firsts, dtype=torch.int32)
+    group_info = torch.tensor(infos, dtype=torch.int32)
+    counts = (group_info & 0xFFFF).to(torch.int64)
+    group_of = torch.repeat_interleave(torch.arange(len(firsts), dtype=torch.int64), counts)
+    local_of = torch.arange(base, dtype=torch.int64) - group_first.to(torch.int64)[group_of]
+    return GroupLayout(base, group_first.to(device), group_info.to(device), group_of.to(device),
+                       local_of.to(device))
+
+
+@dataclass
+class Segments:
+    seg_off: torch.Tensor  # int64 [n * G + 1]
+    links: torch.Tensor    # int32 [E, 2]: (delay << 16 | local target, weight bits)
+
+
+def segments(canon: Canonical, layout: GroupLayout) -> Segments:
+    """Group segments: the links of source s into group g at [seg_off[s*G + g], seg_off[s*G + g + 1]), in
+    canonical order (a stable partition of each source's links by target group)."""
+    dev = canon.tgt.device
+    n, G = canon.n, layout.n_groups
+    group_of = layout.group_of.to(dev)
+    local_of = layout.local_of.to(dev)
+    src = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int64), canon.out_degree())
+    tg = canon.tgt.to(torch.int64)
+    key = src * G + group_of[tg]
+    del src
+    order = torch.argsort(key, stable=True)
+    counts = torch.bincount(key, minlength=n * G)
+    del key
+    seg_off = torch.zeros(n * G + 1, dtype=torch.int64, device=dev)
+    seg_off[1:] = torch.cumsum(counts, 0)
+    del counts
+    x = (canon.d.to(torch.int64)[order] << 16) | local_of[tg[order]]
+    y = canon.w[order].contiguous().view(torch.int32)
+    links = torch.stack([x.to(torch.int32), y], 1).contiguous()
+    return Segments(seg_off, links)

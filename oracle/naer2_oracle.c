@@ -1,0 +1,145 @@
+/*
+ * naer2_oracle.c -- TEST INFRASTRUCTURE ONLY: the CPU restatement that checks libnaer2_hip.so.
+ * Built with -ffp-contract=off so every fp32 operation rounds once, as in the HIP kernels.
+ * The algorithm, the followed reference lines and the deviations are listed in naer2_oracle.h.
+ */
+#include "naer2_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as 1, 2, 3", SC'11):
+ * ten rounds of two 32x32->64 multiplies, the key bumped by the Weyl constants between rounds. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+int or_poisson_count(uint32_t u, const uint32_t* thr) {
+    int k = 0;
+    for (int j = 0; j < OR_POISSON_K; ++j) k += (u >= thr[j]) ? 1 : 0;
+    return k;
+}
+
+static int pop_of(int32_t i, int32_t n_pops, const int32_t* pop_first) {
+    int p = 0;
+    while (p + 1 < n_pops && i >= pop_first[p + 1]) ++p;
+    return p;
+}
+
+int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops,
+           int32_t d_slots, uint64_t seed, uint32_t instance,
+           const int64_t* src_ptr, const int32_t* tgt, const float* w, const int32_t* delay,
+           float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+           int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events) {
+    if (n_neurons < 0 || n_pops < 1 || d_slots < 1 || n_steps < 0) return -1;
+    if (model != OR_LIF_DELTA && model != OR_LIF_EXP) return -1;
+    if (model == OR_LIF_EXP && !isyn) return -1;
+    const int64_t N = n_neurons, D = d_slots;
+    /* per-(slot, target) partial sums of one emission step, with a touched list */
+    float* acc = (float*)malloc(sizeof(float) * (size_t)(N * D + 1));
+    uint8_t* touched = (uint8_t*)calloc((size_t)(N * D + 1), 1);
+    int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N * D + 1));
+    int32_t* pop = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N + 1));
+    if (!acc || !touched || !list || !pop) {
+        free(acc); free(touched); free(list); free(pop);
+        return -1;
+    }
+    for (int64_t i = 0; i < N; ++i) pop[i] = pop_of((int32_t)i, n_pops, pop_first);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t n_ev = 0;
+    int rc = 0;
+
+    for (int32_t k = 0; k < n_steps; ++k) {
+        const int64_t s = step0 + k;
+        /* 1. delivery of the spikes of step s-1: sources ascending, links in canonical order */
+        int64_t n_list = 0;
+        for (int64_t src = 0; src < N; ++src) {
+            if (!prev_spk[src]) continue;
+            for (int64_t l = src_ptr[src]; l < src_ptr[src + 1]; ++l) {
+                const int32_t d = delay[l];
+                if (d < 1 || d > d_slots) { rc = -1; goto done; }
+                const int64_t cell = ((s - 1 + d) % D) * N + tgt[l];
+                if (!touched[cell]) {
+                    touched[cell] = 1;
+                    list[n_list++] = cell;
+                    acc[cell] = 0.0f + w[l];
+                } else if (accum_mode == 1) {
+                    acc[cell] = acc[cell] * 0.5f + w[l];
+                } else {
+                    acc[cell] = acc[cell] + w[l];
+                }
+                ++n_ev;
+            }
+        }
+        for (int64_t j = 0; j < n_list; ++j) {
+            const int64_t cell = list[j];
+            ring[cell] = accum_mode == 1 ? ring[cell] * 0.5f + acc[cell] : ring[cell] + acc[cell];
+            touched[cell] = 0;
+        }
+        /* 2-4. read and clear the due slot, background, LIF update, threshold */
+        const int64_t slot = (s % D) * N;
+        for (int64_t i = 0; i < N; ++i) {
+            const or_pop* P = &pops[pop[i]];
+            const float x = ring[slot + i];
+            ring[slot + i] = 0.0f;
+            const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)((uint64_t)s >> 32), instance};
+            uint32_t r[4];
+            or_philox4x32_10(ctr, key, r);
+            const int cnt = or_poisson_count(r[0], P->poisson_thr);
+            const float in = x + P->j_ext * (float)cnt;
+            uint8_t fired = 0;
+            if (model == OR_LIF_DELTA) {
+                if (ref[i] > 0) {
+                    ref[i] -= 1;
+                } else {
+                    const float vd = v[i] * P->decay;
+                    v[i] = vd + in;
+                    if (v[i] >= P->v_th) {
+                        fired = 1;
+                        v[i] = P->v_reset;
+                        ref[i] = P->ref_steps;
+                    }
+                }
+            } else {
+                /* receive (current += input), then update(t_end) as test/fused_reference.py:86-99 */
+                const float I = isyn[i] + in;
+                if (ref[i] > 0) {
+                    v[i] = P->v_reset;
+                    ref[i] -= 1;
+                } else {
+                    const float dv = v[i] - P->v_rest;
+                    const float pv = P->decay * dv;
+                    const float vr = P->v_rest + pv;
+                    const float pvi = P->p_vi * I;
+                    v[i] = vr + pvi;
+                }
+                isyn[i] = P->p_i * I;
+                /* send(): threshold, reset, refractory (test/fused_reference.py:100-103) */
+                if (v[i] > P->v_th) {
+                    fired = 1;
+                    v[i] = P->v_reset;
+                    ref[i] = P->ref_steps;
+                }
+            }
+            prev_spk[i] = fired;
+            if (raster) raster[(int64_t)k * N + i] = fired;
+        }
+    }
+done:
+    if (events) *events = n_ev;
+    free(acc); free(touched); free(list); free(pop);
+    return rc;
+}

@@ -1,0 +1,59 @@
+/*
+ * naer2_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded CPU restatement of the time-driven spiking-network step that
+ * libnaer2_hip.so runs on the GPU.  It is the checker of the HIP path: only the tests, the smoke check
+ * and the CPU-baseline leg of the benchmark load it, and the product never calls it.
+ *
+ * Followed reference code (MatteoSchmider/naer2):
+ *   - tools/brunel_reference.py:34-48   the time-driven step: t=(n+1)dt, input = ring[n % D] + J*Poisson,
+ *                                        clear, refractory gate, V = V*decay + input, fire at V >= theta,
+ *                                        reset, deliver to the ring D steps later
+ *   - test/networks/PD14/neuron.py:37-60 and test/fused_reference.py:76-107
+ *                                        the exp-synapse LIF: deliver (current += w) -> update with the exact
+ *                                        propagator -> current *= P_I -> threshold V > V_th, reset
+ *   - naer2/transpiler/templates/fused.py:2308-2604 (_deliver) and :1626-1965 (_pull)
+ *                                        delivery of a source's links into the target's ring slots
+ *   - naer2/ring.py:132-245              the ring of arrival slots (here: fp32 input accumulators)
+ *
+ * Deviations from the reference, kept identical on the GPU:
+ *   - refractoriness counts whole steps (ref_steps) instead of comparing float times;
+ *   - propagators are fp32 constants for the fixed dt (no exp in the loop);
+ *   - every delay is a whole number of steps in [1, d_slots];
+ *   - summation order (the reference's GPU order is not deterministic): for target i and arrival step n,
+ *     for each emission step s ascending: p = +0.0f; p += w_l over the links l -> i of sources spiking at
+ *     s with s + d_l = n, in ascending canonical link index; then ring[n % D][i] = ring[n % D][i] + p
+ *     (only for cells that received at least one link).
+ *   - background: count = #{j : u >= thr[j]}, u = word 0 of Philox4x32-10(ctr = {neuron, step lo,
+ *     step hi, instance}, key = {seed lo, seed hi}); input = x + j_ext * (float)count.
+ */
+#ifndef NAER2_ORACLE_H
+#define NAER2_ORACLE_H
+#include <stdint.h>
+
+#define OR_POISSON_K 32
+#define OR_LIF_DELTA 1
+#define OR_LIF_EXP 2
+
+/* Same byte layout as the product's per-population parameter block (fp32 constants + inversion table). */
+typedef struct or_pop {
+    int32_t ref_steps;
+    int32_t reserved;
+    float decay, p_i, p_vi, v_rest, v_th, v_reset, j_ext, reserved_f;
+    uint32_t poisson_thr[OR_POISSON_K];
+} or_pop;
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int or_poisson_count(uint32_t u, const uint32_t* thr);
+
+/* Runs n_steps steps from step0.  Topology: canonical CSR by source (src_ptr [N+1], tgt/w/delay [E]).
+ * prev_spk [N]: the spikes of step0 - 1 on entry, of the last step on return.  raster [n_steps][N] or NULL.
+ * accum_mode 0: p += w (the product's); 1: p = p*0.5f + w and ring = ring*0.5f + p (an order hash, to pin
+ * the order).  events: links delivered (out).  Returns 0, or -1 on a bad argument. */
+int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops,
+           int32_t d_slots, uint64_t seed, uint32_t instance,
+           const int64_t* src_ptr, const int32_t* tgt, const float* w, const int32_t* delay,
+           float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+           int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events);
+
+#endif

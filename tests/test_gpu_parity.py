@@ -1,0 +1,129 @@
+"""Parity of the HIP step (through the C-ABI) with the CPU oracle: spike rasters, potentials, currents,
+refractory counters and ring contents bit for bit, on the same seeded inputs."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from naer2_amd.network import raster_to_bits
+from naer2_amd.networks import brunel, pd14
+from tests.gpu_util import (ProbeNode, assert_state_equal, bits_equal, network_from_topology, oracle_run,
+                            oracle_state)
+from tests.helpers import tiny_topology
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_brunel_config0_bit_exact_2000_steps():
+    _say("case: brunel N=2500 seed=0 2000 steps")
+    net, _ = brunel(device="cuda", seed=0)
+    rt = net.build()
+    st = oracle_state(rt)
+    net.run(2000)
+    torch.cuda.synchronize()
+    raster, events = oracle_run(rt, st, 0, 2000)
+    assert raster.sum() > 1000 and events > 0
+    assert np.array_equal(rt.raster(), raster)
+    assert_state_equal(rt, st)
+
+
+def test_pd14_small_bit_exact_1000_steps():
+    _say("case: pd14 scale=0.05 seed=1 1000 steps")
+    net, _ = pd14(scale=0.05, seed=1, device="cuda")
+    rt = net.build()
+    st = oracle_state(rt)
+    net.run(1000)
+    torch.cuda.synchronize()
+    raster, _ = oracle_run(rt, st, 0, 1000)
+    assert raster.sum() > 100
+    assert np.array_equal(rt.raster(), raster)
+    assert_state_equal(rt, st)
+
+
+@pytest.mark.parametrize("n,n_links,max_delay,p_spike", [
+    (200, 6000, 7, 0.1),      # a few spikes a step
+    (3000, 120000, 40, 0.6),  # bursts: > 512 spiking sources and > 1024 links a group per step
+    (130, 4000, 128, 0.3),    # delays up to the ring's 128 slots, two 64-bit touched masks
+])
+def test_forced_spikes_deliver_like_the_oracle(n, n_links, max_delay, p_spike):
+    """Host-forced spike words (the commanded-node fixture of test/runtimes.py:24-44): the ring input
+    read at every step, and the ring, equal the oracle's."""
+    _say(f"case: forced spikes n={n} links={n_links} D={max_delay} p={p_spike}")
+    topo = tiny_topology(n, n_links, max_delay, seed=n)
+    net, pop = network_from_topology(ProbeNode(), n, topo)
+    rt = net.build()
+    st = oracle_state(rt)
+    rng = np.random.default_rng(n + 1)
+    forced = rng.random((25, n)) < p_spike
+    words = torch.from_numpy(raster_to_bits(forced, rt.layout)).to("cuda")
+    for s in range(forced.shape[0]):
+        if s > 0:
+            rt.bits[(s - 1) % rt.bit_slots] = words[s - 1]
+            st.prev_spk[:] = forced[s - 1]
+        rt.run(1)
+        oracle_run(rt, st, s, 1, record=False)
+        torch.cuda.synchronize()
+        assert bits_equal(rt.v.cpu().numpy(), st.v), f"step {s}: ring input differs"
+    assert_state_equal(rt, st)
+
+
+def test_graph_replays_equal_eager_steps():
+    _say("case: graph windows vs eager, brunel N=2500")
+    a, _ = brunel(device="cuda", seed=5)
+    b, _ = brunel(device="cuda", seed=5)
+    ra, rb = a.build(), b.build()
+    for _ in range(6):  # replays with eager work between them
+        a.run(50, graph_window=50)
+        b.run(50, graph_window=0)
+        torch.cuda.synchronize()
+        _ = (ra.v * 2).sum().item()
+        assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert np.array_equal(ra.raster(), rb.raster())
+    assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
+
+
+def test_continuation_equals_one_run():
+    _say("case: continuation 300+200 vs 500, pd14 scale=0.03")
+    a, _ = pd14(scale=0.03, seed=2, device="cuda")
+    b, _ = pd14(scale=0.03, seed=2, device="cuda")
+    a.run(300)
+    a.run(200)
+    b.run(500)
+    torch.cuda.synchronize()
+    ra, rb = a.runtime, b.runtime
+    assert np.array_equal(ra.raster(), rb.raster())
+    assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert bits_equal(ra.isyn.cpu().numpy(), rb.isyn.cpu().numpy())
+
+
+def test_no_recording_keeps_last_step():
+    _say("case: record=False")
+    a, _ = brunel(device="cuda", seed=7)
+    a.record = False
+    b, _ = brunel(device="cuda", seed=7)
+    a.run(300)
+    b.run(300)
+    torch.cuda.synchronize()
+    assert a.runtime.bit_slots == 2
+    assert np.array_equal(a.runtime.raster(299, 300), b.runtime.raster(299, 300))
+    assert bits_equal(a.runtime.v.cpu().numpy(), b.runtime.v.cpu().numpy())
+
+
+def test_dsl_state_views_write_through():
+    _say("case: population state views")
+    net, pops = brunel(N_E=300, N_I=100, C_E=30, C_I=10, device="cuda", seed=9)
+    rt = net.build()
+    E = pops["E"]
+    assert E.V.shape == (300,)
+    E.V = torch.full((300,), 5.0)
+    assert float(rt.v[:300].min()) == 5.0 and float(rt.v[:300].max()) == 5.0
+    net.step(1.0)
+    assert net.step_index == 10 and abs(net.t - 1.0) < 1e-9

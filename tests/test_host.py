@@ -1,0 +1,169 @@
+"""Host-side logic on the CPU: the description API, connectivity rules, delay rounding, the group layout,
+the group segments the kernel reads, raster <-> spike-word conversion, and the C-ABI's exports and
+argument checks (no compute call: no GPU here)."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import naer2_amd as n2
+from naer2_amd import _lib
+from naer2_amd.network import assemble, bits_to_raster, raster_to_bits
+from naer2_amd.networks import PD14, brunel, pd14
+from naer2_amd.nodes import FixedInDegree, FixedTotalNumber
+from naer2_amd.topology import canonical, delay_steps, group_layout, segments
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_header_declares_exactly_the_exports():
+    text = (ROOT / "include" / "naer2_hip.h").read_text()
+    declared = set(re.findall(r"^int\s+(n2h_\w+)\(", text, flags=re.M))
+    assert declared == set(_lib.EXPORTS)
+    lib = _lib.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.n2h_version() == 1
+
+
+def test_plan_create_rejects_bad_arguments():
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    first = (ctypes.c_int32 * 2)(0, 10)
+    pops = (_lib.PopParams * 1)()
+    tab = _lib.Tables(n_groups=1, group_first=1, group_info=1, seg_off=1, links=1, n_links=1)
+    cases = [
+        (dict(model=3, n_neurons=10, n_pops=1, d_slots=1), "unknown model"),
+        (dict(model=1, n_neurons=0, n_pops=1, d_slots=1), "n_neurons"),
+        (dict(model=1, n_neurons=10, n_pops=0, d_slots=1), "n_pops"),
+        (dict(model=1, n_neurons=10, n_pops=1, d_slots=129), "d_slots"),
+    ]
+    for kw, msg in cases:
+        desc = _lib.ModelDesc(pop_first=first, pops=pops, **kw)
+        rc = lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan))
+        assert rc == -1 and msg in _lib.last_error()
+    bad = (ctypes.c_int32 * 2)(0, 9)
+    desc = _lib.ModelDesc(model=1, n_neurons=10, n_pops=1, d_slots=1, pop_first=bad, pops=pops)
+    assert lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan)) == -1
+    assert "pop_first" in _lib.last_error()
+    pops[0].poisson_thr[3] = 5  # not non-decreasing after zeros? make it decreasing
+    pops[0].poisson_thr[4] = 1
+    desc = _lib.ModelDesc(model=1, n_neurons=10, n_pops=1, d_slots=1, pop_first=first, pops=pops)
+    assert lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan)) == -1
+    assert "poisson_thr" in _lib.last_error()
+    assert lib.n2h_step(None, 1, None) == -1
+    assert lib.n2h_bind_ring(None, None, None, 2) == -1
+
+
+def test_fixed_in_degree_gives_every_target_exactly_k_inputs():
+    net, pops = brunel(N_E=400, N_I=100, C_E=40, C_I=10, device="cpu")
+    E, I = pops["E"], pops["I"]
+    for c in net.connections:
+        k = 40 if c.source is E else 10
+        counts = torch.bincount(c.tgt, minlength=c.target.n)
+        assert bool((counts == k).all())
+        assert bool((c.src[1:] >= c.src[:-1]).all())  # source-major link order
+
+
+def test_fixed_total_number_out_degree_is_the_reference_rule():
+    p = 0.1009
+    nx, ny = 2068, 2068
+    q = math.log(1 - p) / math.log(1 - 1 / (nx * ny))
+    assert FixedTotalNumber(p).per_source(nx, ny) == round(q / nx)
+    total = 0
+    for si, s in enumerate(PD14["size"]):
+        for ti, t in enumerate(PD14["size"]):
+            total += s * FixedTotalNumber(PD14["probabilities"][si][ti]).per_source(s, t)
+    assert total == 298822986  # ~298.8 M synapses at full scale
+
+
+def test_per_source_pattern_callable_is_supported():
+    @n2.pattern
+    def ring_next(position, source_region, target_region=None):
+        n = len(target_region)
+        return n2.Points(torch.tensor([[(position[0] + 1) % n]]))
+
+    A = n2.LIFDelta() * 5
+    conn = A >> ring_next >> A
+    assert conn.tgt.tolist() == [1, 2, 3, 4, 0]
+    conn.weight, conn.delay = 1.0, 0.1
+    assert (A >> FixedInDegree(2) >> A | 0.5).delay.shape == (10,)
+
+
+def test_delay_steps_round_and_clip():
+    d = delay_steps(torch.tensor([0.1, 0.14, 0.15, 0.25, 1.5, 0.01], dtype=torch.float32), 0.1)
+    assert d.tolist() == [1, 1, 2, 2, 15, 1]
+    with pytest.raises(ValueError):
+        delay_steps(torch.tensor([13.0]), 0.1)
+
+
+def test_group_layout_never_straddles_populations():
+    lay = group_layout([130, 64, 1])
+    assert lay.group_first.tolist() == [0, 64, 128, 130, 194]
+    assert (lay.group_info & 0xFFFF).tolist() == [64, 64, 2, 64, 1]
+    assert (lay.group_info >> 16).tolist() == [0, 0, 0, 1, 2]
+    assert lay.group_of[129].item() == 2 and lay.local_of[129].item() == 1
+
+
+def test_segments_hold_each_sources_links_per_group_in_canonical_order():
+    rng = np.random.default_rng(0)
+    n = 300
+    src = torch.from_numpy(rng.integers(0, n, 5000))
+    tgt = torch.from_numpy(rng.integers(0, n, 5000))
+    w = torch.from_numpy(rng.standard_normal(5000).astype(np.float32))
+    d = torch.from_numpy(rng.integers(1, 20, 5000).astype(np.int32))
+    can = canonical(n, src, tgt, w, d)
+    lay = group_layout([200, 100])
+    seg = segments(can, lay)
+    G = lay.n_groups
+    links = seg.links.numpy()
+    gf = lay.group_first.numpy()
+    sp = can.src_ptr.numpy()
+    for s in range(n):
+        row = range(sp[s], sp[s + 1])
+        for g in range(G):
+            lo, hi = int(seg.seg_off[s * G + g]), int(seg.seg_off[s * G + g + 1])
+            expect = [l for l in row if lay.group_of[can.tgt[l]].item() == g]
+            got = links[lo:hi]
+            assert len(got) == len(expect)
+            for (x, y), l in zip(got, expect):
+                assert gf[g] + (x & 0xFFFF) == can.tgt[l].item()
+                assert (x >> 16) == can.d[l].item()
+                assert np.int32(y).view(np.float32) == can.w[l].item()
+
+
+def test_raster_bits_round_trip():
+    lay = group_layout([70, 3, 64])
+    r = np.random.default_rng(1).random((9, 137)) < 0.3
+    assert np.array_equal(bits_to_raster(raster_to_bits(r, lay), lay, 137), r)
+
+
+def test_assemble_pd14_small():
+    net, pops = pd14(scale=0.02, device="cpu")
+    a = assemble(net)
+    assert a.n == sum(p.n for p in pops.values())
+    assert a.isyn is not None and a.family == "lif_exp"
+    assert 1 <= int(a.canon.d.min()) and a.d_slots <= 128
+    # inhibitory weights negative, excitatory non-negative
+    e23 = pops["E_23"]
+    c = [c for c in net.connections if c.source is pops["I_23"]][0]
+    assert float(c.weight.max()) <= 0.0
+    c = [c for c in net.connections if c.source is e23][0]
+    assert float(c.weight.min()) >= 0.0
+
+
+def test_unknown_node_family_is_rejected_at_build():
+    @n2.node
+    class Custom:
+        def __init__(self):
+            self.x = 0.0
+
+    A = Custom() * 3
+    with pytest.raises(NotImplementedError):
+        assemble(A.network)
