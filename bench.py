@@ -79,7 +79,19 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only run 5 plain device-to-device copies of MiB each (the reference kernel that scales "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes), then exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build "
               "(see DESIGN.md for how it would shard)", file=sys.stderr)

@@ -92,15 +92,16 @@ typedef struct n2h_model_desc {
 
 /* Group-segment synapse tables, on the device.  Target group g covers neurons
  * [group_first[g], group_first[g] + (group_info[g] & 0xffff)) of population group_info[g] >> 16;
- * groups never straddle populations.  The links of source s into group g are
- * links[seg_off[s * n_groups + g] .. seg_off[s * n_groups + g + 1]), in canonical link order, each
- * packed as {x = delay_steps << 16 | target - group_first[g], y = fp32 weight bits}. */
+ * groups never straddle populations.  A neuron's source slot is 64 * (its group) + (its index in the
+ * group).  The links of source slot p into group g are
+ * links[seg_off[p * n_groups + g] .. seg_off[p * n_groups + g + 1]), in canonical link order, each packed
+ * as {x = delay_steps << 16 | target - group_first[g], y = fp32 weight bits}. */
 typedef struct n2h_tables {
     int32_t n_groups;
     int32_t reserved;
     const int32_t* group_first;    /* device [n_groups] */
     const int32_t* group_info;     /* device [n_groups] */
-    const int64_t* seg_off;        /* device [n_neurons * n_groups + 1] */
+    const int64_t* seg_off;        /* device [64 * n_groups * n_groups + 1] */
     const uint32_t* links;         /* device [2 * n_links] */
     int64_t n_links;
 } n2h_tables;
@@ -119,9 +120,13 @@ int n2h_plan_destroy(n2h_plan* plan);
 int n2h_bind_state(n2h_plan* plan, float* v, float* i_syn, int32_t* ref);
 
 /* ring: fp32 [d_slots][n_neurons], the input arriving at step n in slot n % d_slots.
- * spike_bits: uint64 [bit_slots][n_groups], step n's spikes in slot n % bit_slots (bit_slots >= 2; slot
- * (step0 - 1) % bit_slots must hold the spikes of step0 - 1, zero before the first step). */
-int n2h_bind_ring(n2h_plan* plan, float* ring, uint64_t* spike_bits, int64_t bit_slots);
+ * spike_bits: uint64 [bit_slots][n_groups], step n's spikes in slot n % bit_slots, bit b of word g =
+ * neuron group_first[g] + b (bit_slots >= 2).
+ * spike_list: int32 [4][1 + 64 * n_groups], step n's spikes in slot n % 4: a count, then the source slots
+ * 64 * g + b of its spikes in any order.
+ * Before a step n0: bit slot and list slot of n0 - 1 hold the spikes of step n0 - 1 (zero before the first
+ * step) and list slot n0 % 4 has count 0. */
+int n2h_bind_ring(n2h_plan* plan, float* ring, uint64_t* spike_bits, int64_t bit_slots, int32_t* spike_list);
 
 /* The device step counter: the next step n2h_step computes. */
 int n2h_set_step(n2h_plan* plan, int64_t step, void* stream);
@@ -131,7 +136,8 @@ int n2h_get_step(n2h_plan* plan, int64_t* step);   /* synchronizes with the devi
  * counter by n_steps. */
 int n2h_step(n2h_plan* plan, int32_t n_steps, void* stream);
 
-/* A captured hipGraph of n_steps steps (each replay continues from the device counter). */
+/* A captured hipGraph of n_steps steps (each replay continues from the device counter).  A graph replays
+ * only at steps with the phase (step mod 4) it was captured at: windows of a multiple of 4 steps keep it. */
 int n2h_graph_create(n2h_plan* plan, int32_t n_steps, void* stream, n2h_graph** out);
 int n2h_graph_launch(n2h_graph* graph, void* stream);
 int n2h_graph_destroy(n2h_graph* graph);

@@ -74,7 +74,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_plan_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(Tables), ctypes.POINTER(vp)], i32),
         "n2h_plan_destroy": ([vp], i32),
         "n2h_bind_state": ([vp, vp, vp, vp], i32),
-        "n2h_bind_ring": ([vp, vp, vp, i64], i32),
+        "n2h_bind_ring": ([vp, vp, vp, i64, vp], i32),
         "n2h_set_step": ([vp, i64, vp], i32),
         "n2h_get_step": ([vp, ctypes.POINTER(i64)], i32),
         "n2h_step": ([vp, i32, vp], i32),

@@ -16,14 +16,17 @@
 //                   accumulators ring[n % d_slots][neuron] (tools/brunel_reference.py:29, buf[dsteps, N]).
 //   update + send   tools/brunel_reference.py:34-48 (delta LIF) and test/fused_reference.py:86-107,
 //                   test/networks/PD14/neuron.py:37-60 (exp-synapse LIF, exact propagator).
-//   emission        fused.py:2611-2644 (_emit_spikes): here the step's spikes are one ballot word per
-//                   group, so the spike order is the neuron order without atomics.
+//   emission        fused.py:2611-2644 (_emit_spikes): a group appends its spiking source slots to the
+//                   step's list with one atomic (as the reference's flat list, fused.py:2637) and writes its
+//                   ballot word to the step's bitmap; the consumer sorts the list (<= 64 spikes) or scans
+//                   the bitmap (bursts), so the delivery order is the source order either way.
 //
 // Summation order (identical in the CPU oracle): for target i and arrival step n, for each emission step s
 // ascending, p = +0.0f, p += w over the links into i of sources spiking at s with s + d = n in ascending
 // canonical link index; then ring[n % D][i] = ring[n % D][i] + p.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -38,6 +41,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kCapS = 512;   // spiking sources staged in LDS per pass
 constexpr int kCapL = 1024;  // links staged in LDS per pass
+constexpr int kListSlots = 4;  // spike lists of steps s-1 (read), s (written), s+1 (cleared), s-2 (idle)
 
 thread_local char g_err[512] = "";
 
@@ -61,10 +65,11 @@ struct StepArgs {
     int32_t d_slots;
     uint32_t instance;
     int64_t bit_slots;
+    int64_t list_stride;     // int32 words per spike-list slot: 1 + 64 * n_groups
     uint32_t key0, key1;
     const int32_t* group_first;
     const int32_t* group_info;
-    const int64_t* seg_off;
+    const int64_t* seg_off;  // [64 * n_groups * n_groups + 1], indexed by (source slot) * n_groups + group
     const uint2* links;
     const n2h_pop_params* pops;
     float* v;
@@ -72,7 +77,11 @@ struct StepArgs {
     int32_t* ref;
     float* ring;
     unsigned long long* bits;
-    const int64_t* step_ctr;
+    int32_t* spikes;         // [4][list_stride]: count, then source slots in any order
+    const int64_t* step_ctr; // graph launches: the window's first step; eager launches: null
+    int64_t step_imm;        // eager launches: the first step of the launch sequence
+    int32_t phase;           // (first step of the sequence) mod 4: the list slots without a load
+    int32_t pad;
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -104,20 +113,165 @@ __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
     return x;
 }
 
-template <int MODEL>
+// Bitonic sort of one int per lane, ascending over the wavefront.
+__device__ __forceinline__ int wave_sort(int v, int lane) {
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int o = __shfl_xor(v, j, kWave);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keep_min ? min(v, o) : max(v, o);
+        }
+    }
+    return v;
+}
+
+// Shared-memory staging of one workgroup.
+struct Stage {
+    int32_t sp[kCapS];           // spiking source slots of this pass, ascending
+    int64_t seg_lo[kCapS];       // first link of each source's segment into this group
+    int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
+    uint2 links[kCapL];          // links of this pass, in contract order
+    unsigned long long touched[kWave][2];  // per target: delays 1..64 and 65..128 with a partial sum
+};
+
+// Bitonic sort of (key, value) pairs, one per lane, ascending by key (keys distinct).
+__device__ __forceinline__ void wave_sort_pairs(int& key, float& val, int lane) {
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int ok = __shfl_xor(key, j, kWave);
+            const float ov = __shfl_xor(val, j, kWave);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            const bool take = keep_min ? (ok < key) : (ok > key);
+            key = take ? ok : key;
+            val = take ? ov : val;
+        }
+    }
+}
+
+// Walk of staged links [b0, min(b0 + 64, nl)): one link per lane.  The links are sorted by
+// (cell = (delay - 1) * 64 + target, position) so that the links of one cell form a run in contract order;
+// the first lane of each run adds the run's weights one after another to the cell's partial sum (cells of
+// different runs are distinct, so the runs proceed in parallel).  Equivalent to the target lanes walking
+// the links one by one in order.
+__device__ __forceinline__ void walk_batch(Stage& st, float* acc, int lane, int b0, int nl) {
+    const int e = b0 + lane;
+    int key = INT_MAX;
+    float w = 0.0f;
+    if (e < nl) {
+        const uint2 ln = st.links[e];
+        const int cell = (static_cast<int>(ln.x >> 16) - 1) * kWave + static_cast<int>(ln.x & 0xffffu);
+        key = (cell << 6) | lane;
+        w = __uint_as_float(ln.y);
+    }
+    wave_sort_pairs(key, w, lane);
+    const bool valid = key != INT_MAX;
+    const int cell = key >> 6;
+    const int prev_cell = __shfl_up(cell, 1, kWave);
+    const bool head = valid && (lane == 0 || prev_cell != cell);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long valids = __ballot(valid);
+    // run of a head: up to the next head or the end of the valid lanes
+    const unsigned long long after = (lane == kWave - 1) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int end = after ? (__ffsll(static_cast<long long>(after)) - 1) : __popcll(valids);
+    int len = head ? end - lane : 0;
+    int max_len = len;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) max_len = max(max_len, __shfl_xor(max_len, off, kWave));
+    float a = 0.0f;
+    if (head) {
+        const int t = cell & (kWave - 1);
+        const int b = cell >> 6;
+        const unsigned long long bit = 1ull << (b & 63);
+        const unsigned long long old = atomicOr(&st.touched[t][b >> 6], bit);
+        a = ((old & bit) ? acc[cell] : 0.0f) + w;
+    }
+    for (int r = 1; r < max_len; ++r) {
+        const float v = __shfl_down(w, r, kWave);
+        if (r < len) a = a + v;
+    }
+    if (head) acc[cell] = a;
+}
+
+// Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
+// contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
+__device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float* acc, int g, int lane, int n_sp) {
+    const int NG = a.n_groups;
+    // segment bounds: every load of the pass issued before the first is used
+    int64_t lo_r[kCapS / kWave], hi_r[kCapS / kWave];
+#pragma unroll
+    for (int u = 0; u < kCapS / kWave; ++u) {
+        const int i = u * kWave + lane;
+        lo_r[u] = hi_r[u] = 0;
+        if (i < n_sp) {
+            const int64_t idx = static_cast<int64_t>(st.sp[i]) * NG + g;
+            lo_r[u] = a.seg_off[idx];
+            hi_r[u] = a.seg_off[idx + 1];
+        }
+    }
+    int carry = 0;
+#pragma unroll
+    for (int u = 0; u < kCapS / kWave; ++u) {
+        if (u * kWave >= n_sp) break;
+        const int i = u * kWave + lane;
+        const int n = static_cast<int>(hi_r[u] - lo_r[u]);
+        const int inc = wave_incl_scan(n, lane);
+        if (i < n_sp) {
+            st.seg_start[i] = carry + inc - n;
+            st.seg_lo[i] = lo_r[u];
+        }
+        carry += __shfl(inc, kWave - 1, kWave);
+    }
+    if (lane == 0) st.seg_start[n_sp] = carry;
+    __syncthreads();
+    const int L = carry;
+    for (int e0 = 0; e0 < L; e0 += kCapL) {
+        const int e1 = min(L, e0 + kCapL);
+        uint2 r[kCapL / kWave];
+#pragma unroll
+        for (int u = 0; u < kCapL / kWave; ++u) {
+            const int e = e0 + u * kWave + lane;
+            r[u] = make_uint2(0u, 0u);
+            if (e < e1) {
+                int lo_i = 0, hi_i = n_sp;  // seg_start[lo_i] <= e < seg_start[hi_i]
+                while (hi_i - lo_i > 1) {
+                    const int mid = (lo_i + hi_i) >> 1;
+                    if (st.seg_start[mid] <= e) lo_i = mid;
+                    else hi_i = mid;
+                }
+                r[u] = a.links[st.seg_lo[lo_i] + (e - st.seg_start[lo_i])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCapL / kWave; ++u) {
+            const int e = e0 + u * kWave + lane;
+            if (e < e1) st.links[e - e0] = r[u];
+        }
+        __syncthreads();
+        const int nl = e1 - e0;
+        for (int b0 = 0; b0 < nl; b0 += kWave) walk_batch(st, acc, lane, b0, nl);
+        __syncthreads();
+    }
+}
+
+// One step s of the whole network for target group g (one wavefront).
+// KW: bitmap words per lane held in registers on the burst path (0: read them in a loop).
+template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
-    __shared__ int32_t sp[kCapS];           // spiking sources of this pass, ascending
-    __shared__ int64_t seg_lo[kCapS];       // first link of each source's segment into this group
-    __shared__ int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
-    __shared__ uint2 stage[kCapL];          // links of this pass, in contract order
-    extern __shared__ float acc[];          // [d_slots][64] partial sums of this emission step
+    __shared__ Stage st;
+    extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
 
     const int g = blockIdx.x;
     const int lane = threadIdx.x;
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
     const int64_t D = a.d_slots;
-    const int64_t s = *a.step_ctr + k;
+    const int64_t s = (a.step_ctr ? *a.step_ctr : a.step_imm) + k;
+    const int slot_prev = (a.phase + k + kListSlots - 1) & (kListSlots - 1);  // step s-1
+    const int slot_cur = (a.phase + k) & (kListSlots - 1);                    // step s
 
     const int first = a.group_first[g];
     const int info = a.group_info[g];
@@ -126,6 +280,11 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     const bool live = lane < count;
     const int nid = first + lane;
     const int64_t cur = (s % D) * N + nid;
+
+    // the spikes of step s-1: their count and (speculatively) the first 64 source slots
+    const int32_t* plist = a.spikes + slot_prev * a.list_stride;
+    const int n_prev = plist[0];
+    const int my_src = plist[1 + lane];
 
     // state and the due ring cell: nothing below writes them before they are used
     float V = 0.0f, I = 0.0f, x = 0.0f;
@@ -137,107 +296,83 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
         x = a.ring[cur];
     }
 
-    // ---- pull intake + delivery of the spikes of step s-1 ----
-    const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
-    const unsigned long long* prev = a.bits + prev_slot * NG;
-    const int W = (NG + kWave - 1) / kWave;
-    const int w_lo = lane * W;
-    const int w_hi = min(NG, w_lo + W);
-    int cnt = 0;
-    for (int w = w_lo; w < w_hi; ++w) cnt += __popcll(prev[w]);
-    const int incl = wave_incl_scan(cnt, lane);
-    const int excl = incl - cnt;
-    const int total = __shfl(incl, kWave - 1, kWave);
-
-    unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 touched by this lane's target
-    for (int r0 = 0; r0 < total; r0 += kCapS) {
-        const int r1 = min(total, r0 + kCapS);
-        if (excl < r1 && excl + cnt > r0) {
-            int rank = excl;
-            for (int w = w_lo; w < w_hi && rank < r1; ++w) {
-                unsigned long long m = prev[w];
-                const int pc = __popcll(m);
-                if (rank + pc <= r0) {
-                    rank += pc;
-                    continue;
-                }
-                const int base = a.group_first[w];
-                while (m) {
-                    const int b = __ffsll(static_cast<long long>(m)) - 1;
-                    if (rank >= r0 && rank < r1) sp[rank - r0] = base + b;
-                    ++rank;
-                    m &= m - 1ull;
-                }
-            }
-        }
+    st.touched[lane][0] = 0ull;
+    st.touched[lane][1] = 0ull;
+    if (n_prev <= kWave) {
+        // sparse path: sort the (atomically appended) source slots, then deliver them
+        const int v = wave_sort(lane < n_prev ? my_src : INT_MAX, lane);
+        st.sp[lane] = v;
         __syncthreads();
-        const int n_sp = r1 - r0;
-        int carry = 0;
-        for (int i0 = 0; i0 < n_sp; i0 += kWave) {
-            const int i = i0 + lane;
-            int n = 0;
-            if (i < n_sp) {
-                const int64_t idx = static_cast<int64_t>(sp[i]) * NG + g;
-                const int64_t lo = a.seg_off[idx];
-                n = static_cast<int>(a.seg_off[idx + 1] - lo);
-                seg_lo[i] = lo;
-            }
-            const int inc = wave_incl_scan(n, lane);
-            if (i < n_sp) seg_start[i] = carry + inc - n;
-            carry += __shfl(inc, kWave - 1, kWave);
+        if (n_prev > 0) deliver_pass(a, st, acc, g, lane, n_prev);
+    } else {
+        // burst path: the spike bitmap of step s-1, lane-contiguous words, ascending source slots
+        const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
+        const unsigned long long* prev = a.bits + prev_slot * NG;
+        const int W = (NG + kWave - 1) / kWave;
+        const int w_lo = lane * W;
+        const int w_hi = min(NG, w_lo + W);
+        unsigned long long wd[KW > 0 ? KW : 1];
+        int cnt = 0;
+        if (KW > 0) {
+#pragma unroll
+            for (int q = 0; q < (KW > 0 ? KW : 1); ++q) wd[q] = (w_lo + q < w_hi) ? prev[w_lo + q] : 0ull;
+#pragma unroll
+            for (int q = 0; q < (KW > 0 ? KW : 1); ++q) cnt += __popcll(wd[q]);
+        } else {
+            for (int w = w_lo; w < w_hi; ++w) cnt += __popcll(prev[w]);
         }
-        if (lane == 0) seg_start[n_sp] = carry;
-        __syncthreads();
-        const int L = carry;
-        for (int e0 = 0; e0 < L; e0 += kCapL) {
-            const int e1 = min(L, e0 + kCapL);
-            for (int e = e0 + lane; e < e1; e += kWave) {
-                int lo_i = 0, hi_i = n_sp;  // seg_start[lo_i] <= e < seg_start[hi_i]
-                while (hi_i - lo_i > 1) {
-                    const int mid = (lo_i + hi_i) >> 1;
-                    if (seg_start[mid] <= e) lo_i = mid;
-                    else hi_i = mid;
-                }
-                stage[e - e0] = a.links[seg_lo[lo_i] + (e - seg_start[lo_i])];
-            }
-            __syncthreads();
-            const int nl = e1 - e0;
-            for (int e = 0; e < nl; ++e) {
-                const uint2 ln = stage[e];
-                if (static_cast<int>(ln.x & 0xffffu) == lane) {
-                    const int b = static_cast<int>(ln.x >> 16) - 1;
-                    const float wv = __uint_as_float(ln.y);
-                    bool seen;
-                    if (b < 64) {
-                        seen = (t0 >> b) & 1ull;
-                        t0 |= 1ull << b;
+        const int incl = wave_incl_scan(cnt, lane);
+        const int excl = incl - cnt;
+        const int total = __shfl(incl, kWave - 1, kWave);
+        for (int r0 = 0; r0 < total; r0 += kCapS) {
+            const int r1 = min(total, r0 + kCapS);
+            if (excl < r1 && excl + cnt > r0) {
+                int rank = excl;
+                for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) {
+                    unsigned long long m;
+                    if (KW > 0) {
+                        m = 0ull;
+#pragma unroll
+                        for (int z = 0; z < (KW > 0 ? KW : 1); ++z) m = (z == q) ? wd[z] : m;
                     } else {
-                        seen = (t1 >> (b - 64)) & 1ull;
-                        t1 |= 1ull << (b - 64);
+                        m = prev[w_lo + q];
                     }
-                    float* c = &acc[b * kWave + lane];
-                    *c = seen ? *c + wv : 0.0f + wv;
+                    const int pc = __popcll(m);
+                    if (rank + pc <= r0) {
+                        rank += pc;
+                        continue;
+                    }
+                    const int base = (w_lo + q) * kWave;
+                    while (m) {
+                        const int b = __ffsll(static_cast<long long>(m)) - 1;
+                        if (rank >= r0 && rank < r1) st.sp[rank - r0] = base + b;
+                        ++rank;
+                        m &= m - 1ull;
+                    }
                 }
             }
             __syncthreads();
+            deliver_pass(a, st, acc, g, lane, r1 - r0);
         }
     }
 
     // ---- ring: add each touched cell's partial sum; the delay-1 cell is the one due now ----
+    __syncthreads();
+    const unsigned long long t0 = st.touched[lane][0], t1 = st.touched[lane][1];
     if (live) {
         unsigned long long m = t0 & ~1ull;
         while (m) {
             const int b = __ffsll(static_cast<long long>(m)) - 1;
             m &= m - 1ull;
             const int64_t idx = ((s + b) % D) * N + nid;  // arrival s - 1 + (b + 1)
-            a.ring[idx] = a.ring[idx] + acc[b * kWave + lane];
+            unsafeAtomicAdd(&a.ring[idx], acc[b * kWave + lane]);  // the cell's one add this step
         }
         m = t1;
         while (m) {
             const int b = __ffsll(static_cast<long long>(m)) - 1 + 64;
             m &= m - 1ull;
             const int64_t idx = ((s + b) % D) * N + nid;
-            a.ring[idx] = a.ring[idx] + acc[b * kWave + lane];
+            unsafeAtomicAdd(&a.ring[idx], acc[b * kWave + lane]);
         }
         if (t0 & 1ull) x = x + acc[lane];
         a.ring[cur] = 0.0f;
@@ -289,8 +424,22 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
         a.ref[nid] = R;
         if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
     }
+
+    // ---- emission: the bitmap word, and the source slots appended to the step's list ----
     const unsigned long long mask = __ballot(fired);
     if (lane == 0) a.bits[(s % a.bit_slots) * NG + g] = mask;
+    int32_t* clist = a.spikes + slot_cur * a.list_stride;
+    if (mask) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(clist, __popcll(mask));
+        base = __shfl(base, 0, kWave);
+        if (fired) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            clist[1 + base + rank] = g * kWave + lane;
+        }
+    }
+    // the list of step s+1 starts empty (its slot was last read during step s-2)
+    if (g == 0 && lane == 0) a.spikes[((slot_cur + 1) & (kListSlots - 1)) * a.list_stride] = 0;
 }
 
 __global__ void advance_kernel(int64_t* ctr, int32_t n) { *ctr += n; }
@@ -309,7 +458,9 @@ struct n2h_plan {
     int32_t* ref;
     float* ring;
     unsigned long long* bits;
+    int32_t* spikes;
     int64_t bit_slots;
+    int64_t host_step;  // the step the next enqueued launch computes (mirror of *step_ctr)
     size_t lds_bytes;
     int device;
 };
@@ -319,11 +470,15 @@ struct n2h_graph {
     hipGraphExec_t exec;
     n2h_plan* plan;
     int32_t n_steps;
+    int32_t phase;  // first step mod 4 of every window it replays
 };
 
 namespace {
 
-int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream) {
+// Enqueue n_steps step kernels.  graph: the steps come from the device counter (the window replays
+// anywhere with the same phase); eager: from the host mirror.  Either way the counter and the mirror
+// advance by n_steps.
+int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     StepArgs a;
     a.n_neurons = p->desc.n_neurons;
     a.n_groups = p->tables.n_groups;
@@ -342,16 +497,34 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream) {
     a.ref = p->ref;
     a.ring = p->ring;
     a.bits = p->bits;
-    a.step_ctr = p->step_ctr;
+    a.step_ctr = graph ? p->step_ctr : nullptr;
+    a.step_imm = p->host_step;
+    a.phase = static_cast<int32_t>(p->host_step & (kListSlots - 1));
+    a.pad = 0;
+    a.spikes = p->spikes;
+    a.list_stride = 1 + static_cast<int64_t>(kWave) * a.n_groups;
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
+    const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
-        if (p->desc.model == N2H_MODEL_LIF_DELTA)
-            hipLaunchKernelGGL(step_kernel<N2H_MODEL_LIF_DELTA>, grid, block, p->lds_bytes, stream, a, k);
-        else
-            hipLaunchKernelGGL(step_kernel<N2H_MODEL_LIF_EXP>, grid, block, p->lds_bytes, stream, a, k);
+        if (p->desc.model == N2H_MODEL_LIF_DELTA) {
+            if (words <= 8)
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 8>), grid, block, p->lds_bytes, stream, a, k);
+            else if (words <= 32)
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 32>), grid, block, p->lds_bytes, stream, a, k);
+            else
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 0>), grid, block, p->lds_bytes, stream, a, k);
+        } else {
+            if (words <= 8)
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 8>), grid, block, p->lds_bytes, stream, a, k);
+            else if (words <= 32)
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 32>), grid, block, p->lds_bytes, stream, a, k);
+            else
+                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 0>), grid, block, p->lds_bytes, stream, a, k);
+        }
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, p->step_ctr, n_steps);
     N2H_HIP(hipGetLastError());
+    p->host_step += n_steps;
     return N2H_OK;
 }
 
@@ -359,7 +532,7 @@ int check_ready(n2h_plan* p) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     if (!p->v || !p->ref || (p->desc.model == N2H_MODEL_LIF_EXP && !p->isyn))
         return fail(N2H_ESTATE, "state not bound (n2h_bind_state)");
-    if (!p->ring || !p->bits) return fail(N2H_ESTATE, "ring not bound (n2h_bind_ring)");
+    if (!p->ring || !p->bits || !p->spikes) return fail(N2H_ESTATE, "ring not bound (n2h_bind_ring)");
     return N2H_OK;
 }
 
@@ -452,11 +625,12 @@ int n2h_bind_state(n2h_plan* p, float* v, float* i_syn, int32_t* ref) {
     return N2H_OK;
 }
 
-int n2h_bind_ring(n2h_plan* p, float* ring, uint64_t* spike_bits, int64_t bit_slots) {
+int n2h_bind_ring(n2h_plan* p, float* ring, uint64_t* spike_bits, int64_t bit_slots, int32_t* spike_list) {
     if (!p) return fail(N2H_EINVAL, "null plan");
-    if (!ring || !spike_bits) return fail(N2H_EINVAL, "null ring pointer");
+    if (!ring || !spike_bits || !spike_list) return fail(N2H_EINVAL, "null ring pointer");
     if (bit_slots < 2) return fail(N2H_EINVAL, "bit_slots must be at least 2");
     p->ring = ring;
+    p->spikes = spike_list;
     p->bits = reinterpret_cast<unsigned long long*>(spike_bits);
     p->bit_slots = bit_slots;
     return N2H_OK;
@@ -467,6 +641,7 @@ int n2h_set_step(n2h_plan* p, int64_t step, void* stream) {
     if (step < 0) return fail(N2H_EINVAL, "negative step");
     hipLaunchKernelGGL(set_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), p->step_ctr, step);
     N2H_HIP(hipGetLastError());
+    p->host_step = step;
     return N2H_OK;
 }
 
@@ -481,7 +656,7 @@ int n2h_step(n2h_plan* p, int32_t n_steps, void* stream) {
     if (rc) return rc;
     if (n_steps < 0) return fail(N2H_EINVAL, "negative n_steps");
     if (n_steps == 0) return N2H_OK;
-    return launch_steps(p, n_steps, static_cast<hipStream_t>(stream));
+    return launch_steps(p, n_steps, static_cast<hipStream_t>(stream), false);
 }
 
 int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out) {
@@ -498,7 +673,9 @@ int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out
         (void)hipStreamDestroy(cap);
         return fail(N2H_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
-    rc = launch_steps(p, n_steps, cap);
+    const int64_t at = p->host_step;
+    rc = launch_steps(p, n_steps, cap, true);
+    p->host_step = at;  // captured, not run
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(cap, &graph);
     (void)hipStreamDestroy(cap);
@@ -513,7 +690,8 @@ int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out
         (void)hipGraphDestroy(graph);
         return fail(N2H_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    n2h_graph* gr = new (std::nothrow) n2h_graph{graph, exec, p, n_steps};
+    n2h_graph* gr = new (std::nothrow)
+        n2h_graph{graph, exec, p, n_steps, static_cast<int32_t>(at & (kListSlots - 1))};
     if (!gr) {
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
@@ -525,7 +703,11 @@ int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out
 
 int n2h_graph_launch(n2h_graph* gr, void* stream) {
     if (!gr) return fail(N2H_EINVAL, "null graph");
+    if ((gr->plan->host_step & (kListSlots - 1)) != gr->phase)
+        return fail(N2H_EINVAL, "graph captured at step phase %d (mod %d), launched at step %lld", gr->phase,
+                    kListSlots, static_cast<long long>(gr->plan->host_step));
     N2H_HIP(hipGraphLaunch(gr->exec, static_cast<hipStream_t>(stream)));
+    gr->plan->host_step += gr->n_steps;
     return N2H_OK;
 }
 

@@ -209,6 +209,8 @@ class Runtime:
         self.ring = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
         self.bit_slots = 2
         self.bits = torch.zeros(self.bit_slots, self.n_groups, dtype=torch.int64, device=self.device)
+        # the spike lists of the last three steps: count, then source slots (64 * group + lane)
+        self.spike_list = torch.zeros(4, 1 + GROUP * self.n_groups, dtype=torch.int32, device=self.device)
         self.record_from = 0
         self.step = 0
         self._graphs: dict = {}
@@ -237,8 +239,8 @@ class Runtime:
 
     def _bind_ring(self) -> None:
         lib = _lib.load()
-        _lib.check(lib.n2h_bind_ring(self._plan, self.ring.data_ptr(), self.bits.data_ptr(), self.bit_slots),
-                   "n2h_bind_ring")
+        _lib.check(lib.n2h_bind_ring(self._plan, self.ring.data_ptr(), self.bits.data_ptr(), self.bit_slots,
+                                     self.spike_list.data_ptr()), "n2h_bind_ring")
         for g in self._graphs.values():
             lib.n2h_graph_destroy(g)
         self._graphs = {}
@@ -278,12 +280,19 @@ class Runtime:
         lib = _lib.load()
         sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         done = 0
-        if graph_window and n_steps >= graph_window:
-            g = self._graphs.get(graph_window)
+        if graph_window and graph_window % 4:
+            raise ValueError("graph windows must be a multiple of 4 steps (the spike lists' phase)")
+        if graph_window and self.step % 4 and n_steps >= graph_window + 4:
+            lead = 4 - self.step % 4  # eager steps up to the captured phase
+            _lib.check(lib.n2h_step(self._plan, lead, sv), "n2h_step")
+            done = lead
+        if graph_window and n_steps - done >= graph_window:
+            key = (graph_window, (self.step + done) % 4)
+            g = self._graphs.get(key)
             if g is None:
                 gp = ctypes.c_void_p()
                 _lib.check(lib.n2h_graph_create(self._plan, graph_window, sv, ctypes.byref(gp)), "n2h_graph_create")
-                g = self._graphs[graph_window] = gp
+                g = self._graphs[key] = gp
             while n_steps - done >= graph_window:
                 _lib.check(lib.n2h_graph_launch(g, sv), "n2h_graph_launch")
                 done += graph_window
@@ -292,6 +301,18 @@ class Runtime:
         self.step += n_steps
         if not self.net.record:
             self.record_from = max(self.step - 1, 0)
+
+    def force_spikes(self, step: int, spiking: np.ndarray) -> None:
+        """Replace the spikes of `step` (bool [n]) before step + 1 runs -- the host-commanded firing of
+        the reference's test fixtures (test/runtimes.py:24-44): bitmap word and spike list both."""
+        words = torch.from_numpy(raster_to_bits(np.asarray(spiking, bool)[None], self.layout)[0]).to(self.device)
+        self.bits[step % self.bit_slots] = words
+        ids = np.nonzero(np.asarray(spiking, bool))[0]
+        slots = (self.layout.group_of.cpu().numpy()[ids] * GROUP + self.layout.local_of.cpu().numpy()[ids])
+        row = torch.zeros(1 + GROUP * self.n_groups, dtype=torch.int32)
+        row[0] = len(slots)
+        row[1:1 + len(slots)] = torch.from_numpy(slots[::-1].astype(np.int32).copy())  # any order
+        self.spike_list[step % 4] = row.to(self.device)
 
     def set_step(self, step: int) -> None:
         """Move the step counter (the Philox counter and ring slot follow the absolute step)."""

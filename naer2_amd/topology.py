@@ -105,25 +105,33 @@ def group_layout(pop_sizes: list[int], device="cpu") -> GroupLayout:
 
 @dataclass
 class Segments:
-    seg_off: torch.Tensor  # int64 [n * G + 1]
+    seg_off: torch.Tensor  # int64 [64 * G * G + 1]
     links: torch.Tensor    # int32 [E, 2]: (delay << 16 | local target, weight bits)
 
 
+def slot_of(layout: GroupLayout) -> torch.Tensor:
+    """int64 [n]: a neuron's source slot, 64 * (its group) + (its index in the group) -- the id the step's
+    spike lists and bitmaps carry."""
+    return layout.group_of * GROUP + layout.local_of
+
+
 def segments(canon: Canonical, layout: GroupLayout) -> Segments:
-    """Group segments: the links of source s into group g at [seg_off[s*G + g], seg_off[s*G + g + 1]), in
-    canonical order (a stable partition of each source's links by target group)."""
+    """Group segments: the links of source slot p into group g at [seg_off[p*G + g], seg_off[p*G + g + 1]),
+    in canonical order (a stable partition of each source's links by target group); padding slots of
+    partly filled groups have no links."""
     dev = canon.tgt.device
     n, G = canon.n, layout.n_groups
     group_of = layout.group_of.to(dev)
     local_of = layout.local_of.to(dev)
-    src = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int64), canon.out_degree())
+    slots = GROUP * G
+    src = torch.repeat_interleave(slot_of(layout).to(dev), canon.out_degree())
     tg = canon.tgt.to(torch.int64)
     key = src * G + group_of[tg]
     del src
     order = torch.argsort(key, stable=True)
-    counts = torch.bincount(key, minlength=n * G)
+    counts = torch.bincount(key, minlength=slots * G)
     del key
-    seg_off = torch.zeros(n * G + 1, dtype=torch.int64, device=dev)
+    seg_off = torch.zeros(slots * G + 1, dtype=torch.int64, device=dev)
     seg_off[1:] = torch.cumsum(counts, 0)
     del counts
     x = (canon.d.to(torch.int64)[order] << 16) | local_of[tg[order]]

@@ -8,7 +8,6 @@ import numpy as np
 import pytest
 import torch
 
-from naer2_amd.network import raster_to_bits
 from naer2_amd.networks import brunel, pd14
 from tests.gpu_util import (ProbeNode, assert_state_equal, bits_equal, network_from_topology, oracle_run,
                             oracle_state)
@@ -49,7 +48,8 @@ def test_pd14_small_bit_exact_1000_steps():
 
 
 @pytest.mark.parametrize("n,n_links,max_delay,p_spike", [
-    (200, 6000, 7, 0.1),      # a few spikes a step
+    (200, 6000, 7, 0.1),      # a few spikes a step: the sorted-list path
+    (500, 20000, 9, 0.125),   # around 64 spikes a step: both paths
     (3000, 120000, 40, 0.6),  # bursts: > 512 spiking sources and > 1024 links a group per step
     (130, 4000, 128, 0.3),    # delays up to the ring's 128 slots, two 64-bit touched masks
 ])
@@ -63,10 +63,9 @@ def test_forced_spikes_deliver_like_the_oracle(n, n_links, max_delay, p_spike):
     st = oracle_state(rt)
     rng = np.random.default_rng(n + 1)
     forced = rng.random((25, n)) < p_spike
-    words = torch.from_numpy(raster_to_bits(forced, rt.layout)).to("cuda")
     for s in range(forced.shape[0]):
         if s > 0:
-            rt.bits[(s - 1) % rt.bit_slots] = words[s - 1]
+            rt.force_spikes(s - 1, forced[s - 1])
             st.prev_spk[:] = forced[s - 1]
         rt.run(1)
         oracle_run(rt, st, s, 1, record=False)
@@ -80,9 +79,11 @@ def test_graph_replays_equal_eager_steps():
     a, _ = brunel(device="cuda", seed=5)
     b, _ = brunel(device="cuda", seed=5)
     ra, rb = a.build(), b.build()
+    a.run(3, graph_window=0)  # off the captured phase: run() steps eagerly up to it
+    b.run(3, graph_window=0)
     for _ in range(6):  # replays with eager work between them
-        a.run(50, graph_window=50)
-        b.run(50, graph_window=0)
+        a.run(52, graph_window=48)
+        b.run(52, graph_window=0)
         torch.cuda.synchronize()
         _ = (ra.v * 2).sum().item()
         assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())

@@ -17,7 +17,7 @@ from naer2_amd import _lib
 from naer2_amd.network import assemble, bits_to_raster, raster_to_bits
 from naer2_amd.networks import PD14, brunel, pd14
 from naer2_amd.nodes import FixedInDegree, FixedTotalNumber
-from naer2_amd.topology import canonical, delay_steps, group_layout, segments
+from naer2_amd.topology import canonical, delay_steps, group_layout, segments, slot_of
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -58,7 +58,7 @@ def test_plan_create_rejects_bad_arguments():
     assert lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan)) == -1
     assert "poisson_thr" in _lib.last_error()
     assert lib.n2h_step(None, 1, None) == -1
-    assert lib.n2h_bind_ring(None, None, None, 2) == -1
+    assert lib.n2h_bind_ring(None, None, None, 2, None) == -1
 
 
 def test_fixed_in_degree_gives_every_target_exactly_k_inputs():
@@ -109,6 +109,7 @@ def test_group_layout_never_straddles_populations():
     assert (lay.group_info & 0xFFFF).tolist() == [64, 64, 2, 64, 1]
     assert (lay.group_info >> 16).tolist() == [0, 0, 0, 1, 2]
     assert lay.group_of[129].item() == 2 and lay.local_of[129].item() == 1
+    assert slot_of(lay)[129].item() == 2 * 64 + 1
 
 
 def test_segments_hold_each_sources_links_per_group_in_canonical_order():
@@ -125,10 +126,13 @@ def test_segments_hold_each_sources_links_per_group_in_canonical_order():
     links = seg.links.numpy()
     gf = lay.group_first.numpy()
     sp = can.src_ptr.numpy()
+    slot = (lay.group_of * 64 + lay.local_of).numpy()
+    assert seg.seg_off.numel() == 64 * G * G + 1
     for s in range(n):
         row = range(sp[s], sp[s + 1])
         for g in range(G):
-            lo, hi = int(seg.seg_off[s * G + g]), int(seg.seg_off[s * G + g + 1])
+            p = slot[s]
+            lo, hi = int(seg.seg_off[p * G + g]), int(seg.seg_off[p * G + g + 1])
             expect = [l for l in row if lay.group_of[can.tgt[l]].item() == g]
             got = links[lo:hi]
             assert len(got) == len(expect)
