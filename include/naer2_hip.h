@@ -106,8 +106,17 @@ typedef struct n2h_tables {
     int64_t n_links;
 } n2h_tables;
 
+/* Launch mapping of a batch (chosen by naer2_amd/autotune.py). */
+typedef struct n2h_tuning {
+    int32_t order;    /* 0: plans in the given order; 1: plans with the most links first */
+    int32_t xcd;      /* 0 or 1: block b runs tile b; k > 1: block b runs tile (b % k) * ceil(T / k) + b / k, so
+                         one plan's consecutive groups share an XCD (blocks are dealt to the k XCDs in turn) */
+    int32_t reserved[2];
+} n2h_tuning;
+
 typedef struct n2h_plan n2h_plan;
 typedef struct n2h_graph n2h_graph;
+typedef struct n2h_batch n2h_batch;
 
 int n2h_version(void);
 int n2h_last_error(char* buf, size_t len);
@@ -141,6 +150,14 @@ int n2h_step(n2h_plan* plan, int32_t n_steps, void* stream);
 int n2h_graph_create(n2h_plan* plan, int32_t n_steps, void* stream, n2h_graph** out);
 int n2h_graph_launch(n2h_graph* graph, void* stream);
 int n2h_graph_destroy(n2h_graph* graph);
+
+/* A batch: the groups of up to 256 independent plans (same model, same step) in one launch per step.
+ * Replaces naer2/fusion.py:31-141 (FusedRound over several runtimes).  The plans' bound buffers must stay
+ * in place while the batch exists; a plan stepped on its own leaves the batch unusable (N2H_ESTATE). */
+int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* tuning, n2h_batch** out);
+int n2h_batch_destroy(n2h_batch* batch);
+int n2h_batch_step(n2h_batch* batch, int32_t n_steps, void* stream);
+int n2h_batch_graph_create(n2h_batch* batch, int32_t n_steps, void* stream, n2h_graph** out);
 
 /* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */
 int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);

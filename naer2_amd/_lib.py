@@ -24,7 +24,8 @@ MAX_SLOTS = 128
 EXPORTS = (
     "n2h_version", "n2h_last_error", "n2h_plan_create", "n2h_plan_destroy", "n2h_bind_state",
     "n2h_bind_ring", "n2h_set_step", "n2h_get_step", "n2h_step", "n2h_graph_create", "n2h_graph_launch",
-    "n2h_graph_destroy", "n2h_status",
+    "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
+    "n2h_batch_graph_create",
 )
 
 
@@ -55,6 +56,13 @@ class Tables(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("xcd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+
+    def as_dict(self) -> dict:
+        return {"order": int(self.order), "xcd": int(self.xcd)}
+
+
 _lib = None
 
 
@@ -82,6 +90,10 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_graph_launch": ([vp, vp], i32),
         "n2h_graph_destroy": ([vp], i32),
         "n2h_status": ([vp, ctypes.POINTER(i32), i32], i32),
+        "n2h_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)], i32),
+        "n2h_batch_destroy": ([vp], i32),
+        "n2h_batch_step": ([vp, i32, vp], i32),
+        "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,75 @@
+"""Launch-mapping autotuner of batched sweeps (FusedRound): plan order and XCD-aware block mapping.
+
+Every candidate mapping computes exactly the same results (each target group's work does not depend on
+where or when it runs), so the tuner times the candidates on the batch's own first steps -- the warmup --
+and keeps the fastest.  Choices are cached per device and batch signature in a JSON file
+($NAER2_AUTOTUNE_CACHE, default ~/.cache/naer2_amd/autotune.json).
+
+Candidates: order 0 (plans as given) or 1 (most links first, so the heaviest groups start first and light
+ones fill the tail); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8
+(a plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+from pathlib import Path
+
+import torch
+
+from ._lib import Tuning
+from .fusion import FusedRound
+
+CANDIDATES = [{"order": o, "xcd": x} for o in (0, 1) for x in (0, 8)]
+
+
+def cache_path() -> Path:
+    return Path(os.environ.get("NAER2_AUTOTUNE_CACHE", Path.home() / ".cache" / "naer2_amd" / "autotune.json"))
+
+
+def signature(runtimes) -> str:
+    h = hashlib.sha1()
+    for rt in runtimes:
+        h.update(f"{rt.n}:{rt.n_groups}:{rt.n_links}:{rt.d_slots};".encode())
+    return f"{torch.cuda.get_device_name(runtimes[0].device)}|{len(runtimes)}|{h.hexdigest()[:16]}"
+
+
+def _load() -> dict:
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(table: dict) -> None:
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(table, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int = 0, use_cache: bool = True):
+    """(best mapping as a dict, {candidate: seconds}).  Advances every runtime by steps x candidates."""
+    key = signature(runtimes)
+    table = _load() if use_cache else {}
+    if use_cache and key in table:
+        return table[key]["best"], table[key]["times"]
+    times = {}
+    for cand in CANDIDATES:
+        fr = FusedRound(runtimes, Tuning(order=cand["order"], xcd=cand["xcd"]), reserve_steps=reserve_steps)
+        fr.launch(graph_window, graph_window)  # capture + first replay, untimed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fr.launch(steps, graph_window)
+        torch.cuda.synchronize()
+        times[json.dumps(cand, sort_keys=True)] = time.perf_counter() - t0
+        fr.close()
+    best = json.loads(min(times, key=times.get))
+    if use_cache:
+        table[key] = {"best": best, "times": times}
+        _save(table)
+    return best, times

@@ -39,9 +39,10 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kCapS = 512;   // spiking sources staged in LDS per pass
-constexpr int kCapL = 1024;  // links staged in LDS per pass
-constexpr int kListSlots = 4;  // spike lists of steps s-1 (read), s (written), s+1 (cleared), s-2 (idle)
+constexpr int kCapS = 256;   // spiking sources staged in LDS per pass
+constexpr int kCapL = 512;   // links staged in LDS per pass (LDS per group: 9.2 KB + 256 B per ring slot)
+constexpr int kListSlots = 4;
+constexpr int kMaxBatch = 256;  // plans in one batched launch  // spike lists of steps s-1 (read), s (written), s+1 (cleared), s-2 (idle)
 
 thread_local char g_err[512] = "";
 
@@ -81,7 +82,7 @@ struct StepArgs {
     const int64_t* step_ctr; // graph launches: the window's first step; eager launches: null
     int64_t step_imm;        // eager launches: the first step of the launch sequence
     int32_t phase;           // (first step of the sequence) mod 4: the list slots without a load
-    int32_t pad;
+    int32_t slot_bits;       // bits of a source slot: ceil(log2(64 * n_groups))
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -113,18 +114,18 @@ __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
     return x;
 }
 
-// Bitonic sort of one int per lane, ascending over the wavefront.
-__device__ __forceinline__ int wave_sort(int v, int lane) {
-#pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int o = __shfl_xor(v, j, kWave);
-            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-            v = keep_min ? min(v, o) : max(v, o);
-        }
+// Rank of each valid lane's key among the valid lanes' keys (keys distinct, < 2^bits), from one ballot
+// per key bit, most significant first: no cross-lane data movement.
+__device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
+    unsigned long long agree = __ballot(valid);  // valid lanes equal to this key on the bits seen so far
+    int rank = 0;
+    for (int p = bits - 1; p >= 0; --p) {
+        const bool one = (key >> p) & 1;
+        const unsigned long long b = __ballot(valid && one);
+        if (one) rank += __popcll(agree & ~b);
+        agree &= one ? b : ~b;
     }
-    return v;
+    return rank;
 }
 
 // Shared-memory staging of one workgroup.
@@ -136,62 +137,49 @@ struct Stage {
     unsigned long long touched[kWave][2];  // per target: delays 1..64 and 65..128 with a partial sum
 };
 
-// Bitonic sort of (key, value) pairs, one per lane, ascending by key (keys distinct).
-__device__ __forceinline__ void wave_sort_pairs(int& key, float& val, int lane) {
-#pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int ok = __shfl_xor(key, j, kWave);
-            const float ov = __shfl_xor(val, j, kWave);
-            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-            const bool take = keep_min ? (ok < key) : (ok > key);
-            key = take ? ok : key;
-            val = take ? ov : val;
-        }
-    }
-}
-
-// Walk of staged links [b0, min(b0 + 64, nl)): one link per lane.  The links are sorted by
-// (cell = (delay - 1) * 64 + target, position) so that the links of one cell form a run in contract order;
-// the first lane of each run adds the run's weights one after another to the cell's partial sum (cells of
-// different runs are distinct, so the runs proceed in parallel).  Equivalent to the target lanes walking
-// the links one by one in order.
+// Walk of staged links [b0, min(b0 + 64, nl)): one link per lane.  Lanes whose links hit the same cell
+// (delay, target) find each other with one ballot per cell bit; the lowest of them adds the weights of
+// all of them to the cell's partial sum in lane (= contract) order, while the other cells proceed in
+// parallel.  Equivalent to the target lanes walking the links one by one in order.
 __device__ __forceinline__ void walk_batch(Stage& st, float* acc, int lane, int b0, int nl) {
     const int e = b0 + lane;
-    int key = INT_MAX;
+    const bool valid = e < nl;
+    int cell = 0;
     float w = 0.0f;
-    if (e < nl) {
+    if (valid) {
         const uint2 ln = st.links[e];
-        const int cell = (static_cast<int>(ln.x >> 16) - 1) * kWave + static_cast<int>(ln.x & 0xffffu);
-        key = (cell << 6) | lane;
+        cell = (static_cast<int>(ln.x >> 16) - 1) * kWave + static_cast<int>(ln.x & 0xffffu);
         w = __uint_as_float(ln.y);
     }
-    wave_sort_pairs(key, w, lane);
-    const bool valid = key != INT_MAX;
-    const int cell = key >> 6;
-    const int prev_cell = __shfl_up(cell, 1, kWave);
-    const bool head = valid && (lane == 0 || prev_cell != cell);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long valids = __ballot(valid);
-    // run of a head: up to the next head or the end of the valid lanes
-    const unsigned long long after = (lane == kWave - 1) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-    const int end = after ? (__ffsll(static_cast<long long>(after)) - 1) : __popcll(valids);
-    int len = head ? end - lane : 0;
-    int max_len = len;
+    unsigned long long same = __ballot(valid);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) max_len = max(max_len, __shfl_xor(max_len, off, kWave));
+    for (int p = 0; p < 13; ++p) {  // cell < 128 * 64
+        const bool one = (cell >> p) & 1;
+        const unsigned long long b = __ballot(one);
+        same &= one ? b : ~b;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const bool head = valid && !(same & below);
+    unsigned long long rest = head ? (same & ~below & ~(1ull << lane)) : 0ull;
+    int run = __popcll(rest);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) run = max(run, __shfl_xor(run, off, kWave));
     float a = 0.0f;
     if (head) {
         const int t = cell & (kWave - 1);
         const int b = cell >> 6;
         const unsigned long long bit = 1ull << (b & 63);
+        const float prior = acc[cell];
         const unsigned long long old = atomicOr(&st.touched[t][b >> 6], bit);
-        a = ((old & bit) ? acc[cell] : 0.0f) + w;
+        a = ((old & bit) ? prior : 0.0f) + w;
     }
-    for (int r = 1; r < max_len; ++r) {
-        const float v = __shfl_down(w, r, kWave);
-        if (r < len) a = a + v;
+    for (int r = 0; r < run; ++r) {
+        const int src = rest ? (__ffsll(static_cast<long long>(rest)) - 1) : lane;
+        const float v = __shfl(w, src, kWave);
+        if (rest) {
+            a = a + v;
+            rest &= rest - 1ull;
+        }
     }
     if (head) acc[cell] = a;
 }
@@ -260,11 +248,7 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float
 // One step s of the whole network for target group g (one wavefront).
 // KW: bitmap words per lane held in registers on the burst path (0: read them in a loop).
 template <int MODEL, int KW>
-__global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
-    __shared__ Stage st;
-    extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
-
-    const int g = blockIdx.x;
+__device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, Stage& st, float* acc) {
     const int lane = threadIdx.x;
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
@@ -300,8 +284,9 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     st.touched[lane][1] = 0ull;
     if (n_prev <= kWave) {
         // sparse path: sort the (atomically appended) source slots, then deliver them
-        const int v = wave_sort(lane < n_prev ? my_src : INT_MAX, lane);
-        st.sp[lane] = v;
+        const bool has = lane < n_prev;
+        const int rank = wave_rank(my_src, has, a.slot_bits);
+        if (has) st.sp[rank] = my_src;
         __syncthreads();
         if (n_prev > 0) deliver_pass(a, st, acc, g, lane, n_prev);
     } else {
@@ -442,7 +427,57 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     if (g == 0 && lane == 0) a.spikes[((slot_cur + 1) & (kListSlots - 1)) * a.list_stride] = 0;
 }
 
+template <int MODEL, int KW>
+__global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
+    __shared__ Stage st;
+    extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
+    step_group<MODEL, KW>(a, blockIdx.x, k, st, acc);
+}
+
+// Batched step: the target groups of many independent networks (plans) in one grid.  Tiles are the
+// plans' groups in plan order; with xcd = k > 1, block b runs tile (b % k) * ceil(T / k) + b / k, so that
+// consecutive tiles -- one plan's groups -- land on the same XCD (blocks are dealt to the k XCDs in turn)
+// and share its L2.
+struct BatchArgs {
+    const StepArgs* plans;     // device [n_plans]
+    const int64_t* step_ctr;   // graph launches: the window's first step; eager launches: null
+    int64_t step_imm;
+    int32_t phase;
+    int32_t n_plans;
+    int32_t n_tiles;
+    int32_t xcd;
+    int32_t first[kMaxBatch + 1];  // first tile of each plan
+};
+
+template <int MODEL, int KW>
+__global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
+    __shared__ Stage st;
+    extern __shared__ float acc[];
+    const int b = blockIdx.x;
+    int tile = b;
+    if (ba.xcd > 1) {
+        const int per = (ba.n_tiles + ba.xcd - 1) / ba.xcd;
+        tile = (b % ba.xcd) * per + b / ba.xcd;
+    }
+    if (tile >= ba.n_tiles) return;
+    int lo = 0, hi = ba.n_plans;  // first[lo] <= tile < first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ba.first[mid] <= tile) lo = mid;
+        else hi = mid;
+    }
+    StepArgs a = ba.plans[lo];
+    a.step_ctr = ba.step_ctr;
+    a.step_imm = ba.step_imm;
+    a.phase = ba.phase;
+    step_group<MODEL, KW>(a, tile - ba.first[lo], k, st, acc);
+}
+
 __global__ void advance_kernel(int64_t* ctr, int32_t n) { *ctr += n; }
+__global__ void advance_many_kernel(int64_t* const* ctrs, int32_t n_ctrs, int32_t n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_ctrs) *ctrs[i] += n;
+}
 __global__ void set_kernel(int64_t* ctr, int64_t v) { *ctr = v; }
 
 }  // namespace
@@ -465,10 +500,26 @@ struct n2h_plan {
     int device;
 };
 
+struct n2h_batch {
+    n2h_plan** plans;        // in launch order
+    int32_t n_plans;
+    int32_t model;
+    int32_t words;           // bitmap words per lane on the burst path (kernel variant)
+    n2h_tuning tuning;
+    StepArgs* args_dev;      // device [n_plans]
+    int64_t** ctrs_dev;      // device [n_plans]: the plans' step counters
+    int64_t* step_ctr;       // the batch's own counter (graph windows)
+    int64_t host_step;
+    int32_t n_tiles;
+    size_t lds_bytes;
+    int32_t first[kMaxBatch + 1];
+};
+
 struct n2h_graph {
     hipGraph_t graph;
     hipGraphExec_t exec;
-    n2h_plan* plan;
+    n2h_plan* plan;          // a plan's window, or
+    n2h_batch* batch;        // a batch's window
     int32_t n_steps;
     int32_t phase;  // first step mod 4 of every window it replays
 };
@@ -478,8 +529,9 @@ namespace {
 // Enqueue n_steps step kernels.  graph: the steps come from the device counter (the window replays
 // anywhere with the same phase); eager: from the host mirror.  Either way the counter and the mirror
 // advance by n_steps.
-int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
+StepArgs make_args(const n2h_plan* p) {
     StepArgs a;
+    memset(&a, 0, sizeof(a));
     a.n_neurons = p->desc.n_neurons;
     a.n_groups = p->tables.n_groups;
     a.d_slots = p->desc.d_slots;
@@ -497,12 +549,19 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     a.ref = p->ref;
     a.ring = p->ring;
     a.bits = p->bits;
-    a.step_ctr = graph ? p->step_ctr : nullptr;
+    a.step_ctr = nullptr;
     a.step_imm = p->host_step;
     a.phase = static_cast<int32_t>(p->host_step & (kListSlots - 1));
-    a.pad = 0;
+    a.slot_bits = 6;
+    while ((1ll << a.slot_bits) < static_cast<int64_t>(kWave) * a.n_groups) ++a.slot_bits;
     a.spikes = p->spikes;
     a.list_stride = 1 + static_cast<int64_t>(kWave) * a.n_groups;
+    return a;
+}
+
+int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
+    StepArgs a = make_args(p);
+    a.step_ctr = graph ? p->step_ctr : nullptr;
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
@@ -525,6 +584,83 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, p->step_ctr, n_steps);
     N2H_HIP(hipGetLastError());
     p->host_step += n_steps;
+    return N2H_OK;
+}
+
+// Capture the launches fn enqueues on a private stream into an instantiated graph.
+template <typename F>
+int capture(F fn, hipGraph_t* graph_out, hipGraphExec_t* exec_out) {
+    hipStream_t cap;
+    N2H_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cap);
+        return fail(N2H_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    int rc = fn(cap);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cap, &graph);
+    (void)hipStreamDestroy(cap);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(N2H_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(N2H_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *graph_out = graph;
+    *exec_out = exec;
+    return N2H_OK;
+}
+
+// Every plan of a batch must still be at the batch's step (none stepped on its own since).
+int batch_in_step(const n2h_batch* bt) {
+    for (int32_t i = 0; i < bt->n_plans; ++i)
+        if (bt->plans[i]->host_step != bt->host_step)
+            return fail(N2H_ESTATE, "plan %d is at step %lld, the batch at %lld", i,
+                        static_cast<long long>(bt->plans[i]->host_step), static_cast<long long>(bt->host_step));
+    return N2H_OK;
+}
+
+template <int MODEL>
+void launch_batch_model(const n2h_batch* bt, const BatchArgs& ba, dim3 grid, hipStream_t stream, int32_t k) {
+    const dim3 block(kWave);
+    if (bt->words <= 8)
+        hipLaunchKernelGGL((batch_kernel<MODEL, 8>), grid, block, bt->lds_bytes, stream, ba, k);
+    else if (bt->words <= 32)
+        hipLaunchKernelGGL((batch_kernel<MODEL, 32>), grid, block, bt->lds_bytes, stream, ba, k);
+    else
+        hipLaunchKernelGGL((batch_kernel<MODEL, 0>), grid, block, bt->lds_bytes, stream, ba, k);
+}
+
+// Enqueue n_steps batched step kernels and advance the batch's and every plan's device counter.
+int launch_batch(n2h_batch* bt, int32_t n_steps, hipStream_t stream, bool graph) {
+    BatchArgs ba;
+    memset(&ba, 0, sizeof(ba));
+    ba.plans = bt->args_dev;
+    ba.step_ctr = graph ? bt->step_ctr : nullptr;
+    ba.step_imm = bt->host_step;
+    ba.phase = static_cast<int32_t>(bt->host_step & (kListSlots - 1));
+    ba.n_plans = bt->n_plans;
+    ba.n_tiles = bt->n_tiles;
+    ba.xcd = bt->tuning.xcd;
+    for (int32_t i = 0; i <= bt->n_plans; ++i) ba.first[i] = bt->first[i];
+    int64_t blocks = bt->n_tiles;
+    if (ba.xcd > 1) blocks = static_cast<int64_t>(ba.xcd) * ((bt->n_tiles + ba.xcd - 1) / ba.xcd);
+    const dim3 grid(static_cast<unsigned>(blocks));
+    for (int32_t k = 0; k < n_steps; ++k) {
+        if (bt->model == N2H_MODEL_LIF_DELTA) launch_batch_model<N2H_MODEL_LIF_DELTA>(bt, ba, grid, stream, k);
+        else launch_batch_model<N2H_MODEL_LIF_EXP>(bt, ba, grid, stream, k);
+    }
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, bt->step_ctr, n_steps);
+    hipLaunchKernelGGL(advance_many_kernel, dim3((bt->n_plans + 63) / 64), dim3(64), 0, stream, bt->ctrs_dev,
+                       bt->n_plans, n_steps);
+    N2H_HIP(hipGetLastError());
+    bt->host_step += n_steps;
     return N2H_OK;
 }
 
@@ -666,32 +802,14 @@ int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out
     int rc = check_ready(p);
     if (rc) return rc;
     if (n_steps <= 0) return fail(N2H_EINVAL, "n_steps must be positive");
-    hipStream_t cap;
-    N2H_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) {
-        (void)hipStreamDestroy(cap);
-        return fail(N2H_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
-    }
     const int64_t at = p->host_step;
-    rc = launch_steps(p, n_steps, cap, true);
-    p->host_step = at;  // captured, not run
     hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(cap, &graph);
-    (void)hipStreamDestroy(cap);
-    if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-    }
-    if (e != hipSuccess) return fail(N2H_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return fail(N2H_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    }
+    rc = capture([&](hipStream_t cap) { return launch_steps(p, n_steps, cap, true); }, &graph, &exec);
+    p->host_step = at;  // captured, not run
+    if (rc) return rc;
     n2h_graph* gr = new (std::nothrow)
-        n2h_graph{graph, exec, p, n_steps, static_cast<int32_t>(at & (kListSlots - 1))};
+        n2h_graph{graph, exec, p, nullptr, n_steps, static_cast<int32_t>(at & (kListSlots - 1))};
     if (!gr) {
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
@@ -703,11 +821,148 @@ int n2h_graph_create(n2h_plan* p, int32_t n_steps, void* stream, n2h_graph** out
 
 int n2h_graph_launch(n2h_graph* gr, void* stream) {
     if (!gr) return fail(N2H_EINVAL, "null graph");
-    if ((gr->plan->host_step & (kListSlots - 1)) != gr->phase)
+    if (gr->batch) {
+        int rc = batch_in_step(gr->batch);
+        if (rc) return rc;
+    }
+    const int64_t at = gr->batch ? gr->batch->host_step : gr->plan->host_step;
+    if ((at & (kListSlots - 1)) != gr->phase)
         return fail(N2H_EINVAL, "graph captured at step phase %d (mod %d), launched at step %lld", gr->phase,
-                    kListSlots, static_cast<long long>(gr->plan->host_step));
+                    kListSlots, static_cast<long long>(at));
     N2H_HIP(hipGraphLaunch(gr->exec, static_cast<hipStream_t>(stream)));
-    gr->plan->host_step += gr->n_steps;
+    if (gr->batch) {
+        gr->batch->host_step += gr->n_steps;
+        for (int32_t i = 0; i < gr->batch->n_plans; ++i) gr->batch->plans[i]->host_step += gr->n_steps;
+    } else {
+        gr->plan->host_step += gr->n_steps;
+    }
+    return N2H_OK;
+}
+
+int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* tuning, n2h_batch** out) {
+    if (!plans || !out) return fail(N2H_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_plans < 1 || n_plans > kMaxBatch) return fail(N2H_EINVAL, "n_plans %d outside [1, %d]", n_plans, kMaxBatch);
+    n2h_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
+    if (tu.xcd < 0 || tu.xcd > 64) return fail(N2H_EINVAL, "tuning.xcd must be in [0, 64]");
+    for (int32_t i = 0; i < n_plans; ++i) {
+        int rc = check_ready(plans[i]);
+        if (rc) return rc;
+        if (plans[i]->desc.model != plans[0]->desc.model) return fail(N2H_EINVAL, "plans of different models");
+        if (plans[i]->host_step != plans[0]->host_step) return fail(N2H_EINVAL, "plans at different steps");
+        for (int32_t j = 0; j < i; ++j)
+            if (plans[j] == plans[i]) return fail(N2H_EINVAL, "plan %d listed twice", i);
+    }
+    n2h_batch* bt = new (std::nothrow) n2h_batch();
+    if (!bt) return fail(N2H_ENOMEM, "host allocation");
+    bt->plans = new (std::nothrow) n2h_plan*[n_plans];
+    if (!bt->plans) {
+        delete bt;
+        return fail(N2H_ENOMEM, "host allocation");
+    }
+    for (int32_t i = 0; i < n_plans; ++i) bt->plans[i] = plans[i];
+    if (tu.order == 1) {  // the most links first: the heaviest tiles start first, light ones fill the tail
+        for (int32_t i = 1; i < n_plans; ++i)
+            for (int32_t j = i; j > 0 && bt->plans[j - 1]->tables.n_links < bt->plans[j]->tables.n_links; --j) {
+                n2h_plan* t = bt->plans[j];
+                bt->plans[j] = bt->plans[j - 1];
+                bt->plans[j - 1] = t;
+            }
+    }
+    bt->n_plans = n_plans;
+    bt->model = plans[0]->desc.model;
+    bt->tuning = tu;
+    bt->host_step = plans[0]->host_step;
+    int max_groups = 0;
+    int64_t tiles = 0;
+    bt->lds_bytes = 0;
+    bt->first[0] = 0;
+    StepArgs* host_args = new (std::nothrow) StepArgs[n_plans];
+    int64_t** host_ctrs = new (std::nothrow) int64_t*[n_plans];
+    if (!host_args || !host_ctrs) {
+        delete[] host_args;
+        delete[] host_ctrs;
+        n2h_batch_destroy(bt);
+        return fail(N2H_ENOMEM, "host allocation");
+    }
+    for (int32_t i = 0; i < n_plans; ++i) {
+        const n2h_plan* p = bt->plans[i];
+        host_args[i] = make_args(p);
+        host_ctrs[i] = p->step_ctr;
+        tiles += p->tables.n_groups;
+        bt->first[i + 1] = static_cast<int32_t>(tiles);
+        if (p->tables.n_groups > max_groups) max_groups = p->tables.n_groups;
+        if (p->lds_bytes > bt->lds_bytes) bt->lds_bytes = p->lds_bytes;
+    }
+    if (tiles > (1ll << 30)) {
+        delete[] host_args;
+        delete[] host_ctrs;
+        n2h_batch_destroy(bt);
+        return fail(N2H_EINVAL, "too many groups in one batch");
+    }
+    bt->n_tiles = static_cast<int32_t>(tiles);
+    bt->words = (max_groups + kWave - 1) / kWave;
+    hipError_t e = hipMalloc(&bt->args_dev, sizeof(StepArgs) * n_plans);
+    if (e == hipSuccess) e = hipMalloc(&bt->ctrs_dev, sizeof(int64_t*) * n_plans);
+    if (e == hipSuccess) e = hipMalloc(&bt->step_ctr, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemcpy(bt->args_dev, host_args, sizeof(StepArgs) * n_plans, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->ctrs_dev, host_ctrs, sizeof(int64_t*) * n_plans, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->step_ctr, &bt->host_step, sizeof(int64_t), hipMemcpyHostToDevice);
+    delete[] host_args;
+    delete[] host_ctrs;
+    if (e != hipSuccess) {
+        n2h_batch_destroy(bt);
+        return fail(N2H_EHIP, "batch setup: %s", hipGetErrorString(e));
+    }
+    *out = bt;
+    return N2H_OK;
+}
+
+int n2h_batch_destroy(n2h_batch* bt) {
+    if (!bt) return N2H_OK;
+    if (bt->args_dev) (void)hipFree(bt->args_dev);
+    if (bt->ctrs_dev) (void)hipFree(bt->ctrs_dev);
+    if (bt->step_ctr) (void)hipFree(bt->step_ctr);
+    delete[] bt->plans;
+    delete bt;
+    return N2H_OK;
+}
+
+int n2h_batch_step(n2h_batch* bt, int32_t n_steps, void* stream) {
+    if (!bt) return fail(N2H_EINVAL, "null batch");
+    if (n_steps < 0) return fail(N2H_EINVAL, "negative n_steps");
+    int rc = batch_in_step(bt);
+    if (rc) return rc;
+    if (n_steps == 0) return N2H_OK;
+    rc = launch_batch(bt, n_steps, static_cast<hipStream_t>(stream), false);
+    if (rc) return rc;
+    for (int32_t i = 0; i < bt->n_plans; ++i) bt->plans[i]->host_step += n_steps;
+    return N2H_OK;
+}
+
+int n2h_batch_graph_create(n2h_batch* bt, int32_t n_steps, void* stream, n2h_graph** out) {
+    (void)stream;
+    if (!bt || !out) return fail(N2H_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_steps <= 0) return fail(N2H_EINVAL, "n_steps must be positive");
+    int rc = batch_in_step(bt);
+    if (rc) return rc;
+    const int64_t at = bt->host_step;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    rc = capture([&](hipStream_t cap) { return launch_batch(bt, n_steps, cap, true); }, &graph, &exec);
+    bt->host_step = at;  // captured, not run
+    if (rc) return rc;
+    n2h_graph* gr = new (std::nothrow)
+        n2h_graph{graph, exec, nullptr, bt, n_steps, static_cast<int32_t>(at & (kListSlots - 1))};
+    if (!gr) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return fail(N2H_ENOMEM, "host allocation");
+    }
+    *out = gr;
     return N2H_OK;
 }
 

@@ -38,6 +38,7 @@ class Network:
         self.device: torch.device = default_device()
         self.graph_window: int = 100      # steps per captured hipGraph in run() (0: no graphs)
         self.record: bool = True          # keep every step's spike bitmap (the raster)
+        self.keep_canonical: bool = True  # keep the canonical synapse table on the device after the build
         self.t: float = 0.0
         self._generator = None
         self._fuse_rounds = True
@@ -206,6 +207,11 @@ class Runtime:
         self.n_groups = self.layout.n_groups
         seg = segments(self.canon, self.layout)
         self.seg_off, self.links = seg.seg_off, seg.links
+        self.n_links = self.canon.n_links
+        self.out_degree = self.canon.out_degree()
+        if not net.keep_canonical:
+            self.canon = None
+        del seg
         self.ring = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
         self.bit_slots = 2
         self.bits = torch.zeros(self.bit_slots, self.n_groups, dtype=torch.int64, device=self.device)
@@ -213,6 +219,7 @@ class Runtime:
         self.spike_list = torch.zeros(4, 1 + GROUP * self.n_groups, dtype=torch.int32, device=self.device)
         self.record_from = 0
         self.step = 0
+        self._batch_capacity = None
         self._graphs: dict = {}
         self._plan = None
         self._create_plan()
@@ -229,7 +236,7 @@ class Runtime:
                               instance=int(self.net.instance) & 0xFFFFFFFF)
         tab = _lib.Tables(n_groups=self.n_groups, group_first=self.layout.group_first.data_ptr(),
                           group_info=self.layout.group_info.data_ptr(), seg_off=self.seg_off.data_ptr(),
-                          links=self.links.data_ptr(), n_links=self.canon.n_links)
+                          links=self.links.data_ptr(), n_links=self.n_links)
         plan = ctypes.c_void_p()
         _lib.check(lib.n2h_plan_create(ctypes.byref(desc), ctypes.byref(tab), ctypes.byref(plan)), "n2h_plan_create")
         self._plan = plan

@@ -99,6 +99,36 @@ def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instan
     return net, pops
 
 
+def sweep_params(n_instances: int = 256, seed: int = 0, n_min: int = 1000, n_max: int = 50000) -> list[dict]:
+    """Parameters of a batched Brunel sweep, all drawn from seed: N log-uniform in [n_min, n_max] (E:I =
+    4:1), fixed in-degree 10 % of N_E and N_I, g uniform in [3, 8], external drive ratio eta uniform in
+    [0.9, 4.0]; the rest as brunel()'s defaults."""
+    import numpy as np
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    out = []
+    for i in range(n_instances):
+        n = int(round(float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))))
+        n_e = int(round(0.8 * n))
+        n_i = n - n_e
+        out.append({"N_E": n_e, "N_I": n_i, "C_E": max(1, int(round(0.1 * n_e))), "C_I": max(1, int(round(0.1 * n_i))),
+                    "g": float(rng.uniform(3.0, 8.0)), "eta": float(rng.uniform(0.9, 4.0)), "seed": seed * 100003 + i,
+                    "instance": i})
+    return out
+
+
+def brunel_sweep(params: list[dict], device=None, keep_canonical: bool = False, record: bool = False):
+    """One built network per parameter set (sweep_params), ready to batch (FusedRound)."""
+    nets = []
+    for p in params:
+        net, _ = brunel(N_E=p["N_E"], N_I=p["N_I"], C_E=p["C_E"], C_I=p["C_I"], g=p["g"], eta=p["eta"],
+                        seed=p["seed"], instance=p["instance"], device=device)
+        net.keep_canonical = keep_canonical
+        net.record = record
+        net.build()
+        nets.append(net)
+    return nets
+
+
 def _configure(net: Network, dt: float, seed: int, device, instance: int) -> None:
     net.resolution = dt
     net.seed = seed

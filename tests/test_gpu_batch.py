@@ -1,0 +1,82 @@
+"""Batched sweeps (FusedRound over n2h_batch_*): every network in a batch computes exactly what it computes
+alone -- rasters, potentials, refractory counters and rings bit for bit -- for every launch mapping the
+autotuner can choose."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from naer2_amd import FusedRound
+from naer2_amd.autotune import CANDIDATES, tune
+from naer2_amd.networks import brunel_sweep, sweep_params
+from tests.gpu_util import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _params():
+    return sweep_params(n_instances=5, seed=3, n_min=300, n_max=3000)
+
+
+def _same(a, b):
+    ra, rb = a.runtime, b.runtime
+    assert ra.step == rb.step
+    assert np.array_equal(ra.raster(), rb.raster())
+    assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert np.array_equal(ra.ref.cpu().numpy(), rb.ref.cpu().numpy())
+    assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
+
+
+def test_batch_equals_plain_for_every_mapping():
+    params = _params()
+    _say(f"case: batch of {len(params)} brunel instances, sizes {[p['N_E'] + p['N_I'] for p in params]}")
+    batched = brunel_sweep(params, device="cuda", record=True)
+    plain = brunel_sweep(params, device="cuda", record=True)
+    total = 0
+    for cand in CANDIDATES:
+        _say(f"case: mapping {cand}")
+        fr = FusedRound([n.runtime for n in batched], cand, reserve_steps=110)
+        fr.launch(110, graph_window=48)
+        fr.close()
+        for n in plain:
+            n.run(110, graph_window=0)
+        total += 110
+        torch.cuda.synchronize()
+        for a, b in zip(batched, plain):
+            _same(a, b)
+    assert sum(int(n.runtime.raster().sum()) for n in plain) > 0
+    assert total == batched[0].runtime.step
+
+
+def test_batch_refuses_a_plan_stepped_alone():
+    _say("case: batch refuses a diverged plan")
+    nets = brunel_sweep(_params()[:3], device="cuda")
+    fr = FusedRound([n.runtime for n in nets], {"order": 1, "xcd": 8})
+    fr.launch(8)
+    nets[1].run(1, graph_window=0)
+    with pytest.raises(RuntimeError, match="step"):
+        fr.launch(4)
+    fr.close()
+
+
+def test_autotune_returns_a_candidate_and_keeps_results():
+    _say("case: autotune on a small batch")
+    params = _params()
+    batched = brunel_sweep(params, device="cuda", record=True)
+    plain = brunel_sweep(params, device="cuda", record=True)
+    best, times = tune([n.runtime for n in batched], steps=48, graph_window=48, reserve_steps=400, use_cache=False)
+    assert best in CANDIDATES and len(times) == len(CANDIDATES)
+    steps = batched[0].runtime.step
+    for n in plain:
+        n.run(steps, graph_window=0)
+    torch.cuda.synchronize()
+    for a, b in zip(batched, plain):
+        _same(a, b)
