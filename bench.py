@@ -70,6 +70,94 @@ def cpu_baseline(rt, state_snapshot, step0: int, n_steps: int) -> dict:
     return {"wall_s": wall, "events": events, "raster": raster, "v": st.v}
 
 
+def _snapshot(rt):
+    return (rt.step, rt.v.clone(), rt.ref.clone(), rt.ring.clone(), rt.bits.clone(), rt.spike_list.clone())
+
+
+def _restore(rt, snap):
+    step, v, ref, ring, bits, lst = snap
+    rt.v.copy_(v)
+    rt.ref.copy_(ref)
+    rt.ring.copy_(ring)
+    rt.bits.copy_(bits)
+    rt.spike_list.copy_(lst)
+    rt.set_step(step)
+
+
+def bench_sweep(args) -> int:
+    """Batched Brunel sweep: --instances networks drawn from --seed (N log-uniform in [1k, 50k], in-degree
+    10 %, g in [3, 8], eta in [0.9, 4]), one batched launch per step (mapping from the autotuner, tuned on
+    the warmup steps) against the same networks run one after another from the same state.  The final
+    states of both must be bit-identical."""
+    from naer2_amd import FusedRound
+    from naer2_amd.autotune import tune
+    from naer2_amd.networks import brunel_sweep, sweep_params
+
+    params = sweep_params(args.instances, seed=args.seed)
+    t0 = time.perf_counter()
+    nets = []
+    for i, p in enumerate(params):
+        nets += brunel_sweep([p], device="cuda", keep_canonical=False, record=False)
+        if i % 16 == 15:
+            print(f"[bench] built {i + 1}/{len(params)} networks ({time.perf_counter() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+    rts = [n.runtime for n in nets]
+    n_syn = sum(rt.n_links for rt in rts)
+    n_neurons = sum(rt.n for rt in rts)
+    torch.cuda.synchronize()
+    print(f"[bench] sweep: {len(rts)} networks, {n_neurons} neurons, {n_syn} synapses, "
+          f"{sum(rt.n_groups for rt in rts)} groups, built in {time.perf_counter() - t0:.0f} s",
+          file=sys.stderr, flush=True)
+    best, times = tune(rts, steps=args.warmup, graph_window=args.window, use_cache=False)
+    print(f"[bench] autotune: {best} from {times}", file=sys.stderr, flush=True)
+    snaps = [_snapshot(rt) for rt in rts]
+    fr = FusedRound(rts, best)
+    fr.launch(args.window, args.window)  # capture, untimed (the state is restored below)
+    torch.cuda.synchronize()
+    for rt, sn in zip(rts, snaps):
+        _restore(rt, sn)
+    fr.close()
+    fr = FusedRound(rts, best)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fr.launch(args.steps, graph_window=args.window)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    fr.close()
+    print(f"[bench] batched {args.steps} steps: {wall_b:.3f} s", file=sys.stderr, flush=True)
+    final_b = [(rt.v.clone(), rt.ref.clone(), rt.ring.clone()) for rt in rts]
+    for rt, sn in zip(rts, snaps):
+        _restore(rt, sn)
+        rt.run(args.window, args.window)  # capture each network's own graph, untimed
+    torch.cuda.synchronize()
+    for rt, sn in zip(rts, snaps):
+        _restore(rt, sn)
+    torch.cuda.synchronize()
+    wall_s = 0.0
+    for i, rt in enumerate(rts):
+        t0 = time.perf_counter()
+        rt.run(args.steps, args.window)
+        torch.cuda.synchronize()
+        wall_s += time.perf_counter() - t0
+    print(f"[bench] sequential {args.steps} steps x {len(rts)}: {wall_s:.3f} s", file=sys.stderr, flush=True)
+    same = all(torch.equal(a[0].view(torch.int32), rt.v.view(torch.int32)) and torch.equal(a[1], rt.ref)
+               and torch.equal(a[2].view(torch.int32), rt.ring.view(torch.int32)) for a, rt in zip(final_b, rts))
+    bio = args.steps * rts[0].dt / 1000.0
+    line = {
+        "metric": "batched Brunel sweep: aggregate instance-bio-s per wall-s, batched vs sequential plain runs",
+        "value": round(len(rts) * bio / wall_b, 4), "unit": "instance_bio_s/wall_s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(rts)} Brunel networks (N log-uniform 1k-50k, in-degree 10 %, g U[3,8], "
+                               f"eta U[0.9,4]), seed {args.seed}", "n_neurons": n_neurons, "n_synapses": n_syn},
+        "sequential_value": round(len(rts) * bio / wall_s, 4),
+        "batched_speedup": round(wall_s / wall_b, 3),
+        "mapping": best, "autotune_s": times,
+        "bit_exact_batched_vs_sequential": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +167,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", choices=["pd14", "sweep"], default="pd14",
+                    help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep")
+    ap.add_argument("--instances", type=int, default=256, help="networks in the sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run 5 plain device-to-device copies of MiB each (the reference kernel that scales "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes), then exit")
@@ -96,9 +187,11 @@ def main() -> int:
         print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build "
               "(see DESIGN.md for how it would shard)", file=sys.stderr)
         return 2
-    if args.window <= 0 or args.steps % args.window or args.warmup % args.window:
-        print("--steps and --warmup must be multiples of --window", file=sys.stderr)
+    if args.window <= 0 or args.window % 4 or args.steps % args.window or args.warmup % args.window:
+        print("--window must be a multiple of 4, --steps and --warmup multiples of --window", file=sys.stderr)
         return 2
+    if args.workload == "sweep":
+        return bench_sweep(args)
 
     from naer2_amd.networks import pd14
 
