@@ -158,6 +158,57 @@ def bench_sweep(args) -> int:
     return 0
 
 
+def bench_bptt(args) -> int:
+    """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
+    1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
+    checkpoints every 25 steps.  Times forward and backward with HIP events (median of --steps iterations
+    after --warmup), and checks the gradients against the FP64 autograd unroll (same spikes)."""
+    from naer2_amd.rlif import RecurrentLIF, RLIFConfig, max_over_time_ce, poisson_input
+    B, C, H, O, T = 128, 700, 1024, 20, 100
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    cfg = RLIFConfig(checkpoint=25)
+    net = RecurrentLIF(C, H, O, cfg, generator=g)
+    x = poisson_input(B, C, T, 1.0, g)
+    labels = torch.randint(0, O, (B,), generator=g, device="cuda")
+    fw, bw = [], []
+    iters = max(1, args.steps)
+    for it in range(args.warmup + iters):
+        net.zero_grad(set_to_none=True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        r, s = net(x)
+        loss = max_over_time_ce(r, labels)
+        e1.record()
+        loss.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            fw.append(e0.elapsed_time(e1))
+            bw.append(e1.elapsed_time(e2))
+    fwd_ms, bwd_ms = float(np.median(fw)), float(np.median(bw))
+    from oracle import rlif_oracle as ro
+    grads = [p.grad.detach() for p in (net.w_in, net.w_rec, net.w_out)]
+    w64 = [p.detach().double() for p in (net.w_in, net.w_rec, net.w_out)]
+    loss64, g64, _ = ro.grads(x.double(), *w64, labels, alpha=cfg.alpha, beta=cfg.beta, theta=cfg.theta,
+                              slope=cfg.slope, gamma=cfg.gamma, mode=0, forced=s.detach().double())
+    errs = {n: ro.rel_err(a, b) for n, a, b in zip(("w_in", "w_rec", "w_out"), grads, g64)}
+    flops = 2.0 * B * T * (C * H + H * H + H * O)
+    line = {
+        "metric": "BPTT backward/forward cost ratio (surrogate-gradient recurrent LIF)",
+        "value": round(bwd_ms / fwd_ms, 4), "unit": "ratio", "n_gpus": 1, "steps": iters, "warmup": args.warmup,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "B=128, 700 Poisson inputs (0-100 Hz), 1024 recurrent LIF, 20 leaky readouts, "
+                               "T=100 x 1 ms, fast-sigmoid slope 25, checkpoints every 25 steps"},
+        "forward_ms": round(fwd_ms, 4), "backward_ms": round(bwd_ms, 4),
+        "forward_dense_equiv_tflops": round(flops / (fwd_ms * 1e-3) / 1e12, 3),
+        "spike_fraction": round(float(s.mean()), 5),
+        "grad_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "loss": float(loss), "loss_fp64": float(loss64),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,7 +218,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["pd14", "sweep"], default="pd14",
+    ap.add_argument("--workload", choices=["pd14", "sweep", "bptt"], default="pd14",
                     help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep")
     ap.add_argument("--instances", type=int, default=256, help="networks in the sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -187,6 +238,8 @@ def main() -> int:
         print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build "
               "(see DESIGN.md for how it would shard)", file=sys.stderr)
         return 2
+    if args.workload == "bptt":
+        return bench_bptt(args)
     if args.window <= 0 or args.window % 4 or args.steps % args.window or args.warmup % args.window:
         print("--window must be a multiple of 4, --steps and --warmup multiples of --window", file=sys.stderr)
         return 2

@@ -159,6 +159,35 @@ int n2h_batch_destroy(n2h_batch* batch);
 int n2h_batch_step(n2h_batch* batch, int32_t n_steps, void* stream);
 int n2h_batch_graph_create(n2h_batch* batch, int32_t n_steps, void* stream, n2h_graph** out);
 
+/* Dense recurrent LIF layer for surrogate-gradient BPTT (no counterpart in the reference, which has no
+ * autograd; the north star's differentiation pattern).  Model and adjoint: naer2_amd/csrc/rlif.hip.
+ * Buffers are fp32, row-major: u [T][B][H] (= x W_in^T), w_rec [H][H], w_rec_t = its transpose,
+ * spikes [T][B][H]; shadow: double [2][B][H], the forward's double-precision copy of (I, V);
+ * checkpoints: double [ceil(T/K)][2][B][H], the shadow before steps 0, K, 2K, ... */
+typedef struct n2h_rlif {
+    int32_t batch, n_hidden, n_steps, checkpoint;  /* checkpoint: K steps between saved states */
+    int32_t mode;                                   /* 0: s = H(V - theta); 1: s = f(V - theta), smooth (tests) */
+    float alpha, beta, theta, slope, gamma;         /* synaptic / membrane decay, threshold, surrogate slope, scale */
+} n2h_rlif;
+int n2h_rlif_last_error(char* buf, size_t len);
+int n2h_rlif_forward(const n2h_rlif* layer, const float* u, const float* w_rec_t, float* i_state, float* v_state,
+                     double* shadow, float* spikes, double* checkpoints, void* stream);
+int n2h_rlif_readout(int32_t n_steps, int64_t bo, float alpha, float beta, const float* z, float* r, void* stream);
+int n2h_rlif_readout_adjoint(int32_t n_steps, int64_t bo, float alpha, float beta, const float* g_r, float* g_j,
+                             void* stream);
+/* gso = W_out^T gJ per step [T][B][H]; writes gI_t for every step to g_i_all [T][B][H];
+ * work: (7 + 2 K) * B * H floats. */
+int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec, const float* w_rec_t,
+                      const float* spikes, const double* checkpoints, const float* gso, float* g_i_all, float* work,
+                      void* stream);
+
+/* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
+ * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list
+ * read, sources ranked, segment bounds, links staged, walked, updated, exit -- into
+ * buffer[group][8] (the last step's stamps; points a step does not reach keep their old value).
+ * NULL turns it off.  Graphs captured before the call keep the setting they were captured with. */
+int n2h_set_trace(n2h_plan* plan, uint64_t* buffer);
+
 /* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */
 int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
 

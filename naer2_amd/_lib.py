@@ -25,7 +25,8 @@ EXPORTS = (
     "n2h_version", "n2h_last_error", "n2h_plan_create", "n2h_plan_destroy", "n2h_bind_state",
     "n2h_bind_ring", "n2h_set_step", "n2h_get_step", "n2h_step", "n2h_graph_create", "n2h_graph_launch",
     "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
-    "n2h_batch_graph_create",
+    "n2h_batch_graph_create", "n2h_set_trace", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
+    "n2h_rlif_readout_adjoint", "n2h_rlif_backward",
 )
 
 
@@ -92,6 +93,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_status": ([vp, ctypes.POINTER(i32), i32], i32),
         "n2h_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)], i32),
         "n2h_batch_destroy": ([vp], i32),
+        "n2h_set_trace": ([vp, vp], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }

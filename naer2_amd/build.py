@@ -14,7 +14,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRC = PKG / "csrc" / "naer2_hip.hip"
+SRCS = [PKG / "csrc" / "naer2_hip.hip", PKG / "csrc" / "rlif.hip"]
 OUT = PKG / "libnaer2_hip.so"
 ARCH = os.environ.get("NAER2_OFFLOAD_ARCH", "gfx950")
 
@@ -28,7 +28,7 @@ FLAGS = [
 def build(verbose: bool = False) -> Path:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc, *FLAGS, str(SRC), "-o", str(tmp)]
+    cmd = [hipcc, *FLAGS, *[str(s) for s in SRCS], "-o", str(tmp), "-lrocblas"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -42,7 +42,15 @@ constexpr int kWave = 64;
 constexpr int kCapS = 256;   // spiking sources staged in LDS per pass
 constexpr int kCapL = 512;   // links staged in LDS per pass (LDS per group: 9.2 KB + 256 B per ring slot)
 constexpr int kListSlots = 4;
-constexpr int kMaxBatch = 256;  // plans in one batched launch  // spike lists of steps s-1 (read), s (written), s+1 (cleared), s-2 (idle)
+constexpr int kMaxBatch = 256;  // plans in one batched launch
+constexpr int kTraceEvents = 8;  // entry, list read, sources ranked, segments, staged, walked, updated, exit
+
+// In-kernel phase stamps (100 MHz wall clock), lane 0 of each group, when tracing is on: kept in LDS and
+// written out at the end, so that the stamps add no global stores to the memory counter the kernel waits on.
+#define N2H_STAMP(ev)                                                                    \
+    do {                                                                                 \
+        if (a.trace && lane == 0) st.stamps[(ev)] = wall_clock64();                      \
+    } while (0)  // spike lists of steps s-1 (read), s (written), s+1 (cleared), s-2 (idle)
 
 thread_local char g_err[512] = "";
 
@@ -83,6 +91,7 @@ struct StepArgs {
     int64_t step_imm;        // eager launches: the first step of the launch sequence
     int32_t phase;           // (first step of the sequence) mod 4: the list slots without a load
     int32_t slot_bits;       // bits of a source slot: ceil(log2(64 * n_groups))
+    unsigned long long* trace;  // null, or [n_groups][kTraceEvents] wall-clock stamps of the last step
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -134,59 +143,85 @@ struct Stage {
     int64_t seg_lo[kCapS];       // first link of each source's segment into this group
     int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
     uint2 links[kCapL];          // links of this pass, in contract order
-    unsigned long long touched[kWave][2];  // per target: delays 1..64 and 65..128 with a partial sum
+    int16_t owner[kCapL];        // source (index into sp) of each staged link
+    uint2 sorted[kCapL];         // the staged links regrouped by target, each target's in contract order
+    int32_t tcount[kWave];       // staged links per target
+    int32_t tstart[kWave];       // first of each target's links in sorted
+    uint32_t param_words[sizeof(n2h_pop_params) / 4];  // the group's population parameters
+    unsigned long long stamps[kTraceEvents];             // phase stamps (tracing)
 };
 
-// Walk of staged links [b0, min(b0 + 64, nl)): one link per lane.  Lanes whose links hit the same cell
-// (delay, target) find each other with one ballot per cell bit; the lowest of them adds the weights of
-// all of them to the cell's partial sum in lane (= contract) order, while the other cells proceed in
-// parallel.  Equivalent to the target lanes walking the links one by one in order.
-__device__ __forceinline__ void walk_batch(Stage& st, float* acc, int lane, int b0, int nl) {
-    const int e = b0 + lane;
-    const bool valid = e < nl;
-    int cell = 0;
-    float w = 0.0f;
-    if (valid) {
-        const uint2 ln = st.links[e];
-        cell = (static_cast<int>(ln.x >> 16) - 1) * kWave + static_cast<int>(ln.x & 0xffffu);
-        w = __uint_as_float(ln.y);
-    }
-    unsigned long long same = __ballot(valid);
-#pragma unroll
-    for (int p = 0; p < 13; ++p) {  // cell < 128 * 64
-        const bool one = (cell >> p) & 1;
-        const unsigned long long b = __ballot(one);
-        same &= one ? b : ~b;
-    }
+// Walk of the nl staged links.  A stable counting sort by target (ranks among equal targets from one
+// ballot per target bit, batch by batch) regroups them so that target lane t then walks exactly its own
+// links, in contract order, adding each weight to its (delay, t) partial sum: the target-lane walk of the
+// reference's _walk (fused.py:939-991) without every lane reading every link.  Touched delays are
+// recorded in the lane's own mask (t0: delays 1..64, t1: 65..128).
+__device__ __forceinline__ void walk_chunk(Stage& st, float* acc, int lane, int nl, unsigned long long& t0,
+                                           unsigned long long& t1) {
+    constexpr int kPer = kCapL / kWave;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const bool head = valid && !(same & below);
-    unsigned long long rest = head ? (same & ~below & ~(1ull << lane)) : 0ull;
-    int run = __popcll(rest);
+    st.tcount[lane] = 0;
+    __syncthreads();
+    uint2 ln[kPer];
+    int off[kPer];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) run = max(run, __shfl_xor(run, off, kWave));
-    float a = 0.0f;
-    if (head) {
-        const int t = cell & (kWave - 1);
-        const int b = cell >> 6;
-        const unsigned long long bit = 1ull << (b & 63);
-        const float prior = acc[cell];
-        const unsigned long long old = atomicOr(&st.touched[t][b >> 6], bit);
-        a = ((old & bit) ? prior : 0.0f) + w;
-    }
-    for (int r = 0; r < run; ++r) {
-        const int src = rest ? (__ffsll(static_cast<long long>(rest)) - 1) : lane;
-        const float v = __shfl(w, src, kWave);
-        if (rest) {
-            a = a + v;
-            rest &= rest - 1ull;
+    for (int u = 0; u < kPer; ++u) {
+        off[u] = 0;
+        ln[u] = make_uint2(0u, 0u);
+        if (u * kWave >= nl) continue;
+        const int e = u * kWave + lane;
+        const bool valid = e < nl;
+        int t = kWave;  // invalid lanes: a target value no valid lane has
+        if (valid) {
+            ln[u] = st.links[e];
+            t = static_cast<int>(ln[u].x & 0xffffu);
+        }
+        unsigned long long same = ~0ull;
+#pragma unroll
+        for (int p = 0; p < 7; ++p) {
+            const bool one = (t >> p) & 1;
+            const unsigned long long bb = __ballot(one);
+            same &= one ? bb : ~bb;
+        }
+        if (valid) {
+            const int c = st.tcount[t];
+            off[u] = c + __popcll(same & below);
+            if (!(same & below)) st.tcount[t] = c + __popcll(same);  // the first lane of the target
         }
     }
-    if (head) acc[cell] = a;
+    __syncthreads();
+    const int cnt = st.tcount[lane];
+    const int incl = wave_incl_scan(cnt, lane);
+    st.tstart[lane] = incl - cnt;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = u * kWave + lane;
+        if (e < nl) st.sorted[st.tstart[ln[u].x & 0xffffu] + off[u]] = ln[u];
+    }
+    __syncthreads();
+    const int j0 = incl - cnt;
+    for (int j = j0; j < incl; ++j) {
+        const uint2 l = st.sorted[j];
+        const int b = static_cast<int>(l.x >> 16) - 1;
+        const float w = __uint_as_float(l.y);
+        bool seen;
+        if (b < 64) {
+            seen = (t0 >> b) & 1ull;
+            t0 |= 1ull << b;
+        } else {
+            seen = (t1 >> (b - 64)) & 1ull;
+            t1 |= 1ull << (b - 64);
+        }
+        float* c = &acc[b * kWave + lane];
+        *c = seen ? *c + w : 0.0f + w;
+    }
 }
 
 // Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
 // contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
-__device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float* acc, int g, int lane, int n_sp) {
+__device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float* acc, int g, int lane, int n_sp,
+                                             unsigned long long& t0, unsigned long long& t1) {
     const int NG = a.n_groups;
     // segment bounds: every load of the pass issued before the first is used
     int64_t lo_r[kCapS / kWave], hi_r[kCapS / kWave];
@@ -215,22 +250,75 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float
     }
     if (lane == 0) st.seg_start[n_sp] = carry;
     __syncthreads();
+    N2H_STAMP(3);
     const int L = carry;
     for (int e0 = 0; e0 < L; e0 += kCapL) {
         const int e1 = min(L, e0 + kCapL);
-        uint2 r[kCapL / kWave];
+        // owner[j]: the source whose segment holds link e0 + j.  Short chunks: a binary search per link;
+        // long ones: the segment starts scattered, then an inclusive max-scan (sources in ascending start
+        // order).
+        constexpr int kPer = kCapL / kWave;
+        if (e1 - e0 <= 2 * kWave) {
 #pragma unroll
-        for (int u = 0; u < kCapL / kWave; ++u) {
+            for (int u = 0; u < 2; ++u) {
+                const int e = e0 + u * kWave + lane;
+                if (e < e1) {
+                    int lo_i = 0, hi_i = n_sp;  // seg_start[lo_i] <= e < seg_start[hi_i]
+                    while (hi_i - lo_i > 1) {
+                        const int mid = (lo_i + hi_i) >> 1;
+                        if (st.seg_start[mid] <= e) lo_i = mid;
+                        else hi_i = mid;
+                    }
+                    st.owner[e - e0] = static_cast<int16_t>(lo_i);
+                }
+            }
+        } else {
+    #pragma unroll
+            for (int q = 0; q < kPer; ++q) st.owner[lane * kPer + q] = -1;
+            int c0 = 0, c1 = n_sp;  // the segment holding e0: seg_start[c0] <= e0 < seg_start[c1]
+            while (c1 - c0 > 1) {
+                const int mid = (c0 + c1) >> 1;
+                if (st.seg_start[mid] <= e0) c0 = mid;
+                else c1 = mid;
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int u = 0; u < kCapS / kWave; ++u) {
+                const int i = u * kWave + lane;
+                if (i < n_sp) {
+                    const int st0 = st.seg_start[i];
+                    if (st0 > e0 && st0 < e1 && st.seg_start[i + 1] > st0) st.owner[st0 - e0] = i;
+                }
+            }
+            if (lane == 0) st.owner[0] = c0;
+            __syncthreads();
+            int run_max = -1;
+            int own[kPer];
+    #pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                run_max = max(run_max, static_cast<int>(st.owner[lane * kPer + q]));
+                own[q] = run_max;
+            }
+            int pre = run_max;
+    #pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int y = __shfl_up(pre, off, kWave);
+                if (lane >= off) pre = max(pre, y);
+            }
+            pre = __shfl_up(pre, 1, kWave);
+            if (lane == 0) pre = -1;
+    #pragma unroll
+            for (int q = 0; q < kPer; ++q) st.owner[lane * kPer + q] = static_cast<int16_t>(max(pre, own[q]));
+            __syncthreads();
+        }
+        uint2 r[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
             const int e = e0 + u * kWave + lane;
             r[u] = make_uint2(0u, 0u);
             if (e < e1) {
-                int lo_i = 0, hi_i = n_sp;  // seg_start[lo_i] <= e < seg_start[hi_i]
-                while (hi_i - lo_i > 1) {
-                    const int mid = (lo_i + hi_i) >> 1;
-                    if (st.seg_start[mid] <= e) lo_i = mid;
-                    else hi_i = mid;
-                }
-                r[u] = a.links[st.seg_lo[lo_i] + (e - st.seg_start[lo_i])];
+                const int i = st.owner[e - e0];
+                r[u] = a.links[st.seg_lo[i] + (e - st.seg_start[i])];
             }
         }
 #pragma unroll
@@ -239,8 +327,9 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float
             if (e < e1) st.links[e - e0] = r[u];
         }
         __syncthreads();
+        N2H_STAMP(4);
         const int nl = e1 - e0;
-        for (int b0 = 0; b0 < nl; b0 += kWave) walk_batch(st, acc, lane, b0, nl);
+        walk_chunk(st, acc, lane, nl, t0, t1);
         __syncthreads();
     }
 }
@@ -252,43 +341,60 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     const int lane = threadIdx.x;
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
-    const int64_t D = a.d_slots;
-    const int64_t s = (a.step_ctr ? *a.step_ctr : a.step_imm) + k;
+    const int D = a.d_slots;
+    if (a.trace && lane < kTraceEvents) st.stamps[lane] = 0ull;
+    N2H_STAMP(0);
     const int slot_prev = (a.phase + k + kListSlots - 1) & (kListSlots - 1);  // step s-1
     const int slot_cur = (a.phase + k) & (kListSlots - 1);                    // step s
 
-    const int first = a.group_first[g];
-    const int info = a.group_info[g];
-    const int count = info & 0xffff;
-    const int pop = info >> 16;
-    const bool live = lane < count;
-    const int nid = first + lane;
-    const int64_t cur = (s % D) * N + nid;
-
-    // the spikes of step s-1: their count and (speculatively) the first 64 source slots
+    // the spikes of step s-1, first (the longest chain starts here): their count and, speculatively, the
+    // first 64 source slots
     const int32_t* plist = a.spikes + slot_prev * a.list_stride;
     const int n_prev = plist[0];
     const int my_src = plist[1 + lane];
 
-    // state and the due ring cell: nothing below writes them before they are used
+    // uniform values in scalar registers: later code then never waits on the vector memory counter for them
+    // (that counter also covers the fire-and-forget ring atomics)
+    const int first = __builtin_amdgcn_readfirstlane(a.group_first[g]);
+    const int info = __builtin_amdgcn_readfirstlane(a.group_info[g]);
+    const int count = info & 0xffff;
+    const int pop = info >> 16;
+    const bool live = lane < count;
+    const int nid = first + lane;
+    // the population's parameters, one 32-bit word per lane, kept in LDS once the delivery is done
+    constexpr int kParamWords = sizeof(n2h_pop_params) / 4;
+    const uint32_t pword = lane < kParamWords ? reinterpret_cast<const uint32_t*>(a.pops + pop)[lane] : 0u;
+
+    // state; the step (a load in graph launches) only addresses the ring and the bit history
     float V = 0.0f, I = 0.0f, x = 0.0f;
     int32_t R = 0;
     if (live) {
         V = a.v[nid];
         R = a.ref[nid];
         if (MODEL == N2H_MODEL_LIF_EXP) I = a.isyn[nid];
-        x = a.ring[cur];
     }
+    int64_t s0 = a.step_imm;
+    if (a.step_ctr) {
+        const uint64_t v = static_cast<uint64_t>(*a.step_ctr);
+        s0 = static_cast<int64_t>(
+            (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)))) << 32) |
+            static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v))));
+    }
+    const int64_t s = s0 + k;
+    const int s_mod = static_cast<int>(s % D);  // ring slot of step s; delays stay in 32-bit arithmetic
+    const int64_t cur = static_cast<int64_t>(s_mod) * N + nid;
+    if (live) x = a.ring[cur];
 
-    st.touched[lane][0] = 0ull;
-    st.touched[lane][1] = 0ull;
+    unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
     if (n_prev <= kWave) {
         // sparse path: sort the (atomically appended) source slots, then deliver them
+        N2H_STAMP(1);
         const bool has = lane < n_prev;
         const int rank = wave_rank(my_src, has, a.slot_bits);
         if (has) st.sp[rank] = my_src;
         __syncthreads();
-        if (n_prev > 0) deliver_pass(a, st, acc, g, lane, n_prev);
+        N2H_STAMP(2);
+        if (n_prev > 0) deliver_pass(a, st, acc, g, lane, n_prev, t0, t1);
     } else {
         // burst path: the spike bitmap of step s-1, lane-contiguous words, ascending source slots
         const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
@@ -337,34 +443,20 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 }
             }
             __syncthreads();
-            deliver_pass(a, st, acc, g, lane, r1 - r0);
+            deliver_pass(a, st, acc, g, lane, r1 - r0, t0, t1);
         }
     }
 
-    // ---- ring: add each touched cell's partial sum; the delay-1 cell is the one due now ----
-    __syncthreads();
-    const unsigned long long t0 = st.touched[lane][0], t1 = st.touched[lane][1];
-    if (live) {
-        unsigned long long m = t0 & ~1ull;
-        while (m) {
-            const int b = __ffsll(static_cast<long long>(m)) - 1;
-            m &= m - 1ull;
-            const int64_t idx = ((s + b) % D) * N + nid;  // arrival s - 1 + (b + 1)
-            unsafeAtomicAdd(&a.ring[idx], acc[b * kWave + lane]);  // the cell's one add this step
-        }
-        m = t1;
-        while (m) {
-            const int b = __ffsll(static_cast<long long>(m)) - 1 + 64;
-            m &= m - 1ull;
-            const int64_t idx = ((s + b) % D) * N + nid;
-            unsafeAtomicAdd(&a.ring[idx], acc[b * kWave + lane]);
-        }
-        if (t0 & 1ull) x = x + acc[lane];
-        a.ring[cur] = 0.0f;
-    }
+    N2H_STAMP(5);
+    // the parameters into LDS before any store or atomic is issued: waiting for their load then waits for
+    // loads only (the memory counter also counts stores and atomics)
+    if (lane < kParamWords) st.param_words[lane] = pword;
+    // the delay-1 cell's partial sum is the one due now: ring[s][i] + p, read in registers
+    if (live && (t0 & 1ull)) x = x + acc[lane];
 
     // ---- background, LIF update, threshold ----
-    const n2h_pop_params& P = a.pops[pop];
+    __syncthreads();
+    const n2h_pop_params& P = *reinterpret_cast<const n2h_pop_params*>(st.param_words);
     bool fired = false;
     if (live) {
         const uint32_t u = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
@@ -405,14 +497,12 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 R = P.ref_steps;
             }
         }
-        a.v[nid] = V;
-        a.ref[nid] = R;
-        if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
     }
 
-    // ---- emission: the bitmap word, and the source slots appended to the step's list ----
+    N2H_STAMP(6);
+    // ---- emission: the source slots appended to the step's list (the one returning atomic, first, so that
+    // waiting for it waits for no other memory operation), then the bitmap word ----
     const unsigned long long mask = __ballot(fired);
-    if (lane == 0) a.bits[(s % a.bit_slots) * NG + g] = mask;
     int32_t* clist = a.spikes + slot_cur * a.list_stride;
     if (mask) {
         int base = 0;
@@ -423,8 +513,36 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             clist[1 + base + rank] = g * kWave + lane;
         }
     }
+    if (lane == 0) a.bits[(s % a.bit_slots) * NG + g] = mask;
     // the list of step s+1 starts empty (its slot was last read during step s-2)
     if (g == 0 && lane == 0) a.spikes[((slot_cur + 1) & (kListSlots - 1)) * a.list_stride] = 0;
+
+    // ---- every remaining write last: nothing after them waits on the memory counter ----
+    if (live) {
+        a.v[nid] = V;
+        a.ref[nid] = R;
+        if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
+        a.ring[cur] = 0.0f;
+        // ring: add each touched cell's partial sum (arrival s - 1 + (b + 1), b < D); the cell's one add
+        unsigned long long m = t0 & ~1ull;
+        while (m) {
+            const int b = __ffsll(static_cast<long long>(m)) - 1;
+            m &= m - 1ull;
+            int slot = s_mod + b;
+            if (slot >= D) slot -= D;
+            unsafeAtomicAdd(&a.ring[static_cast<int64_t>(slot) * N + nid], acc[b * kWave + lane]);
+        }
+        m = t1;
+        while (m) {
+            const int b = __ffsll(static_cast<long long>(m)) - 1 + 64;
+            m &= m - 1ull;
+            int slot = s_mod + b;
+            if (slot >= D) slot -= D;
+            unsafeAtomicAdd(&a.ring[static_cast<int64_t>(slot) * N + nid], acc[b * kWave + lane]);
+        }
+    }
+    N2H_STAMP(7);
+    if (a.trace && lane < kTraceEvents) a.trace[static_cast<int64_t>(g) * kTraceEvents + lane] = st.stamps[lane];
 }
 
 template <int MODEL, int KW>
@@ -496,6 +614,7 @@ struct n2h_plan {
     int32_t* spikes;
     int64_t bit_slots;
     int64_t host_step;  // the step the next enqueued launch computes (mirror of *step_ctr)
+    unsigned long long* trace;
     size_t lds_bytes;
     int device;
 };
@@ -556,6 +675,7 @@ StepArgs make_args(const n2h_plan* p) {
     while ((1ll << a.slot_bits) < static_cast<int64_t>(kWave) * a.n_groups) ++a.slot_bits;
     a.spikes = p->spikes;
     a.list_stride = 1 + static_cast<int64_t>(kWave) * a.n_groups;
+    a.trace = p->trace;
     return a;
 }
 
@@ -971,6 +1091,12 @@ int n2h_graph_destroy(n2h_graph* gr) {
     (void)hipGraphExecDestroy(gr->exec);
     (void)hipGraphDestroy(gr->graph);
     delete gr;
+    return N2H_OK;
+}
+
+int n2h_set_trace(n2h_plan* p, uint64_t* buffer) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    p->trace = reinterpret_cast<unsigned long long*>(buffer);
     return N2H_OK;
 }
 

@@ -321,6 +321,33 @@ class Runtime:
         row[1:1 + len(slots)] = torch.from_numpy(slots[::-1].astype(np.int32).copy())  # any order
         self.spike_list[step % 4] = row.to(self.device)
 
+    TRACE_EVENTS = ("entry", "list read", "sources ranked", "segments", "links staged", "walked", "updated",
+                    "exit")
+
+    def trace(self, n_steps: int = 1) -> dict:
+        """Run n_steps eager steps with in-kernel phase stamps on; return, for the last of them, the mean and
+        max over groups of each phase's duration in microseconds (stamps: 100 MHz wall clock) and the
+        spread of group start and end times."""
+        buf = torch.zeros(self.n_groups, 8, dtype=torch.int64, device=self.device)
+        lib = _lib.load()
+        _lib.check(lib.n2h_set_trace(self._plan, buf.data_ptr()), "n2h_set_trace")
+        try:
+            self.run(n_steps, graph_window=0)
+            torch.cuda.synchronize(self.device)
+        finally:
+            _lib.check(lib.n2h_set_trace(self._plan, None), "n2h_set_trace")
+        t = buf.cpu().numpy().astype(np.int64)
+        out = {}
+        for j in range(1, 8):
+            d = (t[:, j] - t[:, j - 1]) / 100.0
+            ok = (t[:, j] > 0) & (t[:, j - 1] > 0) & (d >= 0)
+            if ok.any():
+                out[f"{self.TRACE_EVENTS[j - 1]} -> {self.TRACE_EVENTS[j]}"] = (float(d[ok].mean()), float(d[ok].max()))
+        out["group starts spread"] = float((t[:, 0].max() - t[:, 0].min()) / 100.0)
+        out["entry -> exit"] = (float(((t[:, 7] - t[:, 0]) / 100.0).mean()), float(((t[:, 7] - t[:, 0]) / 100.0).max()))
+        out["first entry -> last exit"] = float((t[:, 7].max() - t[:, 0].min()) / 100.0)
+        return out
+
     def set_step(self, step: int) -> None:
         """Move the step counter (the Philox counter and ring slot follow the absolute step)."""
         lib = _lib.load()
