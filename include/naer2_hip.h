@@ -181,6 +181,15 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
                       const float* spikes, const double* checkpoints, const float* gso, float* g_i_all, float* work,
                       void* stream);
 
+/* Row intake (the target-major pull): each target's in-links in contract order (ascending canonical link
+ * index), per group an ELL block -- entry k of lane l at rows[2 * (row_base[g] + 64 k + l)], packed
+ * {x = source slot << 7 | (delay_steps - 1), y = fp32 weight bits}; row_len [64 n_groups] in-links per
+ * target slot, row_depth [n_groups] the largest.  Steps that follow at least dense_min_spikes spikes then
+ * walk every in-link against the previous step's spike bitmap (held in LDS: n_groups <= 1536) instead of
+ * looking up the spiking sources' segments; both intakes give bit-identical results.  rows = NULL: off. */
+int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_len,
+                  const int32_t* row_depth, int32_t dense_min_spikes);
+
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list
  * read, sources ranked, segment bounds, links staged, walked, updated, exit -- into

@@ -19,13 +19,14 @@ GROUP = 64
 POISSON_K = 32
 MAX_POPS = 64
 MAX_SLOTS = 128
+DENSE_GROUPS = 1536  # row intake: spike bitmaps of at most this many groups fit in LDS
 
 # every symbol include/naer2_hip.h declares
 EXPORTS = (
     "n2h_version", "n2h_last_error", "n2h_plan_create", "n2h_plan_destroy", "n2h_bind_state",
     "n2h_bind_ring", "n2h_set_step", "n2h_get_step", "n2h_step", "n2h_graph_create", "n2h_graph_launch",
     "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
-    "n2h_batch_graph_create", "n2h_set_trace", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
+    "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_trace", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward",
 )
 
@@ -94,6 +95,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)], i32),
         "n2h_batch_destroy": ([vp], i32),
         "n2h_set_trace": ([vp, vp], i32),
+        "n2h_bind_rows": ([vp, vp, vp, vp, vp, i32], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }

@@ -43,6 +43,7 @@ constexpr int kCapS = 256;   // spiking sources staged in LDS per pass
 constexpr int kCapL = 512;   // links staged in LDS per pass (LDS per group: 9.2 KB + 256 B per ring slot)
 constexpr int kListSlots = 4;
 constexpr int kMaxBatch = 256;  // plans in one batched launch
+constexpr int kDenseWords = 1536;  // row intake: spike bitmaps of up to 1536 groups (98,304 neurons) in LDS
 constexpr int kTraceEvents = 8;  // entry, list read, sources ranked, segments, staged, walked, updated, exit
 
 // In-kernel phase stamps (100 MHz wall clock), lane 0 of each group, when tracing is on: kept in LDS and
@@ -92,6 +93,12 @@ struct StepArgs {
     int32_t phase;           // (first step of the sequence) mod 4: the list slots without a load
     int32_t slot_bits;       // bits of a source slot: ceil(log2(64 * n_groups))
     unsigned long long* trace;  // null, or [n_groups][kTraceEvents] wall-clock stamps of the last step
+    const uint2* rows;          // row intake (null: off): each target's in-links in contract order, ELL per group
+    const int64_t* row_base;    // [n_groups]: first entry of group g's block (entry k of lane l at + 64 k + l)
+    const int32_t* row_len;     // [64 n_groups]: in-links of each target slot
+    const int32_t* row_depth;   // [n_groups]: the group's largest row
+    int32_t dense_min;          // row intake for steps following at least this many spikes
+    int32_t pad2;
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -139,12 +146,17 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
 
 // Shared-memory staging of one workgroup.
 struct Stage {
-    int32_t sp[kCapS];           // spiking source slots of this pass, ascending
-    int64_t seg_lo[kCapS];       // first link of each source's segment into this group
-    int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
-    uint2 links[kCapL];          // links of this pass, in contract order
-    int16_t owner[kCapL];        // source (index into sp) of each staged link
-    uint2 sorted[kCapL];         // the staged links regrouped by target, each target's in contract order
+    union {
+        struct {                     // segment intake
+            int32_t sp[kCapS];           // spiking source slots of this pass, ascending
+            int64_t seg_lo[kCapS];       // first link of each source's segment into this group
+            int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
+            uint2 links[kCapL];          // links of this pass, in contract order
+            int16_t owner[kCapL];        // source (index into sp) of each staged link
+            uint2 sorted[kCapL];         // the staged links regrouped by target, each target's in contract order
+        };
+        unsigned long long bm[kDenseWords];  // row intake: the previous step's spike bitmap
+    };
     int32_t tcount[kWave];       // staged links per target
     int32_t tstart[kWave];       // first of each target's links in sorted
     uint32_t param_words[sizeof(n2h_pop_params) / 4];  // the group's population parameters
@@ -386,7 +398,46 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (live) x = a.ring[cur];
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
-    if (n_prev <= kWave) {
+    if (a.rows && n_prev >= a.dense_min) {
+        // row intake (the pull of the reference's north star): every target lane walks its own in-links in
+        // contract order against the previous step's spike bitmap held in LDS -- the whole table, so this
+        // path pays off when a large share of the sources spiked
+        const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
+        const unsigned long long* prev = a.bits + prev_slot * NG;
+        for (int w = lane; w < NG; w += kWave) st.bm[w] = prev[w];
+        __syncthreads();
+        N2H_STAMP(2);
+        const int len = live ? a.row_len[g * kWave + lane] : 0;
+        const int depth = a.row_depth[g];
+        const uint2* row = a.rows + a.row_base[g] + lane;
+        constexpr int kU = 8;
+        for (int k0 = 0; k0 < depth; k0 += kU) {
+            uint2 e[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) e[u] = (k0 + u < len) ? row[static_cast<int64_t>(k0 + u) * kWave] : make_uint2(0u, 0u);
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (k0 + u < len) {
+                    const uint32_t src = e[u].x >> 7;
+                    if ((st.bm[src >> 6] >> (src & 63)) & 1ull) {
+                        const int b = static_cast<int>(e[u].x & 127u);
+                        const float w = __uint_as_float(e[u].y);
+                        bool seen;
+                        if (b < 64) {
+                            seen = (t0 >> b) & 1ull;
+                            t0 |= 1ull << b;
+                        } else {
+                            seen = (t1 >> (b - 64)) & 1ull;
+                            t1 |= 1ull << (b - 64);
+                        }
+                        float* c = &acc[b * kWave + lane];
+                        *c = seen ? *c + w : 0.0f + w;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    } else if (n_prev <= kWave) {
         // sparse path: sort the (atomically appended) source slots, then deliver them
         N2H_STAMP(1);
         const bool has = lane < n_prev;
@@ -615,6 +666,11 @@ struct n2h_plan {
     int64_t bit_slots;
     int64_t host_step;  // the step the next enqueued launch computes (mirror of *step_ctr)
     unsigned long long* trace;
+    const uint2* rows;
+    const int64_t* row_base;
+    const int32_t* row_len;
+    const int32_t* row_depth;
+    int32_t dense_min;
     size_t lds_bytes;
     int device;
 };
@@ -676,6 +732,11 @@ StepArgs make_args(const n2h_plan* p) {
     a.spikes = p->spikes;
     a.list_stride = 1 + static_cast<int64_t>(kWave) * a.n_groups;
     a.trace = p->trace;
+    a.rows = p->rows;
+    a.row_base = p->row_base;
+    a.row_len = p->row_len;
+    a.row_depth = p->row_depth;
+    a.dense_min = p->dense_min;
     return a;
 }
 
@@ -1091,6 +1152,21 @@ int n2h_graph_destroy(n2h_graph* gr) {
     (void)hipGraphExecDestroy(gr->exec);
     (void)hipGraphDestroy(gr->graph);
     delete gr;
+    return N2H_OK;
+}
+
+int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, const int32_t* row_len,
+                  const int32_t* row_depth, int32_t dense_min_spikes) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (rows && (!row_base || !row_len || !row_depth)) return fail(N2H_EINVAL, "null row table");
+    if (rows && p->tables.n_groups > kDenseWords)
+        return fail(N2H_EINVAL, "row intake holds the spike bitmap in LDS: at most %d groups", kDenseWords);
+    if (dense_min_spikes < 0) return fail(N2H_EINVAL, "negative dense_min_spikes");
+    p->rows = reinterpret_cast<const uint2*>(rows);
+    p->row_base = row_base;
+    p->row_len = row_len;
+    p->row_depth = row_depth;
+    p->dense_min = dense_min_spikes;
     return N2H_OK;
 }
 

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from ._lib import GROUP, MODEL_LIF_DELTA, MODEL_LIF_EXP
-from .topology import Canonical, GroupLayout, canonical, delay_steps, group_layout, segments
+from .topology import Canonical, GroupLayout, canonical, delay_steps, group_layout, rows, segments
 
 _FAMILY_MODEL = {"lif_delta": MODEL_LIF_DELTA, "lif_exp": MODEL_LIF_EXP}
 _EPS = 1e-9
@@ -39,6 +39,10 @@ class Network:
         self.graph_window: int = 100      # steps per captured hipGraph in run() (0: no graphs)
         self.record: bool = True          # keep every step's spike bitmap (the raster)
         self.keep_canonical: bool = True  # keep the canonical synapse table on the device after the build
+        # row intake (target-major pull over every in-link, for steps after many spikes): "auto" builds its
+        # table where the spike bitmap fits in LDS (<= 1536 groups); dense_min_spikes None: n_neurons // 20
+        self.row_intake = "auto"
+        self.dense_min_spikes: int | None = None
         self.t: float = 0.0
         self._generator = None
         self._fuse_rounds = True
@@ -207,6 +211,12 @@ class Runtime:
         self.n_groups = self.layout.n_groups
         seg = segments(self.canon, self.layout)
         self.seg_off, self.links = seg.seg_off, seg.links
+        self.rows = None
+        if net.row_intake and self.n_groups <= _lib.DENSE_GROUPS:
+            self.rows = rows(self.canon, self.layout)
+        elif net.row_intake is True:
+            raise ValueError(f"row intake needs at most {_lib.DENSE_GROUPS} groups ({self.n_groups} here)")
+        self.dense_min = int(net.dense_min_spikes) if net.dense_min_spikes is not None else max(65, self.n // 20)
         self.n_links = self.canon.n_links
         self.out_degree = self.canon.out_degree()
         if not net.keep_canonical:
@@ -242,7 +252,28 @@ class Runtime:
         self._plan = plan
         _lib.check(lib.n2h_bind_state(plan, self.v.data_ptr(), self.isyn.data_ptr() if self.isyn is not None else None,
                                       self.ref.data_ptr()), "n2h_bind_state")
+        self._bind_rows()
         self._bind_ring()
+
+    def _bind_rows(self) -> None:
+        lib = _lib.load()
+        r = self.rows
+        if r is None:
+            _lib.check(lib.n2h_bind_rows(self._plan, None, None, None, None, 0), "n2h_bind_rows")
+        else:
+            _lib.check(lib.n2h_bind_rows(self._plan, r.rows.data_ptr(), r.row_base.data_ptr(), r.row_len.data_ptr(),
+                                         r.row_depth.data_ptr(), int(self.dense_min)), "n2h_bind_rows")
+
+    def set_dense_min(self, spikes: int) -> None:
+        """Steps that follow at least `spikes` spikes use the row intake (results do not change)."""
+        if self.rows is None:
+            raise RuntimeError("no row-intake table was built for this network")
+        self.dense_min = int(spikes)
+        self._bind_rows()
+        lib = _lib.load()
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
 
     def _bind_ring(self) -> None:
         lib = _lib.load()

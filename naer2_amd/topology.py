@@ -138,3 +138,45 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
     y = canon.w[order].contiguous().view(torch.int32)
     links = torch.stack([x.to(torch.int32), y], 1).contiguous()
     return Segments(seg_off, links)
+
+
+@dataclass
+class Rows:
+    """Row-intake tables: each target's in-links in contract order (ascending canonical link index), as one
+    ELL block per group -- entry k of lane l at rows[row_base[g] + 64 k + l]."""
+    rows: torch.Tensor       # int32 [entries, 2]: (source slot << 7 | (delay - 1), weight bits)
+    row_base: torch.Tensor   # int64 [G]
+    row_len: torch.Tensor    # int32 [64 G]: in-links per target slot
+    row_depth: torch.Tensor  # int32 [G]
+
+
+def rows(canon: Canonical, layout: GroupLayout) -> Rows:
+    dev = canon.tgt.device
+    n, G = canon.n, layout.n_groups
+    slot = slot_of(layout).to(dev)
+    group_of = layout.group_of.to(dev)
+    local_of = layout.local_of.to(dev)
+    tg = canon.tgt.to(torch.int64)
+    E = int(tg.numel())
+    deg = torch.bincount(tg, minlength=n)
+    row_len = torch.zeros(GROUP * G, dtype=torch.int64, device=dev)
+    row_len[slot] = deg
+    depth = row_len.view(G, GROUP).max(dim=1).values
+    row_base = torch.zeros(G, dtype=torch.int64, device=dev)
+    if G > 1:
+        row_base[1:] = torch.cumsum(depth * GROUP, 0)[:-1]
+    total = int((depth * GROUP).sum())
+    order = torch.argsort(tg, stable=True)  # by target, canonical order within each target
+    t_sorted = tg[order]
+    row_start = torch.cumsum(deg, 0) - deg
+    rank = torch.arange(E, device=dev) - row_start[t_sorted]
+    pos = row_base[group_of[t_sorted]] + rank * GROUP + local_of[t_sorted]
+    del rank, row_start
+    src = torch.repeat_interleave(slot, canon.out_degree())[order]
+    x = (src << 7) | (canon.d.to(torch.int64)[order] - 1)
+    y = canon.w[order].contiguous().view(torch.int32)
+    del src, order
+    out = torch.zeros(total, 2, dtype=torch.int32, device=dev)
+    out[pos, 0] = x.to(torch.int32)
+    out[pos, 1] = y
+    return Rows(out, row_base, row_len.to(torch.int32), depth.to(torch.int32))

@@ -74,6 +74,45 @@ def test_forced_spikes_deliver_like_the_oracle(n, n_links, max_delay, p_spike):
     assert_state_equal(rt, st)
 
 
+@pytest.mark.parametrize("dense_min", [0, 10**9])
+def test_row_intake_and_segment_intake_agree_with_the_oracle(dense_min):
+    """The same forced spike patterns through the row intake (every step) and the segment intake (never)."""
+    n, n_links, max_delay = 700, 30000, 12
+    _say(f"case: row intake dense_min={dense_min} n={n}")
+    topo = tiny_topology(n, n_links, max_delay, seed=17)
+    net, pop = network_from_topology(ProbeNode(), n, topo)
+    rt = net.build()
+    assert rt.rows is not None
+    rt.set_dense_min(dense_min)
+    st = oracle_state(rt)
+    rng = np.random.default_rng(18)
+    forced = rng.random((20, n)) < np.linspace(0.02, 0.8, 20)[:, None]
+    for s in range(forced.shape[0]):
+        if s > 0:
+            rt.force_spikes(s - 1, forced[s - 1])
+            st.prev_spk[:] = forced[s - 1]
+        rt.run(1)
+        oracle_run(rt, st, s, 1, record=False)
+        torch.cuda.synchronize()
+        assert bits_equal(rt.v.cpu().numpy(), st.v), f"step {s}: ring input differs"
+    assert_state_equal(rt, st)
+
+
+def test_brunel_bursts_row_intake_bit_exact():
+    """A synchronous Brunel regime (strong excitation) whose bursts take the row intake."""
+    _say("case: brunel g=3 bursts, 600 steps")
+    net, _ = brunel(N_E=4000, N_I=1000, C_E=400, C_I=100, g=3.0, eta=3.0, device="cuda", seed=4)
+    net.dense_min_spikes = 200
+    rt = net.build()
+    st = oracle_state(rt)
+    net.run(600)
+    torch.cuda.synchronize()
+    raster, _ = oracle_run(rt, st, 0, 600)
+    assert raster.sum(axis=1).max() >= 200  # bursts did take the row intake
+    assert np.array_equal(rt.raster(), raster)
+    assert_state_equal(rt, st)
+
+
 def test_graph_replays_equal_eager_steps():
     _say("case: graph windows vs eager, brunel N=2500")
     a, _ = brunel(device="cuda", seed=5)

@@ -17,7 +17,7 @@ from naer2_amd import _lib
 from naer2_amd.network import assemble, bits_to_raster, raster_to_bits
 from naer2_amd.networks import PD14, brunel, pd14
 from naer2_amd.nodes import FixedInDegree, FixedTotalNumber
-from naer2_amd.topology import canonical, delay_steps, group_layout, segments, slot_of
+from naer2_amd.topology import canonical, delay_steps, group_layout, rows, segments, slot_of
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -171,3 +171,28 @@ def test_unknown_node_family_is_rejected_at_build():
     A = Custom() * 3
     with pytest.raises(NotImplementedError):
         assemble(A.network)
+
+
+def test_rows_hold_each_targets_in_links_in_canonical_order():
+    rng = np.random.default_rng(3)
+    n = 150
+    src = torch.from_numpy(rng.integers(0, n, 4000))
+    tgt = torch.from_numpy(rng.integers(0, n, 4000))
+    w = torch.from_numpy(rng.standard_normal(4000).astype(np.float32))
+    d = torch.from_numpy(rng.integers(1, 30, 4000).astype(np.int32))
+    can = canonical(n, src, tgt, w, d)
+    lay = group_layout([100, 50])
+    r = rows(can, lay)
+    slot = (lay.group_of * 64 + lay.local_of).numpy()
+    src_of = np.repeat(np.arange(n), np.diff(can.src_ptr.numpy()))
+    tg = can.tgt.numpy()
+    table = r.rows.numpy()
+    for i in range(n):
+        g, lane = int(lay.group_of[i]), int(lay.local_of[i])
+        expect = np.nonzero(tg == i)[0]  # ascending canonical link index
+        assert int(r.row_len[g * 64 + lane]) == len(expect)
+        assert int(r.row_depth[g]) >= len(expect)
+        for k, l in enumerate(expect):
+            x, y = table[int(r.row_base[g]) + 64 * k + lane]
+            assert (x >> 7) == slot[src_of[l]] and (x & 127) + 1 == can.d[l].item()
+            assert np.int32(y).view(np.float32) == can.w[l].item()
