@@ -182,13 +182,21 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
                       void* stream);
 
 /* Row intake (the target-major pull): each target's in-links in contract order (ascending canonical link
- * index), per group an ELL block -- entry k of lane l at rows[2 * (row_base[g] + 64 k + l)], packed
- * {x = source slot << 7 | (delay_steps - 1), y = fp32 weight bits}; row_len [64 n_groups] in-links per
- * target slot, row_depth [n_groups] the largest.  Steps that follow at least dense_min_spikes spikes then
- * walk every in-link against the previous step's spike bitmap (held in LDS: n_groups <= 1536) instead of
- * looking up the spiking sources' segments; both intakes give bit-identical results.  rows = NULL: off. */
-int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_len,
-                  const int32_t* row_depth, int32_t dense_min_spikes);
+ * index).  Per group a block of 16-byte cells [row_depth[g]][64]: cell q of lane l (cell index
+ * row_base[g] + 64 q + l, i.e. int32 words 4 * that index on) holds the lane's entries per*q .. per*q+per-1.
+ *   N2H_ROWS_WIDE    (per = 2): {source slot << 7 | (delay_steps - 1), fp32 weight bits}
+ *   N2H_ROWS_COMPACT (per = 4): source slot << 15 | (delay_steps - 1) << 8 | weight index into wtab[256]
+ *                               (at most 256 distinct weights)
+ * Entries past a target's in-links name the sentinel source slot 64 * n_groups (never spiking).
+ * uniform_delay: the delay in steps of every link when they are all equal (a branch-free walk), else 0.
+ * Steps that follow at least dense_min_spikes spikes walk every in-link against the previous step's spike
+ * bitmap (held in LDS: n_groups <= 1535) instead of looking up the spiking sources' segments; both intakes
+ * give bit-identical results.  rows = NULL: off.
+ * Replaces: the north star's CSR-by-target pull (receives.py); the reference's own pull is fused.py:1626-1965. */
+#define N2H_ROWS_WIDE 0
+#define N2H_ROWS_COMPACT 1
+int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
+                  int32_t format, const float* wtab, int32_t uniform_delay, int32_t dense_min_spikes);
 
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list

@@ -19,7 +19,7 @@ GROUP = 64
 POISSON_K = 32
 MAX_POPS = 64
 MAX_SLOTS = 128
-DENSE_GROUPS = 1536  # row intake: spike bitmaps of at most this many groups fit in LDS
+DENSE_GROUPS = 1535  # row intake: spike bitmaps of at most this many groups fit in LDS
 
 # every symbol include/naer2_hip.h declares
 EXPORTS = (
@@ -95,7 +95,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)], i32),
         "n2h_batch_destroy": ([vp], i32),
         "n2h_set_trace": ([vp, vp], i32),
-        "n2h_bind_rows": ([vp, vp, vp, vp, vp, i32], i32),
+        "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32, i32], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }

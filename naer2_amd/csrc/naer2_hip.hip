@@ -43,7 +43,8 @@ constexpr int kCapS = 256;   // spiking sources staged in LDS per pass
 constexpr int kCapL = 512;   // links staged in LDS per pass (LDS per group: 9.2 KB + 256 B per ring slot)
 constexpr int kListSlots = 4;
 constexpr int kMaxBatch = 256;  // plans in one batched launch
-constexpr int kDenseWords = 1536;  // row intake: spike bitmaps of up to 1536 groups (98,304 neurons) in LDS
+constexpr int kDenseWords = 1536;  // row intake: spike bitmaps of up to 1535 groups (98,240 neurons) in LDS, plus
+                                   // the sentinel word (source slots 64 n_groups .. +63: never spiking)
 constexpr int kTraceEvents = 8;  // entry, list read, sources ranked, segments, staged, walked, updated, exit
 
 // In-kernel phase stamps (100 MHz wall clock), lane 0 of each group, when tracing is on: kept in LDS and
@@ -93,12 +94,14 @@ struct StepArgs {
     int32_t phase;           // (first step of the sequence) mod 4: the list slots without a load
     int32_t slot_bits;       // bits of a source slot: ceil(log2(64 * n_groups))
     unsigned long long* trace;  // null, or [n_groups][kTraceEvents] wall-clock stamps of the last step
-    const uint2* rows;          // row intake (null: off): each target's in-links in contract order, ELL per group
-    const int64_t* row_base;    // [n_groups]: first entry of group g's block (entry k of lane l at + 64 k + l)
-    const int32_t* row_len;     // [64 n_groups]: in-links of each target slot
-    const int32_t* row_depth;   // [n_groups]: the group's largest row
+    const uint4* rows;          // row intake (null: off): each target's in-links in contract order
+    const int64_t* row_base;    // [n_groups]: first 16-byte cell of group g's block (cell q of lane l: + 64 q + l)
+    const int32_t* row_depth;   // [n_groups]: cells per lane
+    const float* row_wtab;      // compact rows: the weight table [256]
+    int32_t row_format;         // N2H_ROWS_WIDE / N2H_ROWS_COMPACT
+    int32_t row_uniform;        // every link's delay cell (delay - 1) when all delays are equal, else -1
     int32_t dense_min;          // row intake for steps following at least this many spikes
-    int32_t pad2;
+    int32_t pad3;
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -144,18 +147,22 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
     return rank;
 }
 
-// Shared-memory staging of one workgroup.
+// Shared-memory staging of one workgroup.  BM: 64-bit words of the row intake's spike bitmap (the
+// groups' words, then the sentinel word); kernel variants size it for the groups they serve.
+template <int BM>
 struct Stage {
     union {
         struct {                     // segment intake
             int32_t sp[kCapS];           // spiking source slots of this pass, ascending
             int64_t seg_lo[kCapS];       // first link of each source's segment into this group
             int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
-            uint2 links[kCapL];          // links of this pass, in contract order
             int16_t owner[kCapL];        // source (index into sp) of each staged link
             uint2 sorted[kCapL];         // the staged links regrouped by target, each target's in contract order
         };
-        unsigned long long bm[kDenseWords];  // row intake: the previous step's spike bitmap
+        struct {                     // row intake
+            unsigned long long bm[BM];   // the previous step's spike bitmap, then a zero word
+            float wtab[256];             // compact rows: the weight table
+        };
     };
     int32_t tcount[kWave];       // staged links per target
     int32_t tstart[kWave];       // first of each target's links in sorted
@@ -163,13 +170,39 @@ struct Stage {
     unsigned long long stamps[kTraceEvents];             // phase stamps (tracing)
 };
 
+// Bitmap words of the kernel variant for KW bitmap words per lane (n_groups <= 64 KW): the groups'
+// words and the sentinel; KW = 0 (more than 2048 groups) has no row intake.
+constexpr int bm_words(int kw) { return kw == 0 ? 1 : (64 * kw < kDenseWords - 1 ? 64 * kw : kDenseWords - 1) + 1; }
+
+// One delivered link of weight w to delay cell b of this lane's target, with the lane's current cell held
+// in a register (cb, cr): a change of cell writes the old one back and fetches the new one (or starts it
+// at +0.0f if untouched), so each cell still sees exactly its links' adds, in order.
+__device__ __forceinline__ void cell_add(float* acc, int lane, int b, float w, int& cb, float& cr,
+                                         unsigned long long& t0, unsigned long long& t1) {
+    if (b != cb) {
+        if (cb >= 0) acc[cb * kWave + lane] = cr;
+        bool seen;
+        if (b < 64) {
+            seen = (t0 >> b) & 1ull;
+            t0 |= 1ull << b;
+        } else {
+            seen = (t1 >> (b - 64)) & 1ull;
+            t1 |= 1ull << (b - 64);
+        }
+        cr = seen ? acc[b * kWave + lane] : 0.0f;
+        cb = b;
+    }
+    cr = cr + w;
+}
+
 // Walk of the nl staged links.  A stable counting sort by target (ranks among equal targets from one
 // ballot per target bit, batch by batch) regroups them so that target lane t then walks exactly its own
 // links, in contract order, adding each weight to its (delay, t) partial sum: the target-lane walk of the
 // reference's _walk (fused.py:939-991) without every lane reading every link.  Touched delays are
 // recorded in the lane's own mask (t0: delays 1..64, t1: 65..128).
-__device__ __forceinline__ void walk_chunk(Stage& st, float* acc, int lane, int nl, unsigned long long& t0,
-                                           unsigned long long& t1) {
+template <class St>
+__device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl, const uint2 (&r)[kCapL / kWave],
+                                           unsigned long long& t0, unsigned long long& t1) {
     constexpr int kPer = kCapL / kWave;
     const unsigned long long below = (1ull << lane) - 1ull;
     st.tcount[lane] = 0;
@@ -185,7 +218,7 @@ __device__ __forceinline__ void walk_chunk(Stage& st, float* acc, int lane, int 
         const bool valid = e < nl;
         int t = kWave;  // invalid lanes: a target value no valid lane has
         if (valid) {
-            ln[u] = st.links[e];
+            ln[u] = r[u];
             t = static_cast<int>(ln[u].x & 0xffffu);
         }
         unsigned long long same = ~0ull;
@@ -213,26 +246,115 @@ __device__ __forceinline__ void walk_chunk(Stage& st, float* acc, int lane, int 
     }
     __syncthreads();
     const int j0 = incl - cnt;
+    int cb = -1;
+    float cr = 0.0f;
     for (int j = j0; j < incl; ++j) {
         const uint2 l = st.sorted[j];
-        const int b = static_cast<int>(l.x >> 16) - 1;
-        const float w = __uint_as_float(l.y);
-        bool seen;
-        if (b < 64) {
-            seen = (t0 >> b) & 1ull;
-            t0 |= 1ull << b;
-        } else {
-            seen = (t1 >> (b - 64)) & 1ull;
-            t1 |= 1ull << (b - 64);
+        cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, __uint_as_float(l.y), cb, cr, t0, t1);
+    }
+    if (cb >= 0) acc[cb * kWave + lane] = cr;
+}
+
+// Row intake of one batch of kRowU 16-byte cells of this lane's row (kRowU * PER entries).  Padding
+// entries name the sentinel source slot, whose bitmap bit is always clear, so no entry needs a bounds
+// test.  The bitmap words and weights of all entries are read from LDS first (one wait).
+//   UNIFORM (every link has delay cell b0): branch-free -- the hit bit, sign-extended to a mask, selects
+//   either the weight or -0.0f, the exact identity of fp32 addition, into the register sum cr, so the sum
+//   sees exactly the spiking sources' weights in row order; `any` records whether there was one.
+//   Otherwise: each hit goes to its own delay cell through cell_add.
+constexpr int kRowU = 8;
+template <int PER, bool UNIFORM, class St>
+__device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, const uint4 (&c)[kRowU], int& cb,
+                                          float& cr, uint32_t& any, unsigned long long& t0, unsigned long long& t1) {
+    constexpr int E = kRowU * PER;
+    const uint32_t* bm32 = reinterpret_cast<const uint32_t*>(st.bm);
+    uint32_t word[E], bw[E], wb[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(&c[e / PER]);
+        const uint32_t x = (PER == 4) ? cw[e % PER] : cw[2 * (e % PER)];
+        const uint32_t src = (PER == 4) ? (x >> 15) : (x >> 7);
+        word[e] = x;
+        bw[e] = bm32[src >> 5];
+        wb[e] = (PER == 4) ? __float_as_uint(st.wtab[x & 255u]) : cw[2 * (e % PER) + 1];
+    }
+    if (UNIFORM) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
+            const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bw[e]), src, 1u));
+            any |= m;
+            cr = cr + __uint_as_float((wb[e] & m) | (0x80000000u & ~m));
         }
-        float* c = &acc[b * kWave + lane];
-        *c = seen ? *c + w : 0.0f + w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
+            if ((bw[e] >> (src & 31u)) & 1u) {
+                const int b = (PER == 4) ? static_cast<int>((word[e] >> 8) & 127u) : static_cast<int>(word[e] & 127u);
+                cell_add(acc, lane, b, __uint_as_float(wb[e]), cb, cr, t0, t1);
+            }
+        }
+    }
+}
+
+// Row intake: this lane walks its target's in-link cells row[64 q], q < depth, against the spike bitmap in
+// LDS.  Two register batches alternate so that one batch's loads are in flight while the other is
+// processed; the loads are unconditional (indices clamped into the group's block), so each stays one
+// dwordx4 and the compiler waits only for the batch it processes.  `ready` runs once the first batch is
+// issued (the bitmap's barrier).  b0: the uniform delay cell (UNIFORM).
+template <int PER, bool UNIFORM, class St, typename Ready>
+__device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, const uint4* row, int depth, int b0,
+                                         uint32_t sentinel, unsigned long long& t0, unsigned long long& t1,
+                                         Ready ready) {
+    if (depth <= 0) {
+        ready();
+        return;
+    }
+    const int last = depth - 1;
+    // a batch past the end of the block: its clamped loads repeat the last cell, so those cells are
+    // replaced by sentinel entries (uniform branch, once per walk)
+    const uint32_t sx = (PER == 4) ? (sentinel << 15) : (sentinel << 7);
+    auto trim = [&](uint4 (&c)[kRowU], int q_first) {
+        if (q_first + kRowU > depth) {
+#pragma unroll
+            for (int u = 0; u < kRowU; ++u)
+                if (q_first + u > last) c[u] = make_uint4(sx, sx, sx, sx);
+        }
+    };
+    uint4 A[kRowU], B[kRowU];
+#pragma unroll
+    for (int u = 0; u < kRowU; ++u) A[u] = row[static_cast<int64_t>(min(u, last)) * kWave];
+    ready();
+    int cb = -1;
+    float cr = 0.0f;
+    uint32_t any = 0u;
+    for (int q0 = 0; q0 < depth; q0 += 2 * kRowU) {
+#pragma unroll
+        for (int u = 0; u < kRowU; ++u) B[u] = row[static_cast<int64_t>(min(q0 + kRowU + u, last)) * kWave];
+        trim(A, q0);
+        row_batch<PER, UNIFORM>(st, acc, lane, A, cb, cr, any, t0, t1);
+        if (q0 + kRowU >= depth) break;
+#pragma unroll
+        for (int u = 0; u < kRowU; ++u) A[u] = row[static_cast<int64_t>(min(q0 + 2 * kRowU + u, last)) * kWave];
+        trim(B, q0 + kRowU);
+        row_batch<PER, UNIFORM>(st, acc, lane, B, cb, cr, any, t0, t1);
+    }
+    if (UNIFORM) {
+        if (any) {  // the row intake is the step's only intake: the cell starts untouched
+            if (b0 < 64) t0 |= 1ull << b0;
+            else t1 |= 1ull << (b0 - 64);
+            acc[b0 * kWave + lane] = cr;
+        }
+    } else if (cb >= 0) {
+        acc[cb * kWave + lane] = cr;
     }
 }
 
 // Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
 // contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
-__device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float* acc, int g, int lane, int n_sp,
+template <class St>
+__device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* acc, int g, int lane, int n_sp,
                                              unsigned long long& t0, unsigned long long& t1) {
     const int NG = a.n_groups;
     // segment bounds: every load of the pass issued before the first is used
@@ -333,15 +455,9 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float
                 r[u] = a.links[st.seg_lo[i] + (e - st.seg_start[i])];
             }
         }
-#pragma unroll
-        for (int u = 0; u < kCapL / kWave; ++u) {
-            const int e = e0 + u * kWave + lane;
-            if (e < e1) st.links[e - e0] = r[u];
-        }
-        __syncthreads();
         N2H_STAMP(4);
         const int nl = e1 - e0;
-        walk_chunk(st, acc, lane, nl, t0, t1);
+        walk_chunk(st, acc, lane, nl, r, t0, t1);
         __syncthreads();
     }
 }
@@ -349,7 +465,8 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, Stage& st, float
 // One step s of the whole network for target group g (one wavefront).
 // KW: bitmap words per lane held in registers on the burst path (0: read them in a loop).
 template <int MODEL, int KW>
-__device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, Stage& st, float* acc) {
+__device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, Stage<bm_words(KW)>& st,
+                                           float* acc) {
     const int lane = threadIdx.x;
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
@@ -398,43 +515,33 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (live) x = a.ring[cur];
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
-    if (a.rows && n_prev >= a.dense_min) {
+    if (KW > 0 && a.rows && n_prev >= a.dense_min) {
         // row intake (the pull of the reference's north star): every target lane walks its own in-links in
         // contract order against the previous step's spike bitmap held in LDS -- the whole table, so this
         // path pays off when a large share of the sources spiked
         const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
         const unsigned long long* prev = a.bits + prev_slot * NG;
         for (int w = lane; w < NG; w += kWave) st.bm[w] = prev[w];
-        __syncthreads();
-        N2H_STAMP(2);
-        const int len = live ? a.row_len[g * kWave + lane] : 0;
-        const int depth = a.row_depth[g];
-        const uint2* row = a.rows + a.row_base[g] + lane;
-        constexpr int kU = 8;
-        for (int k0 = 0; k0 < depth; k0 += kU) {
-            uint2 e[kU];
+        if (lane == 0) st.bm[NG] = 0ull;  // the sentinel word of padding entries
+        const bool compact = a.row_format == N2H_ROWS_COMPACT;
+        if (compact) {
 #pragma unroll
-            for (int u = 0; u < kU; ++u) e[u] = (k0 + u < len) ? row[static_cast<int64_t>(k0 + u) * kWave] : make_uint2(0u, 0u);
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                if (k0 + u < len) {
-                    const uint32_t src = e[u].x >> 7;
-                    if ((st.bm[src >> 6] >> (src & 63)) & 1ull) {
-                        const int b = static_cast<int>(e[u].x & 127u);
-                        const float w = __uint_as_float(e[u].y);
-                        bool seen;
-                        if (b < 64) {
-                            seen = (t0 >> b) & 1ull;
-                            t0 |= 1ull << b;
-                        } else {
-                            seen = (t1 >> (b - 64)) & 1ull;
-                            t1 |= 1ull << (b - 64);
-                        }
-                        float* c = &acc[b * kWave + lane];
-                        *c = seen ? *c + w : 0.0f + w;
-                    }
-                }
-            }
+            for (int j = 0; j < 256 / kWave; ++j) st.wtab[j * kWave + lane] = a.row_wtab[j * kWave + lane];
+        }
+        const int depth = __builtin_amdgcn_readfirstlane(a.row_depth[g]);
+        const uint4* row = a.rows + a.row_base[g] + lane;
+        const int b0 = a.row_uniform;
+        auto ready = [&]() {
+            __syncthreads();
+            N2H_STAMP(2);
+        };
+        const uint32_t sentinel = static_cast<uint32_t>(kWave) * static_cast<uint32_t>(NG);
+        if (compact) {
+            if (b0 >= 0) row_walk<4, true>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
+            else row_walk<4, false>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
+        } else {
+            if (b0 >= 0) row_walk<2, true>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
+            else row_walk<2, false>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
         }
         __syncthreads();
     } else if (n_prev <= kWave) {
@@ -598,7 +705,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
 
 template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
-    __shared__ Stage st;
+    __shared__ Stage<bm_words(KW)> st;
     extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
     step_group<MODEL, KW>(a, blockIdx.x, k, st, acc);
 }
@@ -620,7 +727,7 @@ struct BatchArgs {
 
 template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
-    __shared__ Stage st;
+    __shared__ Stage<bm_words(KW)> st;
     extern __shared__ float acc[];
     const int b = blockIdx.x;
     int tile = b;
@@ -666,10 +773,12 @@ struct n2h_plan {
     int64_t bit_slots;
     int64_t host_step;  // the step the next enqueued launch computes (mirror of *step_ctr)
     unsigned long long* trace;
-    const uint2* rows;
+    const uint4* rows;
     const int64_t* row_base;
-    const int32_t* row_len;
     const int32_t* row_depth;
+    const float* row_wtab;
+    int32_t row_format;
+    int32_t row_uniform;
     int32_t dense_min;
     size_t lds_bytes;
     int device;
@@ -734,10 +843,22 @@ StepArgs make_args(const n2h_plan* p) {
     a.trace = p->trace;
     a.rows = p->rows;
     a.row_base = p->row_base;
-    a.row_len = p->row_len;
     a.row_depth = p->row_depth;
+    a.row_wtab = p->row_wtab;
+    a.row_format = p->row_format;
+    a.row_uniform = p->rows ? p->row_uniform : -1;
     a.dense_min = p->dense_min;
     return a;
+}
+
+// The kernel variant for `words` bitmap words per lane (n_groups <= 64 words).
+template <int MODEL>
+void launch_step_model(int words, dim3 grid, size_t lds, hipStream_t stream, const StepArgs& a, int32_t k) {
+    const dim3 block(kWave);
+    if (words <= 8) hipLaunchKernelGGL((step_kernel<MODEL, 8>), grid, block, lds, stream, a, k);
+    else if (words <= 16) hipLaunchKernelGGL((step_kernel<MODEL, 16>), grid, block, lds, stream, a, k);
+    else if (words <= 32) hipLaunchKernelGGL((step_kernel<MODEL, 32>), grid, block, lds, stream, a, k);
+    else hipLaunchKernelGGL((step_kernel<MODEL, 0>), grid, block, lds, stream, a, k);
 }
 
 int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
@@ -746,21 +867,8 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
-        if (p->desc.model == N2H_MODEL_LIF_DELTA) {
-            if (words <= 8)
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 8>), grid, block, p->lds_bytes, stream, a, k);
-            else if (words <= 32)
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 32>), grid, block, p->lds_bytes, stream, a, k);
-            else
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_DELTA, 0>), grid, block, p->lds_bytes, stream, a, k);
-        } else {
-            if (words <= 8)
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 8>), grid, block, p->lds_bytes, stream, a, k);
-            else if (words <= 32)
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 32>), grid, block, p->lds_bytes, stream, a, k);
-            else
-                hipLaunchKernelGGL((step_kernel<N2H_MODEL_LIF_EXP, 0>), grid, block, p->lds_bytes, stream, a, k);
-        }
+        if (p->desc.model == N2H_MODEL_LIF_DELTA) launch_step_model<N2H_MODEL_LIF_DELTA>(words, grid, p->lds_bytes, stream, a, k);
+        else launch_step_model<N2H_MODEL_LIF_EXP>(words, grid, p->lds_bytes, stream, a, k);
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, p->step_ctr, n_steps);
     N2H_HIP(hipGetLastError());
@@ -812,6 +920,8 @@ void launch_batch_model(const n2h_batch* bt, const BatchArgs& ba, dim3 grid, hip
     const dim3 block(kWave);
     if (bt->words <= 8)
         hipLaunchKernelGGL((batch_kernel<MODEL, 8>), grid, block, bt->lds_bytes, stream, ba, k);
+    else if (bt->words <= 16)
+        hipLaunchKernelGGL((batch_kernel<MODEL, 16>), grid, block, bt->lds_bytes, stream, ba, k);
     else if (bt->words <= 32)
         hipLaunchKernelGGL((batch_kernel<MODEL, 32>), grid, block, bt->lds_bytes, stream, ba, k);
     else
@@ -1155,17 +1265,25 @@ int n2h_graph_destroy(n2h_graph* gr) {
     return N2H_OK;
 }
 
-int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, const int32_t* row_len,
-                  const int32_t* row_depth, int32_t dense_min_spikes) {
+int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
+                  int32_t format, const float* wtab, int32_t uniform_delay, int32_t dense_min_spikes) {
     if (!p) return fail(N2H_EINVAL, "null plan");
-    if (rows && (!row_base || !row_len || !row_depth)) return fail(N2H_EINVAL, "null row table");
-    if (rows && p->tables.n_groups > kDenseWords)
-        return fail(N2H_EINVAL, "row intake holds the spike bitmap in LDS: at most %d groups", kDenseWords);
+    if (rows && (!row_base || !row_depth)) return fail(N2H_EINVAL, "null row table");
+    if (rows && p->tables.n_groups > kDenseWords - 1)
+        return fail(N2H_EINVAL, "row intake holds the spike bitmap in LDS: at most %d groups", kDenseWords - 1);
+    if (rows && format != N2H_ROWS_WIDE && format != N2H_ROWS_COMPACT)
+        return fail(N2H_EINVAL, "unknown row format %d", format);
+    if (rows && format == N2H_ROWS_COMPACT && !wtab) return fail(N2H_EINVAL, "compact rows need a weight table");
+    if (rows && format == N2H_ROWS_COMPACT && static_cast<int64_t>(kWave) * p->tables.n_groups > (1 << 17))
+        return fail(N2H_EINVAL, "compact rows name at most 2^17 source slots");
+    if (uniform_delay < 0 || uniform_delay > N2H_MAX_SLOTS) return fail(N2H_EINVAL, "uniform_delay %d", uniform_delay);
     if (dense_min_spikes < 0) return fail(N2H_EINVAL, "negative dense_min_spikes");
-    p->rows = reinterpret_cast<const uint2*>(rows);
+    p->rows = reinterpret_cast<const uint4*>(rows);
     p->row_base = row_base;
-    p->row_len = row_len;
     p->row_depth = row_depth;
+    p->row_format = format;
+    p->row_wtab = wtab;
+    p->row_uniform = uniform_delay - 1;
     p->dense_min = dense_min_spikes;
     return N2H_OK;
 }

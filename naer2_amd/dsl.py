@@ -251,7 +251,10 @@ class Connection:
     def __init__(self, source: Population, target: Population, src: torch.Tensor, tgt: torch.Tensor,
                  edge_instance=None):
         self.source, self.target = source, target
-        self.src, self.tgt = src, tgt
+        # local indices as int32 (populations hold fewer than 2^31 neurons): 8 bytes per link less than int64
+        self._src, self._tgt = src.to(torch.int32), tgt.to(torch.int32)
+        self._n_links = int(tgt.numel())
+        self._released = False
         self.edge = edge_instance
         self._weight = None
         self._delay = None
@@ -262,7 +265,29 @@ class Connection:
 
     @property
     def shape(self) -> tuple[int]:
-        return (int(self.tgt.numel()),)
+        return (self._n_links,)
+
+    def _check_held(self) -> None:
+        if self._released:
+            raise RuntimeError("the connection's links were released after the build "
+                               "(Network.keep_description = False)")
+
+    @property
+    def src(self) -> torch.Tensor:
+        """int32 [n_links] local source index of each link."""
+        self._check_held()
+        return self._src
+
+    @property
+    def tgt(self) -> torch.Tensor:
+        """int32 [n_links] local target index of each link."""
+        self._check_held()
+        return self._tgt
+
+    def release(self) -> None:
+        """Drop the per-link tensors (the built runtime keeps its own tables)."""
+        self._src = self._tgt = self._weight = self._delay = None
+        self._released = True
 
     def _per_link(self, value, name: str) -> torch.Tensor:
         """fp32 [n_links] on the device of the links (a number is broadcast)."""
@@ -275,6 +300,7 @@ class Connection:
 
     @property
     def weight(self) -> torch.Tensor:
+        self._check_held()
         if self._weight is None:
             raise ValueError("weight not set")
         return self._weight
@@ -286,6 +312,7 @@ class Connection:
 
     @property
     def delay(self) -> torch.Tensor:
+        self._check_held()
         if self._delay is None:
             raise ValueError("delay not set")
         return self._delay

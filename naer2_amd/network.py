@@ -42,7 +42,11 @@ class Network:
         # row intake (target-major pull over every in-link, for steps after many spikes): "auto" builds its
         # table where the spike bitmap fits in LDS (<= 1536 groups); dense_min_spikes None: n_neurons // 20
         self.row_intake = "auto"
+        self.row_format = "auto"          # "auto" (compact where it fits), "wide" or "compact"
         self.dense_min_spikes: int | None = None
+        # keep the connections' per-link tensors after the build (False: release them -- the runtime holds
+        # its own tables; connection weights and delays can then no longer be read or changed)
+        self.keep_description: bool = True
         self.t: float = 0.0
         self._generator = None
         self._fuse_rounds = True
@@ -173,8 +177,8 @@ def assemble(net: Network, device=None) -> Assembled:
     for c in net.connections:
         if c._weight is None or c._delay is None:
             raise ValueError(f"{c!r}: weight and delay must be set")
-        srcs.append(c.src.to(dev) + first_of[c.source])
-        tgts.append(c.tgt.to(dev) + first_of[c.target])
+        srcs.append(c.src.to(dev, torch.int64) + first_of[c.source])
+        tgts.append(c.tgt.to(dev, torch.int64) + first_of[c.target])
         ws.append(c.weight.to(dev))
         ds.append(delay_steps(c.delay.to(dev), dt))
     cat = (lambda xs, dt_: torch.cat(xs) if xs else torch.zeros(0, dtype=dt_, device=dev))
@@ -203,6 +207,9 @@ class Runtime:
         if self.device.type != "cuda":
             raise RuntimeError("naer2_amd runs on the GPU: no CUDA/HIP device is available")
         a = assemble(net, self.device)
+        if not net.keep_description:
+            for c in net.connections:
+                c.release()
         self.pops, self.family, self.model, self.dt = a.pops, a.family, a.model, a.dt
         self.pop_first, self.params, self.canon, self.d_slots = a.pop_first, a.params, a.canon, a.d_slots
         self.n = a.n
@@ -213,7 +220,7 @@ class Runtime:
         self.seg_off, self.links = seg.seg_off, seg.links
         self.rows = None
         if net.row_intake and self.n_groups <= _lib.DENSE_GROUPS:
-            self.rows = rows(self.canon, self.layout)
+            self.rows = rows(self.canon, self.layout, fmt=net.row_format)
         elif net.row_intake is True:
             raise ValueError(f"row intake needs at most {_lib.DENSE_GROUPS} groups ({self.n_groups} here)")
         self.dense_min = int(net.dense_min_spikes) if net.dense_min_spikes is not None else max(65, self.n // 20)
@@ -259,10 +266,12 @@ class Runtime:
         lib = _lib.load()
         r = self.rows
         if r is None:
-            _lib.check(lib.n2h_bind_rows(self._plan, None, None, None, None, 0), "n2h_bind_rows")
+            _lib.check(lib.n2h_bind_rows(self._plan, None, None, None, 0, None, 0, 0), "n2h_bind_rows")
         else:
-            _lib.check(lib.n2h_bind_rows(self._plan, r.rows.data_ptr(), r.row_base.data_ptr(), r.row_len.data_ptr(),
-                                         r.row_depth.data_ptr(), int(self.dense_min)), "n2h_bind_rows")
+            wtab = r.wtab.data_ptr() if r.wtab is not None else None
+            _lib.check(lib.n2h_bind_rows(self._plan, r.rows.data_ptr(), r.row_base.data_ptr(), r.row_depth.data_ptr(),
+                                         int(r.fmt), wtab, int(r.uniform_delay), int(self.dense_min)),
+                       "n2h_bind_rows")
 
     def set_dense_min(self, spikes: int) -> None:
         """Steps that follow at least `spikes` spikes use the row intake (results do not change)."""

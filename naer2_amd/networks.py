@@ -123,6 +123,7 @@ def brunel_sweep(params: list[dict], device=None, keep_canonical: bool = False, 
         net, _ = brunel(N_E=p["N_E"], N_I=p["N_I"], C_E=p["C_E"], C_I=p["C_I"], g=p["g"], eta=p["eta"],
                         seed=p["seed"], instance=p["instance"], device=device)
         net.keep_canonical = keep_canonical
+        net.keep_description = keep_canonical  # hundreds of networks: only the runtime's tables stay on the GPU
         net.record = record
         net.build()
         nets.append(net)

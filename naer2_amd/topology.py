@@ -140,19 +140,53 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
     return Segments(seg_off, links)
 
 
+ROWS_WIDE, ROWS_COMPACT = 0, 1
+COMPACT_WEIGHTS = 256      # distinct weights a compact row table can index
+COMPACT_SLOTS = 1 << 17    # source slots a compact entry can name
+
+
 @dataclass
 class Rows:
-    """Row-intake tables: each target's in-links in contract order (ascending canonical link index), as one
-    ELL block per group -- entry k of lane l at rows[row_base[g] + 64 k + l]."""
-    rows: torch.Tensor       # int32 [entries, 2]: (source slot << 7 | (delay - 1), weight bits)
-    row_base: torch.Tensor   # int64 [G]
+    """Row-intake tables: each target's in-links in contract order (ascending canonical link index).
+
+    Each group is one block of 16-byte cells [depth][64]: cell q of lane l is at cell index
+    row_base[g] + 64 q + l and holds the lane's entries per*q .. per*q + per - 1, so that every lane loads
+    its row 16 bytes at a time and a wavefront's load is 1 KB of contiguous memory.  Two entry formats:
+      wide    (per = 2): two int32 words (source slot << 7 | (delay - 1), weight bits);
+      compact (per = 4): one int32 word  source slot << 15 | (delay - 1) << 8 | weight index, the weight
+                         being wtab[index] -- for networks with at most 256 distinct weights and
+                         2^17 source slots (Brunel-type networks: half the bytes).
+    Entries past a target's in-links name the sentinel source slot 64 G, which never spikes (the kernel
+    keeps a zero bitmap word for it), so the walk needs no bounds test.
+    """
+    rows: torch.Tensor       # int32 [cells * 4]
+    row_base: torch.Tensor   # int64 [G]: first cell of each group
     row_len: torch.Tensor    # int32 [64 G]: in-links per target slot
-    row_depth: torch.Tensor  # int32 [G]
+    row_depth: torch.Tensor  # int32 [G]: cells per lane
+    fmt: int                 # ROWS_WIDE or ROWS_COMPACT
+    wtab: torch.Tensor | None  # float32 [COMPACT_WEIGHTS] (compact)
+    uniform_delay: int       # the delay in steps of every link when all are equal, else 0
 
 
-def rows(canon: Canonical, layout: GroupLayout) -> Rows:
+def _as_u32_bits(v: torch.Tensor) -> torch.Tensor:
+    """int64 values in [0, 2^32) as the int32 tensor with the same 32 bits."""
+    return torch.where(v >= 2 ** 31, v - 2 ** 32, v).to(torch.int32)
+
+
+def rows(canon: Canonical, layout: GroupLayout, fmt: str = "auto") -> Rows:
+    """fmt: "auto" (compact where it fits), "wide" or "compact"."""
     dev = canon.tgt.device
     n, G = canon.n, layout.n_groups
+    wbits = canon.w.contiguous().view(torch.int32)
+    uniq = None
+    if fmt in ("auto", "compact") and GROUP * G <= COMPACT_SLOTS:
+        uniq = torch.unique(wbits)
+        if uniq.numel() > COMPACT_WEIGHTS:
+            uniq = None
+    if fmt == "compact" and uniq is None:
+        raise ValueError("compact rows need at most 256 distinct weights and 2^17 source slots")
+    compact = uniq is not None and fmt != "wide"
+    per = 4 if compact else 2
     slot = slot_of(layout).to(dev)
     group_of = layout.group_of.to(dev)
     local_of = layout.local_of.to(dev)
@@ -161,22 +195,41 @@ def rows(canon: Canonical, layout: GroupLayout) -> Rows:
     deg = torch.bincount(tg, minlength=n)
     row_len = torch.zeros(GROUP * G, dtype=torch.int64, device=dev)
     row_len[slot] = deg
-    depth = row_len.view(G, GROUP).max(dim=1).values
+    depth = (row_len.view(G, GROUP).max(dim=1).values + per - 1) // per
     row_base = torch.zeros(G, dtype=torch.int64, device=dev)
     if G > 1:
         row_base[1:] = torch.cumsum(depth * GROUP, 0)[:-1]
-    total = int((depth * GROUP).sum())
+    cells = int((depth * GROUP).sum())
     order = torch.argsort(tg, stable=True)  # by target, canonical order within each target
     t_sorted = tg[order]
     row_start = torch.cumsum(deg, 0) - deg
     rank = torch.arange(E, device=dev) - row_start[t_sorted]
-    pos = row_base[group_of[t_sorted]] + rank * GROUP + local_of[t_sorted]
-    del rank, row_start
+    cell = row_base[group_of[t_sorted]] + (rank // per) * GROUP + local_of[t_sorted]
+    word = cell * 4 + (rank % per) * (1 if compact else 2)
+    del rank, row_start, cell, t_sorted
     src = torch.repeat_interleave(slot, canon.out_degree())[order]
-    x = (src << 7) | (canon.d.to(torch.int64)[order] - 1)
-    y = canon.w[order].contiguous().view(torch.int32)
-    del src, order
-    out = torch.zeros(total, 2, dtype=torch.int32, device=dev)
-    out[pos, 0] = x.to(torch.int32)
-    out[pos, 1] = y
-    return Rows(out, row_base, row_len.to(torch.int32), depth.to(torch.int32))
+    dm1 = canon.d.to(torch.int64)[order] - 1
+    sentinel = GROUP * G
+    out = torch.zeros(cells * 4, dtype=torch.int32, device=dev)
+    fill = (sentinel << 15) if compact else (sentinel << 7)
+    fill = fill - 2 ** 32 if fill >= 2 ** 31 else fill  # the same 32 bits as int32
+    if compact:
+        out[:] = fill
+    else:
+        out.view(-1, 2)[:, 0] = fill
+    wtab = None
+    if compact:
+        widx = torch.searchsorted(uniq, wbits[order].contiguous())
+        out[word] = _as_u32_bits((src << 15) | (dm1 << 8) | widx)
+        wtab = torch.zeros(COMPACT_WEIGHTS, dtype=torch.int32, device=dev)
+        wtab[:uniq.numel()] = uniq
+        wtab = wtab.view(torch.float32)
+    else:
+        out[word] = _as_u32_bits((src << 7) | dm1)
+        out[word + 1] = wbits[order]
+    del src, dm1, order, word
+    uniform = 0
+    if E and bool((canon.d == canon.d[0]).all()):
+        uniform = int(canon.d[0])
+    return Rows(out, row_base, row_len.to(torch.int32), depth.to(torch.int32),
+                ROWS_COMPACT if compact else ROWS_WIDE, wtab, uniform)

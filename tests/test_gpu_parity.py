@@ -74,15 +74,22 @@ def test_forced_spikes_deliver_like_the_oracle(n, n_links, max_delay, p_spike):
     assert_state_equal(rt, st)
 
 
-@pytest.mark.parametrize("dense_min", [0, 10**9])
-def test_row_intake_and_segment_intake_agree_with_the_oracle(dense_min):
-    """The same forced spike patterns through the row intake (every step) and the segment intake (never)."""
+@pytest.mark.parametrize("dense_min,fmt,uniform", [(0, "wide", False), (10**9, "wide", False), (0, "compact", False),
+                                                   (0, "wide", True), (0, "compact", True)])
+def test_row_intake_and_segment_intake_agree_with_the_oracle(dense_min, fmt, uniform):
+    """The same forced spike patterns through the row intake (every step; wide and compact row entries,
+    mixed delays and one delay for all links) and the segment intake (never)."""
     n, n_links, max_delay = 700, 30000, 12
-    _say(f"case: row intake dense_min={dense_min} n={n}")
-    topo = tiny_topology(n, n_links, max_delay, seed=17)
-    net, pop = network_from_topology(ProbeNode(), n, topo)
+    _say(f"case: row intake dense_min={dense_min} fmt={fmt} uniform={uniform} n={n}")
+    src_ptr, tgt, w, d = tiny_topology(n, n_links, max_delay, seed=17)
+    if fmt == "compact":  # 200 distinct weights of varied magnitude and sign, -0.0 among them
+        w = np.concatenate([w[:199], np.array([-0.0], np.float32)])[np.random.default_rng(5).integers(0, 200, n_links)]
+    if uniform:
+        d = np.full_like(d, 7)
+    net, pop = network_from_topology(ProbeNode(), n, (src_ptr, tgt, w, d))
     rt = net.build()
-    assert rt.rows is not None
+    assert rt.rows is not None and rt.rows.fmt == (1 if fmt == "compact" else 0)
+    assert rt.rows.uniform_delay == (7 if uniform else 0)
     rt.set_dense_min(dense_min)
     st = oracle_state(rt)
     rng = np.random.default_rng(18)

@@ -17,7 +17,7 @@ from naer2_amd import _lib
 from naer2_amd.network import assemble, bits_to_raster, raster_to_bits
 from naer2_amd.networks import PD14, brunel, pd14
 from naer2_amd.nodes import FixedInDegree, FixedTotalNumber
-from naer2_amd.topology import canonical, delay_steps, group_layout, rows, segments, slot_of
+from naer2_amd.topology import ROWS_COMPACT, ROWS_WIDE, canonical, delay_steps, group_layout, rows, segments, slot_of
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -173,26 +173,75 @@ def test_unknown_node_family_is_rejected_at_build():
         assemble(A.network)
 
 
-def test_rows_hold_each_targets_in_links_in_canonical_order():
+@pytest.mark.parametrize("fmt", ["wide", "compact"])
+def test_rows_hold_each_targets_in_links_in_canonical_order(fmt):
     rng = np.random.default_rng(3)
     n = 150
     src = torch.from_numpy(rng.integers(0, n, 4000))
     tgt = torch.from_numpy(rng.integers(0, n, 4000))
-    w = torch.from_numpy(rng.standard_normal(4000).astype(np.float32))
-    d = torch.from_numpy(rng.integers(1, 30, 4000).astype(np.int32))
+    if fmt == "wide":
+        w = torch.from_numpy(rng.standard_normal(4000).astype(np.float32))
+    else:  # a few distinct weights, as in Brunel-type networks (and -0.0 apart from +0.0)
+        w = torch.from_numpy(rng.choice(np.array([0.2, -1.0, 0.0, -0.0, 3.5e-3], np.float32), 4000))
+    d = torch.from_numpy(rng.integers(1, 129, 4000).astype(np.int32))
     can = canonical(n, src, tgt, w, d)
     lay = group_layout([100, 50])
-    r = rows(can, lay)
+    r = rows(can, lay, fmt="auto")
+    assert r.fmt == (ROWS_WIDE if fmt == "wide" else ROWS_COMPACT)
+    per = 2 if fmt == "wide" else 4
     slot = (lay.group_of * 64 + lay.local_of).numpy()
     src_of = np.repeat(np.arange(n), np.diff(can.src_ptr.numpy()))
     tg = can.tgt.numpy()
-    table = r.rows.numpy()
+    words = r.rows.numpy().view(np.uint32)
+    sentinel = 64 * lay.n_groups
+    assert r.uniform_delay == 0
+    for g in range(lay.n_groups):  # padding entries name the never-spiking sentinel slot
+        for lane in range(64):
+            ln = int(r.row_len[g * 64 + lane])
+            for k in range(ln, int(r.row_depth[g]) * per):
+                cell = int(r.row_base[g]) + 64 * (k // per) + lane
+                x = words[cell * 4 + (k % per) * (2 if fmt == "wide" else 1)]
+                assert (x >> (7 if fmt == "wide" else 15)) == sentinel
     for i in range(n):
         g, lane = int(lay.group_of[i]), int(lay.local_of[i])
         expect = np.nonzero(tg == i)[0]  # ascending canonical link index
         assert int(r.row_len[g * 64 + lane]) == len(expect)
-        assert int(r.row_depth[g]) >= len(expect)
+        assert int(r.row_depth[g]) * per >= len(expect)
         for k, l in enumerate(expect):
-            x, y = table[int(r.row_base[g]) + 64 * k + lane]
-            assert (x >> 7) == slot[src_of[l]] and (x & 127) + 1 == can.d[l].item()
-            assert np.int32(y).view(np.float32) == can.w[l].item()
+            cell = int(r.row_base[g]) + 64 * (k // per) + lane
+            if fmt == "wide":
+                x, y = words[cell * 4 + 2 * (k % per)], words[cell * 4 + 2 * (k % per) + 1]
+                assert (x >> 7) == slot[src_of[l]] and (x & 127) + 1 == can.d[l].item()
+                assert np.uint32(y).view(np.float32).view(np.uint32) == can.w[l:l + 1].numpy().view(np.uint32)[0]
+            else:
+                x = words[cell * 4 + (k % per)]
+                assert (x >> 15) == slot[src_of[l]] and ((x >> 8) & 127) + 1 == can.d[l].item()
+                wt = r.wtab.numpy().view(np.uint32)[x & 255]
+                assert wt == can.w[l:l + 1].numpy().view(np.uint32)[0]  # the same bits, -0.0 included
+
+
+def test_released_connections_refuse_link_access():
+    A = n2.LIFDelta() * 10
+    B = n2.LIFDelta() * 10
+    conn = A >> FixedInDegree(3) >> B
+    conn.weight = 0.5
+    conn.delay = 1.0
+    assert conn.shape == (30,) and conn.src.dtype == torch.int32
+    conn.release()
+    assert conn.shape == (30,)
+    for name in ("src", "tgt", "weight", "delay"):
+        with pytest.raises(RuntimeError):
+            getattr(conn, name)
+    with pytest.raises(RuntimeError):
+        conn.weight = 1.0
+
+
+def test_rows_flag_a_uniform_delay():
+    rng = np.random.default_rng(4)
+    n = 90
+    src = torch.from_numpy(rng.integers(0, n, 900))
+    tgt = torch.from_numpy(rng.integers(0, n, 900))
+    w = torch.full((900,), 0.25)
+    d = torch.full((900,), 15, dtype=torch.int32)
+    r = rows(canonical(n, src, tgt, w, d), group_layout([n]))
+    assert r.uniform_delay == 15 and r.fmt == ROWS_COMPACT
