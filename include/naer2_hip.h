@@ -98,7 +98,8 @@ typedef struct n2h_model_desc {
  * as {x = delay_steps << 16 | target - group_first[g], y = fp32 weight bits}. */
 typedef struct n2h_tables {
     int32_t n_groups;
-    int32_t reserved;
+    int32_t uniform_delay;         /* delay_steps of every link when all are equal, else 0: the step then
+                                    * keeps one partial sum per target instead of one per delay */
     const int32_t* group_first;    /* device [n_groups] */
     const int32_t* group_info;     /* device [n_groups] */
     const int64_t* seg_off;        /* device [64 * n_groups * n_groups + 1] */
@@ -188,7 +189,7 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
  *   N2H_ROWS_COMPACT (per = 4): source slot << 15 | (delay_steps - 1) << 8 | weight index into wtab[256]
  *                               (at most 256 distinct weights)
  * Entries past a target's in-links name the sentinel source slot 64 * n_groups (never spiking).
- * uniform_delay: the delay in steps of every link when they are all equal (a branch-free walk), else 0.
+ * With tables->uniform_delay set, the walk is branch-free.
  * Steps that follow at least dense_min_spikes spikes walk every in-link against the previous step's spike
  * bitmap (held in LDS: n_groups <= 1535) instead of looking up the spiking sources' segments; both intakes
  * give bit-identical results.  rows = NULL: off.
@@ -196,7 +197,7 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
 #define N2H_ROWS_WIDE 0
 #define N2H_ROWS_COMPACT 1
 int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
-                  int32_t format, const float* wtab, int32_t uniform_delay, int32_t dense_min_spikes);
+                  int32_t format, const float* wtab, int32_t dense_min_spikes);
 
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list

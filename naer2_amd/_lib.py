@@ -52,7 +52,7 @@ class ModelDesc(ctypes.Structure):
 
 class Tables(ctypes.Structure):
     _fields_ = [
-        ("n_groups", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("n_groups", ctypes.c_int32), ("uniform_delay", ctypes.c_int32),
         ("group_first", ctypes.c_void_p), ("group_info", ctypes.c_void_p), ("seg_off", ctypes.c_void_p),
         ("links", ctypes.c_void_p), ("n_links", ctypes.c_int64),
     ]
@@ -95,7 +95,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)], i32),
         "n2h_batch_destroy": ([vp], i32),
         "n2h_set_trace": ([vp, vp], i32),
-        "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32, i32], i32),
+        "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }

@@ -101,7 +101,7 @@ struct StepArgs {
     int32_t row_format;         // N2H_ROWS_WIDE / N2H_ROWS_COMPACT
     int32_t row_uniform;        // every link's delay cell (delay - 1) when all delays are equal, else -1
     int32_t dense_min;          // row intake for steps following at least this many spikes
-    int32_t pad3;
+    int32_t acc_base;           // the delay cell held at acc[0]: row_uniform for uniform delays (one cell), else 0
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -266,33 +266,37 @@ constexpr int kRowU = 8;
 template <int PER, bool UNIFORM, class St>
 __device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, const uint4 (&c)[kRowU], int& cb,
                                           float& cr, uint32_t& any, unsigned long long& t0, unsigned long long& t1) {
-    constexpr int E = kRowU * PER;
+    constexpr int kHalf = kRowU / 2;  // cells per LDS round (bounds the registers the decoded entries hold)
+    constexpr int E = kHalf * PER;
     const uint32_t* bm32 = reinterpret_cast<const uint32_t*>(st.bm);
-    uint32_t word[E], bw[E], wb[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t* cw = reinterpret_cast<const uint32_t*>(&c[e / PER]);
-        const uint32_t x = (PER == 4) ? cw[e % PER] : cw[2 * (e % PER)];
-        const uint32_t src = (PER == 4) ? (x >> 15) : (x >> 7);
-        word[e] = x;
-        bw[e] = bm32[src >> 5];
-        wb[e] = (PER == 4) ? __float_as_uint(st.wtab[x & 255u]) : cw[2 * (e % PER) + 1];
-    }
-    if (UNIFORM) {
+    for (int h = 0; h < 2; ++h) {
+        uint32_t word[E], bw[E], wb[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
-            const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bw[e]), src, 1u));
-            any |= m;
-            cr = cr + __uint_as_float((wb[e] & m) | (0x80000000u & ~m));
+            const uint32_t* cw = reinterpret_cast<const uint32_t*>(&c[h * kHalf + e / PER]);
+            const uint32_t x = (PER == 4) ? cw[e % PER] : cw[2 * (e % PER)];
+            const uint32_t src = (PER == 4) ? (x >> 15) : (x >> 7);
+            word[e] = x;
+            bw[e] = bm32[src >> 5];
+            wb[e] = (PER == 4) ? __float_as_uint(st.wtab[x & 255u]) : cw[2 * (e % PER) + 1];
         }
-    } else {
+        if (UNIFORM) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
-            if ((bw[e] >> (src & 31u)) & 1u) {
-                const int b = (PER == 4) ? static_cast<int>((word[e] >> 8) & 127u) : static_cast<int>(word[e] & 127u);
-                cell_add(acc, lane, b, __uint_as_float(wb[e]), cb, cr, t0, t1);
+            for (int e = 0; e < E; ++e) {
+                const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
+                const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bw[e]), src, 1u));
+                any |= m;
+                cr = cr + __uint_as_float((wb[e] & m) | (0x80000000u & ~m));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t src = (PER == 4) ? (word[e] >> 15) : (word[e] >> 7);
+                if ((bw[e] >> (src & 31u)) & 1u) {
+                    const int b = (PER == 4) ? static_cast<int>((word[e] >> 8) & 127u) : static_cast<int>(word[e] & 127u);
+                    cell_add(acc, lane, b, __uint_as_float(wb[e]), cb, cr, t0, t1);
+                }
             }
         }
     }
@@ -466,8 +470,11 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
 // KW: bitmap words per lane held in registers on the burst path (0: read them in a loop).
 template <int MODEL, int KW>
 __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, Stage<bm_words(KW)>& st,
-                                           float* acc) {
+                                           float* acc_lds) {
     const int lane = threadIdx.x;
+    // acc[b * 64 + lane]: the partial sum of delay cell b; the LDS holds cells acc_base onwards (one cell for
+    // a uniform delay)
+    float* acc = acc_lds - a.acc_base * kWave;
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
     const int D = a.d_slots;
@@ -778,7 +785,6 @@ struct n2h_plan {
     const int32_t* row_depth;
     const float* row_wtab;
     int32_t row_format;
-    int32_t row_uniform;
     int32_t dense_min;
     size_t lds_bytes;
     int device;
@@ -846,7 +852,8 @@ StepArgs make_args(const n2h_plan* p) {
     a.row_depth = p->row_depth;
     a.row_wtab = p->row_wtab;
     a.row_format = p->row_format;
-    a.row_uniform = p->rows ? p->row_uniform : -1;
+    a.row_uniform = p->tables.uniform_delay - 1;
+    a.acc_base = p->tables.uniform_delay > 0 ? p->tables.uniform_delay - 1 : 0;
     a.dense_min = p->dense_min;
     return a;
 }
@@ -997,6 +1004,8 @@ int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_pl
         if (desc->pops[p].ref_steps < 0) return fail(N2H_EINVAL, "population %d: negative ref_steps", p);
     }
     if (tables->n_groups <= 0) return fail(N2H_EINVAL, "n_groups must be positive");
+    if (tables->uniform_delay < 0 || tables->uniform_delay > desc->d_slots)
+        return fail(N2H_EINVAL, "uniform_delay %d outside [0, d_slots]", tables->uniform_delay);
     if (!tables->group_first || !tables->group_info || !tables->seg_off || (!tables->links && tables->n_links))
         return fail(N2H_EINVAL, "null table pointer");
     int64_t min_groups = 0;
@@ -1017,7 +1026,8 @@ int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_pl
     memcpy(p->pop_first_host, desc->pop_first, sizeof(int32_t) * (desc->n_pops + 1));
     p->desc.pop_first = p->pop_first_host;
     p->desc.pops = nullptr;
-    p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(desc->d_slots);
+    // partial sums: one cell per delay, or the one cell of a uniform delay
+    p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(tables->uniform_delay > 0 ? 1 : desc->d_slots);
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->pops_dev, sizeof(n2h_pop_params) * desc->n_pops);
     if (e == hipSuccess) e = hipMalloc(&p->step_ctr, sizeof(int64_t));
@@ -1266,7 +1276,7 @@ int n2h_graph_destroy(n2h_graph* gr) {
 }
 
 int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
-                  int32_t format, const float* wtab, int32_t uniform_delay, int32_t dense_min_spikes) {
+                  int32_t format, const float* wtab, int32_t dense_min_spikes) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     if (rows && (!row_base || !row_depth)) return fail(N2H_EINVAL, "null row table");
     if (rows && p->tables.n_groups > kDenseWords - 1)
@@ -1276,14 +1286,12 @@ int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, co
     if (rows && format == N2H_ROWS_COMPACT && !wtab) return fail(N2H_EINVAL, "compact rows need a weight table");
     if (rows && format == N2H_ROWS_COMPACT && static_cast<int64_t>(kWave) * p->tables.n_groups > (1 << 17))
         return fail(N2H_EINVAL, "compact rows name at most 2^17 source slots");
-    if (uniform_delay < 0 || uniform_delay > N2H_MAX_SLOTS) return fail(N2H_EINVAL, "uniform_delay %d", uniform_delay);
     if (dense_min_spikes < 0) return fail(N2H_EINVAL, "negative dense_min_spikes");
     p->rows = reinterpret_cast<const uint4*>(rows);
     p->row_base = row_base;
     p->row_depth = row_depth;
     p->row_format = format;
     p->row_wtab = wtab;
-    p->row_uniform = uniform_delay - 1;
     p->dense_min = dense_min_spikes;
     return N2H_OK;
 }

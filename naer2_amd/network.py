@@ -226,6 +226,9 @@ class Runtime:
         self.dense_min = int(net.dense_min_spikes) if net.dense_min_spikes is not None else max(65, self.n // 20)
         self.n_links = self.canon.n_links
         self.out_degree = self.canon.out_degree()
+        # every link's delay in steps when all are equal (one partial sum per target in the step), else 0
+        d = self.canon.d
+        self.uniform_delay = int(d[0]) if d.numel() and bool((d == d[0]).all()) else 0
         if not net.keep_canonical:
             self.canon = None
         del seg
@@ -251,7 +254,8 @@ class Runtime:
         desc = _lib.ModelDesc(model=self.model, n_neurons=self.n, n_pops=len(self.pops), d_slots=self.d_slots,
                               pop_first=first_arr, pops=pops_arr, seed=int(self.net.seed) & (2**64 - 1),
                               instance=int(self.net.instance) & 0xFFFFFFFF)
-        tab = _lib.Tables(n_groups=self.n_groups, group_first=self.layout.group_first.data_ptr(),
+        tab = _lib.Tables(n_groups=self.n_groups, uniform_delay=self.uniform_delay,
+                          group_first=self.layout.group_first.data_ptr(),
                           group_info=self.layout.group_info.data_ptr(), seg_off=self.seg_off.data_ptr(),
                           links=self.links.data_ptr(), n_links=self.n_links)
         plan = ctypes.c_void_p()
@@ -266,11 +270,11 @@ class Runtime:
         lib = _lib.load()
         r = self.rows
         if r is None:
-            _lib.check(lib.n2h_bind_rows(self._plan, None, None, None, 0, None, 0, 0), "n2h_bind_rows")
+            _lib.check(lib.n2h_bind_rows(self._plan, None, None, None, 0, None, 0), "n2h_bind_rows")
         else:
             wtab = r.wtab.data_ptr() if r.wtab is not None else None
             _lib.check(lib.n2h_bind_rows(self._plan, r.rows.data_ptr(), r.row_base.data_ptr(), r.row_depth.data_ptr(),
-                                         int(r.fmt), wtab, int(r.uniform_delay), int(self.dense_min)),
+                                         int(r.fmt), wtab, int(self.dense_min)),
                        "n2h_bind_rows")
 
     def set_dense_min(self, spikes: int) -> None:
