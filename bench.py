@@ -70,27 +70,13 @@ def cpu_baseline(rt, state_snapshot, step0: int, n_steps: int) -> dict:
     return {"wall_s": wall, "events": events, "raster": raster, "v": st.v}
 
 
-def _snapshot(rt):
-    return (rt.step, rt.v.clone(), rt.ref.clone(), rt.ring.clone(), rt.bits.clone(), rt.spike_list.clone())
-
-
-def _restore(rt, snap):
-    step, v, ref, ring, bits, lst = snap
-    rt.v.copy_(v)
-    rt.ref.copy_(ref)
-    rt.ring.copy_(ring)
-    rt.bits.copy_(bits)
-    rt.spike_list.copy_(lst)
-    rt.set_step(step)
-
-
 def bench_sweep(args) -> int:
     """Batched Brunel sweep: --instances networks drawn from --seed (N log-uniform in [1k, 50k], in-degree
     10 %, g in [3, 8], eta in [0.9, 4]), one batched launch per step (mapping from the autotuner, tuned on
     the warmup steps) against the same networks run one after another from the same state.  The final
     states of both must be bit-identical."""
     from naer2_amd import FusedRound
-    from naer2_amd.autotune import tune
+    from naer2_amd.autotune import tune, tune_intake
     from naer2_amd.networks import brunel_sweep, sweep_params
 
     params = sweep_params(args.instances, seed=args.seed)
@@ -108,14 +94,19 @@ def bench_sweep(args) -> int:
     print(f"[bench] sweep: {len(rts)} networks, {n_neurons} neurons, {n_syn} synapses, "
           f"{sum(rt.n_groups for rt in rts)} groups, built in {time.perf_counter() - t0:.0f} s",
           file=sys.stderr, flush=True)
+    if args.tune_intake:
+        t0 = time.perf_counter()
+        for rt in rts:
+            tune_intake(rt, steps=200, graph_window=args.window)
+        print(f"[bench] intake thresholds tuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     best, times = tune(rts, steps=args.warmup, graph_window=args.window, use_cache=False)
     print(f"[bench] autotune: {best} from {times}", file=sys.stderr, flush=True)
-    snaps = [_snapshot(rt) for rt in rts]
+    snaps = [rt.snapshot() for rt in rts]
     fr = FusedRound(rts, best)
     fr.launch(args.window, args.window)  # capture, untimed (the state is restored below)
     torch.cuda.synchronize()
     for rt, sn in zip(rts, snaps):
-        _restore(rt, sn)
+        rt.restore(sn)
     fr.close()
     fr = FusedRound(rts, best)
     torch.cuda.synchronize()
@@ -127,11 +118,11 @@ def bench_sweep(args) -> int:
     print(f"[bench] batched {args.steps} steps: {wall_b:.3f} s", file=sys.stderr, flush=True)
     final_b = [(rt.v.clone(), rt.ref.clone(), rt.ring.clone()) for rt in rts]
     for rt, sn in zip(rts, snaps):
-        _restore(rt, sn)
+        rt.restore(sn)
         rt.run(args.window, args.window)  # capture each network's own graph, untimed
     torch.cuda.synchronize()
     for rt, sn in zip(rts, snaps):
-        _restore(rt, sn)
+        rt.restore(sn)
     torch.cuda.synchronize()
     wall_s = 0.0
     for i, rt in enumerate(rts):
@@ -221,6 +212,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=["pd14", "sweep", "bptt"], default="pd14",
                     help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep")
     ap.add_argument("--instances", type=int, default=256, help="networks in the sweep")
+    ap.add_argument("--tune-intake", action="store_true",
+                    help="sweep: time each network's row-intake thresholds first (naer2_amd.autotune.tune_intake)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run 5 plain device-to-device copies of MiB each (the reference kernel that scales "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes), then exit")

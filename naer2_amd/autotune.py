@@ -8,6 +8,11 @@ and keeps the fastest.  Choices are cached per device and batch signature in a J
 Candidates: order 0 (plans as given) or 1 (most links first, so the heaviest groups start first and light
 ones fill the tail); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8
 (a plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments).
+
+tune_intake picks, per network, the spike count from which a step takes the row intake (every in-link
+walked against the spike bitmap) instead of the segment intake (the spiking sources' segments): both give
+bit-identical results, and which is faster depends on the in-degree and the activity regime, so each
+candidate is timed from the same snapshot of the state, which is restored afterwards.
 """
 from __future__ import annotations
 
@@ -72,4 +77,34 @@ def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int
     if use_cache:
         table[key] = {"best": best, "times": times}
         _save(table)
+    return best, times
+
+
+INTAKE_FRACTIONS = (0.002, 0.005, 0.01, 0.02, 0.05)
+
+
+def tune_intake(rt, steps: int = 200, graph_window: int = 100, fractions=INTAKE_FRACTIONS):
+    """Set rt's row-intake threshold to the fastest of max(65, f * n) for f in fractions, or never; the
+    state is restored afterwards (recording restarts there).  Returns (threshold, {threshold: seconds});
+    a no-op without a row table."""
+    if rt.rows is None:
+        return None, {}
+    never = 1 << 30
+    cands = sorted({max(65, int(f * rt.n)) for f in fractions}) + [never]
+    rt._ensure_slots(max(steps, graph_window) + 4)  # the recorded history must not grow under the snapshot
+    snap = rt.snapshot()
+    times = {}
+    for c in cands:
+        rt.set_dense_min(c)
+        rt.restore(snap)
+        rt.run(graph_window, graph_window)  # capture, untimed
+        rt.restore(snap)
+        torch.cuda.synchronize(rt.device)
+        t0 = time.perf_counter()
+        rt.run(steps, graph_window)
+        torch.cuda.synchronize(rt.device)
+        times[c] = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    rt.set_dense_min(best)
+    rt.restore(snap)
     return best, times

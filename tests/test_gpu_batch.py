@@ -10,7 +10,7 @@ import pytest
 import torch
 
 from naer2_amd import FusedRound
-from naer2_amd.autotune import CANDIDATES, tune
+from naer2_amd.autotune import CANDIDATES, tune, tune_intake
 from naer2_amd.networks import brunel_sweep, sweep_params
 from tests.gpu_util import bits_equal
 
@@ -80,3 +80,25 @@ def test_autotune_returns_a_candidate_and_keeps_results():
     torch.cuda.synchronize()
     for a, b in zip(batched, plain):
         _same(a, b)
+
+
+def test_intake_tuning_restores_the_state_and_changes_no_result():
+    """tune_intake times several row-intake thresholds from one snapshot and restores it: the run that
+    follows equals an untuned run bit for bit (the intakes give identical results)."""
+    p = dict(N_E=3200, N_I=800, C_E=320, C_I=80, g=3.2, eta=3.0, seed=8, instance=0)
+    _say("case: intake tuning, bursting brunel N=4000")
+    a, b = brunel_sweep([p, p], device="cuda", keep_canonical=False, record=True)
+    a.run(40)
+    b.run(40)
+    best, times = tune_intake(a.runtime, steps=60, graph_window=20)
+    assert best in times and len(times) >= 2
+    assert a.runtime.step == b.runtime.step == 40
+    a.run(200)
+    b.run(200)
+    torch.cuda.synchronize()
+    ra, rb = a.runtime, b.runtime
+    assert np.array_equal(ra.raster(40, 240), rb.raster(40, 240))
+    assert ra.raster(40, 240).sum(axis=1).max() >= 200  # bursts: the thresholds mattered
+    assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert np.array_equal(ra.ref.cpu().numpy(), rb.ref.cpu().numpy())
+    assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
