@@ -79,7 +79,7 @@ def bench_sweep(args) -> int:
     from naer2_amd.autotune import tune, tune_intake
     from naer2_amd.networks import brunel_sweep, sweep_params
 
-    params = sweep_params(args.instances, seed=args.seed)
+    params = sweep_params(args.instances or 256, seed=args.seed)
     t0 = time.perf_counter()
     nets = []
     for i, p in enumerate(params):
@@ -149,6 +149,79 @@ def bench_sweep(args) -> int:
     return 0
 
 
+def bench_pd14grad(args) -> int:
+    """Batched, differentiable PD14: --instances seed-distinct full-scale microcircuits in one launch per
+    step, --steps steps (500 ms bio at 5000), loss = squared error of the per-population rates against
+    seed-drawn targets in [0.5, 10] Hz, gradient with respect to the 8x8 weight scales from the surrogate
+    adjoint (truncated at --cutoff, scaled by --gamma) with checkpoints every --checkpoint steps.  Times the
+    forward (checkpointed, graph windows) and
+    the backward (recompute + adjoint) with HIP events, and checks the same adjoint on a
+    4-neuron-per-population PD14 subsample against the FP64 autograd unroll (on the same spike trains),
+    whose own gradient FP64 central differences pin in the CPU tests."""
+    import numpy as np
+
+    from naer2_amd.adjoint import RateLossGrad
+    from naer2_amd.networks import pd14
+
+    n_inst, T, K = args.instances or 4, args.steps, args.checkpoint
+    rng = np.random.default_rng(args.seed)
+    alpha = np.ones((8, 8), dtype=np.float32)
+    targets = rng.uniform(0.5, 10.0, (n_inst, 8))
+    t0 = time.perf_counter()
+    rts = []
+    for i in range(n_inst):
+        net, _ = pd14(seed=args.seed + i, instance=i, device="cuda", weight_scale=alpha)
+        net.record = True
+        net.keep_canonical = True
+        rts.append(net.build())
+    torch.cuda.synchronize()
+    n_syn = sum(rt.n_links for rt in rts)
+    print(f"[bench] built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s",
+          file=sys.stderr, flush=True)
+    sur = {"cutoff": args.cutoff, "gamma": args.gamma}
+    grad = RateLossGrad(rts, alpha, checkpoint=K, **sur)
+    t1 = time.perf_counter()
+    loss, g = grad(T, targets)
+    wall = time.perf_counter() - t1
+    fw, bw = grad.last["forward_ms"], grad.last["backward_ms"]
+    print(f"[bench] loss {loss:.4g}, forward {fw:.0f} ms, backward {bw:.0f} ms, wall {wall:.1f} s",
+          file=sys.stderr, flush=True)
+    # the check: the same adjoint on a 4-neuron-per-population subsample against the FP64 unroll
+    from oracle import adjoint_oracle as ao
+    Tc = 500
+    sub, _ = pd14(sizes=[4] * 8, prob_scale=10.0, seed=args.seed, device="cuda", weight_scale=alpha)
+    sub.record, sub.keep_canonical = True, True
+    srt = sub.build()
+    v0, i0 = srt.v.cpu().numpy().copy(), srt.isyn.cpu().numpy().copy()
+    ts = rng.uniform(0.5, 10.0, (1, 8))
+    _, gs = RateLossGrad([srt], alpha, checkpoint=100, **sur)(Tc, ts)
+    d = ao.net_from(srt.pop_first, srt.params, srt.canon.to("cpu"), v0, i0, srt.dt)
+    counts = ao.poisson_counts(d["thr"], seed=sub.seed, instance=sub.instance, step0=0, n_steps=Tc)
+    a64 = torch.ones(8, 8, dtype=torch.float64, requires_grad=True)
+    _, rates, _, _ = ao.unroll(d, a64, Tc, counts, mode="spike", forced=srt.raster(0, Tc), **sur)
+    ao.loss(rates, ts[0]).backward()
+    ref = a64.grad.numpy()
+    err = float(np.linalg.norm(gs - ref) / max(np.linalg.norm(ref), 1e-300))
+    bio = T * rts[0].dt / 1000.0
+    line = {
+        "metric": "batched differentiable PD14: backward/forward cost ratio (rate-loss surrogate adjoint)",
+        "value": round(bw / fw, 3), "unit": "ratio", "n_gpus": 1, "steps": T, "warmup": 0,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n_inst} seed-distinct full-scale PD14 (config 1) in one launch a step, {T} steps "
+                               f"of 0.1 ms, loss sum (rate - target)^2 over populations, targets U[0.5, 10] Hz, "
+                               f"grad wrt the 8x8 weight scales, checkpoints every {K} steps, seed {args.seed}",
+                   "n_synapses": n_syn},
+        "forward_ms": round(fw, 1), "backward_ms": round(bw, 1),
+        "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
+        "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
+        "surrogate": {"kappa": 25.0, **sur},
+        "grad_rel_err_vs_fp64_subsample": err,
+        "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_bptt(args) -> int:
     """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
     1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
@@ -209,9 +282,15 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["pd14", "sweep", "bptt"], default="pd14",
-                    help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep")
-    ap.add_argument("--instances", type=int, default=256, help="networks in the sweep")
+    ap.add_argument("--workload", choices=["pd14", "sweep", "bptt", "pd14grad"], default="pd14",
+                    help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep; bptt: the "
+                         "recurrent-LIF BPTT; pd14grad: the batched differentiable PD14")
+    ap.add_argument("--instances", type=int, default=None, help="networks (sweep: 256, pd14grad: 4)")
+    ap.add_argument("--checkpoint", type=int, default=500, help="pd14grad: steps between saved states")
+    ap.add_argument("--cutoff", type=float, default=0.2,
+                    help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
+    ap.add_argument("--gamma", type=float, default=0.01,
+                    help="pd14grad: surrogate scale (0.1 and above diverge over 500 ms at full scale)")
     ap.add_argument("--tune-intake", action="store_true",
                     help="sweep: time each network's row-intake thresholds first (naer2_amd.autotune.tune_intake)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -228,11 +307,13 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build (it would "
+              "shard by target groups, with one all-gather of the step's spike lists)", file=sys.stderr)
         return 2
     if args.workload == "bptt":
         return bench_bptt(args)
+    if args.workload == "pd14grad":
+        return bench_pd14grad(args)
     if args.window <= 0 or args.window % 4 or args.steps % args.window or args.warmup % args.window:
         print("--window must be a multiple of 4, --steps and --warmup multiples of --window", file=sys.stderr)
         return 2

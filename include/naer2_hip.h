@@ -199,6 +199,52 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
 int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
                   int32_t format, const float* wtab, int32_t dense_min_spikes);
 
+/* Surrogate-gradient adjoint of the LIF_EXP step for a per-population rate loss (no counterpart in the
+ * reference, which is forward-only: the north star's differentiation pattern applied to the plain path).
+ * One call runs reverse step `step` for up to 16 networks in one launch.  Per network: its out-links
+ * (canonical CSR by source) packed {target | (delay_steps - 1) << 17 | target population << 24, base weight
+ * bits}; alpha[target pop][source pop] the weight scales the forward ran with (its weights are alpha * base);
+ * pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset, dL/ds of one spike of population p, 0};
+ * this step's potentials before the threshold (n2h_set_vtrace) and spikes; the adjoint state gV, gI [n],
+ * the ring gx [ring_slots = max delay + 1][n] of the input adjoints of the following steps (zero at the start),
+ * and g_sum [n][n_pops], where every spiking source j adds sum over its links into population a of
+ * w gx_{step+d} (dL/dalpha[a][pop j] = sum over j in pop j of g_sum[j][a]).  The reset and the refractory
+ * gate carry no gradient. */
+typedef struct n2h_adj_net {
+    int32_t n_neurons;           /* <= 2^17 */
+    int32_t n_pops;              /* <= 8 */
+    int32_t ring_slots;
+    int32_t reserved;
+    const int32_t* pop_first;    /* device [n_pops + 1] */
+    const int64_t* out_ptr;      /* device [n + 1] */
+    const uint32_t* out_links;   /* device [2 * n_links] */
+    const float* alpha;          /* device [n_pops][n_pops] */
+    const float* pop_coef;       /* device [n_pops][8] */
+    const float* vtrace;         /* device [n] */
+    const uint8_t* spikes;       /* device [n] */
+    float* gx_ring;              /* device [ring_slots][n] */
+    float* g_v;                  /* device [n] */
+    float* g_i;                  /* device [n] */
+    double* g_sum;               /* device [n][n_pops] */
+} n2h_adj_net;
+/* The surrogate: ds/dV = gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - V_th| / dv <= cutoff, else 0
+ * (dv = V_th - V_reset); neurons beyond the cutoff that did not spike skip their out-link gather. */
+typedef struct n2h_surrogate {
+    float kappa;
+    float cutoff;
+    float gamma;
+    float reserved;
+} n2h_surrogate;
+int n2h_adj_last_error(char* buf, size_t len);
+int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h_surrogate* sur, void* stream);
+
+/* Potential trace for the adjoint (no counterpart in the reference, which is forward-only; the north star's
+ * differentiation pattern): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
+ * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a
+ * refractory step.  Results are unchanged.  Captured graphs and batches keep the setting they were made
+ * with. */
+int n2h_set_vtrace(n2h_plan* plan, float* buffer, int64_t step0, int32_t n_steps);
+
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list
  * read, sources ranked, segment bounds, links staged, walked, updated, exit -- into

@@ -26,7 +26,7 @@ EXPORTS = (
     "n2h_version", "n2h_last_error", "n2h_plan_create", "n2h_plan_destroy", "n2h_bind_state",
     "n2h_bind_ring", "n2h_set_step", "n2h_get_step", "n2h_step", "n2h_graph_create", "n2h_graph_launch",
     "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
-    "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_trace", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
+    "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_vtrace", "n2h_set_trace", "n2h_adj_last_error", "n2h_adj_step", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward",
 )
 
@@ -96,6 +96,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_batch_destroy": ([vp], i32),
         "n2h_set_trace": ([vp, vp], i32),
         "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
+        "n2h_set_vtrace": ([vp, vp, i64, i32], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
     }

@@ -1,0 +1,204 @@
+"""Surrogate-gradient adjoint of a batch of exponential-synapse LIF networks (the LIF_EXP step, e.g. PD14)
+for a per-population rate loss, with respect to the weight scales alpha[target pop][source pop]:
+
+    grad = RateLossGrad(runtimes, alpha, checkpoint=500)   # built with weights alpha * base (pd14(weight_scale=...))
+    loss, g = grad(n_steps, targets)                       # targets [n_networks][n_pops] in Hz; g [P][P]
+
+loss = sum over networks and populations of (rate - target)^2 (rates in Hz over the n_steps).  The forward is
+the plain step (bit-exact, one batched launch per step, FusedRound), with the state saved every `checkpoint`
+steps.  The backward runs the segments in reverse: each is recomputed from its checkpoint with the
+potentials before the threshold traced (n2h_set_vtrace), then every step is reversed by n2h_adj_step -- one
+launch for all networks, a wavefront per source gathering its out-links' input adjoints
+(naer2_amd/csrc/adjoint.hip states the recursion).  The surrogate derivative is
+gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - theta| / dv <= cutoff and 0 beyond (dv = theta -
+V_reset); the cutoff makes the gradient sparse (only neurons near the threshold, or spiking, gather).  The
+reset and refractoriness carry no gradient (through the hard reset the adjoint diverges).
+
+The reference is forward-only: this is the north star's differentiation pattern (a hand-written adjoint of
+delivery plus the LIF update, checkpointed every K steps) applied to the plain PD14 path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MODEL_LIF_EXP
+from .fusion import FusedRound
+
+
+class AdjNet(ctypes.Structure):
+    _fields_ = [("n_neurons", ctypes.c_int32), ("n_pops", ctypes.c_int32), ("ring_slots", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in (
+                    "pop_first", "out_ptr", "out_links", "alpha", "pop_coef", "vtrace", "spikes", "gx_ring", "g_v",
+                    "g_i", "g_sum")]
+
+
+class Surrogate(ctypes.Structure):
+    _fields_ = [("kappa", ctypes.c_float), ("cutoff", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("reserved", ctypes.c_float)]
+
+
+def _bind(lib):
+    if getattr(lib, "_adj_bound", False):
+        return lib
+    lib.n2h_adj_last_error.argtypes, lib.n2h_adj_last_error.restype = [ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int32
+    lib.n2h_adj_step.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Surrogate),
+                                 ctypes.c_void_p]
+    lib.n2h_adj_step.restype = ctypes.c_int32
+    lib._adj_bound = True
+    return lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        buf = ctypes.create_string_buffer(512)
+        _lib.load().n2h_adj_last_error(buf, len(buf))
+        raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+
+def out_links(rt, alpha: torch.Tensor):
+    """(out_ptr int64 [n + 1], links int32 [E, 2]): rt's canonical CSR by source packed for the adjoint,
+    {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight / alpha[pt][ps]."""
+    c = rt.canon
+    if c is None:
+        raise RuntimeError("the adjoint needs the canonical synapse table (Network.keep_canonical = True)")
+    dev = rt.device
+    pf = torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int64), device=dev)
+    tgt = c.tgt.to(dev).to(torch.int64)
+    src = torch.repeat_interleave(torch.arange(rt.n, device=dev), c.out_degree().to(dev))
+    pt = torch.searchsorted(pf, tgt, right=True) - 1
+    ps = torch.searchsorted(pf, src, right=True) - 1
+    x = tgt | ((c.d.to(dev).to(torch.int64) - 1) << 17) | (pt << 24)
+    scale = alpha[pt, ps]
+    if bool((scale == 0).any()):
+        raise ValueError("a weight scale of a connected population pair is 0: its base weights are unknown")
+    base = c.w.to(dev) / scale
+    links = torch.stack([x.to(torch.int32), base.contiguous().view(torch.int32)], dim=1).contiguous()
+    return c.src_ptr.to(dev).to(torch.int64).contiguous(), links
+
+
+class RateLossGrad:
+    def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
+                 cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None):
+        if not runtimes or len(runtimes) > 16:
+            raise ValueError("RateLossGrad: 1 to 16 networks")
+        for rt in runtimes:
+            if rt.model != MODEL_LIF_EXP:
+                raise ValueError("RateLossGrad: exponential-synapse LIF networks only")
+            if len(rt.pops) > 8:
+                raise ValueError("RateLossGrad: at most 8 populations")
+            if not rt.net.record:
+                raise ValueError("RateLossGrad: the networks must record their spikes (Network.record)")
+            if rt.n > (1 << 17):
+                raise ValueError("RateLossGrad: at most 2^17 neurons a network")
+        if len({rt.step for rt in runtimes}) != 1:
+            raise ValueError("RateLossGrad: the networks are at different steps")
+        if checkpoint < 1:
+            raise ValueError("checkpoint must be positive")
+        self.runtimes = list(runtimes)
+        self.dev = runtimes[0].device
+        P = len(runtimes[0].pops)
+        if any(len(rt.pops) != P for rt in runtimes):
+            raise ValueError("RateLossGrad: every network needs the same populations")
+        self.P = P
+        self.alpha = torch.as_tensor(np.asarray(alpha, dtype=np.float32), device=self.dev).reshape(P, P).contiguous()
+        self.checkpoint, self.window, self.tuning = int(checkpoint), int(graph_window), tuning
+        self.sur = Surrogate(float(kappa), float(cutoff), float(gamma), 0.0)
+        self.tables = [out_links(rt, self.alpha) for rt in runtimes]
+        self.pop_first = [torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int32), device=self.dev) for rt in runtimes]
+        self.ring_slots = max(rt.d_slots for rt in runtimes) + 1
+        self.last = {}
+
+    def _counts(self, rt, s0: int, T: int) -> np.ndarray:
+        """Spikes per population over steps [s0, s0 + T)."""
+        gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
+        per_n = torch.zeros(rt.n, dtype=torch.int64, device=self.dev)
+        for a in range(s0, s0 + T, 500):
+            b = min(s0 + T, a + 500)
+            per_n += ((rt.raster_bits(a, b)[:, gof] >> lof) & 1).sum(dim=0)
+        pf = np.asarray(rt.pop_first, dtype=np.int64)
+        cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=self.dev), per_n.cumsum(0)]).cpu().numpy()
+        return (cs[pf[1:]] - cs[pf[:-1]]).astype(np.float64)
+
+    def __call__(self, n_steps: int, targets):
+        rts, K, T = self.runtimes, self.checkpoint, int(n_steps)
+        if T < 1:
+            raise ValueError("n_steps must be positive")
+        targets = np.asarray(targets, dtype=np.float64).reshape(len(rts), self.P)
+        lib = _bind(_lib.load())
+        s0 = rts[0].step
+        rec_from = [rt.record_from for rt in rts]
+        # ---- forward, checkpointed ----
+        t_fw = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_fw[0].record()
+        fr = FusedRound(rts, self.tuning, reserve_steps=T + 1)
+        ckpts = []
+        for k0 in range(0, T, K):
+            ckpts.append([rt.snapshot() for rt in rts])
+            fr.launch(min(K, T - k0), self.window)
+        fr.close()
+        t_fw[1].record()
+        final = [rt.snapshot() for rt in rts]
+        dt_s = rts[0].dt * 1e-3
+        loss, c_pops = 0.0, []
+        for i, rt in enumerate(rts):
+            sizes = np.diff(np.asarray(rt.pop_first, dtype=np.int64)).astype(np.float64)
+            rates = self._counts(rt, s0, T) / (sizes * T * dt_s)
+            loss += float(((rates - targets[i]) ** 2).sum())
+            c_pops.append(2.0 * (rates - targets[i]) / (sizes * T * dt_s))
+        # ---- backward ----
+        t_bw = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_bw[0].record()
+        R = self.ring_slots
+        st = []
+        for i, rt in enumerate(rts):
+            coef = np.zeros((self.P, 8), dtype=np.float32)
+            for p, prm in enumerate(rt.params):
+                coef[p] = [prm.decay, prm.p_i, prm.p_vi, prm.v_reset, prm.v_th, prm.v_th - prm.v_reset, c_pops[i][p], 0.0]
+            st.append({
+                "coef": torch.as_tensor(coef, device=self.dev),
+                "gx": torch.zeros(R, rt.n, device=self.dev), "gv": torch.zeros(rt.n, device=self.dev),
+                "gi": torch.zeros(rt.n, device=self.dev),
+                "gsum": torch.zeros(rt.n, self.P, dtype=torch.float64, device=self.dev),
+                "vtr": torch.empty(min(K, T), rt.n, device=self.dev),
+                "spk": torch.empty(min(K, T), rt.n, dtype=torch.uint8, device=self.dev),
+            })
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        nets = (AdjNet * len(rts))()
+        for k in reversed(range(len(ckpts))):
+            start = s0 + k * K
+            L = min(K, T - k * K)
+            for rt, ck, sd in zip(rts, ckpts[k], st):
+                rt.restore(ck)
+                _lib.check(lib.n2h_set_vtrace(rt._plan, sd["vtr"].data_ptr(), start, L), "n2h_set_vtrace")
+            fr = FusedRound(rts, self.tuning, reserve_steps=L)
+            fr.launch(L, 0)
+            fr.close()
+            for rt, sd in zip(rts, st):
+                _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0), "n2h_set_vtrace")
+                gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
+                sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
+            for t in reversed(range(L)):
+                for i, (rt, sd, (optr, ol)) in enumerate(zip(rts, st, self.tables)):
+                    nets[i] = AdjNet(rt.n, self.P, R, 0, self.pop_first[i].data_ptr(), optr.data_ptr(), ol.data_ptr(),
+                                     self.alpha.data_ptr(), sd["coef"].data_ptr(), sd["vtr"][t].data_ptr(),
+                                     sd["spk"][t].data_ptr(), sd["gx"].data_ptr(), sd["gv"].data_ptr(),
+                                     sd["gi"].data_ptr(), sd["gsum"].data_ptr())
+                _check(lib.n2h_adj_step(nets, len(rts), start + t, ctypes.byref(self.sur), stream), "n2h_adj_step")
+        t_bw[1].record()
+        for rt, sn, rf in zip(rts, final, rec_from):
+            rt.restore(sn)
+            rt.record_from = rf  # the history still holds every step since (the final snapshot's bits)
+        g = torch.zeros(self.P, self.P, dtype=torch.float64, device=self.dev)
+        for rt, sd in zip(rts, st):
+            pf = np.asarray(rt.pop_first, dtype=np.int64)
+            cs = torch.cat([torch.zeros(1, self.P, dtype=torch.float64, device=self.dev), sd["gsum"].cumsum(0)])
+            for b in range(self.P):
+                g[:, b] += cs[pf[b + 1]] - cs[pf[b]]
+        torch.cuda.synchronize(self.dev)
+        self.last = {"forward_ms": t_fw[0].elapsed_time(t_fw[1]), "backward_ms": t_bw[0].elapsed_time(t_bw[1]),
+                     "c_pops": c_pops}
+        return loss, g.cpu().numpy()

@@ -102,6 +102,10 @@ struct StepArgs {
     int32_t row_uniform;        // every link's delay cell (delay - 1) when all delays are equal, else -1
     int32_t dense_min;          // row intake for steps following at least this many spikes
     int32_t acc_base;           // the delay cell held at acc[0]: row_uniform for uniform delays (one cell), else 0
+    float* vtrace;              // null, or [vtrace_steps][n_neurons]: the potential before the threshold of
+    int64_t vtrace_step0;       //   steps vtrace_step0 .. + vtrace_steps - 1 (+inf on refractory steps)
+    int32_t vtrace_steps;
+    int32_t pad4;
 };
 
 // Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
@@ -623,6 +627,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     __syncthreads();
     const n2h_pop_params& P = *reinterpret_cast<const n2h_pop_params*>(st.param_words);
     bool fired = false;
+    float vpre = __builtin_inff();  // the potential the threshold sees (+inf: refractory)
     if (live) {
         const uint32_t u = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
                                         static_cast<uint32_t>(static_cast<uint64_t>(s) >> 32), a.instance,
@@ -637,6 +642,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             } else {
                 const float vd = V * P.decay;
                 V = vd + in;
+                vpre = V;
                 if (V >= P.v_th) {
                     fired = true;
                     V = P.v_reset;
@@ -654,6 +660,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 const float vr = P.v_rest + pv;
                 const float pvi = P.p_vi * I;
                 V = vr + pvi;
+                vpre = V;
             }
             I = P.p_i * I;
             if (V > P.v_th) {
@@ -688,6 +695,10 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         a.ref[nid] = R;
         if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
         a.ring[cur] = 0.0f;
+        if (a.vtrace) {
+            const int64_t q = s - a.vtrace_step0;
+            if (q >= 0 && q < a.vtrace_steps) a.vtrace[q * N + nid] = vpre;
+        }
         // ring: add each touched cell's partial sum (arrival s - 1 + (b + 1), b < D); the cell's one add
         unsigned long long m = t0 & ~1ull;
         while (m) {
@@ -786,6 +797,9 @@ struct n2h_plan {
     const float* row_wtab;
     int32_t row_format;
     int32_t dense_min;
+    float* vtrace;
+    int64_t vtrace_step0;
+    int32_t vtrace_steps;
     size_t lds_bytes;
     int device;
 };
@@ -855,6 +869,9 @@ StepArgs make_args(const n2h_plan* p) {
     a.row_uniform = p->tables.uniform_delay - 1;
     a.acc_base = p->tables.uniform_delay > 0 ? p->tables.uniform_delay - 1 : 0;
     a.dense_min = p->dense_min;
+    a.vtrace = p->vtrace;
+    a.vtrace_step0 = p->vtrace_step0;
+    a.vtrace_steps = p->vtrace_steps;
     return a;
 }
 
@@ -1293,6 +1310,15 @@ int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, co
     p->row_format = format;
     p->row_wtab = wtab;
     p->dense_min = dense_min_spikes;
+    return N2H_OK;
+}
+
+int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (buffer && n_steps <= 0) return fail(N2H_EINVAL, "n_steps must be positive");
+    p->vtrace = buffer;
+    p->vtrace_step0 = step0;
+    p->vtrace_steps = buffer ? n_steps : 0;
     return N2H_OK;
 }
 

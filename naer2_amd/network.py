@@ -396,23 +396,43 @@ class Runtime:
         out["first entry -> last exit"] = float((t[:, 7].max() - t[:, 0].min()) / 100.0)
         return out
 
+    def trace_potentials(self, buffer: torch.Tensor | None, step0: int = 0) -> None:
+        """Write each neuron's potential as the threshold sees it (+inf on refractory steps) for steps
+        step0 .. step0 + len(buffer) - 1 into buffer [steps, n] (float32, on the device); None: off.  Results
+        are unchanged; captured windows are dropped so that the next run sees the setting."""
+        lib = _lib.load()
+        if buffer is None:
+            _lib.check(lib.n2h_set_vtrace(self._plan, None, 0, 0), "n2h_set_vtrace")
+        else:
+            if buffer.dtype != torch.float32 or buffer.dim() != 2 or buffer.shape[1] != self.n or not buffer.is_contiguous():
+                raise ValueError("buffer: contiguous float32 [steps, n_neurons]")
+            _lib.check(lib.n2h_set_vtrace(self._plan, buffer.data_ptr(), int(step0), int(buffer.shape[0])),
+                       "n2h_set_vtrace")
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
+
     def snapshot(self) -> tuple:
         """A device copy of the dynamic state (step, V, I, refractory counters, ring, spike bits and lists)."""
         return (self.step, self.v.clone(), None if self.isyn is None else self.isyn.clone(), self.ref.clone(),
                 self.ring.clone(), self.bits.clone(), self.spike_list.clone())
 
     def restore(self, snap: tuple) -> None:
-        """Back to a snapshot() of this runtime; recording restarts at its step.  The bit history must not
-        have been resized since (reserve it first)."""
+        """Back to a snapshot() of this runtime; recording restarts at its step.  A bit history grown since
+        is replaced by the snapshot's (not while a FusedRound holds this runtime: it keeps the addresses)."""
         step, v, isyn, ref, ring, bits, lst = snap
-        if bits.shape != self.bits.shape:
-            raise ValueError("the spike-bit history was resized since the snapshot")
         self.v.copy_(v)
         if isyn is not None:
             self.isyn.copy_(isyn)
         self.ref.copy_(ref)
         self.ring.copy_(ring)
-        self.bits.copy_(bits)
+        if bits.shape == self.bits.shape:
+            self.bits.copy_(bits)
+        else:
+            if getattr(self, "_batch_capacity", None) is not None:
+                raise RuntimeError("restore: the bit history moved under a FusedRound")
+            self.bits, self.bit_slots = bits.clone(), int(bits.shape[0])
+            self._bind_ring()
         self.spike_list.copy_(lst)
         self.set_step(step)
 

@@ -67,12 +67,16 @@ def brunel(N_E: int = 2000, N_I: int = 500, C_E: int = 200, C_I: int = 50, J: fl
 
 
 def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instance: int = 0,
-         k_ext_scale: float = 1.0):
-    """(network, {name: population}); scale shrinks every population (connectivity follows the rule)."""
+         k_ext_scale: float = 1.0, weight_scale=None, sizes=None, prob_scale: float = 1.0):
+    """(network, {name: population}); scale shrinks every population (connectivity follows the rule).
+    weight_scale: [8][8] factors alpha[target][source] on the connections' weights (default all 1);
+    sizes: explicit population sizes instead of scale; prob_scale multiplies the connection probabilities
+    (capped at 0.9), for small subsampled circuits."""
     pops = {}
     net = None
-    for name, k_ext, size in zip(PD14["name"], PD14["K_ext"], PD14["size"]):
-        n = max(1, int(round(size * scale)))
+    size_list = sizes if sizes is not None else [max(1, int(round(sz * scale))) for sz in PD14["size"]]
+    for name, k_ext, n in zip(PD14["name"], PD14["K_ext"], size_list):
+        n = int(n)
         pops[name] = LIFExp(K_ext=int(round(k_ext * k_ext_scale))) * n
         if net is None:
             net = pops[name].network
@@ -86,7 +90,7 @@ def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instan
         p.potential = vm + vs * torch.randn(p.n, generator=gen, device=dev)
     for si, src in enumerate(PD14["name"]):
         for ti, tgt in enumerate(PD14["name"]):
-            prob = PD14["probabilities"][si][ti]
+            prob = min(0.9, PD14["probabilities"][si][ti] * prob_scale)
             if prob == 0.0:
                 continue
             w_mean = 2 * _J_E if (src == "E_4" and tgt == "E_23") else PD14["weight_mean"][si]
@@ -95,7 +99,10 @@ def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instan
             d_mean = PD14["delay_mean"][si]
             conn.delay = (d_mean + 0.5 * d_mean * torch.randn(n_links, generator=gen, device=dev)).clamp_(min=0.1)
             w = w_mean + 0.1 * abs(w_mean) * torch.randn(n_links, generator=gen, device=dev)
-            conn.weight = w.clamp_(min=0.0) if w_mean >= 0 else w.clamp_(max=0.0)
+            w = w.clamp_(min=0.0) if w_mean >= 0 else w.clamp_(max=0.0)
+            if weight_scale is not None:
+                w = w * float(weight_scale[ti][si])
+            conn.weight = w
     return net, pops
 
 

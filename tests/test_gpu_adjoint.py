@@ -1,0 +1,108 @@
+"""The rate-loss adjoint on the GPU (naer2_amd/adjoint.py, n2h_adj_step) against the FP64 autograd unroll
+of oracle/adjoint_oracle.py on the same spike trains (the fp32 run's, forced), batched over a
+4-neuron-per-population PD14 subsample and a 2 % PD14; checkpoint-interval and batching invariance."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from naer2_amd.adjoint import RateLossGrad
+from naer2_amd.networks import pd14
+from oracle import adjoint_oracle as ao
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _nets(alpha):
+    a, _ = pd14(sizes=[4] * 8, prob_scale=10.0, seed=3, device="cuda", weight_scale=alpha, instance=0)
+    b, _ = pd14(scale=0.02, seed=1, device="cuda", weight_scale=alpha, instance=1)
+    for n in (a, b):
+        n.record = True
+        n.keep_canonical = True
+    return [a, b]
+
+
+def _alpha(seed=2):
+    rng = np.random.default_rng(seed)
+    return (1.0 + 0.1 * rng.standard_normal((8, 8))).astype(np.float32)
+
+
+@pytest.mark.parametrize("sur", [{}, {"cutoff": 0.5, "gamma": 0.5}])
+def test_adjoint_matches_fp64_autograd_on_the_same_spikes(sur):
+    T = 200
+    alpha = _alpha()
+    nets = _nets(alpha)
+    rts = [n.build() for n in nets]
+    _say(f"case: adjoint vs fp64 autograd, networks of {[rt.n for rt in rts]} neurons, T={T}, checkpoint 64, "
+         f"surrogate {sur}")
+    init = [(rt.v.cpu().numpy().copy(), rt.isyn.cpu().numpy().copy()) for rt in rts]
+    # the fp32 trajectory the adjoint differentiates: its potentials before the threshold, traced
+    traces = []
+    for rt in rts:
+        sn = rt.snapshot()
+        buf = torch.empty(T, rt.n, device="cuda")
+        rt.trace_potentials(buf, 0)
+        rt.run(T, 0)
+        rt.trace_potentials(None)
+        torch.cuda.synchronize()
+        traces.append(buf.cpu().numpy().astype(np.float64))
+        rt.restore(sn)
+    targets = np.random.default_rng(9).uniform(0.5, 10.0, (2, 8))
+    grad = RateLossGrad(rts, alpha, checkpoint=64, **sur)
+    loss, g = grad(T, targets)
+    a64 = torch.as_tensor(alpha.astype(np.float64), dtype=torch.float64).requires_grad_(True)
+    total = 0.0
+    g_rec = np.zeros((8, 8))
+    for i, rt in enumerate(rts):
+        d = ao.net_from(rt.pop_first, rt.params, rt.canon.to("cpu"), init[i][0], init[i][1], rt.dt)
+        pop = d["pop_of"]
+        d["w"] = d["w"] / alpha.astype(np.float64)[pop[d["tgt"]], pop[d["src"]]]  # base weights
+        counts = ao.poisson_counts(d["thr"], seed=rt.net.seed, instance=rt.net.instance, step0=0, n_steps=T)
+        forced = rt.raster(0, T)
+        assert forced.sum() > 0
+        _, rates, _, _ = ao.unroll(d, a64, T, counts, mode="spike", forced=forced, **sur)
+        total = total + ao.loss(rates, targets[i])
+        # the kernel's recursion in float64 along the fp32 trajectory (its traced potentials)
+        vp = traces[i]
+        rg = np.isfinite(vp).astype(np.float64)
+        vp = np.where(rg > 0, vp, 0.0)
+        c = ao.loss_grad_per_spike(d, rates.detach().numpy(), targets[i], T)
+        g_rec += ao.adjoint(d, alpha.astype(np.float64), forced.astype(np.float64), vp, rg, c, **sur)
+    total.backward()
+    ref = a64.grad.numpy()
+    assert abs(loss - float(total.detach())) <= 1e-9 * max(1.0, abs(float(total.detach())))
+    err_rec = np.linalg.norm(g - g_rec) / np.linalg.norm(g_rec)
+    err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    _say(f"loss {loss:.6g}, |grad| {np.linalg.norm(ref):.4g}; rel err vs the fp64 recursion on the fp32 "
+         f"trajectory {err_rec:.3g}, vs the fp64 autograd unroll {err:.3g}")
+    assert err_rec <= 1e-5   # the kernel's arithmetic
+    assert err <= 3e-4       # plus fp32 against fp64 potentials in the surrogate
+    assert np.abs(ref).max() > 0
+
+
+def test_gradient_is_independent_of_checkpoints_and_batching():
+    T = 160
+    alpha = _alpha(4)
+    targets = np.random.default_rng(1).uniform(0.5, 10.0, (2, 8))
+    _say(f"case: checkpoint 32 vs 160, batched vs alone, T={T}")
+    res = []
+    for K in (32, 160):
+        rts = [n.build() for n in _nets(alpha)]
+        res.append(RateLossGrad(rts, alpha, checkpoint=K)(T, targets))
+    assert res[0][0] == res[1][0] and np.array_equal(res[0][1], res[1][1])
+    alone = []
+    for i in range(2):
+        rt = _nets(alpha)[i].build()
+        alone.append(RateLossGrad([rt], alpha, checkpoint=64)(T, targets[i:i + 1]))
+    g_alone = np.zeros((8, 8))
+    for _, g in alone:
+        g_alone += g
+    assert np.array_equal(g_alone, res[0][1])
