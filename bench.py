@@ -54,20 +54,80 @@ def _events_and_spikes(rt, step_lo: int, step_hi: int, out_deg: torch.Tensor) ->
     return ev, sp
 
 
-def cpu_baseline(rt, state_snapshot, step0: int, n_steps: int) -> dict:
-    """The CPU oracle (single thread, this host) on n_steps of the same workload from the same state.
-    Returns its rate and whether its raster equals the GPU's for those steps."""
-    from oracle import oracle
+def _lambdas(rt) -> list:
+    """Poisson mean of each population's background per step."""
+    out = []
+    for p in rt.pops:
+        inst = p.instance
+        out.append(inst.rate_ext * rt.dt if hasattr(inst, "rate_ext") else inst.K_ext * inst.nu_ext * rt.dt)
+    return out
+
+
+def cpu_baselines(net, rt, args, kind: str) -> dict:
+    """CPU baselines on the host's cores, from the GPU's state after the timed run:
+    the C oracle on one thread and on `threads` (OpenMP over target ranges; both checked bit for bit
+    against the GPU's next steps), and the reference's own algorithm (tools/brunel_reference.py's
+    plain-torch loop, oracle/cpu_reference.py) on `threads`."""
     from naer2_amd.params import params_bytes
+    from oracle import cpu_reference, oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    model = oracle.LIF_EXP if kind == "pd14" else oracle.LIF_DELTA
     c = rt.canon.to("cpu")
-    v, isyn, ref, ring, prev = state_snapshot
-    st = oracle.State(rt.n, rt.d_slots, v, isyn, ref, ring, prev)
+    prev = rt.raster(rt.step - 1, rt.step)[0].astype(np.uint8)
+    snap = (rt.v.cpu().numpy(), None if rt.isyn is None else rt.isyn.cpu().numpy(), rt.ref.cpu().numpy(),
+            rt.ring.cpu().numpy(), prev)
+    # bounded samples of a few seconds of CPU work each (scaled by --cpu-steps, 2000 by default)
+    f = args.cpu_steps / 2000.0
+    c0 = rt.step
+    n1 = int(2000 * f) if kind == "pd14" else int(50000 * f)        # one thread
+    n_mt = int(20000 * f) if kind == "pd14" else int(100000 * f)    # every core
+    net.run(max(n1, n_mt), graph_window=0)
+    torch.cuda.synchronize()
+    gpu_raster = rt.raster(c0, c0 + max(n1, n_mt))
+    gpu_v = rt.v.cpu().numpy()
+    pb = params_bytes(rt.params)
+    arrays = (c.src_ptr.numpy(), c.tgt.numpy(), c.w.numpy(), c.d.numpy())
+    bio_step = rt.dt / 1000.0
+
+    def state():
+        v, i, r, ring, pv = snap
+        return oracle.State(rt.n, rt.d_slots, v.copy(), None if i is None else i.copy(), r.copy(), ring.copy(),
+                            pv.copy())
+
+    st = state()
     t0 = time.perf_counter()
-    raster, events = oracle.run(oracle.LIF_EXP, rt.pop_first, params_bytes(rt.params), rt.d_slots, rt.net.seed,
-                                rt.net.instance, c.src_ptr.numpy(), c.tgt.numpy(), c.w.numpy(), c.d.numpy(), st,
-                                step0, n_steps, True)
-    wall = time.perf_counter() - t0
-    return {"wall_s": wall, "events": events, "raster": raster, "v": st.v}
+    r1, ev1 = oracle.run(model, rt.pop_first, pb, rt.d_slots, rt.net.seed, rt.net.instance, *arrays, st, c0, n1, True)
+    w1 = time.perf_counter() - t0
+    same1 = bool(np.array_equal(gpu_raster[:n1], r1))
+    par = oracle.Parallel(rt.n, *arrays, threads=threads)
+    st = state()
+    t0 = time.perf_counter()
+    rm, evm = par.run(model, rt.pop_first, pb, rt.d_slots, rt.net.seed, rt.net.instance, st, c0, n_mt, True)
+    wm = time.perf_counter() - t0
+    par.close()
+    samem = bool(np.array_equal(gpu_raster[:n_mt], rm)) and (
+        n_mt != max(n1, n_mt) or bool(np.array_equal(gpu_v.view(np.uint32), st.v.view(np.uint32))))
+    n_ref = int(100 * f) if kind == "pd14" else int(20000 * f)       # the reference's torch loop
+    v, i, _, _, _ = snap
+    ref = cpu_reference.run(model, rt.pop_first, rt.params, _lambdas(rt), c, rt.d_slots, v, i if i is not None else v,
+                            n_ref, threads)
+    print(f"[bench] cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1}); oracle {threads} threads "
+          f"{n_mt} steps {wm:.1f} s (bit-exact {samem}); reference algorithm (torch) {threads} threads {n_ref} steps "
+          f"{ref['seconds']:.1f} s", file=sys.stderr, flush=True)
+    name = "PD14" if kind == "pd14" else "Brunel"
+    return {
+        "value": round(n_mt * bio_step / wm, 6), "unit": "bio_s/wall_s", "cores": threads, "kind": "port",
+        "sample": f"{n_mt} steps ({n_mt * bio_step * 1000:.0f} ms bio) of the same {name} network from the GPU state "
+                  f"after step {c0}: the C oracle (-O2) on {threads} host threads (OpenMP, target ranges); "
+                  f"raster bit-exact vs GPU: {samem}",
+        "events_per_s": round(evm / wm, 1), "bit_exact_vs_gpu": samem and same1,
+        "single_thread": {"value": round(n1 * bio_step / w1, 6), "cores": 1, "steps": n1,
+                          "events_per_s": round(ev1 / w1, 1), "bit_exact_vs_gpu": same1},
+        "reference_algorithm": {"value": round(n_ref * bio_step / ref["seconds"], 6), "cores": threads,
+                                "steps": n_ref, "events_per_s": round(ref["events"] / ref["seconds"], 1),
+                                "what": "tools/brunel_reference.py's plain-torch time-driven loop (a sparse product "
+                                        "per delay class and step), restated for this network on the CPU"},
+    }
 
 
 def bench_sweep(args) -> int:
@@ -222,6 +282,103 @@ def bench_pd14grad(args) -> int:
     return 0
 
 
+def bench_plain(args) -> int:
+    """The plain path: full-scale PD14 (config 1, the headline) or Brunel config 0.  Warmup, then exactly
+    --steps steps in graph windows, timed with HIP events on the launch stream; the CPU baselines on the
+    same network from the GPU's state."""
+    from naer2_amd.networks import brunel, pd14
+
+    kind = args.workload
+    t_build = time.perf_counter()
+    if kind == "pd14":
+        net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
+    else:
+        net, pops = brunel(seed=args.seed, device="cuda")
+    rt = net.build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    print(f"[bench] built {kind}: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, "
+          f"{rt.d_slots} ring slots in {t_build:.1f} s", file=sys.stderr, flush=True)
+    net.record = True
+    rt._ensure_slots(args.warmup + args.steps + 400)
+
+    net.run(args.warmup, graph_window=args.window)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    s0 = rt.step
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    net.run(args.steps, graph_window=args.window)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    s1 = rt.step
+    print(f"[bench] timed {args.steps} steps: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream",
+          file=sys.stderr, flush=True)
+
+    out_deg = rt.canon.out_degree()
+    events, spikes_timed = _events_and_spikes(rt, s0 - 1, s1 - 1, out_deg)
+    bio_s = args.steps * rt.dt / 1000.0
+    rtf = bio_s / wall
+    ms_per_step = wall * 1000.0 / args.steps
+
+    # roofline of the step kernel: algorithmic bytes per launch (N (8 n_state + 8) + 8 E + 8 S; n_state 3 for
+    # the exponential synapse: V, I, refractory counter; 2 for the delta synapse)
+    n_state = 3 if kind == "pd14" else 2
+    bytes_per_step = rt.n * (8 * n_state + 8) + 8 * events / args.steps + 8 * spikes_timed / args.steps
+    k_ms = gpu_ms / args.steps  # HIP events on the launch stream over the timed region, per launch
+    achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
+    traffic, occupancy = None, None
+    tfile = ROOT / "profiles" / "pd14_step_traffic.json"
+    if kind == "pd14" and tfile.exists():
+        prof = json.loads(tfile.read_text())
+        traffic, occupancy = prof.get("hbm_bytes_per_launch"), prof.get("occupancy")
+
+    rates = net.rates(t_skip_ms=(s0 * rt.dt))
+    rates_hz = {name: round(rates[p], 3) for name, p in pops.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baselines(net, rt, args, kind)
+    line = {
+        "metric": "PD14 real-time factor + synaptic events/s" if kind == "pd14" else
+                  "Brunel real-time factor + synaptic events/s",
+        "value": round(rtf, 4),
+        "unit": "bio_s/wall_s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": ("pd14 full scale (config 1): 77169 LIF-exp neurons, 8 populations, "
+                                f"{rt.canon.n_links} synapses, Poisson background" if kind == "pd14" else
+                                "brunel (config 0): N_E 2000, N_I 500 delta-synapse LIF, C_E 200, C_I 50, J 0.2 mV, "
+                                f"g 5, D 1.5 ms, {rt.canon.n_links} synapses, Poisson drive (eta 2)") +
+                               f", dt 0.1 ms, {args.steps} steps timed, seed {args.seed}",
+                   "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
+                   "graph_window": args.window, "parallelism": "single GPU"},
+        "events_per_s": round(events / wall, 1),
+        "spikes_per_step": round(spikes_timed / args.steps, 3),
+        "events_per_step": round(events / args.steps, 1),
+        "rates_hz": rates_hz,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel_avg_us": round(k_ms * 1000.0, 3), "occupancy": occupancy,
+                     "algorithmic_bytes_per_launch": round(bytes_per_step, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_bptt(args) -> int:
     """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
     1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
@@ -279,12 +436,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--window", type=int, default=100, help="steps per captured hipGraph")
-    ap.add_argument("--cpu-steps", type=int, default=2000, help="steps of the CPU oracle sample")
+    ap.add_argument("--cpu-steps", type=int, default=2000,
+                    help="scales the CPU samples (PD14: 2000 steps on one thread, 10x that on every core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["pd14", "sweep", "bptt", "pd14grad"], default="pd14",
-                    help="pd14: the headline PD14 real-time factor; sweep: the batched Brunel sweep; bptt: the "
-                         "recurrent-LIF BPTT; pd14grad: the batched differentiable PD14")
+    ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad"], default="pd14",
+                    help="pd14: the headline PD14 real-time factor (config 1); brunel: config 0; sweep: the batched "
+                         "Brunel sweep; bptt: the recurrent-LIF BPTT; pd14grad: the batched differentiable PD14")
     ap.add_argument("--instances", type=int, default=None, help="networks (sweep: 256, pd14grad: 4)")
     ap.add_argument("--checkpoint", type=int, default=500, help="pd14grad: steps between saved states")
     ap.add_argument("--cutoff", type=float, default=0.2,
@@ -320,105 +478,7 @@ def main() -> int:
     if args.workload == "sweep":
         return bench_sweep(args)
 
-    from naer2_amd.networks import pd14
-
-    t_build = time.perf_counter()
-    net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
-    rt = net.build()
-    torch.cuda.synchronize()
-    t_build = time.perf_counter() - t_build
-    print(f"[bench] built PD14: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, "
-          f"{rt.d_slots} ring slots in {t_build:.1f} s", file=sys.stderr, flush=True)
-    net.record = True
-    rt._ensure_slots(args.warmup + args.steps + 400)
-
-    net.run(args.warmup, graph_window=args.window)
-    torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-
-    s0 = rt.step
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    net.run(args.steps, graph_window=args.window)
-    end.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(end)
-    s1 = rt.step
-    print(f"[bench] timed {args.steps} steps: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream",
-          file=sys.stderr, flush=True)
-
-    out_deg = rt.canon.out_degree()
-    events, spikes_timed = _events_and_spikes(rt, s0 - 1, s1 - 1, out_deg)
-    bio_s = args.steps * rt.dt / 1000.0
-    rtf = bio_s / wall
-    ms_per_step = wall * 1000.0 / args.steps
-
-    # roofline of the step kernel: algorithmic bytes per launch (N (8 n_state + 8) + 8 E + 8 S, n_state = 3)
-    n_state = 3
-    bytes_per_step = rt.n * (8 * n_state + 8) + 8 * events / args.steps + 8 * spikes_timed / args.steps
-    k_ms = gpu_ms / args.steps  # HIP events on the launch stream over the timed region, per launch
-    achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / "pd14_step_traffic.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-
-    rates = net.rates(t_skip_ms=(s0 * rt.dt))
-    rates_hz = {name: round(rates[p], 3) for name, p in pops.items()}
-
-    cpu = None
-    if not args.no_cpu:
-        snap = (rt.v.cpu().numpy(), rt.isyn.cpu().numpy(), rt.ref.cpu().numpy(), rt.ring.cpu().numpy(),
-                rt.raster(rt.step - 1, rt.step)[0].astype(np.uint8))
-        c0 = rt.step
-        net.run(args.cpu_steps, graph_window=0)
-        torch.cuda.synchronize()
-        gpu_raster = rt.raster(c0, c0 + args.cpu_steps)
-        res = cpu_baseline(rt, snap, c0, args.cpu_steps)
-        same = bool(np.array_equal(gpu_raster, res["raster"])) and bool(
-            np.array_equal(rt.v.cpu().numpy().view(np.uint32), res["v"].view(np.uint32)))
-        cpu_bio = args.cpu_steps * rt.dt / 1000.0
-        cpu = {"value": round(cpu_bio / res["wall_s"], 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps ({cpu_bio * 1000:.0f} ms bio) of the same PD14 network from the "
-                         f"GPU state after step {c0}; single-thread C oracle (-O2); "
-                         f"{res['events'] / res['wall_s']:.3e} events/s; raster and V bit-exact vs GPU: {same}",
-               "events_per_s": round(res["events"] / res["wall_s"], 1), "bit_exact_vs_gpu": same}
-        print(f"[bench] cpu oracle {args.cpu_steps} steps: {res['wall_s']:.1f} s, bit-exact {same}",
-              file=sys.stderr, flush=True)
-
-    line = {
-        "metric": "PD14 real-time factor + synaptic events/s",
-        "value": round(rtf, 4),
-        "unit": "bio_s/wall_s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 6),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "pd14 full scale (config 1): 77169 LIF-exp neurons, 8 populations, "
-                               f"{rt.canon.n_links} synapses, Poisson background, dt 0.1 ms, "
-                               f"{args.steps} steps timed, seed {args.seed}",
-                   "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
-                   "graph_window": args.window, "parallelism": "single GPU"},
-        "events_per_s": round(events / wall, 1),
-        "spikes_per_step": round(spikes_timed / args.steps, 3),
-        "events_per_step": round(events / args.steps, 1),
-        "rates_hz": rates_hz,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "kernel_avg_us": round(k_ms * 1000.0, 3),
-                     "algorithmic_bytes_per_launch": round(bytes_per_step, 1)},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    return 0
+    return bench_plain(args)
 
 
 if __name__ == "__main__":

@@ -5,6 +5,8 @@
  */
 #include "naer2_oracle.h"
 
+#include <omp.h>
+
 #include <stdlib.h>
 #include <string.h>
 
@@ -141,5 +143,211 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
 done:
     if (events) *events = n_ev;
     free(acc); free(touched); free(list); free(pop);
+    return rc;
+}
+
+/* ---- the same step, OpenMP-parallel over target ranges (the CPU baseline at all host cores) ----
+ * Thread t owns the targets [lo_t, hi_t): its own source-major list of the links into them (canonical
+ * order kept per source), its partial sums, its ring cells and its neurons' update.  Every per-target sum
+ * therefore sees the same links in the same order as or_run, and the results are identical. */
+struct or_mt {
+    int32_t n_neurons, n_threads;
+    int64_t* lo;        /* [T + 1] target range bounds */
+    int64_t** ptr;      /* per thread [N + 1] */
+    int32_t** tgt;
+    float** w;
+    int32_t** d;
+};
+
+or_mt* or_mt_create(int32_t n_neurons, const int64_t* src_ptr, const int32_t* tgt, const float* w,
+                    const int32_t* delay, int32_t n_threads) {
+    if (n_neurons < 1 || n_threads < 1) return NULL;
+    if (n_threads > n_neurons) n_threads = n_neurons;
+    or_mt* m = (or_mt*)calloc(1, sizeof(or_mt));
+    if (!m) return NULL;
+    const int64_t N = n_neurons, T = n_threads;
+    m->n_neurons = n_neurons;
+    m->n_threads = n_threads;
+    m->lo = (int64_t*)malloc(sizeof(int64_t) * (size_t)(T + 1));
+    m->ptr = (int64_t**)calloc((size_t)T, sizeof(int64_t*));
+    m->tgt = (int32_t**)calloc((size_t)T, sizeof(int32_t*));
+    m->w = (float**)calloc((size_t)T, sizeof(float*));
+    m->d = (int32_t**)calloc((size_t)T, sizeof(int32_t*));
+    if (!m->lo || !m->ptr || !m->tgt || !m->w || !m->d) {
+        or_mt_destroy(m);
+        return NULL;
+    }
+    for (int64_t t = 0; t <= T; ++t) m->lo[t] = N * t / T;
+    int ok = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(static, 1)
+    for (int64_t t = 0; t < T; ++t) {
+        const int64_t lo = m->lo[t], hi = m->lo[t + 1];
+        int64_t cnt = 0;
+        for (int64_t l = 0; l < src_ptr[N]; ++l) cnt += (tgt[l] >= lo && tgt[l] < hi);
+        int64_t* p = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
+        int32_t* tg = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt + 1));
+        float* ww = (float*)malloc(sizeof(float) * (size_t)(cnt + 1));
+        int32_t* dd = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt + 1));
+        m->ptr[t] = p;
+        m->tgt[t] = tg;
+        m->w[t] = ww;
+        m->d[t] = dd;
+        if (!p || !tg || !ww || !dd) {
+#pragma omp atomic write
+            ok = 0;
+            continue;
+        }
+        int64_t k = 0;
+        for (int64_t s = 0; s < N; ++s) {
+            p[s] = k;
+            for (int64_t l = src_ptr[s]; l < src_ptr[s + 1]; ++l) {
+                if (tgt[l] >= lo && tgt[l] < hi) {
+                    tg[k] = tgt[l];
+                    ww[k] = w[l];
+                    dd[k] = delay[l];
+                    ++k;
+                }
+            }
+        }
+        p[N] = k;
+    }
+    if (!ok) {
+        or_mt_destroy(m);
+        return NULL;
+    }
+    return m;
+}
+
+void or_mt_destroy(or_mt* m) {
+    if (!m) return;
+    for (int32_t t = 0; t < m->n_threads; ++t) {
+        if (m->ptr) free(m->ptr[t]);
+        if (m->tgt) free(m->tgt[t]);
+        if (m->w) free(m->w[t]);
+        if (m->d) free(m->d[t]);
+    }
+    free(m->ptr);
+    free(m->tgt);
+    free(m->w);
+    free(m->d);
+    free(m->lo);
+    free(m);
+}
+
+int or_mt_run(or_mt* m, int model, int32_t n_pops, const int32_t* pop_first, const or_pop* pops, int32_t d_slots,
+              uint64_t seed, uint32_t instance, float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+              int64_t step0, int32_t n_steps, uint8_t* raster, int64_t* events) {
+    if (!m || n_pops < 1 || d_slots < 1 || n_steps < 0) return -1;
+    if (model != OR_LIF_DELTA && model != OR_LIF_EXP) return -1;
+    if (model == OR_LIF_EXP && !isyn) return -1;
+    const int64_t N = m->n_neurons, D = d_slots;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t n_ev = 0;
+    int rc = 0;
+#pragma omp parallel num_threads(m->n_threads) reduction(+ : n_ev)
+    {
+        const int t = omp_get_thread_num();
+        const int64_t lo = m->lo[t], hi = m->lo[t + 1], nt = hi - lo;
+        const int64_t* p = m->ptr[t];
+        const int32_t* tg = m->tgt[t];
+        const float* ww = m->w[t];
+        const int32_t* dd = m->d[t];
+        float* acc = (float*)malloc(sizeof(float) * (size_t)(nt * D + 1));
+        uint8_t* touched = (uint8_t*)calloc((size_t)(nt * D + 1), 1);
+        int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nt * D + 1));
+        int32_t* pop = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nt + 1));
+        int mine_ok = acc && touched && list && pop;
+        if (mine_ok)
+            for (int64_t i = lo; i < hi; ++i) pop[i - lo] = pop_of((int32_t)i, n_pops, pop_first);
+        for (int32_t k = 0; k < n_steps; ++k) {
+            const int64_t s = step0 + k;
+            int64_t n_list = 0;
+            if (mine_ok) {
+                for (int64_t src = 0; src < N; ++src) {
+                    if (!prev_spk[src]) continue;
+                    for (int64_t l = p[src]; l < p[src + 1]; ++l) {
+                        const int32_t d = dd[l];
+                        if (d < 1 || d > d_slots) {
+                            mine_ok = 0;
+                            break;
+                        }
+                        const int64_t cell = ((s - 1 + d) % D) * nt + (tg[l] - lo);
+                        if (!touched[cell]) {
+                            touched[cell] = 1;
+                            list[n_list++] = cell;
+                            acc[cell] = 0.0f + ww[l];
+                        } else {
+                            acc[cell] = acc[cell] + ww[l];
+                        }
+                        ++n_ev;
+                    }
+                }
+                for (int64_t j = 0; j < n_list; ++j) {
+                    const int64_t cell = list[j];
+                    const int64_t slot = cell / nt, i = lo + cell % nt;
+                    ring[slot * N + i] = ring[slot * N + i] + acc[cell];
+                    touched[cell] = 0;
+                }
+            }
+#pragma omp barrier
+            if (mine_ok) {
+                const int64_t slot = (s % D) * N;
+                for (int64_t i = lo; i < hi; ++i) {
+                    const or_pop* P = &pops[pop[i - lo]];
+                    const float x = ring[slot + i];
+                    ring[slot + i] = 0.0f;
+                    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)((uint64_t)s >> 32), instance};
+                    uint32_t r[4];
+                    or_philox4x32_10(ctr, key, r);
+                    const int cnt = or_poisson_count(r[0], P->poisson_thr);
+                    const float in = x + P->j_ext * (float)cnt;
+                    uint8_t fired = 0;
+                    if (model == OR_LIF_DELTA) {
+                        if (ref[i] > 0) {
+                            ref[i] -= 1;
+                        } else {
+                            const float vd = v[i] * P->decay;
+                            v[i] = vd + in;
+                            if (v[i] >= P->v_th) {
+                                fired = 1;
+                                v[i] = P->v_reset;
+                                ref[i] = P->ref_steps;
+                            }
+                        }
+                    } else {
+                        const float I = isyn[i] + in;
+                        if (ref[i] > 0) {
+                            v[i] = P->v_reset;
+                            ref[i] -= 1;
+                        } else {
+                            const float dv = v[i] - P->v_rest;
+                            const float pv = P->decay * dv;
+                            const float vr = P->v_rest + pv;
+                            const float pvi = P->p_vi * I;
+                            v[i] = vr + pvi;
+                        }
+                        isyn[i] = P->p_i * I;
+                        if (v[i] > P->v_th) {
+                            fired = 1;
+                            v[i] = P->v_reset;
+                            ref[i] = P->ref_steps;
+                        }
+                    }
+                    prev_spk[i] = fired;
+                    if (raster) raster[(int64_t)k * N + i] = fired;
+                }
+            }
+#pragma omp barrier
+        }
+        if (!mine_ok) {
+#pragma omp atomic write
+            rc = -1;
+        }
+        free(acc);
+        free(touched);
+        free(list);
+        free(pop);
+    }
+    if (events) *events = n_ev;
     return rc;
 }

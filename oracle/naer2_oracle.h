@@ -56,4 +56,15 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
            float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
            int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events);
 
+
+/* The same step, OpenMP-parallel over contiguous target ranges: identical results (every per-target sum
+ * sees the same links in the same order).  or_mt_create splits the canonical CSR by target range. */
+typedef struct or_mt or_mt;
+or_mt* or_mt_create(int32_t n_neurons, const int64_t* src_ptr, const int32_t* tgt, const float* w,
+                    const int32_t* delay, int32_t n_threads);
+void or_mt_destroy(or_mt* m);
+int or_mt_run(or_mt* m, int model, int32_t n_pops, const int32_t* pop_first, const or_pop* pops, int32_t d_slots,
+              uint64_t seed, uint32_t instance, float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+              int64_t step0, int32_t n_steps, uint8_t* raster, int64_t* events);
+
 #endif

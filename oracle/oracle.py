@@ -41,6 +41,13 @@ def load():
                                ctypes.c_uint32, P, P, P, P, P, P, P, P, P, ctypes.c_int64, ctypes.c_int32, P,
                                ctypes.c_int32, P]
         lib.or_run.restype = ctypes.c_int
+        lib.or_mt_create.argtypes = [ctypes.c_int32, P, P, P, P, ctypes.c_int32]
+        lib.or_mt_create.restype = P
+        lib.or_mt_destroy.argtypes = [P]
+        lib.or_mt_destroy.restype = None
+        lib.or_mt_run.argtypes = [P, ctypes.c_int, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint32,
+                                  P, P, P, P, P, ctypes.c_int64, ctypes.c_int32, P, P]
+        lib.or_mt_run.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -97,6 +104,46 @@ def run(model: int, pop_first, pops_bytes: bytes, d_slots: int, seed: int, insta
     if rc != 0:
         raise RuntimeError(f"or_run failed ({rc})")
     return (raster.astype(bool) if record else None), int(ev.value)
+
+
+class Parallel:
+    """The oracle's step on `threads` host threads (OpenMP, contiguous target ranges): the same results as
+    run(), for timing the CPU baseline at every core.  Setup (splitting the links) is done once here."""
+
+    def __init__(self, n: int, src_ptr, tgt, w, delay, threads: int):
+        lib = load()
+        self._keep = [np.ascontiguousarray(src_ptr, dtype=np.int64), np.ascontiguousarray(tgt, dtype=np.int32),
+                      np.ascontiguousarray(w, dtype=np.float32), np.ascontiguousarray(delay, dtype=np.int32)]
+        self.n, self.threads = int(n), int(threads)
+        self._h = lib.or_mt_create(self.n, *[_ptr(a) for a in self._keep], self.threads)
+        if not self._h:
+            raise RuntimeError("or_mt_create failed")
+        self._keep = None  # the handle holds its own split copies
+
+    def run(self, model: int, pop_first, pops_bytes: bytes, d_slots: int, seed: int, instance: int, state: State,
+            step0: int, n_steps: int, record: bool = True):
+        lib = load()
+        pf = np.ascontiguousarray(pop_first, dtype=np.int32)
+        pops = ctypes.create_string_buffer(pops_bytes, len(pops_bytes))
+        raster = np.zeros((n_steps, self.n), dtype=np.uint8) if record else None
+        ev = ctypes.c_int64(0)
+        rc = lib.or_mt_run(self._h, model, len(pf) - 1, _ptr(pf), ctypes.cast(pops, ctypes.c_void_p), d_slots, seed,
+                           instance, _ptr(state.v), _ptr(state.isyn), _ptr(state.ref), _ptr(state.ring),
+                           _ptr(state.prev_spk), step0, n_steps, _ptr(raster), ctypes.byref(ev))
+        if rc != 0:
+            raise RuntimeError(f"or_mt_run failed ({rc})")
+        return (raster.astype(bool) if record else None), int(ev.value)
+
+    def close(self):
+        if self._h:
+            load().or_mt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- NumPy brute force (tiny sizes)

@@ -223,3 +223,23 @@ def test_exp_lif_state_stays_finite_in_a_small_network():
     r, ev = oracle.run(oracle.LIF_EXP, [0, n], params_bytes([pd14_params()]), 20, 3, 0, *topo, st, 0, 500)
     assert np.isfinite(st.v).all() and np.isfinite(st.isyn).all()
     assert r.sum() > 0 and ev > 0
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_oracle_equals_the_serial_one(threads):
+    """The OpenMP oracle (target ranges per thread, the CPU baseline at every core) gives exactly the
+    serial oracle's raster and state."""
+    n = 300
+    topo = tiny_topology(n, 12000, 20, seed=12, wscale=0.05)
+    mk = lambda: oracle.State(n, 20, np.full(n, -60.0, np.float32), np.zeros(n, np.float32))
+    a, b = mk(), mk()
+    pb = params_bytes([pd14_params(), pd14_params()])
+    ra, ea = oracle.run(oracle.LIF_EXP, [0, 200, n], pb, 20, 3, 0, *topo, a, 0, 400)
+    par = oracle.Parallel(n, *topo, threads=threads)
+    rb, eb = par.run(oracle.LIF_EXP, [0, 200, n], pb, 20, 3, 0, b, 0, 400)
+    par.close()
+    assert ra.sum() > 0 and ea == eb
+    assert np.array_equal(ra, rb)
+    for x, y in ((a.v, b.v), (a.isyn, b.isyn), (a.ring, b.ring)):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    assert np.array_equal(a.ref, b.ref)
