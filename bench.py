@@ -132,11 +132,12 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
 
 def bench_sweep(args) -> int:
     """Batched Brunel sweep: --instances networks drawn from --seed (N log-uniform in [1k, 50k], in-degree
-    10 %, g in [3, 8], eta in [0.9, 4]), one batched launch per step (mapping from the autotuner, tuned on
-    the warmup steps) against the same networks run one after another from the same state.  The final
+    10 %, g in [3, 8], eta in [0.9, 4]), --warmup batched steps, then one batched launch per step (mapping
+    from the autotuner: each candidate timed on --warmup steps from the same state) against the same
+    networks run one after another from the same state.  The final
     states of both must be bit-identical."""
     from naer2_amd import FusedRound
-    from naer2_amd.autotune import tune, tune_intake
+    from naer2_amd.autotune import measure_work, tune, tune_intake
     from naer2_amd.networks import brunel_sweep, sweep_params
 
     params = sweep_params(args.instances or 256, seed=args.seed)
@@ -159,16 +160,23 @@ def bench_sweep(args) -> int:
         for rt in rts:
             tune_intake(rt, steps=200, graph_window=args.window)
         print(f"[bench] intake thresholds tuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    best, times = tune(rts, steps=args.warmup, graph_window=args.window, use_cache=False)
+    fr = FusedRound(rts, {"order": 1, "xcd": 0})  # warmup: past the onset transient, untimed
+    fr.launch(args.warmup, args.window)
+    fr.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    weights = measure_work(rts, steps=200, graph_window=args.window)
+    print(f"[bench] per-network work measured in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    best, times = tune(rts, steps=args.warmup, graph_window=args.window, use_cache=False, weights=weights)
     print(f"[bench] autotune: {best} from {times}", file=sys.stderr, flush=True)
     snaps = [rt.snapshot() for rt in rts]
-    fr = FusedRound(rts, best)
+    fr = FusedRound(rts, best, weights=weights)
     fr.launch(args.window, args.window)  # capture, untimed (the state is restored below)
     torch.cuda.synchronize()
     for rt, sn in zip(rts, snaps):
         rt.restore(sn)
     fr.close()
-    fr = FusedRound(rts, best)
+    fr = FusedRound(rts, best, weights=weights)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fr.launch(args.steps, graph_window=args.window)

@@ -5,9 +5,11 @@ where or when it runs), so the tuner times the candidates on the batch's own fir
 and keeps the fastest.  Choices are cached per device and batch signature in a JSON file
 ($NAER2_AUTOTUNE_CACHE, default ~/.cache/naer2_amd/autotune.json).
 
-Candidates: order 0 (plans as given) or 1 (most links first, so the heaviest groups start first and light
-ones fill the tail); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8
-(a plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments).
+Candidates: order 0 (plans as given), 1 (most links first, so the heaviest groups start first and light
+ones fill the tail) or 2 (most delivered links per group first, measured on each network alone:
+measure_work); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8 (a
+plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments).  Every
+candidate runs from the same snapshot of the state, which is restored afterwards.
 
 tune_intake picks, per network, the spike count from which a step takes the row intake (every in-link
 walked against the spike bitmap) instead of the segment intake (the spiking sources' segments): both give
@@ -27,7 +29,7 @@ import torch
 from ._lib import Tuning
 from .fusion import FusedRound
 
-CANDIDATES = [{"order": o, "xcd": x} for o in (0, 1) for x in (0, 8)]
+CANDIDATES = [{"order": o, "xcd": x} for o in (0, 1, 2) for x in (0, 8)]
 
 
 def cache_path() -> Path:
@@ -57,15 +59,43 @@ def _save(table: dict) -> None:
         pass
 
 
-def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int = 0, use_cache: bool = True):
-    """(best mapping as a dict, {candidate: seconds}).  Advances every runtime by steps x candidates."""
+def measure_work(runtimes, steps: int = 200, graph_window: int = 100) -> list[float]:
+    """Delivered links per step and target group of each runtime, run alone for `steps` steps from its
+    current state (which is restored afterwards): the order-2 weight."""
+    out = []
+    for rt in runtimes:
+        rec = rt.net.record
+        rt.net.record = True
+        rt._ensure_slots(steps + graph_window + 4)
+        snap = rt.snapshot()
+        s0 = rt.step
+        rt.run(steps, graph_window)
+        raster = rt.raster(s0, rt.step)
+        od = rt.out_degree.cpu().numpy()
+        ev = float(od[raster.nonzero()[1]].sum()) / steps
+        rt.restore(snap)
+        rt.net.record = rec
+        out.append(ev / rt.n_groups)
+    return out
+
+
+def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int = 0, use_cache: bool = True,
+         weights: list | None = None):
+    """(best mapping as a dict, {candidate: seconds}).  Order 2 needs weights (measure_work); without them
+    it is left out.  The runtimes end in the state they started in."""
     key = signature(runtimes)
     table = _load() if use_cache else {}
     if use_cache and key in table:
         return table[key]["best"], table[key]["times"]
+    snaps = [rt.snapshot() for rt in runtimes]
     times = {}
     for cand in CANDIDATES:
-        fr = FusedRound(runtimes, Tuning(order=cand["order"], xcd=cand["xcd"]), reserve_steps=reserve_steps)
+        if cand["order"] == 2 and weights is None:
+            continue
+        for rt, sn in zip(runtimes, snaps):
+            rt.restore(sn)
+        fr = FusedRound(runtimes, Tuning(order=cand["order"], xcd=cand["xcd"]), reserve_steps=reserve_steps,
+                        weights=weights)
         fr.launch(graph_window, graph_window)  # capture + first replay, untimed
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -73,6 +103,8 @@ def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int
         torch.cuda.synchronize()
         times[json.dumps(cand, sort_keys=True)] = time.perf_counter() - t0
         fr.close()
+    for rt, sn in zip(runtimes, snaps):
+        rt.restore(sn)
     best = json.loads(min(times, key=times.get))
     if use_cache:
         table[key] = {"best": best, "times": times}

@@ -5,7 +5,8 @@ grid).  FusedRound puts the target groups of up to 256 independent built network
 into one launch per step (n2h_batch_*), with the launch mapping (plan order, XCD-aware block -> group
 mapping) chosen by naer2_amd/autotune.py.  Windows of steps are captured into hipGraphs and replayed.
 
-    FusedRound(runtimes, tuning)    the batch over these runtimes (in this order unless tuning reorders)
+    FusedRound(runtimes, tuning, weights=None)   the batch over these runtimes (in this order unless tuning
+                                    reorders: order 1 most links first, order 2 heaviest weight first)
     .covers(runtimes)               whether a round's runs are exactly these runtimes'
     .launch(n_steps, graph_window)  n_steps steps of every runtime (torch's current stream)
 
@@ -22,7 +23,8 @@ from ._lib import Tuning
 
 
 class FusedRound:
-    def __init__(self, runtimes: list, tuning: Tuning | dict | None = None, reserve_steps: int = 0):
+    def __init__(self, runtimes: list, tuning: Tuning | dict | None = None, reserve_steps: int = 0,
+                 weights: list | None = None):
         if not runtimes:
             raise ValueError("FusedRound: no runtimes")
         if len(runtimes) > 256:
@@ -36,6 +38,13 @@ class FusedRound:
             tuning = Tuning(order=int(tuning.get("order", 0)), xcd=int(tuning.get("xcd", 0)))
         self.tuning = tuning if tuning is not None else Tuning(order=0, xcd=0)
         self.runtimes = list(runtimes)
+        launch_tuning = self.tuning
+        if self.tuning.order == 2:  # heaviest groups first, by a measured weight per runtime
+            if weights is None or len(weights) != len(self.runtimes):
+                raise ValueError("FusedRound: order 2 needs one weight per runtime (autotune.measure_work)")
+            perm = sorted(range(len(self.runtimes)), key=lambda i: -float(weights[i]))
+            self.runtimes = [self.runtimes[i] for i in perm]
+            launch_tuning = Tuning(order=0, xcd=self.tuning.xcd)
         for rt in self.runtimes:  # the bit histories must not move while the batch holds their addresses
             rt._ensure_slots(reserve_steps)
             rt._batch_capacity = rt.step + reserve_steps if rt.net.record else None
@@ -45,7 +54,7 @@ class FusedRound:
         lib = _lib.load()
         arr = (ctypes.c_void_p * len(self.runtimes))(*[rt._plan.value for rt in self.runtimes])
         out = ctypes.c_void_p()
-        _lib.check(lib.n2h_batch_create(arr, len(self.runtimes), ctypes.byref(self.tuning), ctypes.byref(out)),
+        _lib.check(lib.n2h_batch_create(arr, len(self.runtimes), ctypes.byref(launch_tuning), ctypes.byref(out)),
                    "n2h_batch_create")
         self._batch = out
         self._graphs: dict = {}

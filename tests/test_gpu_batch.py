@@ -10,7 +10,7 @@ import pytest
 import torch
 
 from naer2_amd import FusedRound
-from naer2_amd.autotune import CANDIDATES, tune, tune_intake
+from naer2_amd.autotune import CANDIDATES, measure_work, tune, tune_intake
 from naer2_amd.networks import brunel_sweep, sweep_params
 from tests.gpu_util import bits_equal
 
@@ -41,9 +41,10 @@ def test_batch_equals_plain_for_every_mapping():
     batched = brunel_sweep(params, device="cuda", record=True)
     plain = brunel_sweep(params, device="cuda", record=True)
     total = 0
+    weights = [3.0, 1.0, 4.0, 1.5, 9.0][:len(params)]  # order 2: a permutation of the plans
     for cand in CANDIDATES:
         _say(f"case: mapping {cand}")
-        fr = FusedRound([n.runtime for n in batched], cand, reserve_steps=110)
+        fr = FusedRound([n.runtime for n in batched], cand, reserve_steps=110, weights=weights)
         fr.launch(110, graph_window=48)
         fr.close()
         for n in plain:
@@ -72,14 +73,26 @@ def test_autotune_returns_a_candidate_and_keeps_results():
     params = _params()
     batched = brunel_sweep(params, device="cuda", record=True)
     plain = brunel_sweep(params, device="cuda", record=True)
-    best, times = tune([n.runtime for n in batched], steps=48, graph_window=48, reserve_steps=400, use_cache=False)
+    rts = [n.runtime for n in batched]
+    for n in batched + plain:  # off step 0, with spikes in the history
+        n.run(40, graph_window=0)
+    weights = measure_work(rts, steps=48, graph_window=48)
+    assert all(w > 0 for w in weights) and all(rt.step == 40 for rt in rts)
+    best, times = tune(rts, steps=48, graph_window=48, reserve_steps=400, use_cache=False, weights=weights)
     assert best in CANDIDATES and len(times) == len(CANDIDATES)
-    steps = batched[0].runtime.step
+    assert all(rt.step == 40 for rt in rts)  # tuning and measuring restore the state
+    fr = FusedRound(rts, best, reserve_steps=100, weights=weights)
+    fr.launch(100, graph_window=48)
+    fr.close()
     for n in plain:
-        n.run(steps, graph_window=0)
+        n.run(100, graph_window=0)
     torch.cuda.synchronize()
     for a, b in zip(batched, plain):
-        _same(a, b)
+        ra, rb = a.runtime, b.runtime
+        assert ra.step == rb.step == 140
+        assert np.array_equal(ra.raster(40, 140), rb.raster(40, 140))
+        assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+        assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
 
 
 def test_intake_tuning_restores_the_state_and_changes_no_result():
