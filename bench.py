@@ -275,7 +275,7 @@ def bench_pd14grad(args) -> int:
         "metric": "batched differentiable PD14: backward/forward cost ratio (rate-loss surrogate adjoint)",
         "value": round(bw / fw, 3), "unit": "ratio", "n_gpus": 1, "steps": T, "warmup": 0,
         "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{n_inst} seed-distinct full-scale PD14 (config 1) in one launch a step, {T} steps "
+        "config": {"workload": f"{n_inst} seed-distinct full-scale PD14 microcircuits (77169 neurons each) in one launch a step, {T} steps "
                                f"of 0.1 ms, loss sum (rate - target)^2 over populations, targets U[0.5, 10] Hz, "
                                f"grad wrt the 8x8 weight scales, checkpoints every {K} steps, seed {args.seed}",
                    "n_synapses": n_syn},
@@ -291,7 +291,7 @@ def bench_pd14grad(args) -> int:
 
 
 def bench_plain(args) -> int:
-    """The plain path: full-scale PD14 (config 1, the headline) or Brunel config 0.  Warmup, then exactly
+    """The plain path: the full-scale PD14 microcircuit (the headline) or the 2500-neuron Brunel network.  Warmup, then exactly
     --steps steps in graph windows, timed with HIP events on the launch stream; the CPU baselines on the
     same network from the GPU's state."""
     from naer2_amd.networks import brunel, pd14
@@ -366,9 +366,9 @@ def bench_plain(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": ("pd14 full scale (config 1): 77169 LIF-exp neurons, 8 populations, "
+        "config": {"workload": ("pd14 full scale: 77169 LIF-exp neurons, 8 populations, "
                                 f"{rt.canon.n_links} synapses, Poisson background" if kind == "pd14" else
-                                "brunel (config 0): N_E 2000, N_I 500 delta-synapse LIF, C_E 200, C_I 50, J 0.2 mV, "
+                                "brunel: N_E 2000, N_I 500 delta-synapse LIF, C_E 200, C_I 50, J 0.2 mV, "
                                 f"g 5, D 1.5 ms, {rt.canon.n_links} synapses, Poisson drive (eta 2)") +
                                f", dt 0.1 ms, {args.steps} steps timed, seed {args.seed}",
                    "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
@@ -449,7 +449,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad"], default="pd14",
-                    help="pd14: the headline PD14 real-time factor (config 1); brunel: config 0; sweep: the batched "
+                    help="pd14: the headline PD14 real-time factor; brunel: the 2500-neuron Brunel network; sweep: the batched "
                          "Brunel sweep; bptt: the recurrent-LIF BPTT; pd14grad: the batched differentiable PD14")
     ap.add_argument("--instances", type=int, default=None, help="networks (sweep: 256, pd14grad: 4)")
     ap.add_argument("--checkpoint", type=int, default=500, help="pd14grad: steps between saved states")

@@ -161,7 +161,7 @@ int n2h_batch_step(n2h_batch* batch, int32_t n_steps, void* stream);
 int n2h_batch_graph_create(n2h_batch* batch, int32_t n_steps, void* stream, n2h_graph** out);
 
 /* Dense recurrent LIF layer for surrogate-gradient BPTT (no counterpart in the reference, which has no
- * autograd; the north star's differentiation pattern).  Model and adjoint: naer2_amd/csrc/rlif.hip.
+ * autograd: a surrogate-gradient adjoint beside the plain path).  Model and adjoint: naer2_amd/csrc/rlif.hip.
  * Buffers are fp32, row-major: u [T][B][H] (= x W_in^T), w_rec [H][H], w_rec_t = its transpose,
  * spikes [T][B][H]; shadow: double [2][B][H], the forward's double-precision copy of (I, V);
  * checkpoints: double [ceil(T/K)][2][B][H], the shadow before steps 0, K, 2K, ... */
@@ -193,14 +193,14 @@ int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec,
  * Steps that follow at least dense_min_spikes spikes walk every in-link against the previous step's spike
  * bitmap (held in LDS: n_groups <= 1535) instead of looking up the spiking sources' segments; both intakes
  * give bit-identical results.  rows = NULL: off.
- * Replaces: the north star's CSR-by-target pull (receives.py); the reference's own pull is fused.py:1626-1965. */
+ * Replaces: the reference's pull, fused.py:1626-1965 (here target-major, as receives.py names it). */
 #define N2H_ROWS_WIDE 0
 #define N2H_ROWS_COMPACT 1
 int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
                   int32_t format, const float* wtab, int32_t dense_min_spikes);
 
 /* Surrogate-gradient adjoint of the LIF_EXP step for a per-population rate loss (no counterpart in the
- * reference, which is forward-only: the north star's differentiation pattern applied to the plain path).
+ * reference, which is forward-only: a surrogate-gradient adjoint beside the plain path).
  * One call runs reverse step `step` for up to 16 networks in one launch.  Per network: its out-links
  * (canonical CSR by source) packed {target | (delay_steps - 1) << 17 | target population << 24, base weight
  * bits}; alpha[target pop][source pop] the weight scales the forward ran with (its weights are alpha * base);
@@ -238,8 +238,7 @@ typedef struct n2h_surrogate {
 int n2h_adj_last_error(char* buf, size_t len);
 int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h_surrogate* sur, void* stream);
 
-/* Potential trace for the adjoint (no counterpart in the reference, which is forward-only; the north star's
- * differentiation pattern): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
+/* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
  * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a
  * refractory step.  Results are unchanged.  Captured graphs and batches keep the setting they were made
  * with. */

@@ -14,8 +14,8 @@ gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - theta| / dv <= cutoff an
 V_reset); the cutoff makes the gradient sparse (only neurons near the threshold, or spiking, gather).  The
 reset and refractoriness carry no gradient (through the hard reset the adjoint diverges).
 
-The reference is forward-only: this is the north star's differentiation pattern (a hand-written adjoint of
-delivery plus the LIF update, checkpointed every K steps) applied to the plain PD14 path.
+The reference is forward-only: this is a hand-written adjoint of delivery plus the LIF update, checkpointed
+every K steps, added beside the plain PD14 path.
 """
 from __future__ import annotations
 

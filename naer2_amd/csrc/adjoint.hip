@@ -1,8 +1,7 @@
 // Surrogate-gradient adjoint of the exponential-synapse LIF step (the LIF_EXP model of naer2_hip.hip) for a
 // per-population rate loss, batched over networks: one launch per reverse step.
 //
-// The reference is forward-only; this is the north star's differentiation pattern applied to the plain
-// path.  Forward step t of neuron i (naer2_hip.hip, step_group):
+// The reference is forward-only; this adjoint is added beside its plain path.  Forward step t of neuron i (naer2_hip.hip, step_group):
 //     I'_t = I_{t-1} + x_t + ext_t
 //     V^pre_t = V_rest + P_V (V_{t-1} - V_rest) + P_VI I'_t   (r_t = 1), or V_reset while refractory (r_t = 0)
 //     I_t = P_I I'_t;  s_t = [V^pre_t > theta];  V_t = V^pre_t (1 - s_t) + V_reset s_t

@@ -527,7 +527,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
     if (KW > 0 && a.rows && n_prev >= a.dense_min) {
-        // row intake (the pull of the reference's north star): every target lane walks its own in-links in
+        // row intake (a CSR-by-target pull): every target lane walks its own in-links in
         // contract order against the previous step's spike bitmap held in LDS -- the whole table, so this
         // path pays off when a large share of the sources spiked
         const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;

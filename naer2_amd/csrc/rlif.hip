@@ -20,7 +20,7 @@
 // spikes); every K steps the shadow is checkpointed, and the backward pass recomputes the shadow's V inside
 // each segment from its checkpoint and the stored spikes -- the same trajectory whatever K -- for the
 // surrogate derivative.
-// There is no counterpart in the reference (SURVEY: batching, autograd and BPTT do not exist in naer2).
+// There is no counterpart in the reference: autograd and BPTT do not exist in naer2.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 

@@ -10,7 +10,7 @@ Dynamics (dt = 1 ms; alpha = exp(-dt/tau_syn), beta = exp(-dt/tau_m)):
     I_t = alpha I_{t-1} + W_in x_t + W_rec s_{t-1};  V_t = beta V_{t-1} + I_t - theta s_{t-1}
     s_t = H(V_t - theta), surrogate ds/dV = 1 / (1 + slope |V_t - theta|)^2 (fast sigmoid, slope 25)
     J_t = alpha J_{t-1} + W_out s_t;  R_t = beta R_{t-1} + J_t (leaky readout integrators)
-The reference has no autograd (naer2 is forward-only); this is the north star's differentiation pattern.
+The reference has no autograd (naer2 is forward-only): a surrogate-gradient adjoint added beside its plain path.
 Plain library GEMMs (the input projection, the readout projection, the per-step W_rec^T product of the
 adjoint and the weight gradients) run through rocBLAS / hipBLASLt; delivery and the LIF update and their
 adjoints are the hand-written kernels.
