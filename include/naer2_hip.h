@@ -254,6 +254,117 @@ int n2h_set_trace(n2h_plan* plan, uint64_t* buffer);
 /* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */
 int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * The event-driven, exact-time network (Brunel's model A as the reference runs it, test/networks/Brunel):
+ * delta-synapse LIF neurons whose potential changes only at an input's exact arrival time e + D and which
+ * fire inside delivery at that time; Poisson sources with exact spike times; one homogeneous delay D on
+ * every connection (= the window).  Window k emits spikes; window k + 1 delivers all of them.  Per target,
+ * a window's inputs run in ascending (arrival, connection, emission time, source, link); inputs at one arrival
+ * time are received one by one, then the threshold is checked once (the reference's send after every
+ * delivery).  Receive: if a >= t_ref_end: V = V * exp((last - a) / tau) + w, last = a.  Threshold at t:
+ * V >= theta: spike at t, V = V_r, t_ref_end = last = t + t_ref.  exp and log are this library's own fp32
+ * functions (identical in the CPU oracle).  Poisson source i, window k: next < 0: next = interval(0); while
+ * next < t_end(k): spike at next, next += interval(m); interval(m) = -log(u) / rate, u from word 0 of
+ * Philox4x32-10(ctr = {i, k, m, instance}, key = seed), t_end(k) = (float)((double)(k + 1) * window).
+ *
+ * What each entry point replaces in the reference:
+ *   n2h_ev_create / n2h_ev_destroy   naer2/runtime.py:213-323 (PopulationRuntime of queue populations) with
+ *                                    naer2/ring.py:132-245 (the queues) and naer2/history.py:76-130 (region
+ *                                    histories, EmissionHistory(regions=...))
+ *   n2h_ev_run                       naer2/network.py:80-152 (Network.step over D): per window the fused
+ *                                    kernel's queue pull (fused.py:2084-2301, _pull_queues), the merged
+ *                                    delivery (fused.py:2428-2472 with _merge_segments 1300-1370 and
+ *                                    _walk_batch 1421-1526), spikes inside delivery (_emit_delivered
+ *                                    2667-2702) and the sorted regions (_sort_region 2715-2758)
+ *   n2h_ev_graph_*                   naer2/graph.py:35-136 (captured windows)
+ *   n2h_ev_bind_record               naer2/history.py:119-124 (EmissionHistory.spikes: a run's (e, node))
+ *   n2h_ev_status                    naer2/runtime.py:724-735 (check: flags raised by the kernel)
+ * ------------------------------------------------------------------------------------------------------------ */
+#define N2H_EV_NEURON 0      /* delta-synapse LIF with exact times (test/networks/Brunel/neuron.py:6-45) */
+#define N2H_EV_POISSON 1     /* Poisson source with exact times (test/networks/Brunel/poisson.py:7-27) */
+#define N2H_EV_TRAIN 2       /* listed spike times per source (the reference tests' commanded sources) */
+#define N2H_EV_MAX_SOURCES 8 /* source populations of one input stream */
+#define N2H_EV_FLAGS 4
+#define N2H_EV_FLAG_REGION 0   /* a group emitted more spikes in a window than its region holds (lost) */
+#define N2H_EV_FLAG_SEGMENT 1  /* a source had more than 2048 links into one group (the excess lost) */
+#define N2H_EV_FLAG_RECORD 2   /* the record buffer was full (a window not recorded) */
+#define N2H_EV_FLAG_SORT 3     /* internal: more spikes than the sort holds */
+
+typedef struct n2h_ev_pop {
+    int32_t kind;        /* N2H_EV_* */
+    int32_t region_cap;  /* spikes one group of the population can emit per window (<= 2^20) */
+    float tau_inv;       /* neuron: 1 / tau, 1/ms */
+    float theta;         /* neuron: threshold, fires at V >= theta */
+    float v_reset;       /* neuron: V_r */
+    float t_ref;         /* neuron: refractory period, ms */
+    float rate_inv;      /* Poisson: 1 / rate, ms (inf: silent) */
+    int32_t stream;      /* neuron: index of its input stream, -1: no inputs */
+} n2h_ev_pop;
+
+/* An input stream: the spikes of its source populations, listed in the order of the target populations'
+ * incoming connections (connection index order), merged by arrival.  Target populations whose incoming
+ * connections come from the same populations in the same order share one stream. */
+typedef struct n2h_ev_stream {
+    int32_t n_sources;
+    int32_t source_pop[N2H_EV_MAX_SOURCES];
+    float delay;         /* == window */
+} n2h_ev_stream;
+
+/* Neurons 0..n-1, population p owning [pop_first[p], pop_first[p+1]); groups of <= 64 neurons of one
+ * population, a population's groups contiguous and 64 neurons apart.  seg_off / links: the group segments of
+ * n2h_tables (x & 0xffff = target lane); at most one connection per (source, target) population pair, so a
+ * segment holds the links of one connection, in its link order.  train_ptr [n + 1] / train_t: sorted spike
+ * times of N2H_EV_TRAIN sources (else NULL). */
+typedef struct n2h_ev_desc {
+    int32_t n_neurons;
+    int32_t n_pops;
+    int32_t n_groups;
+    int32_t n_streams;
+    const int32_t* pop_first;        /* host [n_pops + 1] */
+    const n2h_ev_pop* pops;          /* host [n_pops] */
+    const n2h_ev_stream* streams;    /* host [n_streams] */
+    float window;                    /* ms: the common delay */
+    uint32_t instance;               /* Philox counter word 3 */
+    uint64_t seed;                   /* Philox key */
+    const int32_t* group_first;      /* device [n_groups] */
+    const int32_t* group_info;       /* device [n_groups]: count | pop << 16 */
+    const int64_t* seg_off;          /* device [64 * n_groups * n_groups + 1] */
+    const uint32_t* links;           /* device [2 * n_links] */
+    int64_t n_links;
+    const int64_t* train_ptr;        /* device [n + 1] or NULL */
+    const float* train_t;            /* device or NULL */
+} n2h_ev_desc;
+
+typedef struct n2h_ev_plan n2h_ev_plan;
+typedef struct n2h_ev_graph n2h_ev_graph;
+
+int n2h_ev_last_error(char* buf, size_t len);
+/* Validates the description and allocates the window buffers (regions, input streams, sort scratch) on the
+ * current device; the window counter starts at 0 with empty input streams. */
+int n2h_ev_create(const n2h_ev_desc* desc, n2h_ev_plan** out);
+int n2h_ev_destroy(n2h_ev_plan* plan);
+/* State, fp32 [n] device tensors: V, the time of V's last exact change, the refractory end, and (Poisson
+ * sources) the next spike time (-1 before the first draw). */
+int n2h_ev_bind_state(n2h_ev_plan* plan, float* v, float* last_time, float* t_ref_end, float* next_spike);
+/* Every window's spikes in (population, time, neuron) order appended at node/time[*count...] (device), and
+ * the window's spike count at window_counts[k % n_window_slots]; a window that does not fit is not recorded
+ * (flag).  node NULL: off. */
+int n2h_ev_bind_record(n2h_ev_plan* plan, int32_t* node, float* time, int32_t* count, int64_t capacity,
+                       int32_t* window_counts, int32_t n_window_slots);
+/* The next window to run; clear_inputs: empty the input streams (a fresh start). */
+int n2h_ev_set_window(n2h_ev_plan* plan, int64_t window, int32_t clear_inputs, void* stream);
+int n2h_ev_get_window(n2h_ev_plan* plan, int64_t* window);   /* synchronizes */
+/* Enqueue n_windows windows (two kernels each) on stream; never synchronizes. */
+int n2h_ev_run(n2h_ev_plan* plan, int32_t n_windows, void* stream);
+int n2h_ev_graph_create(n2h_ev_plan* plan, int32_t n_windows, void* stream, n2h_ev_graph** out);
+int n2h_ev_graph_launch(n2h_ev_graph* graph, void* stream);
+int n2h_ev_graph_destroy(n2h_ev_graph* graph);
+/* flags[0 .. n) = the sticky N2H_EV_FLAG_* words (synchronizes). */
+int n2h_ev_status(n2h_ev_plan* plan, int32_t* flags, int32_t n);
+/* The library's fp32 exp and log over n inputs (no counterpart in the reference: a check of the functions
+ * that the event path's results depend on). */
+int n2h_ev_math(const float* x, float* exp_x, const float* u, float* log_u, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

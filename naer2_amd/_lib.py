@@ -28,7 +28,14 @@ EXPORTS = (
     "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
     "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_vtrace", "n2h_set_trace", "n2h_adj_last_error", "n2h_adj_step", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward",
+    "n2h_ev_last_error", "n2h_ev_create", "n2h_ev_destroy", "n2h_ev_bind_state", "n2h_ev_bind_record",
+    "n2h_ev_set_window", "n2h_ev_get_window", "n2h_ev_run", "n2h_ev_graph_create", "n2h_ev_graph_launch",
+    "n2h_ev_graph_destroy", "n2h_ev_status", "n2h_ev_math",
 )
+
+EV_NEURON, EV_POISSON, EV_TRAIN = 0, 1, 2
+EV_MAX_SOURCES = 8
+EV_FLAGS = ("region", "segment", "record", "sort")
 
 
 class PopParams(ctypes.Structure):
@@ -65,6 +72,28 @@ class Tuning(ctypes.Structure):
         return {"order": int(self.order), "xcd": int(self.xcd)}
 
 
+class EvPop(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("region_cap", ctypes.c_int32), ("tau_inv", ctypes.c_float),
+                ("theta", ctypes.c_float), ("v_reset", ctypes.c_float), ("t_ref", ctypes.c_float),
+                ("rate_inv", ctypes.c_float), ("stream", ctypes.c_int32)]
+
+
+class EvStream(ctypes.Structure):
+    _fields_ = [("n_sources", ctypes.c_int32), ("source_pop", ctypes.c_int32 * EV_MAX_SOURCES),
+                ("delay", ctypes.c_float)]
+
+
+class EvDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_neurons", ctypes.c_int32), ("n_pops", ctypes.c_int32), ("n_groups", ctypes.c_int32),
+        ("n_streams", ctypes.c_int32), ("pop_first", ctypes.POINTER(ctypes.c_int32)),
+        ("pops", ctypes.POINTER(EvPop)), ("streams", ctypes.POINTER(EvStream)), ("window", ctypes.c_float),
+        ("instance", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("group_first", ctypes.c_void_p),
+        ("group_info", ctypes.c_void_p), ("seg_off", ctypes.c_void_p), ("links", ctypes.c_void_p),
+        ("n_links", ctypes.c_int64), ("train_ptr", ctypes.c_void_p), ("train_t", ctypes.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -99,6 +128,19 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_set_vtrace": ([vp, vp, i64, i32], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
+        "n2h_ev_last_error": ([ctypes.c_char_p, ctypes.c_size_t], i32),
+        "n2h_ev_create": ([ctypes.POINTER(EvDesc), ctypes.POINTER(vp)], i32),
+        "n2h_ev_destroy": ([vp], i32),
+        "n2h_ev_bind_state": ([vp, vp, vp, vp, vp], i32),
+        "n2h_ev_bind_record": ([vp, vp, vp, vp, i64, vp, i32], i32),
+        "n2h_ev_set_window": ([vp, i64, i32, vp], i32),
+        "n2h_ev_get_window": ([vp, ctypes.POINTER(i64)], i32),
+        "n2h_ev_run": ([vp, i32, vp], i32),
+        "n2h_ev_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
+        "n2h_ev_graph_launch": ([vp, vp], i32),
+        "n2h_ev_graph_destroy": ([vp], i32),
+        "n2h_ev_status": ([vp, ctypes.POINTER(i32), i32], i32),
+        "n2h_ev_math": ([vp, vp, vp, vp, i64, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -117,3 +159,11 @@ def last_error() -> str:
 def check(rc: int, what: str) -> None:
     if rc != N2H_OK:
         raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def ev_check(rc: int, what: str) -> None:
+    """check() for the n2h_ev_* entry points (their own error text)."""
+    if rc != N2H_OK:
+        buf = ctypes.create_string_buffer(512)
+        load().n2h_ev_last_error(buf, len(buf))
+        raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")

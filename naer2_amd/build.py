@@ -14,7 +14,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRCS = [PKG / "csrc" / "naer2_hip.hip", PKG / "csrc" / "rlif.hip", PKG / "csrc" / "adjoint.hip"]
+SRCS = [PKG / "csrc" / "naer2_hip.hip", PKG / "csrc" / "rlif.hip", PKG / "csrc" / "adjoint.hip", PKG / "csrc" / "event.hip"]
 OUT = PKG / "libnaer2_hip.so"
 ARCH = os.environ.get("NAER2_OFFLOAD_ARCH", "gfx950")
 

@@ -1,0 +1,1050 @@
+// event.hip -- the event-driven, exact-time spiking network on MI355X (gfx950), and its C-ABI (n2h_ev_*).
+//
+// Brunel's model A as the reference runs it (test/networks/Brunel): delta-synapse LIF neurons whose potential
+// changes only at an input's exact arrival time e + D, firing inside delivery at that time, driven by Poisson
+// sources with exact spike times; one homogeneous delay D on every connection.  Since every delay is D, the
+// spikes emitted in one window of length D are exactly the inputs of the next window, so a window is:
+//   1. ev_deliver_kernel, one 64-lane wavefront per group: a neuron group walks the window's input stream
+//      (the previous window's spikes of its source populations, merged by arrival), gathers each spike's links
+//      into the group (the group segments of n2h_tables), sorts them by target lane (stable), and every lane
+//      runs its neuron's inputs in order -- receive (decay to the arrival, add the weight, unless refractory),
+//      and the threshold once per distinct arrival time; a source group draws its spikes for the window.
+//      Spikes go to the group's region of the window.
+//   2. ev_sort_kernel, one workgroup of 1024 threads: the window's spikes in (population, time, neuron) order
+//      (recorded, and each source population's list sorted), then one merged stream per distinct set of
+//      source populations: (arrival, connection, time, neuron) order, the input of the next window.
+// Both read the window index from a device counter (the sort advances it), so windows replay in hipGraphs.
+//
+// Reference behaviour followed (MatteoSchmider/naer2):
+//   naer2/transpiler/templates/fused.py:2084-2301 (_pull_queues): a source run's spikes in (e, node) order;
+//     fused.py:1300-1370 (_merge_segments): the connections' queues merged by arrival e + D_c, ties to the
+//     lower connection; fused.py:1421-1526 (_walk_batch): links unpacked per segment, sorted stably into one
+//     inbox per neuron, each neuron's inputs at one time received one by one, then its send once;
+//   fused.py:2667-2702 (_emit_delivered) and 2715-2758 (_sort_region): spikes inside delivery at the arrival
+//     time, a region per program, sorted by (e, node);
+//   test/networks/Brunel/neuron.py:34-45 (send / receive) and poisson.py:21-24 (update).
+// The fp32 exp and log here are this build's own (Cody-Waite + Taylor; a least-squares log1p fit), computed
+// with exactly the operations of the oracle, so that spike times match the oracle's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "naer2_hip.h"
+#include "philox.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kEnt = 256;          // input-stream entries a group takes per round (4 per lane)
+constexpr int kCapL = 2048;        // links a group stages per round (LDS: 8 B staged + 8 B inbox each)
+constexpr int kSortThreads = 1024;
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define EV_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(N2H_EHIP, "%s: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+struct EvPopDev {
+    int32_t kind;
+    int32_t cap;        // region entries per group
+    float tau_inv, theta, v_reset, t_ref, rate_inv;
+    int32_t stream;     // neuron populations: their input stream, -1: none
+    int32_t group0;     // first group
+    int32_t n_groups;
+    int32_t first;      // first neuron
+    int32_t n;
+    int64_t region0;    // first region entry of group0
+};
+
+struct EvStreamDev {
+    int32_t n_src;
+    int32_t src[N2H_EV_MAX_SOURCES];
+    float delay;
+    int64_t base;       // first entry in the stream arrays
+};
+
+struct EvArgs {
+    int32_t n_groups;
+    int32_t n_pops;
+    int32_t n_streams;
+    int32_t rec_windows;
+    const int32_t* group_first;
+    const int32_t* group_info;
+    const int64_t* seg_off;
+    const uint2* links;
+    const int64_t* train_ptr;
+    const float* train_t;
+    const EvPopDev* pops;
+    const EvStreamDev* streams;
+    float* v;
+    float* last;
+    float* tre;
+    float* nxt;
+    int32_t* reg_lane;
+    float* reg_e;
+    int32_t* reg_count;
+    int32_t* stream_count;
+    int32_t* stream_slot;
+    float* stream_a;
+    unsigned long long* keys;     // [keys_cap]: sort scratch beyond LDS
+    unsigned long long* sorted;   // [keys_cap]: the window's spikes, (population, time, neuron) keys
+    int64_t keys_cap;
+    int32_t* group_prefix;        // [n_groups + 1]
+    int64_t* window;              // the device window counter
+    int32_t* flags;               // [N2H_EV_FLAGS]
+    int32_t* rec_node;
+    float* rec_e;
+    int32_t* rec_count;
+    int32_t* rec_wcount;
+    int64_t rec_cap;
+    float W;
+    uint32_t key0, key1, instance;
+};
+
+// ---------------------------------------------------------------- the contract's fp32 math
+
+// exp(x) for x <= 88: x = n ln2 + r (Cody-Waite: n * 0.693145751953125 is exact), the degree-7 Taylor
+// polynomial of exp(r) by Horner, times 2^n; 0 below -87.
+__device__ __forceinline__ float ev_exp(float x) {
+    if (x < -87.0f) return 0.0f;
+    const float n = rintf(x * 1.44269502f);
+    float r = x - n * 0.693145751953125f;
+    r = r - n * 1.42860677e-06f;
+    float p = 1.98412701e-04f;
+    p = p * r + 1.38888892e-03f;
+    p = p * r + 8.33333377e-03f;
+    p = p * r + 4.16666679e-02f;
+    p = p * r + 1.66666672e-01f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, static_cast<int>(n));
+}
+
+// log(u) for finite u > 0: u = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = t P(t) with t = m - 1 exact and P a
+// degree-10 least-squares fit of log1p(t)/t on [sqrt(1/2) - 1, sqrt(2) - 1].
+__device__ __forceinline__ float ev_log(float u) {
+    const uint32_t b = __float_as_uint(u);
+    int e = static_cast<int>((b >> 23) & 0xFF) - 127;
+    float m = __uint_as_float((b & 0x7FFFFFu) | 0x3F800000u);
+    if (m > 1.41421354f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float t = m - 1.0f;
+    float p = 0.0674023256f;
+    p = p * t + -0.117272295f;
+    p = p * t + 0.119239263f;
+    p = p * t + -0.123992212f;
+    p = p * t + 0.142126068f;
+    p = p * t + -0.166680425f;
+    p = p * t + 0.200025767f;
+    p = p * t + -0.250000238f;
+    p = p * t + 0.333333015f;
+    p = p * t + -0.5f;
+    p = p * t + 1.0f;
+    const float fe = static_cast<float>(e);
+    float r = t * p;
+    r = r + fe * 1.42860677e-06f;
+    return r + fe * 0.693145751953125f;
+}
+
+// One exponential interval of a Poisson source (its m-th draw in window k), in ms.
+__device__ __forceinline__ float ev_interval(const EvArgs& a, uint32_t node, int64_t k, uint32_t m, float rate_inv) {
+    const uint32_t x = philox_word0(node, static_cast<uint32_t>(k), m, a.instance, a.key0, a.key1);
+    const float u = (static_cast<float>(x >> 9) + 0.5f) * 1.1920928955078125e-07f;
+    return (-ev_log(u)) * rate_inv;
+}
+
+__device__ __forceinline__ float window_time(int64_t k, float W) {
+    return static_cast<float>(static_cast<double>(k) * static_cast<double>(W));
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off, kWave));
+    return x;
+}
+
+// The lanes in `fire` spike at t[lane]: appended to the group's region in lane order.  emitted is uniform.
+__device__ __forceinline__ void emit(const EvArgs& a, int64_t rbase, int cap, int& emitted, bool fire, int lane,
+                                     float t) {
+    const uint64_t m = __ballot(fire);
+    if (m) {
+        const int pos = emitted + __popcll(m & ((1ull << lane) - 1ull));
+        if (fire && pos < cap) {
+            a.reg_lane[rbase + pos] = lane;
+            a.reg_e[rbase + pos] = t;
+        }
+        emitted += __popcll(m);
+    }
+}
+
+struct DeliverLds {
+    int32_t off[kEnt + 1];   // the round's entries: first staged link (exclusive scan), off[r] = links
+    int64_t lo[kEnt];        // first link of the entry's segment
+    float a[kEnt];           // arrival time
+    uint32_t st_t[kCapL];    // staged link: target lane | entry << 8
+    float st_w[kCapL];
+    float in_w[kCapL];       // inboxes: the links by target lane, stable
+    float in_a[kCapL];
+    int32_t cnt[kWave];
+    int32_t start[kWave];
+    int32_t run[kWave];
+};
+
+// One group of a window.  Neuron groups: the input stream, then the final threshold; source groups: the
+// window's draws.  Region count and state written last.
+__global__ __launch_bounds__(kWave) void ev_deliver_kernel(EvArgs a) {
+    __shared__ DeliverLds s;
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int info = a.group_info[g];
+    const int cnt = info & 0xFFFF;
+    const EvPopDev P = a.pops[info >> 16];
+    const int64_t k = *a.window;
+    const int64_t rbase = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+    const bool act = lane < cnt;
+    const int node = a.group_first[g] + lane;
+    int emitted = 0;
+
+    if (P.kind == N2H_EV_POISSON) {
+        const float t_end = window_time(k + 1, a.W);
+        float next = act ? a.nxt[node] : 0.0f;
+        uint32_t m = 0;
+        if (act && next < 0.0f) next = ev_interval(a, static_cast<uint32_t>(node), k, m++, P.rate_inv);
+        bool want = act && next < t_end;
+        while (__ballot(want)) {
+            emit(a, rbase, P.cap, emitted, want, lane, next);
+            if (want) next = next + ev_interval(a, static_cast<uint32_t>(node), k, m++, P.rate_inv);
+            want = act && next < t_end;
+        }
+        if (act) a.nxt[node] = next;
+    } else if (P.kind == N2H_EV_TRAIN) {
+        const float t_start = window_time(k, a.W), t_end = window_time(k + 1, a.W);
+        int64_t j = 0, hi = 0;
+        if (act) {
+            int64_t lo = a.train_ptr[node];
+            hi = a.train_ptr[node + 1];
+            int64_t h = hi;
+            while (lo < h) {  // the first listed time >= t_start
+                const int64_t mid = (lo + h) >> 1;
+                if (a.train_t[mid] < t_start) lo = mid + 1;
+                else h = mid;
+            }
+            j = lo;
+        }
+        float t = (act && j < hi) ? a.train_t[j] : 0.0f;
+        bool want = act && j < hi && t < t_end;
+        while (__ballot(want)) {
+            emit(a, rbase, P.cap, emitted, want, lane, t);
+            if (want) {
+                ++j;
+                t = j < hi ? a.train_t[j] : 0.0f;
+            }
+            want = want && j < hi && t < t_end;
+        }
+    } else {
+        float V = act ? a.v[node] : 0.0f;
+        float last = act ? a.last[node] : 0.0f;
+        float tre = act ? a.tre[node] : 0.0f;
+        float aprev = 0.0f;
+        bool pend = false;
+        if (P.stream >= 0) {
+            const EvStreamDev& S = a.streams[P.stream];
+            const int32_t n_s = a.stream_count[P.stream];
+            const int32_t* sslot = a.stream_slot + S.base;
+            const float* sa = a.stream_a + S.base;
+            s.cnt[lane] = 0;
+            s.run[lane] = 0;
+            const uint64_t lt = (1ull << lane) - 1ull;
+            for (int base = 0; base < n_s;) {
+                // this round's entries: 4 per lane, their segments into the group
+                int nl[4], incl[4];
+                int64_t lo[4];
+                float ae[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int idx = base + j * kWave + lane;
+                    const bool ok = idx < n_s;
+                    const int32_t slot = ok ? sslot[idx] : 0;
+                    ae[j] = ok ? sa[idx] : 0.0f;
+                    const int64_t seg = static_cast<int64_t>(slot) * a.n_groups + g;
+                    lo[j] = ok ? a.seg_off[seg] : 0;
+                    const int64_t hi = ok ? a.seg_off[seg + 1] : 0;
+                    nl[j] = static_cast<int>(hi - lo[j]);
+                }
+                int carry = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int x = wave_incl_scan(nl[j], lane);
+                    incl[j] = x + carry;
+                    carry += __shfl(x, kWave - 1, kWave);
+                }
+                // the longest prefix of entries whose links fit in LDS (incl is non-decreasing in entry order)
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    r += __popcll(__ballot(incl[j] <= kCapL && base + j * kWave + lane < n_s));
+                int T;
+                if (r == 0) {  // one segment longer than the stage: its first kCapL links only, flagged
+                    r = 1;
+                    T = kCapL;
+                    if (lane == 0) atomicOr(&a.flags[N2H_EV_FLAG_SEGMENT], 1);
+                } else {
+                    const int jl = (r - 1) >> 6, ll = (r - 1) & 63;
+                    int v = incl[0];
+                    if (jl == 1) v = incl[1];
+                    if (jl == 2) v = incl[2];
+                    if (jl == 3) v = incl[3];
+                    T = __shfl(v, ll, kWave);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int e = j * kWave + lane;
+                    if (e < r) {
+                        s.off[e] = incl[j] - nl[j];
+                        s.lo[e] = lo[j];
+                        s.a[e] = ae[j];
+                    }
+                }
+                if (lane == 0) s.off[r] = T;
+                __syncthreads();
+                // pass 1: every link staged with its target lane and entry, counted per target
+                for (int f0 = 0; f0 < T; f0 += kWave) {
+                    const int f = f0 + lane;
+                    if (f < T) {
+                        int pos = 0;
+#pragma unroll
+                        for (int step = kEnt / 2; step > 0; step >>= 1)
+                            if (pos + step < r && s.off[pos + step] <= f) pos += step;
+                        const uint2 lk = a.links[s.lo[pos] + (f - s.off[pos])];
+                        const uint32_t t = lk.x & 63u;  // (the target lane: x & 0xffff < 64 by construction)
+                        s.st_t[f] = t | (static_cast<uint32_t>(pos) << 8);
+                        s.st_w[f] = __uint_as_float(lk.y);
+                        atomicAdd(&s.cnt[t], 1);
+                    }
+                }
+                __syncthreads();
+                const int c = s.cnt[lane];
+                const int st = wave_incl_scan(c, lane) - c;
+                s.start[lane] = st;
+                const int depth = wave_max(c);
+                __syncthreads();
+                // pass 2: each link to its place in its target's inbox, stable in link order
+                for (int f0 = 0; f0 < T; f0 += kWave) {
+                    const int f = f0 + lane;
+                    const bool ok = f < T;
+                    const uint32_t tw = ok ? s.st_t[f] : 0u;
+                    const uint32_t t = tw & 0xFFu;
+                    uint64_t match = __ballot(ok);
+#pragma unroll
+                    for (int b = 0; b < 6; ++b) {
+                        const uint64_t bb = __ballot(ok && ((t >> b) & 1u));
+                        match &= ((t >> b) & 1u) ? bb : ~bb;
+                    }
+                    if (ok) {
+                        const int p = s.start[t] + s.run[t] + __popcll(match & lt);
+                        s.in_w[p] = s.st_w[f];
+                        s.in_a[p] = s.a[tw >> 8];
+                    }
+                    __syncthreads();
+                    if (ok && (match >> lane) == 1ull) s.run[t] += __popcll(match);  // the last lane of its target
+                    __syncthreads();
+                }
+                // the walk: every lane its neuron's inputs of the round, in order
+                for (int q = 0; q < depth; ++q) {
+                    const bool has = act && q < c;
+                    const float w = has ? s.in_w[st + q] : 0.0f;
+                    const float ar = has ? s.in_a[st + q] : 0.0f;
+                    const bool fire = has && pend && ar != aprev && V >= P.theta;
+                    if (fire) {
+                        V = P.v_reset;
+                        tre = aprev + P.t_ref;
+                        last = tre;
+                    }
+                    emit(a, rbase, P.cap, emitted, fire, lane, aprev);
+                    if (has) {
+                        if (ar >= tre) {
+                            V = V * ev_exp((last - ar) * P.tau_inv) + w;
+                            last = ar;
+                        }
+                        aprev = ar;
+                        pend = true;
+                    }
+                }
+                s.cnt[lane] = 0;
+                s.run[lane] = 0;
+                __syncthreads();
+                base += r;
+            }
+        }
+        const bool fire = act && pend && V >= P.theta;
+        if (fire) {
+            V = P.v_reset;
+            tre = aprev + P.t_ref;
+            last = tre;
+        }
+        emit(a, rbase, P.cap, emitted, fire, lane, aprev);
+        if (act) {
+            a.v[node] = V;
+            a.last[node] = last;
+            a.tre[node] = tre;
+        }
+    }
+    if (lane == 0) {
+        a.reg_count[g] = min(emitted, P.cap);
+        if (emitted > P.cap) atomicOr(&a.flags[N2H_EV_FLAG_REGION], 1);
+    }
+}
+
+// ---------------------------------------------------------------- the window's sort (one workgroup)
+
+// Bitonic network over keys[0, n) (n a power of two) in place, 1024 threads.  LDS: n <= L keys in s_k.
+// Beyond L the keys stay in global memory: stages whose partners are L or more apart run there, the
+// others on chunks of L keys staged in LDS.  The direction of a pair at stage `size` is ascending iff
+// (index & size) == 0, whichever memory holds it.
+template <int L>
+__device__ void bitonic_local(unsigned long long* s_k, int n, int64_t off, int size_lo, int size_hi, int stride_hi) {
+    // stages (size, stride) for size in [size_lo, size_hi], stride from min(size/2, stride_hi) down to 1, on
+    // the n keys at global index off..off+n-1 held in s_k
+    const int tid = threadIdx.x;
+    for (int size = size_lo; size <= size_hi; size <<= 1) {
+        for (int stride = min(size >> 1, stride_hi); stride > 0; stride >>= 1) {
+            for (int p = tid; p < (n >> 1); p += kSortThreads) {
+                const int i = 2 * p - (p & (stride - 1));
+                const int j = i + stride;
+                const bool asc = ((off + i) & size) == 0;
+                const unsigned long long x = s_k[i], y = s_k[j];
+                if ((x > y) == asc) {
+                    s_k[i] = y;
+                    s_k[j] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int L>
+__device__ void block_sort(unsigned long long* keys, int64_t n, unsigned long long* s_k, bool in_lds) {
+    const int tid = threadIdx.x;
+    if (in_lds) {
+        bitonic_local<L>(s_k, static_cast<int>(n), 0, 2, static_cast<int>(n), static_cast<int>(n));
+        return;
+    }
+    // 1. every chunk of L sorted in LDS (stages of size <= L)
+    for (int64_t c0 = 0; c0 < n; c0 += L) {
+        for (int i = tid; i < L; i += kSortThreads) s_k[i] = keys[c0 + i];
+        __syncthreads();
+        bitonic_local<L>(s_k, L, c0, 2, L, L);
+        for (int i = tid; i < L; i += kSortThreads) keys[c0 + i] = s_k[i];
+        __syncthreads();
+    }
+    // 2. larger stages: partners >= L apart in global memory, then the rest per chunk in LDS
+    for (int64_t size = 2 * static_cast<int64_t>(L); size <= n; size <<= 1) {
+        for (int64_t stride = size >> 1; stride >= L; stride >>= 1) {
+            for (int64_t p = tid; p < (n >> 1); p += kSortThreads) {
+                const int64_t i = 2 * p - (p & (stride - 1));
+                const int64_t j = i + stride;
+                const bool asc = (i & size) == 0;
+                const unsigned long long x = keys[i], y = keys[j];
+                if ((x > y) == asc) {
+                    keys[i] = y;
+                    keys[j] = x;
+                }
+            }
+            __syncthreads();
+        }
+        for (int64_t c0 = 0; c0 < n; c0 += L) {
+            for (int i = tid; i < L; i += kSortThreads) s_k[i] = keys[c0 + i];
+            __syncthreads();
+            // stages of this size with stride < L: (c0 + i) & size gives the direction
+            const int tid2 = threadIdx.x;
+            for (int stride = L >> 1; stride > 0; stride >>= 1) {
+                for (int p = tid2; p < (L >> 1); p += kSortThreads) {
+                    const int i = 2 * p - (p & (stride - 1));
+                    const int j = i + stride;
+                    const bool asc = ((c0 + i) & size) == 0;
+                    const unsigned long long x = s_k[i], y = s_k[j];
+                    if ((x > y) == asc) {
+                        s_k[i] = y;
+                        s_k[j] = x;
+                    }
+                }
+                __syncthreads();
+            }
+            for (int i = tid; i < L; i += kSortThreads) keys[c0 + i] = s_k[i];
+            __syncthreads();
+        }
+    }
+}
+
+// Exclusive scan of one value per thread over the workgroup; returns the thread's prefix, *total the sum.
+__device__ int block_excl_scan(int x, int* s_w, int* total) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const int incl = wave_incl_scan(x, lane);
+    if (lane == kWave - 1) s_w[wv] = incl;
+    __syncthreads();
+    if (wv == 0) {
+        const int y = lane < kSortThreads / kWave ? s_w[lane] : 0;
+        const int yi = wave_incl_scan(y, lane);
+        if (lane < kSortThreads / kWave) s_w[lane] = yi - y;
+        if (lane == kSortThreads / kWave - 1) s_w[kSortThreads / kWave] = yi;
+    }
+    __syncthreads();
+    const int r = s_w[wv] + incl - x;
+    *total = s_w[kSortThreads / kWave];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ int64_t pow2_at_least(int64_t n) {
+    int64_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+template <int L>
+__global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
+    extern __shared__ unsigned long long s_k[];
+    __shared__ int s_w[kSortThreads / kWave + 1];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const int64_t k = *a.window;
+    // 1. every group's region: its place in the window's list
+    const int G = a.n_groups;
+    const int per = (G + kSortThreads - 1) / kSortThreads;
+    const int g0 = min(tid * per, G), g1 = min(g0 + per, G);
+    int part = 0;
+    for (int g = g0; g < g1; ++g) part += a.reg_count[g];
+    int S;
+    int pre = block_excl_scan(part, s_w, &S);
+    for (int g = g0; g < g1; ++g) {
+        a.group_prefix[g] = pre;
+        pre += a.reg_count[g];
+    }
+    if (tid == 0) a.group_prefix[G] = S;
+    __syncthreads();
+    if (S > a.keys_cap) {  // cannot happen: the keys hold every region entry
+        if (tid == 0) atomicOr(&a.flags[N2H_EV_FLAG_SORT], 1);
+        S = static_cast<int>(a.keys_cap);
+    }
+    // 2. keys (population << 56 | time bits << 24 | neuron within its population), sorted
+    int64_t n2 = pow2_at_least(S > 0 ? S : 1);
+    bool in_lds = n2 <= L;
+    unsigned long long* keys = in_lds ? s_k : a.keys;
+    for (int g = wv; g < G; g += kSortThreads / kWave) {
+        const int info = a.group_info[g];
+        const int pop = info >> 16;
+        const int c = a.reg_count[g];
+        const int b = a.group_prefix[g];
+        const EvPopDev& P = a.pops[pop];
+        const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+        const uint32_t local0 = static_cast<uint32_t>(a.group_first[g] - P.first);
+        for (int j = lane; j < c; j += kWave) {
+            if (b + j >= S) break;
+            const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
+            keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
+                          (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+        }
+    }
+    for (int64_t i = S + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
+    __syncthreads();
+    block_sort<L>(keys, n2, s_k, in_lds);
+    // 3. the record, and the sorted list for the streams
+    const bool rec = a.rec_node != nullptr;
+    int rc0 = 0;
+    bool rec_ok = false;
+    if (rec) {
+        rc0 = *a.rec_count;
+        rec_ok = static_cast<int64_t>(rc0) + S <= a.rec_cap;
+    }
+    for (int i = tid; i < S; i += kSortThreads) {
+        const unsigned long long key = keys[i];
+        a.sorted[i] = key;
+        if (rec_ok) {
+            const int pop = static_cast<int>(key >> 56);
+            a.rec_node[rc0 + i] = a.pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
+            a.rec_e[rc0 + i] = __uint_as_float(static_cast<uint32_t>((key >> 24) & 0xFFFFFFFFull));
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && rec) {
+        if (rec_ok) *a.rec_count = rc0 + S;
+        else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
+        a.rec_wcount[k % a.rec_windows] = S;
+    }
+    // 4. one stream per distinct source set: (arrival, connection, time, neuron) order
+    for (int si = 0; si < a.n_streams; ++si) {
+        const EvStreamDev& St = a.streams[si];
+        int base[N2H_EV_MAX_SOURCES + 1], off[N2H_EV_MAX_SOURCES];
+        base[0] = 0;
+        for (int j = 0; j < St.n_src; ++j) {
+            const EvPopDev& Q = a.pops[St.src[j]];
+            off[j] = a.group_prefix[Q.group0];
+            base[j + 1] = base[j] + (a.group_prefix[Q.group0 + Q.n_groups] - off[j]);
+        }
+        const int ns = base[St.n_src];
+        n2 = pow2_at_least(ns > 0 ? ns : 1);
+        in_lds = n2 <= L;
+        keys = in_lds ? s_k : a.keys;
+        for (int i = tid; i < ns; i += kSortThreads) {
+            int j = 0;
+            while (i >= base[j + 1]) ++j;
+            const unsigned long long ka = a.sorted[off[j] + (i - base[j])];
+            const float e = __uint_as_float(static_cast<uint32_t>((ka >> 24) & 0xFFFFFFFFull));
+            const float arr = e + St.delay;
+            keys[i] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) | static_cast<uint32_t>(i);
+        }
+        for (int64_t i = ns + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
+        __syncthreads();
+        block_sort<L>(keys, n2, s_k, in_lds);
+        for (int i = tid; i < ns; i += kSortThreads) {
+            const unsigned long long kb = keys[i];
+            const int pos = static_cast<int>(kb & 0xFFFFFFFFull);
+            int j = 0;
+            while (pos >= base[j + 1]) ++j;
+            const unsigned long long ka = a.sorted[off[j] + (pos - base[j])];
+            const EvPopDev& Q = a.pops[static_cast<int>(ka >> 56)];
+            const int local = static_cast<int>(ka & 0xFFFFFFull);
+            a.stream_slot[St.base + i] = (Q.group0 + local / kWave) * kWave + (local & (kWave - 1));
+            a.stream_a[St.base + i] = __uint_as_float(static_cast<uint32_t>(kb >> 32));
+        }
+        if (tid == 0) a.stream_count[si] = ns;
+        __syncthreads();
+    }
+    if (tid == 0) *a.window = k + 1;
+}
+
+__global__ void ev_set_kernel(int64_t* window, int64_t k, int32_t* stream_count, int n_streams, int clear) {
+    *window = k;
+    if (clear)
+        for (int i = 0; i < n_streams; ++i) stream_count[i] = 0;
+}
+
+__global__ void ev_math_kernel(const float* x, float* ex, const float* u, float* lg, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ex[i] = ev_exp(x[i]);
+        lg[i] = ev_log(u[i]);
+    }
+}
+
+}  // namespace
+
+struct n2h_ev_plan {
+    EvArgs args;
+    int32_t n_neurons = 0;
+    int32_t sort_lds = 0;         // keys sorted in LDS (8192 or 4096)
+    std::vector<EvPopDev> pops;
+    std::vector<EvStreamDev> streams;
+    EvPopDev* pops_dev = nullptr;
+    EvStreamDev* streams_dev = nullptr;
+    int32_t* reg_lane = nullptr;
+    float* reg_e = nullptr;
+    int32_t* reg_count = nullptr;
+    int32_t* stream_count = nullptr;
+    int32_t* stream_slot = nullptr;
+    float* stream_a = nullptr;
+    unsigned long long* keys = nullptr;
+    unsigned long long* sorted = nullptr;
+    int32_t* group_prefix = nullptr;
+    int64_t* window = nullptr;
+    int32_t* flags = nullptr;
+    int64_t host_window = 0;
+    bool bound = false;
+};
+
+struct n2h_ev_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    n2h_ev_plan* plan;
+    int32_t n_windows;
+};
+
+namespace {
+
+void free_plan(n2h_ev_plan* p) {
+    if (!p) return;
+    void* bufs[] = {p->pops_dev, p->streams_dev, p->reg_lane, p->reg_e, p->reg_count, p->stream_count, p->stream_slot,
+                    p->stream_a, p->keys, p->sorted, p->group_prefix, p->window, p->flags};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+template <typename T>
+int dev_alloc(T** out, int64_t count) {
+    const size_t bytes = static_cast<size_t>(count > 0 ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(out), bytes);
+    if (e != hipSuccess) return fail(N2H_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    e = hipMemset(*out, 0, bytes);
+    if (e != hipSuccess) return fail(N2H_EHIP, "hipMemset: %s", hipGetErrorString(e));
+    return N2H_OK;
+}
+
+int launch_windows(n2h_ev_plan* p, int32_t n, hipStream_t stream) {
+    const dim3 grid(static_cast<unsigned>(p->args.n_groups)), block(kWave);
+    const size_t lds = static_cast<size_t>(p->sort_lds) * sizeof(unsigned long long);
+    for (int32_t i = 0; i < n; ++i) {
+        hipLaunchKernelGGL(ev_deliver_kernel, grid, block, 0, stream, p->args);
+        if (p->sort_lds == 8192)
+            hipLaunchKernelGGL(ev_sort_kernel<8192>, dim3(1), dim3(kSortThreads), lds, stream, p->args);
+        else
+            hipLaunchKernelGGL(ev_sort_kernel<4096>, dim3(1), dim3(kSortThreads), lds, stream, p->args);
+    }
+    EV_HIP(hipGetLastError());
+    return N2H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int n2h_ev_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return N2H_EINVAL;
+    snprintf(buf, len, "%s", g_err);
+    return N2H_OK;
+}
+
+int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
+    if (!d || !out) return fail(N2H_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->n_neurons < 0 || d->n_pops < 1 || d->n_pops > N2H_MAX_POPS || d->n_groups < 1 || d->n_streams < 0)
+        return fail(N2H_EINVAL, "bad sizes (neurons %d, populations %d, groups %d, streams %d)", d->n_neurons,
+                    d->n_pops, d->n_groups, d->n_streams);
+    if (!d->pop_first || !d->pops || (d->n_streams && !d->streams) || !d->group_first || !d->group_info ||
+        !d->seg_off || (d->n_links && !d->links))
+        return fail(N2H_EINVAL, "a required pointer is null");
+    if (!(d->window > 0.0f)) return fail(N2H_EINVAL, "the window must be positive");
+    if (d->pop_first[0] != 0 || d->pop_first[d->n_pops] != d->n_neurons)
+        return fail(N2H_EINVAL, "pop_first must run from 0 to n_neurons");
+    n2h_ev_plan* p = new (std::nothrow) n2h_ev_plan();
+    if (!p) return fail(N2H_ENOMEM, "plan");
+    int rc = N2H_OK;
+    std::vector<int32_t> info(static_cast<size_t>(d->n_groups)), first(static_cast<size_t>(d->n_groups));
+    p->n_neurons = d->n_neurons;
+    p->pops.resize(static_cast<size_t>(d->n_pops));
+    if (hipMemcpy(info.data(), d->group_info, info.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(first.data(), d->group_first, first.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        free_plan(p);
+        return fail(N2H_EHIP, "reading the group layout");
+    }
+    for (int32_t q = 0; q < d->n_pops; ++q) {
+        const n2h_ev_pop& s = d->pops[q];
+        EvPopDev& t = p->pops[static_cast<size_t>(q)];
+        memset(&t, 0, sizeof(t));
+        if (s.kind != N2H_EV_NEURON && s.kind != N2H_EV_POISSON && s.kind != N2H_EV_TRAIN) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: unknown kind %d", q, s.kind);
+        }
+        if (s.region_cap < 1 || s.region_cap > (1 << 20)) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: region_cap %d out of [1, 2^20]", q, s.region_cap);
+        }
+        if (s.kind == N2H_EV_NEURON && (s.stream < -1 || s.stream >= d->n_streams)) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: stream %d out of range", q, s.stream);
+        }
+        if (s.kind == N2H_EV_TRAIN && (!d->train_ptr || !d->train_t)) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: spike trains need train_ptr and train_t", q);
+        }
+        if (d->pop_first[q + 1] - d->pop_first[q] >= (1 << 24)) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: at most 2^24 - 1 neurons", q);
+        }
+        t.kind = s.kind;
+        t.cap = s.region_cap;
+        t.tau_inv = s.tau_inv;
+        t.theta = s.theta;
+        t.v_reset = s.v_reset;
+        t.t_ref = s.t_ref;
+        t.rate_inv = s.rate_inv;
+        t.stream = s.kind == N2H_EV_NEURON ? s.stream : -1;
+        t.first = d->pop_first[q];
+        t.n = d->pop_first[q + 1] - d->pop_first[q];
+        t.group0 = -1;
+    }
+    // every population's groups: contiguous, in population order, each within its population
+    for (int32_t g = 0; g < d->n_groups; ++g) {
+        const int q = info[static_cast<size_t>(g)] >> 16, c = info[static_cast<size_t>(g)] & 0xFFFF;
+        if (q < 0 || q >= d->n_pops || c < 1 || c > kWave) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "group %d: bad group_info", g);
+        }
+        EvPopDev& t = p->pops[static_cast<size_t>(q)];
+        if (t.group0 < 0) t.group0 = g;
+        else if (t.group0 + t.n_groups != g) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "population %d: groups not contiguous", q);
+        }
+        if (first[static_cast<size_t>(g)] != t.first + kWave * t.n_groups) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "group %d: group_first must be 64 neurons apart within a population", g);
+        }
+        ++t.n_groups;
+    }
+    int64_t regions = 0;
+    for (auto& t : p->pops) {
+        if (t.group0 < 0) t.group0 = 0;
+        t.region0 = regions;
+        regions += static_cast<int64_t>(t.n_groups) * t.cap;
+    }
+    int64_t stream_total = 0;
+    p->streams.resize(static_cast<size_t>(d->n_streams));
+    for (int32_t si = 0; si < d->n_streams; ++si) {
+        const n2h_ev_stream& s = d->streams[si];
+        EvStreamDev& t = p->streams[static_cast<size_t>(si)];
+        memset(&t, 0, sizeof(t));
+        if (s.n_sources < 1 || s.n_sources > N2H_EV_MAX_SOURCES) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "stream %d: 1 to %d source populations", si, N2H_EV_MAX_SOURCES);
+        }
+        if (s.delay != d->window) {
+            free_plan(p);
+            return fail(N2H_EINVAL, "stream %d: its delay must equal the window (one homogeneous delay)", si);
+        }
+        t.n_src = s.n_sources;
+        t.delay = s.delay;
+        t.base = stream_total;
+        for (int j = 0; j < s.n_sources; ++j) {
+            const int q = s.source_pop[j];
+            for (int j2 = 0; j2 < j; ++j2)
+                if (s.source_pop[j2] == q) {
+                    free_plan(p);
+                    return fail(N2H_EINVAL, "stream %d: population %d twice", si, q);
+                }
+            if (q < 0 || q >= d->n_pops) {
+                free_plan(p);
+                return fail(N2H_EINVAL, "stream %d: source population %d out of range", si, q);
+            }
+            t.src[j] = q;
+            stream_total += static_cast<int64_t>(p->pops[static_cast<size_t>(q)].n_groups) * p->pops[static_cast<size_t>(q)].cap;
+        }
+    }
+    int64_t keys_cap = 1;
+    while (keys_cap < regions) keys_cap <<= 1;
+    if (keys_cap >= (1ll << 31)) {
+        free_plan(p);
+        return fail(N2H_EINVAL, "%lld region entries: at most 2^30 per window", static_cast<long long>(regions));
+    }
+    int dev = 0, lds_max = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    p->sort_lds = lds_max >= 8192 * 8 + 4096 ? 8192 : 4096;
+    if (p->sort_lds == 8192 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(ev_sort_kernel<8192>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)
+        p->sort_lds = 4096;
+    (void)hipGetLastError();
+    if ((rc = dev_alloc(&p->pops_dev, d->n_pops)) || (rc = dev_alloc(&p->streams_dev, d->n_streams)) ||
+        (rc = dev_alloc(&p->reg_lane, regions)) || (rc = dev_alloc(&p->reg_e, regions)) ||
+        (rc = dev_alloc(&p->reg_count, d->n_groups)) || (rc = dev_alloc(&p->stream_count, d->n_streams)) ||
+        (rc = dev_alloc(&p->stream_slot, stream_total)) || (rc = dev_alloc(&p->stream_a, stream_total)) ||
+        (rc = dev_alloc(&p->keys, keys_cap)) || (rc = dev_alloc(&p->sorted, keys_cap)) ||
+        (rc = dev_alloc(&p->group_prefix, d->n_groups + 1)) || (rc = dev_alloc(&p->window, 1)) ||
+        (rc = dev_alloc(&p->flags, N2H_EV_FLAGS))) {
+        free_plan(p);
+        return rc;
+    }
+    if (hipMemcpy(p->pops_dev, p->pops.data(), p->pops.size() * sizeof(EvPopDev), hipMemcpyHostToDevice) != hipSuccess ||
+        (d->n_streams && hipMemcpy(p->streams_dev, p->streams.data(), p->streams.size() * sizeof(EvStreamDev),
+                                   hipMemcpyHostToDevice) != hipSuccess)) {
+        free_plan(p);
+        return fail(N2H_EHIP, "copying the parameters");
+    }
+    EvArgs& a = p->args;
+    memset(&a, 0, sizeof(a));
+    a.n_groups = d->n_groups;
+    a.n_pops = d->n_pops;
+    a.n_streams = d->n_streams;
+    a.rec_windows = 1;
+    a.group_first = d->group_first;
+    a.group_info = d->group_info;
+    a.seg_off = d->seg_off;
+    a.links = reinterpret_cast<const uint2*>(d->links);
+    a.train_ptr = d->train_ptr;
+    a.train_t = d->train_t;
+    a.pops = p->pops_dev;
+    a.streams = p->streams_dev;
+    a.reg_lane = p->reg_lane;
+    a.reg_e = p->reg_e;
+    a.reg_count = p->reg_count;
+    a.stream_count = p->stream_count;
+    a.stream_slot = p->stream_slot;
+    a.stream_a = p->stream_a;
+    a.keys = p->keys;
+    a.sorted = p->sorted;
+    a.keys_cap = keys_cap;
+    a.group_prefix = p->group_prefix;
+    a.window = p->window;
+    a.flags = p->flags;
+    a.W = d->window;
+    a.key0 = static_cast<uint32_t>(d->seed);
+    a.key1 = static_cast<uint32_t>(d->seed >> 32);
+    a.instance = d->instance;
+    *out = p;
+    return N2H_OK;
+}
+
+int n2h_ev_destroy(n2h_ev_plan* p) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    (void)hipDeviceSynchronize();
+    free_plan(p);
+    return N2H_OK;
+}
+
+int n2h_ev_bind_state(n2h_ev_plan* p, float* v, float* last_time, float* t_ref_end, float* next_spike) {
+    if (!p || !v || !last_time || !t_ref_end || !next_spike) return fail(N2H_EINVAL, "null argument");
+    p->args.v = v;
+    p->args.last = last_time;
+    p->args.tre = t_ref_end;
+    p->args.nxt = next_spike;
+    p->bound = true;
+    return N2H_OK;
+}
+
+int n2h_ev_bind_record(n2h_ev_plan* p, int32_t* node, float* time, int32_t* count, int64_t capacity,
+                       int32_t* window_counts, int32_t n_window_slots) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (!node) {
+        p->args.rec_node = nullptr;
+        p->args.rec_e = nullptr;
+        p->args.rec_count = nullptr;
+        p->args.rec_wcount = nullptr;
+        p->args.rec_cap = 0;
+        p->args.rec_windows = 1;
+        return N2H_OK;
+    }
+    if (!time || !count || !window_counts || capacity < 0 || n_window_slots < 1)
+        return fail(N2H_EINVAL, "record: time, count and window_counts are required, capacity >= 0, slots >= 1");
+    p->args.rec_node = node;
+    p->args.rec_e = time;
+    p->args.rec_count = count;
+    p->args.rec_wcount = window_counts;
+    p->args.rec_cap = capacity;
+    p->args.rec_windows = n_window_slots;
+    return N2H_OK;
+}
+
+int n2h_ev_set_window(n2h_ev_plan* p, int64_t window, int32_t clear_inputs, void* stream) {
+    if (!p || window < 0) return fail(N2H_EINVAL, "bad argument");
+    hipLaunchKernelGGL(ev_set_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), p->window, window,
+                       p->stream_count, p->args.n_streams, clear_inputs ? 1 : 0);
+    EV_HIP(hipGetLastError());
+    p->host_window = window;
+    return N2H_OK;
+}
+
+int n2h_ev_get_window(n2h_ev_plan* p, int64_t* window) {
+    if (!p || !window) return fail(N2H_EINVAL, "null argument");
+    EV_HIP(hipMemcpy(window, p->window, sizeof(int64_t), hipMemcpyDeviceToHost));
+    return N2H_OK;
+}
+
+int n2h_ev_run(n2h_ev_plan* p, int32_t n_windows, void* stream) {
+    if (!p || n_windows < 0) return fail(N2H_EINVAL, "bad argument");
+    if (!p->bound) return fail(N2H_ESTATE, "state not bound (n2h_ev_bind_state)");
+    int rc = launch_windows(p, n_windows, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
+    p->host_window += n_windows;
+    return N2H_OK;
+}
+
+int n2h_ev_graph_create(n2h_ev_plan* p, int32_t n_windows, void* stream, n2h_ev_graph** out) {
+    (void)stream;
+    if (!p || n_windows < 1 || !out) return fail(N2H_EINVAL, "bad argument");
+    if (!p->bound) return fail(N2H_ESTATE, "state not bound (n2h_ev_bind_state)");
+    *out = nullptr;
+    hipStream_t cap;
+    EV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cap);
+        return fail(N2H_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    int rc = launch_windows(p, n_windows, cap);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cap, &graph);
+    (void)hipStreamDestroy(cap);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(N2H_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(N2H_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    n2h_ev_graph* gr = new (std::nothrow) n2h_ev_graph{graph, exec, p, n_windows};
+    if (!gr) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return fail(N2H_ENOMEM, "graph");
+    }
+    *out = gr;
+    return N2H_OK;
+}
+
+int n2h_ev_graph_launch(n2h_ev_graph* g, void* stream) {
+    if (!g) return fail(N2H_EINVAL, "null graph");
+    EV_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    g->plan->host_window += g->n_windows;
+    return N2H_OK;
+}
+
+int n2h_ev_graph_destroy(n2h_ev_graph* g) {
+    if (!g) return N2H_OK;
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    return N2H_OK;
+}
+
+int n2h_ev_status(n2h_ev_plan* p, int32_t* flags, int32_t n) {
+    if (!p || !flags || n < 1) return fail(N2H_EINVAL, "bad argument");
+    int32_t f[N2H_EV_FLAGS];
+    EV_HIP(hipMemcpy(f, p->flags, sizeof(f), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) flags[i] = i < N2H_EV_FLAGS ? f[i] : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_math(const float* x, float* exp_x, const float* u, float* log_u, int64_t n, void* stream) {
+    if (n < 0 || (n && (!x || !exp_x || !u || !log_u))) return fail(N2H_EINVAL, "bad argument");
+    if (n == 0) return N2H_OK;
+    const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+    hipLaunchKernelGGL(ev_math_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), x, exp_x, u,
+                       log_u, n);
+    EV_HIP(hipGetLastError());
+    return N2H_OK;
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@
 #include <new>
 
 #include "naer2_hip.h"
+#include "philox.h"
 
 namespace {
 
@@ -107,25 +108,6 @@ struct StepArgs {
     int32_t vtrace_steps;
     int32_t pad4;
 };
-
-// Philox4x32-10 (Salmon et al., SC'11), word 0 of the output block.
-__device__ __forceinline__ uint32_t philox_word0(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                                 uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return c0;
-}
 
 // Inclusive prefix sum over the 64 lanes of the wavefront.
 __device__ __forceinline__ int wave_incl_scan(int x, int lane) {

@@ -108,7 +108,7 @@ class Population:
     """`size` neurons of one node type (naer2/population.py:22-143).  Its dynamic state lives in the
     network's device tensors once built; before that, values set here are kept as the initial state."""
 
-    _STATE = {"V", "potential", "current", "ref"}
+    _STATE = {"V", "potential", "current", "ref", "last_time", "t_ref_end", "next_spike"}
 
     def __init__(self, instance, shape):
         family_of(instance)

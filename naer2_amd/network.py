@@ -89,8 +89,15 @@ class Network:
     @property
     def runtime(self) -> "Runtime":
         if self._rt is None:
-            self._rt = Runtime(self)
+            from .event import EventRuntime, is_event_net
+            self._rt = EventRuntime(self) if is_event_net(self) else Runtime(self)
         return self._rt
+
+    @property
+    def event_driven(self) -> bool:
+        """True for networks of the event-driven, exact-time node types (naer2_amd/event.py)."""
+        from .event import is_event_net
+        return is_event_net(self)
 
     def build(self) -> "Runtime":
         return self.runtime
@@ -100,25 +107,35 @@ class Network:
         return self._rt.step if self._rt is not None else 0
 
     def step(self, t_step: float) -> None:
-        """Advance the network by t_step ms, a whole number of steps of `resolution`."""
+        """Advance the network by t_step ms: a whole number of steps of `resolution`, or for an event-driven
+        network a whole number of windows (its common delay)."""
+        if self.event_driven:
+            rt = self.runtime
+            rt.advance(t_step)
+            self.t = rt.window_index * rt.window
+            return
         n = t_step / self.resolution
         if abs(n - round(n)) > 1e-6 * max(1.0, abs(n)):
             raise ValueError(f"t_step {t_step} is not a whole number of steps of {self.resolution} ms")
         self.run(int(round(n)))
 
     def run(self, n_steps: int, graph_window: int | None = None) -> None:
+        """n_steps steps of `resolution` (event-driven networks: windows)."""
         rt = self.runtime
         rt.run(n_steps, self.graph_window if graph_window is None else graph_window)
-        self.t = rt.step * self.resolution
+        self.t = rt.window_index * rt.window if self.event_driven else rt.step * self.resolution
 
     def spikes(self, pop=None) -> tuple[np.ndarray, np.ndarray]:
         """(steps, neuron ids) of every recorded spike, ascending by step then id; neuron ids are the
-        population's own (local) indices when pop is given, the network's otherwise."""
+        population's own (local) indices when pop is given, the network's otherwise.  Event-driven networks:
+        (spike times in ms, neuron ids), window by window."""
         return self.runtime.spikes(pop)
 
     def rates(self, t_skip_ms: float = 0.0) -> dict:
         """Mean rate in Hz of each population over the recorded steps after t_skip_ms."""
         rt = self.runtime
+        if self.event_driven:
+            return rt.rates(t_skip_ms)
         steps, ids = rt.spikes(None)
         first = int(round(t_skip_ms / self.resolution))
         dur = (rt.step - first) * self.resolution / 1000.0

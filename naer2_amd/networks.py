@@ -5,6 +5,10 @@ brunel(): Brunel (2000) model A, time-driven (tools/brunel_reference.py:14-48), 
   V_r=10 mV, t_ref=2 ms, dt=0.1 ms.  External drive: Poisson counts per step of mean
   C_E eta nu_thr dt with nu_thr = theta / (J C_E tau) (test/networks/Brunel/net.py:83-91), weight J.
   Initial potentials uniform in [0, theta).
+brunel_exact(): Brunel (2000) model A as the reference's own event-driven network (test/networks/Brunel/
+  net.py:1-118): exact-time delta LIF neurons (E, I) and N_E Poisson sources P at eta nu_thr, fixed in-degree
+  C_E from E, C_I = gamma C_E from I, C_E from P, one delay D everywhere; initial potentials uniform in
+  [0, theta).
 pd14(): the Potjans & Diesmann (2014) microcircuit of test/networks/PD14/net.py:25-108 without the thalamic
   population, with a Poisson background of K_ext inputs at 8 Hz (J_ext = 87.81 pA) instead of the DC shift
   of V_rest, 8 populations, fixed-total-number connectivity (connection.py:9-31), weights normal +-10% with
@@ -16,7 +20,7 @@ from __future__ import annotations
 import torch
 
 from .network import Network
-from .nodes import FixedInDegree, FixedTotalNumber, LIFDelta, LIFExp
+from .nodes import FixedInDegree, FixedTotalNumber, LIFDelta, LIFDeltaExact, LIFExp, PoissonExact
 
 _J_E = 0.08781
 _J_I = -0.35124
@@ -64,6 +68,61 @@ def brunel(N_E: int = 2000, N_I: int = 500, C_E: int = 200, C_I: int = 50, J: fl
         ci = I >> FixedInDegree(C_I) >> target
         ci.weight, ci.delay = -g * J, D
     return net, {"E": E, "I": I}
+
+
+BRUNEL_EXACT = {
+    "N": 10000,  # N_E + N_I
+    "f_E": 0.8,
+    "epsilon": 0.1,  # connection probability (strong scaling: C_E = epsilon N_E)
+    "gamma": 0.25,   # C_I / C_E
+    "J": 0.1,        # mV
+    "g": 5.0,
+    "eta": 2.0,
+    "D": 1.5,        # ms, every connection
+    "neuron": {"tau": 20.0, "theta": 20.0, "V_r": 10.0, "t_ref": 2.0},
+}
+
+
+def brunel_exact(N: int | None = None, K: int | None = None, eta: float | None = None, g: float | None = None,
+                 desc: dict = BRUNEL_EXACT, seed: int = 0, device=None, instance: int = 0):
+    """(network, {"E", "I", "P": populations}): test/networks/Brunel/net.py:47-80 (build).  N: N_E + N_I;
+    K: a fixed in-degree C_E (None: epsilon N_E)."""
+    N = N or desc["N"]
+    eta = desc["eta"] if eta is None else eta
+    g = desc["g"] if g is None else g
+    J, D, nrn = desc["J"], desc["D"], desc["neuron"]
+    N_E = round(desc["f_E"] * N)
+    N_I = N - N_E
+    C_E = K if K is not None else round(desc["epsilon"] * N_E)
+    C_I = max(round(desc["gamma"] * C_E), 1)
+    rate = eta * nrn["theta"] / (J * C_E * nrn["tau"])  # spikes per ms
+    E = LIFDeltaExact(**nrn) * N_E
+    I = LIFDeltaExact(**nrn) * N_I
+    P = PoissonExact(rate) * N_E
+    net = E.network
+    net.merge(I.network)
+    net.merge(P.network)
+    _configure(net, 0.1, seed, device, instance)
+    gen = net.generator
+    E.V = torch.rand(N_E, generator=gen, device=gen.device) * nrn["theta"]
+    I.V = torch.rand(N_I, generator=gen, device=gen.device) * nrn["theta"]
+    for target in (E, I):
+        for src, C, w in ((E, C_E, J), (I, C_I, -g * J), (P, C_E, J)):
+            conn = src >> FixedInDegree(C) >> target
+            conn.weight, conn.delay = w, D
+    return net, {"E": E, "I": I, "P": P}
+
+
+def brunel_exact_theory(N: int | None = None, K: int | None = None, eta: float | None = None,
+                        desc: dict = BRUNEL_EXACT) -> dict:
+    """test/networks/Brunel/net.py:83-91: C_E, C_I, nu_thr and the external rate per source (Hz)."""
+    N = N or desc["N"]
+    eta = desc["eta"] if eta is None else eta
+    N_E = round(desc["f_E"] * N)
+    C_E = K if K is not None else round(desc["epsilon"] * N_E)
+    nu_thr = desc["neuron"]["theta"] / (desc["J"] * C_E * desc["neuron"]["tau"])
+    return {"C_E": C_E, "C_I": max(round(desc["gamma"] * C_E), 1), "nu_thr_Hz": 1000 * nu_thr,
+            "rate_ext_Hz": 1000 * eta * nu_thr}
 
 
 def pd14(scale: float = 1.0, seed: int = 0, dt: float = 0.1, device=None, instance: int = 0,

@@ -8,6 +8,12 @@ Node families (one hand-written kernel each, naer2_amd/csrc/naer2_hip.hip):
   LIFExp    "lif_exp"    LIF with exponential current synapses and the exact propagator
                          (test/networks/PD14/neuron.py:6-60), with a Poisson background of K_ext inputs at
                          nu_ext of weight J_ext instead of the reference's DC shift of V_rest (neuron.py:28-29).
+  LIFDeltaExact  "event_lif"      Brunel_neuron (test/networks/Brunel/neuron.py:6-45): the event-driven delta LIF
+                                  with exact input and spike times (naer2_amd/event.py, csrc/event.hip)
+  PoissonExact   "event_poisson"  Brunel_poisson (test/networks/Brunel/poisson.py:7-27): exponential intervals,
+                                  exact spike times
+  SpikeTrains    "event_train"    listed spike times per source (the commanded sources of the reference's
+                                  tests, test/nodes/commanded.py)
 Patterns:
   FixedInDegree(K)          every target draws K sources uniformly with replacement
                             (test/networks/Brunel/connection.py:7-33, tools/brunel_reference.py:20-21)
@@ -61,6 +67,45 @@ class LIFExp:
     def params(self, dt: float):
         return lif_exp_params(dt, self.t_mem, self.t_syn, self.R_mem, self.t_ref, self.E_leak, self.V_thresh,
                               self.V_reset, self.J_ext, self.K_ext * self.nu_ext * dt)
+
+
+@node
+class LIFDeltaExact:
+    """Event-driven delta-synapse LIF with exact times: receive decays V to the input's arrival time and adds
+    the weight unless refractory; after the inputs at one time, V >= theta fires at that time."""
+    family = "event_lif"
+    state = ("V", "last_time", "t_ref_end")
+
+    def __init__(self, tau: float = 20.0, theta: float = 20.0, V_r: float = 10.0, t_ref: float = 2.0):
+        self.tau, self.theta, self.V_r, self.t_ref = tau, theta, V_r, t_ref
+        self.V = 0.0
+        self.last_time = 0.0
+        self.t_ref_end = 0.0
+
+
+@node
+class PoissonExact:
+    """A Poisson source with exact spike times: rate in spikes per ms."""
+    family = "event_poisson"
+    state = ("next_spike",)
+
+    def __init__(self, rate: float):
+        if rate < 0:
+            raise ValueError("rate must be >= 0")
+        self.rate = rate
+        self.next_spike = -1.0
+
+
+@node
+class SpikeTrains:
+    """Sources firing at listed times: times[i] the (ascending, >= 0) spike times of source i, ms."""
+    family = "event_train"
+    state = ()
+
+    def __init__(self, times):
+        self.times = [sorted(float(t) for t in ts) for ts in times]
+        if any(t < 0 for ts in self.times for t in ts):
+            raise ValueError("spike times must be >= 0")
 
 
 @pattern
