@@ -387,6 +387,155 @@ def bench_plain(args) -> int:
     return 0
 
 
+def _event_oracle_net(net, rt):
+    """The oracle's description of a built event-driven network (host copies of its canonical table)."""
+    from oracle import event as ev
+
+    index = {p: i for i, p in enumerate(rt.pops)}
+    pops = []
+    for p in rt.pops:
+        e = ev.EvPop()
+        if p.family == "event_lif":
+            e.kind = ev.NEURON
+            e.tau_inv = float(np.float32(1.0 / p.instance.tau))
+            e.theta, e.v_reset, e.t_ref = p.instance.theta, p.instance.V_r, p.instance.t_ref
+        else:
+            e.kind = ev.POISSON
+            e.rate_inv = float(np.float32(1.0 / p.instance.rate))
+        pops.append(e)
+    conns = net.connections
+    return ev.EvNet(rt.pop_first.astype(np.int32), pops, np.array([index[c.source] for c in conns], np.int32),
+                    np.array([index[c.target] for c in conns], np.int32), rt.window,
+                    rt.canon.src_ptr.cpu().numpy().astype(np.int64), rt.canon.tgt.cpu().numpy(),
+                    rt.canon.w.cpu().numpy(), seed=int(net.seed), instance=int(net.instance))
+
+
+def bench_brunel_exact(args) -> int:
+    """The event-driven, exact-time Brunel model A (the reference's own Brunel network, test/networks/Brunel):
+    N = N_E + N_I exact-time delta LIF neurons and N_E Poisson sources, fixed in-degree C_E = 0.1 N_E, one delay
+    D = 1.5 ms.  A step is one window of D (a deliver and a sort launch); --ev-warmup windows, then
+    --ev-windows timed in captured windows with recording off; a split run with HIP events around each kernel
+    for the deliver kernel's roofline; the CPU oracle from the GPU's state after warmup."""
+    from naer2_amd.networks import brunel_exact, brunel_exact_theory
+
+    N = args.neurons or 50000
+    t_build = time.perf_counter()
+    net, pops = brunel_exact(N, seed=args.seed, device="cuda")
+    net.record = False
+    net.keep_canonical = not args.no_cpu
+    rt = net.build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    W = rt.window
+    print(f"[bench] built brunel_exact: {rt.n} nodes ({N} neurons), {rt.n_links} synapses, {rt.n_groups} groups "
+          f"in {t_build:.1f} s", file=sys.stderr, flush=True)
+    gw = args.ev_graph
+    warm, n_win = args.ev_warmup, args.ev_windows
+    rt.run(warm - 1, gw)
+    net.record = True
+    rt.run(1, 0)  # the last warmup window recorded: the CPU oracle starts from it
+    net.record = False
+    torch.cuda.synchronize()
+    prev_t, prev_n = rt.spikes()
+    snap = (rt.v.clone(), rt.last.clone(), rt.tre.clone(), rt.nxt.clone())
+    print(f"[bench] warmup {warm} windows done", file=sys.stderr, flush=True)
+    c0 = rt.counters()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    rt.run(n_win, gw)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    c1 = rt.counters()
+    rt.check()
+    events, spikes = c1[0] - c0[0], c1[1] - c0[1]
+    bio_s = n_win * W / 1000.0
+    rtf = bio_s / wall
+    print(f"[bench] timed {n_win} windows: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream", file=sys.stderr,
+          flush=True)
+    # the deliver kernel alone: HIP events around each kernel of a split run
+    n_split = min(100, n_win)
+    c2 = rt.counters()
+    d_ms, s_ms = rt.run_timed(n_split)
+    c3 = rt.counters()
+    ev_split, sp_split = c3[0] - c2[0], c3[1] - c2[1]
+    n_nrn = sum(p.n for p in pops.values() if p.family == "event_lif")
+    n_src = sum(p.n for p in pops.values() if p.family == "event_poisson")
+    g_nrn = sum((p.n + 63) // 64 for p in pops.values() if p.family == "event_lif")
+    # algorithmic bytes of one window's deliver launch: 8 B per link walked, 24 B per (input spike, neuron group)
+    # (the stream entry and the two segment offsets), 24 B of neuron state read and written, 8 B per source's
+    # next spike, 8 B per spike emitted into a region
+    bytes_win = (8.0 * ev_split + 24.0 * g_nrn * sp_split + 8.0 * sp_split) / n_split + 24.0 * n_nrn + 8.0 * n_src
+    k_ms = d_ms / n_split
+    achieved = bytes_win / (k_ms * 1e-3) / 1e9
+    traffic = occupancy = None
+    tfile = ROOT / "profiles" / "brunel_exact_traffic.json"
+    if tfile.exists() and N == 50000:
+        prof = json.loads(tfile.read_text())
+        traffic, occupancy = prof.get("hbm_bytes_per_launch"), prof.get("occupancy")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import event as ev
+
+        evn = _event_oracle_net(net, rt)
+        st = ev.EvState(snap[0].cpu().numpy(), snap[1].cpu().numpy(), snap[2].cpu().numpy(), snap[3].cpu().numpy(),
+                        prev_n.astype(np.int32), prev_t.astype(np.float32), warm)
+        k = args.ev_cpu_windows
+        tc = time.perf_counter()
+        _, _, cnt, ev_cpu = ev.run(evn, st, k, record=False)
+        tc = time.perf_counter() - tc
+        cpu = {"value": round(k * W / 1000.0 / tc, 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
+               "sample": f"{k} windows ({k * W:.1f} ms bio) of the same network from the GPU's state after warmup, "
+                         f"serial C oracle (oracle/event_oracle.c), {ev_cpu} inputs, {tc:.2f} s",
+               "events_per_s": round(ev_cpu / tc, 1), "threads_available": os.cpu_count()}
+    # steady-state rates: 100 ms more, recorded
+    t_rec = rt.window_index * W
+    net.record = True
+    rt.run(67, 0)
+    net.record = False
+    r = rt.rates(t_skip_ms=t_rec)
+    rates = {name: r[p] for name, p in pops.items()}
+    ref_rtf = 1.0 / 2.94
+    line = {
+        "metric": "event-driven Brunel real-time factor + synaptic events/s",
+        "value": round(rtf, 4),
+        "unit": "bio_s/wall_s",
+        "n_gpus": 1,
+        "steps": n_win,
+        "warmup": warm,
+        "ms_per_step": round(wall * 1000.0 / n_win, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(rtf / ref_rtf, 2) if N == 50000 else None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"brunel_exact: event-driven Brunel model A with exact spike times, N {N} "
+                               f"(N_E {pops['E'].n}, N_I {pops['I'].n}), {pops['P'].n} Poisson sources at "
+                               f"{brunel_exact_theory(N)['rate_ext_Hz']:.1f} Hz, C_E {brunel_exact_theory(N)['C_E']}, "
+                               f"J 0.1 mV, g 5, D 1.5 ms, {rt.n_links} synapses; a step = one 1.5 ms window; "
+                               f"{n_win} windows timed, seed {args.seed}",
+                   "n_neurons": N, "n_synapses": rt.n_links, "window_ms": W, "graph_window": gw,
+                   "parallelism": "single GPU"},
+        "vs_baseline_note": "reference: N = 50,000 at 2.94 ms wall per simulated ms on an RTX 4090, captured "
+                            "(test/networks/Brunel/net.py:77-78)",
+        "events_per_s": round(events / wall, 1),
+        "spikes_per_window": round(spikes / n_win, 1),
+        "events_per_window": round(events / n_win, 1),
+        "rates_hz": {name: round(float(v), 3) for name, v in rates.items()},
+        "kernel_split_ms_per_window": {"deliver": round(d_ms / n_split, 4), "sort": round(s_ms / n_split, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel": "ev_deliver_kernel", "kernel_avg_us": round(k_ms * 1000.0, 3),
+                     "occupancy": occupancy, "algorithmic_bytes_per_launch": round(bytes_win, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_bptt(args) -> int:
     """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
     1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
@@ -448,9 +597,16 @@ def main() -> int:
                     help="scales the CPU samples (PD14: 2000 steps on one thread, 10x that on every core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad"], default="pd14",
+    ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad", "brunel_exact"],
+                    default="pd14",
                     help="pd14: the headline PD14 real-time factor; brunel: the 2500-neuron Brunel network; sweep: the batched "
-                         "Brunel sweep; bptt: the recurrent-LIF BPTT; pd14grad: the batched differentiable PD14")
+                         "Brunel sweep; bptt: the recurrent-LIF BPTT; pd14grad: the batched differentiable PD14; "
+                         "brunel_exact: the event-driven Brunel model with exact spike times")
+    ap.add_argument("--neurons", type=int, default=None, help="brunel_exact: N = N_E + N_I (default 50,000)")
+    ap.add_argument("--ev-windows", type=int, default=667, help="brunel_exact: timed windows of 1.5 ms")
+    ap.add_argument("--ev-warmup", type=int, default=67, help="brunel_exact: untimed windows first")
+    ap.add_argument("--ev-graph", type=int, default=23, help="brunel_exact: windows per captured hipGraph")
+    ap.add_argument("--ev-cpu-windows", type=int, default=10, help="brunel_exact: windows the CPU oracle runs")
     ap.add_argument("--instances", type=int, default=None, help="networks (sweep: 256, pd14grad: 4)")
     ap.add_argument("--checkpoint", type=int, default=500, help="pd14grad: steps between saved states")
     ap.add_argument("--cutoff", type=float, default=0.2,
@@ -480,6 +636,8 @@ def main() -> int:
         return bench_bptt(args)
     if args.workload == "pd14grad":
         return bench_pd14grad(args)
+    if args.workload == "brunel_exact":
+        return bench_brunel_exact(args)
     if args.window <= 0 or args.window % 4 or args.steps % args.window or args.warmup % args.window:
         print("--window must be a multiple of 4, --steps and --warmup multiples of --window", file=sys.stderr)
         return 2

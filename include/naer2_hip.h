@@ -361,6 +361,13 @@ int n2h_ev_graph_launch(n2h_ev_graph* graph, void* stream);
 int n2h_ev_graph_destroy(n2h_ev_graph* graph);
 /* flags[0 .. n) = the sticky N2H_EV_FLAG_* words (synchronizes). */
 int n2h_ev_status(n2h_ev_plan* plan, int32_t* flags, int32_t n);
+/* counts[0] = inputs delivered (links walked), counts[1] = spikes emitted, since the plan was made
+ * (synchronizes). */
+int n2h_ev_counters(n2h_ev_plan* plan, int64_t* counts, int32_t n);
+/* n2h_ev_run with a HIP event around each kernel: the summed durations of the deliver and the sort kernels
+ * (synchronizes; not capturable).  The reference's in-kernel phase trace (naer2/trace.py) is the nearest
+ * counterpart. */
+int n2h_ev_run_timed(n2h_ev_plan* plan, int32_t n_windows, void* stream, double* deliver_ms, double* sort_ms);
 /* The library's fp32 exp and log over n inputs (no counterpart in the reference: a check of the functions
  * that the event path's results depend on). */
 int n2h_ev_math(const float* x, float* exp_x, const float* u, float* log_u, int64_t n, void* stream);

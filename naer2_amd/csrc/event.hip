@@ -40,8 +40,11 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kEnt = 256;          // input-stream entries a group takes per round (4 per lane)
+constexpr int kEnt = 256;          // input-stream entries a group takes per round (one per thread)
 constexpr int kCapL = 2048;        // links a group stages per round (LDS: 8 B staged + 8 B inbox each)
+constexpr int kLinkU = 8;          // link loads per lane in flight while staging
+constexpr int kWaves = 4;          // wavefronts per group's workgroup (staging); wave 0 walks
+constexpr int kThreads = kWaves * kWave;
 constexpr int kSortThreads = 1024;
 
 thread_local char g_err[512] = "";
@@ -113,6 +116,7 @@ struct EvArgs {
     int32_t* rec_count;
     int32_t* rec_wcount;
     int64_t rec_cap;
+    unsigned long long* counters; // [2]: inputs delivered, spikes emitted (since the plan was made)
     float W;
     uint32_t key0, key1, instance;
 };
@@ -206,34 +210,42 @@ __device__ __forceinline__ void emit(const EvArgs& a, int64_t rbase, int cap, in
 }
 
 struct DeliverLds {
-    int32_t off[kEnt + 1];   // the round's entries: first staged link (exclusive scan), off[r] = links
-    int64_t lo[kEnt];        // first link of the entry's segment
-    float a[kEnt];           // arrival time
-    uint32_t st_t[kCapL];    // staged link: target lane | entry << 8
+    int32_t off[kEnt + 1];          // the round's entries: first staged link (exclusive scan), off[r] = links
+    int64_t lo[kEnt];               // first link of the entry's segment
+    float a[kEnt];                  // arrival time
+    uint32_t st_t[kCapL];           // staged link: target lane | entry << 8
     float st_w[kCapL];
-    float in_w[kCapL];       // inboxes: the links by target lane, stable
+    float in_w[kCapL];              // inboxes: the links by target lane, stable
     float in_a[kCapL];
-    int32_t cnt[kWave];
-    int32_t start[kWave];
-    int32_t run[kWave];
+    int32_t cnt[kWaves][kWave];     // links per (staging wave, target lane)
+    int32_t wstart[kWaves][kWave];  // a wave's first inbox place per target lane
+    int32_t wrun[kWaves][kWave];    // links a wave has placed per target lane
+    int32_t scan[kWaves + 1];
+    int32_t T;
+    int32_t depth;
 };
 
-// One group of a window.  Neuron groups: the input stream, then the final threshold; source groups: the
-// window's draws.  Region count and state written last.
-__global__ __launch_bounds__(kWave) void ev_deliver_kernel(EvArgs a) {
+// One group of a window, one workgroup of kWaves wavefronts.  Neuron groups: all waves stage each round of the
+// input stream (every wave a contiguous quarter of the round's links, so the per-target order stays the link
+// order), wave 0 -- one lane per neuron -- walks the inboxes; then the final threshold.  Source groups: wave 0
+// draws the window's spikes.  Region count and state written last.
+__global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
     __shared__ DeliverLds s;
     const int g = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
     const int info = a.group_info[g];
     const int cnt = info & 0xFFFF;
     const EvPopDev P = a.pops[info >> 16];
     const int64_t k = *a.window;
     const int64_t rbase = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
-    const bool act = lane < cnt;
+    const bool act = wv == 0 && lane < cnt;
     const int node = a.group_first[g] + lane;
     int emitted = 0;
 
     if (P.kind == N2H_EV_POISSON) {
+        if (wv) return;
         const float t_end = window_time(k + 1, a.W);
         float next = act ? a.nxt[node] : 0.0f;
         uint32_t m = 0;
@@ -246,6 +258,7 @@ __global__ __launch_bounds__(kWave) void ev_deliver_kernel(EvArgs a) {
         }
         if (act) a.nxt[node] = next;
     } else if (P.kind == N2H_EV_TRAIN) {
+        if (wv) return;
         const float t_start = window_time(k, a.W), t_end = window_time(k + 1, a.W);
         int64_t j = 0, hi = 0;
         if (act) {
@@ -280,130 +293,163 @@ __global__ __launch_bounds__(kWave) void ev_deliver_kernel(EvArgs a) {
             const int32_t n_s = a.stream_count[P.stream];
             const int32_t* sslot = a.stream_slot + S.base;
             const float* sa = a.stream_a + S.base;
-            s.cnt[lane] = 0;
-            s.run[lane] = 0;
+            s.cnt[wv][lane] = 0;
+            s.wrun[wv][lane] = 0;
             const uint64_t lt = (1ull << lane) - 1ull;
             for (int base = 0; base < n_s;) {
-                // this round's entries: 4 per lane, their segments into the group
-                int nl[4], incl[4];
-                int64_t lo[4];
-                float ae[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int idx = base + j * kWave + lane;
-                    const bool ok = idx < n_s;
-                    const int32_t slot = ok ? sslot[idx] : 0;
-                    ae[j] = ok ? sa[idx] : 0.0f;
-                    const int64_t seg = static_cast<int64_t>(slot) * a.n_groups + g;
-                    lo[j] = ok ? a.seg_off[seg] : 0;
-                    const int64_t hi = ok ? a.seg_off[seg + 1] : 0;
-                    nl[j] = static_cast<int>(hi - lo[j]);
-                }
+                // this round's entries, one per thread: the segment of the spike into the group
+                const int idx = base + tid;
+                const bool ok = idx < n_s;
+                const int32_t slot = ok ? sslot[idx] : 0;
+                const float ae = ok ? sa[idx] : 0.0f;
+                const int64_t seg = static_cast<int64_t>(slot) * a.n_groups + g;
+                const int64_t lo = ok ? a.seg_off[seg] : 0;
+                const int nl = ok ? static_cast<int>(a.seg_off[seg + 1] - lo) : 0;
+                // inclusive scan over the workgroup (entry order = thread order)
+                const int x = wave_incl_scan(nl, lane);
+                if (lane == kWave - 1) s.scan[wv] = x;
+                __syncthreads();
                 int carry = 0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int x = wave_incl_scan(nl[j], lane);
-                    incl[j] = x + carry;
-                    carry += __shfl(x, kWave - 1, kWave);
-                }
+                for (int w = 0; w < kWaves; ++w)
+                    if (w < wv) carry += s.scan[w];
+                const int incl = x + carry;
                 // the longest prefix of entries whose links fit in LDS (incl is non-decreasing in entry order)
-                int r = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    r += __popcll(__ballot(incl[j] <= kCapL && base + j * kWave + lane < n_s));
+                int r = __syncthreads_count(ok && incl <= kCapL);
+                if (r > 0 && tid == r - 1) s.T = incl;
+                if (r > 0 && tid < r) {
+                    s.off[tid] = incl - nl;
+                    s.lo[tid] = lo;
+                    s.a[tid] = ae;
+                }
+                __syncthreads();
                 int T;
                 if (r == 0) {  // one segment longer than the stage: its first kCapL links only, flagged
                     r = 1;
                     T = kCapL;
-                    if (lane == 0) atomicOr(&a.flags[N2H_EV_FLAG_SEGMENT], 1);
-                } else {
-                    const int jl = (r - 1) >> 6, ll = (r - 1) & 63;
-                    int v = incl[0];
-                    if (jl == 1) v = incl[1];
-                    if (jl == 2) v = incl[2];
-                    if (jl == 3) v = incl[3];
-                    T = __shfl(v, ll, kWave);
-                }
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int e = j * kWave + lane;
-                    if (e < r) {
-                        s.off[e] = incl[j] - nl[j];
-                        s.lo[e] = lo[j];
-                        s.a[e] = ae[j];
+                    if (tid == 0) {
+                        s.off[0] = 0;
+                        s.lo[0] = lo;
+                        s.a[0] = ae;
+                        atomicOr(&a.flags[N2H_EV_FLAG_SEGMENT], 1);
                     }
+                } else {
+                    T = s.T;
                 }
-                if (lane == 0) s.off[r] = T;
+                if (tid == 0) {
+                    s.off[r] = T;
+                    if (T) atomicAdd(&a.counters[0], static_cast<unsigned long long>(T));
+                }
                 __syncthreads();
-                // pass 1: every link staged with its target lane and entry, counted per target
-                for (int f0 = 0; f0 < T; f0 += kWave) {
-                    const int f = f0 + lane;
-                    if (f < T) {
-                        int pos = 0;
+                // pass 1: each wave stages its quarter of the links, counts per target lane; kLinkU loads per
+                // lane in flight (indices clamped into the quarter, results past its end ignored)
+                const int Q = ((T + kThreads - 1) / kThreads) * kWave;
+                const int q0 = wv * Q, q1 = min(q0 + Q, T);
+                for (int f0 = q0; f0 < q1; f0 += kWave * kLinkU) {
+                    int pos[kLinkU];
+                    uint2 lk[kLinkU];
+#pragma unroll
+                    for (int u = 0; u < kLinkU; ++u) {
+                        const int f = min(f0 + u * kWave + lane, q1 - 1);
+                        int p = 0;
 #pragma unroll
                         for (int step = kEnt / 2; step > 0; step >>= 1)
-                            if (pos + step < r && s.off[pos + step] <= f) pos += step;
-                        const uint2 lk = a.links[s.lo[pos] + (f - s.off[pos])];
-                        const uint32_t t = lk.x & 63u;  // (the target lane: x & 0xffff < 64 by construction)
-                        s.st_t[f] = t | (static_cast<uint32_t>(pos) << 8);
-                        s.st_w[f] = __uint_as_float(lk.y);
-                        atomicAdd(&s.cnt[t], 1);
+                            if (p + step < r && s.off[p + step] <= f) p += step;
+                        pos[u] = p;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLinkU; ++u) {
+                        const int f = min(f0 + u * kWave + lane, q1 - 1);
+                        lk[u] = a.links[s.lo[pos[u]] + (f - s.off[pos[u]])];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLinkU; ++u) {
+                        const int f = f0 + u * kWave + lane;
+                        if (f < q1) {
+                            const uint32_t t = lk[u].x & 63u;  // (the target lane: x & 0xffff < 64 by construction)
+                            s.st_t[f] = t | (static_cast<uint32_t>(pos[u]) << 8);
+                            s.st_w[f] = __uint_as_float(lk[u].y);
+                            atomicAdd(&s.cnt[wv][t], 1);
+                        }
                     }
                 }
                 __syncthreads();
-                const int c = s.cnt[lane];
-                const int st = wave_incl_scan(c, lane) - c;
-                s.start[lane] = st;
-                const int depth = wave_max(c);
+                // inbox places: target lane t's inbox starts at the links to lanes < t; within it, wave w's
+                // links follow those of waves < w
+                int c = 0;
+                if (wv == 0) {
+                    int tot = 0;
+#pragma unroll
+                    for (int w = 0; w < kWaves; ++w) tot += s.cnt[w][lane];
+                    int st = wave_incl_scan(tot, lane) - tot;
+#pragma unroll
+                    for (int w = 0; w < kWaves; ++w) {
+                        s.wstart[w][lane] = st;
+                        st += s.cnt[w][lane];
+                    }
+                    c = tot;
+                    const int d = wave_max(tot);
+                    if (lane == 0) s.depth = d;
+                }
                 __syncthreads();
-                // pass 2: each link to its place in its target's inbox, stable in link order
-                for (int f0 = 0; f0 < T; f0 += kWave) {
-                    const int f = f0 + lane;
-                    const bool ok = f < T;
-                    const uint32_t tw = ok ? s.st_t[f] : 0u;
+                // pass 2: each wave places its quarter in order, stable ranks among equal targets per chunk
+                for (int f0 = 0; f0 < Q; f0 += kWave) {
+                    const int f = q0 + f0 + lane;
+                    const bool fo = f < q1;
+                    const uint32_t tw = fo ? s.st_t[f] : 0u;
                     const uint32_t t = tw & 0xFFu;
-                    uint64_t match = __ballot(ok);
+                    uint64_t match = __ballot(fo);
 #pragma unroll
                     for (int b = 0; b < 6; ++b) {
-                        const uint64_t bb = __ballot(ok && ((t >> b) & 1u));
+                        const uint64_t bb = __ballot(fo && ((t >> b) & 1u));
                         match &= ((t >> b) & 1u) ? bb : ~bb;
                     }
-                    if (ok) {
-                        const int p = s.start[t] + s.run[t] + __popcll(match & lt);
+                    if (fo) {
+                        const int p = s.wstart[wv][t] + s.wrun[wv][t] + __popcll(match & lt);
                         s.in_w[p] = s.st_w[f];
                         s.in_a[p] = s.a[tw >> 8];
                     }
-                    __syncthreads();
-                    if (ok && (match >> lane) == 1ull) s.run[t] += __popcll(match);  // the last lane of its target
-                    __syncthreads();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (fo && (match >> lane) == 1ull) s.wrun[wv][t] += __popcll(match);  // the last lane of t
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                // the walk: every lane its neuron's inputs of the round, in order
-                for (int q = 0; q < depth; ++q) {
-                    const bool has = act && q < c;
-                    const float w = has ? s.in_w[st + q] : 0.0f;
-                    const float ar = has ? s.in_a[st + q] : 0.0f;
-                    const bool fire = has && pend && ar != aprev && V >= P.theta;
-                    if (fire) {
-                        V = P.v_reset;
-                        tre = aprev + P.t_ref;
-                        last = tre;
-                    }
-                    emit(a, rbase, P.cap, emitted, fire, lane, aprev);
-                    if (has) {
-                        if (ar >= tre) {
-                            V = V * ev_exp((last - ar) * P.tau_inv) + w;
-                            last = ar;
+                __syncthreads();
+                // the walk: wave 0, every lane its neuron's inputs of the round, in order
+                if (wv == 0) {
+                    const int depth = s.depth;
+                    const int st = s.wstart[0][lane];
+                    for (int q = 0; q < depth; ++q) {
+                        const bool has = act && q < c;
+                        const float w = has ? s.in_w[st + q] : 0.0f;
+                        const float ar = has ? s.in_a[st + q] : 0.0f;
+                        const bool fire = has && pend && ar != aprev && V >= P.theta;
+                        if (fire) {
+                            V = P.v_reset;
+                            tre = aprev + P.t_ref;
+                            last = tre;
                         }
-                        aprev = ar;
-                        pend = true;
+                        emit(a, rbase, P.cap, emitted, fire, lane, aprev);
+                        if (has) {
+                            if (ar >= tre) {
+                                V = V * ev_exp((last - ar) * P.tau_inv) + w;
+                                last = ar;
+                            }
+                            aprev = ar;
+                            pend = true;
+                        }
                     }
                 }
-                s.cnt[lane] = 0;
-                s.run[lane] = 0;
+                s.cnt[wv][lane] = 0;
+                s.wrun[wv][lane] = 0;
                 __syncthreads();
                 base += r;
             }
         }
+        if (wv) return;
         const bool fire = act && pend && V >= P.theta;
         if (fire) {
             V = P.v_reset;
@@ -417,7 +463,7 @@ __global__ __launch_bounds__(kWave) void ev_deliver_kernel(EvArgs a) {
             a.tre[node] = tre;
         }
     }
-    if (lane == 0) {
+    if (tid == 0) {
         a.reg_count[g] = min(emitted, P.cap);
         if (emitted > P.cap) atomicOr(&a.flags[N2H_EV_FLAG_REGION], 1);
     }
@@ -534,8 +580,10 @@ template <int L>
 __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     extern __shared__ unsigned long long s_k[];
     __shared__ int s_w[kSortThreads / kWave + 1];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    __shared__ EvPopDev s_pops[N2H_MAX_POPS];
+    const int tid = threadIdx.x;
     const int64_t k = *a.window;
+    if (tid < a.n_pops) s_pops[tid] = a.pops[tid];
     // 1. every group's region: its place in the window's list
     const int G = a.n_groups;
     const int per = (G + kSortThreads - 1) / kSortThreads;
@@ -558,16 +606,17 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     int64_t n2 = pow2_at_least(S > 0 ? S : 1);
     bool in_lds = n2 <= L;
     unsigned long long* keys = in_lds ? s_k : a.keys;
-    for (int g = wv; g < G; g += kSortThreads / kWave) {
+    // one thread per group (a group holds few spikes of a window): its region's entries to their keys
+    for (int g = tid; g < G; g += kSortThreads) {
         const int info = a.group_info[g];
-        const int pop = info >> 16;
         const int c = a.reg_count[g];
+        if (c == 0) continue;
+        const int pop = info >> 16;
         const int b = a.group_prefix[g];
-        const EvPopDev& P = a.pops[pop];
+        const EvPopDev& P = s_pops[pop];
         const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
-        const uint32_t local0 = static_cast<uint32_t>(a.group_first[g] - P.first);
-        for (int j = lane; j < c; j += kWave) {
-            if (b + j >= S) break;
+        const uint32_t local0 = static_cast<uint32_t>(g - P.group0) * kWave;
+        for (int j = 0; j < c && b + j < S; ++j) {
             const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
             keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
                           (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
@@ -589,11 +638,12 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         a.sorted[i] = key;
         if (rec_ok) {
             const int pop = static_cast<int>(key >> 56);
-            a.rec_node[rc0 + i] = a.pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
+            a.rec_node[rc0 + i] = s_pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
             a.rec_e[rc0 + i] = __uint_as_float(static_cast<uint32_t>((key >> 24) & 0xFFFFFFFFull));
         }
     }
     __syncthreads();
+    if (tid == 0) atomicAdd(&a.counters[1], static_cast<unsigned long long>(S));
     if (tid == 0 && rec) {
         if (rec_ok) *a.rec_count = rc0 + S;
         else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
@@ -605,7 +655,7 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         int base[N2H_EV_MAX_SOURCES + 1], off[N2H_EV_MAX_SOURCES];
         base[0] = 0;
         for (int j = 0; j < St.n_src; ++j) {
-            const EvPopDev& Q = a.pops[St.src[j]];
+            const EvPopDev& Q = s_pops[St.src[j]];
             off[j] = a.group_prefix[Q.group0];
             base[j + 1] = base[j] + (a.group_prefix[Q.group0 + Q.n_groups] - off[j]);
         }
@@ -630,7 +680,7 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
             int j = 0;
             while (pos >= base[j + 1]) ++j;
             const unsigned long long ka = a.sorted[off[j] + (pos - base[j])];
-            const EvPopDev& Q = a.pops[static_cast<int>(ka >> 56)];
+            const EvPopDev& Q = s_pops[static_cast<int>(ka >> 56)];
             const int local = static_cast<int>(ka & 0xFFFFFFull);
             a.stream_slot[St.base + i] = (Q.group0 + local / kWave) * kWave + (local & (kWave - 1));
             a.stream_a[St.base + i] = __uint_as_float(static_cast<uint32_t>(kb >> 32));
@@ -676,6 +726,7 @@ struct n2h_ev_plan {
     int32_t* group_prefix = nullptr;
     int64_t* window = nullptr;
     int32_t* flags = nullptr;
+    unsigned long long* counters = nullptr;
     int64_t host_window = 0;
     bool bound = false;
 };
@@ -692,7 +743,7 @@ namespace {
 void free_plan(n2h_ev_plan* p) {
     if (!p) return;
     void* bufs[] = {p->pops_dev, p->streams_dev, p->reg_lane, p->reg_e, p->reg_count, p->stream_count, p->stream_slot,
-                    p->stream_a, p->keys, p->sorted, p->group_prefix, p->window, p->flags};
+                    p->stream_a, p->keys, p->sorted, p->group_prefix, p->window, p->flags, p->counters};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -708,15 +759,20 @@ int dev_alloc(T** out, int64_t count) {
     return N2H_OK;
 }
 
-int launch_windows(n2h_ev_plan* p, int32_t n, hipStream_t stream) {
-    const dim3 grid(static_cast<unsigned>(p->args.n_groups)), block(kWave);
+// Enqueue n windows.  ev: null, or 3 events per window recorded before the deliver kernel, between the two
+// kernels and after the sort kernel (timing).
+int launch_windows(n2h_ev_plan* p, int32_t n, hipStream_t stream, hipEvent_t* ev = nullptr) {
+    const dim3 grid(static_cast<unsigned>(p->args.n_groups)), block(kThreads);
     const size_t lds = static_cast<size_t>(p->sort_lds) * sizeof(unsigned long long);
     for (int32_t i = 0; i < n; ++i) {
+        if (ev) EV_HIP(hipEventRecord(ev[3 * i], stream));
         hipLaunchKernelGGL(ev_deliver_kernel, grid, block, 0, stream, p->args);
+        if (ev) EV_HIP(hipEventRecord(ev[3 * i + 1], stream));
         if (p->sort_lds == 8192)
             hipLaunchKernelGGL(ev_sort_kernel<8192>, dim3(1), dim3(kSortThreads), lds, stream, p->args);
         else
             hipLaunchKernelGGL(ev_sort_kernel<4096>, dim3(1), dim3(kSortThreads), lds, stream, p->args);
+        if (ev) EV_HIP(hipEventRecord(ev[3 * i + 2], stream));
     }
     EV_HIP(hipGetLastError());
     return N2H_OK;
@@ -869,7 +925,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
         (rc = dev_alloc(&p->stream_slot, stream_total)) || (rc = dev_alloc(&p->stream_a, stream_total)) ||
         (rc = dev_alloc(&p->keys, keys_cap)) || (rc = dev_alloc(&p->sorted, keys_cap)) ||
         (rc = dev_alloc(&p->group_prefix, d->n_groups + 1)) || (rc = dev_alloc(&p->window, 1)) ||
-        (rc = dev_alloc(&p->flags, N2H_EV_FLAGS))) {
+        (rc = dev_alloc(&p->flags, N2H_EV_FLAGS)) || (rc = dev_alloc(&p->counters, 2))) {
         free_plan(p);
         return rc;
     }
@@ -905,6 +961,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
     a.group_prefix = p->group_prefix;
     a.window = p->window;
     a.flags = p->flags;
+    a.counters = p->counters;
     a.W = d->window;
     a.key0 = static_cast<uint32_t>(d->seed);
     a.key1 = static_cast<uint32_t>(d->seed >> 32);
@@ -1034,6 +1091,41 @@ int n2h_ev_status(n2h_ev_plan* p, int32_t* flags, int32_t n) {
     int32_t f[N2H_EV_FLAGS];
     EV_HIP(hipMemcpy(f, p->flags, sizeof(f), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) flags[i] = i < N2H_EV_FLAGS ? f[i] : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_counters(n2h_ev_plan* p, int64_t* counts, int32_t n) {
+    if (!p || !counts || n < 1) return fail(N2H_EINVAL, "bad argument");
+    unsigned long long c[2];
+    EV_HIP(hipMemcpy(c, p->counters, sizeof(c), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) counts[i] = i < 2 ? static_cast<int64_t>(c[i]) : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_run_timed(n2h_ev_plan* p, int32_t n_windows, void* stream, double* deliver_ms, double* sort_ms) {
+    if (!p || n_windows < 1 || !deliver_ms || !sort_ms) return fail(N2H_EINVAL, "bad argument");
+    if (!p->bound) return fail(N2H_ESTATE, "state not bound (n2h_ev_bind_state)");
+    std::vector<hipEvent_t> ev(static_cast<size_t>(3 * n_windows), nullptr);
+    int rc = N2H_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(N2H_EHIP, "hipEventCreate");
+    if (!rc) rc = launch_windows(p, n_windows, static_cast<hipStream_t>(stream), ev.data());
+    if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(N2H_EHIP, "hipEventSynchronize");
+    double d = 0.0, s = 0.0;
+    for (int32_t i = 0; !rc && i < n_windows; ++i) {
+        float a = 0.0f, b = 0.0f;
+        if (hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]) != hipSuccess)
+            rc = fail(N2H_EHIP, "hipEventElapsedTime");
+        d += a;
+        s += b;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    p->host_window += n_windows;
+    *deliver_ms = d;
+    *sort_ms = s;
     return N2H_OK;
 }
 

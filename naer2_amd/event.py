@@ -312,7 +312,8 @@ class EventRuntime:
             self.window_index += n_windows
             self.record_from = self.window_index
             return
-        chunk = max(1, min(n_windows, max(graph_window, 1) * max(1, 200 // max(graph_window, 1)), 200))
+        gw = graph_window if 0 < graph_window <= 200 else 0
+        chunk = min(n_windows, (200 // gw) * gw if gw else 200)
         self._bind_record(chunk)
         done = 0
         while done < n_windows:
@@ -328,6 +329,24 @@ class EventRuntime:
         if abs(k - round(k)) > 1e-6 * max(1.0, abs(k)):
             raise ValueError(f"{t_ms} ms is not a whole number of windows of {self.window} ms")
         self.run(int(round(k)), self.net.graph_window)
+
+    def counters(self) -> tuple[int, int]:
+        """(inputs delivered, spikes emitted) since the build (synchronizes)."""
+        c = (ctypes.c_int64 * 2)()
+        _lib.ev_check(_lib.load().n2h_ev_counters(self._plan, c, 2), "n2h_ev_counters")
+        return int(c[0]), int(c[1])
+
+    def run_timed(self, n_windows: int) -> tuple[float, float]:
+        """Run n_windows eager windows with HIP events around each kernel: (deliver ms, sort ms) summed.
+        Recording is switched off for them."""
+        self._unbind_record()
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        d, s = ctypes.c_double(), ctypes.c_double()
+        _lib.ev_check(_lib.load().n2h_ev_run_timed(self._plan, int(n_windows), stream, ctypes.byref(d),
+                                                   ctypes.byref(s)), "n2h_ev_run_timed")
+        self.window_index += n_windows
+        self.record_from = self.window_index
+        return float(d.value), float(s.value)
 
     def check(self) -> None:
         """Raise if a kernel flagged lost spikes or links (the reference's check(), runtime.py:724-735)."""
