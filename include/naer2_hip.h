@@ -259,7 +259,7 @@ int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
  * delta-synapse LIF neurons whose potential changes only at an input's exact arrival time e + D and which
  * fire inside delivery at that time; Poisson sources with exact spike times; one homogeneous delay D on
  * every connection (= the window).  Window k emits spikes; window k + 1 delivers all of them.  Per target,
- * a window's inputs run in ascending (arrival, connection, emission time, source, link); inputs at one arrival
+ * a window's inputs run in ascending (arrival, connection, source, link); inputs at one arrival
  * time are received one by one, then the threshold is checked once (the reference's send after every
  * delivery).  Receive: if a >= t_ref_end: V = V * exp((last - a) / tau) + w, last = a.  Threshold at t:
  * V >= theta: spike at t, V = V_r, t_ref_end = last = t + t_ref.  exp and log are this library's own fp32
@@ -361,6 +361,13 @@ int n2h_ev_graph_launch(n2h_ev_graph* graph, void* stream);
 int n2h_ev_graph_destroy(n2h_ev_graph* graph);
 /* flags[0 .. n) = the sticky N2H_EV_FLAG_* words (synchronizes). */
 int n2h_ev_status(n2h_ev_plan* plan, int32_t* flags, int32_t n);
+/* In-kernel phase trace of the windows that follow (the reference's naer2/trace.py stamps): with a buffer
+ * uint64 [n_groups + 1][8], every neuron group g writes row g for the last window it ran -- its start, the
+ * 100 MHz ticks it spent on the input-stream entries, link staging, inbox places, inbox fill and the walk
+ * (summed over its rounds), its rounds, and its end -- and the sort kernel writes row n_groups: start, group
+ * prefix done, record done, first stream sorted, streams written (three times), end.  NULL: off.  Graphs
+ * captured before the call keep the setting they were captured with. */
+int n2h_ev_set_trace(n2h_ev_plan* plan, uint64_t* buffer);
 /* counts[0] = inputs delivered (links walked), counts[1] = spikes emitted, since the plan was made
  * (synchronizes). */
 int n2h_ev_counters(n2h_ev_plan* plan, int64_t* counts, int32_t n);

@@ -10,15 +10,17 @@
 //      runs its neuron's inputs in order -- receive (decay to the arrival, add the weight, unless refractory),
 //      and the threshold once per distinct arrival time; a source group draws its spikes for the window.
 //      Spikes go to the group's region of the window.
-//   2. ev_sort_kernel, one workgroup of 1024 threads: the window's spikes in (population, time, neuron) order
-//      (recorded, and each source population's list sorted), then one merged stream per distinct set of
-//      source populations: (arrival, connection, time, neuron) order, the input of the next window.
+//   2. ev_sort_kernel, one workgroup of 1024 threads: one merged stream per distinct set of source
+//      populations, in (arrival, connection, neuron) order -- the input of the next window -- by a counting
+//      sort into arrival-time bins; with a record bound, also the window's spikes in (population, time,
+//      neuron) order.
 // Both read the window index from a device counter (the sort advances it), so windows replay in hipGraphs.
 //
 // Reference behaviour followed (MatteoSchmider/naer2):
 //   naer2/transpiler/templates/fused.py:2084-2301 (_pull_queues): a source run's spikes in (e, node) order;
 //     fused.py:1300-1370 (_merge_segments): the connections' queues merged by arrival e + D_c, ties to the
-//     lower connection; fused.py:1421-1526 (_walk_batch): links unpacked per segment, sorted stably into one
+//     lower connection (here a connection's spikes of one arrival time by source neuron: the reference's
+//     (e, node) order differs only when two emission times round to the same fp32 arrival); fused.py:1421-1526 (_walk_batch): links unpacked per segment, sorted stably into one
 //     inbox per neuron, each neuron's inputs at one time received one by one, then its send once;
 //   fused.py:2667-2702 (_emit_delivered) and 2715-2758 (_sort_region): spikes inside delivery at the arrival
 //     time, a region per program, sorted by (e, node);
@@ -46,6 +48,11 @@ constexpr int kLinkU = 8;          // link loads per lane in flight while stagin
 constexpr int kWaves = 4;          // wavefronts per group's workgroup (staging); wave 0 walks
 constexpr int kThreads = kWaves * kWave;
 constexpr int kSortThreads = 1024;
+constexpr int kPerThread = 2;                          // keys per thread in the register sort
+constexpr int kSmallSort = kPerThread * kSortThreads;  // key counts up to this sort in registers (bitonic_small)
+constexpr int kBins = 2048;                            // arrival-time bins of a window's stream sort
+constexpr int kBinCap = 2048;                          // spikes a stream bins in LDS (more: the bitonic network)
+constexpr int kBinSortMax = 16;                        // spikes one bin's thread sorts (more: the bitonic network)
 
 thread_local char g_err[512] = "";
 
@@ -106,7 +113,6 @@ struct EvArgs {
     int32_t* stream_slot;
     float* stream_a;
     unsigned long long* keys;     // [keys_cap]: sort scratch beyond LDS
-    unsigned long long* sorted;   // [keys_cap]: the window's spikes, (population, time, neuron) keys
     int64_t keys_cap;
     int32_t* group_prefix;        // [n_groups + 1]
     int64_t* window;              // the device window counter
@@ -117,6 +123,7 @@ struct EvArgs {
     int32_t* rec_wcount;
     int64_t rec_cap;
     unsigned long long* counters; // [2]: inputs delivered, spikes emitted (since the plan was made)
+    unsigned long long* trace;    // null, or [n_groups + 1][8] phase stamps of the last window (100 MHz clock)
     float W;
     uint32_t key0, key1, instance;
 };
@@ -124,21 +131,39 @@ struct EvArgs {
 // ---------------------------------------------------------------- the contract's fp32 math
 
 // exp(x) for x <= 88: x = n ln2 + r (Cody-Waite: n * 0.693145751953125 is exact), the degree-7 Taylor
-// polynomial of exp(r) by Horner, times 2^n; 0 below -87.
+// polynomial of exp(r) by Estrin's scheme (a short dependency chain for the walk), times 2^n; 0 below -87.
 __device__ __forceinline__ float ev_exp(float x) {
     if (x < -87.0f) return 0.0f;
     const float n = rintf(x * 1.44269502f);
     float r = x - n * 0.693145751953125f;
     r = r - n * 1.42860677e-06f;
-    float p = 1.98412701e-04f;
-    p = p * r + 1.38888892e-03f;
-    p = p * r + 8.33333377e-03f;
-    p = p * r + 4.16666679e-02f;
-    p = p * r + 1.66666672e-01f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    return ldexpf(p, static_cast<int>(n));
+    const float r2 = r * r;
+    const float r4 = r2 * r2;
+    const float q01 = 1.0f + 1.0f * r;
+    const float q23 = 0.5f + 1.66666672e-01f * r;
+    const float q45 = 4.16666679e-02f + 8.33333377e-03f * r;
+    const float q67 = 1.38888892e-03f + 1.98412701e-04f * r;
+    const float q03 = q01 + r2 * q23;
+    const float q47 = q45 + r2 * q67;
+    return ldexpf(q03 + r4 * q47, static_cast<int>(n));
+}
+
+// ev_exp without a branch: the same value for every finite x (below -87 the reduction still runs, then 0 is
+// selected), for the walk's straight-line loop.
+__device__ __forceinline__ float ev_exp_sel(float x) {
+    const float n = rintf(x * 1.44269502f);
+    float r = x - n * 0.693145751953125f;
+    r = r - n * 1.42860677e-06f;
+    const float r2 = r * r;
+    const float r4 = r2 * r2;
+    const float q01 = 1.0f + 1.0f * r;
+    const float q23 = 0.5f + 1.66666672e-01f * r;
+    const float q45 = 4.16666679e-02f + 8.33333377e-03f * r;
+    const float q67 = 1.38888892e-03f + 1.98412701e-04f * r;
+    const float q03 = q01 + r2 * q23;
+    const float q47 = q45 + r2 * q67;
+    const float y = ldexpf(q03 + r4 * q47, static_cast<int>(fmaxf(n, -1000.0f)));
+    return x < -87.0f ? 0.0f : y;
 }
 
 // log(u) for finite u > 0: u = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = t P(t) with t = m - 1 exact and P a
@@ -296,7 +321,20 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
             s.cnt[wv][lane] = 0;
             s.wrun[wv][lane] = 0;
             const uint64_t lt = (1ull << lane) - 1ull;
+            const bool tr = a.trace != nullptr && tid == 0;
+            unsigned long long t_in = tr ? wall_clock64() : 0ull, t_ph = t_in;
+            unsigned long long ph[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
+            int rounds = 0;
+#define EV_PHASE(i)                                 \
+    do {                                            \
+        if (tr) {                                   \
+            const unsigned long long t_ = wall_clock64(); \
+            ph[(i)] += t_ - t_ph;                   \
+            t_ph = t_;                              \
+        }                                           \
+    } while (0)
             for (int base = 0; base < n_s;) {
+                ++rounds;
                 // this round's entries, one per thread: the segment of the spike into the group
                 const int idx = base + tid;
                 const bool ok = idx < n_s;
@@ -341,6 +379,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                     if (T) atomicAdd(&a.counters[0], static_cast<unsigned long long>(T));
                 }
                 __syncthreads();
+                EV_PHASE(0);
                 // pass 1: each wave stages its quarter of the links, counts per target lane; kLinkU loads per
                 // lane in flight (indices clamped into the quarter, results past its end ignored)
                 const int Q = ((T + kThreads - 1) / kThreads) * kWave;
@@ -374,6 +413,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                     }
                 }
                 __syncthreads();
+                EV_PHASE(1);
                 // inbox places: target lane t's inbox starts at the links to lanes < t; within it, wave w's
                 // links follow those of waves < w
                 int c = 0;
@@ -392,6 +432,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                     if (lane == 0) s.depth = d;
                 }
                 __syncthreads();
+                EV_PHASE(2);
                 // pass 2: each wave places its quarter in order, stable ranks among equal targets per chunk
                 for (int f0 = 0; f0 < Q; f0 += kWave) {
                     const int f = q0 + f0 + lane;
@@ -418,35 +459,50 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
                 __syncthreads();
+                EV_PHASE(3);
                 // the walk: wave 0, every lane its neuron's inputs of the round, in order
                 if (wv == 0) {
                     const int depth = s.depth;
                     const int st = s.wstart[0][lane];
+                    // (the next input read one iteration ahead: the LDS latency off the chain through V)
+                    float w_n = c > 0 ? s.in_w[st] : 0.0f;
+                    float a_n = c > 0 ? s.in_a[st] : 0.0f;
+                    // (branch-free: every lane computes both outcomes and selects)
                     for (int q = 0; q < depth; ++q) {
                         const bool has = act && q < c;
-                        const float w = has ? s.in_w[st + q] : 0.0f;
-                        const float ar = has ? s.in_a[st + q] : 0.0f;
+                        const float w = w_n;
+                        const float ar = a_n;
+                        if (q + 1 < c) {
+                            w_n = s.in_w[st + q + 1];
+                            a_n = s.in_a[st + q + 1];
+                        }
                         const bool fire = has && pend && ar != aprev && V >= P.theta;
-                        if (fire) {
-                            V = P.v_reset;
-                            tre = aprev + P.t_ref;
-                            last = tre;
-                        }
+                        const float t_end_ref = aprev + P.t_ref;
+                        V = fire ? P.v_reset : V;
+                        tre = fire ? t_end_ref : tre;
+                        last = fire ? t_end_ref : last;
                         emit(a, rbase, P.cap, emitted, fire, lane, aprev);
-                        if (has) {
-                            if (ar >= tre) {
-                                V = V * ev_exp((last - ar) * P.tau_inv) + w;
-                                last = ar;
-                            }
-                            aprev = ar;
-                            pend = true;
-                        }
+                        const bool recv = has && ar >= tre;
+                        const float vn = V * ev_exp_sel((last - ar) * P.tau_inv) + w;
+                        V = recv ? vn : V;
+                        last = recv ? ar : last;
+                        aprev = has ? ar : aprev;
+                        pend = pend || has;
                     }
                 }
                 s.cnt[wv][lane] = 0;
                 s.wrun[wv][lane] = 0;
                 __syncthreads();
+                EV_PHASE(4);
                 base += r;
+            }
+#undef EV_PHASE
+            if (tr) {
+                unsigned long long* row = a.trace + static_cast<int64_t>(g) * 8;
+                row[0] = t_in;
+                for (int i = 0; i < 5; ++i) row[1 + i] = ph[i];
+                row[6] = static_cast<unsigned long long>(rounds);
+                row[7] = wall_clock64();
             }
         }
         if (wv) return;
@@ -497,9 +553,80 @@ __device__ void bitonic_local(unsigned long long* s_k, int n, int64_t off, int s
     }
 }
 
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int m) {
+    const int lo = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v)), m, kWave);
+    const int hi = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v >> 32)), m, kWave);
+    return (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+
+// One in-register stage of the bitonic network on a thread's kPerThread consecutive keys (elements
+// kPerThread t .. + kPerThread - 1): pairs STRIDE apart (STRIDE < kPerThread), ascending where
+// (index & size) == 0.
+template <int STRIDE>
+__device__ __forceinline__ void ce_local(unsigned long long (&x)[kPerThread], int i0, int size) {
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        if (j & STRIDE) continue;
+        const bool asc = ((i0 + j) & size) == 0;
+        if ((x[j] > x[j + STRIDE]) == asc) {
+            const unsigned long long t = x[j];
+            x[j] = x[j + STRIDE];
+            x[j + STRIDE] = t;
+        }
+    }
+}
+
+// The bitonic network over kSmallSort keys in s_k, in registers: thread t holds elements kPerThread t ..
+// kPerThread t + kPerThread - 1; pairs less than kPerThread apart meet in the thread, pairs up to
+// 64 kPerThread / 2 apart by lane shuffles within a wavefront, farther pairs through LDS.
+__device__ void bitonic_small(unsigned long long* s_k) {
+    const int t = static_cast<int>(threadIdx.x);
+    const int i0 = kPerThread * t;
+    unsigned long long x[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) x[j] = s_k[i0 + j];
+    for (int size = 2; size <= kSmallSort; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride < kPerThread) {
+                if (stride == 1) ce_local<1>(x, i0, size);
+                if constexpr (kPerThread > 2) {
+                    if (stride == 2) ce_local<2>(x, i0, size);
+                }
+                if constexpr (kPerThread > 4) {
+                    if (stride == 4) ce_local<4>(x, i0, size);
+                }
+            } else {
+                unsigned long long y[kPerThread];
+                if (stride / kPerThread < kWave) {
+#pragma unroll
+                    for (int j = 0; j < kPerThread; ++j) y[j] = shfl_xor64(x[j], stride / kPerThread);
+                } else {
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < kPerThread; ++j) s_k[i0 + j] = x[j];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < kPerThread; ++j) y[j] = s_k[(i0 + j) ^ stride];
+                }
+                const bool take_min = ((i0 & stride) == 0) == ((i0 & size) == 0);
+#pragma unroll
+                for (int j = 0; j < kPerThread; ++j) x[j] = take_min ? min(x[j], y[j]) : max(x[j], y[j]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) s_k[i0 + j] = x[j];
+    __syncthreads();
+}
+
 template <int L>
 __device__ void block_sort(unsigned long long* keys, int64_t n, unsigned long long* s_k, bool in_lds) {
     const int tid = threadIdx.x;
+    if (in_lds && n == kSmallSort) {
+        bitonic_small(s_k);
+        return;
+    }
     if (in_lds) {
         bitonic_local<L>(s_k, static_cast<int>(n), 0, 2, static_cast<int>(n), static_cast<int>(n));
         return;
@@ -576,14 +703,27 @@ __device__ __forceinline__ int64_t pow2_at_least(int64_t n) {
     return p;
 }
 
+// An arrival time's bin among kBins over the window's arrivals: non-decreasing in arr (a subtraction and a
+// multiplication by a positive constant round monotonically), clamped.
+__device__ __forceinline__ int arrival_bin(float arr, float a0, float inv_bw) {
+    const float x = (arr - a0) * inv_bw;
+    return x <= 0.0f ? 0 : (x >= static_cast<float>(kBins - 1) ? kBins - 1 : static_cast<int>(x));
+}
+
 template <int L>
 __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     extern __shared__ unsigned long long s_k[];
     __shared__ int s_w[kSortThreads / kWave + 1];
     __shared__ EvPopDev s_pops[N2H_MAX_POPS];
+    __shared__ int s_bin[kBins + 1];     // a stream's spikes per arrival-time bin, then the bins' offsets
+    __shared__ int s_fill[kBins];
+    __shared__ int s_c[N2H_MAX_POPS];    // a stream's connection index of each population (-1: not a source)
+    __shared__ int s_n;
     const int tid = threadIdx.x;
     const int64_t k = *a.window;
     if (tid < a.n_pops) s_pops[tid] = a.pops[tid];
+    unsigned long long* trow = (a.trace != nullptr && tid == 0) ? a.trace + static_cast<int64_t>(a.n_groups) * 8 : nullptr;
+    if (trow) trow[0] = wall_clock64();
     // 1. every group's region: its place in the window's list
     const int G = a.n_groups;
     const int per = (G + kSortThreads - 1) / kSortThreads;
@@ -598,97 +738,157 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     }
     if (tid == 0) a.group_prefix[G] = S;
     __syncthreads();
+    if (trow) trow[1] = wall_clock64();
     if (S > a.keys_cap) {  // cannot happen: the keys hold every region entry
         if (tid == 0) atomicOr(&a.flags[N2H_EV_FLAG_SORT], 1);
         S = static_cast<int>(a.keys_cap);
     }
-    // 2. keys (population << 56 | time bits << 24 | neuron within its population), sorted
-    int64_t n2 = pow2_at_least(S > 0 ? S : 1);
-    bool in_lds = n2 <= L;
-    unsigned long long* keys = in_lds ? s_k : a.keys;
-    // one thread per group (a group holds few spikes of a window): its region's entries to their keys
-    for (int g = tid; g < G; g += kSortThreads) {
-        const int info = a.group_info[g];
-        const int c = a.reg_count[g];
-        if (c == 0) continue;
-        const int pop = info >> 16;
-        const int b = a.group_prefix[g];
-        const EvPopDev& P = s_pops[pop];
-        const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
-        const uint32_t local0 = static_cast<uint32_t>(g - P.group0) * kWave;
-        for (int j = 0; j < c && b + j < S; ++j) {
-            const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
-            keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
-                          (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+    // 2. the record (when one is bound): the window's spikes in (population, time, neuron) order, keys
+    //    population << 56 | time bits << 24 | neuron within its population
+    if (a.rec_node != nullptr) {
+        int64_t n2 = pow2_at_least(S > 0 ? S : 1);
+        if (n2 < kSmallSort) n2 = kSmallSort;  // (padding sorts in registers)
+        const bool in_lds = n2 <= L;
+        unsigned long long* keys = in_lds ? s_k : a.keys;
+        for (int g = tid; g < G; g += kSortThreads) {  // one thread per group: a group holds few spikes
+            const int c = a.reg_count[g];
+            if (c == 0) continue;
+            const int pop = a.group_info[g] >> 16;
+            const int b = a.group_prefix[g];
+            const EvPopDev& P = s_pops[pop];
+            const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+            const uint32_t local0 = static_cast<uint32_t>(g - P.group0) * kWave;
+            for (int j = 0; j < c && b + j < S; ++j) {
+                const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
+                keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
+                              (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+            }
         }
-    }
-    for (int64_t i = S + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
-    __syncthreads();
-    block_sort<L>(keys, n2, s_k, in_lds);
-    // 3. the record, and the sorted list for the streams
-    const bool rec = a.rec_node != nullptr;
-    int rc0 = 0;
-    bool rec_ok = false;
-    if (rec) {
-        rc0 = *a.rec_count;
-        rec_ok = static_cast<int64_t>(rc0) + S <= a.rec_cap;
-    }
-    for (int i = tid; i < S; i += kSortThreads) {
-        const unsigned long long key = keys[i];
-        a.sorted[i] = key;
-        if (rec_ok) {
+        for (int64_t i = S + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
+        __syncthreads();
+        block_sort<L>(keys, n2, s_k, in_lds);
+        const int rc0 = *a.rec_count;
+        const bool rec_ok = static_cast<int64_t>(rc0) + S <= a.rec_cap;
+        for (int i = tid; rec_ok && i < S; i += kSortThreads) {
+            const unsigned long long key = keys[i];
             const int pop = static_cast<int>(key >> 56);
             a.rec_node[rc0 + i] = s_pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
             a.rec_e[rc0 + i] = __uint_as_float(static_cast<uint32_t>((key >> 24) & 0xFFFFFFFFull));
         }
+        __syncthreads();
+        if (tid == 0) {
+            if (rec_ok) *a.rec_count = rc0 + S;
+            else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
+            a.rec_wcount[k % a.rec_windows] = S;
+        }
     }
-    __syncthreads();
     if (tid == 0) atomicAdd(&a.counters[1], static_cast<unsigned long long>(S));
-    if (tid == 0 && rec) {
-        if (rec_ok) *a.rec_count = rc0 + S;
-        else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
-        a.rec_wcount[k % a.rec_windows] = S;
-    }
-    // 4. one stream per distinct source set: (arrival, connection, time, neuron) order
+    if (trow) trow[2] = wall_clock64();
+    // 3. one stream per distinct source set, in (arrival, connection, neuron) order: keys
+    //    arrival bits << 32 | connection << 28 | source slot.  Up to kBinCap spikes: a counting sort into
+    //    kBins arrival-time bins (monotone in the arrival), then each bin's few keys sorted by one thread;
+    //    more spikes, or a crowded bin: the bitonic network.
+    const float inv_bw = static_cast<float>(kBins) / a.W;
     for (int si = 0; si < a.n_streams; ++si) {
         const EvStreamDev& St = a.streams[si];
-        int base[N2H_EV_MAX_SOURCES + 1], off[N2H_EV_MAX_SOURCES];
-        base[0] = 0;
+        if (tid < a.n_pops) s_c[tid] = -1;
+        if (tid == 0) s_n = 0;
+        for (int b = tid; b < kBins; b += kSortThreads) s_bin[b] = 0;
+        __syncthreads();
+        if (tid < St.n_src) s_c[St.src[tid]] = tid;
+        int ns = 0;
         for (int j = 0; j < St.n_src; ++j) {
             const EvPopDev& Q = s_pops[St.src[j]];
-            off[j] = a.group_prefix[Q.group0];
-            base[j + 1] = base[j] + (a.group_prefix[Q.group0 + Q.n_groups] - off[j]);
+            ns += a.group_prefix[Q.group0 + Q.n_groups] - a.group_prefix[Q.group0];
         }
-        const int ns = base[St.n_src];
-        n2 = pow2_at_least(ns > 0 ? ns : 1);
-        in_lds = n2 <= L;
-        keys = in_lds ? s_k : a.keys;
-        for (int i = tid; i < ns; i += kSortThreads) {
-            int j = 0;
-            while (i >= base[j + 1]) ++j;
-            const unsigned long long ka = a.sorted[off[j] + (i - base[j])];
-            const float e = __uint_as_float(static_cast<uint32_t>((ka >> 24) & 0xFFFFFFFFull));
-            const float arr = e + St.delay;
-            keys[i] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) | static_cast<uint32_t>(i);
-        }
-        for (int64_t i = ns + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
+        const bool binned = ns <= kBinCap;
+        int64_t n2 = pow2_at_least(ns > 0 ? ns : 1);
+        if (n2 < kSmallSort) n2 = kSmallSort;
+        const bool in_lds = binned || n2 <= L;
+        unsigned long long* keys = in_lds ? s_k : a.keys;
+        const float a0 = window_time(k, a.W) + St.delay;
         __syncthreads();
-        block_sort<L>(keys, n2, s_k, in_lds);
+        for (int g = tid; g < G; g += kSortThreads) {
+            const int pop = a.group_info[g] >> 16;
+            const int cix = s_c[pop];
+            const int c = cix < 0 ? 0 : a.reg_count[g];
+            if (c == 0) continue;
+            const EvPopDev& P = s_pops[pop];
+            const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+            const int p0 = atomicAdd(&s_n, c);
+            for (int j = 0; j < c && p0 + j < ns; ++j) {
+                const float arr = a.reg_e[rb + j] + St.delay;
+                const uint32_t slot = static_cast<uint32_t>(g) * kWave + static_cast<uint32_t>(a.reg_lane[rb + j]);
+                keys[p0 + j] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) |
+                               (static_cast<unsigned long long>(cix) << 28) | slot;
+                if (binned) atomicAdd(&s_bin[arrival_bin(arr, a0, inv_bw)], 1);
+            }
+        }
+        __syncthreads();
+        const unsigned long long* out = keys;
+        bool crowded = false;
+        if (binned) {
+            bool big = false;
+            for (int b = tid; b < kBins; b += kSortThreads) big = big || s_bin[b] > kBinSortMax;
+            crowded = __syncthreads_or(big) != 0;
+        }
+        if (binned && !crowded) {
+            unsigned long long* el2 = s_k + kBinCap;
+            int cnt2[kBins / kSortThreads];
+            int sum = 0;
+#pragma unroll
+            for (int j = 0; j < kBins / kSortThreads; ++j) {
+                cnt2[j] = s_bin[tid * (kBins / kSortThreads) + j];
+                sum += cnt2[j];
+            }
+            int total;
+            int off = block_excl_scan(sum, s_w, &total);
+#pragma unroll
+            for (int j = 0; j < kBins / kSortThreads; ++j) {
+                const int b = tid * (kBins / kSortThreads) + j;
+                s_bin[b] = off;
+                s_fill[b] = 0;
+                off += cnt2[j];
+            }
+            if (tid == 0) s_bin[kBins] = total;
+            __syncthreads();
+            for (int i = tid; i < ns; i += kSortThreads) {
+                const unsigned long long key = keys[i];
+                const int b = arrival_bin(__uint_as_float(static_cast<uint32_t>(key >> 32)), a0, inv_bw);
+                el2[s_bin[b] + atomicAdd(&s_fill[b], 1)] = key;
+            }
+            __syncthreads();
+            for (int b = tid; b < kBins; b += kSortThreads) {  // each bin's few keys: insertion sort
+                const int lo = s_bin[b], hi = s_bin[b + 1];
+                for (int i = lo + 1; i < hi; ++i) {
+                    const unsigned long long x = el2[i];
+                    int j = i - 1;
+                    while (j >= lo && el2[j] > x) {
+                        el2[j + 1] = el2[j];
+                        --j;
+                    }
+                    el2[j + 1] = x;
+                }
+            }
+            __syncthreads();
+            out = el2;
+        } else {
+            for (int64_t i = ns + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
+            __syncthreads();
+            block_sort<L>(keys, n2, s_k, in_lds && n2 <= L);
+        }
+        if (trow && si == 0) trow[3] = wall_clock64();
         for (int i = tid; i < ns; i += kSortThreads) {
-            const unsigned long long kb = keys[i];
-            const int pos = static_cast<int>(kb & 0xFFFFFFFFull);
-            int j = 0;
-            while (pos >= base[j + 1]) ++j;
-            const unsigned long long ka = a.sorted[off[j] + (pos - base[j])];
-            const EvPopDev& Q = s_pops[static_cast<int>(ka >> 56)];
-            const int local = static_cast<int>(ka & 0xFFFFFFull);
-            a.stream_slot[St.base + i] = (Q.group0 + local / kWave) * kWave + (local & (kWave - 1));
+            const unsigned long long kb = out[i];
+            a.stream_slot[St.base + i] = static_cast<int32_t>(kb & 0x0FFFFFFFull);
             a.stream_a[St.base + i] = __uint_as_float(static_cast<uint32_t>(kb >> 32));
         }
         if (tid == 0) a.stream_count[si] = ns;
         __syncthreads();
     }
+    if (trow) trow[4] = trow[5] = trow[6] = wall_clock64();
     if (tid == 0) *a.window = k + 1;
+    if (trow) trow[7] = wall_clock64();
 }
 
 __global__ void ev_set_kernel(int64_t* window, int64_t k, int32_t* stream_count, int n_streams, int clear) {
@@ -722,7 +922,6 @@ struct n2h_ev_plan {
     int32_t* stream_slot = nullptr;
     float* stream_a = nullptr;
     unsigned long long* keys = nullptr;
-    unsigned long long* sorted = nullptr;
     int32_t* group_prefix = nullptr;
     int64_t* window = nullptr;
     int32_t* flags = nullptr;
@@ -743,7 +942,7 @@ namespace {
 void free_plan(n2h_ev_plan* p) {
     if (!p) return;
     void* bufs[] = {p->pops_dev, p->streams_dev, p->reg_lane, p->reg_e, p->reg_count, p->stream_count, p->stream_slot,
-                    p->stream_a, p->keys, p->sorted, p->group_prefix, p->window, p->flags, p->counters};
+                    p->stream_a, p->keys, p->group_prefix, p->window, p->flags, p->counters};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -913,7 +1112,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
     int dev = 0, lds_max = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    p->sort_lds = lds_max >= 8192 * 8 + 4096 ? 8192 : 4096;
+    p->sort_lds = lds_max >= 8192 * 8 + 24 * 1024 ? 8192 : 4096;  // (+ the sort kernel's static LDS, ~20 KB)
     if (p->sort_lds == 8192 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(ev_sort_kernel<8192>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)
@@ -923,7 +1122,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
         (rc = dev_alloc(&p->reg_lane, regions)) || (rc = dev_alloc(&p->reg_e, regions)) ||
         (rc = dev_alloc(&p->reg_count, d->n_groups)) || (rc = dev_alloc(&p->stream_count, d->n_streams)) ||
         (rc = dev_alloc(&p->stream_slot, stream_total)) || (rc = dev_alloc(&p->stream_a, stream_total)) ||
-        (rc = dev_alloc(&p->keys, keys_cap)) || (rc = dev_alloc(&p->sorted, keys_cap)) ||
+        (rc = dev_alloc(&p->keys, keys_cap)) ||
         (rc = dev_alloc(&p->group_prefix, d->n_groups + 1)) || (rc = dev_alloc(&p->window, 1)) ||
         (rc = dev_alloc(&p->flags, N2H_EV_FLAGS)) || (rc = dev_alloc(&p->counters, 2))) {
         free_plan(p);
@@ -956,7 +1155,6 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
     a.stream_slot = p->stream_slot;
     a.stream_a = p->stream_a;
     a.keys = p->keys;
-    a.sorted = p->sorted;
     a.keys_cap = keys_cap;
     a.group_prefix = p->group_prefix;
     a.window = p->window;
@@ -1091,6 +1289,12 @@ int n2h_ev_status(n2h_ev_plan* p, int32_t* flags, int32_t n) {
     int32_t f[N2H_EV_FLAGS];
     EV_HIP(hipMemcpy(f, p->flags, sizeof(f), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) flags[i] = i < N2H_EV_FLAGS ? f[i] : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_set_trace(n2h_ev_plan* p, uint64_t* buffer) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    p->args.trace = reinterpret_cast<unsigned long long*>(buffer);
     return N2H_OK;
 }
 

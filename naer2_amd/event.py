@@ -348,6 +348,43 @@ class EventRuntime:
         self.record_from = self.window_index
         return float(d.value), float(s.value)
 
+    TRACE_PHASES = ("entries", "staging", "inbox places", "inbox fill", "walk")
+    SORT_EVENTS = ("start", "group prefix", "record", "first stream sorted", "streams written", "-", "-", "end")
+
+    def trace(self, n_windows: int = 1) -> dict:
+        """Run n_windows eager windows with the in-kernel phase trace on; for the last one: per phase of the
+        neuron groups the mean and max over groups of its summed duration (us), the rounds, the groups'
+        start spread and their longest run, and the sort kernel's phase durations (us)."""
+        lib = _lib.load()
+        buf = torch.zeros(self.n_groups + 1, 8, dtype=torch.int64, device=self.device)
+        self._unbind_record()
+        self._drop_graphs()
+        _lib.ev_check(lib.n2h_ev_set_trace(self._plan, buf.data_ptr()), "n2h_ev_set_trace")
+        try:
+            self.run(n_windows, 0)
+            torch.cuda.synchronize(self.device)
+        finally:
+            _lib.ev_check(lib.n2h_ev_set_trace(self._plan, None), "n2h_ev_set_trace")
+        t = buf.cpu().numpy().astype(np.int64)
+        rows = t[:-1]
+        used = rows[:, 7] > 0
+        out = {}
+        for j, name in enumerate(self.TRACE_PHASES):
+            d = rows[used, 1 + j] / 100.0
+            out[name] = (float(d.mean()), float(d.max())) if d.size else (0.0, 0.0)
+        if used.any():
+            out["rounds"] = (float(rows[used, 6].mean()), int(rows[used, 6].max()))
+            out["group starts spread"] = float((rows[used, 0].max() - rows[used, 0].min()) / 100.0)
+            out["group run"] = (float(((rows[used, 7] - rows[used, 0]) / 100.0).mean()),
+                                float(((rows[used, 7] - rows[used, 0]) / 100.0).max()))
+        srow = t[-1]
+        out["sort"] = {f"{self.SORT_EVENTS[j - 1]} -> {self.SORT_EVENTS[j]}": float((srow[j] - srow[j - 1]) / 100.0)
+                       for j in (1, 2, 3, 4)}
+        out["sort"]["streams written -> end"] = float((srow[7] - srow[4]) / 100.0)
+        if used.any():
+            out["deliver end -> sort start"] = float((srow[0] - rows[used, 7].max()) / 100.0)
+        return out
+
     def check(self) -> None:
         """Raise if a kernel flagged lost spikes or links (the reference's check(), runtime.py:724-735)."""
         flags = (ctypes.c_int32 * _lib.EV_FLAGS.__len__())()

@@ -148,10 +148,15 @@ def ev_exp(x) -> np.float32:
     n = np.rint(F32(x * F32(1.44269502)))
     r = F32(x - F32(n * F32(0.693145751953125)))
     r = F32(r - F32(n * F32(1.42860677e-06)))
-    p = F32(1.98412701e-04)
-    for c in (1.38888892e-03, 8.33333377e-03, 4.16666679e-02, 1.66666672e-01, 0.5, 1.0, 1.0):
-        p = F32(F32(p * r) + F32(c))
-    return F32(np.ldexp(p, int(n)))
+    r2 = F32(r * r)
+    r4 = F32(r2 * r2)
+    q01 = F32(F32(1.0) + F32(F32(1.0) * r))
+    q23 = F32(F32(0.5) + F32(F32(1.66666672e-01) * r))
+    q45 = F32(F32(4.16666679e-02) + F32(F32(8.33333377e-03) * r))
+    q67 = F32(F32(1.38888892e-03) + F32(F32(1.98412701e-04) * r))
+    q03 = F32(q01 + F32(r2 * q23))
+    q47 = F32(q45 + F32(r2 * q67))
+    return F32(np.ldexp(F32(q03 + F32(r4 * q47)), int(n)))
 
 
 def ev_log(u) -> np.float32:
@@ -181,7 +186,7 @@ def interval(node: int, window: int, m: int, seed: int, instance: int, rate_inv:
 
 def heap_run(net: EvNet, v0, n_windows: int):
     """The same network as ONE global event queue instead of windows: every input is a heap entry keyed
-    (arrival, connection, emission, source, link); a neuron's pending threshold check runs as soon as the
+    (arrival, connection, source, link); a neuron's pending threshold check runs as soon as the
     queue's next arrival is later than its last input (before that arrival is taken), so its spike's inputs
     enter the queue in time.  Sources are generated window by window as in the contract; spikes emitted in
     window j are delivered only when j + 1 < n_windows (what the windowed run covers).  Pure Python: tiny
@@ -206,7 +211,7 @@ def heap_run(net: EvNet, v0, n_windows: int):
         for link in range(int(net.src_ptr[node]), int(net.src_ptr[node + 1])):
             t = int(net.tgt[link])
             c = conn_of[(int(pop_of[node]), int(pop_of[t]))]
-            heapq.heappush(heap, (float(a), c, float(e), node, link, t, win + 1))
+            heapq.heappush(heap, (float(a), c, node, link, t, win + 1))
 
     for k in range(n_windows):
         t_start = F32(np.float64(k) * np.float64(W))
@@ -253,7 +258,7 @@ def heap_run(net: EvNet, v0, n_windows: int):
             # every pending check is at the current time (all earlier ones were flushed): run them first
             flush()
             continue
-        a_, c, e, node, link, t, win = heapq.heappop(heap)
+        a_, c, node, link, t, win = heapq.heappop(heap)
         P = net.pops[int(pop_of[t])]
         if a >= tre[t]:
             v[t] = F32(F32(v[t] * ev_exp(F32(F32(last[t] - a) * F32(P.tau_inv)))) + F32(net.w[link]))

@@ -22,21 +22,22 @@ static uint32_t f_bits(float f) {
 }
 
 /* exp(x) for x <= 88: Cody-Waite reduction x = n ln2 + r (ln 2 split so that n * LN2_HI is exact), the degree-7
- * Taylor polynomial of exp(r) by Horner, scaled by 2^n; 0 below -87. */
+ * Taylor polynomial of exp(r) by Estrin's scheme (pairs, then r^2, then r^4: a short dependency chain), scaled
+ * by 2^n; 0 below -87. */
 float or_ev_exp(float x) {
     if (x < -87.0f) return 0.0f;
     const float n = rintf(x * 1.44269502f);
     float r = x - n * 0.693145751953125f;
     r = r - n * 1.42860677e-06f;
-    float p = 1.98412701e-04f;  /* 1/7! */
-    p = p * r + 1.38888892e-03f; /* 1/6! */
-    p = p * r + 8.33333377e-03f; /* 1/5! */
-    p = p * r + 4.16666679e-02f; /* 1/4! */
-    p = p * r + 1.66666672e-01f; /* 1/3! */
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    return ldexpf(p, (int)n);
+    const float r2 = r * r;
+    const float r4 = r2 * r2;
+    const float q01 = 1.0f + 1.0f * r;
+    const float q23 = 0.5f + 1.66666672e-01f * r;
+    const float q45 = 4.16666679e-02f + 8.33333377e-03f * r;
+    const float q67 = 1.38888892e-03f + 1.98412701e-04f * r;
+    const float q03 = q01 + r2 * q23;
+    const float q47 = q45 + r2 * q67;
+    return ldexpf(q03 + r4 * q47, (int)n);
 }
 
 /* log(u) for finite u > 0: u = 2^e m with m in [sqrt(1/2), sqrt(2)), log(m) = t P(t), t = m - 1 (exact), P a
@@ -91,7 +92,6 @@ static int entry_cmp(const void* x, const void* y) {
     const entry* q = (const entry*)y;
     if (p->a != q->a) return p->a < q->a ? -1 : 1;
     if (p->c != q->c) return p->c < q->c ? -1 : 1;
-    if (p->e != q->e) return p->e < q->e ? -1 : 1;
     return (p->node > q->node) - (p->node < q->node);
 }
 

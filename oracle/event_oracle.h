@@ -22,8 +22,10 @@
  * The contract (identical on the GPU):
  *   - Every connection has one delay, all equal: the window W.  Window k emits spikes; window k + 1 delivers
  *     all of them.  t_end(k) = (float)((double)(k + 1) * W), t_start(k) = (float)((double)k * W).
- *   - Per target neuron, the inputs of a window in ascending (a, c, e, source, link): a = e + W (fp32),
- *     c the connection index, e the emission time, link the canonical link index.  For each input: if its a
+ *   - Per target neuron, the inputs of a window in ascending (a, c, source, link): a = e + W (fp32), c the
+ *     connection index, link the canonical link index.  (The reference orders a connection's spikes of one
+ *     arrival time by emission time, then source; the two differ only when two emission times round to the
+ *     same fp32 arrival.)  For each input: if its a
  *     differs from the previous input's, the threshold is checked at the previous a first; then, if
  *     a >= t_ref_end: V = V * ev_exp((last - a) * tau_inv) + w, last = a.  After the window's last input, the
  *     threshold is checked at its a.  Check at t: if V >= theta: spike at t, V = V_r,

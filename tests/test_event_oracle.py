@@ -120,6 +120,26 @@ def test_connection_order_breaks_ties():
     assert st.v[0].view(np.uint32) == v.view(np.uint32)
 
 
+def test_equal_arrivals_from_distinct_emissions_by_source():
+    """Two emission times that round to one fp32 arrival (1022.5 and the next float, + 1.5 = 1024.0 both):
+    inputs of one connection with equal arrival in source-neuron order (the source emitting later is neuron 0
+    here, so it goes first), then the threshold once."""
+    e1 = F32(1022.5)
+    e2 = np.nextafter(e1, F32(2000.0))
+    assert F32(e1 + F32(1.5)) == F32(e2 + F32(1.5))
+    w0, w1 = 7.0, 1e-3  # source 0 fires at e2 (later), source 1 at e1
+    net, _ = train_net([[[float(e2)], [float(e1)]]], 1, [[(0, 0, w0), (1, 0, w1)]], 3.0)
+    evn, v0 = oracle_net(net)
+    st = ev.EvState.initial(evn, v0)
+    ev.run(evn, st, int(1024 / 1.5) + 3)
+    a = F32(1024.0)
+    v = F32(F32(F32(3.0) * ev.ev_exp(F32(F32(F32(0.0) - a) * F32(1.0 / 20.0)))) + F32(w0))
+    v = F32(v + F32(w1))
+    assert st.v[0].view(np.uint32) == v.view(np.uint32)
+    hn, ht, hc, hv = ev.heap_run(evn, v0, int(1024 / 1.5) + 3)
+    assert hv[0].view(np.uint32) == v.view(np.uint32)
+
+
 def _tiny_random(seed: int, with_trains: bool):
     import naer2_amd as n2
     import torch

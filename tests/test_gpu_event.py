@@ -96,7 +96,7 @@ def test_chunked_and_graph_windows_equal_eager():
 @pytest.mark.parametrize("seed", [1, 2])
 def test_ties_and_trains_parity(seed):
     """Synchronous spike trains (equal times across sources and connections) with Poisson and recurrent
-    inputs: arrival ties broken by connection, then time and neuron."""
+    inputs: arrival ties broken by connection, then source neuron."""
     from tests.test_event_oracle import _tiny_random
 
     import naer2_amd  # noqa: F401
@@ -108,6 +108,25 @@ def test_ties_and_trains_parity(seed):
     st, nodes, tms, cnt = _oracle(evn, v0, 40)
     rt = net.runtime
     rt.run(40)
+    _assert_parity(rt, st, nodes, tms, cnt)
+
+
+def test_equal_arrivals_from_distinct_emissions():
+    """Emission times 1022.5 and the next float both arrive at 1024.0: the GPU orders them by source neuron as
+    the oracle does (the stream's bins hold equal arrivals together)."""
+    e1 = F32(1022.5)
+    e2 = np.nextafter(e1, F32(2000.0))
+    rng = np.random.default_rng(9)
+    times = [[float(e2)], [float(e1)]] + [[float(t)] for t in rng.uniform(1020.0, 1025.0, 30).astype(F32)]
+    links = [(s, t, float(w)) for s in range(len(times)) for t in range(8)
+             for w in [rng.choice([7.0, 1e-3, -2.0, 0.3])]]
+    net, T = train_net([times], 8, [links], 3.0, device="cuda")
+    n_win = int(1026 / 1.5) + 2
+    _say("equal arrivals", n_win)
+    evn, v0 = oracle_net(net)
+    st, nodes, tms, cnt = _oracle(evn, v0, n_win)
+    rt = net.runtime
+    rt.run(n_win, 50)
     _assert_parity(rt, st, nodes, tms, cnt)
 
 
