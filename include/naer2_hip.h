@@ -361,6 +361,10 @@ int n2h_ev_graph_launch(n2h_ev_graph* graph, void* stream);
 int n2h_ev_graph_destroy(n2h_ev_graph* graph);
 /* flags[0 .. n) = the sticky N2H_EV_FLAG_* words (synchronizes). */
 int n2h_ev_status(n2h_ev_plan* plan, int32_t* flags, int32_t n);
+/* Launch mapping of the deliver kernel: tuning->xcd = 0: block b runs group b; k > 1: the blocks dealt to each
+ * of k XCDs run runs of 16 consecutive groups (neighbouring groups share their segments' cache lines in one
+ * L2).  Results do not change.  Graphs captured before the call keep the mapping they were captured with. */
+int n2h_ev_set_tuning(n2h_ev_plan* plan, const n2h_tuning* tuning);
 /* In-kernel phase trace of the windows that follow (the reference's naer2/trace.py stamps): with a buffer
  * uint64 [n_groups + 1][8], every neuron group g writes row g for the last window it ran -- its start, the
  * 100 MHz ticks it spent on the input-stream entries, link staging, inbox places, inbox fill and the walk

@@ -47,6 +47,7 @@ constexpr int kCapL = 2048;        // links a group stages per round (LDS: 8 B s
 constexpr int kLinkU = 8;          // link loads per lane in flight while staging
 constexpr int kWaves = 4;          // wavefronts per group's workgroup (staging); wave 0 walks
 constexpr int kThreads = kWaves * kWave;
+constexpr int kXcdRun = 16;        // consecutive groups one XCD runs in a row (one 128-byte line of offsets)
 constexpr int kSortThreads = 1024;
 constexpr int kPerThread = 2;                          // keys per thread in the register sort
 constexpr int kSmallSort = kPerThread * kSortThreads;  // key counts up to this sort in registers (bitonic_small)
@@ -124,6 +125,7 @@ struct EvArgs {
     int64_t rec_cap;
     unsigned long long* counters; // [2]: inputs delivered, spikes emitted (since the plan was made)
     unsigned long long* trace;    // null, or [n_groups + 1][8] phase stamps of the last window (100 MHz clock)
+    int32_t xcd;                  // 0: block b runs group b; k > 1: runs of kXcdRun consecutive groups per XCD
     float W;
     uint32_t key0, key1, instance;
 };
@@ -254,9 +256,19 @@ struct DeliverLds {
 // input stream (every wave a contiguous quarter of the round's links, so the per-target order stays the link
 // order), wave 0 -- one lane per neuron -- walks the inboxes; then the final threshold.  Source groups: wave 0
 // draws the window's spikes.  Region count and state written last.
+// Block b's group: b itself, or with k XCDs (blocks are dealt to the XCDs in turn) runs of kXcdRun
+// consecutive groups per XCD, so that neighbouring groups -- whose segments of one source and whose offset
+// entries lie side by side in memory -- share an L2.
+__device__ __forceinline__ int group_of_block(int b, int k) {
+    if (k <= 1) return b;
+    const int x = b % k, q = b / k;
+    return (q / kXcdRun) * (k * kXcdRun) + x * kXcdRun + q % kXcdRun;
+}
+
 __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
     __shared__ DeliverLds s;
-    const int g = blockIdx.x;
+    const int g = group_of_block(static_cast<int>(blockIdx.x), a.xcd);
+    if (g >= a.n_groups) return;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wv = tid / kWave;
@@ -333,16 +345,23 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
             t_ph = t_;                              \
         }                                           \
     } while (0)
+            // the entries of a round, one per thread (a spike of the stream and its segment into the group),
+            // loaded a round ahead: the stream entry while this round stages, its segment bounds after that
+            bool pf_ok = tid < n_s;
+            int32_t pf_slot = pf_ok ? sslot[tid] : 0;
+            float pf_a = pf_ok ? sa[tid] : 0.0f;
+            int64_t pf_lo = 0, pf_hi = 0;
+            if (pf_ok) {
+                const int64_t seg = static_cast<int64_t>(pf_slot) * a.n_groups + g;
+                pf_lo = a.seg_off[seg];
+                pf_hi = a.seg_off[seg + 1];
+            }
             for (int base = 0; base < n_s;) {
                 ++rounds;
-                // this round's entries, one per thread: the segment of the spike into the group
-                const int idx = base + tid;
-                const bool ok = idx < n_s;
-                const int32_t slot = ok ? sslot[idx] : 0;
-                const float ae = ok ? sa[idx] : 0.0f;
-                const int64_t seg = static_cast<int64_t>(slot) * a.n_groups + g;
-                const int64_t lo = ok ? a.seg_off[seg] : 0;
-                const int nl = ok ? static_cast<int>(a.seg_off[seg + 1] - lo) : 0;
+                const bool ok = pf_ok;
+                const float ae = pf_a;
+                const int64_t lo = pf_lo;
+                const int nl = ok ? static_cast<int>(pf_hi - pf_lo) : 0;
                 // inclusive scan over the workgroup (entry order = thread order)
                 const int x = wave_incl_scan(nl, lane);
                 if (lane == kWave - 1) s.scan[wv] = x;
@@ -380,6 +399,12 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                 }
                 __syncthreads();
                 EV_PHASE(0);
+                {
+                    const int nidx = base + r + tid;
+                    pf_ok = nidx < n_s;
+                    pf_slot = pf_ok ? sslot[nidx] : 0;
+                    pf_a = pf_ok ? sa[nidx] : 0.0f;
+                }
                 // pass 1: each wave stages its quarter of the links, counts per target lane; kLinkU loads per
                 // lane in flight (indices clamped into the quarter, results past its end ignored)
                 const int Q = ((T + kThreads - 1) / kThreads) * kWave;
@@ -414,6 +439,11 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                 }
                 __syncthreads();
                 EV_PHASE(1);
+                if (pf_ok) {
+                    const int64_t seg = static_cast<int64_t>(pf_slot) * a.n_groups + g;
+                    pf_lo = a.seg_off[seg];
+                    pf_hi = a.seg_off[seg + 1];
+                }
                 // inbox places: target lane t's inbox starts at the links to lanes < t; within it, wave w's
                 // links follow those of waves < w
                 int c = 0;
@@ -464,10 +494,10 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                 if (wv == 0) {
                     const int depth = s.depth;
                     const int st = s.wstart[0][lane];
-                    // (the next input read one iteration ahead: the LDS latency off the chain through V)
+                    // (the next input read one iteration ahead: the LDS latency off the chain through V;
+                    // branch-free: every lane computes both outcomes and selects)
                     float w_n = c > 0 ? s.in_w[st] : 0.0f;
                     float a_n = c > 0 ? s.in_a[st] : 0.0f;
-                    // (branch-free: every lane computes both outcomes and selects)
                     for (int q = 0; q < depth; ++q) {
                         const bool has = act && q < c;
                         const float w = w_n;
@@ -961,7 +991,10 @@ int dev_alloc(T** out, int64_t count) {
 // Enqueue n windows.  ev: null, or 3 events per window recorded before the deliver kernel, between the two
 // kernels and after the sort kernel (timing).
 int launch_windows(n2h_ev_plan* p, int32_t n, hipStream_t stream, hipEvent_t* ev = nullptr) {
-    const dim3 grid(static_cast<unsigned>(p->args.n_groups)), block(kThreads);
+    const int k = p->args.xcd;
+    const int64_t span = static_cast<int64_t>(k) * kXcdRun;
+    const int64_t blocks = k > 1 ? (p->args.n_groups + span - 1) / span * span : p->args.n_groups;
+    const dim3 grid(static_cast<unsigned>(blocks)), block(kThreads);
     const size_t lds = static_cast<size_t>(p->sort_lds) * sizeof(unsigned long long);
     for (int32_t i = 0; i < n; ++i) {
         if (ev) EV_HIP(hipEventRecord(ev[3 * i], stream));
@@ -1289,6 +1322,13 @@ int n2h_ev_status(n2h_ev_plan* p, int32_t* flags, int32_t n) {
     int32_t f[N2H_EV_FLAGS];
     EV_HIP(hipMemcpy(f, p->flags, sizeof(f), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) flags[i] = i < N2H_EV_FLAGS ? f[i] : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_set_tuning(n2h_ev_plan* p, const n2h_tuning* t) {
+    if (!p || !t) return fail(N2H_EINVAL, "null argument");
+    if (t->xcd < 0 || t->xcd > 64) return fail(N2H_EINVAL, "xcd %d out of [0, 64]", t->xcd);
+    p->args.xcd = t->xcd;
     return N2H_OK;
 }
 

@@ -1,4 +1,4 @@
-"""BASELINE INFRASTRUCTURE ONLY: the reference's own CPU-capable algorithm, timed on the host's cores.
+"""CPU-BASELINE TIMING ONLY: the reference's own CPU-capable algorithm, timed on the host's cores.
 
 tools/brunel_reference.py:14-48 is a plain-torch time-driven loop:
 - take the due slot of a delayed-input buffer plus J x Poisson(lambda) background counts;

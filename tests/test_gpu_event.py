@@ -59,15 +59,17 @@ def test_math_bit_exact():
     assert np.array_equal(gl.view(np.uint32), ol.view(np.uint32))
 
 
-@pytest.mark.parametrize("N,K,windows,graph", [(500, 50, 100, 0), (2000, 200, 100, 20)])
-def test_brunel_exact_parity(N, K, windows, graph):
+@pytest.mark.parametrize("N,K,windows,graph,xcd", [(500, 50, 100, 0, 0), (2000, 200, 100, 20, 0),
+                                                   (3000, 300, 60, 20, 8)])
+def test_brunel_exact_parity(N, K, windows, graph, xcd):
     from naer2_amd.networks import brunel_exact
 
-    _say("brunel_exact parity", N, K, windows, graph)
+    _say("brunel_exact parity", N, K, windows, graph, xcd)
     net, _ = brunel_exact(N, K=K, device="cuda")
     evn, v0 = oracle_net(net)
     st, nodes, tms, cnt = _oracle(evn, v0, windows)
     rt = net.runtime
+    rt.set_xcd(xcd)
     rt.run(windows, graph)
     assert cnt.sum() > 50
     _assert_parity(rt, st, nodes, tms, cnt)
