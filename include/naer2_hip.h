@@ -361,6 +361,15 @@ int n2h_ev_graph_launch(n2h_ev_graph* graph, void* stream);
 int n2h_ev_graph_destroy(n2h_ev_graph* graph);
 /* flags[0 .. n) = the sticky N2H_EV_FLAG_* words (synchronizes). */
 int n2h_ev_status(n2h_ev_plan* plan, int32_t* flags, int32_t n);
+/* The delivery log (naer2/runtime.py:747-752, PopulationRuntime.deliveries: (slot, e, connection, source,
+ * target) in log order): while bound, every delivered input appends records[i] = {window, connection index,
+ * source (index in its population), target (index in its population)} and arrival[i] = its arrival time at
+ * i = (*count)++ -- per target in delivery order, targets interleaved; past `capacity` the count still grows
+ * and nothing is written.  conn_index: host [n_pops][n_pops], the connection from population s into t at
+ * [t * n_pops + s].  A separate kernel variant runs while logging; records NULL: off.  Graphs captured before
+ * the call keep the setting they were captured with. */
+int n2h_ev_bind_log(n2h_ev_plan* plan, const int32_t* conn_index, int32_t* records, float* arrival, int32_t* count,
+                    int64_t capacity);
 /* Launch mapping of the deliver kernel: tuning->xcd = 0: block b runs group b; k > 1: the blocks dealt to each
  * of k XCDs run runs of 16 consecutive groups (neighbouring groups share their segments' cache lines in one
  * L2).  Results do not change.  Graphs captured before the call keep the mapping they were captured with. */

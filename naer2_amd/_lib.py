@@ -30,7 +30,7 @@ EXPORTS = (
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward",
     "n2h_ev_last_error", "n2h_ev_create", "n2h_ev_destroy", "n2h_ev_bind_state", "n2h_ev_bind_record",
     "n2h_ev_set_window", "n2h_ev_get_window", "n2h_ev_run", "n2h_ev_graph_create", "n2h_ev_graph_launch",
-    "n2h_ev_graph_destroy", "n2h_ev_status", "n2h_ev_set_tuning", "n2h_ev_set_trace", "n2h_ev_counters", "n2h_ev_run_timed", "n2h_ev_math",
+    "n2h_ev_graph_destroy", "n2h_ev_status", "n2h_ev_bind_log", "n2h_ev_set_tuning", "n2h_ev_set_trace", "n2h_ev_counters", "n2h_ev_run_timed", "n2h_ev_math",
 )
 
 EV_NEURON, EV_POISSON, EV_TRAIN = 0, 1, 2
@@ -140,6 +140,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_ev_graph_launch": ([vp, vp], i32),
         "n2h_ev_graph_destroy": ([vp], i32),
         "n2h_ev_status": ([vp, ctypes.POINTER(i32), i32], i32),
+        "n2h_ev_bind_log": ([vp, vp, vp, vp, vp, i64], i32),
         "n2h_ev_set_tuning": ([vp, ctypes.POINTER(Tuning)], i32),
         "n2h_ev_set_trace": ([vp, vp], i32),
         "n2h_ev_counters": ([vp, ctypes.POINTER(i64), i32], i32),

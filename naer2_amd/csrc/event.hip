@@ -126,6 +126,11 @@ struct EvArgs {
     unsigned long long* counters; // [2]: inputs delivered, spikes emitted (since the plan was made)
     unsigned long long* trace;    // null, or [n_groups + 1][8] phase stamps of the last window (100 MHz clock)
     int32_t xcd;                  // 0: block b runs group b; k > 1: runs of kXcdRun consecutive groups per XCD
+    int32_t* log_rec;             // delivery log (the logging kernel variant): [log_cap][4] {window, connection,
+    float* log_a;                 //   source, target} and [log_cap] arrival, appended at *log_count
+    int32_t* log_count;
+    const int32_t* log_conn;      // [n_pops][n_pops]: connection index from source population s into t at t * n + s
+    int64_t log_cap;
     float W;
     uint32_t key0, key1, instance;
 };
@@ -236,8 +241,11 @@ __device__ __forceinline__ void emit(const EvArgs& a, int64_t rbase, int cap, in
     }
 }
 
+template <bool LOG>
 struct DeliverLds {
     int32_t off[kEnt + 1];          // the round's entries: first staged link (exclusive scan), off[r] = links
+    int32_t slot[LOG ? kEnt : 1];   // (delivery log) the entry's source slot
+    int32_t in_e[LOG ? kCapL : 1];  // (delivery log) the inbox input's entry
     int64_t lo[kEnt];               // first link of the entry's segment
     float a[kEnt];                  // arrival time
     uint32_t st_t[kCapL];           // staged link: target lane | entry << 8
@@ -265,8 +273,34 @@ __device__ __forceinline__ int group_of_block(int b, int k) {
     return (q / kXcdRun) * (k * kXcdRun) + x * kXcdRun + q % kXcdRun;
 }
 
+// The delivery log (the reference's deliveries(): (slot, e, connection, source, target) in delivery order per
+// target): the lanes in `has` append {window, connection, source, target} and the arrival, one wave-wide
+// atomic per call; past the capacity the count still grows and nothing is written.
+__device__ __forceinline__ void log_input(const EvArgs& a, const int32_t* ent_slot, const int32_t* in_e, int at,
+                                          bool has, int lane, int64_t k, float ar, int tpop, int target) {
+    const uint64_t m = __ballot(has);
+    if (!m) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(a.log_count, __popcll(m));
+    base = __shfl(base, 0, kWave);
+    if (!has) return;
+    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (pos >= a.log_cap) return;
+    const int32_t slot = ent_slot[in_e[at]];
+    const int sg = slot / kWave;
+    const int spop = a.group_info[sg] >> 16;
+    const int source = (sg - a.pops[spop].group0) * kWave + (slot & (kWave - 1));
+    int32_t* rec = a.log_rec + 4 * static_cast<int64_t>(pos);
+    rec[0] = static_cast<int32_t>(k);
+    rec[1] = a.log_conn[tpop * a.n_pops + spop];
+    rec[2] = source;
+    rec[3] = target;
+    a.log_a[pos] = ar;
+}
+
+template <bool LOG>
 __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
-    __shared__ DeliverLds s;
+    __shared__ DeliverLds<LOG> s;
     const int g = group_of_block(static_cast<int>(blockIdx.x), a.xcd);
     if (g >= a.n_groups) return;
     const int tid = threadIdx.x;
@@ -360,6 +394,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                 ++rounds;
                 const bool ok = pf_ok;
                 const float ae = pf_a;
+                const int32_t cur_slot = pf_slot;
                 const int64_t lo = pf_lo;
                 const int nl = ok ? static_cast<int>(pf_hi - pf_lo) : 0;
                 // inclusive scan over the workgroup (entry order = thread order)
@@ -378,6 +413,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                     s.off[tid] = incl - nl;
                     s.lo[tid] = lo;
                     s.a[tid] = ae;
+                    if constexpr (LOG) s.slot[tid] = cur_slot;
                 }
                 __syncthreads();
                 int T;
@@ -388,6 +424,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                         s.off[0] = 0;
                         s.lo[0] = lo;
                         s.a[0] = ae;
+                        if constexpr (LOG) s.slot[0] = cur_slot;
                         atomicOr(&a.flags[N2H_EV_FLAG_SEGMENT], 1);
                     }
                 } else {
@@ -479,6 +516,7 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                         const int p = s.wstart[wv][t] + s.wrun[wv][t] + __popcll(match & lt);
                         s.in_w[p] = s.st_w[f];
                         s.in_a[p] = s.a[tw >> 8];
+                        if constexpr (LOG) s.in_e[p] = static_cast<int32_t>(tw >> 8);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -512,6 +550,8 @@ __global__ __launch_bounds__(kThreads) void ev_deliver_kernel(EvArgs a) {
                         tre = fire ? t_end_ref : tre;
                         last = fire ? t_end_ref : last;
                         emit(a, rbase, P.cap, emitted, fire, lane, aprev);
+                        if constexpr (LOG) log_input(a, s.slot, s.in_e, st + q, has, lane, k, ar, info >> 16,
+                                                     (g - P.group0) * kWave + lane);
                         const bool recv = has && ar >= tre;
                         const float vn = V * ev_exp_sel((last - ar) * P.tau_inv) + w;
                         V = recv ? vn : V;
@@ -956,6 +996,7 @@ struct n2h_ev_plan {
     int64_t* window = nullptr;
     int32_t* flags = nullptr;
     unsigned long long* counters = nullptr;
+    int32_t* log_conn = nullptr;
     int64_t host_window = 0;
     bool bound = false;
 };
@@ -972,7 +1013,7 @@ namespace {
 void free_plan(n2h_ev_plan* p) {
     if (!p) return;
     void* bufs[] = {p->pops_dev, p->streams_dev, p->reg_lane, p->reg_e, p->reg_count, p->stream_count, p->stream_slot,
-                    p->stream_a, p->keys, p->group_prefix, p->window, p->flags, p->counters};
+                    p->stream_a, p->keys, p->group_prefix, p->window, p->flags, p->counters, p->log_conn};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -998,7 +1039,8 @@ int launch_windows(n2h_ev_plan* p, int32_t n, hipStream_t stream, hipEvent_t* ev
     const size_t lds = static_cast<size_t>(p->sort_lds) * sizeof(unsigned long long);
     for (int32_t i = 0; i < n; ++i) {
         if (ev) EV_HIP(hipEventRecord(ev[3 * i], stream));
-        hipLaunchKernelGGL(ev_deliver_kernel, grid, block, 0, stream, p->args);
+        if (p->args.log_rec) hipLaunchKernelGGL(ev_deliver_kernel<true>, grid, block, 0, stream, p->args);
+        else hipLaunchKernelGGL(ev_deliver_kernel<false>, grid, block, 0, stream, p->args);
         if (ev) EV_HIP(hipEventRecord(ev[3 * i + 1], stream));
         if (p->sort_lds == 8192)
             hipLaunchKernelGGL(ev_sort_kernel<8192>, dim3(1), dim3(kSortThreads), lds, stream, p->args);
@@ -1322,6 +1364,32 @@ int n2h_ev_status(n2h_ev_plan* p, int32_t* flags, int32_t n) {
     int32_t f[N2H_EV_FLAGS];
     EV_HIP(hipMemcpy(f, p->flags, sizeof(f), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) flags[i] = i < N2H_EV_FLAGS ? f[i] : 0;
+    return N2H_OK;
+}
+
+int n2h_ev_bind_log(n2h_ev_plan* p, const int32_t* conn_index, int32_t* records, float* arrival, int32_t* count,
+                    int64_t capacity) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (!records) {
+        p->args.log_rec = nullptr;
+        p->args.log_a = nullptr;
+        p->args.log_count = nullptr;
+        p->args.log_cap = 0;
+        return N2H_OK;
+    }
+    if (!conn_index || !arrival || !count || capacity < 0)
+        return fail(N2H_EINVAL, "log: conn_index, arrival and count are required, capacity >= 0");
+    const int n2 = p->args.n_pops * p->args.n_pops;
+    if (!p->log_conn) {
+        const int rc = dev_alloc(&p->log_conn, n2);
+        if (rc) return rc;
+    }
+    EV_HIP(hipMemcpy(p->log_conn, conn_index, static_cast<size_t>(n2) * sizeof(int32_t), hipMemcpyHostToDevice));
+    p->args.log_conn = p->log_conn;
+    p->args.log_rec = records;
+    p->args.log_a = arrival;
+    p->args.log_count = count;
+    p->args.log_cap = capacity;
     return N2H_OK;
 }
 

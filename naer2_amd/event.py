@@ -175,6 +175,7 @@ class EventRuntime:
             self.train_ptr = torch.from_numpy(ptr).to(dev)
             self.train_t = torch.from_numpy(flat).to(dev)
         self._stream_of = stream_of
+        self._log = None
         self._graphs: dict = {}
         self._plan = None
         self._create_plan()
@@ -332,6 +333,41 @@ class EventRuntime:
         if abs(k - round(k)) > 1e-6 * max(1.0, abs(k)):
             raise ValueError(f"{t_ms} ms is not a whole number of windows of {self.window} ms")
         self.run(int(round(k)), self.net.graph_window)
+
+    def log_deliveries(self, capacity: int | None) -> None:
+        """Log every delivered input of the windows that follow (the reference's delivery log); None: off.
+        deliveries() reads it."""
+        lib = _lib.load()
+        self._drop_graphs()
+        if capacity is None:
+            _lib.ev_check(lib.n2h_ev_bind_log(self._plan, None, None, None, None, 0), "n2h_ev_bind_log")
+            self._log = None
+            return
+        P = len(self.pops)
+        index = {p: i for i, p in enumerate(self.pops)}
+        conn = np.full(P * P, -1, np.int32)
+        for ci, c in enumerate(self.net.connections):
+            conn[index[c.target] * P + index[c.source]] = ci
+        rec = torch.zeros(max(int(capacity), 1), 4, dtype=torch.int32, device=self.device)
+        arr = torch.zeros(max(int(capacity), 1), dtype=torch.float32, device=self.device)
+        cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._log = (rec, arr, cnt, conn)
+        _lib.ev_check(lib.n2h_ev_bind_log(self._plan, conn.ctypes.data_as(ctypes.c_void_p), rec.data_ptr(),
+                                          arr.data_ptr(), cnt.data_ptr(), int(capacity)), "n2h_ev_bind_log")
+
+    def deliveries(self) -> list[tuple[int, float, int, int, int]]:
+        """The delivery log since log_deliveries(): (window, arrival, connection, source, target), source and
+        target as indices in their populations, per target in delivery order (naer2/runtime.py:747-752)."""
+        if self._log is None:
+            raise RuntimeError("no delivery log bound (log_deliveries)")
+        rec, arr, cnt, _ = self._log
+        torch.cuda.synchronize(self.device)
+        n = int(cnt.item())
+        if n > rec.shape[0]:
+            raise RuntimeError(f"the delivery log overflowed ({n} inputs, capacity {rec.shape[0]})")
+        r = rec[:n].cpu().tolist()
+        a = arr[:n].cpu().tolist()
+        return [(k, ai, c, s, t) for (k, c, s, t), ai in zip(r, a)]
 
     def set_xcd(self, k: int) -> None:
         """Deliver-kernel mapping: 0 one block per group in order; k > 1 runs of 16 consecutive groups per

@@ -45,7 +45,7 @@ def _lib():
                                        ctypes.c_float]
         lib.or_ev_interval.restype = ctypes.c_float
         lib.or_ev_run.argtypes = [i32, i32, P, P, i32, P, P, ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32,
-                                  P, P, P, P, P, P, P, P, P, P, P, P, i64, i64, i32, P, P, i64, P, P, P]
+                                  P, P, P, P, P, P, P, P, P, P, P, P, i64, i64, i32, P, P, i64, P, P, P, P, P, i64, P]
         lib.or_ev_run.restype = ctypes.c_int
         _ready = True
     return lib
@@ -97,8 +97,10 @@ class EvState:
                        self.prev_e.copy(), self.window)
 
 
-def run(net: EvNet, st: EvState, n_windows: int, record: bool = True):
-    """Advance st by n_windows windows (in place).  Returns (nodes, times, per-window counts, events)."""
+def run(net: EvNet, st: EvState, n_windows: int, record: bool = True, log: list | None = None,
+        log_cap: int = 0):
+    """Advance st by n_windows windows (in place).  Returns (nodes, times, per-window counts, events).  log: a
+    list that receives the delivery log (window, arrival, connection, source, target), per target in order."""
     lib = _lib()
     n = net.n
     pops = (EvPop * len(net.pops))(*net.pops)
@@ -122,13 +124,21 @@ def run(net: EvNet, st: EvState, n_windows: int, record: bool = True):
     wcount = np.zeros(max(n_windows, 1), np.int32)
     rec_n = ctypes.c_int64(0)
     events = ctypes.c_int64(0)
+    log_rec = np.zeros((max(log_cap, 1), 4), np.int32) if log is not None else None
+    log_a = np.zeros(max(log_cap, 1), np.float32) if log is not None else None
+    log_n = ctypes.c_int64(0)
     rc = lib.or_ev_run(n, len(net.pops), _p(pf), pops, len(cs), _p(cs), _p(ct), ctypes.c_float(net.window),
                        ctypes.c_uint64(net.seed), ctypes.c_uint32(net.instance), _p(sp), _p(tg), _p(w), _p(tp),
                        _p(tt), _p(st.v), _p(st.last), _p(st.tre), _p(st.nxt), ctypes.byref(prev_n), _p(prev_node),
                        _p(prev_e), cap, st.window, n_windows, _p(rec_node), _p(rec_e), rec_cap, ctypes.byref(rec_n),
-                       _p(wcount), ctypes.byref(events))
+                       _p(wcount), ctypes.byref(events), _p(log_rec), _p(log_a), int(log_cap), ctypes.byref(log_n))
     if rc != 0:
         raise RuntimeError(f"or_ev_run failed ({rc})")
+    if log is not None:
+        if log_n.value > log_cap:
+            raise RuntimeError(f"the delivery log overflowed ({log_n.value} > {log_cap})")
+        for (k, c, s_, t), a in zip(log_rec[:log_n.value].tolist(), log_a[:log_n.value].tolist()):
+            log.append((k, a, c, s_, t))
     st.prev_node = prev_node[:prev_n.value].copy()
     st.prev_e = prev_e[:prev_n.value].copy()
     st.window += n_windows
@@ -184,14 +194,15 @@ def interval(node: int, window: int, m: int, seed: int, instance: int, rate_inv:
 
 # ---------------------------------------------------------------- an independent formulation: one global queue
 
-def heap_run(net: EvNet, v0, n_windows: int):
+def heap_run(net: EvNet, v0, n_windows: int, log: list | None = None):
     """The same network as ONE global event queue instead of windows: every input is a heap entry keyed
     (arrival, connection, source, link); a neuron's pending threshold check runs as soon as the
     queue's next arrival is later than its last input (before that arrival is taken), so its spike's inputs
     enter the queue in time.  Sources are generated window by window as in the contract; spikes emitted in
     window j are delivered only when j + 1 < n_windows (what the windowed run covers).  Pure Python: tiny
     networks only.  Returns (nodes, times) sorted by (population, time, node) within each window, and
-    per-window counts."""
+    per-window counts; log (a list) receives every delivered input as (window, arrival, connection, source,
+    target) in global delivery order."""
     n = net.n
     pf = np.asarray(net.pop_first, np.int64)
     pop_of = np.searchsorted(pf, np.arange(n), side="right") - 1
@@ -260,6 +271,8 @@ def heap_run(net: EvNet, v0, n_windows: int):
             continue
         a_, c, node, link, t, win = heapq.heappop(heap)
         P = net.pops[int(pop_of[t])]
+        if log is not None:
+            log.append((win, float(a), c, node - int(pf[pop_of[node]]), t - int(pf[pop_of[t]])))
         if a >= tre[t]:
             v[t] = F32(F32(v[t] * ev_exp(F32(F32(last[t] - a) * F32(P.tau_inv)))) + F32(net.w[link]))
             last[t] = a

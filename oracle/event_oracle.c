@@ -152,7 +152,8 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
               const int64_t* src_ptr, const int32_t* tgt, const float* w, const int64_t* train_ptr,
               const float* train_t, float* v, float* last, float* tre, float* nxt, int64_t* prev_n,
               int32_t* prev_node, float* prev_e, int64_t prev_cap, int64_t k0, int32_t n_windows, int32_t* rec_node,
-              float* rec_e, int64_t rec_cap, int64_t* rec_n, int32_t* rec_wcount, int64_t* events) {
+              float* rec_e, int64_t rec_cap, int64_t* rec_n, int32_t* rec_wcount, int64_t* events, int32_t* log_rec,
+              float* log_a, int64_t log_cap, int64_t* log_n) {
     if (n < 0 || n_pops < 1 || n_windows < 0 || !(window > 0.0f)) return -1;
     int rc = 0;
     spikes prev = {0, 0, 0}, cur = {0, 0, 0};
@@ -162,6 +163,8 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
     int64_t* fill = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
     float* in_a = NULL;
     float* in_w = NULL;
+    int32_t* in_c = NULL;
+    int32_t* in_s = NULL;
     int64_t in_cap = 0;
     int32_t* pop_at = (int32_t*)malloc(((size_t)n + 1) * sizeof(int32_t));
     if (!cnt || !fill || !pop_at) {
@@ -227,6 +230,20 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
                     goto done;
                 }
                 in_w = xw;
+                if (log_rec) {
+                    int32_t* xc = (int32_t*)realloc(in_c, (size_t)total * sizeof(int32_t));
+                    if (!xc) {
+                        rc = -2;
+                        goto done;
+                    }
+                    in_c = xc;
+                    int32_t* xs = (int32_t*)realloc(in_s, (size_t)total * sizeof(int32_t));
+                    if (!xs) {
+                        rc = -2;
+                        goto done;
+                    }
+                    in_s = xs;
+                }
                 in_cap = total;
             }
             int64_t acc = 0;
@@ -239,6 +256,10 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
                     if (tgt[l] >= lo && tgt[l] < hi) {
                         in_a[fill[tgt[l]]] = ent[q].a;
                         in_w[fill[tgt[l]]] = w[l];
+                        if (log_rec) {
+                            in_c[fill[tgt[l]]] = ent[q].c;
+                            in_s[fill[tgt[l]]] = ent[q].node;
+                        }
                         fill[tgt[l]]++;
                     }
             if (events) *events += total;
@@ -249,6 +270,17 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
                 int pend = 0;
                 for (int64_t q = 0; q < cnt[i]; ++q, ++at) {
                     const float a = in_a[at];
+                    if (log_rec) {
+                        if (*log_n < log_cap) {
+                            int32_t* r = log_rec + 4 * *log_n;
+                            r[0] = (int32_t)k;
+                            r[1] = in_c[at];
+                            r[2] = in_s[at] - pop_first[pop_at[in_s[at]]];
+                            r[3] = i - lo;
+                            log_a[*log_n] = a;
+                        }
+                        ++*log_n;
+                    }
                     if (pend && a != aprev)
                         if ((rc = check(i, T, aprev, P, v, last, tre, &cur))) goto done;
                     if (a >= tre[i]) {
@@ -321,6 +353,8 @@ done:
     free(fill);
     free(in_a);
     free(in_w);
+    free(in_c);
+    free(in_s);
     free(pop_at);
     return rc;
 }

@@ -66,14 +66,16 @@ float or_ev_interval(uint32_t node, int64_t window, uint32_t m, uint64_t seed, u
  * prev: the spikes of window k0 - 1 on entry, of the last window on return ((pop, e, node) order):
  *   *prev_n entries of prev_node / prev_e, capacity prev_cap.
  * rec_node / rec_e (capacity rec_cap, *rec_n advanced) and rec_wcount [n_windows]: every window's spikes in
- * (pop, e, node) order, or NULL.  events: inputs delivered (out).  Returns 0, -1 on a bad argument, -2 when
- * memory or a capacity runs out. */
+ * (pop, e, node) order, or NULL.  events: inputs delivered (out).  log_rec / log_a (capacity log_cap, *log_n
+ * advanced, or NULL): every delivered input as {window, connection, source, target} (indices in their
+ * populations) and its arrival, per target in delivery order, targets ascending.  Returns 0, -1 on a bad
+ * argument, -2 when memory or a capacity runs out. */
 int or_ev_run(int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_ev_pop* pops,
               int32_t n_conns, const int32_t* conn_src, const int32_t* conn_tgt, float window, uint64_t seed,
               uint32_t instance, const int64_t* src_ptr, const int32_t* tgt, const float* w,
               const int64_t* train_ptr, const float* train_t, float* v, float* last, float* tre, float* nxt,
               int64_t* prev_n, int32_t* prev_node, float* prev_e, int64_t prev_cap, int64_t k0, int32_t n_windows,
               int32_t* rec_node, float* rec_e, int64_t rec_cap, int64_t* rec_n, int32_t* rec_wcount,
-              int64_t* events);
+              int64_t* events, int32_t* log_rec, float* log_a, int64_t log_cap, int64_t* log_n);
 
 #endif

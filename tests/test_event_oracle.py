@@ -188,6 +188,27 @@ def test_windows_equal_one_event_queue(seed, trains):
     assert np.array_equal(st.v.view(np.uint32), hv.view(np.uint32))
 
 
+def test_delivery_log_matches_event_queue():
+    """The oracle's delivery log (window, arrival, connection, source, target) -- per target the inputs in
+    delivery order -- equals the event queue's, and holds every input once."""
+    net = _tiny_random(4, True)
+    evn, v0 = oracle_net(net)
+    st = ev.EvState.initial(evn, v0)
+    log: list = []
+    _, _, _, events = ev.run(evn, st, 30, log=log, log_cap=200000)
+    hlog: list = []
+    ev.heap_run(evn, v0, 30, log=hlog)
+    assert len(log) == events == len(hlog) > 500
+    # targets are indices within their population: key by (the connection's target population, target)
+    tpop = {c: int(t) for c, t in enumerate(evn.conn_tgt)}
+    by_o, by_h = {}, {}
+    for r in log:
+        by_o.setdefault((tpop[r[2]], r[4]), []).append(r)
+    for r in hlog:
+        by_h.setdefault((tpop[r[2]], r[4]), []).append(r)
+    assert by_o == by_h
+
+
 def test_chunked_runs_equal_one_run():
     net = _tiny_random(3, True)
     evn, v0 = oracle_net(net)

@@ -189,6 +189,32 @@ def test_reference_brunel_gpu():
         assert len(cvs) > 200 and sum(cvs) / len(cvs) > 0.3
 
 
+def test_delivery_log():
+    """The GPU's delivery log (window, arrival, connection, source, target) equals the oracle's: per target the
+    same inputs in the same order, arrival times bit for bit (naer2/runtime.py:747-752's deliveries())."""
+    from naer2_amd.networks import brunel_exact
+
+    net, _ = brunel_exact(1000, K=100, device="cuda", seed=2)
+    evn, v0 = oracle_net(net)
+    st = ev.EvState.initial(evn, v0)
+    olog: list = []
+    ev.run(evn, st, 30, log=olog, log_cap=5_000_000)
+    rt = net.runtime
+    rt.log_deliveries(5_000_000)
+    _say("delivery log", len(olog))
+    rt.run(30, 10)
+    glog = rt.deliveries()
+    rt.log_deliveries(None)
+    assert len(glog) == len(olog) > 1000
+    tpop = {c: int(t) for c, t in enumerate(evn.conn_tgt)}
+    by_o, by_g = {}, {}
+    for r in olog:
+        by_o.setdefault((tpop[r[2]], r[4]), []).append((r[0], F32(r[1]).view(np.uint32), r[2], r[3]))
+    for r in glog:
+        by_g.setdefault((tpop[r[2]], r[4]), []).append((r[0], F32(r[1]).view(np.uint32), r[2], r[3]))
+    assert by_o == by_g
+
+
 def test_lost_spikes_raise():
     """A region too small for a window's spikes (history_cap) is flagged, never overrun: check() raises."""
     import naer2_amd as n2
