@@ -256,10 +256,11 @@ int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
 
 /* ------------------------------------------------------------------------------------------------------------
  * The event-driven, exact-time network (Brunel's model A as the reference runs it, test/networks/Brunel):
- * delta-synapse LIF neurons whose potential changes only at an input's exact arrival time e + D and which
- * fire inside delivery at that time; Poisson sources with exact spike times; one homogeneous delay D on
- * every connection (= the window).  Window k emits spikes; window k + 1 delivers all of them.  Per target,
- * a window's inputs run in ascending (arrival, connection, source, link); inputs at one arrival
+ * delta-synapse LIF neurons whose potential changes only at an input's exact arrival time e + D_c and which
+ * fire inside delivery at that time; Poisson sources with exact spike times; one delay D_c per connection,
+ * each a whole multiple lag_c of the window (the smallest delay).  Window k emits spikes; window k + lag_c
+ * delivers them through connection c.  Per target, a window's inputs run in ascending (arrival, connection,
+ * source, link); inputs at one arrival
  * time are received one by one, then the threshold is checked once (the reference's send after every
  * delivery).  Receive: if a >= t_ref_end: V = V * exp((last - a) / tau) + w, last = a.  Threshold at t:
  * V >= theta: spike at t, V = V_r, t_ref_end = last = t + t_ref.  exp and log are this library's own fp32
@@ -284,6 +285,7 @@ int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
 #define N2H_EV_POISSON 1     /* Poisson source with exact times (test/networks/Brunel/poisson.py:7-27) */
 #define N2H_EV_TRAIN 2       /* listed spike times per source (the reference tests' commanded sources) */
 #define N2H_EV_MAX_SOURCES 8 /* source populations of one input stream */
+#define N2H_EV_MAX_LAG 64    /* a delay of at most 64 windows */
 #define N2H_EV_FLAGS 4
 #define N2H_EV_FLAG_REGION 0   /* a group emitted more spikes in a window than its region holds (lost) */
 #define N2H_EV_FLAG_SEGMENT 1  /* a source had more than 2048 links into one group (the excess lost) */
@@ -303,11 +305,13 @@ typedef struct n2h_ev_pop {
 
 /* An input stream: the spikes of its source populations, listed in the order of the target populations'
  * incoming connections (connection index order), merged by arrival.  Target populations whose incoming
- * connections come from the same populations in the same order share one stream. */
+ * connections come from the same populations with the same delays in the same order share one stream.
+ * Source j's connection has delay[j] = lag[j] windows: window k delivers its spikes of window k - lag[j]. */
 typedef struct n2h_ev_stream {
     int32_t n_sources;
     int32_t source_pop[N2H_EV_MAX_SOURCES];
-    float delay;         /* == window */
+    float delay[N2H_EV_MAX_SOURCES];    /* ms (fp32), a whole multiple of the window */
+    int32_t lag[N2H_EV_MAX_SOURCES];    /* delay / window, 1 .. N2H_EV_MAX_LAG */
 } n2h_ev_stream;
 
 /* Neurons 0..n-1, population p owning [pop_first[p], pop_first[p+1]); groups of <= 64 neurons of one
@@ -323,7 +327,7 @@ typedef struct n2h_ev_desc {
     const int32_t* pop_first;        /* host [n_pops + 1] */
     const n2h_ev_pop* pops;          /* host [n_pops] */
     const n2h_ev_stream* streams;    /* host [n_streams] */
-    float window;                    /* ms: the common delay */
+    float window;                    /* ms: the smallest delay (every delay a whole multiple of it) */
     uint32_t instance;               /* Philox counter word 3 */
     uint64_t seed;                   /* Philox key */
     const int32_t* group_first;      /* device [n_groups] */

@@ -35,6 +35,7 @@ EXPORTS = (
 
 EV_NEURON, EV_POISSON, EV_TRAIN = 0, 1, 2
 EV_MAX_SOURCES = 8
+EV_MAX_LAG = 64
 EV_FLAGS = ("region", "segment", "record", "sort")
 
 
@@ -80,7 +81,7 @@ class EvPop(ctypes.Structure):
 
 class EvStream(ctypes.Structure):
     _fields_ = [("n_sources", ctypes.c_int32), ("source_pop", ctypes.c_int32 * EV_MAX_SOURCES),
-                ("delay", ctypes.c_float)]
+                ("delay", ctypes.c_float * EV_MAX_SOURCES), ("lag", ctypes.c_int32 * EV_MAX_SOURCES)]
 
 
 class EvDesc(ctypes.Structure):

@@ -29,6 +29,7 @@
 // with exactly the operations of the oracle, so that spike times match the oracle's bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -86,7 +87,8 @@ struct EvPopDev {
 struct EvStreamDev {
     int32_t n_src;
     int32_t src[N2H_EV_MAX_SOURCES];
-    float delay;
+    float delay[N2H_EV_MAX_SOURCES];   // per source: its connection's delay (ms) ...
+    int32_t lag[N2H_EV_MAX_SOURCES];   // ... = lag windows
     int64_t base;       // first entry in the stream arrays
 };
 
@@ -131,6 +133,11 @@ struct EvArgs {
     int32_t* log_count;
     const int32_t* log_conn;      // [n_pops][n_pops]: connection index from source population s into t at t * n + s
     int64_t log_cap;
+    int32_t n_hist;               // windows of sorted spikes kept (the largest lag; 1: streams from the regions)
+    int32_t pad_h;
+    unsigned long long* hist_key; // [n_hist][keys_cap]: a window's spikes, (population, time, neuron) keys
+    int32_t* hist_off;            // [n_hist][n_pops + 1]: each population's first key in its window's slot
+    int64_t* hist_win;            // [n_hist]: the window a slot holds (-1: none)
     float W;
     uint32_t key0, key1, instance;
 };
@@ -813,9 +820,11 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         if (tid == 0) atomicOr(&a.flags[N2H_EV_FLAG_SORT], 1);
         S = static_cast<int>(a.keys_cap);
     }
-    // 2. the record (when one is bound): the window's spikes in (population, time, neuron) order, keys
-    //    population << 56 | time bits << 24 | neuron within its population
-    if (a.rec_node != nullptr) {
+    // 2. the window's spikes in (population, time, neuron) order, keys population << 56 | time bits << 24 |
+    //    neuron within its population: for the record (when one is bound) and the history of windows (delays
+    //    of several windows)
+    const bool multi = a.n_hist > 1;
+    if (a.rec_node != nullptr || multi) {
         int64_t n2 = pow2_at_least(S > 0 ? S : 1);
         if (n2 < kSmallSort) n2 = kSmallSort;  // (padding sorts in registers)
         const bool in_lds = n2 <= L;
@@ -837,20 +846,33 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         for (int64_t i = S + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
         __syncthreads();
         block_sort<L>(keys, n2, s_k, in_lds);
-        const int rc0 = *a.rec_count;
-        const bool rec_ok = static_cast<int64_t>(rc0) + S <= a.rec_cap;
-        for (int i = tid; rec_ok && i < S; i += kSortThreads) {
-            const unsigned long long key = keys[i];
-            const int pop = static_cast<int>(key >> 56);
-            a.rec_node[rc0 + i] = s_pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
-            a.rec_e[rc0 + i] = __uint_as_float(static_cast<uint32_t>((key >> 24) & 0xFFFFFFFFull));
+        if (multi) {
+            const int hs = static_cast<int>(k % a.n_hist);
+            unsigned long long* hk = a.hist_key + static_cast<int64_t>(hs) * a.keys_cap;
+            for (int i = tid; i < S; i += kSortThreads) hk[i] = keys[i];
+            if (tid <= a.n_pops) {
+                int32_t* ho = a.hist_off + static_cast<int64_t>(hs) * (a.n_pops + 1);
+                ho[tid] = tid < a.n_pops ? a.group_prefix[s_pops[tid].group0] : S;
+            }
+            if (tid == 0) a.hist_win[hs] = k;
+        }
+        if (a.rec_node != nullptr) {
+            const int rc0 = *a.rec_count;
+            const bool rec_ok = static_cast<int64_t>(rc0) + S <= a.rec_cap;
+            for (int i = tid; rec_ok && i < S; i += kSortThreads) {
+                const unsigned long long key = keys[i];
+                const int pop = static_cast<int>(key >> 56);
+                a.rec_node[rc0 + i] = s_pops[pop].first + static_cast<int32_t>(key & 0xFFFFFFull);
+                a.rec_e[rc0 + i] = __uint_as_float(static_cast<uint32_t>((key >> 24) & 0xFFFFFFFFull));
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (rec_ok) *a.rec_count = rc0 + S;
+                else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
+                a.rec_wcount[k % a.rec_windows] = S;
+            }
         }
         __syncthreads();
-        if (tid == 0) {
-            if (rec_ok) *a.rec_count = rc0 + S;
-            else atomicOr(&a.flags[N2H_EV_FLAG_RECORD], 1);
-            a.rec_wcount[k % a.rec_windows] = S;
-        }
     }
     if (tid == 0) atomicAdd(&a.counters[1], static_cast<unsigned long long>(S));
     if (trow) trow[2] = wall_clock64();
@@ -866,31 +888,64 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         for (int b = tid; b < kBins; b += kSortThreads) s_bin[b] = 0;
         __syncthreads();
         if (tid < St.n_src) s_c[St.src[tid]] = tid;
+        // source j's spikes: of this window (lag 1, from the regions or the history), or of window
+        // k + 1 - lag from the history; hs_j its history slot (-1: none yet), jb_j its first key there
         int ns = 0;
+        int hs_j[N2H_EV_MAX_SOURCES], jb_j[N2H_EV_MAX_SOURCES], nb_j[N2H_EV_MAX_SOURCES + 1];
         for (int j = 0; j < St.n_src; ++j) {
             const EvPopDev& Q = s_pops[St.src[j]];
-            ns += a.group_prefix[Q.group0 + Q.n_groups] - a.group_prefix[Q.group0];
+            nb_j[j] = ns;
+            if (!multi) {
+                ns += a.group_prefix[Q.group0 + Q.n_groups] - a.group_prefix[Q.group0];
+                continue;
+            }
+            const int64_t w = k + 1 - St.lag[j];
+            const int hs = static_cast<int>(((w % a.n_hist) + a.n_hist) % a.n_hist);
+            hs_j[j] = (w >= 0 && a.hist_win[hs] == w) ? hs : -1;
+            if (hs_j[j] >= 0) {
+                const int32_t* ho = a.hist_off + static_cast<int64_t>(hs) * (a.n_pops + 1);
+                jb_j[j] = ho[St.src[j]];
+                ns += ho[St.src[j] + 1] - ho[St.src[j]];
+            }
         }
+        nb_j[St.n_src] = ns;
         const bool binned = ns <= kBinCap;
         int64_t n2 = pow2_at_least(ns > 0 ? ns : 1);
         if (n2 < kSmallSort) n2 = kSmallSort;
         const bool in_lds = binned || n2 <= L;
         unsigned long long* keys = in_lds ? s_k : a.keys;
-        const float a0 = window_time(k, a.W) + St.delay;
+        const float a0 = window_time(k + 1, a.W);  // (the earliest arrival of the next window)
         __syncthreads();
-        for (int g = tid; g < G; g += kSortThreads) {
-            const int pop = a.group_info[g] >> 16;
-            const int cix = s_c[pop];
-            const int c = cix < 0 ? 0 : a.reg_count[g];
-            if (c == 0) continue;
-            const EvPopDev& P = s_pops[pop];
-            const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
-            const int p0 = atomicAdd(&s_n, c);
-            for (int j = 0; j < c && p0 + j < ns; ++j) {
-                const float arr = a.reg_e[rb + j] + St.delay;
-                const uint32_t slot = static_cast<uint32_t>(g) * kWave + static_cast<uint32_t>(a.reg_lane[rb + j]);
-                keys[p0 + j] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) |
-                               (static_cast<unsigned long long>(cix) << 28) | slot;
+        if (!multi) {
+            for (int g = tid; g < G; g += kSortThreads) {
+                const int pop = a.group_info[g] >> 16;
+                const int cix = s_c[pop];
+                const int c = cix < 0 ? 0 : a.reg_count[g];
+                if (c == 0) continue;
+                const EvPopDev& P = s_pops[pop];
+                const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+                const int p0 = atomicAdd(&s_n, c);
+                const float dl = St.delay[cix];
+                for (int j = 0; j < c && p0 + j < ns; ++j) {
+                    const float arr = a.reg_e[rb + j] + dl;
+                    const uint32_t slot = static_cast<uint32_t>(g) * kWave + static_cast<uint32_t>(a.reg_lane[rb + j]);
+                    keys[p0 + j] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) |
+                                   (static_cast<unsigned long long>(cix) << 28) | slot;
+                    if (binned) atomicAdd(&s_bin[arrival_bin(arr, a0, inv_bw)], 1);
+                }
+            }
+        } else {
+            for (int i = tid; i < ns; i += kSortThreads) {
+                int j = 0;
+                while (i >= nb_j[j + 1]) ++j;
+                const unsigned long long ka =
+                    a.hist_key[static_cast<int64_t>(hs_j[j]) * a.keys_cap + jb_j[j] + (i - nb_j[j])];
+                const EvPopDev& Q = s_pops[static_cast<int>(ka >> 56)];
+                const uint32_t local = static_cast<uint32_t>(ka & 0xFFFFFFull);
+                const float arr = __uint_as_float(static_cast<uint32_t>((ka >> 24) & 0xFFFFFFFFull)) + St.delay[j];
+                const uint32_t slot = (static_cast<uint32_t>(Q.group0) + local / kWave) * kWave + (local & (kWave - 1));
+                keys[i] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) |
+                          (static_cast<unsigned long long>(j) << 28) | slot;
                 if (binned) atomicAdd(&s_bin[arrival_bin(arr, a0, inv_bw)], 1);
             }
         }
@@ -961,10 +1016,13 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     if (trow) trow[7] = wall_clock64();
 }
 
-__global__ void ev_set_kernel(int64_t* window, int64_t k, int32_t* stream_count, int n_streams, int clear) {
+__global__ void ev_set_kernel(int64_t* window, int64_t k, int32_t* stream_count, int n_streams, int64_t* hist_win,
+                              int n_hist, int clear) {
     *window = k;
-    if (clear)
+    if (clear) {
         for (int i = 0; i < n_streams; ++i) stream_count[i] = 0;
+        for (int i = 0; hist_win && i < n_hist; ++i) hist_win[i] = -1;
+    }
 }
 
 __global__ void ev_math_kernel(const float* x, float* ex, const float* u, float* lg, int64_t n) {
@@ -997,6 +1055,9 @@ struct n2h_ev_plan {
     int32_t* flags = nullptr;
     unsigned long long* counters = nullptr;
     int32_t* log_conn = nullptr;
+    unsigned long long* hist_key = nullptr;
+    int32_t* hist_off = nullptr;
+    int64_t* hist_win = nullptr;
     int64_t host_window = 0;
     bool bound = false;
 };
@@ -1013,7 +1074,8 @@ namespace {
 void free_plan(n2h_ev_plan* p) {
     if (!p) return;
     void* bufs[] = {p->pops_dev, p->streams_dev, p->reg_lane, p->reg_e, p->reg_count, p->stream_count, p->stream_slot,
-                    p->stream_a, p->keys, p->group_prefix, p->window, p->flags, p->counters, p->log_conn};
+                    p->stream_a, p->keys, p->group_prefix, p->window, p->flags, p->counters, p->log_conn,
+                    p->hist_key, p->hist_off, p->hist_win};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -1147,6 +1209,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
         regions += static_cast<int64_t>(t.n_groups) * t.cap;
     }
     int64_t stream_total = 0;
+    int32_t max_lag = 1;
     p->streams.resize(static_cast<size_t>(d->n_streams));
     for (int32_t si = 0; si < d->n_streams; ++si) {
         const n2h_ev_stream& s = d->streams[si];
@@ -1156,15 +1219,21 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
             free_plan(p);
             return fail(N2H_EINVAL, "stream %d: 1 to %d source populations", si, N2H_EV_MAX_SOURCES);
         }
-        if (s.delay != d->window) {
-            free_plan(p);
-            return fail(N2H_EINVAL, "stream %d: its delay must equal the window (one homogeneous delay)", si);
-        }
         t.n_src = s.n_sources;
-        t.delay = s.delay;
         t.base = stream_total;
         for (int j = 0; j < s.n_sources; ++j) {
             const int q = s.source_pop[j];
+            const double lagd = static_cast<double>(s.lag[j]) * static_cast<double>(d->window);
+            if (s.lag[j] < 1 || s.lag[j] > N2H_EV_MAX_LAG || !(s.delay[j] > 0.0f) ||
+                std::fabs(lagd - static_cast<double>(s.delay[j])) > 1e-5 * static_cast<double>(s.delay[j])) {
+                free_plan(p);
+                return fail(N2H_EINVAL, "stream %d source %d: delay %g must be lag %d (1..%d) windows of %g", si, j,
+                            static_cast<double>(s.delay[j]), s.lag[j], N2H_EV_MAX_LAG,
+                            static_cast<double>(d->window));
+            }
+            t.delay[j] = s.delay[j];
+            t.lag[j] = s.lag[j];
+            if (s.lag[j] > max_lag) max_lag = s.lag[j];
             for (int j2 = 0; j2 < j; ++j2)
                 if (s.source_pop[j2] == q) {
                     free_plan(p);
@@ -1199,7 +1268,10 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
         (rc = dev_alloc(&p->stream_slot, stream_total)) || (rc = dev_alloc(&p->stream_a, stream_total)) ||
         (rc = dev_alloc(&p->keys, keys_cap)) ||
         (rc = dev_alloc(&p->group_prefix, d->n_groups + 1)) || (rc = dev_alloc(&p->window, 1)) ||
-        (rc = dev_alloc(&p->flags, N2H_EV_FLAGS)) || (rc = dev_alloc(&p->counters, 2))) {
+        (rc = dev_alloc(&p->flags, N2H_EV_FLAGS)) || (rc = dev_alloc(&p->counters, 2)) ||
+        (max_lag > 1 && ((rc = dev_alloc(&p->hist_key, keys_cap * max_lag)) ||
+                         (rc = dev_alloc(&p->hist_off, static_cast<int64_t>(max_lag) * (d->n_pops + 1))) ||
+                         (rc = dev_alloc(&p->hist_win, max_lag))))) {
         free_plan(p);
         return rc;
     }
@@ -1235,6 +1307,17 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
     a.window = p->window;
     a.flags = p->flags;
     a.counters = p->counters;
+    a.n_hist = max_lag;
+    a.hist_key = p->hist_key;
+    a.hist_off = p->hist_off;
+    a.hist_win = p->hist_win;
+    if (max_lag > 1) {
+        std::vector<int64_t> none(static_cast<size_t>(max_lag), -1);
+        if (hipMemcpy(p->hist_win, none.data(), none.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+            free_plan(p);
+            return fail(N2H_EHIP, "clearing the window history");
+        }
+    }
     a.W = d->window;
     a.key0 = static_cast<uint32_t>(d->seed);
     a.key1 = static_cast<uint32_t>(d->seed >> 32);
@@ -1286,7 +1369,7 @@ int n2h_ev_bind_record(n2h_ev_plan* p, int32_t* node, float* time, int32_t* coun
 int n2h_ev_set_window(n2h_ev_plan* p, int64_t window, int32_t clear_inputs, void* stream) {
     if (!p || window < 0) return fail(N2H_EINVAL, "bad argument");
     hipLaunchKernelGGL(ev_set_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), p->window, window,
-                       p->stream_count, p->args.n_streams, clear_inputs ? 1 : 0);
+                       p->stream_count, p->args.n_streams, p->hist_win, p->args.n_hist, clear_inputs ? 1 : 0);
     EV_HIP(hipGetLastError());
     p->host_window = window;
     return N2H_OK;

@@ -6,9 +6,9 @@ Reference: test/networks/Brunel/neuron.py:6-45 and poisson.py:7-27 (the node typ
 (Network.step: every population runs every D, the sources' derived timestep), naer2/history.py:119-124
 (a run's spikes as (e, node)).
 
-Every connection must have one delay value, the same for all: the window W.  The spikes of window k are the
-inputs of window k + 1, so a window is two kernel launches (deliver, sort) with no host involvement; windows
-replay in captured hipGraphs.  There is no CPU fallback: without the HIP library or a GPU this raises.
+Every connection has one delay value, a whole multiple (its lag) of the smallest one, the window W.  The
+spikes of window k are the inputs of window k + lag through each connection, so a window is two kernel
+launches (deliver, sort) with no host involvement; windows replay in captured hipGraphs.  There is no CPU fallback: without the HIP library or a GPU this raises.
 """
 from __future__ import annotations
 
@@ -103,10 +103,18 @@ class EventRuntime:
             srcs.append(c.src.to(dev, torch.int64) + first_of[c.source])
             tgts.append(c.tgt.to(dev, torch.int64) + first_of[c.target])
             ws.append(c.weight.to(dev))
-        if delays and any(np.float32(d) != np.float32(delays[0]) for d in delays):
-            raise NotImplementedError(f"one homogeneous delay for every connection (got {sorted(set(delays))})")
-        W = float(np.float32(delays[0])) if delays else float(np.float32(getattr(net, "window", 1.0)))
+        # the window: the smallest delay; every connection's delay a whole number (its lag) of windows
+        W = float(min(np.float32(d) for d in delays)) if delays else float(np.float32(getattr(net, "window", 1.0)))
         self.window = W
+        self.conn_delay, self.conn_lag = [], []
+        for c in conns:
+            d = float(np.float32(_uniform(c.delay, repr(c))))
+            lag = int(round(d / W))
+            if lag < 1 or lag > _lib.EV_MAX_LAG or abs(lag * W - d) > 1e-5 * d:
+                raise NotImplementedError(f"{c!r}: delay {d} ms is not a whole number (<= {_lib.EV_MAX_LAG}) of "
+                                          f"windows of {W} ms (the smallest delay)")
+            self.conn_delay.append(d)
+            self.conn_lag.append(lag)
         cat = (lambda xs, t: torch.cat(xs) if xs else torch.zeros(0, dtype=t, device=dev))
         e_all = cat(srcs, torch.int64)
         canon = canonical(n, e_all, cat(tgts, torch.int64), cat(ws, torch.float32),
@@ -123,13 +131,14 @@ class EventRuntime:
         self.out_degree = canon.out_degree()
         self.canon = canon if net.keep_canonical else None
         del seg
-        # input streams: one per distinct list of source populations (in connection order)
+        # input streams: one per distinct list of (source population, delay, lag), in connection order
         self.streams: list[tuple] = []
         stream_of = {}
         for p in pops:
             if p.family != "event_lif":
                 continue
-            srcs_p = tuple(index[c.source] for c in conns if c.target is p)
+            srcs_p = tuple((index[c.source], self.conn_delay[ci], self.conn_lag[ci])
+                           for ci, c in enumerate(conns) if c.target is p)
             if not srcs_p:
                 stream_of[p] = -1
                 continue
@@ -212,9 +221,10 @@ class EventRuntime:
         streams = (_lib.EvStream * max(len(self.streams), 1))()
         for i, srcs in enumerate(self.streams):
             streams[i].n_sources = len(srcs)
-            for j, q in enumerate(srcs):
+            for j, (q, d, lag) in enumerate(srcs):
                 streams[i].source_pop[j] = q
-            streams[i].delay = self.window
+                streams[i].delay[j] = d
+                streams[i].lag[j] = lag
         first = (ctypes.c_int32 * (P + 1))(*[int(x) for x in self.pop_first])
         d = _lib.EvDesc(n_neurons=self.n, n_pops=P, n_groups=self.n_groups, n_streams=len(self.streams),
                         pop_first=first, pops=pops, streams=streams, window=self.window,
@@ -326,6 +336,10 @@ class EventRuntime:
             self._drain(self.window_index, w)
             self.window_index += w
             done += w
+
+    @property
+    def max_lag(self) -> int:
+        return max(self.conn_lag) if self.conn_lag else 1
 
     def advance(self, t_ms: float) -> None:
         """Advance by t_ms, a whole number of windows."""

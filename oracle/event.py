@@ -44,8 +44,9 @@ def _lib():
         lib.or_ev_interval.argtypes = [ctypes.c_uint32, i64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
                                        ctypes.c_float]
         lib.or_ev_interval.restype = ctypes.c_float
-        lib.or_ev_run.argtypes = [i32, i32, P, P, i32, P, P, ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32,
-                                  P, P, P, P, P, P, P, P, P, P, P, P, i64, i64, i32, P, P, i64, P, P, P, P, P, i64, P]
+        lib.or_ev_run.argtypes = [i32, i32, P, P, i32, P, P, P, P, ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32,
+                                  P, P, P, P, P, P, P, P, P, P, P, P, P, i64, i64, i32, P, P, i64, P, P, P, P, P, i64,
+                                  P]
         lib.or_ev_run.restype = ctypes.c_int
         _ready = True
     return lib
@@ -62,7 +63,7 @@ class EvNet:
     pops: list                      # EvPop per population
     conn_src: np.ndarray            # int32 [C]: source population of each connection (creation order)
     conn_tgt: np.ndarray            # int32 [C]
-    window: float                   # the common delay, ms
+    window: float                   # the smallest delay, ms (every delay a whole multiple of it)
     src_ptr: np.ndarray             # int64 [n + 1]: canonical CSR by source
     tgt: np.ndarray                 # int32 [E]
     w: np.ndarray                   # float32 [E]
@@ -70,10 +71,18 @@ class EvNet:
     instance: int = 0
     train_ptr: np.ndarray | None = None  # int64 [n + 1]
     train_t: np.ndarray | None = None    # float32
+    conn_delay: np.ndarray | None = None  # float32 [C]: each connection's delay (None: all the window)
 
     @property
     def n(self) -> int:
         return int(self.pop_first[-1])
+
+    def delays(self) -> tuple[np.ndarray, np.ndarray]:
+        """(delay fp32, lag = delay / window) per connection."""
+        d = (np.full(len(self.conn_src), self.window, np.float32) if self.conn_delay is None
+             else np.ascontiguousarray(self.conn_delay, np.float32))
+        lag = np.rint(d.astype(np.float64) / np.float64(np.float32(self.window))).astype(np.int32)
+        return d, lag
 
 
 @dataclass
@@ -85,6 +94,7 @@ class EvState:
     prev_node: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     prev_e: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     window: int = 0
+    prev_win: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
 
     @staticmethod
     def initial(net: EvNet, v0) -> "EvState":
@@ -94,7 +104,7 @@ class EvState:
 
     def copy(self) -> "EvState":
         return EvState(self.v.copy(), self.last.copy(), self.tre.copy(), self.nxt.copy(), self.prev_node.copy(),
-                       self.prev_e.copy(), self.window)
+                       self.prev_e.copy(), self.window, self.prev_win.copy())
 
 
 def run(net: EvNet, st: EvState, n_windows: int, record: bool = True, log: list | None = None,
@@ -112,11 +122,19 @@ def run(net: EvNet, st: EvState, n_windows: int, record: bool = True, log: list 
     w = np.ascontiguousarray(net.w, np.float32)
     tp = None if net.train_ptr is None else np.ascontiguousarray(net.train_ptr, np.int64)
     tt = None if net.train_t is None else np.ascontiguousarray(net.train_t, np.float32)
+    dly, lag = net.delays()
+    M = int(lag.max()) if lag.size else 1
     cap = max(n * 8, 1024) + (0 if tt is None else int(tt.size))
-    prev_node = np.zeros(cap, np.int32)
-    prev_e = np.zeros(cap, np.float32)
+    pcap = cap * M
+    prev_node = np.zeros(pcap, np.int32)
+    prev_e = np.zeros(pcap, np.float32)
+    prev_win = np.zeros(pcap, np.int64)
     prev_node[:st.prev_node.size] = st.prev_node
     prev_e[:st.prev_e.size] = st.prev_e
+    if st.prev_win.size == st.prev_node.size:
+        prev_win[:st.prev_win.size] = st.prev_win
+    else:  # (the spikes of the window before: a state made by hand)
+        prev_win[:st.prev_node.size] = st.window - 1
     prev_n = ctypes.c_int64(st.prev_node.size)
     rec_cap = cap * max(n_windows, 1) if record else 0
     rec_node = np.zeros(rec_cap, np.int32) if record else None
@@ -127,10 +145,11 @@ def run(net: EvNet, st: EvState, n_windows: int, record: bool = True, log: list 
     log_rec = np.zeros((max(log_cap, 1), 4), np.int32) if log is not None else None
     log_a = np.zeros(max(log_cap, 1), np.float32) if log is not None else None
     log_n = ctypes.c_int64(0)
-    rc = lib.or_ev_run(n, len(net.pops), _p(pf), pops, len(cs), _p(cs), _p(ct), ctypes.c_float(net.window),
-                       ctypes.c_uint64(net.seed), ctypes.c_uint32(net.instance), _p(sp), _p(tg), _p(w), _p(tp),
-                       _p(tt), _p(st.v), _p(st.last), _p(st.tre), _p(st.nxt), ctypes.byref(prev_n), _p(prev_node),
-                       _p(prev_e), cap, st.window, n_windows, _p(rec_node), _p(rec_e), rec_cap, ctypes.byref(rec_n),
+    rc = lib.or_ev_run(n, len(net.pops), _p(pf), pops, len(cs), _p(cs), _p(ct), _p(dly), _p(lag),
+                       ctypes.c_float(net.window), ctypes.c_uint64(net.seed), ctypes.c_uint32(net.instance), _p(sp),
+                       _p(tg), _p(w), _p(tp), _p(tt), _p(st.v), _p(st.last), _p(st.tre), _p(st.nxt),
+                       ctypes.byref(prev_n), _p(prev_node), _p(prev_e), _p(prev_win), pcap, st.window, n_windows,
+                       _p(rec_node), _p(rec_e), rec_cap, ctypes.byref(rec_n),
                        _p(wcount), ctypes.byref(events), _p(log_rec), _p(log_a), int(log_cap), ctypes.byref(log_n))
     if rc != 0:
         raise RuntimeError(f"or_ev_run failed ({rc})")
@@ -141,6 +160,7 @@ def run(net: EvNet, st: EvState, n_windows: int, record: bool = True, log: list 
             log.append((k, a, c, s_, t))
     st.prev_node = prev_node[:prev_n.value].copy()
     st.prev_e = prev_e[:prev_n.value].copy()
+    st.prev_win = prev_win[:prev_n.value].copy()
     st.window += n_windows
     k = rec_n.value
     nodes = rec_node[:k].copy() if record else None
@@ -208,6 +228,7 @@ def heap_run(net: EvNet, v0, n_windows: int, log: list | None = None):
     pop_of = np.searchsorted(pf, np.arange(n), side="right") - 1
     conn_of = {(int(s), int(t)): c for c, (s, t) in enumerate(zip(net.conn_src, net.conn_tgt))}
     W = F32(net.window)
+    dly, lag = net.delays()
     v = np.asarray(v0, np.float32).copy()
     last = np.zeros(n, np.float32)
     tre = np.zeros(n, np.float32)
@@ -216,13 +237,13 @@ def heap_run(net: EvNet, v0, n_windows: int, log: list | None = None):
 
     def emit(node: int, e, win: int) -> None:
         spikes[win].append((int(pop_of[node]), float(e), node))
-        if win + 1 >= n_windows:
-            return
-        a = F32(F32(e) + W)
         for link in range(int(net.src_ptr[node]), int(net.src_ptr[node + 1])):
             t = int(net.tgt[link])
             c = conn_of[(int(pop_of[node]), int(pop_of[t]))]
-            heapq.heappush(heap, (float(a), c, node, link, t, win + 1))
+            if win + int(lag[c]) >= n_windows:
+                continue
+            a = F32(F32(e) + F32(dly[c]))
+            heapq.heappush(heap, (float(a), c, node, link, t, win + int(lag[c])))
 
     for k in range(n_windows):
         t_start = F32(np.float64(k) * np.float64(W))

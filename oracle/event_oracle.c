@@ -148,15 +148,24 @@ static int check(int32_t i, int32_t pop, float t, const or_ev_pop* P, float* v, 
 }
 
 int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_pop* pops, int32_t n_conns,
-              const int32_t* conn_src, const int32_t* conn_tgt, float window, uint64_t seed, uint32_t instance,
+              const int32_t* conn_src, const int32_t* conn_tgt, const float* conn_delay, const int32_t* conn_lag,
+              float window, uint64_t seed, uint32_t instance,
               const int64_t* src_ptr, const int32_t* tgt, const float* w, const int64_t* train_ptr,
               const float* train_t, float* v, float* last, float* tre, float* nxt, int64_t* prev_n,
-              int32_t* prev_node, float* prev_e, int64_t prev_cap, int64_t k0, int32_t n_windows, int32_t* rec_node,
+              int32_t* prev_node, float* prev_e, int64_t* prev_win, int64_t prev_cap, int64_t k0, int32_t n_windows,
+              int32_t* rec_node,
               float* rec_e, int64_t rec_cap, int64_t* rec_n, int32_t* rec_wcount, int64_t* events, int32_t* log_rec,
               float* log_a, int64_t log_cap, int64_t* log_n) {
     if (n < 0 || n_pops < 1 || n_windows < 0 || !(window > 0.0f)) return -1;
+    int32_t M = 1;  /* windows of spikes kept: the largest lag */
+    for (int32_t c = 0; c < n_conns; ++c) {
+        if (conn_lag[c] < 1) return -1;
+        if (conn_lag[c] > M) M = conn_lag[c];
+    }
     int rc = 0;
-    spikes prev = {0, 0, 0}, cur = {0, 0, 0};
+    spikes cur = {0, 0, 0};
+    spikes* hist = (spikes*)calloc((size_t)M, sizeof(spikes));
+    int64_t* hist_w = (int64_t*)malloc((size_t)M * sizeof(int64_t));
     entry* ent = NULL;
     int64_t ent_cap = 0;
     int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
@@ -167,13 +176,22 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
     int32_t* in_s = NULL;
     int64_t in_cap = 0;
     int32_t* pop_at = (int32_t*)malloc(((size_t)n + 1) * sizeof(int32_t));
-    if (!cnt || !fill || !pop_at) {
+    if (!cnt || !fill || !pop_at || !hist || !hist_w) {
         rc = -2;
         goto done;
     }
     for (int32_t i = 0; i < n; ++i) pop_at[i] = pop_of(n_pops, pop_first, i);
-    for (int64_t j = 0; j < *prev_n; ++j)
-        if ((rc = push(&prev, pop_at[prev_node[j]], prev_e[j], prev_node[j]))) goto done;
+    for (int32_t j = 0; j < M; ++j) hist_w[j] = -1;
+    for (int64_t j = 0; j < *prev_n; ++j) {
+        const int64_t pw = prev_win[j];
+        if (pw < k0 - M || pw >= k0) continue;
+        const int32_t sl = (int32_t)(pw % M);
+        if (hist_w[sl] != pw) {
+            hist[sl].n = 0;
+            hist_w[sl] = pw;
+        }
+        if ((rc = push(&hist[sl], pop_at[prev_node[j]], prev_e[j], prev_node[j]))) goto done;
+    }
     if (events) *events = 0;
 
     for (int32_t kw = 0; kw < n_windows; ++kw) {
@@ -187,8 +205,11 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
             int64_t ne = 0;
             for (int32_t c = 0; c < n_conns; ++c) {
                 if (conn_tgt[c] != T) continue;
-                for (int64_t j = 0; j < prev.n; ++j) {
-                    if (prev.s[j].pop != conn_src[c]) continue;
+                const int64_t from = k - conn_lag[c];  /* the emission window whose spikes arrive now */
+                if (from < 0 || hist_w[from % M] != from) continue;
+                const spikes* prev = &hist[from % M];
+                for (int64_t j = 0; j < prev->n; ++j) {
+                    if (prev->s[j].pop != conn_src[c]) continue;
                     if (ne == ent_cap) {
                         ent_cap = ent_cap ? 2 * ent_cap : 1024;
                         entry* x = (entry*)realloc(ent, (size_t)ent_cap * sizeof(entry));
@@ -198,10 +219,10 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
                         }
                         ent = x;
                     }
-                    ent[ne].a = prev.s[j].e + window;
+                    ent[ne].a = prev->s[j].e + conn_delay[c];
                     ent[ne].c = c;
-                    ent[ne].e = prev.s[j].e;
-                    ent[ne].node = prev.s[j].node;
+                    ent[ne].e = prev->s[j].e;
+                    ent[ne].node = prev->s[j].node;
                     ++ne;
                 }
             }
@@ -332,21 +353,36 @@ int or_ev_run(int32_t n, int32_t n_pops, const int32_t* pop_first, const or_ev_p
             }
             *rec_n += cur.n;
         }
-        spikes t = prev;
-        prev = cur;
+        const int32_t sl = (int32_t)(k % M);
+        spikes t = hist[sl];
+        hist[sl] = cur;
+        hist_w[sl] = k;
         cur = t;
     }
-    if (prev.n > prev_cap) {
-        rc = -2;
-        goto done;
+    {
+        /* the spikes of the last M windows, oldest first */
+        int64_t np = 0;
+        const int64_t k_end = k0 + n_windows;
+        for (int64_t pw = k_end - M; pw < k_end; ++pw) {
+            if (pw < 0 || hist_w[pw % M] != pw) continue;
+            const spikes* h = &hist[pw % M];
+            if (np + h->n > prev_cap) {
+                rc = -2;
+                goto done;
+            }
+            for (int64_t j = 0; j < h->n; ++j, ++np) {
+                prev_node[np] = h->s[j].node;
+                prev_e[np] = h->s[j].e;
+                prev_win[np] = pw;
+            }
+        }
+        *prev_n = np;
     }
-    for (int64_t j = 0; j < prev.n; ++j) {
-        prev_node[j] = prev.s[j].node;
-        prev_e[j] = prev.s[j].e;
-    }
-    *prev_n = prev.n;
 done:
-    free(prev.s);
+    if (hist)
+        for (int32_t j = 0; j < M; ++j) free(hist[j].s);
+    free(hist);
+    free(hist_w);
     free(cur.s);
     free(ent);
     free(cnt);

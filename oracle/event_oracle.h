@@ -20,9 +20,10 @@
  *   BRUNEL.md / TODO.md:148-171 (D-1..D-3) exact arrival e + D; spikes inside delivery at the arrival time
  *
  * The contract (identical on the GPU):
- *   - Every connection has one delay, all equal: the window W.  Window k emits spikes; window k + 1 delivers
- *     all of them.  t_end(k) = (float)((double)(k + 1) * W), t_start(k) = (float)((double)k * W).
- *   - Per target neuron, the inputs of a window in ascending (a, c, source, link): a = e + W (fp32), c the
+ *   - Every connection c has one delay D_c, a whole multiple lag_c >= 1 of the window W (the smallest delay).
+ *     Window k emits spikes; window k + lag_c delivers them through c.  t_end(k) = (float)((double)(k + 1) * W),
+ *     t_start(k) = (float)((double)k * W).
+ *   - Per target neuron, the inputs of a window in ascending (a, c, source, link): a = e + D_c (fp32), c the
  *     connection index, link the canonical link index.  (The reference orders a connection's spikes of one
  *     arrival time by emission time, then source; the two differ only when two emission times round to the
  *     same fp32 arrival.)  For each input: if its a
@@ -60,21 +61,24 @@ float or_ev_log(float u);
 float or_ev_interval(uint32_t node, int64_t window, uint32_t m, uint64_t seed, uint32_t instance, float rate_inv);
 
 /* Runs n_windows windows from k0.  Connections: conn_src/conn_tgt population of each (one connection per
- * (source, target) population pair), in creation order.  Topology: canonical CSR by source over all
+ * (source, target) population pair), in creation order, conn_delay (ms, fp32) and conn_lag (= delay / window).  Topology: canonical CSR by source over all
  * connections.  Trains: train_ptr [n + 1] / train_t (sorted per source), or NULL.
  * State [n]: v, last, tre (t_ref_end), nxt (Poisson next spike, -1 before the first).
- * prev: the spikes of window k0 - 1 on entry, of the last window on return ((pop, e, node) order):
- *   *prev_n entries of prev_node / prev_e, capacity prev_cap.
+ * prev: the spikes of the M = max lag windows before k0 on entry, of the last M windows on return (each
+ *   window's in (pop, e, node) order, oldest window first): *prev_n entries of prev_node / prev_e / prev_win
+ *   (the emission window), capacity prev_cap.
  * rec_node / rec_e (capacity rec_cap, *rec_n advanced) and rec_wcount [n_windows]: every window's spikes in
  * (pop, e, node) order, or NULL.  events: inputs delivered (out).  log_rec / log_a (capacity log_cap, *log_n
  * advanced, or NULL): every delivered input as {window, connection, source, target} (indices in their
  * populations) and its arrival, per target in delivery order, targets ascending.  Returns 0, -1 on a bad
  * argument, -2 when memory or a capacity runs out. */
 int or_ev_run(int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_ev_pop* pops,
-              int32_t n_conns, const int32_t* conn_src, const int32_t* conn_tgt, float window, uint64_t seed,
+              int32_t n_conns, const int32_t* conn_src, const int32_t* conn_tgt, const float* conn_delay,
+              const int32_t* conn_lag, float window, uint64_t seed,
               uint32_t instance, const int64_t* src_ptr, const int32_t* tgt, const float* w,
               const int64_t* train_ptr, const float* train_t, float* v, float* last, float* tre, float* nxt,
-              int64_t* prev_n, int32_t* prev_node, float* prev_e, int64_t prev_cap, int64_t k0, int32_t n_windows,
+              int64_t* prev_n, int32_t* prev_node, float* prev_e, int64_t* prev_win, int64_t prev_cap, int64_t k0,
+              int32_t n_windows,
               int32_t* rec_node, float* rec_e, int64_t rec_cap, int64_t* rec_n, int32_t* rec_wcount,
               int64_t* events, int32_t* log_rec, float* log_a, int64_t log_cap, int64_t* log_n);
 

@@ -25,7 +25,8 @@ def oracle_net(net, canon=None):
         cat = (lambda xs, t: torch.cat(xs) if xs else torch.zeros(0, dtype=t))
         s = cat(src, torch.int64)
         canon = canonical(n, s, cat(tgt, torch.int64), cat(w, torch.float32), torch.ones(s.numel(), dtype=torch.int32))
-    window = float(np.float32(float(conns[0].delay.reshape(-1)[0]))) if conns else 1.0
+    delays = np.array([float(np.float32(float(c.delay.reshape(-1)[0]))) for c in conns], np.float32)
+    window = float(delays.min()) if conns else 1.0
     eps = []
     for p in pops:
         e = ev.EvPop()
@@ -65,7 +66,7 @@ def oracle_net(net, canon=None):
                      np.array([index[c.target] for c in conns], np.int32), window,
                      canon.src_ptr.cpu().numpy().astype(np.int64), canon.tgt.cpu().numpy().astype(np.int32),
                      canon.w.cpu().numpy().astype(np.float32), seed=int(net.seed), instance=int(net.instance),
-                     train_ptr=train_ptr, train_t=train_t)
+                     train_ptr=train_ptr, train_t=train_t, conn_delay=delays)
     return evnet, v0
 
 

@@ -140,7 +140,7 @@ def test_equal_arrivals_from_distinct_emissions_by_source():
     assert hv[0].view(np.uint32) == v.view(np.uint32)
 
 
-def _tiny_random(seed: int, with_trains: bool):
+def _tiny_random(seed: int, with_trains: bool, delays=None):
     import naer2_amd as n2
     import torch
 
@@ -163,11 +163,13 @@ def _tiny_random(seed: int, with_trains: bool):
         pops.append(S)
     E.V = torch.as_tensor(rng.uniform(0, 20, 24), dtype=torch.float32)
     I.V = torch.as_tensor(rng.uniform(0, 18, 6), dtype=torch.float32)
+    j = 0
     for tgt in (E, I):
         for src, K, w in ((E, 6, 1.2), (I, 2, -4.0), (P, 5, 1.5)) + (((pops[3], 2, 3.0),) if with_trains else ()):
             c = src >> n2.FixedInDegree(K) >> tgt
             c.weight = w
-            c.delay = 1.5
+            c.delay = 1.5 if delays is None else delays[j % len(delays)]
+            j += 1
     return net
 
 
@@ -207,6 +209,30 @@ def test_delivery_log_matches_event_queue():
     for r in hlog:
         by_h.setdefault((tpop[r[2]], r[4]), []).append(r)
     assert by_o == by_h
+
+
+@pytest.mark.parametrize("seed,delays", [(5, (1.5, 3.0, 4.5)), (6, (0.5, 1.0, 1.5, 0.5)), (7, (2.0, 1.0))])
+def test_several_delays_equal_one_event_queue(seed, delays):
+    """Connections with different delays (whole multiples of the smallest, the window; the reference's delay
+    classes, TODO.md:148-157): a spike reaches window k + delay / window, still in (arrival, connection,
+    source) order per target -- windows equal the global event queue bit for bit."""
+    net = _tiny_random(seed, True, delays)
+    evn, v0 = oracle_net(net)
+    assert evn.window == min(delays)
+    st = ev.EvState.initial(evn, v0)
+    n_windows = int(60 / evn.window)
+    nodes, tms, cnt, _ = ev.run(evn, st, n_windows)
+    hn, ht, hc, hv = ev.heap_run(evn, v0, n_windows)
+    assert cnt.sum() > 100
+    assert np.array_equal(cnt, hc)
+    assert np.array_equal(nodes, hn)
+    assert np.array_equal(tms.view(np.uint32), ht.view(np.uint32))
+    assert np.array_equal(st.v.view(np.uint32), hv.view(np.uint32))
+    # chunked runs carry the last windows' spikes
+    b = ev.EvState.initial(evn, v0)
+    parts = [ev.run(evn, b, k) for k in (n_windows // 3, 1, n_windows - n_windows // 3 - 1)]
+    assert np.array_equal(np.concatenate([p[0] for p in parts]), nodes)
+    assert np.array_equal(b.v.view(np.uint32), st.v.view(np.uint32))
 
 
 def test_chunked_runs_equal_one_run():

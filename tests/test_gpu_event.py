@@ -113,6 +113,44 @@ def test_ties_and_trains_parity(seed):
     _assert_parity(rt, st, nodes, tms, cnt)
 
 
+@pytest.mark.parametrize("seed,delays", [(5, (1.5, 3.0, 4.5)), (6, (0.5, 1.0, 1.5, 0.5))])
+def test_several_delays_parity(seed, delays):
+    """Connections with delays of several windows (the smallest delay is the window): a window's input stream
+    merges the sources' spikes of several past windows, from the sort kernel's history of windows."""
+    from tests.test_event_oracle import _tiny_random
+
+    net = _tiny_random(seed, True, delays)
+    net.device = torch.device("cuda")
+    evn, v0 = oracle_net(net)
+    n_win = int(60 / evn.window)
+    _say("several delays", seed, delays, n_win)
+    st, nodes, tms, cnt = _oracle(evn, v0, n_win)
+    rt = net.runtime
+    rt.run(n_win, 20)
+    _assert_parity(rt, st, nodes, tms, cnt)
+
+
+def test_brunel_several_delays_parity():
+    """Brunel with E delays of 1.5 ms, I delays of 3.0 ms and source delays of 0.75 ms (window 0.75 ms, lags
+    2, 4 and 1), 2,000 neurons, eager and captured windows, chunked: bit for bit with the oracle."""
+    from naer2_amd.networks import brunel_exact
+
+    net, pops = brunel_exact(2000, K=200, device="cuda", seed=4)
+    for c in net.connections:
+        c.delay = {"E": 1.5, "I": 3.0, "P": 0.75}[[k for k, p in pops.items() if p is c.source][0]]
+    evn, v0 = oracle_net(net)
+    assert evn.window == F32(0.75)
+    n_win = 160
+    _say("brunel several delays", n_win)
+    st, nodes, tms, cnt = _oracle(evn, v0, n_win)
+    rt = net.runtime
+    assert rt.max_lag == 4
+    rt.run(37, 0)
+    rt.run(100, 25)
+    rt.run(23, 0)
+    _assert_parity(rt, st, nodes, tms, cnt)
+
+
 def test_equal_arrivals_from_distinct_emissions():
     """Emission times 1022.5 and the next float both arrive at 1024.0: the GPU orders them by source neuron as
     the oracle does (the stream's bins hold equal arrivals together)."""
