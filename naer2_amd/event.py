@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -69,6 +70,84 @@ def region_cap(pop, window: float) -> int:
     return min(most, 1 << 20)
 
 
+@dataclass
+class EventPlan:
+    """The host-side description of an event-driven network (no GPU needed): populations in creation order,
+    the window, each connection's delay and lag, the input streams and the region capacities."""
+    pops: list
+    pop_first: np.ndarray
+    window: float
+    conn_delay: list
+    conn_lag: list
+    streams: list          # per stream: ((source population, delay, lag), ...) in connection order
+    stream_of: dict        # neuron population -> its stream (-1: no inputs)
+    caps: list             # region capacity per population
+
+    @property
+    def n(self) -> int:
+        return int(self.pop_first[-1])
+
+
+def describe(net) -> EventPlan:
+    """Validate an event-driven network and plan its windows and streams (raises ValueError /
+    NotImplementedError for what the event path does not run)."""
+    pops = sorted(net.populations, key=lambda p: p._order)
+    fams = {p.family for p in pops}
+    if fams - set(EVENT_FAMILIES):
+        raise NotImplementedError(f"event-driven and time-driven node types in one network: {sorted(fams)}")
+    if len(pops) > _lib.MAX_POPS:
+        raise ValueError(f"at most {_lib.MAX_POPS} populations")
+    sizes = [p.n for p in pops]
+    if any(sz >= 1 << 24 for sz in sizes):
+        raise ValueError("at most 2^24 - 1 neurons per population")
+    pop_first = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    index = {p: i for i, p in enumerate(pops)}
+    conns = list(net.connections)
+    pairs, raw = set(), []
+    for c in conns:
+        if c._weight is None or c._delay is None:
+            raise ValueError(f"{c!r}: weight and delay must be set")
+        if c.target.family != "event_lif":
+            raise ValueError(f"{c!r}: only event_lif populations receive inputs")
+        key = (index[c.source], index[c.target])
+        if key in pairs:
+            raise NotImplementedError("at most one connection per (source, target) population pair")
+        pairs.add(key)
+        raw.append(float(np.float32(_uniform(c.delay, repr(c)))))
+    # the window: the smallest delay; every connection's delay a whole number (its lag) of windows
+    delays = [d for d in raw if not math.isnan(d)]
+    W = float(min(delays)) if delays else float(np.float32(getattr(net, "window", 1.0)))
+    conn_delay, conn_lag = [], []
+    for c, d in zip(conns, raw):
+        if math.isnan(d):
+            d = W
+        lag = int(round(d / W))
+        if lag < 1 or lag > _lib.EV_MAX_LAG or abs(lag * W - d) > 1e-5 * d:
+            raise NotImplementedError(f"{c!r}: delay {d} ms is not a whole number (<= {_lib.EV_MAX_LAG}) of "
+                                      f"windows of {W} ms (the smallest delay)")
+        conn_delay.append(d)
+        conn_lag.append(lag)
+    # input streams: one per distinct list of (source population, delay, lag), in connection order
+    streams: list = []
+    stream_of = {}
+    for p in pops:
+        if p.family != "event_lif":
+            continue
+        srcs_p = tuple((index[c.source], conn_delay[ci], conn_lag[ci]) for ci, c in enumerate(conns) if c.target is p)
+        if not srcs_p:
+            stream_of[p] = -1
+            continue
+        if len(srcs_p) > _lib.EV_MAX_SOURCES:
+            raise NotImplementedError(f"at most {_lib.EV_MAX_SOURCES} source populations per target")
+        if srcs_p not in streams:
+            streams.append(srcs_p)
+        stream_of[p] = streams.index(srcs_p)
+    for p in pops:
+        if p.family == "event_train" and len(p.instance.times) != p.n:
+            raise ValueError(f"{p!r}: {len(p.instance.times)} spike trains for {p.n} sources")
+    return EventPlan(pops, pop_first, W, conn_delay, conn_lag, streams, stream_of, [region_cap(p, W) for p in pops])
+
+
 class EventRuntime:
     """The built event-driven network: device tensors, the C plan, captured windows, the drained record."""
 
@@ -78,43 +157,18 @@ class EventRuntime:
         if self.device.type != "cuda":
             raise RuntimeError("naer2_amd runs on the GPU: no CUDA/HIP device is available")
         dev = self.device
-        pops = sorted(net.populations, key=lambda p: p._order)
-        self.pops = pops
+        plan = describe(net)
+        pops = self.pops = plan.pops
+        self.pop_first = plan.pop_first
+        self.n = n = plan.n
+        W = self.window = plan.window
+        self.conn_delay, self.conn_lag = plan.conn_delay, plan.conn_lag
         sizes = [p.n for p in pops]
-        self.pop_first = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
-        self.n = n = int(self.pop_first[-1])
-        index = {p: i for i, p in enumerate(pops)}
         first_of = {p: int(f) for p, f in zip(pops, self.pop_first[:-1])}
         conns = list(net.connections)
-        delays, pairs = [], set()
-        srcs, tgts, ws = [], [], []
-        for c in conns:
-            if c._weight is None or c._delay is None:
-                raise ValueError(f"{c!r}: weight and delay must be set")
-            if c.target.family != "event_lif":
-                raise ValueError(f"{c!r}: only event_lif populations receive inputs")
-            key = (index[c.source], index[c.target])
-            if key in pairs:
-                raise NotImplementedError("at most one connection per (source, target) population pair")
-            pairs.add(key)
-            d = _uniform(c.delay, repr(c))
-            if not math.isnan(d):
-                delays.append(d)
-            srcs.append(c.src.to(dev, torch.int64) + first_of[c.source])
-            tgts.append(c.tgt.to(dev, torch.int64) + first_of[c.target])
-            ws.append(c.weight.to(dev))
-        # the window: the smallest delay; every connection's delay a whole number (its lag) of windows
-        W = float(min(np.float32(d) for d in delays)) if delays else float(np.float32(getattr(net, "window", 1.0)))
-        self.window = W
-        self.conn_delay, self.conn_lag = [], []
-        for c in conns:
-            d = float(np.float32(_uniform(c.delay, repr(c))))
-            lag = int(round(d / W))
-            if lag < 1 or lag > _lib.EV_MAX_LAG or abs(lag * W - d) > 1e-5 * d:
-                raise NotImplementedError(f"{c!r}: delay {d} ms is not a whole number (<= {_lib.EV_MAX_LAG}) of "
-                                          f"windows of {W} ms (the smallest delay)")
-            self.conn_delay.append(d)
-            self.conn_lag.append(lag)
+        srcs = [c.src.to(dev, torch.int64) + first_of[c.source] for c in conns]
+        tgts = [c.tgt.to(dev, torch.int64) + first_of[c.target] for c in conns]
+        ws = [c.weight.to(dev) for c in conns]
         cat = (lambda xs, t: torch.cat(xs) if xs else torch.zeros(0, dtype=t, device=dev))
         e_all = cat(srcs, torch.int64)
         canon = canonical(n, e_all, cat(tgts, torch.int64), cat(ws, torch.float32),
@@ -131,23 +185,9 @@ class EventRuntime:
         self.out_degree = canon.out_degree()
         self.canon = canon if net.keep_canonical else None
         del seg
-        # input streams: one per distinct list of (source population, delay, lag), in connection order
-        self.streams: list[tuple] = []
-        stream_of = {}
-        for p in pops:
-            if p.family != "event_lif":
-                continue
-            srcs_p = tuple((index[c.source], self.conn_delay[ci], self.conn_lag[ci])
-                           for ci, c in enumerate(conns) if c.target is p)
-            if not srcs_p:
-                stream_of[p] = -1
-                continue
-            if len(srcs_p) > _lib.EV_MAX_SOURCES:
-                raise NotImplementedError(f"at most {_lib.EV_MAX_SOURCES} source populations per target")
-            if srcs_p not in self.streams:
-                self.streams.append(srcs_p)
-            stream_of[p] = self.streams.index(srcs_p)
-        self.caps = [region_cap(p, W) for p in pops]
+        self.streams = plan.streams
+        stream_of = plan.stream_of
+        self.caps = plan.caps
         # state
         self.v = torch.zeros(n, dtype=torch.float32, device=dev)
         self.last = torch.zeros(n, dtype=torch.float32, device=dev)

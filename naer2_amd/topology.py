@@ -54,6 +54,37 @@ def delay_steps(delay_ms: torch.Tensor, dt: float) -> torch.Tensor:
     return steps.to(torch.int32)
 
 
+SORT_CHUNK = 1 << 30  # links sorted at once (torch's sort takes at most 2^31 - 1 elements)
+
+
+def _stable_order_by_key(key: torch.Tensor, n_keys: int, chunk: int | None = None) -> torch.Tensor:
+    """The positions a stable sort by key (int64 in [0, n_keys)) gives each element: chunk by chunk, a stable
+    sort within the chunk, then each key's run placed after the same key's elements of earlier chunks."""
+    chunk = SORT_CHUNK if chunk is None else chunk
+    E = key.numel()
+    dev = key.device
+    if E <= chunk:
+        order = torch.argsort(key, stable=True)
+        pos = torch.empty_like(order)
+        pos[order] = torch.arange(E, device=dev)
+        return pos
+    counts = torch.zeros(n_keys, dtype=torch.int64, device=dev)
+    for a in range(0, E, chunk):
+        counts += torch.bincount(key[a:a + chunk], minlength=n_keys)
+    base = torch.cumsum(counts, 0) - counts  # the first position of each key
+    pos = torch.empty(E, dtype=torch.int64, device=dev)
+    for a in range(0, E, chunk):
+        k = key[a:a + chunk]
+        order = torch.argsort(k, stable=True)
+        ks = k[order]
+        cnt = torch.bincount(ks, minlength=n_keys)
+        first = torch.cumsum(cnt, 0) - cnt
+        pos[a + order] = base[ks] + (torch.arange(ks.numel(), device=dev) - first[ks])
+        base += cnt
+        del order, ks, cnt, first
+    return pos
+
+
 def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: torch.Tensor) -> Canonical:
     """The canonical CSR of links given in any source order (a stable sort by source keeps each source's
     links in their given order)."""
@@ -64,12 +95,17 @@ def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: 
         raise ValueError("link target out of range")
     if d.numel() and int(d.min()) < 1:
         raise ValueError("link delays must be at least one step")
-    order = torch.argsort(src, stable=True)
     counts = torch.bincount(src, minlength=n)
     src_ptr = torch.zeros(n + 1, dtype=torch.int64, device=src.device)
     src_ptr[1:] = torch.cumsum(counts, 0)
-    return Canonical(n, src_ptr, tgt[order].to(torch.int32), w[order].to(torch.float32),
-                     d[order].to(torch.int32))
+    pos = _stable_order_by_key(src, n)
+    out_t = torch.empty(tgt.numel(), dtype=torch.int32, device=src.device)
+    out_w = torch.empty(tgt.numel(), dtype=torch.float32, device=src.device)
+    out_d = torch.empty(tgt.numel(), dtype=torch.int32, device=src.device)
+    out_t[pos] = tgt.to(torch.int32)
+    out_w[pos] = w.to(torch.float32)
+    out_d[pos] = d.to(torch.int32)
+    return Canonical(n, src_ptr, out_t, out_w, out_d)
 
 
 @dataclass
@@ -128,15 +164,33 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
     tg = canon.tgt.to(torch.int64)
     key = src * G + group_of[tg]
     del src
-    order = torch.argsort(key, stable=True)
-    counts = torch.bincount(key, minlength=slots * G)
-    del key
+    counts = torch.zeros(slots * G, dtype=torch.int64, device=dev)
+    for a in range(0, key.numel(), SORT_CHUNK):
+        counts += torch.bincount(key[a:a + SORT_CHUNK], minlength=slots * G)
     seg_off = torch.zeros(slots * G + 1, dtype=torch.int64, device=dev)
     seg_off[1:] = torch.cumsum(counts, 0)
     del counts
-    x = (canon.d.to(torch.int64)[order] << 16) | local_of[tg[order]]
-    y = canon.w[order].contiguous().view(torch.int32)
-    links = torch.stack([x.to(torch.int32), y], 1).contiguous()
+    # (links are grouped by source already: a stable sort by (source slot, group) chunk by chunk of sources)
+    links = torch.empty(key.numel(), 2, dtype=torch.int32, device=dev)
+    E = key.numel()
+    sp = canon.src_ptr
+    a = 0
+    while a < E:
+        # a chunk of whole sources, at most SORT_CHUNK links (one source's links always fit)
+        b = min(E, a + SORT_CHUNK)
+        if b < E:
+            b = int(sp[int(torch.searchsorted(sp, torch.tensor(b, device=dev), right=True)) - 1])
+            if b <= a:
+                raise ValueError(f"a source with more than {SORT_CHUNK} links")
+        k = key[a:b]
+        order = torch.argsort(k, stable=True)
+        t = tg[a:b][order]
+        x = (canon.d[a:b].to(torch.int64)[order] << 16) | local_of[t]
+        links[a:b, 0] = x.to(torch.int32)
+        links[a:b, 1] = canon.w[a:b][order].contiguous().view(torch.int32)
+        del k, order, t, x
+        a = b
+    del key
     return Segments(seg_off, links)
 
 

@@ -245,3 +245,25 @@ def test_rows_flag_a_uniform_delay():
     d = torch.full((900,), 15, dtype=torch.int32)
     r = rows(canonical(n, src, tgt, w, d), group_layout([n]))
     assert r.uniform_delay == 15 and r.fmt == ROWS_COMPACT
+
+
+def test_tables_built_in_chunks_equal_whole(monkeypatch):
+    """Past 2^30 links the canonical and segment tables are sorted chunk by chunk (torch's sort takes at most
+    2^31 - 1 elements): forced on a small network, the tables are identical."""
+    from naer2_amd import topology as T
+
+    g = torch.Generator().manual_seed(0)
+    n, E = 500, 20000
+    src = torch.randint(0, n, (E,), generator=g)
+    tgt = torch.randint(0, n, (E,), generator=g)
+    w = torch.randn(E, generator=g)
+    d = torch.randint(1, 5, (E,), generator=g).to(torch.int32)
+    lay = T.group_layout([300, 200])
+    c1 = T.canonical(n, src, tgt, w, d)
+    s1 = T.segments(c1, lay)
+    monkeypatch.setattr(T, "SORT_CHUNK", 997)
+    c2 = T.canonical(n, src, tgt, w, d)
+    s2 = T.segments(c2, lay)
+    for a, b in ((c1.src_ptr, c2.src_ptr), (c1.tgt, c2.tgt), (c1.w, c2.w), (c1.d, c2.d),
+                 (s1.seg_off, s2.seg_off), (s1.links, s2.links)):
+        assert torch.equal(a, b)
