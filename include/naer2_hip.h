@@ -58,12 +58,23 @@ extern "C" {
 #define N2H_GROUP 64        /* neurons per target group: one wavefront */
 #define N2H_POISSON_K 32    /* entries of the Poisson inversion table */
 #define N2H_MAX_POPS 64
-#define N2H_MAX_SLOTS 128   /* ring slots = the largest delay in steps */
+#define N2H_MAX_SLOTS 128   /* ring slots: the largest delay in steps + the largest stride - 1 */
+#define N2H_MAX_STRIDE 64   /* steps per update of a population with its own, coarser timestep */
 
-/* Parameters of one population (fp32 constants precomputed on the host for the fixed dt). */
+/* Parameters of one population (fp32 constants precomputed on the host for its update interval).
+ *
+ * Multi-rate (naer2/population.py:191-204 _own_timestep, 382-410 _execute_update_kernel; naer2/network.py:80-152):
+ * a population whose timestep is stride * dt runs windows of stride steps.  It updates at the steps s with
+ * (s + 1) % stride == 0 only, once, with every parameter below computed for stride * dt (propagators, the
+ * Poisson mean, ref_steps counted in its own updates: test/networks/PD14/neuron.py:37-49 integrates over
+ * time - last_time).  Its input is the sum of the ring slots of its window, ((r[s-stride+1] + r[s-stride+2])
+ * + ... + r[s-1]) + r[s]; it fires at its update steps only, spikes delivered as every other with their
+ * delay from the step of the update.  At its other steps the step kernel only takes its intake into the
+ * ring.  d_slots must be at least the largest delay + the largest stride - 1, so that no arrival lands in
+ * a slot of a window before it is read. */
 typedef struct n2h_pop_params {
-    int32_t ref_steps;   /* steps a neuron stays refractory after a spike */
-    int32_t reserved;
+    int32_t ref_steps;   /* updates a neuron stays refractory after a spike */
+    int32_t stride;      /* steps per update: 0 or 1 every step, else at most N2H_MAX_STRIDE */
     float decay;         /* LIF_DELTA: exp(-dt/tau_m); LIF_EXP: P_V = exp(-dt/tau_m) */
     float p_i;           /* LIF_EXP: P_I = exp(-dt/tau_syn) */
     float p_vi;          /* LIF_EXP: R tau_syn/(tau_m - tau_syn) (P_V - P_I) */
@@ -82,7 +93,8 @@ typedef struct n2h_model_desc {
     int32_t model;                 /* N2H_MODEL_* */
     int32_t n_neurons;
     int32_t n_pops;
-    int32_t d_slots;               /* ring slots: the largest delay in steps (every delay in [1, d_slots]) */
+    int32_t d_slots;               /* ring slots: the largest delay in steps (every delay in [1, d_slots]),
+                                    * + the largest population stride - 1 */
     const int32_t* pop_first;      /* host [n_pops + 1] */
     const n2h_pop_params* pops;    /* host [n_pops] */
     uint64_t seed;                 /* Philox key */

@@ -19,6 +19,7 @@ GROUP = 64
 POISSON_K = 32
 MAX_POPS = 64
 MAX_SLOTS = 128
+MAX_STRIDE = 64
 DENSE_GROUPS = 1535  # row intake: spike bitmaps of at most this many groups fit in LDS
 
 # every symbol include/naer2_hip.h declares
@@ -41,7 +42,7 @@ EV_FLAGS = ("region", "segment", "record", "sort")
 
 class PopParams(ctypes.Structure):
     _fields_ = [
-        ("ref_steps", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("ref_steps", ctypes.c_int32), ("stride", ctypes.c_int32),
         ("decay", ctypes.c_float), ("p_i", ctypes.c_float), ("p_vi", ctypes.c_float),
         ("v_rest", ctypes.c_float), ("v_th", ctypes.c_float), ("v_reset", ctypes.c_float),
         ("j_ext", ctypes.c_float), ("reserved_f", ctypes.c_float),

@@ -94,6 +94,8 @@ class RateLossGrad:
                 raise ValueError("RateLossGrad: the networks must record their spikes (Network.record)")
             if rt.n > (1 << 17):
                 raise ValueError("RateLossGrad: at most 2^17 neurons a network")
+            if any(p.stride > 1 for p in rt.params):
+                raise ValueError("RateLossGrad: every population must update every step (no timestep of its own)")
         if len({rt.step for rt in runtimes}) != 1:
             raise ValueError("RateLossGrad: the networks are at different steps")
         if checkpoint < 1:

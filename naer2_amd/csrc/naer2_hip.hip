@@ -106,7 +106,7 @@ struct StepArgs {
     float* vtrace;              // null, or [vtrace_steps][n_neurons]: the potential before the threshold of
     int64_t vtrace_step0;       //   steps vtrace_step0 .. + vtrace_steps - 1 (+inf on refractory steps)
     int32_t vtrace_steps;
-    int32_t pad4;
+    int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
 };
 
 // Inclusive prefix sum over the 64 lanes of the wavefront.
@@ -505,7 +505,12 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     const int64_t s = s0 + k;
     const int s_mod = static_cast<int>(s % D);  // ring slot of step s; delays stay in 32-bit arithmetic
     const int64_t cur = static_cast<int64_t>(s_mod) * N + nid;
-    if (live) x = a.ring[cur];
+    // multi-rate: a population with stride k > 1 updates at the steps s with (s + 1) % k == 0 only, over the
+    // sum of its window's k ring slots; at its other steps it only takes its intake into the ring
+    int stride = 1;
+    if (a.multirate) stride = __builtin_amdgcn_readfirstlane(a.pops[pop].stride);
+    const bool upd = stride <= 1 || static_cast<uint64_t>(s + 1) % static_cast<uint64_t>(stride) == 0ull;
+    if (live && upd) x = a.ring[cur];
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
     if (KW > 0 && a.rows && n_prev >= a.dense_min) {
@@ -610,7 +615,18 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     const n2h_pop_params& P = *reinterpret_cast<const n2h_pop_params*>(st.param_words);
     bool fired = false;
     float vpre = __builtin_inff();  // the potential the threshold sees (+inf: refractory)
-    if (live) {
+    if (live && upd) {
+        if (stride > 1) {
+            // the window's earlier slots, oldest first: ((r[s-k+1] + r[s-k+2]) + ... + r[s-1]) + r[s]
+            int q = s_mod - (stride - 1);
+            if (q < 0) q += D;
+            float xw = a.ring[static_cast<int64_t>(q) * N + nid];
+            for (int j = 2; j < stride; ++j) {
+                if (++q == D) q = 0;
+                xw = xw + a.ring[static_cast<int64_t>(q) * N + nid];
+            }
+            x = xw + x;
+        }
         const uint32_t u = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
                                         static_cast<uint32_t>(static_cast<uint64_t>(s) >> 32), a.instance,
                                         a.key0, a.key1);
@@ -673,16 +689,24 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
 
     // ---- every remaining write last: nothing after them waits on the memory counter ----
     if (live) {
-        a.v[nid] = V;
-        a.ref[nid] = R;
-        if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
-        a.ring[cur] = 0.0f;
+        if (upd) {
+            a.v[nid] = V;
+            a.ref[nid] = R;
+            if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
+            a.ring[cur] = 0.0f;
+            int q = s_mod;
+            for (int j = 1; j < stride; ++j) {  // the window's earlier slots, read above
+                q = (q == 0) ? D - 1 : q - 1;
+                a.ring[static_cast<int64_t>(q) * N + nid] = 0.0f;
+            }
+        }
         if (a.vtrace) {
             const int64_t q = s - a.vtrace_step0;
             if (q >= 0 && q < a.vtrace_steps) a.vtrace[q * N + nid] = vpre;
         }
-        // ring: add each touched cell's partial sum (arrival s - 1 + (b + 1), b < D); the cell's one add
-        unsigned long long m = t0 & ~1ull;
+        // ring: add each touched cell's partial sum (arrival s - 1 + (b + 1), b < D); the cell's one add.
+        // Without an update the delay-1 cell (arrival s) stays in the ring for the window's sum.
+        unsigned long long m = upd ? (t0 & ~1ull) : t0;
         while (m) {
             const int b = __ffsll(static_cast<long long>(m)) - 1;
             m &= m - 1ull;
@@ -782,6 +806,7 @@ struct n2h_plan {
     float* vtrace;
     int64_t vtrace_step0;
     int32_t vtrace_steps;
+    int32_t multirate;      // some population's stride > 1
     size_t lds_bytes;
     int device;
 };
@@ -854,6 +879,7 @@ StepArgs make_args(const n2h_plan* p) {
     a.vtrace = p->vtrace;
     a.vtrace_step0 = p->vtrace_step0;
     a.vtrace_steps = p->vtrace_steps;
+    a.multirate = p->multirate;
     return a;
 }
 
@@ -1001,6 +1027,10 @@ int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_pl
             if (desc->pops[p].poisson_thr[j] < desc->pops[p].poisson_thr[j - 1])
                 return fail(N2H_EINVAL, "population %d: poisson_thr not non-decreasing", p);
         if (desc->pops[p].ref_steps < 0) return fail(N2H_EINVAL, "population %d: negative ref_steps", p);
+        const int stride = desc->pops[p].stride;
+        if (stride < 0 || stride > N2H_MAX_STRIDE || stride > desc->d_slots)
+            return fail(N2H_EINVAL, "population %d: stride %d outside [0, min(%d, d_slots)]", p, stride,
+                        N2H_MAX_STRIDE);
     }
     if (tables->n_groups <= 0) return fail(N2H_EINVAL, "n_groups must be positive");
     if (tables->uniform_delay < 0 || tables->uniform_delay > desc->d_slots)
@@ -1025,6 +1055,7 @@ int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_pl
     memcpy(p->pop_first_host, desc->pop_first, sizeof(int32_t) * (desc->n_pops + 1));
     p->desc.pop_first = p->pop_first_host;
     p->desc.pops = nullptr;
+    for (int q = 0; q < desc->n_pops; ++q) p->multirate |= desc->pops[q].stride > 1 ? 1 : 0;
     // partial sums: one cell per delay, or the one cell of a uniform delay
     p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(tables->uniform_delay > 0 ? 1 : desc->d_slots);
     hipError_t e = hipGetDevice(&p->device);
@@ -1298,6 +1329,8 @@ int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, co
 int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     if (buffer && n_steps <= 0) return fail(N2H_EINVAL, "n_steps must be positive");
+    if (buffer && p->multirate)
+        return fail(N2H_EINVAL, "potential traces need every population updating every step (stride 1)");
     p->vtrace = buffer;
     p->vtrace_step0 = step0;
     p->vtrace_steps = buffer ? n_steps : 0;

@@ -118,6 +118,9 @@ class Population:
         object.__setattr__(self, "_init", {})
         object.__setattr__(self, "_order", next(_ORDER))
         object.__setattr__(self, "_net_slice", None)  # (network, first neuron) once built
+        # its own clock (naer2/population.py:191-204): None updates every step of the network's resolution;
+        # a whole multiple of it updates once per that many steps
+        object.__setattr__(self, "timestep", None)
         from .network import Network
         object.__setattr__(self, "network", Network(self))
 
@@ -145,6 +148,8 @@ class Population:
             else:
                 self._init[name] = t.clone()
             return
+        if name == "timestep" and self._net_slice is not None and value != self.timestep:
+            raise RuntimeError("the network is built: set timestep before its first step")
         object.__setattr__(self, name, value)
 
     def __getattr__(self, name):

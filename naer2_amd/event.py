@@ -97,6 +97,10 @@ def describe(net) -> EventPlan:
         raise NotImplementedError(f"event-driven and time-driven node types in one network: {sorted(fams)}")
     if len(pops) > _lib.MAX_POPS:
         raise ValueError(f"at most {_lib.MAX_POPS} populations")
+    clocked = [p for p in pops if getattr(p, "timestep", None) is not None]
+    if clocked:
+        raise NotImplementedError(f"{clocked[0]!r}: event-driven populations run at their inputs' exact times, "
+                                  f"not on a timestep of their own")
     sizes = [p.n for p in pops]
     if any(sz >= 1 << 24 for sz in sizes):
         raise ValueError("at most 2^24 - 1 neurons per population")

@@ -11,6 +11,7 @@ captured into hipGraphs and replayed.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -189,7 +190,12 @@ def assemble(net: Network, device=None) -> Assembled:
     pop_first = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
     n = int(pop_first[-1])
     first_of = {p: int(f) for p, f in zip(pops, pop_first[:-1])}
-    params = [p.instance.params(dt) for p in pops]
+    strides = [pop_stride(p, dt) for p in pops]
+    params = []
+    for p, k in zip(pops, strides):
+        par = p.instance.params(k * dt)  # propagators, Poisson mean and refractory updates for its interval
+        par.stride = k
+        params.append(par)
     srcs, tgts, ws, ds = [], [], [], []
     for c in net.connections:
         if c._weight is None or c._delay is None:
@@ -212,7 +218,27 @@ def assemble(net: Network, device=None) -> Assembled:
         ref[sl] = p._init.get("ref", torch.tensor(0)).to(dev, torch.int32)
         if isyn is not None:
             isyn[sl] = p._init.get("current", torch.tensor(float(p.instance.current))).to(dev, torch.float32)
-    return Assembled(pops, family, model, dt, pop_first, params, canon, canon.max_delay, v, isyn, ref)
+    # a window's slots stay unwritten by later arrivals until the window's update reads them
+    d_slots = canon.max_delay + max(strides) - 1
+    if d_slots > _lib.MAX_SLOTS:
+        raise ValueError(f"largest delay {canon.max_delay} steps + largest stride {max(strides)} - 1 exceeds "
+                         f"{_lib.MAX_SLOTS} ring slots")
+    return Assembled(pops, family, model, dt, pop_first, params, canon, d_slots, v, isyn, ref)
+
+
+def pop_stride(pop, dt: float) -> int:
+    """Steps of `dt` per update of `pop`: 1 without a timestep of its own, else timestep / dt, which must be
+    a whole number (naer2/population.py:191-204: a clocked population runs windows of its timestep)."""
+    ts = getattr(pop, "timestep", None)
+    if ts is None:
+        return 1
+    k = float(ts) / dt
+    if not math.isfinite(k) or k < 1 or abs(k - round(k)) > 1e-6 * k:
+        raise ValueError(f"{pop!r}: timestep {ts} is not a whole multiple (>= 1) of the resolution {dt}")
+    k = int(round(k))
+    if k > _lib.MAX_STRIDE:
+        raise ValueError(f"{pop!r}: timestep {ts} is more than {_lib.MAX_STRIDE} steps of {dt}")
+    return k
 
 
 class Runtime:

@@ -35,6 +35,16 @@ int or_poisson_count(uint32_t u, const uint32_t* thr) {
     return k;
 }
 
+/* 1 if some population updates every stride > 1 steps; -1 if a stride is out of range. */
+static int multirate(int32_t n_pops, const or_pop* pops, int32_t d_slots) {
+    int any = 0;
+    for (int32_t p = 0; p < n_pops; ++p) {
+        if (pops[p].stride < 0 || pops[p].stride > OR_MAX_STRIDE || pops[p].stride > d_slots) return -1;
+        any |= pops[p].stride > 1;
+    }
+    return any;
+}
+
 static int pop_of(int32_t i, int32_t n_pops, const int32_t* pop_first) {
     int p = 0;
     while (p + 1 < n_pops && i >= pop_first[p + 1]) ++p;
@@ -49,6 +59,8 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
     if (n_neurons < 0 || n_pops < 1 || d_slots < 1 || n_steps < 0) return -1;
     if (model != OR_LIF_DELTA && model != OR_LIF_EXP) return -1;
     if (model == OR_LIF_EXP && !isyn) return -1;
+    const int mr = multirate(n_pops, pops, d_slots);
+    if (mr < 0 || (mr && accum_mode != 0)) return -1;
     const int64_t N = n_neurons, D = d_slots;
     /* per-(slot, target) partial sums of one emission step, with a touched list */
     float* acc = (float*)malloc(sizeof(float) * (size_t)(N * D + 1));
@@ -95,8 +107,28 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
         const int64_t slot = (s % D) * N;
         for (int64_t i = 0; i < N; ++i) {
             const or_pop* P = &pops[pop[i]];
-            const float x = ring[slot + i];
+            /* multi-rate (population.py:382-410): stride k > 1 updates at (s + 1) % k == 0 only, over its
+             * window's slots oldest first, ((r[s-k+1] + r[s-k+2]) + ... + r[s-1]) + r[s]; other steps leave
+             * the ring and the state as they are and emit nothing */
+            const int64_t k_i = P->stride > 1 ? P->stride : 1;
+            if ((s + 1) % k_i != 0) {
+                prev_spk[i] = 0;
+                if (raster) raster[(int64_t)k * N + i] = 0;
+                continue;
+            }
+            float x = ring[slot + i];
             ring[slot + i] = 0.0f;
+            if (k_i > 1) {
+                int64_t q = (s - k_i + 1) % D;
+                float xw = ring[q * N + i];
+                ring[q * N + i] = 0.0f;
+                for (int64_t j = 2; j < k_i; ++j) {
+                    q = (q + 1) % D;
+                    xw = xw + ring[q * N + i];
+                    ring[q * N + i] = 0.0f;
+                }
+                x = xw + x;
+            }
             const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)((uint64_t)s >> 32), instance};
             uint32_t r[4];
             or_philox4x32_10(ctr, key, r);
@@ -238,6 +270,7 @@ int or_mt_run(or_mt* m, int model, int32_t n_pops, const int32_t* pop_first, con
               uint64_t seed, uint32_t instance, float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
               int64_t step0, int32_t n_steps, uint8_t* raster, int64_t* events) {
     if (!m || n_pops < 1 || d_slots < 1 || n_steps < 0) return -1;
+    if (multirate(n_pops, pops, d_slots) != 0) return -1; /* every population at stride 1 */
     if (model != OR_LIF_DELTA && model != OR_LIF_EXP) return -1;
     if (model == OR_LIF_EXP && !isyn) return -1;
     const int64_t N = m->n_neurons, D = d_slots;

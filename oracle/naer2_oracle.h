@@ -26,19 +26,26 @@
  *     (only for cells that received at least one link).
  *   - background: count = #{j : u >= thr[j]}, u = word 0 of Philox4x32-10(ctr = {neuron, step lo,
  *     step hi, instance}, key = {seed lo, seed hi}); input = x + j_ext * (float)count.
+ *
+ * Multi-rate (naer2/population.py:191-204, 382-410; naer2/network.py:80-152): a population whose timestep is
+ * stride * dt (its parameters computed for that interval) updates at the steps s with (s + 1) % stride == 0
+ * only; x is then its window's ring slots summed oldest first, ((r[s-k+1] + r[s-k+2]) + ... + r[s-1]) + r[s],
+ * each slot cleared; its other steps leave the ring and its state alone and emit no spike.  d_slots must be
+ * at least the largest delay + the largest stride - 1.  (or_mt_run takes stride-1 networks only.)
  */
 #ifndef NAER2_ORACLE_H
 #define NAER2_ORACLE_H
 #include <stdint.h>
 
 #define OR_POISSON_K 32
+#define OR_MAX_STRIDE 64
 #define OR_LIF_DELTA 1
 #define OR_LIF_EXP 2
 
 /* Same byte layout as the product's per-population parameter block (fp32 constants + inversion table). */
 typedef struct or_pop {
     int32_t ref_steps;
-    int32_t reserved;
+    int32_t stride;   /* steps per update (0 or 1: every step) */
     float decay, p_i, p_vi, v_rest, v_th, v_reset, j_ext, reserved_f;
     uint32_t poisson_thr[OR_POISSON_K];
 } or_pop;
