@@ -99,6 +99,14 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
     r1, ev1 = oracle.run(model, rt.pop_first, pb, rt.d_slots, rt.net.seed, rt.net.instance, *arrays, st, c0, n1, True)
     w1 = time.perf_counter() - t0
     same1 = bool(np.array_equal(gpu_raster[:n1], r1))
+    name = "PD14" if kind == "pd14" else "Brunel"
+    if any(p.stride > 1 for p in rt.params):  # the threaded oracle and the torch loop are single-rate only
+        print(f"[bench] cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1})", file=sys.stderr, flush=True)
+        return {"value": round(n1 * bio_step / w1, 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
+                "sample": f"{n1} steps ({n1 * bio_step * 1000:.0f} ms bio) of the same multi-rate {name} network from "
+                          f"the GPU state after step {c0}: the C oracle (-O2) on one host thread; raster bit-exact "
+                          f"vs GPU: {same1}",
+                "events_per_s": round(ev1 / w1, 1), "bit_exact_vs_gpu": same1}
     par = oracle.Parallel(rt.n, *arrays, threads=threads)
     st = state()
     t0 = time.perf_counter()
@@ -114,7 +122,6 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
     print(f"[bench] cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1}); oracle {threads} threads "
           f"{n_mt} steps {wm:.1f} s (bit-exact {samem}); reference algorithm (torch) {threads} threads {n_ref} steps "
           f"{ref['seconds']:.1f} s", file=sys.stderr, flush=True)
-    name = "PD14" if kind == "pd14" else "Brunel"
     return {
         "value": round(n_mt * bio_step / wm, 6), "unit": "bio_s/wall_s", "cores": threads, "kind": "port",
         "sample": f"{n_mt} steps ({n_mt * bio_step * 1000:.0f} ms bio) of the same {name} network from the GPU state "
@@ -302,6 +309,12 @@ def bench_plain(args) -> int:
         net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
     else:
         net, pops = brunel(seed=args.seed, device="cuda")
+    timesteps = {}
+    for spec in args.timestep or []:
+        name, _, ms = spec.partition("=")
+        if name not in pops:
+            raise SystemExit(f"--timestep: no population {name!r} (have {sorted(pops)})")
+        pops[name].timestep = timesteps[name] = float(ms)
     rt = net.build()
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
@@ -334,15 +347,19 @@ def bench_plain(args) -> int:
     rtf = bio_s / wall
     ms_per_step = wall * 1000.0 / args.steps
 
-    # roofline of the step kernel: algorithmic bytes per launch (N (8 n_state + 8) + 8 E + 8 S; n_state 3 for
-    # the exponential synapse: V, I, refractory counter; 2 for the delta synapse)
+    # roofline of the step kernel: algorithmic bytes per launch (sum over populations of n (8 n_state / k + 8)
+    # + 8 E + 8 S; n_state 3 for the exponential synapse: V, I, refractory counter; 2 for the delta synapse;
+    # k the population's steps per update, whose state is read and written once per k steps while its ring
+    # slots are read and cleared once each)
     n_state = 3 if kind == "pd14" else 2
-    bytes_per_step = rt.n * (8 * n_state + 8) + 8 * events / args.steps + 8 * spikes_timed / args.steps
+    strides = [p.stride if p.stride > 1 else 1 for p in rt.params]
+    state_bytes = sum(pop.n * (8 * n_state / k + 8) for pop, k in zip(rt.pops, strides))
+    bytes_per_step = state_bytes + 8 * events / args.steps + 8 * spikes_timed / args.steps
     k_ms = gpu_ms / args.steps  # HIP events on the launch stream over the timed region, per launch
     achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
     traffic, occupancy = None, None
     tfile = ROOT / "profiles" / "pd14_step_traffic.json"
-    if kind == "pd14" and tfile.exists():
+    if kind == "pd14" and tfile.exists() and not timesteps:
         prof = json.loads(tfile.read_text())
         traffic, occupancy = prof.get("hbm_bytes_per_launch"), prof.get("occupancy")
 
@@ -372,6 +389,7 @@ def bench_plain(args) -> int:
                                 f"g 5, D 1.5 ms, {rt.canon.n_links} synapses, Poisson drive (eta 2)") +
                                f", dt 0.1 ms, {args.steps} steps timed, seed {args.seed}",
                    "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
+                   **({"timesteps_ms": timesteps} if timesteps else {}),
                    "graph_window": args.window, "parallelism": "single GPU"},
         "events_per_s": round(events / wall, 1),
         "spikes_per_step": round(spikes_timed / args.steps, 3),
@@ -596,6 +614,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2000,
                     help="scales the CPU samples (PD14: 2000 steps on one thread, 10x that on every core)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timestep", action="append", metavar="POP=MS",
+                    help="pd14/brunel: a population's own timestep, a whole multiple of dt (repeatable)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad", "brunel_exact"],
                     default="pd14",

@@ -341,6 +341,10 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
     }
 }
 
+template <class St>
+__device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
+                                              unsigned long long& t0, unsigned long long& t1);
+
 // Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
 // contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
 template <class St>
@@ -375,7 +379,14 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
     if (lane == 0) st.seg_start[n_sp] = carry;
     __syncthreads();
     N2H_STAMP(3);
-    const int L = carry;
+    deliver_links(a, st, acc, lane, n_sp, carry, t0, t1);
+}
+
+// The L links of the n_sp sources whose segment bounds are in st (seg_lo, seg_start): staged in chunks of
+// kCapL in contract order and walked into acc.
+template <class St>
+__device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
+                                              unsigned long long& t0, unsigned long long& t1) {
     for (int e0 = 0; e0 < L; e0 += kCapL) {
         const int e1 = min(L, e0 + kCapL);
         // owner[j]: the source whose segment holds link e0 + j.  Short chunks: a binary search per link;
@@ -490,7 +501,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     // state; the step (a load in graph launches) only addresses the ring and the bit history
     float V = 0.0f, I = 0.0f, x = 0.0f;
     int32_t R = 0;
-    if (live) {
+    if (live && !a.multirate) {  // (multi-rate plans: at update steps only, below)
         V = a.v[nid];
         R = a.ref[nid];
         if (MODEL == N2H_MODEL_LIF_EXP) I = a.isyn[nid];
@@ -511,6 +522,11 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (a.multirate) stride = __builtin_amdgcn_readfirstlane(a.pops[pop].stride);
     const bool upd = stride <= 1 || static_cast<uint64_t>(s + 1) % static_cast<uint64_t>(stride) == 0ull;
     if (live && upd) x = a.ring[cur];
+    if (live && upd && a.multirate) {
+        V = a.v[nid];
+        R = a.ref[nid];
+        if (MODEL == N2H_MODEL_LIF_EXP) I = a.isyn[nid];
+    }
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
     if (KW > 0 && a.rows && n_prev >= a.dense_min) {
@@ -543,14 +559,35 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         }
         __syncthreads();
     } else if (n_prev <= kWave) {
-        // sparse path: sort the (atomically appended) source slots, then deliver them
+        // sparse path: sort the (atomically appended) source slots, then deliver them.  Each lane's
+        // segment bounds are loaded before the ranking ballots, which hide their latency, and land in LDS
+        // at the lane's rank.
         N2H_STAMP(1);
         const bool has = lane < n_prev;
+        int64_t lo = 0, hi = 0;
+        if (has) {
+            const int64_t idx = static_cast<int64_t>(my_src) * NG + g;
+            lo = a.seg_off[idx];
+            hi = a.seg_off[idx + 1];
+        }
         const int rank = wave_rank(my_src, has, a.slot_bits);
-        if (has) st.sp[rank] = my_src;
+        if (has) {
+            st.sp[rank] = my_src;
+            st.seg_lo[rank] = lo;
+            st.seg_start[rank] = static_cast<int>(hi - lo);  // the length, until the scan below
+        }
         __syncthreads();
         N2H_STAMP(2);
-        if (n_prev > 0) deliver_pass(a, st, acc, g, lane, n_prev, t0, t1);
+        if (n_prev > 0) {
+            const int n = has ? st.seg_start[lane] : 0;
+            const int inc = wave_incl_scan(n, lane);
+            const int total = __shfl(inc, kWave - 1, kWave);
+            if (has) st.seg_start[lane] = inc - n;
+            if (lane == 0) st.seg_start[n_prev] = total;
+            __syncthreads();
+            N2H_STAMP(3);
+            deliver_links(a, st, acc, lane, n_prev, total, t0, t1);
+        }
     } else {
         // burst path: the spike bitmap of step s-1, lane-contiguous words, ascending source slots
         const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;

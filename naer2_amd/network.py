@@ -126,6 +126,56 @@ class Network:
         rt.run(n_steps, self.graph_window if graph_window is None else graph_window)
         self.t = rt.window_index * rt.window if self.event_driven else rt.step * self.resolution
 
+    def plan(self, max_launches: int = 1000, until: float | None = None) -> list[list[tuple]]:
+        """A dry run of the next launches (naer2/network.py:154-204): each launch the list of
+        (population, t_start, t_end, gated) whose window it integrates, at most max_launches of them or up to
+        network time `until`; nothing runs and nothing changes.
+
+        Time-driven: one launch per step of `resolution`, every population's intake in every launch; a
+        population with a timestep of its own (k steps) integrates its window [t_end - k dt, t_end) in the
+        launch of the window's last step, at the steps s with (s + 1) % k == 0.  Event-driven: one launch per
+        window of the smallest delay; receiving populations are gated (a group with no input arriving in the
+        window does no work), sources emit every window."""
+        pops = sorted(self.populations, key=lambda p: p._order)
+        if self.event_driven:
+            from .event import describe
+            W = describe(self).window
+            k0 = self._rt.window_index if self._rt is not None else 0
+            out = []
+            while len(out) < max_launches:
+                t0, t1 = k0 * W, (k0 + 1) * W
+                if until is not None and t1 > until + 1e-9 * max(1.0, abs(until)):
+                    break
+                out.append([(p, t0, t1, p.family == "event_lif") for p in pops])
+                k0 += 1
+            return out
+        dt = float(self.resolution)
+        strides = [pop_stride(p, dt) for p in pops]
+        s = self.step_index
+        out = []
+        while len(out) < max_launches:
+            if until is not None and (s + 1) * dt > until + 1e-9 * max(1.0, abs(until)):
+                break
+            out.append([(p, (s + 1 - k) * dt, (s + 1) * dt, False) for p, k in zip(pops, strides)
+                        if (s + 1) % k == 0])
+            s += 1
+        return out
+
+    def run_planned(self, max_launches: int = 1000, until: float | None = None) -> int:
+        """The planned launches (plan()) run eagerly, one kernel sequence each; returns the launches made."""
+        n = len(self.plan(max_launches, until))
+        if n:
+            self.run(n, graph_window=0)
+        return n
+
+    def run_graph(self, max_launches: int = 1000, until: float | None = None) -> int:
+        """As run_planned, replayed from captured hipGraph windows (naer2/network.py:262-321): one capture
+        per window length, every later window a replay."""
+        n = len(self.plan(max_launches, until))
+        if n:
+            self.run(n)
+        return n
+
     def spikes(self, pop=None) -> tuple[np.ndarray, np.ndarray]:
         """(steps, neuron ids) of every recorded spike, ascending by step then id; neuron ids are the
         population's own (local) indices when pop is given, the network's otherwise.  Event-driven networks:

@@ -148,3 +148,17 @@ def test_potential_traces_refuse_multirate():
     rt = net.build()
     with pytest.raises(RuntimeError, match="stride"):
         rt.trace_potentials(torch.zeros(10, rt.n, device="cuda"))
+
+
+def test_run_planned_and_run_graph_equal_run():
+    _say("case: multi-rate brunel, run_planned + run_graph vs run")
+    a, b = _brunel_mixed(4), _brunel_mixed(4)
+    a.build(), b.build()
+    assert a.run_planned(37) == 37
+    assert a.run_graph(until=20.0) == 200 - 37
+    b.run(200, graph_window=0)
+    torch.cuda.synchronize()
+    ra, rb = a.runtime, b.runtime
+    assert ra.step == rb.step == 200 and abs(a.t - 20.0) < 1e-9
+    assert np.array_equal(ra.raster(), rb.raster())
+    assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())

@@ -197,3 +197,32 @@ def test_exp_lif_stride_parameters():
     assert p.stride == 4 and p.ref_steps == max(0, round(2.0 / 0.4) - 1)
     assert p.p_i == np.float32(math.exp(-0.4 / 0.5))
     assert a.d_slots == 1 + 4 - 1
+
+
+def test_plan_lists_each_populations_windows():
+    """Network.plan (naer2/network.py:154-204 -- test/test_scheduling.py:141-170 checks step() runs as
+    planned): E every step, I (0.3 ms) in every third launch with its window; `until` bounds the plan."""
+    net, E, I = _two_pops(ts_i=0.3)
+    launches = net.plan(max_launches=7)
+    assert len(launches) == 7
+    for s, launch in enumerate(launches):
+        got = {p: (round(t0, 9), round(t1, 9), g) for p, t0, t1, g in launch}
+        assert got[E] == (round(s * 0.1, 9), round((s + 1) * 0.1, 9), False)
+        if (s + 1) % 3 == 0:
+            assert got[I] == (round((s - 2) * 0.1, 9), round((s + 1) * 0.1, 9), False)
+        else:
+            assert I not in got
+    assert len(net.plan(until=0.5)) == 5
+    assert net.plan(max_launches=0) == []
+
+
+def test_plan_of_an_event_driven_network_gates_receivers():
+    from naer2_amd import PoissonExact
+    src, tgt = PoissonExact(rate=10.0) * 20, LIFDeltaExact() * 30
+    net = src.network + tgt.network
+    c = src >> FixedInDegree(5) >> tgt
+    c.weight, c.delay = 0.1, 1.5
+    launches = net.plan(max_launches=3)
+    assert [[(p, round(a, 9), round(b, 9), g) for p, a, b, g in L] for L in launches] == [
+        [(src, round(k * 1.5, 9), round((k + 1) * 1.5, 9), False), (tgt, round(k * 1.5, 9), round((k + 1) * 1.5, 9), True)]
+        for k in range(3)]
