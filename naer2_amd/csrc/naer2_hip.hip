@@ -109,15 +109,22 @@ struct StepArgs {
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
 };
 
-// Inclusive prefix sum over the 64 lanes of the wavefront.
-__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int y = __shfl_up(x, off, kWave);
-        if (lane >= off) x += y;
-    }
+// Inclusive prefix sum over the 64 lanes of the wavefront, in DPP lane moves (no LDS round trip): a
+// Hillis-Steele scan inside each row of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row
+// add 0), then row 0's total into rows 1 and 3 (row_bcast:15) and row 1's running total into rows 2 and 3
+// (row_bcast:31).
+__device__ __forceinline__ int wave_incl_scan(int x, int /*lane*/) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
     return x;
 }
+
+// The value of lane 63 (the total of an inclusive scan), as a scalar.
+__device__ __forceinline__ int last_lane(int x) { return __builtin_amdgcn_readlane(x, kWave - 1); }
 
 // Rank of each valid lane's key among the valid lanes' keys (keys distinct, < 2^bits), from one ballot
 // per key bit, most significant first: no cross-lane data movement.
@@ -374,7 +381,7 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
             st.seg_start[i] = carry + inc - n;
             st.seg_lo[i] = lo_r[u];
         }
-        carry += __shfl(inc, kWave - 1, kWave);
+        carry += last_lane(inc);
     }
     if (lane == 0) st.seg_start[n_sp] = carry;
     __syncthreads();
@@ -522,6 +529,11 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (a.multirate) stride = __builtin_amdgcn_readfirstlane(a.pops[pop].stride);
     const bool upd = stride <= 1 || static_cast<uint64_t>(s + 1) % static_cast<uint64_t>(stride) == 0ull;
     if (live && upd) x = a.ring[cur];
+    // the background draw needs only the neuron and the step: computed here, under the latency of the
+    // spike-list load, and kept in a register (the empty asm pins it here)
+    uint32_t u_bg = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
+                                 static_cast<uint32_t>(static_cast<uint64_t>(s) >> 32), a.instance, a.key0, a.key1);
+    asm volatile("" : "+v"(u_bg));
     if (live && upd && a.multirate) {
         V = a.v[nid];
         R = a.ref[nid];
@@ -581,7 +593,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         if (n_prev > 0) {
             const int n = has ? st.seg_start[lane] : 0;
             const int inc = wave_incl_scan(n, lane);
-            const int total = __shfl(inc, kWave - 1, kWave);
+            const int total = last_lane(inc);
             if (has) st.seg_start[lane] = inc - n;
             if (lane == 0) st.seg_start[n_prev] = total;
             __syncthreads();
@@ -607,7 +619,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         }
         const int incl = wave_incl_scan(cnt, lane);
         const int excl = incl - cnt;
-        const int total = __shfl(incl, kWave - 1, kWave);
+        const int total = last_lane(incl);
         for (int r0 = 0; r0 < total; r0 += kCapS) {
             const int r1 = min(total, r0 + kCapS);
             if (excl < r1 && excl + cnt > r0) {
@@ -664,9 +676,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             }
             x = xw + x;
         }
-        const uint32_t u = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
-                                        static_cast<uint32_t>(static_cast<uint64_t>(s) >> 32), a.instance,
-                                        a.key0, a.key1);
+        const uint32_t u = u_bg;
         int c = 0;
 #pragma unroll
         for (int j = 0; j < N2H_POISSON_K; ++j) c += (u >= P.poisson_thr[j]) ? 1 : 0;
@@ -714,7 +724,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (mask) {
         int base = 0;
         if (lane == 0) base = atomicAdd(clist, __popcll(mask));
-        base = __shfl(base, 0, kWave);
+        base = __builtin_amdgcn_readlane(base, 0);
         if (fired) {
             const int rank = __popcll(mask & ((1ull << lane) - 1ull));
             clist[1 + base + rank] = g * kWave + lane;
