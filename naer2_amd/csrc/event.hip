@@ -219,12 +219,16 @@ __device__ __forceinline__ float window_time(int64_t k, float W) {
     return static_cast<float>(static_cast<double>(k) * static_cast<double>(W));
 }
 
-__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int y = __shfl_up(x, off, kWave);
-        if (lane >= off) x += y;
-    }
+// Inclusive prefix sum over the 64 lanes of a wavefront in DPP lane moves: a scan inside each row of 16
+// lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row add 0), then row 0's total into rows 1
+// and 3 (row_bcast:15) and row 1's running total into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ int wave_incl_scan(int x, int /*lane*/) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
     return x;
 }
 
@@ -289,7 +293,7 @@ __device__ __forceinline__ void log_input(const EvArgs& a, const int32_t* ent_sl
     if (!m) return;
     int base = 0;
     if (lane == 0) base = atomicAdd(a.log_count, __popcll(m));
-    base = __shfl(base, 0, kWave);
+    base = __builtin_amdgcn_readlane(base, 0);
     if (!has) return;
     const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
     if (pos >= a.log_cap) return;
