@@ -123,6 +123,23 @@ __device__ __forceinline__ int wave_incl_scan(int x, int /*lane*/) {
     return x;
 }
 
+// Inclusive max-scan over the 64 lanes in DPP lane moves, for values whose running maximum from lane 0 is
+// never below 0: lanes shifted in from outside a row contribute 0, which then changes no result.
+__device__ __forceinline__ int wave_incl_max_nonneg(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15, rows 1 and 3
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31, rows 2 and 3
+    return x;
+}
+
+// The value of lane - 1 (wave_shr:1), `first` in lane 0.
+__device__ __forceinline__ int wave_shift_up1(int x, int first) {
+    return __builtin_amdgcn_update_dpp(first, x, 0x138, 0xf, 0xf, false);
+}
+
 // The value of lane 63 (the total of an inclusive scan), as a scalar.
 __device__ __forceinline__ int last_lane(int x) { return __builtin_amdgcn_readlane(x, kWave - 1); }
 
@@ -441,14 +458,8 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
                 run_max = max(run_max, static_cast<int>(st.owner[lane * kPer + q]));
                 own[q] = run_max;
             }
-            int pre = run_max;
-    #pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const int y = __shfl_up(pre, off, kWave);
-                if (lane >= off) pre = max(pre, y);
-            }
-            pre = __shfl_up(pre, 1, kWave);
-            if (lane == 0) pre = -1;
+            // the running owner over the lanes before this one (lane 0's first entry is c0 >= 0)
+            const int pre = wave_shift_up1(wave_incl_max_nonneg(run_max), -1);
     #pragma unroll
             for (int q = 0; q < kPer; ++q) st.owner[lane * kPer + q] = static_cast<int16_t>(max(pre, own[q]));
             __syncthreads();
