@@ -40,8 +40,14 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kCapS = 256;   // spiking sources staged in LDS per pass
+constexpr int kCapS = 256;   // spiking sources staged in LDS per pass (single-plan kernel)
 constexpr int kCapL = 512;   // links staged in LDS per pass (LDS per group: 9.2 KB + 256 B per ring slot)
+constexpr int kRowU = 8;     // row-intake 16-byte cells in flight per lane and batch
+// The batched kernel stages less per pass and keeps fewer row cells in flight: 96 VGPRs and < 8 KB of LDS
+// per group, so that 20 groups are resident per CU instead of 16 (the batch is throughput-bound).
+constexpr int kBatchCapS = 128;
+constexpr int kBatchCapL = 384;
+constexpr int kBatchRowU = 4;
 constexpr int kListSlots = 4;
 constexpr int kMaxBatch = 256;  // plans in one batched launch
 constexpr int kDenseWords = 1536;  // row intake: spike bitmaps of up to 1535 groups (98,240 neurons) in LDS, plus
@@ -159,23 +165,27 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
 
 // Shared-memory staging of one workgroup.  BM: 64-bit words of the row intake's spike bitmap (the
 // groups' words, then the sentinel word); kernel variants size it for the groups they serve.
-template <int BM>
+// CAPS / CAPL: spiking sources and links staged per pass; RU: row cells in flight per lane and batch.
+template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU>
 struct Stage {
+    static constexpr int kS = CAPS;
+    static constexpr int kL = CAPL;
+    static constexpr int kRU = RU;
     union {
         struct {                     // segment intake
-            int32_t sp[kCapS];           // spiking source slots of this pass, ascending
-            int64_t seg_lo[kCapS];       // first link of each source's segment into this group
-            int32_t seg_start[kCapS + 1];  // exclusive scan of the segment lengths
-            int16_t owner[kCapL];        // source (index into sp) of each staged link
-            uint2 sorted[kCapL];         // the staged links regrouped by target, each target's in contract order
+            int32_t sp[CAPS];            // spiking source slots of this pass, ascending
+            int64_t seg_lo[CAPS];        // first link of each source's segment into this group
+            int32_t seg_start[CAPS + 1];  // exclusive scan of the segment lengths
+            int16_t owner[CAPL];         // source (index into sp) of each staged link
+            uint2 sorted[CAPL];          // the staged links regrouped by target, each target's in contract order
+            int32_t tcount[kWave];       // staged links per target
+            int32_t tstart[kWave];       // first of each target's links in sorted
         };
         struct {                     // row intake
             unsigned long long bm[BM];   // the previous step's spike bitmap, then a zero word
             float wtab[256];             // compact rows: the weight table
         };
     };
-    int32_t tcount[kWave];       // staged links per target
-    int32_t tstart[kWave];       // first of each target's links in sorted
     uint32_t param_words[sizeof(n2h_pop_params) / 4];  // the group's population parameters
     unsigned long long stamps[kTraceEvents];             // phase stamps (tracing)
 };
@@ -211,9 +221,9 @@ __device__ __forceinline__ void cell_add(float* acc, int lane, int b, float w, i
 // reference's _walk (fused.py:939-991) without every lane reading every link.  Touched delays are
 // recorded in the lane's own mask (t0: delays 1..64, t1: 65..128).
 template <class St>
-__device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl, const uint2 (&r)[kCapL / kWave],
+__device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl, const uint2 (&r)[St::kL / kWave],
                                            unsigned long long& t0, unsigned long long& t1) {
-    constexpr int kPer = kCapL / kWave;
+    constexpr int kPer = St::kL / kWave;
     const unsigned long long below = (1ull << lane) - 1ull;
     st.tcount[lane] = 0;
     __syncthreads();
@@ -265,18 +275,17 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
     if (cb >= 0) acc[cb * kWave + lane] = cr;
 }
 
-// Row intake of one batch of kRowU 16-byte cells of this lane's row (kRowU * PER entries).  Padding
+// Row intake of one batch of St::kRU 16-byte cells of this lane's row (kRU * PER entries).  Padding
 // entries name the sentinel source slot, whose bitmap bit is always clear, so no entry needs a bounds
 // test.  The bitmap words and weights of all entries are read from LDS first (one wait).
 //   UNIFORM (every link has delay cell b0): branch-free -- the hit bit, sign-extended to a mask, selects
 //   either the weight or -0.0f, the exact identity of fp32 addition, into the register sum cr, so the sum
 //   sees exactly the spiking sources' weights in row order; `any` records whether there was one.
 //   Otherwise: each hit goes to its own delay cell through cell_add.
-constexpr int kRowU = 8;
 template <int PER, bool UNIFORM, class St>
-__device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, const uint4 (&c)[kRowU], int& cb,
+__device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, const uint4 (&c)[St::kRU], int& cb,
                                           float& cr, uint32_t& any, unsigned long long& t0, unsigned long long& t1) {
-    constexpr int kHalf = kRowU / 2;  // cells per LDS round (bounds the registers the decoded entries hold)
+    constexpr int kHalf = St::kRU / 2;  // cells per LDS round (bounds the registers the decoded entries hold)
     constexpr int E = kHalf * PER;
     const uint32_t* bm32 = reinterpret_cast<const uint32_t*>(st.bm);
 #pragma unroll
@@ -321,6 +330,7 @@ template <int PER, bool UNIFORM, class St, typename Ready>
 __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, const uint4* row, int depth, int b0,
                                          uint32_t sentinel, unsigned long long& t0, unsigned long long& t1,
                                          Ready ready) {
+    constexpr int RU = St::kRU;
     if (depth <= 0) {
         ready();
         return;
@@ -329,29 +339,29 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
     // a batch past the end of the block: its clamped loads repeat the last cell, so those cells are
     // replaced by sentinel entries (uniform branch, once per walk)
     const uint32_t sx = (PER == 4) ? (sentinel << 15) : (sentinel << 7);
-    auto trim = [&](uint4 (&c)[kRowU], int q_first) {
-        if (q_first + kRowU > depth) {
+    auto trim = [&](uint4 (&c)[RU], int q_first) {
+        if (q_first + RU > depth) {
 #pragma unroll
-            for (int u = 0; u < kRowU; ++u)
+            for (int u = 0; u < RU; ++u)
                 if (q_first + u > last) c[u] = make_uint4(sx, sx, sx, sx);
         }
     };
-    uint4 A[kRowU], B[kRowU];
+    uint4 A[RU], B[RU];
 #pragma unroll
-    for (int u = 0; u < kRowU; ++u) A[u] = row[static_cast<int64_t>(min(u, last)) * kWave];
+    for (int u = 0; u < RU; ++u) A[u] = row[static_cast<int64_t>(min(u, last)) * kWave];
     ready();
     int cb = -1;
     float cr = 0.0f;
     uint32_t any = 0u;
-    for (int q0 = 0; q0 < depth; q0 += 2 * kRowU) {
+    for (int q0 = 0; q0 < depth; q0 += 2 * RU) {
 #pragma unroll
-        for (int u = 0; u < kRowU; ++u) B[u] = row[static_cast<int64_t>(min(q0 + kRowU + u, last)) * kWave];
+        for (int u = 0; u < RU; ++u) B[u] = row[static_cast<int64_t>(min(q0 + RU + u, last)) * kWave];
         trim(A, q0);
         row_batch<PER, UNIFORM>(st, acc, lane, A, cb, cr, any, t0, t1);
-        if (q0 + kRowU >= depth) break;
+        if (q0 + RU >= depth) break;
 #pragma unroll
-        for (int u = 0; u < kRowU; ++u) A[u] = row[static_cast<int64_t>(min(q0 + 2 * kRowU + u, last)) * kWave];
-        trim(B, q0 + kRowU);
+        for (int u = 0; u < RU; ++u) A[u] = row[static_cast<int64_t>(min(q0 + 2 * RU + u, last)) * kWave];
+        trim(B, q0 + RU);
         row_batch<PER, UNIFORM>(st, acc, lane, B, cb, cr, any, t0, t1);
     }
     if (UNIFORM) {
@@ -376,9 +386,9 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
                                              unsigned long long& t0, unsigned long long& t1) {
     const int NG = a.n_groups;
     // segment bounds: every load of the pass issued before the first is used
-    int64_t lo_r[kCapS / kWave], hi_r[kCapS / kWave];
+    int64_t lo_r[St::kS / kWave], hi_r[St::kS / kWave];
 #pragma unroll
-    for (int u = 0; u < kCapS / kWave; ++u) {
+    for (int u = 0; u < St::kS / kWave; ++u) {
         const int i = u * kWave + lane;
         lo_r[u] = hi_r[u] = 0;
         if (i < n_sp) {
@@ -389,7 +399,7 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
     }
     int carry = 0;
 #pragma unroll
-    for (int u = 0; u < kCapS / kWave; ++u) {
+    for (int u = 0; u < St::kS / kWave; ++u) {
         if (u * kWave >= n_sp) break;
         const int i = u * kWave + lane;
         const int n = static_cast<int>(hi_r[u] - lo_r[u]);
@@ -407,16 +417,16 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
 }
 
 // The L links of the n_sp sources whose segment bounds are in st (seg_lo, seg_start): staged in chunks of
-// kCapL in contract order and walked into acc.
+// the stage's kL in contract order and walked into acc.
 template <class St>
 __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
                                               unsigned long long& t0, unsigned long long& t1) {
-    for (int e0 = 0; e0 < L; e0 += kCapL) {
-        const int e1 = min(L, e0 + kCapL);
+    for (int e0 = 0; e0 < L; e0 += St::kL) {
+        const int e1 = min(L, e0 + St::kL);
         // owner[j]: the source whose segment holds link e0 + j.  Short chunks: a binary search per link;
         // long ones: the segment starts scattered, then an inclusive max-scan (sources in ascending start
         // order).
-        constexpr int kPer = kCapL / kWave;
+        constexpr int kPer = St::kL / kWave;
         if (e1 - e0 <= 2 * kWave) {
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -442,7 +452,7 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
             }
             __syncthreads();
     #pragma unroll
-            for (int u = 0; u < kCapS / kWave; ++u) {
+            for (int u = 0; u < St::kS / kWave; ++u) {
                 const int i = u * kWave + lane;
                 if (i < n_sp) {
                     const int st0 = st.seg_start[i];
@@ -483,9 +493,8 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
 
 // One step s of the whole network for target group g (one wavefront).
 // KW: bitmap words per lane held in registers on the burst path (0: read them in a loop).
-template <int MODEL, int KW>
-__device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, Stage<bm_words(KW)>& st,
-                                           float* acc_lds) {
+template <int MODEL, int KW, class St>
+__device__ __forceinline__ void step_group(const StepArgs& a, const int g, const int32_t k, St& st, float* acc_lds) {
     const int lane = threadIdx.x;
     // acc[b * 64 + lane]: the partial sum of delay cell b; the LDS holds cells acc_base onwards (one cell for
     // a uniform delay)
@@ -631,8 +640,8 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         const int incl = wave_incl_scan(cnt, lane);
         const int excl = incl - cnt;
         const int total = last_lane(incl);
-        for (int r0 = 0; r0 < total; r0 += kCapS) {
-            const int r1 = min(total, r0 + kCapS);
+        for (int r0 = 0; r0 < total; r0 += St::kS) {
+            const int r1 = min(total, r0 + St::kS);
             if (excl < r1 && excl + cnt > r0) {
                 int rank = excl;
                 for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) {
@@ -809,7 +818,7 @@ struct BatchArgs {
 
 template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
-    __shared__ Stage<bm_words(KW)> st;
+    __shared__ Stage<bm_words(KW), kBatchCapS, kBatchCapL, kBatchRowU> st;
     extern __shared__ float acc[];
     const int b = blockIdx.x;
     int tile = b;
@@ -1010,6 +1019,8 @@ void launch_batch_model(const n2h_batch* bt, const BatchArgs& ba, dim3 grid, hip
     const dim3 block(kWave);
     if (bt->words <= 8)
         hipLaunchKernelGGL((batch_kernel<MODEL, 8>), grid, block, bt->lds_bytes, stream, ba, k);
+    else if (bt->words <= 13)  // up to 832 groups a plan (Brunel networks up to 53k neurons) at 20 groups per CU
+        hipLaunchKernelGGL((batch_kernel<MODEL, 13>), grid, block, bt->lds_bytes, stream, ba, k);
     else if (bt->words <= 16)
         hipLaunchKernelGGL((batch_kernel<MODEL, 16>), grid, block, bt->lds_bytes, stream, ba, k);
     else if (bt->words <= 32)

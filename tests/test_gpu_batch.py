@@ -115,3 +115,25 @@ def test_intake_tuning_restores_the_state_and_changes_no_result():
     assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
     assert np.array_equal(ra.ref.cpu().numpy(), rb.ref.cpu().numpy())
     assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
+
+
+def test_batch_of_large_networks_equals_plain_runs():
+    """Networks of 513-832 groups take the batched kernel variant with a 13-word bitmap per lane (and the
+    batch's smaller staging passes): the same results as each network alone, through sparse, burst and
+    row-intake steps."""
+    params = sweep_params(n_instances=3, seed=11, n_min=34000, n_max=52000)
+    params[1]["g"], params[1]["eta"] = 3.0, 3.5  # a bursting one: bursts take the row intake
+    _say(f"case: batch of large brunel instances, sizes {[p['N_E'] + p['N_I'] for p in params]}")
+    batched = brunel_sweep(params, device="cuda", record=True)
+    plain = brunel_sweep(params, device="cuda", record=True)
+    assert all(513 <= n.runtime.n_groups <= 832 for n in batched)
+    fr = FusedRound([n.runtime for n in batched], {"order": 1, "xcd": 0}, reserve_steps=144)
+    fr.launch(144, graph_window=48)
+    fr.close()
+    for n in plain:
+        n.run(144, graph_window=0)
+    torch.cuda.synchronize()
+    for a, b in zip(batched, plain):
+        _same(a, b)
+    counts = np.concatenate([n.runtime.raster().sum(axis=1) for n in plain])
+    assert (counts <= 64).any() and (counts > 64).any()  # the sparse path and a bitmap or row intake
