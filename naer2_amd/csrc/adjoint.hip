@@ -90,11 +90,14 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
 #pragma unroll
         for (int p = 0; p < kAdjPops; ++p) bins[p] += (tp == static_cast<uint32_t>(p)) ? c : 0.0f;
     }
-    // fixed butterfly: every lane ends with the same sums, in the same order on every run
+    // fixed butterfly: every lane ends with the same sums, in the same order on every run (a neuron that
+    // gathered nothing has all-zero sums: no exchange)
+    if (e1 > e0) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
+        for (int off = kWave / 2; off > 0; off >>= 1) {
 #pragma unroll
-        for (int p = 0; p < kAdjPops; ++p) bins[p] += __shfl_xor(bins[p], off, kWave);
+            for (int p = 0; p < kAdjPops; ++p) bins[p] += __shfl_xor(bins[p], off, kWave);
+        }
     }
     if (lane != 0) return;
 
