@@ -29,6 +29,7 @@
 // with exactly the operations of the oracle, so that spike times match the oracle's bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -54,7 +55,8 @@ constexpr int kPerThread = 2;                          // keys per thread in the
 constexpr int kSmallSort = kPerThread * kSortThreads;  // key counts up to this sort in registers (bitonic_small)
 constexpr int kBins = 2048;                            // arrival-time bins of a window's stream sort
 constexpr int kBinCap = 2048;                          // spikes a stream bins in LDS (more: the bitonic network)
-constexpr int kBinSortMax = 16;                        // spikes one bin's thread sorts (more: the bitonic network)
+constexpr int kBinRankMax = 256;                       // spikes of one bin placed by rank (more: the bitonic network)
+constexpr int kPrefixLds = 4096;                       // group prefix held in LDS (networks of < 4096 groups)
 
 thread_local char g_err[512] = "";
 
@@ -800,6 +802,7 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     __shared__ int s_fill[kBins];
     __shared__ int s_c[N2H_MAX_POPS];    // a stream's connection index of each population (-1: not a source)
     __shared__ int s_n;
+    __shared__ int s_gp[kPrefixLds];     // the group prefix (G < kPrefixLds): a spike's group by binary search
     const int tid = threadIdx.x;
     const int64_t k = *a.window;
     if (tid < a.n_pops) s_pops[tid] = a.pops[tid];
@@ -813,12 +816,26 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
     for (int g = g0; g < g1; ++g) part += a.reg_count[g];
     int S;
     int pre = block_excl_scan(part, s_w, &S);
+    const bool lds_prefix = G < kPrefixLds;
     for (int g = g0; g < g1; ++g) {
         a.group_prefix[g] = pre;
+        if (lds_prefix) s_gp[g] = pre;
         pre += a.reg_count[g];
     }
-    if (tid == 0) a.group_prefix[G] = S;
+    if (tid == 0) {
+        a.group_prefix[G] = S;
+        if (lds_prefix) s_gp[G] = S;
+    }
     __syncthreads();
+    // the group holding entry i of the window's list, among groups [lo, hi) with s_gp[lo] <= i < s_gp[hi]
+    auto group_of_entry = [&](int i, int lo, int hi) {
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_gp[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
     if (trow) trow[1] = wall_clock64();
     if (S > a.keys_cap) {  // cannot happen: the keys hold every region entry
         if (tid == 0) atomicOr(&a.flags[N2H_EV_FLAG_SORT], 1);
@@ -833,18 +850,31 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         if (n2 < kSmallSort) n2 = kSmallSort;  // (padding sorts in registers)
         const bool in_lds = n2 <= L;
         unsigned long long* keys = in_lds ? s_k : a.keys;
-        for (int g = tid; g < G; g += kSortThreads) {  // one thread per group: a group holds few spikes
-            const int c = a.reg_count[g];
-            if (c == 0) continue;
-            const int pop = a.group_info[g] >> 16;
-            const int b = a.group_prefix[g];
-            const EvPopDev& P = s_pops[pop];
-            const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
-            const uint32_t local0 = static_cast<uint32_t>(g - P.group0) * kWave;
-            for (int j = 0; j < c && b + j < S; ++j) {
-                const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
-                keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
-                              (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+        if (lds_prefix) {  // one thread per spike: every region load in flight at once
+            for (int i = tid; i < S; i += kSortThreads) {
+                const int g = group_of_entry(i, 0, G);
+                const int j = i - s_gp[g];
+                const int pop = a.group_info[g] >> 16;
+                const EvPopDev& P = s_pops[pop];
+                const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+                const uint32_t local = static_cast<uint32_t>(g - P.group0) * kWave + static_cast<uint32_t>(a.reg_lane[rb + j]);
+                keys[i] = (static_cast<unsigned long long>(pop) << 56) |
+                          (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+            }
+        } else {
+            for (int g = tid; g < G; g += kSortThreads) {  // one thread per group: a group holds few spikes
+                const int c = a.reg_count[g];
+                if (c == 0) continue;
+                const int pop = a.group_info[g] >> 16;
+                const int b = a.group_prefix[g];
+                const EvPopDev& P = s_pops[pop];
+                const int64_t rb = P.region0 + static_cast<int64_t>(g - P.group0) * P.cap;
+                const uint32_t local0 = static_cast<uint32_t>(g - P.group0) * kWave;
+                for (int j = 0; j < c && b + j < S; ++j) {
+                    const uint32_t local = local0 + static_cast<uint32_t>(a.reg_lane[rb + j]);
+                    keys[b + j] = (static_cast<unsigned long long>(pop) << 56) |
+                                  (static_cast<unsigned long long>(__float_as_uint(a.reg_e[rb + j])) << 24) | local;
+                }
             }
         }
         for (int64_t i = S + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
@@ -920,7 +950,29 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         unsigned long long* keys = in_lds ? s_k : a.keys;
         const float a0 = window_time(k + 1, a.W);  // (the earliest arrival of the next window)
         __syncthreads();
-        if (!multi) {
+        if (!multi && lds_prefix) {  // one thread per spike of the stream's sources
+            int nb_r[N2H_EV_MAX_SOURCES];  // (a register copy: the source search below indexes it statically)
+#pragma unroll
+            for (int q = 0; q < N2H_EV_MAX_SOURCES; ++q) nb_r[q] = q < St.n_src ? nb_j[q] : INT_MAX;
+            for (int i = tid; i < ns; i += kSortThreads) {
+                int j = 0;
+#pragma unroll
+                for (int q = 1; q < N2H_EV_MAX_SOURCES; ++q) j = (i >= nb_r[q]) ? q : j;
+                const int pop = St.src[j];
+                const EvPopDev& Q = s_pops[pop];
+                int nbj = nb_r[0];
+#pragma unroll
+                for (int q = 1; q < N2H_EV_MAX_SOURCES; ++q) nbj = (q == j) ? nb_r[q] : nbj;
+                const int ip = s_gp[Q.group0] + (i - nbj);  // its entry in the window's list
+                const int g = group_of_entry(ip, Q.group0, Q.group0 + Q.n_groups);
+                const int64_t rb = Q.region0 + static_cast<int64_t>(g - Q.group0) * Q.cap + (ip - s_gp[g]);
+                const float arr = a.reg_e[rb] + St.delay[j];
+                const uint32_t slot = static_cast<uint32_t>(g) * kWave + static_cast<uint32_t>(a.reg_lane[rb]);
+                keys[i] = (static_cast<unsigned long long>(__float_as_uint(arr)) << 32) |
+                          (static_cast<unsigned long long>(j) << 28) | slot;
+                if (binned) atomicAdd(&s_bin[arrival_bin(arr, a0, inv_bw)], 1);
+            }
+        } else if (!multi) {
             for (int g = tid; g < G; g += kSortThreads) {
                 const int pop = a.group_info[g] >> 16;
                 const int cix = s_c[pop];
@@ -954,11 +1006,12 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
             }
         }
         __syncthreads();
+        if (trow && si == 0) trow[5] = wall_clock64();
         const unsigned long long* out = keys;
         bool crowded = false;
         if (binned) {
             bool big = false;
-            for (int b = tid; b < kBins; b += kSortThreads) big = big || s_bin[b] > kBinSortMax;
+            for (int b = tid; b < kBins; b += kSortThreads) big = big || s_bin[b] > kBinRankMax;
             crowded = __syncthreads_or(big) != 0;
         }
         if (binned && !crowded) {
@@ -987,20 +1040,26 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
                 el2[s_bin[b] + atomicAdd(&s_fill[b], 1)] = key;
             }
             __syncthreads();
-            for (int b = tid; b < kBins; b += kSortThreads) {  // each bin's few keys: insertion sort
-                const int lo = s_bin[b], hi = s_bin[b + 1];
-                for (int i = lo + 1; i < hi; ++i) {
-                    const unsigned long long x = el2[i];
-                    int j = i - 1;
-                    while (j >= lo && el2[j] > x) {
-                        el2[j + 1] = el2[j];
-                        --j;
-                    }
-                    el2[j + 1] = x;
-                }
-            }
+            // each key's place: its bin's start + the keys of its bin below it (keys are distinct; a bin's
+            // keys are read by all its threads at once, LDS broadcasts).  Spikes at one exact time (a
+            // neuron fires at the arrival of the input that crosses its threshold) crowd one bin.
+            static_assert(kBinCap <= 2 * kSortThreads, "two keys a thread");
+            const int i0 = tid, i1 = tid + kSortThreads;
+            const unsigned long long x0 = i0 < ns ? keys[i0] : 0ull;
+            const unsigned long long x1 = i1 < ns ? keys[i1] : 0ull;
+            __syncthreads();  // every key read before the placement overwrites keys
+            if (trow && si == 0) trow[6] = wall_clock64();
+            auto place = [&](unsigned long long x) {
+                const int b = arrival_bin(__uint_as_float(static_cast<uint32_t>(x >> 32)), a0, inv_bw);
+                const int hi = s_bin[b + 1];
+                int r = s_bin[b];
+                for (int q = r; q < hi; ++q) r += (el2[q] < x) ? 1 : 0;
+                keys[r] = x;
+            };
+            if (i0 < ns) place(x0);
+            if (i1 < ns) place(x1);
             __syncthreads();
-            out = el2;
+            out = keys;
         } else {
             for (int64_t i = ns + tid; i < n2; i += kSortThreads) keys[i] = ~0ull;
             __syncthreads();
@@ -1015,7 +1074,7 @@ __global__ __launch_bounds__(kSortThreads) void ev_sort_kernel(EvArgs a) {
         if (tid == 0) a.stream_count[si] = ns;
         __syncthreads();
     }
-    if (trow) trow[4] = trow[5] = trow[6] = wall_clock64();
+    if (trow) trow[4] = wall_clock64();
     if (tid == 0) *a.window = k + 1;
     if (trow) trow[7] = wall_clock64();
 }
@@ -1260,7 +1319,7 @@ int n2h_ev_create(const n2h_ev_desc* d, n2h_ev_plan** out) {
     int dev = 0, lds_max = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    p->sort_lds = lds_max >= 8192 * 8 + 24 * 1024 ? 8192 : 4096;  // (+ the sort kernel's static LDS, ~20 KB)
+    p->sort_lds = lds_max >= 8192 * 8 + 40 * 1024 ? 8192 : 4096;  // (+ the sort kernel's static LDS, ~36 KB)
     if (p->sort_lds == 8192 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(ev_sort_kernel<8192>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)

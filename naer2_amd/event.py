@@ -454,7 +454,8 @@ class EventRuntime:
         return float(d.value), float(s.value)
 
     TRACE_PHASES = ("entries", "staging", "inbox places", "inbox fill", "walk")
-    SORT_EVENTS = ("start", "group prefix", "record", "first stream sorted", "streams written", "-", "-", "end")
+    SORT_EVENTS = ("start", "group prefix", "record", "first stream sorted", "streams written", "first stream keyed",
+                   "first stream binned", "end")
 
     def trace(self, n_windows: int = 1) -> dict:
         """Run n_windows eager windows with the in-kernel phase trace on; for the last one: per phase of the
@@ -486,6 +487,10 @@ class EventRuntime:
         out["sort"] = {f"{self.SORT_EVENTS[j - 1]} -> {self.SORT_EVENTS[j]}": float((srow[j] - srow[j - 1]) / 100.0)
                        for j in (1, 2, 3, 4)}
         out["sort"]["streams written -> end"] = float((srow[7] - srow[4]) / 100.0)
+        if srow[5] and srow[6]:  # (the first stream's keys, then its binning, inside record -> sorted)
+            out["sort"]["record -> first stream keyed"] = float((srow[5] - srow[2]) / 100.0)
+            out["sort"]["keyed -> binned"] = float((srow[6] - srow[5]) / 100.0)
+            out["sort"]["binned -> first stream sorted"] = float((srow[3] - srow[6]) / 100.0)
         if used.any():
             out["deliver end -> sort start"] = float((srow[0] - rows[used, 7].max()) / 100.0)
         return out
