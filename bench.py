@@ -254,7 +254,7 @@ def bench_pd14grad(args) -> int:
     print(f"[bench] built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s",
           file=sys.stderr, flush=True)
     sur = {"cutoff": args.cutoff, "gamma": args.gamma}
-    grad = RateLossGrad(rts, alpha, checkpoint=K, **sur)
+    grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, **sur)
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
@@ -289,7 +289,7 @@ def bench_pd14grad(args) -> int:
         "forward_ms": round(fw, 1), "backward_ms": round(bw, 1),
         "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
         "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
-        "surrogate": {"kappa": 25.0, **sur},
+        "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay,
         "grad_rel_err_vs_fp64_subsample": err,
         "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
     }
@@ -608,7 +608,8 @@ def bench_bptt(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 10000; pd14grad: 5000, i.e. 500 ms of bio time)")
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--window", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--cpu-steps", type=int, default=2000,
@@ -629,6 +630,9 @@ def main() -> int:
     ap.add_argument("--ev-cpu-windows", type=int, default=10, help="brunel_exact: windows the CPU oracle runs")
     ap.add_argument("--instances", type=int, default=None, help="networks (sweep: 256, pd14grad: 4)")
     ap.add_argument("--checkpoint", type=int, default=500, help="pd14grad: steps between saved states")
+    ap.add_argument("--long-delay", type=int, default=4,
+                    help="pd14grad: out-links with at least this delay (steps) are gathered once per window of "
+                         "that many reverse steps")
     ap.add_argument("--cutoff", type=float, default=0.2,
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,
@@ -639,6 +643,8 @@ def main() -> int:
                     help="only run 5 plain device-to-device copies of MiB each (the reference kernel that scales "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes), then exit")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 5000 if args.workload == "pd14grad" else 10000
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")

@@ -213,28 +213,35 @@ int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base,
 
 /* Surrogate-gradient adjoint of the LIF_EXP step for a per-population rate loss (no counterpart in the
  * reference, which is forward-only: a surrogate-gradient adjoint beside the plain path).
- * One call runs reverse step `step` for up to 16 networks in one launch.  Per network: its out-links
- * (canonical CSR by source) packed {target | (delay_steps - 1) << 17 | target population << 24, base weight
- * bits}; alpha[target pop][source pop] the weight scales the forward ran with (its weights are alpha * base);
- * pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset, dL/ds of one spike of population p, 0};
- * this step's potentials before the threshold (n2h_set_vtrace) and spikes; the adjoint state gV, gI [n],
- * the ring gx [ring_slots = max delay + 1][n] of the input adjoints of the following steps (zero at the start),
- * and g_sum [n][n_pops], where every spiking source j adds sum over its links into population a of
- * w gx_{step+d} (dL/dalpha[a][pop j] = sum over j in pop j of g_sum[j][a]).  The reset and the refractory
- * gate carry no gradient. */
+ * One n2h_adj_step call runs reverse step `step` for up to 16 networks in one launch.  Per network: its
+ * out-links (canonical CSR by source) split by delay into a short list (delay < long_delay steps) and a long
+ * list (delay >= long_delay), each kept in canonical order and packed {target | (delay_steps - 1) << 17 |
+ * target population << 24, base weight bits}; alpha[target pop][source pop] the weight scales the forward
+ * ran with (its weights are alpha * base); pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset,
+ * dL/ds of one spike of population p, 0}; this step's potentials before the threshold (n2h_set_vtrace) and
+ * spikes; the adjoint state gV, gI [n]; the ring gx [n][ring_slots + long_delay - 1] of the input adjoints of
+ * the following steps (zero at the start; ring_slots > the largest delay; step s in slot s mod ring_slots,
+ * the first long_delay - 1 slots mirrored past the end), and g_sum [n][n_pops], where every spiking source
+ * j adds sum over its links into population a of w gx_{step+d} (dL/dalpha[a][pop j] = sum over j in pop j
+ * of g_sum[j][a]).  The long links' sums come from long_bins [n][8], which n2h_adj_window filled for the
+ * window holding this step (a neuron reads its row only where it is within the surrogate's support or
+ * spiked).  The reset and the refractory gate carry no gradient. */
 typedef struct n2h_adj_net {
     int32_t n_neurons;           /* <= 2^17 */
     int32_t n_pops;              /* <= 8 */
     int32_t ring_slots;
-    int32_t reserved;
+    int32_t long_delay;          /* 1 .. 8: the window length and the long-list delay bound */
     const int32_t* pop_first;    /* device [n_pops + 1] */
-    const int64_t* out_ptr;      /* device [n + 1] */
-    const uint32_t* out_links;   /* device [2 * n_links] */
+    const int64_t* out_ptr;      /* device [n + 1], short list */
+    const uint32_t* out_links;   /* device [2 * n_short] */
+    const int64_t* long_ptr;     /* device [n + 1], long list */
+    const uint32_t* long_links;  /* device [2 * n_long] */
     const float* alpha;          /* device [n_pops][n_pops] */
     const float* pop_coef;       /* device [n_pops][8] */
-    const float* vtrace;         /* device [n] */
-    const uint8_t* spikes;       /* device [n] */
-    float* gx_ring;              /* device [ring_slots][n] */
+    const float* vtrace;         /* device [n] (n2h_adj_window: [n_win][n] from the window's first step) */
+    const uint8_t* spikes;       /* device [n] (n2h_adj_window: [n_win][n]) */
+    float* gx_ring;              /* device [n][ring_slots + long_delay - 1] */
+    float* long_bins;            /* device [n][8] of this step (n2h_adj_window: [n_win][n][8]) */
     float* g_v;                  /* device [n] */
     float* g_i;                  /* device [n] */
     double* g_sum;               /* device [n][n_pops] */
@@ -248,7 +255,20 @@ typedef struct n2h_surrogate {
     float reserved;
 } n2h_surrogate;
 int n2h_adj_last_error(char* buf, size_t len);
-int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h_surrogate* sur, void* stream);
+/* win_step = step - the first step of its window (n2h_adj_window); work as for n2h_adj_window, unchanged
+ * since that call. */
+int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t win_step, const n2h_surrogate* sur,
+                 int32_t* work, int64_t work_words, void* stream);
+/* The long links of a window of reverse steps [step0, step0 + n_win), n_win <= long_delay, run before that
+ * window's n2h_adj_step calls (after step0 + n_win's): for every source j and window step w where j is within
+ * the surrogate's support or spiked, long_bins[w][j][a] = sum over j's long links into population a of
+ * w_base gx_{step0+w+d}.  Every such input adjoint lies past the window (d >= long_delay >= n_win), so one
+ * pass over a source's long links serves the whole window, and a link's W input adjoints are adjacent in
+ * the ring.  The sums do not depend on where windows start.  work: device int32 scratch of at least
+ * 16 + (1 + long_delay) * (neurons over the networks) words, where the call lists the neurons that gather
+ * (the window's and each step's) for the gather kernels. */
+int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32_t n_win, const n2h_surrogate* sur,
+                   int32_t* work, int64_t work_words, void* stream);
 
 /* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
  * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a

@@ -7,9 +7,11 @@ for a per-population rate loss, with respect to the weight scales alpha[target p
 loss = sum over networks and populations of (rate - target)^2 (rates in Hz over the n_steps).  The forward is
 the plain step (bit-exact, one batched launch per step, FusedRound), with the state saved every `checkpoint`
 steps.  The backward runs the segments in reverse: each is recomputed from its checkpoint with the
-potentials before the threshold traced (n2h_set_vtrace), then every step is reversed by n2h_adj_step -- one
-launch for all networks, a wavefront per source gathering its out-links' input adjoints
-(naer2_amd/csrc/adjoint.hip states the recursion).  The surrogate derivative is
+potentials before the threshold traced (n2h_set_vtrace), then reversed in windows of `long_delay` steps:
+n2h_adj_window gathers the input adjoints of every out-link with a delay of at least long_delay once per
+window (they all lie past it), then n2h_adj_step reverses each step of the window -- one launch for all
+networks, a wavefront per source gathering its short links' input adjoints and adding the window's long
+sums (naer2_amd/csrc/adjoint.hip states the recursion).  The surrogate derivative is
 gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - theta| / dv <= cutoff and 0 beyond (dv = theta -
 V_reset); the cutoff makes the gradient sparse (only neurons near the threshold, or spiking, gather).  The
 reset and refractoriness carry no gradient (through the hard reset the adjoint diverges).
@@ -31,9 +33,9 @@ from .fusion import FusedRound
 
 class AdjNet(ctypes.Structure):
     _fields_ = [("n_neurons", ctypes.c_int32), ("n_pops", ctypes.c_int32), ("ring_slots", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in (
-                    "pop_first", "out_ptr", "out_links", "alpha", "pop_coef", "vtrace", "spikes", "gx_ring", "g_v",
-                    "g_i", "g_sum")]
+                ("long_delay", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in (
+                    "pop_first", "out_ptr", "out_links", "long_ptr", "long_links", "alpha", "pop_coef", "vtrace",
+                    "spikes", "gx_ring", "long_bins", "g_v", "g_i", "g_sum")]
 
 
 class Surrogate(ctypes.Structure):
@@ -45,9 +47,12 @@ def _bind(lib):
     if getattr(lib, "_adj_bound", False):
         return lib
     lib.n2h_adj_last_error.argtypes, lib.n2h_adj_last_error.restype = [ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int32
-    lib.n2h_adj_step.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Surrogate),
-                                 ctypes.c_void_p]
+    lib.n2h_adj_step.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                 ctypes.POINTER(Surrogate), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
     lib.n2h_adj_step.restype = ctypes.c_int32
+    lib.n2h_adj_window.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                   ctypes.POINTER(Surrogate), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    lib.n2h_adj_window.restype = ctypes.c_int32
     lib._adj_bound = True
     return lib
 
@@ -59,9 +64,11 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
 
 
-def out_links(rt, alpha: torch.Tensor):
-    """(out_ptr int64 [n + 1], links int32 [E, 2]): rt's canonical CSR by source packed for the adjoint,
-    {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight / alpha[pt][ps]."""
+def out_links(rt, alpha: torch.Tensor, long_delay: int = 1):
+    """(short_ptr, short_links, long_ptr, long_links): rt's canonical CSR by source packed for the adjoint,
+    {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight / alpha[pt][ps],
+    split into the links with delay < long_delay and the rest (ptr int64 [n + 1], links int32 [E, 2]), each
+    in canonical order."""
     c = rt.canon
     if c is None:
         raise RuntimeError("the adjoint needs the canonical synapse table (Network.keep_canonical = True)")
@@ -77,12 +84,21 @@ def out_links(rt, alpha: torch.Tensor):
         raise ValueError("a weight scale of a connected population pair is 0: its base weights are unknown")
     base = c.w.to(dev) / scale
     links = torch.stack([x.to(torch.int32), base.contiguous().view(torch.int32)], dim=1).contiguous()
-    return c.src_ptr.to(dev).to(torch.int64).contiguous(), links
+    is_long = c.d.to(dev) >= long_delay
+    out = []
+    for keep in (~is_long, is_long):
+        cnt = torch.bincount(src[keep], minlength=rt.n)
+        ptr = torch.zeros(rt.n + 1, dtype=torch.int64, device=dev)
+        ptr[1:] = cnt.cumsum(0)
+        sel = links[keep] if bool(keep.any()) else torch.zeros(1, 2, dtype=torch.int32, device=dev)  # never empty
+        out += [ptr, sel.contiguous()]
+    return tuple(out)
 
 
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
-                 cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None):
+                 cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
+                 long_delay: int = 4):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -100,6 +116,8 @@ class RateLossGrad:
             raise ValueError("RateLossGrad: the networks are at different steps")
         if checkpoint < 1:
             raise ValueError("checkpoint must be positive")
+        if not 1 <= long_delay <= 8:
+            raise ValueError("long_delay: 1 to 8 steps")
         self.runtimes = list(runtimes)
         self.dev = runtimes[0].device
         P = len(runtimes[0].pops)
@@ -109,9 +127,12 @@ class RateLossGrad:
         self.alpha = torch.as_tensor(np.asarray(alpha, dtype=np.float32), device=self.dev).reshape(P, P).contiguous()
         self.checkpoint, self.window, self.tuning = int(checkpoint), int(graph_window), tuning
         self.sur = Surrogate(float(kappa), float(cutoff), float(gamma), 0.0)
-        self.tables = [out_links(rt, self.alpha) for rt in runtimes]
+        self.long_delay = int(long_delay)
+        self.tables = [out_links(rt, self.alpha, self.long_delay) for rt in runtimes]
         self.pop_first = [torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int32), device=self.dev) for rt in runtimes]
         self.ring_slots = max(rt.d_slots for rt in runtimes) + 1
+        if any(int(rt.canon.d.max()) >= self.ring_slots for rt in runtimes if rt.canon.d.numel()):
+            raise ValueError("RateLossGrad: a delay reaches the ring length")
         self.last = {}
 
     def _counts(self, rt, s0: int, T: int) -> np.ndarray:
@@ -154,7 +175,7 @@ class RateLossGrad:
         # ---- backward ----
         t_bw = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t_bw[0].record()
-        R = self.ring_slots
+        R, W = self.ring_slots, self.long_delay
         st = []
         for i, rt in enumerate(rts):
             coef = np.zeros((self.P, 8), dtype=np.float32)
@@ -162,7 +183,8 @@ class RateLossGrad:
                 coef[p] = [prm.decay, prm.p_i, prm.p_vi, prm.v_reset, prm.v_th, prm.v_th - prm.v_reset, c_pops[i][p], 0.0]
             st.append({
                 "coef": torch.as_tensor(coef, device=self.dev),
-                "gx": torch.zeros(R, rt.n, device=self.dev), "gv": torch.zeros(rt.n, device=self.dev),
+                "gx": torch.zeros(rt.n, R + W - 1, device=self.dev), "gv": torch.zeros(rt.n, device=self.dev),
+                "lb": torch.zeros(W, rt.n, 8, device=self.dev),
                 "gi": torch.zeros(rt.n, device=self.dev),
                 "gsum": torch.zeros(rt.n, self.P, dtype=torch.float64, device=self.dev),
                 "vtr": torch.empty(min(K, T), rt.n, device=self.dev),
@@ -170,6 +192,13 @@ class RateLossGrad:
             })
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         nets = (AdjNet * len(rts))()
+        for i, (rt, sd, (sptr, sl, lptr, ll)) in enumerate(zip(rts, st, self.tables)):
+            nets[i] = AdjNet(rt.n, self.P, R, W, self.pop_first[i].data_ptr(), sptr.data_ptr(), sl.data_ptr(),
+                             lptr.data_ptr(), ll.data_ptr(), self.alpha.data_ptr(), sd["coef"].data_ptr(), 0, 0,
+                             sd["gx"].data_ptr(), 0, sd["gv"].data_ptr(), sd["gi"].data_ptr(), sd["gsum"].data_ptr())
+        sur = ctypes.byref(self.sur)
+        work = torch.zeros(16 + (1 + W) * sum(rt.n for rt in rts), dtype=torch.int32, device=self.dev)
+        wp, wn = work.data_ptr(), work.numel()
         for k in reversed(range(len(ckpts))):
             start = s0 + k * K
             L = min(K, T - k * K)
@@ -183,13 +212,18 @@ class RateLossGrad:
                 _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0), "n2h_set_vtrace")
                 gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
                 sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
-            for t in reversed(range(L)):
-                for i, (rt, sd, (optr, ol)) in enumerate(zip(rts, st, self.tables)):
-                    nets[i] = AdjNet(rt.n, self.P, R, 0, self.pop_first[i].data_ptr(), optr.data_ptr(), ol.data_ptr(),
-                                     self.alpha.data_ptr(), sd["coef"].data_ptr(), sd["vtr"][t].data_ptr(),
-                                     sd["spk"][t].data_ptr(), sd["gx"].data_ptr(), sd["gv"].data_ptr(),
-                                     sd["gi"].data_ptr(), sd["gsum"].data_ptr())
-                _check(lib.n2h_adj_step(nets, len(rts), start + t, ctypes.byref(self.sur), stream), "n2h_adj_step")
+            # windows of W steps from the segment's end: [t0, t1)
+            for t1 in range(L, 0, -W):
+                t0 = max(0, t1 - W)
+                for i, sd in enumerate(st):
+                    nets[i].vtrace, nets[i].spikes = sd["vtr"][t0].data_ptr(), sd["spk"][t0].data_ptr()
+                    nets[i].long_bins = sd["lb"].data_ptr()
+                _check(lib.n2h_adj_window(nets, len(rts), start + t0, t1 - t0, sur, wp, wn, stream), "n2h_adj_window")
+                for t in reversed(range(t0, t1)):
+                    for i, sd in enumerate(st):
+                        nets[i].vtrace, nets[i].spikes = sd["vtr"][t].data_ptr(), sd["spk"][t].data_ptr()
+                        nets[i].long_bins = sd["lb"][t - t0].data_ptr()
+                    _check(lib.n2h_adj_step(nets, len(rts), start + t, t - t0, sur, wp, wn, stream), "n2h_adj_step")
         t_bw[1].record()
         for rt, sn, rf in zip(rts, final, rec_from):
             rt.restore(sn)

@@ -1,5 +1,5 @@
 // Surrogate-gradient adjoint of the exponential-synapse LIF step (the LIF_EXP model of naer2_hip.hip) for a
-// per-population rate loss, batched over networks: one launch per reverse step.
+// per-population rate loss, batched over networks: two launches per reverse step, two per window of long links.
 //
 // The reference is forward-only; this adjoint is added beside its plain path.  Forward step t of neuron i (naer2_hip.hip, step_group):
 //     I'_t = I_{t-1} + x_t + ext_t
@@ -18,8 +18,17 @@
 //     gVpre  = (1 - s_t) gV + r_t h(V^pre_t) gs
 //     gI'    = r_t P_VI gVpre + P_I gI;   gV <- r_t P_V gVpre;   gI <- gI';   gx_t[i] = gI'
 //     dL/dalpha[a][pop i] += s_t[i] * sum over out-links i -> k in pop a of w gx_{t+d}[k]
-// One wavefront per source neuron gathers its out-links (lanes stride the list; per-lane sums per target
-// population, then a fixed butterfly): every sum has a fixed order, so results are deterministic.
+// A wavefront gathers the out-links of one neuron that needs them (lanes stride the list; per-lane sums per
+// target population, then a fixed butterfly): every sum has a fixed order, so results are deterministic.
+//
+// Windows.  The out-links are split by delay: short (d < W) and long (d >= W), W = long_delay.  For a window
+// of reverse steps [t0, t0 + W) every long link's input adjoints gx_{t0+w+d} (w < W) lie past the window, so
+// adj_long_kernel reads each gathering source's long links once per window and gathers a link's W values
+// in one contiguous read: the ring is target-major, gx[k][slot], step s in slot s mod R, with the first
+// W - 1 slots mirrored past slot R - 1 so that W consecutive steps never wrap.  Each step, adj_short_kernel
+// walks only the short links and adds the window's long sums (bins[a] = short sum + long sum), then
+// adj_update_kernel updates every neuron.  The split depends on the link alone, so the sums do not depend on
+// where windows, checkpoints or batches start.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,6 +45,8 @@ constexpr int kWave = 64;
 constexpr int kAdjPops = 8;      // target-population bins per lane
 constexpr int kAdjNets = 16;     // networks in one launch
 constexpr int kWavesPerBlock = 4;
+constexpr int kMaxWindow = 8;
+constexpr int kUnroll = 4;        // links a lane loads per pass of a gather loop
 
 thread_local char g_err[512] = "";
 
@@ -51,61 +62,244 @@ struct AdjBatch {
     n2h_adj_net net[kAdjNets];
     int32_t first_block[kAdjNets + 1];
     int32_t n_nets;
-    int32_t ring_pos;  // step mod ring_slots: the slot this step's gx is written to
+    int32_t ring_pos;  // step (window: its first step) mod ring_slots
+    int32_t n_win;     // window: its steps
+    int32_t win_step;  // step: its place in the window
+    int32_t n_total;   // neurons over the networks: the capacity of each work list
+    int32_t* work;     // counters and work lists
     n2h_surrogate sur;
 };
 
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBatch b) {
+// The network of this block and the first neuron of this wavefront: a block covers kWavesPerBlock * 64
+// consecutive neurons of one network, a lane one neuron.
+__device__ inline int adj_wave_base(const AdjBatch& b, int& q) {
     const int blk = blockIdx.x;
-    int q = 0;
+    q = 0;
     while (q + 1 < b.n_nets && b.first_block[q + 1] <= blk) ++q;
-    const n2h_adj_net& a = b.net[q];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int j = (blk - b.first_block[q]) * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6);
-    if (j >= a.n_neurons) return;
-    const int n = a.n_neurons;
-    const int R = a.ring_slots;
+    return ((blk - b.first_block[q]) * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6)) * kWave;
+}
 
+__device__ inline int adj_pop(const n2h_adj_net& a, int j) {
     int pj = 0;
     while (pj + 1 < a.n_pops && a.pop_first[pj + 1] <= j) ++pj;
+    return pj;
+}
+
+// |V - theta| / dv: +inf on refractory steps (the trace holds +inf there)
+__device__ inline float adj_u(float vp, float theta, float dv) { return fabsf(vp - theta) / dv; }
+
+// one link {target | (d - 1) << 17 | target pop << 24, base weight}: the ring offset of gx_{t+d}[target]
+// (pos = t mod R; d < R) and the target population
+__device__ inline int64_t adj_link_slot(uint32_t x, int pos, int R, int RS, uint32_t& tp) {
+    const uint32_t tgt = x & 0x1ffffu;
+    const int d = static_cast<int>((x >> 17) & 127u) + 1;
+    tp = x >> 24;
+    int slot = pos + d;
+    slot -= (slot >= R) ? R : 0;
+    return static_cast<int64_t>(tgt) * RS + slot;
+}
+
+// Work lists.  Few neurons gather (a few percent a step), and they cluster in the more active populations,
+// so a lane-per-neuron wavefront walking its own gatherers in turn leaves most wavefronts idle behind a few.
+// Instead adj_mark_kernel appends every gatherer to a list (the window's, entry mask << 21 | net << 17 |
+// neuron, and each window step's, net << 17 | neuron), and the gather kernels give each list entry a whole
+// wavefront in grid-stride order.  List order depends on the schedule, but every entry's sums are its own,
+// in a fixed order, so results do not.  work: [0] window count, [1 + w] step w's count, lists from word 16.
+constexpr int kAdjListHead = 16;
+constexpr int kGatherBlocks = 1280;  // 5 blocks of 4 wavefronts per CU
+
+__device__ inline int64_t adj_list(const AdjBatch& b, int which) {  // which 0: window, 1 + w: step w
+    return kAdjListHead + static_cast<int64_t>(which) * b.n_total;
+}
+
+// lane per neuron: its window mask (steps where it is within the support or spiked), appended to the lists
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBatch b) {
+    int q;
+    const int base = adj_wave_base(b, q);
+    const n2h_adj_net& a = b.net[q];
+    const int n = a.n_neurons;
+    if (base >= n) return;  // uniform over the wavefront
+    const int lane = threadIdx.x & (kWave - 1);
+    const int j = base + lane;
+    uint32_t mask = 0;
+    if (j < n) {
+        const float* C = a.pop_coef + adj_pop(a, j) * 8;
+        const float theta = C[4], dv = C[5];
+        for (int w = 0; w < b.n_win; ++w) {
+            const bool s = a.spikes[static_cast<int64_t>(w) * n + j] != 0;
+            const bool near = adj_u(a.vtrace[static_cast<int64_t>(w) * n + j], theta, dv) <= b.sur.cutoff;
+            mask |= (s || near) ? (1u << w) : 0u;
+        }
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int which = 0; which <= b.n_win; ++which) {
+        const bool in = which == 0 ? mask != 0 : ((mask >> (which - 1)) & 1u) != 0;
+        const uint64_t bal = __ballot(in);
+        if (bal == 0) continue;  // uniform
+        int first = 0;
+        if (lane == 0) first = atomicAdd(b.work + which, __popcll(bal));
+        first = __shfl(first, 0, kWave);
+        const uint32_t ent = (which == 0 ? mask << 21 : 0u) | (static_cast<uint32_t>(q) << 17) | static_cast<uint32_t>(j);
+        if (in) b.work[adj_list(b, which) + first + __popcll(bal & below)] = static_cast<int32_t>(ent);
+    }
+}
+
+// The window's gatherers, a wavefront each: their long links, a link's W input adjoints in one contiguous
+// read; lanes stride the list with per-lane sums per (target population, window step), then a fixed
+// butterfly; long_bins rows of the steps in the entry's mask.
+template <int W>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n_waves = gridDim.x * kWavesPerBlock;
+    const int cnt = b.work[0];
+    const int32_t* list = b.work + adj_list(b, 0);
+    const int ww = lane >> 3, pp = lane & 7;  // lane ww * 8 + pp stores bins[pp][ww]
+    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < cnt; i += n_waves) {
+        const uint32_t ent = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(list[i]));
+        const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
+        const uint32_t mk = ent >> 21;
+        const n2h_adj_net& a = b.net[q];
+        const int n = a.n_neurons, R = a.ring_slots, RS = R + W - 1;
+        float bins[kAdjPops][W];
+#pragma unroll
+        for (int p = 0; p < kAdjPops; ++p)
+#pragma unroll
+            for (int w = 0; w < W; ++w) bins[p][w] = 0.0f;
+        const int64_t e0 = a.long_ptr[jj], e1 = a.long_ptr[jj + 1];
+        // kUnroll links a lane per pass, every load issued before the sums; the lane's links are added in
+        // list order (e, e + 64, ...) as in a plain loop
+        const uint2* links = reinterpret_cast<const uint2*>(a.long_links);
+        for (int64_t e = e0 + lane; e < e1; e += kUnroll * kWave) {
+            uint2 l[kUnroll];
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const int64_t er = e + r * kWave;
+                l[r] = er < e1 ? links[er] : make_uint2(0u, 0u);  // padding: target 0, delay 1, unused
+            }
+            float gw[kUnroll][W];
+            uint32_t tp[kUnroll];
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const float* g = a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]);
+#pragma unroll
+                for (int w = 0; w < W; ++w) gw[r][w] = g[w];
+            }
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const bool live = e + r * kWave < e1;
+                const float wt = __uint_as_float(l[r].y);
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const float c = wt * gw[r][w];
+#pragma unroll
+                    for (int p = 0; p < kAdjPops; ++p)
+                        bins[p][w] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[p][w] + c : bins[p][w];
+                }
+            }
+        }
+        if (e1 > e0) {  // fixed butterfly: every lane ends with every sum
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+                for (int p = 0; p < kAdjPops; ++p)
+#pragma unroll
+                    for (int w = 0; w < W; ++w) bins[p][w] += __shfl_xor(bins[p][w], off, kWave);
+            }
+        }
+        float v = 0.0f;
+#pragma unroll
+        for (int p = 0; p < kAdjPops; ++p)
+#pragma unroll
+            for (int w = 0; w < W; ++w) v = (ww == w && pp == p) ? bins[p][w] : v;
+        if (lane < W * kAdjPops && ((mk >> ww) & 1u)) a.long_bins[(static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp] = v;
+    }
+}
+
+// This step's gatherers, a wavefront each: their short links (lanes stride the list, per-lane sums per
+// target population, then a fixed butterfly); row = short sums + the window's long sums, in place.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBatch b) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n_waves = gridDim.x * kWavesPerBlock;
+    const int cnt = b.work[1 + b.win_step];
+    const int32_t* list = b.work + adj_list(b, 1 + b.win_step);
+    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < cnt; i += n_waves) {
+        const uint32_t ent = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(list[i]));
+        const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
+        const n2h_adj_net& a = b.net[q];
+        const int R = a.ring_slots, RS = R + a.long_delay - 1;
+        float bins[kAdjPops];
+#pragma unroll
+        for (int p = 0; p < kAdjPops; ++p) bins[p] = 0.0f;
+        const int64_t e0 = a.out_ptr[jj], e1 = a.out_ptr[jj + 1];
+        const uint2* links = reinterpret_cast<const uint2*>(a.out_links);
+        for (int64_t e = e0 + lane; e < e1; e += kUnroll * kWave) {  // as in adj_long_kernel
+            uint2 l[kUnroll];
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const int64_t er = e + r * kWave;
+                l[r] = er < e1 ? links[er] : make_uint2(0u, 0u);
+            }
+            float g[kUnroll];
+            uint32_t tp[kUnroll];
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) g[r] = a.gx_ring[adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r])];
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const bool live = e + r * kWave < e1;
+                const float c = __uint_as_float(l[r].y) * g[r];
+#pragma unroll
+                for (int p = 0; p < kAdjPops; ++p)
+                    bins[p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[p] + c : bins[p];
+            }
+        }
+        if (e1 > e0) {  // fixed butterfly: every lane ends with the same sums, in the same order on every run
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+                for (int p = 0; p < kAdjPops; ++p) bins[p] += __shfl_xor(bins[p], off, kWave);
+            }
+        }
+        if (lane < kAdjPops) {
+            float v = 0.0f;
+#pragma unroll
+            for (int p = 0; p < kAdjPops; ++p) v = (lane == p) ? bins[p] : v;
+            float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops;
+            row[lane] = v + row[lane];
+        }
+    }
+}
+
+// Lane per neuron: the update of the reverse step from the neuron's summed out-link adjoints (the rows
+// adj_short_kernel completed; zero for a neuron that did not gather).
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_update_kernel(AdjBatch b) {
+    int q;
+    const int base = adj_wave_base(b, q);
+    const n2h_adj_net& a = b.net[q];
+    const int j = base + static_cast<int>(threadIdx.x & (kWave - 1));
+    if (j >= a.n_neurons) return;
+    const int R = a.ring_slots, RS = R + a.long_delay - 1;
+
+    const int pj = adj_pop(a, j);
     const float* C = a.pop_coef + pj * 8;
     const float P_V = C[0], P_I = C[1], P_VI = C[2], theta = C[4], dv = C[5], c_loss = C[6];
     const float vp = a.vtrace[j];
     const bool s = a.spikes[j] != 0;
-    const float u = fabsf(vp - theta) / dv;  // +inf on refractory steps
-    const bool near = u <= b.sur.cutoff;     // the surrogate is nonzero
-    float bins[kAdjPops];
+    const float u = adj_u(vp, theta, dv);
+    const bool near = u <= b.sur.cutoff;  // the surrogate is nonzero (the predicate adj_mark_kernel listed)
+    float mine[kAdjPops];
 #pragma unroll
-    for (int p = 0; p < kAdjPops; ++p) bins[p] = 0.0f;
-    const int64_t e0 = a.out_ptr[j], e1 = (near || s) ? a.out_ptr[j + 1] : e0;
-    for (int64_t e = e0 + lane; e < e1; e += kWave) {
-        const uint2 l = reinterpret_cast<const uint2*>(a.out_links)[e];
-        const uint32_t tgt = l.x & 0x1ffffu;
-        const int d = static_cast<int>((l.x >> 17) & 127u) + 1;
-        const uint32_t tp = l.x >> 24;
-        int slot = b.ring_pos + d;
-        slot -= (slot >= R) ? R : 0;
-        const float g = a.gx_ring[static_cast<int64_t>(slot) * n + tgt];
-        const float c = __uint_as_float(l.y) * g;
-#pragma unroll
-        for (int p = 0; p < kAdjPops; ++p) bins[p] += (tp == static_cast<uint32_t>(p)) ? c : 0.0f;
+    for (int p = 0; p < kAdjPops; ++p) mine[p] = 0.0f;
+    if (near || s) {
+        const float4* lb = reinterpret_cast<const float4*>(a.long_bins + static_cast<int64_t>(j) * kAdjPops);
+        const float4 l0 = lb[0], l1 = lb[1];
+        mine[0] = l0.x; mine[1] = l0.y; mine[2] = l0.z; mine[3] = l0.w;
+        mine[4] = l1.x; mine[5] = l1.y; mine[6] = l1.z; mine[7] = l1.w;
     }
-    // fixed butterfly: every lane ends with the same sums, in the same order on every run (a neuron that
-    // gathered nothing has all-zero sums: no exchange)
-    if (e1 > e0) {
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-#pragma unroll
-            for (int p = 0; p < kAdjPops; ++p) bins[p] += __shfl_xor(bins[p], off, kWave);
-        }
-    }
-    if (lane != 0) return;
-
     float coup = 0.0f;
-    for (int p = 0; p < a.n_pops; ++p) coup += a.alpha[p * a.n_pops + pj] * bins[p];
+    for (int p = 0; p < a.n_pops; ++p) coup += a.alpha[p * a.n_pops + pj] * mine[p];
     if (s) {
         double* gs = a.g_sum + static_cast<int64_t>(j) * a.n_pops;
-        for (int p = 0; p < a.n_pops; ++p) gs[p] += static_cast<double>(bins[p]);
+        for (int p = 0; p < a.n_pops; ++p) gs[p] += static_cast<double>(mine[p]);
     }
     const float gv = a.g_v[j], gi = a.g_i[j];
     float gip, gv_new;
@@ -122,7 +316,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
     }
     a.g_v[j] = gv_new;
     a.g_i[j] = gip;
-    a.gx_ring[static_cast<int64_t>(b.ring_pos) * n + j] = gip;
+    float* row = a.gx_ring + static_cast<int64_t>(j) * RS;
+    row[b.ring_pos] = gip;
+    if (b.ring_pos < a.long_delay - 1) row[b.ring_pos + R] = gip;  // the mirror
 }
 
 }  // namespace
@@ -135,37 +331,111 @@ int n2h_adj_last_error(char* buf, size_t len) {
     return N2H_OK;
 }
 
-int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h_surrogate* sur, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// checks the networks and fills the launch block (first blocks, ring position of `step`)
+int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h_surrogate* sur, AdjBatch& b,
+                int64_t& blocks) {
     if (!nets || n_nets < 1 || n_nets > kAdjNets) return fail(N2H_EINVAL, "1 to %d networks per launch", kAdjNets);
     if (step < 0) return fail(N2H_EINVAL, "negative step");
     if (!sur || !(sur->kappa > 0.0f) || !(sur->cutoff >= 0.0f) || !(sur->gamma >= 0.0f))
         return fail(N2H_EINVAL, "surrogate: kappa > 0, cutoff >= 0, gamma >= 0");
-    AdjBatch b;
     memset(&b, 0, sizeof(b));
     b.n_nets = n_nets;
     b.sur = *sur;
-    int64_t blocks = 0;
-    int ring = -1;
+    blocks = 0;
+    int ring = -1, win = -1;
     for (int i = 0; i < n_nets; ++i) {
         const n2h_adj_net& a = nets[i];
         if (a.n_neurons <= 0 || a.n_neurons > (1 << 17)) return fail(N2H_EINVAL, "net %d: n_neurons outside [1, 2^17]", i);
         if (a.n_pops < 1 || a.n_pops > kAdjPops) return fail(N2H_EINVAL, "net %d: 1 to %d populations", i, kAdjPops);
         if (a.ring_slots < 2 || a.ring_slots > N2H_MAX_SLOTS + 1) return fail(N2H_EINVAL, "net %d: ring_slots", i);
-        if (!a.pop_first || !a.out_ptr || !a.out_links || !a.pop_coef || !a.alpha || !a.vtrace || !a.spikes ||
-            !a.gx_ring || !a.g_v || !a.g_i || !a.g_sum)
+        if (a.long_delay < 1 || a.long_delay > kMaxWindow)
+            return fail(N2H_EINVAL, "net %d: long_delay outside [1, %d]", i, kMaxWindow);
+        if (!a.pop_first || !a.out_ptr || !a.out_links || !a.long_ptr || !a.long_links || !a.pop_coef || !a.alpha ||
+            !a.vtrace || !a.spikes || !a.gx_ring || !a.long_bins || !a.g_v || !a.g_i || !a.g_sum)
             return fail(N2H_EINVAL, "net %d: null pointer", i);
         if (ring >= 0 && ring != a.ring_slots) return fail(N2H_EINVAL, "every network needs the same ring_slots");
+        if (win >= 0 && win != a.long_delay) return fail(N2H_EINVAL, "every network needs the same long_delay");
         ring = a.ring_slots;
+        win = a.long_delay;
         b.net[i] = a;
         b.first_block[i] = static_cast<int32_t>(blocks);
-        blocks += (a.n_neurons + kWavesPerBlock - 1) / kWavesPerBlock;
+        blocks += (a.n_neurons + kWave * kWavesPerBlock - 1) / (kWave * kWavesPerBlock);
     }
     b.first_block[n_nets] = static_cast<int32_t>(blocks);
     b.ring_pos = static_cast<int32_t>(step % ring);
-    hipLaunchKernelGGL(adj_step_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0,
-                       static_cast<hipStream_t>(stream), b);
+    return N2H_OK;
+}
+
+// the work buffer holds the counters and 1 + long_delay lists of every neuron of the networks
+int adj_bind_work(AdjBatch& b, const n2h_adj_net* nets, int32_t n_nets, int32_t* work, int64_t work_words) {
+    int64_t total = 0;
+    for (int i = 0; i < n_nets; ++i) total += nets[i].n_neurons;
+    const int64_t need = kAdjListHead + (1 + static_cast<int64_t>(nets[0].long_delay)) * total;
+    if (!work || work_words < need)
+        return fail(N2H_EINVAL, "work: %lld int32 words needed (16 + (1 + long_delay) * neurons), %lld given",
+                    static_cast<long long>(need), static_cast<long long>(work_words));
+    b.work = work;
+    b.n_total = static_cast<int32_t>(total);
+    return N2H_OK;
+}
+
+template <int W>
+void launch_window(int64_t blocks, hipStream_t stream, const AdjBatch& b) {
+    hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, stream, b);
+    hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t win_step, const n2h_surrogate* sur,
+                 int32_t* work, int64_t work_words, void* stream) {
+    AdjBatch b;
+    int64_t blocks;
+    int rc = adj_prepare(nets, n_nets, step, sur, b, blocks);
+    if (rc != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
+    if (win_step < 0 || win_step >= nets[0].long_delay)
+        return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
+    b.win_step = win_step;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(adj_short_kernel, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, s, b);
+    hipLaunchKernelGGL(adj_update_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, s, b);
     const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(N2H_EHIP, "adj_step_kernel: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(N2H_EHIP, "adjoint step kernels: %s", hipGetErrorString(e));
+    return N2H_OK;
+}
+
+int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32_t n_win, const n2h_surrogate* sur,
+                   int32_t* work, int64_t work_words, void* stream) {
+    AdjBatch b;
+    int64_t blocks;
+    int rc = adj_prepare(nets, n_nets, step0, sur, b, blocks);
+    if (rc != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
+    const int W = nets[0].long_delay;
+    if (n_win < 1 || n_win > W) return fail(N2H_EINVAL, "n_win %d outside [1, long_delay = %d]", n_win, W);
+    b.n_win = n_win;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemsetAsync(work, 0, kAdjListHead * sizeof(int32_t), s);  // the list counters
+    if (e != hipSuccess) return fail(N2H_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    switch (W) {
+        case 1: launch_window<1>(blocks, s, b); break;
+        case 2: launch_window<2>(blocks, s, b); break;
+        case 3: launch_window<3>(blocks, s, b); break;
+        case 4: launch_window<4>(blocks, s, b); break;
+        case 5: launch_window<5>(blocks, s, b); break;
+        case 6: launch_window<6>(blocks, s, b); break;
+        case 7: launch_window<7>(blocks, s, b); break;
+        default: launch_window<8>(blocks, s, b); break;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(N2H_EHIP, "adjoint window kernels: %s", hipGetErrorString(e));
     return N2H_OK;
 }
 
