@@ -241,6 +241,7 @@ typedef struct n2h_adj_net {
     const float* vtrace;         /* device [n] (n2h_adj_window: [n_win][n] from the window's first step) */
     const uint8_t* spikes;       /* device [n] (n2h_adj_window: [n_win][n]) */
     float* gx_ring;              /* device [n][ring_slots + long_delay - 1] */
+    float* gx_recent;            /* device [long_delay][n]: the short ring, step s in row s mod long_delay */
     float* long_bins;            /* device [n][8] of this step (n2h_adj_window: [n_win][n][8]) */
     float* g_v;                  /* device [n] */
     float* g_i;                  /* device [n] */
@@ -264,11 +265,14 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
  * the surrogate's support or spiked, long_bins[w][j][a] = sum over j's long links into population a of
  * w_base gx_{step0+w+d}.  Every such input adjoint lies past the window (d >= long_delay >= n_win), so one
  * pass over a source's long links serves the whole window, and a link's W input adjoints are adjacent in
- * the ring.  The sums do not depend on where windows start.  work: device int32 scratch of at least
- * 16 + (1 + long_delay) * (neurons over the networks) words, where the call lists the neurons that gather
- * (the window's and each step's) for the gather kernels. */
+ * the ring.  The sums do not depend on where windows start.  work: device int32 scratch of
+ * n2h_adj_work_words words, where the call lists the neurons that gather (the window's and each step's) for
+ * the gather kernels. */
 int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32_t n_win, const n2h_surrogate* sur,
                    int32_t* work, int64_t work_words, void* stream);
+/* The int32 words of the work buffer of n2h_adj_window / n2h_adj_step for networks of these sizes (-1: bad
+ * arguments). */
+int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay);
 
 /* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
  * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a

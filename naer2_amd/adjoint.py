@@ -35,7 +35,7 @@ class AdjNet(ctypes.Structure):
     _fields_ = [("n_neurons", ctypes.c_int32), ("n_pops", ctypes.c_int32), ("ring_slots", ctypes.c_int32),
                 ("long_delay", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in (
                     "pop_first", "out_ptr", "out_links", "long_ptr", "long_links", "alpha", "pop_coef", "vtrace",
-                    "spikes", "gx_ring", "long_bins", "g_v", "g_i", "g_sum")]
+                    "spikes", "gx_ring", "gx_recent", "long_bins", "g_v", "g_i", "g_sum")]
 
 
 class Surrogate(ctypes.Structure):
@@ -53,6 +53,8 @@ def _bind(lib):
     lib.n2h_adj_window.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.POINTER(Surrogate), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
     lib.n2h_adj_window.restype = ctypes.c_int32
+    lib.n2h_adj_work_words.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32]
+    lib.n2h_adj_work_words.restype = ctypes.c_int64
     lib._adj_bound = True
     return lib
 
@@ -184,7 +186,7 @@ class RateLossGrad:
             st.append({
                 "coef": torch.as_tensor(coef, device=self.dev),
                 "gx": torch.zeros(rt.n, R + W - 1, device=self.dev), "gv": torch.zeros(rt.n, device=self.dev),
-                "lb": torch.zeros(W, rt.n, 8, device=self.dev),
+                "lb": torch.zeros(W, rt.n, 8, device=self.dev), "gr": torch.zeros(W, rt.n, device=self.dev),
                 "gi": torch.zeros(rt.n, device=self.dev),
                 "gsum": torch.zeros(rt.n, self.P, dtype=torch.float64, device=self.dev),
                 "vtr": torch.empty(min(K, T), rt.n, device=self.dev),
@@ -195,9 +197,14 @@ class RateLossGrad:
         for i, (rt, sd, (sptr, sl, lptr, ll)) in enumerate(zip(rts, st, self.tables)):
             nets[i] = AdjNet(rt.n, self.P, R, W, self.pop_first[i].data_ptr(), sptr.data_ptr(), sl.data_ptr(),
                              lptr.data_ptr(), ll.data_ptr(), self.alpha.data_ptr(), sd["coef"].data_ptr(), 0, 0,
-                             sd["gx"].data_ptr(), 0, sd["gv"].data_ptr(), sd["gi"].data_ptr(), sd["gsum"].data_ptr())
+                             sd["gx"].data_ptr(), sd["gr"].data_ptr(), 0, sd["gv"].data_ptr(), sd["gi"].data_ptr(),
+                             sd["gsum"].data_ptr())
         sur = ctypes.byref(self.sur)
-        work = torch.zeros(16 + (1 + W) * sum(rt.n for rt in rts), dtype=torch.int32, device=self.dev)
+        sizes = (ctypes.c_int32 * len(rts))(*[rt.n for rt in rts])
+        words = lib.n2h_adj_work_words(sizes, len(rts), W)
+        if words < 0:
+            raise RuntimeError("n2h_adj_work_words: bad network sizes")
+        work = torch.zeros(words, dtype=torch.int32, device=self.dev)
         wp, wn = work.data_ptr(), work.numel()
         for k in reversed(range(len(ckpts))):
             start = s0 + k * K

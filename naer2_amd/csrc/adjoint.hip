@@ -65,7 +65,8 @@ struct AdjBatch {
     int32_t ring_pos;  // step (window: its first step) mod ring_slots
     int32_t n_win;     // window: its steps
     int32_t win_step;  // step: its place in the window
-    int32_t n_total;   // neurons over the networks: the capacity of each work list
+    int32_t recent_pos;  // step mod long_delay: this step's slot of the short ring
+    int32_t sub_cap;   // entries a work sub-list holds
     int32_t* work;     // counters and work lists
     n2h_surrogate sur;
 };
@@ -104,12 +105,63 @@ __device__ inline int64_t adj_link_slot(uint32_t x, int pos, int R, int RS, uint
 // Instead adj_mark_kernel appends every gatherer to a list (the window's, entry mask << 21 | net << 17 |
 // neuron, and each window step's, net << 17 | neuron), and the gather kernels give each list entry a whole
 // wavefront in grid-stride order.  List order depends on the schedule, but every entry's sums are its own,
-// in a fixed order, so results do not.  work: [0] window count, [1 + w] step w's count, lists from word 16.
-constexpr int kAdjListHead = 16;
+// in a fixed order, so results do not.  Each list (which = 0: the window's, 1 + w: step w's) is kept as
+// kSubLists sub-lists, a wavefront appending to sub-list (its index mod kSubLists), each counter on a cache
+// line of its own: appends to one counter serialize in the L2.
 constexpr int kGatherBlocks = 1280;  // 5 blocks of 4 wavefronts per CU
+constexpr int kSubLists = 16;
+constexpr int kCounterStride = 32;   // int32 words: 128 B
+constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
 
-__device__ inline int64_t adj_list(const AdjBatch& b, int which) {  // which 0: window, 1 + w: step w
-    return kAdjListHead + static_cast<int64_t>(which) * b.n_total;
+__device__ inline int64_t adj_counter(int which, int c) {
+    return static_cast<int64_t>(which * kSubLists + c) * kCounterStride;
+}
+__device__ inline int64_t adj_sublist(const AdjBatch& b, int which, int c) {
+    return kAdjListHead + static_cast<int64_t>(which * kSubLists + c) * b.sub_cap;
+}
+
+// entry i of list `which` over its sub-lists in order (i < the list's total)
+struct AdjList {
+    int cnt[kSubLists];
+    int total;
+};
+__device__ inline AdjList adj_list_counts(const AdjBatch& b, int which) {
+    AdjList l;
+    l.total = 0;
+#pragma unroll
+    for (int c = 0; c < kSubLists; ++c) {
+        l.cnt[c] = b.work[adj_counter(which, c)];
+        l.total += l.cnt[c];
+    }
+    return l;
+}
+__device__ inline uint32_t adj_list_entry(const AdjBatch& b, const AdjList& l, int which, int i) {
+    int k = i, sel = kSubLists - 1;
+    bool found = false;
+#pragma unroll
+    for (int c = 0; c < kSubLists; ++c) {
+        if (!found) {
+            if (k < l.cnt[c]) {
+                sel = c;
+                found = true;
+            } else {
+                k -= l.cnt[c];
+            }
+        }
+    }
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(b.work[adj_sublist(b, which, sel) + k]));
+}
+
+// one short link {target | (d - 1) << 17 | ...}, d < long_delay: the offset of gx_{t+d}[target] in the
+// short ring [long_delay][n] (pos = t mod long_delay; clamped into the ring whatever the link holds)
+__device__ inline int64_t adj_recent_slot(uint32_t x, int pos, int W, int n, uint32_t& tp) {
+    const uint32_t tgt = x & 0x1ffffu;
+    const int d = static_cast<int>((x >> 17) & 127u) + 1;
+    tp = x >> 24;
+    int slot = pos + d;
+    slot -= (slot >= W) ? W : 0;
+    slot = slot < W ? slot : W - 1;
+    return static_cast<int64_t>(slot) * n + tgt;
 }
 
 // lane per neuron: its window mask (steps where it is within the support or spiked), appended to the lists
@@ -132,15 +184,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
         }
     }
     const uint64_t below = (1ull << lane) - 1ull;
+    const int sub = static_cast<int>((blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kSubLists);
     for (int which = 0; which <= b.n_win; ++which) {
         const bool in = which == 0 ? mask != 0 : ((mask >> (which - 1)) & 1u) != 0;
         const uint64_t bal = __ballot(in);
         if (bal == 0) continue;  // uniform
         int first = 0;
-        if (lane == 0) first = atomicAdd(b.work + which, __popcll(bal));
+        if (lane == 0) first = atomicAdd(b.work + adj_counter(which, sub), __popcll(bal));
         first = __shfl(first, 0, kWave);
         const uint32_t ent = (which == 0 ? mask << 21 : 0u) | (static_cast<uint32_t>(q) << 17) | static_cast<uint32_t>(j);
-        if (in) b.work[adj_list(b, which) + first + __popcll(bal & below)] = static_cast<int32_t>(ent);
+        if (in) b.work[adj_sublist(b, which, sub) + first + __popcll(bal & below)] = static_cast<int32_t>(ent);
     }
 }
 
@@ -151,11 +204,10 @@ template <int W>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
     const int lane = threadIdx.x & (kWave - 1);
     const int n_waves = gridDim.x * kWavesPerBlock;
-    const int cnt = b.work[0];
-    const int32_t* list = b.work + adj_list(b, 0);
+    const AdjList list = adj_list_counts(b, 0);
     const int ww = lane >> 3, pp = lane & 7;  // lane ww * 8 + pp stores bins[pp][ww]
-    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < cnt; i += n_waves) {
-        const uint32_t ent = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(list[i]));
+    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
+        const uint32_t ent = adj_list_entry(b, list, 0, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const uint32_t mk = ent >> 21;
         const n2h_adj_net& a = b.net[q];
@@ -220,13 +272,12 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBatch b) {
     const int lane = threadIdx.x & (kWave - 1);
     const int n_waves = gridDim.x * kWavesPerBlock;
-    const int cnt = b.work[1 + b.win_step];
-    const int32_t* list = b.work + adj_list(b, 1 + b.win_step);
-    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < cnt; i += n_waves) {
-        const uint32_t ent = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(list[i]));
+    const AdjList list = adj_list_counts(b, 1 + b.win_step);
+    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
+        const uint32_t ent = adj_list_entry(b, list, 1 + b.win_step, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const n2h_adj_net& a = b.net[q];
-        const int R = a.ring_slots, RS = R + a.long_delay - 1;
+        const int W = a.long_delay, n = a.n_neurons;
         float bins[kAdjPops];
 #pragma unroll
         for (int p = 0; p < kAdjPops; ++p) bins[p] = 0.0f;
@@ -242,7 +293,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBa
             float g[kUnroll];
             uint32_t tp[kUnroll];
 #pragma unroll
-            for (int r = 0; r < kUnroll; ++r) g[r] = a.gx_ring[adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r])];
+            for (int r = 0; r < kUnroll; ++r) g[r] = a.gx_recent[adj_recent_slot(l[r].x, b.recent_pos, W, n, tp[r])];
 #pragma unroll
             for (int r = 0; r < kUnroll; ++r) {
                 const bool live = e + r * kWave < e1;
@@ -319,6 +370,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_update_kernel(AdjB
     float* row = a.gx_ring + static_cast<int64_t>(j) * RS;
     row[b.ring_pos] = gip;
     if (b.ring_pos < a.long_delay - 1) row[b.ring_pos + R] = gip;  // the mirror
+    a.gx_recent[static_cast<int64_t>(b.recent_pos) * a.n_neurons + j] = gip;
 }
 
 }  // namespace
@@ -355,7 +407,7 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
         if (a.long_delay < 1 || a.long_delay > kMaxWindow)
             return fail(N2H_EINVAL, "net %d: long_delay outside [1, %d]", i, kMaxWindow);
         if (!a.pop_first || !a.out_ptr || !a.out_links || !a.long_ptr || !a.long_links || !a.pop_coef || !a.alpha ||
-            !a.vtrace || !a.spikes || !a.gx_ring || !a.long_bins || !a.g_v || !a.g_i || !a.g_sum)
+            !a.vtrace || !a.spikes || !a.gx_ring || !a.gx_recent || !a.long_bins || !a.g_v || !a.g_i || !a.g_sum)
             return fail(N2H_EINVAL, "net %d: null pointer", i);
         if (ring >= 0 && ring != a.ring_slots) return fail(N2H_EINVAL, "every network needs the same ring_slots");
         if (win >= 0 && win != a.long_delay) return fail(N2H_EINVAL, "every network needs the same long_delay");
@@ -367,19 +419,23 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
     }
     b.first_block[n_nets] = static_cast<int32_t>(blocks);
     b.ring_pos = static_cast<int32_t>(step % ring);
+    b.recent_pos = static_cast<int32_t>(step % win);
+    b.sub_cap = static_cast<int32_t>(kWave * ((blocks * kWavesPerBlock + kSubLists - 1) / kSubLists));
     return N2H_OK;
 }
 
-// the work buffer holds the counters and 1 + long_delay lists of every neuron of the networks
-int adj_bind_work(AdjBatch& b, const n2h_adj_net* nets, int32_t n_nets, int32_t* work, int64_t work_words) {
-    int64_t total = 0;
-    for (int i = 0; i < n_nets; ++i) total += nets[i].n_neurons;
-    const int64_t need = kAdjListHead + (1 + static_cast<int64_t>(nets[0].long_delay)) * total;
+// the work buffer: the counters, then 1 + long_delay lists of kSubLists sub-lists of sub_cap entries
+int64_t adj_work_words(int64_t blocks, int32_t long_delay) {
+    const int64_t sub_cap = kWave * ((blocks * kWavesPerBlock + kSubLists - 1) / kSubLists);
+    return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * sub_cap;
+}
+
+int adj_bind_work(AdjBatch& b, int64_t blocks, int32_t long_delay, int32_t* work, int64_t work_words) {
+    const int64_t need = adj_work_words(blocks, long_delay);
     if (!work || work_words < need)
-        return fail(N2H_EINVAL, "work: %lld int32 words needed (16 + (1 + long_delay) * neurons), %lld given",
+        return fail(N2H_EINVAL, "work: %lld int32 words needed (n2h_adj_work_words), %lld given",
                     static_cast<long long>(need), static_cast<long long>(work_words));
     b.work = work;
-    b.n_total = static_cast<int32_t>(total);
     return N2H_OK;
 }
 
@@ -393,13 +449,23 @@ void launch_window(int64_t blocks, hipStream_t stream, const AdjBatch& b) {
 
 extern "C" {
 
+int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay) {
+    if (!n_neurons || n_nets < 1 || n_nets > kAdjNets || long_delay < 1 || long_delay > kMaxWindow) return -1;
+    int64_t blocks = 0;
+    for (int i = 0; i < n_nets; ++i) {
+        if (n_neurons[i] < 1) return -1;
+        blocks += (n_neurons[i] + kWave * kWavesPerBlock - 1) / (kWave * kWavesPerBlock);
+    }
+    return adj_work_words(blocks, long_delay);
+}
+
 int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t win_step, const n2h_surrogate* sur,
                  int32_t* work, int64_t work_words, void* stream) {
     AdjBatch b;
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, blocks, nets[0].long_delay, work, work_words)) != N2H_OK) return rc;
     if (win_step < 0 || win_step >= nets[0].long_delay)
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
@@ -417,12 +483,12 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step0, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, blocks, nets[0].long_delay, work, work_words)) != N2H_OK) return rc;
     const int W = nets[0].long_delay;
     if (n_win < 1 || n_win > W) return fail(N2H_EINVAL, "n_win %d outside [1, long_delay = %d]", n_win, W);
     b.n_win = n_win;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = hipMemsetAsync(work, 0, kAdjListHead * sizeof(int32_t), s);  // the list counters
+    hipError_t e = hipMemsetAsync(work, 0, kAdjListHead * sizeof(int32_t), s);  // the counters
     if (e != hipSuccess) return fail(N2H_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
     switch (W) {
         case 1: launch_window<1>(blocks, s, b); break;

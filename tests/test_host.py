@@ -24,7 +24,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def test_header_declares_exactly_the_exports():
     text = (ROOT / "include" / "naer2_hip.h").read_text()
-    declared = set(re.findall(r"^int\s+(n2h_\w+)\(", text, flags=re.M))
+    declared = set(re.findall(r"^(?:int|int64_t)\s+(n2h_\w+)\(", text, flags=re.M))
     assert declared == set(_lib.EXPORTS)
     lib = _lib.load()
     for name in declared:
