@@ -254,7 +254,7 @@ def bench_pd14grad(args) -> int:
     print(f"[bench] built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s",
           file=sys.stderr, flush=True)
     sur = {"cutoff": args.cutoff, "gamma": args.gamma}
-    grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, **sur)
+    grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, long_parts=args.long_parts, **sur)
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
@@ -289,7 +289,7 @@ def bench_pd14grad(args) -> int:
         "forward_ms": round(fw, 1), "backward_ms": round(bw, 1),
         "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
         "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
-        "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay,
+        "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay, "long_parts": args.long_parts,
         "grad_rel_err_vs_fp64_subsample": err,
         "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
     }
@@ -633,6 +633,8 @@ def main() -> int:
     ap.add_argument("--long-delay", type=int, default=4,
                     help="pd14grad: out-links with at least this delay (steps) are gathered once per window of "
                          "that many reverse steps")
+    ap.add_argument("--long-parts", type=int, default=8,
+                    help="pd14grad: target ranges the long out-links are walked in, one launch each")
     ap.add_argument("--cutoff", type=float, default=0.2,
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,

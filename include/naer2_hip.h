@@ -215,7 +215,8 @@ int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base,
  * reference, which is forward-only: a surrogate-gradient adjoint beside the plain path).
  * One n2h_adj_step call runs reverse step `step` for up to 16 networks in one launch.  Per network: its
  * out-links (canonical CSR by source) split by delay into a short list (delay < long_delay steps) and a long
- * list (delay >= long_delay), each kept in canonical order and packed {target | (delay_steps - 1) << 17 |
+ * list (delay >= long_delay), the long list further split by target range (range k: targets t with
+ * t * long_parts / n == k), each run kept in canonical order and packed {target | (delay_steps - 1) << 17 |
  * target population << 24, base weight bits}; alpha[target pop][source pop] the weight scales the forward
  * ran with (its weights are alpha * base); pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset,
  * dL/ds of one spike of population p, 0}; this step's potentials before the threshold (n2h_set_vtrace) and
@@ -231,10 +232,12 @@ typedef struct n2h_adj_net {
     int32_t n_pops;              /* <= 8 */
     int32_t ring_slots;
     int32_t long_delay;          /* 1 .. 8: the window length and the long-list delay bound */
+    int32_t long_parts;          /* 1 .. 64: target ranges of the long list, one n2h_adj_window launch each */
+    int32_t reserved;
     const int32_t* pop_first;    /* device [n_pops + 1] */
     const int64_t* out_ptr;      /* device [n + 1], short list */
     const uint32_t* out_links;   /* device [2 * n_short] */
-    const int64_t* long_ptr;     /* device [n + 1], long list */
+    const int64_t* long_ptr;     /* device [n * long_parts + 1], long list by (source, target range) */
     const uint32_t* long_links;  /* device [2 * n_long] */
     const float* alpha;          /* device [n_pops][n_pops] */
     const float* pop_coef;       /* device [n_pops][8] */

@@ -29,11 +29,13 @@ import torch
 from . import _lib
 from ._lib import MODEL_LIF_EXP
 from .fusion import FusedRound
+from .topology import _stable_order_by_key
 
 
 class AdjNet(ctypes.Structure):
     _fields_ = [("n_neurons", ctypes.c_int32), ("n_pops", ctypes.c_int32), ("ring_slots", ctypes.c_int32),
-                ("long_delay", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in (
+                ("long_delay", ctypes.c_int32), ("long_parts", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+                    (k, ctypes.c_void_p) for k in (
                     "pop_first", "out_ptr", "out_links", "long_ptr", "long_links", "alpha", "pop_coef", "vtrace",
                     "spikes", "gx_ring", "gx_recent", "long_bins", "g_v", "g_i", "g_sum")]
 
@@ -66,11 +68,12 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
 
 
-def out_links(rt, alpha: torch.Tensor, long_delay: int = 1):
+def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1):
     """(short_ptr, short_links, long_ptr, long_links): rt's canonical CSR by source packed for the adjoint,
     {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight / alpha[pt][ps],
-    split into the links with delay < long_delay and the rest (ptr int64 [n + 1], links int32 [E, 2]), each
-    in canonical order."""
+    split into the links with delay < long_delay (ptr int64 [n + 1]) and the rest, these by (source, target
+    range k = target * long_parts // n) (ptr int64 [n * long_parts + 1]); links int32 [E, 2], each run in
+    canonical order."""
     c = rt.canon
     if c is None:
         raise RuntimeError("the adjoint needs the canonical synapse table (Network.keep_canonical = True)")
@@ -88,11 +91,16 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1):
     links = torch.stack([x.to(torch.int32), base.contiguous().view(torch.int32)], dim=1).contiguous()
     is_long = c.d.to(dev) >= long_delay
     out = []
-    for keep in (~is_long, is_long):
-        cnt = torch.bincount(src[keep], minlength=rt.n)
-        ptr = torch.zeros(rt.n + 1, dtype=torch.int64, device=dev)
+    for keep, parts in ((~is_long, 1), (is_long, long_parts)):
+        key = src[keep] * parts + (tgt[keep] * parts) // rt.n
+        cnt = torch.bincount(key, minlength=rt.n * parts)
+        ptr = torch.zeros(rt.n * parts + 1, dtype=torch.int64, device=dev)
         ptr[1:] = cnt.cumsum(0)
-        sel = links[keep] if bool(keep.any()) else torch.zeros(1, 2, dtype=torch.int32, device=dev)  # never empty
+        if bool(keep.any()):
+            sel = torch.empty_like(links[keep])
+            sel[_stable_order_by_key(key, rt.n * parts)] = links[keep]  # canonical order within each run
+        else:
+            sel = torch.zeros(1, 2, dtype=torch.int32, device=dev)  # never empty
         out += [ptr, sel.contiguous()]
     return tuple(out)
 
@@ -100,7 +108,7 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1):
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
                  cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
-                 long_delay: int = 4):
+                 long_delay: int = 4, long_parts: int = 8):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -120,6 +128,8 @@ class RateLossGrad:
             raise ValueError("checkpoint must be positive")
         if not 1 <= long_delay <= 8:
             raise ValueError("long_delay: 1 to 8 steps")
+        if not 1 <= long_parts <= 64:
+            raise ValueError("long_parts: 1 to 64 target ranges")
         self.runtimes = list(runtimes)
         self.dev = runtimes[0].device
         P = len(runtimes[0].pops)
@@ -130,7 +140,8 @@ class RateLossGrad:
         self.checkpoint, self.window, self.tuning = int(checkpoint), int(graph_window), tuning
         self.sur = Surrogate(float(kappa), float(cutoff), float(gamma), 0.0)
         self.long_delay = int(long_delay)
-        self.tables = [out_links(rt, self.alpha, self.long_delay) for rt in runtimes]
+        self.long_parts = int(long_parts)
+        self.tables = [out_links(rt, self.alpha, self.long_delay, self.long_parts) for rt in runtimes]
         self.pop_first = [torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int32), device=self.dev) for rt in runtimes]
         self.ring_slots = max(rt.d_slots for rt in runtimes) + 1
         if any(int(rt.canon.d.max()) >= self.ring_slots for rt in runtimes if rt.canon.d.numel()):
@@ -195,7 +206,7 @@ class RateLossGrad:
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         nets = (AdjNet * len(rts))()
         for i, (rt, sd, (sptr, sl, lptr, ll)) in enumerate(zip(rts, st, self.tables)):
-            nets[i] = AdjNet(rt.n, self.P, R, W, self.pop_first[i].data_ptr(), sptr.data_ptr(), sl.data_ptr(),
+            nets[i] = AdjNet(rt.n, self.P, R, W, self.long_parts, 0, self.pop_first[i].data_ptr(), sptr.data_ptr(), sl.data_ptr(),
                              lptr.data_ptr(), ll.data_ptr(), self.alpha.data_ptr(), sd["coef"].data_ptr(), 0, 0,
                              sd["gx"].data_ptr(), sd["gr"].data_ptr(), 0, sd["gv"].data_ptr(), sd["gi"].data_ptr(),
                              sd["gsum"].data_ptr())

@@ -24,7 +24,8 @@
 // Windows.  The out-links are split by delay: short (d < W) and long (d >= W), W = long_delay.  For a window
 // of reverse steps [t0, t0 + W) every long link's input adjoints gx_{t0+w+d} (w < W) lie past the window, so
 // adj_long_kernel reads each gathering source's long links once per window and gathers a link's W values
-// in one contiguous read: the ring is target-major, gx[k][slot], step s in slot s mod R, with the first
+// in one contiguous read (the long links are walked in long_parts target ranges, one launch each, so that a
+// launch's gathers fit the L2s; the ranges' sums are added in order): the ring is target-major, gx[k][slot], step s in slot s mod R, with the first
 // W - 1 slots mirrored past slot R - 1 so that W consecutive steps never wrap.  Each step, adj_short_kernel
 // walks only the short links and adds the window's long sums (bins[a] = short sum + long sum), then
 // adj_update_kernel updates every neuron.  The split depends on the link alone, so the sums do not depend on
@@ -46,6 +47,7 @@ constexpr int kAdjPops = 8;      // target-population bins per lane
 constexpr int kAdjNets = 16;     // networks in one launch
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaxWindow = 8;
+constexpr int kMaxParts = 64;      // target ranges of the long links
 constexpr int kUnroll = 4;        // links a lane loads per pass of a gather loop
 
 thread_local char g_err[512] = "";
@@ -67,6 +69,7 @@ struct AdjBatch {
     int32_t win_step;  // step: its place in the window
     int32_t recent_pos;  // step mod long_delay: this step's slot of the short ring
     int32_t sub_cap;   // entries a work sub-list holds
+    int32_t part;      // long kernel: the target range of this launch
     int32_t* work;     // counters and work lists
     n2h_surrogate sur;
 };
@@ -98,6 +101,39 @@ __device__ inline int64_t adj_link_slot(uint32_t x, int pos, int R, int RS, uint
     int slot = pos + d;
     slot -= (slot >= R) ? R : 0;
     return static_cast<int64_t>(tgt) * RS + slot;
+}
+
+// Sums v[0..V) (V a power of 2, 8 to 64) over the 64 lanes of the wavefront by recursive halving, a fixed
+// tree: at the stage of lane offset o, while more than one element is left, a lane keeps one half of its
+// elements (the upper half if its lane bit o is set) and adds its partner's copy of that half.  Lane l ends
+// with element l >> (6 - log2 V) in v[0]; 2 V - 2 + log2 (64 / V) lane exchanges instead of a butterfly's 6 V.
+template <int L, int O, int V>
+__device__ inline void adj_halve(float (&v)[V], int lane) {  // L elements left, lane offset O
+    if constexpr (O > 0) {
+        if constexpr (L > 1) {
+            constexpr int H = L / 2;
+            const bool upper = (lane & O) != 0;
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                const float send = upper ? v[i] : v[i + H];
+                const float keep = upper ? v[i + H] : v[i];
+                v[i] = keep + __shfl_xor(send, O, kWave);
+            }
+            adj_halve<H, O / 2, V>(v, lane);
+        } else {
+            v[0] += __shfl_xor(v[0], O, kWave);
+            adj_halve<1, O / 2, V>(v, lane);
+        }
+    }
+}
+template <int V>
+__device__ inline float adj_reduce_halving(float (&v)[V], int lane) {
+    adj_halve<V, kWave / 2, V>(v, lane);
+    return v[0];
+}
+template <int V>
+constexpr int adj_halving_shift() {
+    return V == 64 ? 0 : V == 32 ? 1 : V == 16 ? 2 : 3;
 }
 
 // Work lists.  Few neurons gather (a few percent a step), and they cluster in the more active populations,
@@ -202,22 +238,26 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
 // butterfly; long_bins rows of the steps in the entry's mask.
 template <int W>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
+    constexpr int WP = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;  // window steps padded to a power of 2
+    constexpr int V = kAdjPops * WP;
+    constexpr int SH = adj_halving_shift<V>();
     const int lane = threadIdx.x & (kWave - 1);
     const int n_waves = gridDim.x * kWavesPerBlock;
     const AdjList list = adj_list_counts(b, 0);
-    const int ww = lane >> 3, pp = lane & 7;  // lane ww * 8 + pp stores bins[pp][ww]
+    const int idx = lane >> SH;                    // the element this lane ends with: w * 8 + p
+    const int ww = idx >> 3, pp = idx & 7;
+    const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
     for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
         const uint32_t ent = adj_list_entry(b, list, 0, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const uint32_t mk = ent >> 21;
         const n2h_adj_net& a = b.net[q];
         const int n = a.n_neurons, R = a.ring_slots, RS = R + W - 1;
-        float bins[kAdjPops][W];
+        float bins[V];
 #pragma unroll
-        for (int p = 0; p < kAdjPops; ++p)
-#pragma unroll
-            for (int w = 0; w < W; ++w) bins[p][w] = 0.0f;
-        const int64_t e0 = a.long_ptr[jj], e1 = a.long_ptr[jj + 1];
+        for (int k = 0; k < V; ++k) bins[k] = 0.0f;
+        const int64_t seg = static_cast<int64_t>(jj) * a.long_parts + b.part;
+        const int64_t e0 = a.long_ptr[seg], e1 = a.long_ptr[seg + 1];
         // kUnroll links a lane per pass, every load issued before the sums; the lane's links are added in
         // list order (e, e + 64, ...) as in a plain loop
         const uint2* links = reinterpret_cast<const uint2*>(a.long_links);
@@ -245,25 +285,17 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
                     const float c = wt * gw[r][w];
 #pragma unroll
                     for (int p = 0; p < kAdjPops; ++p)
-                        bins[p][w] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[p][w] + c : bins[p][w];
+                        bins[w * kAdjPops + p] =
+                            (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c : bins[w * kAdjPops + p];
                 }
             }
         }
-        if (e1 > e0) {  // fixed butterfly: every lane ends with every sum
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-#pragma unroll
-                for (int p = 0; p < kAdjPops; ++p)
-#pragma unroll
-                    for (int w = 0; w < W; ++w) bins[p][w] += __shfl_xor(bins[p][w], off, kWave);
-            }
+        const float v = (e1 > e0) ? adj_reduce_halving<V>(bins, lane) : 0.0f;  // uniform branch
+        // the rows of the steps in the mask: target ranges added in launch order (the first one stores)
+        if (writer && ((mk >> ww) & 1u)) {
+            float* row = a.long_bins + (static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp;
+            *row = b.part == 0 ? v : *row + v;
         }
-        float v = 0.0f;
-#pragma unroll
-        for (int p = 0; p < kAdjPops; ++p)
-#pragma unroll
-            for (int w = 0; w < W; ++w) v = (ww == w && pp == p) ? bins[p][w] : v;
-        if (lane < W * kAdjPops && ((mk >> ww) & 1u)) a.long_bins[(static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp] = v;
     }
 }
 
@@ -303,19 +335,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBa
                     bins[p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[p] + c : bins[p];
             }
         }
-        if (e1 > e0) {  // fixed butterfly: every lane ends with the same sums, in the same order on every run
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-#pragma unroll
-                for (int p = 0; p < kAdjPops; ++p) bins[p] += __shfl_xor(bins[p], off, kWave);
-            }
-        }
-        if (lane < kAdjPops) {
-            float v = 0.0f;
-#pragma unroll
-            for (int p = 0; p < kAdjPops; ++p) v = (lane == p) ? bins[p] : v;
-            float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops;
-            row[lane] = v + row[lane];
+        const float v = (e1 > e0) ? adj_reduce_halving<kAdjPops>(bins, lane) : 0.0f;  // lane l: population l >> 3
+        if ((lane & 7) == 0) {
+            float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops + (lane >> 3);
+            *row = v + *row;
         }
     }
 }
@@ -406,11 +429,14 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
         if (a.ring_slots < 2 || a.ring_slots > N2H_MAX_SLOTS + 1) return fail(N2H_EINVAL, "net %d: ring_slots", i);
         if (a.long_delay < 1 || a.long_delay > kMaxWindow)
             return fail(N2H_EINVAL, "net %d: long_delay outside [1, %d]", i, kMaxWindow);
+        if (a.long_parts < 1 || a.long_parts > kMaxParts)
+            return fail(N2H_EINVAL, "net %d: long_parts outside [1, %d]", i, kMaxParts);
         if (!a.pop_first || !a.out_ptr || !a.out_links || !a.long_ptr || !a.long_links || !a.pop_coef || !a.alpha ||
             !a.vtrace || !a.spikes || !a.gx_ring || !a.gx_recent || !a.long_bins || !a.g_v || !a.g_i || !a.g_sum)
             return fail(N2H_EINVAL, "net %d: null pointer", i);
         if (ring >= 0 && ring != a.ring_slots) return fail(N2H_EINVAL, "every network needs the same ring_slots");
         if (win >= 0 && win != a.long_delay) return fail(N2H_EINVAL, "every network needs the same long_delay");
+        if (i > 0 && a.long_parts != nets[0].long_parts) return fail(N2H_EINVAL, "every network needs the same long_parts");
         ring = a.ring_slots;
         win = a.long_delay;
         b.net[i] = a;
@@ -440,9 +466,12 @@ int adj_bind_work(AdjBatch& b, int64_t blocks, int32_t long_delay, int32_t* work
 }
 
 template <int W>
-void launch_window(int64_t blocks, hipStream_t stream, const AdjBatch& b) {
+void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
     hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, stream, b);
-    hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
+    for (int part = 0; part < b.net[0].long_parts; ++part) {  // one target range a launch
+        b.part = part;
+        hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
+    }
 }
 
 }  // namespace

@@ -109,21 +109,22 @@ def test_gradient_is_independent_of_checkpoints_and_batching():
 
 
 def test_gradient_agrees_over_window_lengths():
-    """The long-link windows only regroup each source's sum (short links + long links): every window length,
-    with checkpoints that are not a multiple of it, gives the single-step gradient to fp32 rounding."""
+    """The long-link windows and target ranges only regroup each source's sum (short links + long links, the
+    long ones range by range): every window length and range count, with checkpoints that are not a multiple
+    of the window, gives the single-step gradient to fp32 rounding."""
     T = 120
     alpha = _alpha(5)
     targets = np.random.default_rng(2).uniform(0.5, 10.0, (2, 8))
     sur = {"cutoff": 0.5, "gamma": 0.5}
     res = {}
-    for W in (1, 3, 4, 8):
-        _say(f"case: long_delay {W}, checkpoint 50, T={T}")
+    for W, P in ((1, 1), (3, 5), (4, 8), (8, 16), (4, 1)):
+        _say(f"case: long_delay {W}, long_parts {P}, checkpoint 50, T={T}")
         rts = [n.build() for n in _nets(alpha)]
-        res[W] = RateLossGrad(rts, alpha, checkpoint=50, long_delay=W, **sur)(T, targets)
-    g1 = res[1][1]
+        res[W, P] = RateLossGrad(rts, alpha, checkpoint=50, long_delay=W, long_parts=P, **sur)(T, targets)
+    g1 = res[1, 1][1]
     assert np.abs(g1).max() > 0
-    for W in (3, 4, 8):
-        assert res[W][0] == res[1][0]
-        err = np.linalg.norm(res[W][1] - g1) / np.linalg.norm(g1)
-        _say(f"long_delay {W}: rel diff {err:.3g}")
+    for key in ((3, 5), (4, 8), (8, 16), (4, 1)):
+        assert res[key][0] == res[1, 1][0]
+        err = np.linalg.norm(res[key][1] - g1) / np.linalg.norm(g1)
+        _say(f"long_delay, long_parts {key}: rel diff {err:.3g}")
         assert err <= 1e-5
