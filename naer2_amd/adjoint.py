@@ -231,16 +231,17 @@ class RateLossGrad:
                 gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
                 sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
             # windows of W steps from the segment's end: [t0, t1)
+            # row addresses by arithmetic on the base pointers (a tensor view per row costs microseconds of host
+            # time a step, as much as the step's kernels): vtrace fp32 [K][n], spikes u8 [K][n], long_bins [W][n][8]
+            base = [(sd["vtr"].data_ptr(), sd["spk"].data_ptr(), sd["lb"].data_ptr(), rt.n) for sd, rt in zip(st, rts)]
             for t1 in range(L, 0, -W):
                 t0 = max(0, t1 - W)
-                for i, sd in enumerate(st):
-                    nets[i].vtrace, nets[i].spikes = sd["vtr"][t0].data_ptr(), sd["spk"][t0].data_ptr()
-                    nets[i].long_bins = sd["lb"].data_ptr()
+                for i, (vb, sb, lb, n) in enumerate(base):
+                    nets[i].vtrace, nets[i].spikes, nets[i].long_bins = vb + 4 * t0 * n, sb + t0 * n, lb
                 _check(lib.n2h_adj_window(nets, len(rts), start + t0, t1 - t0, sur, wp, wn, stream), "n2h_adj_window")
                 for t in reversed(range(t0, t1)):
-                    for i, sd in enumerate(st):
-                        nets[i].vtrace, nets[i].spikes = sd["vtr"][t].data_ptr(), sd["spk"][t].data_ptr()
-                        nets[i].long_bins = sd["lb"][t - t0].data_ptr()
+                    for i, (vb, sb, lb, n) in enumerate(base):
+                        nets[i].vtrace, nets[i].spikes, nets[i].long_bins = vb + 4 * t * n, sb + t * n, lb + 32 * (t - t0) * n
                     _check(lib.n2h_adj_step(nets, len(rts), start + t, t - t0, sur, wp, wn, stream), "n2h_adj_step")
         t_bw[1].record()
         for rt, sn, rf in zip(rts, final, rec_from):
