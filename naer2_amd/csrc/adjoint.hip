@@ -144,7 +144,8 @@ constexpr int adj_halving_shift() {
 // in a fixed order, so results do not.  Each list (which = 0: the window's, 1 + w: step w's) is kept as
 // kSubLists sub-lists, a wavefront appending to sub-list (its index mod kSubLists), each counter on a cache
 // line of its own: appends to one counter serialize in the L2.
-constexpr int kGatherBlocks = 1280;  // 5 blocks of 4 wavefronts per CU
+constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
+constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSubLists = 16;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
@@ -470,7 +471,7 @@ void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
     hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, stream, b);
     for (int part = 0; part < b.net[0].long_parts; ++part) {  // one target range a launch
         b.part = part;
-        hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
+        hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kLongBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
     }
 }
 
@@ -499,7 +500,7 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(adj_short_kernel, dim3(kGatherBlocks), dim3(kWave * kWavesPerBlock), 0, s, b);
+    hipLaunchKernelGGL(adj_short_kernel, dim3(kShortBlocks), dim3(kWave * kWavesPerBlock), 0, s, b);
     hipLaunchKernelGGL(adj_update_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, s, b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(N2H_EHIP, "adjoint step kernels: %s", hipGetErrorString(e));
