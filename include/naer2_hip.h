@@ -164,7 +164,8 @@ int n2h_graph_create(n2h_plan* plan, int32_t n_steps, void* stream, n2h_graph** 
 int n2h_graph_launch(n2h_graph* graph, void* stream);
 int n2h_graph_destroy(n2h_graph* graph);
 
-/* A batch: the groups of up to 256 independent plans (same model, same step) in one launch per step.
+/* A batch: the groups of up to 256 independent plans (at the same step) in one launch per step.  Plans of
+ * both models may share a batch (a heterogeneous round: each group runs its own network's model).
  * Replaces naer2/fusion.py:31-141 (FusedRound over several runtimes).  The plans' bound buffers must stay
  * in place while the batch exists; a plan stepped on its own leaves the batch unusable (N2H_ESTATE). */
 int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* tuning, n2h_batch** out);

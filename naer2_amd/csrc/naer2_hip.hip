@@ -113,6 +113,7 @@ struct StepArgs {
     int64_t vtrace_step0;       //   steps vtrace_step0 .. + vtrace_steps - 1 (+inf on refractory steps)
     int32_t vtrace_steps;
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
+    int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
 };
 
 // Inclusive prefix sum over the 64 lanes of the wavefront, in DPP lane moves (no LDS round trip): a
@@ -816,6 +817,11 @@ struct BatchArgs {
     int32_t first[kMaxBatch + 1];  // first tile of each plan
 };
 
+// MODEL = kModelMixed: a batch of networks of both models (a heterogeneous fused round, naer2/fusion.py:61-116
+// over runtimes of different node types); each tile takes its own network's model, a branch uniform over
+// the workgroup.
+constexpr int kModelMixed = -1;
+
 template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
     __shared__ Stage<bm_words(KW), kBatchCapS, kBatchCapL, kBatchRowU> st;
@@ -837,7 +843,12 @@ __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
     a.step_ctr = ba.step_ctr;
     a.step_imm = ba.step_imm;
     a.phase = ba.phase;
-    step_group<MODEL, KW>(a, tile - ba.first[lo], k, st, acc);
+    if constexpr (MODEL == kModelMixed) {
+        if (a.model == N2H_MODEL_LIF_DELTA) step_group<N2H_MODEL_LIF_DELTA, KW>(a, tile - ba.first[lo], k, st, acc);
+        else step_group<N2H_MODEL_LIF_EXP, KW>(a, tile - ba.first[lo], k, st, acc);
+    } else {
+        step_group<MODEL, KW>(a, tile - ba.first[lo], k, st, acc);
+    }
 }
 
 __global__ void advance_kernel(int64_t* ctr, int32_t n) { *ctr += n; }
@@ -911,6 +922,7 @@ StepArgs make_args(const n2h_plan* p) {
     StepArgs a;
     memset(&a, 0, sizeof(a));
     a.n_neurons = p->desc.n_neurons;
+    a.model = p->desc.model;
     a.n_groups = p->tables.n_groups;
     a.d_slots = p->desc.d_slots;
     a.instance = p->desc.instance;
@@ -1046,7 +1058,8 @@ int launch_batch(n2h_batch* bt, int32_t n_steps, hipStream_t stream, bool graph)
     const dim3 grid(static_cast<unsigned>(blocks));
     for (int32_t k = 0; k < n_steps; ++k) {
         if (bt->model == N2H_MODEL_LIF_DELTA) launch_batch_model<N2H_MODEL_LIF_DELTA>(bt, ba, grid, stream, k);
-        else launch_batch_model<N2H_MODEL_LIF_EXP>(bt, ba, grid, stream, k);
+        else if (bt->model == N2H_MODEL_LIF_EXP) launch_batch_model<N2H_MODEL_LIF_EXP>(bt, ba, grid, stream, k);
+        else launch_batch_model<kModelMixed>(bt, ba, grid, stream, k);
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, bt->step_ctr, n_steps);
     hipLaunchKernelGGL(advance_many_kernel, dim3((bt->n_plans + 63) / 64), dim3(64), 0, stream, bt->ctrs_dev,
@@ -1250,7 +1263,6 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     for (int32_t i = 0; i < n_plans; ++i) {
         int rc = check_ready(plans[i]);
         if (rc) return rc;
-        if (plans[i]->desc.model != plans[0]->desc.model) return fail(N2H_EINVAL, "plans of different models");
         if (plans[i]->host_step != plans[0]->host_step) return fail(N2H_EINVAL, "plans at different steps");
         for (int32_t j = 0; j < i; ++j)
             if (plans[j] == plans[i]) return fail(N2H_EINVAL, "plan %d listed twice", i);
@@ -1273,6 +1285,8 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     }
     bt->n_plans = n_plans;
     bt->model = plans[0]->desc.model;
+    for (int32_t i = 1; i < n_plans; ++i)
+        if (plans[i]->desc.model != bt->model) bt->model = kModelMixed;  // both models: the mixed kernel
     bt->tuning = tu;
     bt->host_step = plans[0]->host_step;
     int max_groups = 0;

@@ -137,3 +137,43 @@ def test_batch_of_large_networks_equals_plain_runs():
         _same(a, b)
     counts = np.concatenate([n.runtime.raster().sum(axis=1) for n in plain])
     assert (counts <= 64).any() and (counts > 64).any()  # the sparse path and a bitmap or row intake
+
+
+def test_batch_of_both_models_equals_plain_runs():
+    """A heterogeneous round (naer2/fusion.py:61-116 over runtimes of different node types): delta-synapse
+    Brunel networks and exponential-synapse PD14 microcircuits interleaved in one launch, each group running
+    its own network's model, give the same rasters, potentials, currents, refractory counters and rings as
+    each network alone, under two launch mappings."""
+    from naer2_amd._lib import MODEL_LIF_EXP
+    from naer2_amd.networks import pd14
+
+    params = sweep_params(n_instances=3, seed=5, n_min=500, n_max=4000)
+
+    def build():
+        rts = [n.runtime for n in brunel_sweep(params, device="cuda", record=True)]
+        for i, scale in enumerate((0.02, 0.03)):
+            net, _ = pd14(scale=scale, seed=7 + i, device="cuda", instance=i)
+            net.record = True
+            rts.append(net.build())
+        return [rts[i] for i in (0, 3, 1, 4, 2)]  # the models interleaved in batch order
+
+    batched, plain = build(), build()
+    assert {rt.model for rt in batched} == {MODEL_LIF_EXP, batched[0].model} and batched[0].model != MODEL_LIF_EXP
+    _say(f"case: mixed batch, sizes {[rt.n for rt in batched]}")
+    for cand in ({"order": 0, "xcd": 0}, {"order": 1, "xcd": 8}):
+        _say(f"case: mapping {cand}")
+        fr = FusedRound(batched, cand, reserve_steps=96)
+        fr.launch(96, graph_window=48)
+        fr.close()
+        for rt in plain:
+            rt.run(96, 0)
+        torch.cuda.synchronize()
+        for a, b in zip(batched, plain):
+            assert a.step == b.step
+            assert np.array_equal(a.raster(), b.raster())
+            assert bits_equal(a.v.cpu().numpy(), b.v.cpu().numpy())
+            assert np.array_equal(a.ref.cpu().numpy(), b.ref.cpu().numpy())
+            assert bits_equal(a.ring.cpu().numpy(), b.ring.cpu().numpy())
+            if a.model == MODEL_LIF_EXP:
+                assert bits_equal(a.isyn.cpu().numpy(), b.isyn.cpu().numpy())
+    assert all(int(rt.raster().sum()) > 0 for rt in plain)
