@@ -1,0 +1,105 @@
+"""Host side of the rate-loss adjoint (naer2_amd/adjoint.py, n2h_adj_*) on the CPU: the out-link tables the
+windowed reverse step walks, decoded link by link against the canonical CSR they come from, and the C-ABI's
+work sizing and argument errors (calls that fail validation before any GPU work)."""
+from __future__ import annotations
+
+import ctypes
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+from naer2_amd import _lib
+from naer2_amd.adjoint import AdjNet, Surrogate, _bind, out_links
+from naer2_amd.topology import canonical
+
+N2H_EINVAL = -1  # include/naer2_hip.h: a bad argument
+
+
+def _fake_runtime(seed=4, n=70, e=900, d_max=12):
+    """A canonical CSR of e links given in a shuffled source order (so canonical order within a source is
+    not target order), 3 populations, delays 1..d_max."""
+    g = torch.Generator().manual_seed(seed)
+    src = torch.randint(0, n, (e,), generator=g)
+    tgt = torch.randint(0, n, (e,), generator=g)
+    w = torch.randn(e, generator=g) * 10.0
+    d = torch.randint(1, d_max + 1, (e,), generator=g)
+    pop_first = [0, 25, 50, n]
+    return SimpleNamespace(canon=canonical(n, src, tgt, w, d), device="cpu", n=n, pop_first=pop_first)
+
+
+def _decode(links):
+    x = links[:, 0].numpy().astype(np.int64) & 0xFFFFFFFF
+    return x & 0x1FFFF, ((x >> 17) & 127) + 1, x >> 24, links[:, 1].contiguous().view(torch.float32).numpy()
+
+
+@pytest.mark.parametrize("long_delay,parts", [(1, 1), (4, 3), (4, 8), (13, 2)])
+def test_out_link_tables_split_every_link_once_in_canonical_order(long_delay, parts):
+    rt = _fake_runtime()
+    c = rt.canon
+    alpha = torch.tensor(np.random.default_rng(1).uniform(0.5, 2.0, (3, 3)), dtype=torch.float32)
+    sptr, sl, lptr, ll = out_links(rt, alpha, long_delay, parts)
+    n = rt.n
+    assert sptr.shape == (n + 1,) and lptr.shape == (n * parts + 1,)
+    pop = np.repeat(np.arange(3), np.diff(rt.pop_first))
+    ptr, tgt, dl, w = c.src_ptr.numpy(), c.tgt.numpy(), c.d.numpy(), c.w.numpy()
+    seen = np.zeros(c.n_links, dtype=int)
+    tables = ((sptr.numpy(), sl, 1, lambda dd: dd < long_delay), (lptr.numpy(), ll, parts, lambda dd: dd >= long_delay))
+    for tptr, links, P, keep in tables:
+        if tptr[-1] == 0:
+            continue  # an empty table (one padding row)
+        t_tgt, t_d, t_pop, t_w = _decode(links)
+        for j in range(n):
+            canon_idx = np.arange(ptr[j], ptr[j + 1])
+            mine = canon_idx[keep(dl[canon_idx])]
+            got = []
+            for k in range(P):
+                run = range(tptr[j * P + k], tptr[j * P + k + 1])
+                # the run: source j's links into target range k, in canonical order
+                want = mine[(tgt[mine] * P) // n == k]
+                assert len(run) == len(want)
+                for e, ci in zip(run, want):
+                    assert t_tgt[e] == tgt[ci] and t_d[e] == dl[ci] and t_pop[e] == pop[tgt[ci]]
+                    base = np.float32(w[ci]) / np.float32(alpha[pop[tgt[ci]], pop[j]].item())
+                    assert t_w[e] == base  # the base weight, bit for bit
+                    seen[ci] += 1
+                got += list(want)
+            assert got == sorted(got) or P > 1  # one run: canonical order throughout
+    assert (seen == 1).all()  # every link in exactly one table, once
+
+
+def test_work_words_follow_the_list_layout():
+    lib = _bind(_lib.load())
+    sizes = [77169, 77169, 77169, 77169]
+    arr = (ctypes.c_int32 * 4)(*sizes)
+    blocks = sum((s + 255) // 256 for s in sizes)
+    sub_cap = 64 * ((blocks * 4 + 15) // 16)
+    assert lib.n2h_adj_work_words(arr, 4, 4) == 9 * 16 * 32 + 5 * 16 * sub_cap
+    assert lib.n2h_adj_work_words(arr, 4, 8) == 9 * 16 * 32 + 9 * 16 * sub_cap
+    assert lib.n2h_adj_work_words(arr, 0, 4) == -1
+    assert lib.n2h_adj_work_words(arr, 4, 9) == -1
+    bad = (ctypes.c_int32 * 1)(0)
+    assert lib.n2h_adj_work_words(bad, 1, 4) == -1
+
+
+def test_adjoint_calls_refuse_bad_arguments_before_any_launch():
+    lib = _bind(_lib.load())
+    sur = Surrogate(25.0, 0.2, 0.01, 0.0)
+    nets = (AdjNet * 1)()
+    nets[0] = AdjNet(100, 8, 40, 4, 8, 0)  # every pointer null
+    buf = ctypes.create_string_buffer(512)
+    assert lib.n2h_adj_window(nets, 1, 0, 4, ctypes.byref(sur), None, 0, None) == N2H_EINVAL
+    lib.n2h_adj_last_error(buf, len(buf))
+    assert b"null pointer" in buf.value
+    nets[0].long_delay = 9
+    assert lib.n2h_adj_step(nets, 1, 0, 0, ctypes.byref(sur), None, 0, None) == N2H_EINVAL
+    lib.n2h_adj_last_error(buf, len(buf))
+    assert b"long_delay" in buf.value
+    nets[0].long_delay, nets[0].long_parts = 4, 65
+    assert lib.n2h_adj_step(nets, 1, 0, 0, ctypes.byref(sur), None, 0, None) == N2H_EINVAL
+    lib.n2h_adj_last_error(buf, len(buf))
+    assert b"long_parts" in buf.value
+    bad_sur = Surrogate(0.0, 0.2, 0.01, 0.0)
+    assert lib.n2h_adj_window(nets, 1, 0, 4, ctypes.byref(bad_sur), None, 0, None) == N2H_EINVAL
+    assert lib.n2h_adj_window(nets, 0, 0, 4, ctypes.byref(sur), None, 0, None) == N2H_EINVAL
