@@ -633,8 +633,9 @@ def main() -> int:
     ap.add_argument("--long-delay", type=int, default=4,
                     help="pd14grad: out-links with at least this delay (steps) are gathered once per window of "
                          "that many reverse steps")
-    ap.add_argument("--long-parts", type=int, default=8,
-                    help="pd14grad: target ranges the long out-links are walked in, one launch each")
+    ap.add_argument("--long-parts", type=int, default=24,
+                    help="pd14grad: target ranges the long out-links are walked in (a multiple of 8: one launch, "
+                         "the XCDs taking ranges in turn; otherwise one launch a range)")
     ap.add_argument("--cutoff", type=float, default=0.2,
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,

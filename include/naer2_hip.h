@@ -233,7 +233,7 @@ typedef struct n2h_adj_net {
     int32_t n_pops;              /* <= 8 */
     int32_t ring_slots;
     int32_t long_delay;          /* 1 .. 8: the window length and the long-list delay bound */
-    int32_t long_parts;          /* 1 .. 64: target ranges of the long list, one n2h_adj_window launch each */
+    int32_t long_parts;          /* 1 .. 64: target ranges of the long list (a multiple of 8: one launch for all) */
     int32_t reserved;
     const int32_t* pop_first;    /* device [n_pops + 1] */
     const int64_t* out_ptr;      /* device [n + 1], short list */
@@ -275,8 +275,10 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
 int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32_t n_win, const n2h_surrogate* sur,
                    int32_t* work, int64_t work_words, void* stream);
 /* The int32 words of the work buffer of n2h_adj_window / n2h_adj_step for networks of these sizes (-1: bad
- * arguments). */
-int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay);
+ * arguments).  With long_parts a multiple of 8 the buffer also holds the range sums of every possible window
+ * entry (n * long_parts * 8 * long_delay rounded up to a power of 2, in floats): n2h_adj_window then walks all
+ * ranges in one launch, the XCDs taking ranges in turn, and adds their sums in range order. */
+int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay, int32_t long_parts);
 
 /* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
  * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a

@@ -55,7 +55,7 @@ def _bind(lib):
     lib.n2h_adj_window.argtypes = [ctypes.POINTER(AdjNet), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.POINTER(Surrogate), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
     lib.n2h_adj_window.restype = ctypes.c_int32
-    lib.n2h_adj_work_words.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32]
+    lib.n2h_adj_work_words.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.n2h_adj_work_words.restype = ctypes.c_int64
     lib._adj_bound = True
     return lib
@@ -108,7 +108,7 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1)
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
                  cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
-                 long_delay: int = 4, long_parts: int = 8):
+                 long_delay: int = 4, long_parts: int = 24):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -212,7 +212,7 @@ class RateLossGrad:
                              sd["gsum"].data_ptr())
         sur = ctypes.byref(self.sur)
         sizes = (ctypes.c_int32 * len(rts))(*[rt.n for rt in rts])
-        words = lib.n2h_adj_work_words(sizes, len(rts), W)
+        words = lib.n2h_adj_work_words(sizes, len(rts), W, self.long_parts)
         if words < 0:
             raise RuntimeError("n2h_adj_work_words: bad network sizes")
         work = torch.zeros(words, dtype=torch.int32, device=self.dev)

@@ -69,7 +69,8 @@ struct AdjBatch {
     int32_t win_step;  // step: its place in the window
     int32_t recent_pos;  // step mod long_delay: this step's slot of the short ring
     int32_t sub_cap;   // entries a work sub-list holds
-    int32_t part;      // long kernel: the target range of this launch
+    int32_t part;      // long kernel, one range a launch: the target range of this launch
+    float* partial;    // long kernel, ranges split over the XCDs: [window entry][range][8 W'] sums
     int32_t* work;     // counters and work lists
     n2h_surrogate sur;
 };
@@ -146,6 +147,8 @@ constexpr int adj_halving_shift() {
 // line of its own: appends to one counter serialize in the L2.
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
+constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
+static_assert(kSplitBlocks % 8 == 0, "the split long kernel deals workgroups to 8 XCDs in turn");
 constexpr int kSubLists = 16;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
@@ -238,17 +241,37 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
 // read; lanes stride the list with per-lane sums per (target population, window step), then a fixed
 // butterfly; long_bins rows of the steps in the entry's mask.
 template <int W>
+constexpr int adj_win_pad() { return W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8; }  // W padded to a power of 2
+
+// SPLIT = false: one launch per target range (b.part), each adding its sums to the long_bins rows in launch
+// order.  SPLIT = true (long_parts a multiple of kXcds): one launch for all ranges; workgroup b runs on XCD
+// b mod kXcds (workgroups are dealt to the XCDs in turn), whose wavefronts walk ranges x, x + kXcds, ... in
+// that order, so each XCD's L2 holds the ring rows of one range at a time and no two L2s hold the same
+// rows; the range sums go to `partial`, which adj_combine_kernel adds in range order (the same sums, in the
+// same order, as the launch-per-range path).
+constexpr int kXcds = 8;
+
+template <int W, bool SPLIT>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
-    constexpr int WP = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;  // window steps padded to a power of 2
+    constexpr int WP = adj_win_pad<W>();
     constexpr int V = kAdjPops * WP;
     constexpr int SH = adj_halving_shift<V>();
     const int lane = threadIdx.x & (kWave - 1);
-    const int n_waves = gridDim.x * kWavesPerBlock;
     const AdjList list = adj_list_counts(b, 0);
     const int idx = lane >> SH;                    // the element this lane ends with: w * 8 + p
     const int ww = idx >> 3, pp = idx & 7;
     const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
-    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int xcd = SPLIT ? static_cast<int>(blockIdx.x % kXcds) : 0;
+    const int f0 = SPLIT ? static_cast<int>(blockIdx.x / kXcds) * kWavesPerBlock + wave
+                         : static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+    const int fs = SPLIT ? static_cast<int>(gridDim.x / kXcds) * kWavesPerBlock : static_cast<int>(gridDim.x) * kWavesPerBlock;
+    const int P = b.net[0].long_parts;
+    const int64_t F = SPLIT ? static_cast<int64_t>(P / kXcds) * list.total : list.total;
+    for (int64_t f = f0; f < F; f += fs) {
+        const int m = SPLIT ? static_cast<int>(f / list.total) : 0;  // this XCD's m-th range
+        const int i = static_cast<int>(f - static_cast<int64_t>(m) * list.total);
+        const int part = SPLIT ? xcd + kXcds * m : b.part;
         const uint32_t ent = adj_list_entry(b, list, 0, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const uint32_t mk = ent >> 21;
@@ -257,7 +280,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
         float bins[V];
 #pragma unroll
         for (int k = 0; k < V; ++k) bins[k] = 0.0f;
-        const int64_t seg = static_cast<int64_t>(jj) * a.long_parts + b.part;
+        const int64_t seg = static_cast<int64_t>(jj) * a.long_parts + part;
         const int64_t e0 = a.long_ptr[seg], e1 = a.long_ptr[seg + 1];
         // kUnroll links a lane per pass, every load issued before the sums; the lane's links are added in
         // list order (e, e + 64, ...) as in a plain loop
@@ -292,11 +315,36 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
             }
         }
         const float v = (e1 > e0) ? adj_reduce_halving<V>(bins, lane) : 0.0f;  // uniform branch
-        // the rows of the steps in the mask: target ranges added in launch order (the first one stores)
-        if (writer && ((mk >> ww) & 1u)) {
+        if constexpr (SPLIT) {
+            if (writer) b.partial[(static_cast<int64_t>(i) * P + part) * V + idx] = v;
+        } else if (writer && ((mk >> ww) & 1u)) {
+            // the rows of the steps in the mask: target ranges added in launch order (the first one stores)
             float* row = a.long_bins + (static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp;
-            *row = b.part == 0 ? v : *row + v;
+            *row = part == 0 ? v : *row + v;
         }
+    }
+}
+
+// The window's gatherers, a wavefront each: lane w * 8 + p adds the entry's range sums of (window step w,
+// target population p) in range order into its long_bins row (steps in the entry's mask).
+template <int W>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(AdjBatch b) {
+    constexpr int V = kAdjPops * adj_win_pad<W>();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n_waves = gridDim.x * kWavesPerBlock;
+    const AdjList list = adj_list_counts(b, 0);
+    const int P = b.net[0].long_parts;
+    const int ww = lane >> 3, pp = lane & 7;
+    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
+        const uint32_t ent = adj_list_entry(b, list, 0, i);
+        const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
+        const uint32_t mk = ent >> 21;
+        if (lane >= W * kAdjPops || !((mk >> ww) & 1u)) continue;
+        const float* src = b.partial + static_cast<int64_t>(i) * P * V + lane;
+        float sum = src[0];
+        for (int r = 1; r < P; ++r) sum = sum + src[static_cast<int64_t>(r) * V];
+        const n2h_adj_net& a = b.net[q];
+        a.long_bins[(static_cast<int64_t>(ww) * a.n_neurons + jj) * kAdjPops + pp] = sum;
     }
 }
 
@@ -451,27 +499,47 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
     return N2H_OK;
 }
 
-// the work buffer: the counters, then 1 + long_delay lists of kSubLists sub-lists of sub_cap entries
-int64_t adj_work_words(int64_t blocks, int32_t long_delay) {
+// the work buffer: the counters, then 1 + long_delay lists of kSubLists sub-lists of sub_cap entries, then
+// (long_parts a multiple of kXcds) the range sums of every possible window entry
+int64_t adj_lists_words(int64_t blocks, int32_t long_delay) {
     const int64_t sub_cap = kWave * ((blocks * kWavesPerBlock + kSubLists - 1) / kSubLists);
     return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * sub_cap;
 }
+int64_t adj_partial_floats(int64_t n_total, int32_t long_delay, int32_t long_parts) {
+    if (long_parts % kXcds != 0) return 0;
+    const int wp = long_delay <= 1 ? 1 : long_delay <= 2 ? 2 : long_delay <= 4 ? 4 : 8;
+    return n_total * long_parts * kAdjPops * wp;
+}
+int64_t adj_work_words(int64_t blocks, int64_t n_total, int32_t long_delay, int32_t long_parts) {
+    return adj_lists_words(blocks, long_delay) + adj_partial_floats(n_total, long_delay, long_parts);
+}
 
-int adj_bind_work(AdjBatch& b, int64_t blocks, int32_t long_delay, int32_t* work, int64_t work_words) {
-    const int64_t need = adj_work_words(blocks, long_delay);
+int adj_bind_work(AdjBatch& b, const n2h_adj_net* nets, int32_t n_nets, int64_t blocks, int32_t* work,
+                  int64_t work_words) {
+    int64_t n_total = 0;
+    for (int i = 0; i < n_nets; ++i) n_total += nets[i].n_neurons;
+    const int32_t W = nets[0].long_delay, P = nets[0].long_parts;
+    const int64_t need = adj_work_words(blocks, n_total, W, P);
     if (!work || work_words < need)
         return fail(N2H_EINVAL, "work: %lld int32 words needed (n2h_adj_work_words), %lld given",
                     static_cast<long long>(need), static_cast<long long>(work_words));
     b.work = work;
+    b.partial = adj_partial_floats(n_total, W, P) ? reinterpret_cast<float*>(work + adj_lists_words(blocks, W)) : nullptr;
     return N2H_OK;
 }
 
 template <int W>
 void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
-    hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, stream, b);
+    const dim3 block(kWave * kWavesPerBlock);
+    hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), block, 0, stream, b);
+    if (b.partial) {  // the ranges split over the XCDs, then added in order
+        hipLaunchKernelGGL((adj_long_kernel<W, true>), dim3(kSplitBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL(adj_combine_kernel<W>, dim3(kShortBlocks), block, 0, stream, b);
+        return;
+    }
     for (int part = 0; part < b.net[0].long_parts; ++part) {  // one target range a launch
         b.part = part;
-        hipLaunchKernelGGL(adj_long_kernel<W>, dim3(kLongBlocks), dim3(kWave * kWavesPerBlock), 0, stream, b);
+        hipLaunchKernelGGL((adj_long_kernel<W, false>), dim3(kLongBlocks), block, 0, stream, b);
     }
 }
 
@@ -479,14 +547,16 @@ void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
 
 extern "C" {
 
-int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay) {
+int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay, int32_t long_parts) {
     if (!n_neurons || n_nets < 1 || n_nets > kAdjNets || long_delay < 1 || long_delay > kMaxWindow) return -1;
-    int64_t blocks = 0;
+    if (long_parts < 1 || long_parts > kMaxParts) return -1;
+    int64_t blocks = 0, n_total = 0;
     for (int i = 0; i < n_nets; ++i) {
         if (n_neurons[i] < 1) return -1;
         blocks += (n_neurons[i] + kWave * kWavesPerBlock - 1) / (kWave * kWavesPerBlock);
+        n_total += n_neurons[i];
     }
-    return adj_work_words(blocks, long_delay);
+    return adj_work_words(blocks, n_total, long_delay, long_parts);
 }
 
 int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t win_step, const n2h_surrogate* sur,
@@ -495,7 +565,7 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, blocks, nets[0].long_delay, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, blocks, work, work_words)) != N2H_OK) return rc;
     if (win_step < 0 || win_step >= nets[0].long_delay)
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
@@ -513,7 +583,7 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step0, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, blocks, nets[0].long_delay, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, blocks, work, work_words)) != N2H_OK) return rc;
     const int W = nets[0].long_delay;
     if (n_win < 1 || n_win > W) return fail(N2H_EINVAL, "n_win %d outside [1, long_delay = %d]", n_win, W);
     b.n_win = n_win;

@@ -75,12 +75,17 @@ def test_work_words_follow_the_list_layout():
     arr = (ctypes.c_int32 * 4)(*sizes)
     blocks = sum((s + 255) // 256 for s in sizes)
     sub_cap = 64 * ((blocks * 4 + 15) // 16)
-    assert lib.n2h_adj_work_words(arr, 4, 4) == 9 * 16 * 32 + 5 * 16 * sub_cap
-    assert lib.n2h_adj_work_words(arr, 4, 8) == 9 * 16 * 32 + 9 * 16 * sub_cap
-    assert lib.n2h_adj_work_words(arr, 0, 4) == -1
-    assert lib.n2h_adj_work_words(arr, 4, 9) == -1
+    lists = {4: 9 * 16 * 32 + 5 * 16 * sub_cap, 8: 9 * 16 * 32 + 9 * 16 * sub_cap, 3: 9 * 16 * 32 + 4 * 16 * sub_cap}
+    assert lib.n2h_adj_work_words(arr, 4, 4, 1) == lists[4]  # one launch a range: the lists only
+    assert lib.n2h_adj_work_words(arr, 4, 8, 5) == lists[8]
+    n = sum(sizes)
+    assert lib.n2h_adj_work_words(arr, 4, 4, 16) == lists[4] + n * 16 * 8 * 4  # + the range sums
+    assert lib.n2h_adj_work_words(arr, 4, 3, 8) == lists[3] + n * 8 * 8 * 4   # 3 steps padded to 4
+    assert lib.n2h_adj_work_words(arr, 0, 4, 8) == -1
+    assert lib.n2h_adj_work_words(arr, 4, 9, 8) == -1
+    assert lib.n2h_adj_work_words(arr, 4, 4, 65) == -1
     bad = (ctypes.c_int32 * 1)(0)
-    assert lib.n2h_adj_work_words(bad, 1, 4) == -1
+    assert lib.n2h_adj_work_words(bad, 1, 4, 8) == -1
 
 
 def test_adjoint_calls_refuse_bad_arguments_before_any_launch():

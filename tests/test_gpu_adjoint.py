@@ -117,13 +117,13 @@ def test_gradient_agrees_over_window_lengths():
     targets = np.random.default_rng(2).uniform(0.5, 10.0, (2, 8))
     sur = {"cutoff": 0.5, "gamma": 0.5}
     res = {}
-    for W, P in ((1, 1), (3, 5), (4, 8), (8, 16), (4, 1)):
+    for W, P in ((1, 1), (3, 5), (4, 8), (8, 16), (4, 1), (4, 24)):
         _say(f"case: long_delay {W}, long_parts {P}, checkpoint 50, T={T}")
         rts = [n.build() for n in _nets(alpha)]
         res[W, P] = RateLossGrad(rts, alpha, checkpoint=50, long_delay=W, long_parts=P, **sur)(T, targets)
     g1 = res[1, 1][1]
     assert np.abs(g1).max() > 0
-    for key in ((3, 5), (4, 8), (8, 16), (4, 1)):
+    for key in ((3, 5), (4, 8), (8, 16), (4, 1), (4, 24)):
         assert res[key][0] == res[1, 1][0]
         err = np.linalg.norm(res[key][1] - g1) / np.linalg.norm(g1)
         _say(f"long_delay, long_parts {key}: rel diff {err:.3g}")
