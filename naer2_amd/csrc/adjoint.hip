@@ -24,12 +24,13 @@
 // Windows.  The out-links are split by delay: short (d < W) and long (d >= W), W = long_delay.  For a window
 // of reverse steps [t0, t0 + W) every long link's input adjoints gx_{t0+w+d} (w < W) lie past the window, so
 // adj_long_kernel reads each gathering source's long links once per window and gathers a link's W values
-// in one contiguous read (the long links are walked in long_parts target ranges, one launch each, so that a
-// launch's gathers fit the L2s; the ranges' sums are added in order): the ring is target-major, gx[k][slot], step s in slot s mod R, with the first
-// W - 1 slots mirrored past slot R - 1 so that W consecutive steps never wrap.  Each step, adj_short_kernel
-// walks only the short links and adds the window's long sums (bins[a] = short sum + long sum), then
-// adj_update_kernel updates every neuron.  The split depends on the link alone, so the sums do not depend on
-// where windows, checkpoints or batches start.
+// in one contiguous read: the ring is target-major, gx[k][slot], step s in slot s mod R, with the first W - 1
+// slots mirrored past slot R - 1 so that W consecutive steps never wrap.  The long links are walked in
+// long_parts target ranges so that the L2s hold the rows being read (adj_long_kernel<W, true>: all ranges in one
+// launch over the XCDs, adj_combine_kernel adding their sums in order; adj_long_kernel<W, false>: one
+// launch a range).  Each step, adj_step_kernel walks only the short links, adds the window's long sums (bins[a] =
+// short sum + long sum) and updates every neuron.  The split depends on the link alone, so the sums do not
+// depend on where windows, checkpoints or batches start.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -77,12 +78,12 @@ struct AdjBatch {
 
 // The network of this block and the first neuron of this wavefront: a block covers kWavesPerBlock * 64
 // consecutive neurons of one network, a lane one neuron.
-__device__ inline int adj_wave_base(const AdjBatch& b, int& q) {
-    const int blk = blockIdx.x;
+__device__ inline int adj_wave_base_at(const AdjBatch& b, int blk, int& q) {
     q = 0;
     while (q + 1 < b.n_nets && b.first_block[q + 1] <= blk) ++q;
     return ((blk - b.first_block[q]) * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6)) * kWave;
 }
+__device__ inline int adj_wave_base(const AdjBatch& b, int& q) { return adj_wave_base_at(b, blockIdx.x, q); }
 
 __device__ inline int adj_pop(const n2h_adj_net& a, int j) {
     int pj = 0;
@@ -348,11 +349,28 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(Adj
     }
 }
 
-// This step's gatherers, a wavefront each: their short links (lanes stride the list, per-lane sums per
-// target population, then a fixed butterfly); row = short sums + the window's long sums, in place.
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBatch b) {
+__device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a, int j, const float (&mine)[kAdjPops]);
+
+// One reverse step in one launch.  Workgroups [0, kShortBlocks) take this step's gatherers, a wavefront
+// each: their short links (lanes stride the list, per-lane sums per target population, then a fixed
+// reduction), plus the window's long sums (short + long), then lane 0 updates the neuron.  The other
+// workgroups update every neuron that did not gather, a lane each.  Neither half waits for the other: a
+// gather reads the adjoints of steps t + 1 .. t + W - 1 only, never those this step writes.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBatch b) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int n_waves = gridDim.x * kWavesPerBlock;
+    if (static_cast<int>(blockIdx.x) >= kShortBlocks) {
+        int q;
+        const int base = adj_wave_base_at(b, static_cast<int>(blockIdx.x) - kShortBlocks, q);
+        const n2h_adj_net& a = b.net[q];
+        const int j = base + lane;
+        if (j >= a.n_neurons) return;
+        const float* C = a.pop_coef + adj_pop(a, j) * 8;
+        if (a.spikes[j] != 0 || adj_u(a.vtrace[j], C[4], C[5]) <= b.sur.cutoff) return;  // a gatherer
+        const float none[kAdjPops] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        adj_update_neuron(b, a, j, none);
+        return;
+    }
+    const int n_waves = kShortBlocks * kWavesPerBlock;
     const AdjList list = adj_list_counts(b, 1 + b.win_step);
     for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
         const uint32_t ent = adj_list_entry(b, list, 1 + b.win_step, i);
@@ -385,21 +403,17 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_short_kernel(AdjBa
             }
         }
         const float v = (e1 > e0) ? adj_reduce_halving<kAdjPops>(bins, lane) : 0.0f;  // lane l: population l >> 3
-        if ((lane & 7) == 0) {
-            float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops + (lane >> 3);
-            *row = v + *row;
-        }
+        const float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops;
+        float tot[kAdjPops];
+#pragma unroll
+        for (int p = 0; p < kAdjPops; ++p) tot[p] = __shfl(v, p * 8, kWave) + row[p];  // short + long
+        if (lane == 0) adj_update_neuron(b, a, jj, tot);
     }
 }
 
-// Lane per neuron: the update of the reverse step from the neuron's summed out-link adjoints (the rows
-// adj_short_kernel completed; zero for a neuron that did not gather).
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_update_kernel(AdjBatch b) {
-    int q;
-    const int base = adj_wave_base(b, q);
-    const n2h_adj_net& a = b.net[q];
-    const int j = base + static_cast<int>(threadIdx.x & (kWave - 1));
-    if (j >= a.n_neurons) return;
+// The update of neuron j in this reverse step from its summed out-link adjoints (zero when it did not
+// gather).
+__device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a, int j, const float (&mine)[kAdjPops]) {
     const int R = a.ring_slots, RS = R + a.long_delay - 1;
 
     const int pj = adj_pop(a, j);
@@ -408,16 +422,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_update_kernel(AdjB
     const float vp = a.vtrace[j];
     const bool s = a.spikes[j] != 0;
     const float u = adj_u(vp, theta, dv);
-    const bool near = u <= b.sur.cutoff;  // the surrogate is nonzero (the predicate adj_mark_kernel listed)
-    float mine[kAdjPops];
-#pragma unroll
-    for (int p = 0; p < kAdjPops; ++p) mine[p] = 0.0f;
-    if (near || s) {
-        const float4* lb = reinterpret_cast<const float4*>(a.long_bins + static_cast<int64_t>(j) * kAdjPops);
-        const float4 l0 = lb[0], l1 = lb[1];
-        mine[0] = l0.x; mine[1] = l0.y; mine[2] = l0.z; mine[3] = l0.w;
-        mine[4] = l1.x; mine[5] = l1.y; mine[6] = l1.z; mine[7] = l1.w;
-    }
+    const bool near = u <= b.sur.cutoff;  // the surrogate is nonzero
     float coup = 0.0f;
     for (int p = 0; p < a.n_pops; ++p) coup += a.alpha[p * a.n_pops + pj] * mine[p];
     if (s) {
@@ -570,8 +575,8 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(adj_short_kernel, dim3(kShortBlocks), dim3(kWave * kWavesPerBlock), 0, s, b);
-    hipLaunchKernelGGL(adj_update_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave * kWavesPerBlock), 0, s, b);
+    hipLaunchKernelGGL(adj_step_kernel, dim3(static_cast<unsigned>(kShortBlocks + blocks)), dim3(kWave * kWavesPerBlock), 0,
+                       s, b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(N2H_EHIP, "adjoint step kernels: %s", hipGetErrorString(e));
     return N2H_OK;
