@@ -1,25 +1,35 @@
-"""PD14 real-time factor and synaptic events/s on one MI355X.
+"""PD14 real-time factor and synaptic events/s on one MI355X, with the batched, BPTT and differentiable
+configurations measured beside it on the same GPU.
 
-    python bench.py [--gpus 1] [--steps 10000] [--warmup 1000] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 1000] [--no-cpu] [--no-secondary]
 
-Workload: the full-scale Potjans & Diesmann (2014) microcircuit of naer2's test/networks/PD14 (77,169 LIF
-neurons with exponential current synapses, ~298.8 M synapses by the reference's fixed-total-number rule,
-Poisson background, dt = 0.1 ms), built with seed 0 from synthetic random draws.  A step is one fused
-kernel launch advancing the whole network by dt; `steps` of them are timed (default 10,000 = 1 s of
-biological time) after `warmup` untimed ones, replayed from captured hipGraphs with every input resident
-in HBM.
+Headline workload: the full-scale Potjans & Diesmann (2014) microcircuit of naer2's test/networks/PD14
+(77,169 LIF neurons with exponential current synapses, ~298.8 M synapses by the reference's
+fixed-total-number rule, Poisson background, dt = 0.1 ms), built with seed 0 from synthetic random draws.
+A step is one fused kernel launch advancing the whole network by dt.  `warmup` untimed steps run first;
+then every hipGraph the timed region replays is captured (untimed), and exactly `steps` steps are timed
+between two device synchronizations, with every input resident in HBM.
 
-value: real-time factor = simulated (bio) seconds / wall seconds.  Also reported: synaptic events/s
-(links delivered per wall second), the roofline of the step kernel (algorithmic bytes per launch over its
-average duration: HIP events recorded on the launch stream around the timed region, divided by the
-launches in it), and the CPU oracle's rate on this host's
-cores over a bounded sample of the same workload, from the same state.
+value: real-time factor = simulated (bio) seconds / wall seconds over the timed steps.  Beside it:
+- synaptic events/s (links delivered per wall second) and the step kernel's roofline (algorithmic bytes per
+  launch over its average duration: HIP events on the launch stream around the timed region, divided by
+  the launches in it);
+- `full_config`: the configuration's own run, 1 s of bio time (10,000 steps) after 1,000 settling steps;
+- `cpu_baseline`: the CPU oracle on this host's usable cores over a bounded sample of the same network from
+  the GPU's state, checked bit for bit against the GPU's next steps;
+- `configs`: the other configurations at their full sizes -- the plain Brunel network, the batched Brunel
+  sweep against 256 sequential plain runs, the BPTT cost ratio and gradient error, the batched
+  differentiable PD14's cost ratio and gradient error -- each skipped, with the reason, when the run's time
+  budget (--budget-s) would not hold it.
+`--workload` runs one of them alone and prints its own line.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -33,11 +43,64 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
+def _process_start() -> float:
+    """perf_counter() value at this process's start (the interpreter's and torch's imports included)."""
+    try:
+        import psutil
+        return time.perf_counter() - max(0.0, time.time() - psutil.Process().create_time())
+    except Exception:
+        return time.perf_counter()
+
+
+T_START = _process_start()
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.perf_counter() - T_START:6.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+# ----------------------------------------------------------------------------------------------- host facts
+
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPUs, the CPUs this process may run on and its cgroup CPU quota."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+        except (OSError, StopIteration):
+            model = "unknown"
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    for path in ("/sys/fs/cgroup/cpu.max",):
+        try:
+            q, p = open(path).read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(p)
+        except (OSError, ValueError):
+            pass
+    usable = affinity if quota is None else max(1, min(affinity, int(math.floor(quota + 1e-9))))
+    return {"model_name": model, "nproc": nproc, "affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+            "usable_cpus": usable}
+
+
+# ----------------------------------------------------------------------------------------------- plain path
+
 def _events_and_spikes(rt, step_lo: int, step_hi: int, out_deg: torch.Tensor) -> tuple[int, int]:
     """(links delivered, spikes) for the spikes of steps [step_lo, step_hi): each spike of source s is
     delivered over its out-degree."""
     words = rt.raster_bits(step_lo, step_hi)  # int64 [steps, G]
-    G = rt.n_groups
     gf = rt.layout.group_first.to(torch.int64)
     cnt = (rt.layout.group_info & 0xFFFF).to(torch.int64)
     lane = torch.arange(64, device=words.device)
@@ -63,27 +126,76 @@ def _lambdas(rt) -> list:
     return out
 
 
-def cpu_baselines(net, rt, args, kind: str) -> dict:
-    """CPU baselines on the host's cores, from the GPU's state after the timed run:
-    the C oracle on one thread and on `threads` (OpenMP over target ranges; both checked bit for bit
-    against the GPU's next steps), and the reference's own algorithm (tools/brunel_reference.py's
-    plain-torch loop, oracle/cpu_reference.py) on `threads`."""
+def timed_run(net, rt, n_steps: int, window: int, out_deg: torch.Tensor, kind: str) -> dict:
+    """Exactly n_steps steps from the current state, every graph captured beforehand: wall time between two
+    device synchronizations, HIP events on the launch stream, delivered links and spikes, the roofline."""
+    gw = n_steps if n_steps <= 2000 else window  # short runs: one graph of exactly n_steps
+    rt.prepare(n_steps, gw)
+    s0 = rt.step
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    net.run(n_steps, graph_window=gw)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    events, spikes = _events_and_spikes(rt, s0 - 1, rt.step - 1, out_deg)
+    bio_s = n_steps * rt.dt / 1000.0
+    # algorithmic bytes of one launch: per population n (8 n_state / k + 8) + 8 E + 8 S; n_state 3 for the
+    # exponential synapse (V, I, refractory counter), 2 for the delta synapse; k the population's steps per
+    # update (its state is read and written once per k steps, its ring slots read and cleared every step)
+    n_state = 3 if kind == "pd14" else 2
+    strides = [p.stride if p.stride > 1 else 1 for p in rt.params]
+    state_bytes = sum(pop.n * (8 * n_state / k + 8) for pop, k in zip(rt.pops, strides))
+    bytes_per_step = state_bytes + 8 * events / n_steps + 8 * spikes / n_steps
+    k_ms = gpu_ms / n_steps
+    achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
+    return {"steps": n_steps, "from_step": s0, "graph_window": gw, "wall_s": wall, "gpu_ms": gpu_ms,
+            "rtf": bio_s / wall, "ms_per_step": wall * 1000.0 / n_steps, "events": events, "spikes": spikes,
+            "events_per_s": events / wall, "kernel_avg_us": k_ms * 1000.0, "bytes_per_step": bytes_per_step,
+            "achieved_gbs": achieved}
+
+
+def _roofline(m: dict, traffic, occupancy, kernel: str) -> dict:
+    return {"bound": "hbm", "achieved": round(m["achieved_gbs"], 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(m["achieved_gbs"] / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": kernel,
+            "kernel_avg_us": round(m["kernel_avg_us"], 3), "occupancy": occupancy,
+            "algorithmic_bytes_per_launch": round(m["bytes_per_step"], 1),
+            "region": f"steps {m['from_step']}..{m['from_step'] + m['steps'] - 1}"}
+
+
+def cpu_baselines(net, rt, args, kind: str, cpu: dict) -> dict:
+    """CPU baselines on the host, from the GPU's state: the C oracle (built here with -O3 -march=native) on
+    one thread and on every usable CPU (OpenMP over target ranges; both checked bit for bit against the
+    GPU's next steps), and the reference's own algorithm (tools/brunel_reference.py's plain-torch loop,
+    oracle/cpu_reference.py) on every usable CPU."""
     from naer2_amd.params import params_bytes
     from oracle import cpu_reference, oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    native = True
+    try:
+        oracle.use_native()
+    except (OSError, subprocess.CalledProcessError, RuntimeError) as e:  # no compiler: the portable -O2 build
+        native = False
+        log(f"cpu: native oracle build failed ({e}); using the portable build")
+    build = "-O3 -march=native" if native else "-O2 (portable)"
+    threads = cpu["usable_cpus"]
     model = oracle.LIF_EXP if kind == "pd14" else oracle.LIF_DELTA
     c = rt.canon.to("cpu")
     prev = rt.raster(rt.step - 1, rt.step)[0].astype(np.uint8)
     snap = (rt.v.cpu().numpy(), None if rt.isyn is None else rt.isyn.cpu().numpy(), rt.ref.cpu().numpy(),
             rt.ring.cpu().numpy(), prev)
-    # bounded samples of a few seconds of CPU work each (scaled by --cpu-steps, 2000 by default)
+    # bounded samples of a few seconds of CPU work each (scaled by --cpu-steps)
     f = args.cpu_steps / 2000.0
     c0 = rt.step
-    n1 = int(2000 * f) if kind == "pd14" else int(50000 * f)        # one thread
-    n_mt = int(20000 * f) if kind == "pd14" else int(100000 * f)    # every core
-    net.run(max(n1, n_mt), graph_window=0)
+    n1 = max(4, int(2000 * f)) if kind == "pd14" else max(4, int(50000 * f))                  # one thread
+    n_mt = max(4, int(1000 * f * threads)) if kind == "pd14" else max(4, int(100000 * f))       # every CPU
+    n_gpu = max(n1, n_mt)
+    rt._ensure_slots(n_gpu)
+    net.run(n_gpu, graph_window=0)
     torch.cuda.synchronize()
-    gpu_raster = rt.raster(c0, c0 + max(n1, n_mt))
+    gpu_raster = rt.raster(c0, c0 + n_gpu)
     gpu_v = rt.v.cpu().numpy()
     pb = params_bytes(rt.params)
     arrays = (c.src_ptr.numpy(), c.tgt.numpy(), c.w.numpy(), c.d.numpy())
@@ -100,13 +212,14 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
     w1 = time.perf_counter() - t0
     same1 = bool(np.array_equal(gpu_raster[:n1], r1))
     name = "PD14" if kind == "pd14" else "Brunel"
+    host = {k: cpu[k] for k in ("model_name", "nproc", "affinity_cpus", "cgroup_cpu_quota")}
     if any(p.stride > 1 for p in rt.params):  # the threaded oracle and the torch loop are single-rate only
-        print(f"[bench] cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1})", file=sys.stderr, flush=True)
+        log(f"cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1})")
         return {"value": round(n1 * bio_step / w1, 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
                 "sample": f"{n1} steps ({n1 * bio_step * 1000:.0f} ms bio) of the same multi-rate {name} network from "
-                          f"the GPU state after step {c0}: the C oracle (-O2) on one host thread; raster bit-exact "
-                          f"vs GPU: {same1}",
-                "events_per_s": round(ev1 / w1, 1), "bit_exact_vs_gpu": same1}
+                          f"the GPU state after step {c0}: the C oracle ({build}) on one host thread; raster "
+                          f"bit-exact vs GPU: {same1}",
+                "events_per_s": round(ev1 / w1, 1), "bit_exact_vs_gpu": same1, "host": host}
     par = oracle.Parallel(rt.n, *arrays, threads=threads)
     st = state()
     t0 = time.perf_counter()
@@ -114,20 +227,20 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
     wm = time.perf_counter() - t0
     par.close()
     samem = bool(np.array_equal(gpu_raster[:n_mt], rm)) and (
-        n_mt != max(n1, n_mt) or bool(np.array_equal(gpu_v.view(np.uint32), st.v.view(np.uint32))))
-    n_ref = int(100 * f) if kind == "pd14" else int(20000 * f)       # the reference's torch loop
+        n_mt != n_gpu or bool(np.array_equal(gpu_v.view(np.uint32), st.v.view(np.uint32))))
+    n_ref = max(4, int(100 * f)) if kind == "pd14" else max(4, int(20000 * f))  # the reference's torch loop
     v, i, _, _, _ = snap
     ref = cpu_reference.run(model, rt.pop_first, rt.params, _lambdas(rt), c, rt.d_slots, v, i if i is not None else v,
                             n_ref, threads)
-    print(f"[bench] cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1}); oracle {threads} threads "
-          f"{n_mt} steps {wm:.1f} s (bit-exact {samem}); reference algorithm (torch) {threads} threads {n_ref} steps "
-          f"{ref['seconds']:.1f} s", file=sys.stderr, flush=True)
+    log(f"cpu: oracle 1 thread {n1} steps {w1:.1f} s (bit-exact {same1}); oracle {threads} threads {n_mt} steps "
+        f"{wm:.1f} s (bit-exact {samem}); reference algorithm (torch) {threads} threads {n_ref} steps "
+        f"{ref['seconds']:.1f} s")
     return {
         "value": round(n_mt * bio_step / wm, 6), "unit": "bio_s/wall_s", "cores": threads, "kind": "port",
         "sample": f"{n_mt} steps ({n_mt * bio_step * 1000:.0f} ms bio) of the same {name} network from the GPU state "
-                  f"after step {c0}: the C oracle (-O2) on {threads} host threads (OpenMP, target ranges); "
-                  f"raster bit-exact vs GPU: {samem}",
-        "events_per_s": round(evm / wm, 1), "bit_exact_vs_gpu": samem and same1,
+                  f"after step {c0}: the C oracle ({build}, built on this host) on {threads} host threads (OpenMP, "
+                  f"target ranges; every CPU this process may use); raster bit-exact vs GPU: {samem}",
+        "events_per_s": round(evm / wm, 1), "bit_exact_vs_gpu": samem and same1, "host": host,
         "single_thread": {"value": round(n1 * bio_step / w1, 6), "cores": 1, "steps": n1,
                           "events_per_s": round(ev1 / w1, 1), "bit_exact_vs_gpu": same1},
         "reference_algorithm": {"value": round(n_ref * bio_step / ref["seconds"], 6), "cores": threads,
@@ -137,108 +250,197 @@ def cpu_baselines(net, rt, args, kind: str) -> dict:
     }
 
 
-def bench_sweep(args) -> int:
-    """Batched Brunel sweep: --instances networks drawn from --seed (N log-uniform in [1k, 50k], in-degree
-    10 %, g in [3, 8], eta in [0.9, 4]), --warmup batched steps, then one batched launch per step (mapping
-    from the autotuner: each candidate timed on --warmup steps from the same state) against the same
-    networks run one after another from the same state.  The final
-    states of both must be bit-identical."""
+def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: dict) -> dict:
+    """The plain path: the full-scale PD14 microcircuit (the headline) or the 2500-neuron Brunel network.
+    `warmup` steps, then exactly `steps` timed steps; then the configuration's own 1 s of bio time after
+    1,000 settling steps (when the timed run is not already that); the CPU baselines from the GPU's state."""
+    from naer2_amd.networks import brunel, pd14
+
+    t_build = time.perf_counter()
+    if kind == "pd14":
+        net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
+    else:
+        net, pops = brunel(seed=args.seed, device="cuda")
+    timesteps = {}
+    for spec in args.timestep or []:
+        name, _, ms = spec.partition("=")
+        if name not in pops:
+            raise SystemExit(f"--timestep: no population {name!r} (have {sorted(pops)})")
+        pops[name].timestep = timesteps[name] = float(ms)
+    rt = net.build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    log(f"built {kind}: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, {rt.d_slots} ring "
+        f"slots in {t_build:.1f} s")
+    net.record = True
+    out_deg = rt.canon.out_degree()
+
+    if warmup:
+        rt.prepare(warmup, args.window)
+        net.run(warmup, graph_window=args.window)
+    torch.cuda.synchronize()
+    log(f"warmup {warmup} steps done")
+    m = timed_run(net, rt, steps, args.window, out_deg, kind)
+    log(f"timed {steps} steps: {m['wall_s'] * 1e3:.3f} ms wall, {m['gpu_ms']:.3f} ms on the stream, RTF {m['rtf']:.3f}")
+    rates = net.rates(t_skip_ms=(m["from_step"] * rt.dt))
+    timed_rates = {name: round(rates[p], 3) for name, p in pops.items()}
+
+    # the configuration's own run: 1 s of bio time after 1,000 settling steps
+    settle, full_n = 1000, int(round(1000.0 / rt.dt))
+    if m["from_step"] >= settle and steps >= full_n:
+        full, full_rates = m, timed_rates
+    else:
+        if rt.step < settle:
+            net.run(settle - rt.step, graph_window=0)
+        full = timed_run(net, rt, full_n, args.window, out_deg, kind)
+        rr = net.rates(t_skip_ms=(full["from_step"] * rt.dt))
+        full_rates = {name: round(rr[p], 3) for name, p in pops.items()}
+        log(f"full config: {full_n} steps from step {full['from_step']}: RTF {full['rtf']:.3f}, "
+            f"{full['kernel_avg_us']:.2f} us per launch")
+
+    traffic, occupancy = None, None
+    tfile = ROOT / "profiles" / "pd14_step_traffic.json"
+    if kind == "pd14" and tfile.exists() and not timesteps:
+        prof = json.loads(tfile.read_text())
+        traffic, occupancy = prof.get("hbm_bytes_per_launch"), prof.get("occupancy")
+    kernel = "step_kernel<2, 32> (1210 groups)" if kind == "pd14" else "step_kernel<1, 8>"
+
+    cpu_line = cpu_baselines(net, rt, args, kind, cpu) if cpu_leg else None
+    workload = ("pd14 full scale: 77169 LIF-exp neurons, 8 populations, "
+                f"{rt.canon.n_links} synapses, Poisson background" if kind == "pd14" else
+                "brunel: N_E 2000, N_I 500 delta-synapse LIF, C_E 200, C_I 50, J 0.2 mV, "
+                f"g 5, D 1.5 ms, {rt.canon.n_links} synapses, Poisson drive (eta 2)")
+    line = {
+        "metric": "PD14 real-time factor + synaptic events/s" if kind == "pd14" else
+                  "Brunel real-time factor + synaptic events/s",
+        "value": round(m["rtf"], 4),
+        "unit": "bio_s/wall_s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(m["ms_per_step"], 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload + f", dt 0.1 ms, seed {args.seed}; value over {steps} steps after "
+                               f"{warmup} warmup steps",
+                   "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
+                   **({"timesteps_ms": timesteps} if timesteps else {}),
+                   "graph_window": m["graph_window"], "parallelism": "single GPU"},
+        "events_per_s": round(m["events_per_s"], 1),
+        "spikes_per_step": round(m["spikes"] / steps, 3),
+        "events_per_step": round(m["events"] / steps, 1),
+        "rates_hz": timed_rates,
+        "roofline": _roofline(m, traffic, occupancy, kernel),
+        "full_config": {
+            "what": "1 s of bio time (10,000 steps) after 1,000 settling steps, graph windows of "
+                    f"{full['graph_window']} steps",
+            "rtf": round(full["rtf"], 4), "ms_per_step": round(full["ms_per_step"], 6),
+            "events_per_s": round(full["events_per_s"], 1), "spikes_per_step": round(full["spikes"] / full["steps"], 3),
+            "events_per_step": round(full["events"] / full["steps"], 1), "rates_hz": full_rates,
+            "roofline": _roofline(full, traffic, occupancy, kernel),
+            "reference_published": "PD14 ~25 us per 0.1 ms round (RTF ~4) on an RTX 4090, captured windows, DC "
+                                   "background (TODO.md:47)" if kind == "pd14" else None,
+        },
+        "cpu_baseline": cpu_line,
+    }
+    del rt, net
+    return line
+
+
+# ----------------------------------------------------------------------------------------------- configs 2-4
+
+def measure_sweep(args, steps: int, warmup: int, window: int, instances: int) -> dict:
+    """Batched Brunel sweep: `instances` networks drawn from --seed (N log-uniform in [1k, 50k], in-degree
+    10 %, g in [3, 8], eta in [0.9, 4]), `warmup` batched steps, then one batched launch per step (mapping
+    from the autotuner: each candidate timed from the same state) against the same networks run one after
+    another from the same state.  The final states of both must be bit-identical."""
     from naer2_amd import FusedRound
     from naer2_amd.autotune import measure_work, tune, tune_intake
     from naer2_amd.networks import brunel_sweep, sweep_params
 
-    params = sweep_params(args.instances or 256, seed=args.seed)
+    params = sweep_params(instances, seed=args.seed)
     t0 = time.perf_counter()
     nets = []
     for i, p in enumerate(params):
         nets += brunel_sweep([p], device="cuda", keep_canonical=False, record=False)
-        if i % 16 == 15:
-            print(f"[bench] built {i + 1}/{len(params)} networks ({time.perf_counter() - t0:.0f} s)",
-                  file=sys.stderr, flush=True)
+        if i % 32 == 31:
+            log(f"sweep: built {i + 1}/{len(params)} networks")
     rts = [n.runtime for n in nets]
     n_syn = sum(rt.n_links for rt in rts)
     n_neurons = sum(rt.n for rt in rts)
     torch.cuda.synchronize()
-    print(f"[bench] sweep: {len(rts)} networks, {n_neurons} neurons, {n_syn} synapses, "
-          f"{sum(rt.n_groups for rt in rts)} groups, built in {time.perf_counter() - t0:.0f} s",
-          file=sys.stderr, flush=True)
+    t_build = time.perf_counter() - t0
+    log(f"sweep: {len(rts)} networks, {n_neurons} neurons, {n_syn} synapses, {sum(rt.n_groups for rt in rts)} "
+        f"groups, built in {t_build:.0f} s")
     if args.tune_intake:
-        t0 = time.perf_counter()
         for rt in rts:
-            tune_intake(rt, steps=200, graph_window=args.window)
-        print(f"[bench] intake thresholds tuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            tune_intake(rt, steps=200, graph_window=window)
     fr = FusedRound(rts, {"order": 1, "xcd": 0})  # warmup: past the onset transient, untimed
-    fr.launch(args.warmup, args.window)
+    fr.launch(warmup, window)
     fr.close()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    weights = measure_work(rts, steps=200, graph_window=args.window)
-    print(f"[bench] per-network work measured in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    best, times = tune(rts, steps=args.warmup, graph_window=args.window, use_cache=False, weights=weights)
-    print(f"[bench] autotune: {best} from {times}", file=sys.stderr, flush=True)
+    weights = measure_work(rts, steps=200, graph_window=window)
+    best, times = tune(rts, steps=min(warmup, 1000), graph_window=window, use_cache=False, weights=weights)
+    log(f"sweep: autotune {best} from {times}")
     snaps = [rt.snapshot() for rt in rts]
     fr = FusedRound(rts, best, weights=weights)
-    fr.launch(args.window, args.window)  # capture, untimed (the state is restored below)
-    torch.cuda.synchronize()
-    for rt, sn in zip(rts, snaps):
-        rt.restore(sn)
-    fr.close()
-    fr = FusedRound(rts, best, weights=weights)
+    fr.prepare(steps, window)  # every graph the timed launch replays, captured untimed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fr.launch(args.steps, graph_window=args.window)
+    fr.launch(steps, graph_window=window)
     torch.cuda.synchronize()
     wall_b = time.perf_counter() - t0
     fr.close()
-    print(f"[bench] batched {args.steps} steps: {wall_b:.3f} s", file=sys.stderr, flush=True)
+    log(f"sweep: batched {steps} steps: {wall_b:.3f} s")
     final_b = [(rt.v.clone(), rt.ref.clone(), rt.ring.clone()) for rt in rts]
     for rt, sn in zip(rts, snaps):
         rt.restore(sn)
-        rt.run(args.window, args.window)  # capture each network's own graph, untimed
-    torch.cuda.synchronize()
-    for rt, sn in zip(rts, snaps):
-        rt.restore(sn)
+        rt.prepare(steps, window)  # every graph of the sequential runs, captured untimed
     torch.cuda.synchronize()
     wall_s = 0.0
-    for i, rt in enumerate(rts):
+    for rt in rts:
         t0 = time.perf_counter()
-        rt.run(args.steps, args.window)
+        rt.run(steps, window)
         torch.cuda.synchronize()
         wall_s += time.perf_counter() - t0
-    print(f"[bench] sequential {args.steps} steps x {len(rts)}: {wall_s:.3f} s", file=sys.stderr, flush=True)
+    log(f"sweep: sequential {steps} steps x {len(rts)}: {wall_s:.3f} s")
     same = all(torch.equal(a[0].view(torch.int32), rt.v.view(torch.int32)) and torch.equal(a[1], rt.ref)
                and torch.equal(a[2].view(torch.int32), rt.ring.view(torch.int32)) for a, rt in zip(final_b, rts))
-    bio = args.steps * rts[0].dt / 1000.0
-    line = {
+    bio = steps * rts[0].dt / 1000.0
+    out = {
         "metric": "batched Brunel sweep: aggregate instance-bio-s per wall-s, batched vs sequential plain runs",
         "value": round(len(rts) * bio / wall_b, 4), "unit": "instance_bio_s/wall_s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{len(rts)} Brunel networks (N log-uniform 1k-50k, in-degree 10 %, g U[3,8], "
-                               f"eta U[0.9,4]), seed {args.seed}", "n_neurons": n_neurons, "n_synapses": n_syn},
+                               f"eta U[0.9,4]), seed {args.seed}, {steps} steps = {bio:g} s bio each",
+                   "n_neurons": n_neurons, "n_synapses": n_syn},
         "sequential_value": round(len(rts) * bio / wall_s, 4),
+        "batched_wall_s": round(wall_b, 4), "sequential_wall_s": round(wall_s, 4),
         "batched_speedup": round(wall_s / wall_b, 3),
         "mapping": best, "autotune_s": times,
         "bit_exact_batched_vs_sequential": bool(same),
+        "build_s": round(t_build, 1),
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    del fr, rts, nets, snaps, final_b
+    torch.cuda.empty_cache()
+    return out
 
 
-def bench_pd14grad(args) -> int:
-    """Batched, differentiable PD14: --instances seed-distinct full-scale microcircuits in one launch per
-    step, --steps steps (500 ms bio at 5000), loss = squared error of the per-population rates against
+def measure_pd14grad(args, steps: int, instances: int) -> dict:
+    """Batched, differentiable PD14: `instances` seed-distinct full-scale microcircuits in one launch per
+    step, `steps` steps (500 ms bio at 5000), loss = squared error of the per-population rates against
     seed-drawn targets in [0.5, 10] Hz, gradient with respect to the 8x8 weight scales from the surrogate
     adjoint (truncated at --cutoff, scaled by --gamma) with checkpoints every --checkpoint steps.  Times the
-    forward (checkpointed, graph windows) and
-    the backward (recompute + adjoint) with HIP events, and checks the same adjoint on a
-    4-neuron-per-population PD14 subsample against the FP64 autograd unroll (on the same spike trains),
-    whose own gradient FP64 central differences pin in the CPU tests."""
-    import numpy as np
-
+    forward (checkpointed, graph windows) and the backward (recompute + adjoint) with HIP events, and checks
+    the same adjoint on a 4-neuron-per-population PD14 subsample against the FP64 autograd unroll (on the
+    same spike trains), whose own gradient FP64 central differences pin in the CPU tests."""
     from naer2_amd.adjoint import RateLossGrad
     from naer2_amd.networks import pd14
 
-    n_inst, T, K = args.instances or 4, args.steps, args.checkpoint
+    n_inst, T, K = instances, steps, args.checkpoint
     rng = np.random.default_rng(args.seed)
     alpha = np.ones((8, 8), dtype=np.float32)
     targets = rng.uniform(0.5, 10.0, (n_inst, 8))
@@ -251,16 +453,16 @@ def bench_pd14grad(args) -> int:
         rts.append(net.build())
     torch.cuda.synchronize()
     n_syn = sum(rt.n_links for rt in rts)
-    print(f"[bench] built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s",
-          file=sys.stderr, flush=True)
+    log(f"pd14grad: built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s")
     sur = {"cutoff": args.cutoff, "gamma": args.gamma}
     grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, long_parts=args.long_parts, **sur)
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
     fw, bw = grad.last["forward_ms"], grad.last["backward_ms"]
-    print(f"[bench] loss {loss:.4g}, forward {fw:.0f} ms, backward {bw:.0f} ms, wall {wall:.1f} s",
-          file=sys.stderr, flush=True)
+    log(f"pd14grad: loss {loss:.4g}, forward {fw:.0f} ms, backward {bw:.0f} ms, wall {wall:.1f} s")
+    del grad, rts
+    torch.cuda.empty_cache()
     # the check: the same adjoint on a 4-neuron-per-population subsample against the FP64 unroll
     from oracle import adjoint_oracle as ao
     Tc = 500
@@ -277,14 +479,15 @@ def bench_pd14grad(args) -> int:
     ao.loss(rates, ts[0]).backward()
     ref = a64.grad.numpy()
     err = float(np.linalg.norm(gs - ref) / max(np.linalg.norm(ref), 1e-300))
-    bio = T * rts[0].dt / 1000.0
-    line = {
+    bio = T * 0.1 / 1000.0
+    return {
         "metric": "batched differentiable PD14: backward/forward cost ratio (rate-loss surrogate adjoint)",
         "value": round(bw / fw, 3), "unit": "ratio", "n_gpus": 1, "steps": T, "warmup": 0,
         "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{n_inst} seed-distinct full-scale PD14 microcircuits (77169 neurons each) in one launch a step, {T} steps "
-                               f"of 0.1 ms, loss sum (rate - target)^2 over populations, targets U[0.5, 10] Hz, "
-                               f"grad wrt the 8x8 weight scales, checkpoints every {K} steps, seed {args.seed}",
+        "config": {"workload": f"{n_inst} seed-distinct full-scale PD14 microcircuits (77169 neurons each) in one "
+                               f"launch a step, {T} steps of 0.1 ms, loss sum (rate - target)^2 over populations, "
+                               f"targets U[0.5, 10] Hz, grad wrt the 8x8 weight scales, checkpoints every {K} steps, "
+                               f"seed {args.seed}",
                    "n_synapses": n_syn},
         "forward_ms": round(fw, 1), "backward_ms": round(bw, 1),
         "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
@@ -293,117 +496,58 @@ def bench_pd14grad(args) -> int:
         "grad_rel_err_vs_fp64_subsample": err,
         "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
     }
-    print(json.dumps(line), flush=True)
-    return 0
 
 
-def bench_plain(args) -> int:
-    """The plain path: the full-scale PD14 microcircuit (the headline) or the 2500-neuron Brunel network.  Warmup, then exactly
-    --steps steps in graph windows, timed with HIP events on the launch stream; the CPU baselines on the
-    same network from the GPU's state."""
-    from naer2_amd.networks import brunel, pd14
-
-    kind = args.workload
-    t_build = time.perf_counter()
-    if kind == "pd14":
-        net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
-    else:
-        net, pops = brunel(seed=args.seed, device="cuda")
-    timesteps = {}
-    for spec in args.timestep or []:
-        name, _, ms = spec.partition("=")
-        if name not in pops:
-            raise SystemExit(f"--timestep: no population {name!r} (have {sorted(pops)})")
-        pops[name].timestep = timesteps[name] = float(ms)
-    rt = net.build()
-    torch.cuda.synchronize()
-    t_build = time.perf_counter() - t_build
-    print(f"[bench] built {kind}: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, "
-          f"{rt.d_slots} ring slots in {t_build:.1f} s", file=sys.stderr, flush=True)
-    net.record = True
-    rt._ensure_slots(args.warmup + args.steps + 400)
-
-    net.run(args.warmup, graph_window=args.window)
-    torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-
-    s0 = rt.step
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    net.run(args.steps, graph_window=args.window)
-    end.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(end)
-    s1 = rt.step
-    print(f"[bench] timed {args.steps} steps: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream",
-          file=sys.stderr, flush=True)
-
-    out_deg = rt.canon.out_degree()
-    events, spikes_timed = _events_and_spikes(rt, s0 - 1, s1 - 1, out_deg)
-    bio_s = args.steps * rt.dt / 1000.0
-    rtf = bio_s / wall
-    ms_per_step = wall * 1000.0 / args.steps
-
-    # roofline of the step kernel: algorithmic bytes per launch (sum over populations of n (8 n_state / k + 8)
-    # + 8 E + 8 S; n_state 3 for the exponential synapse: V, I, refractory counter; 2 for the delta synapse;
-    # k the population's steps per update, whose state is read and written once per k steps while its ring
-    # slots are read and cleared once each)
-    n_state = 3 if kind == "pd14" else 2
-    strides = [p.stride if p.stride > 1 else 1 for p in rt.params]
-    state_bytes = sum(pop.n * (8 * n_state / k + 8) for pop, k in zip(rt.pops, strides))
-    bytes_per_step = state_bytes + 8 * events / args.steps + 8 * spikes_timed / args.steps
-    k_ms = gpu_ms / args.steps  # HIP events on the launch stream over the timed region, per launch
-    achieved = bytes_per_step / (k_ms * 1e-3) / 1e9
-    traffic, occupancy = None, None
-    tfile = ROOT / "profiles" / "pd14_step_traffic.json"
-    if kind == "pd14" and tfile.exists() and not timesteps:
-        prof = json.loads(tfile.read_text())
-        traffic, occupancy = prof.get("hbm_bytes_per_launch"), prof.get("occupancy")
-
-    rates = net.rates(t_skip_ms=(s0 * rt.dt))
-    rates_hz = {name: round(rates[p], 3) for name, p in pops.items()}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baselines(net, rt, args, kind)
-    line = {
-        "metric": "PD14 real-time factor + synaptic events/s" if kind == "pd14" else
-                  "Brunel real-time factor + synaptic events/s",
-        "value": round(rtf, 4),
-        "unit": "bio_s/wall_s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 6),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": ("pd14 full scale: 77169 LIF-exp neurons, 8 populations, "
-                                f"{rt.canon.n_links} synapses, Poisson background" if kind == "pd14" else
-                                "brunel: N_E 2000, N_I 500 delta-synapse LIF, C_E 200, C_I 50, J 0.2 mV, "
-                                f"g 5, D 1.5 ms, {rt.canon.n_links} synapses, Poisson drive (eta 2)") +
-                               f", dt 0.1 ms, {args.steps} steps timed, seed {args.seed}",
-                   "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
-                   **({"timesteps_ms": timesteps} if timesteps else {}),
-                   "graph_window": args.window, "parallelism": "single GPU"},
-        "events_per_s": round(events / wall, 1),
-        "spikes_per_step": round(spikes_timed / args.steps, 3),
-        "events_per_step": round(events / args.steps, 1),
-        "rates_hz": rates_hz,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "kernel_avg_us": round(k_ms * 1000.0, 3), "occupancy": occupancy,
-                     "algorithmic_bytes_per_launch": round(bytes_per_step, 1)},
-        "cpu_baseline": cpu,
+def measure_bptt(args, iters: int, warmup: int) -> dict:
+    """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
+    1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
+    checkpoints every 25 steps.  Times forward and backward with HIP events (median of `iters` iterations
+    after `warmup`), and checks the gradients against the FP64 autograd unroll (same spikes)."""
+    from naer2_amd.rlif import RecurrentLIF, RLIFConfig, max_over_time_ce, poisson_input
+    B, C, H, O, T = 128, 700, 1024, 20, 100
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    cfg = RLIFConfig(checkpoint=25)
+    net = RecurrentLIF(C, H, O, cfg, generator=g)
+    x = poisson_input(B, C, T, 1.0, g)
+    labels = torch.randint(0, O, (B,), generator=g, device="cuda")
+    fw, bw = [], []
+    for it in range(warmup + iters):
+        net.zero_grad(set_to_none=True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        r, s = net(x)
+        loss = max_over_time_ce(r, labels)
+        e1.record()
+        loss.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if it >= warmup:
+            fw.append(e0.elapsed_time(e1))
+            bw.append(e1.elapsed_time(e2))
+    fwd_ms, bwd_ms = float(np.median(fw)), float(np.median(bw))
+    from oracle import rlif_oracle as ro
+    grads = [p.grad.detach() for p in (net.w_in, net.w_rec, net.w_out)]
+    w64 = [p.detach().double() for p in (net.w_in, net.w_rec, net.w_out)]
+    loss64, g64, _ = ro.grads(x.double(), *w64, labels, alpha=cfg.alpha, beta=cfg.beta, theta=cfg.theta,
+                              slope=cfg.slope, gamma=cfg.gamma, mode=0, forced=s.detach().double())
+    errs = {n: ro.rel_err(a, b) for n, a, b in zip(("w_in", "w_rec", "w_out"), grads, g64)}
+    flops = 2.0 * B * T * (C * H + H * H + H * O)
+    return {
+        "metric": "BPTT backward/forward cost ratio (surrogate-gradient recurrent LIF)",
+        "value": round(bwd_ms / fwd_ms, 4), "unit": "ratio", "n_gpus": 1, "steps": iters, "warmup": warmup,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "B=128, 700 Poisson inputs (0-100 Hz), 1024 recurrent LIF, 20 leaky readouts, "
+                               "T=100 x 1 ms, fast-sigmoid slope 25, checkpoints every 25 steps"},
+        "forward_ms": round(fwd_ms, 4), "backward_ms": round(bwd_ms, 4),
+        "forward_dense_equiv_tflops": round(flops / (fwd_ms * 1e-3) / 1e12, 3),
+        "spike_fraction": round(float(s.mean()), 5),
+        "grad_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "grad_rel_err_max": float(f"{max(errs.values()):.3e}"),
+        "loss": float(loss.detach()), "loss_fp64": float(loss64),
     }
-    print(json.dumps(line), flush=True)
-    return 0
 
+
+# ----------------------------------------------------------------------------------------------- event path
 
 def _event_oracle_net(net, rt):
     """The oracle's description of a built event-driven network (host copies of its canonical table)."""
@@ -428,7 +572,7 @@ def _event_oracle_net(net, rt):
                     rt.canon.w.cpu().numpy(), seed=int(net.seed), instance=int(net.instance))
 
 
-def bench_brunel_exact(args) -> int:
+def measure_brunel_exact(args) -> dict:
     """The event-driven, exact-time Brunel model A (the reference's own Brunel network, test/networks/Brunel):
     N = N_E + N_I exact-time delta LIF neurons and N_E Poisson sources, fixed in-degree C_E = 0.1 N_E, one delay
     D = 1.5 ms.  A step is one window of D (a deliver and a sort launch); --ev-warmup windows, then
@@ -445,8 +589,8 @@ def bench_brunel_exact(args) -> int:
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
     W = rt.window
-    print(f"[bench] built brunel_exact: {rt.n} nodes ({N} neurons), {rt.n_links} synapses, {rt.n_groups} groups "
-          f"in {t_build:.1f} s", file=sys.stderr, flush=True)
+    log(f"built brunel_exact: {rt.n} nodes ({N} neurons), {rt.n_links} synapses, {rt.n_groups} groups in "
+        f"{t_build:.1f} s")
     gw = args.ev_graph
     warm, n_win = args.ev_warmup, args.ev_windows
     rt.run(warm - 1, gw)
@@ -456,7 +600,6 @@ def bench_brunel_exact(args) -> int:
     torch.cuda.synchronize()
     prev_t, prev_n = rt.spikes()
     snap = (rt.v.clone(), rt.last.clone(), rt.tre.clone(), rt.nxt.clone())
-    print(f"[bench] warmup {warm} windows done", file=sys.stderr, flush=True)
     c0 = rt.counters()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -466,15 +609,12 @@ def bench_brunel_exact(args) -> int:
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(end)
     c1 = rt.counters()
     rt.check()
     events, spikes = c1[0] - c0[0], c1[1] - c0[1]
     bio_s = n_win * W / 1000.0
     rtf = bio_s / wall
-    print(f"[bench] timed {n_win} windows: {wall:.3f} s wall, {gpu_ms:.1f} ms on the stream", file=sys.stderr,
-          flush=True)
-    # the deliver kernel alone: HIP events around each kernel of a split run
+    log(f"brunel_exact: timed {n_win} windows: {wall:.3f} s wall")
     n_split = min(100, n_win)
     c2 = rt.counters()
     d_ms, s_ms = rt.run_timed(n_split)
@@ -508,8 +648,7 @@ def bench_brunel_exact(args) -> int:
         cpu = {"value": round(k * W / 1000.0 / tc, 6), "unit": "bio_s/wall_s", "cores": 1, "kind": "port",
                "sample": f"{k} windows ({k * W:.1f} ms bio) of the same network from the GPU's state after warmup, "
                          f"serial C oracle (oracle/event_oracle.c), {ev_cpu} inputs, {tc:.2f} s",
-               "events_per_s": round(ev_cpu / tc, 1), "threads_available": os.cpu_count()}
-    # steady-state rates: 100 ms more, recorded
+               "events_per_s": round(ev_cpu / tc, 1)}
     t_rec = rt.window_index * W
     net.record = True
     rt.run(67, 0)
@@ -517,19 +656,11 @@ def bench_brunel_exact(args) -> int:
     r = rt.rates(t_skip_ms=t_rec)
     rates = {name: r[p] for name, p in pops.items()}
     ref_rtf = 1.0 / 2.94
-    line = {
+    return {
         "metric": "event-driven Brunel real-time factor + synaptic events/s",
-        "value": round(rtf, 4),
-        "unit": "bio_s/wall_s",
-        "n_gpus": 1,
-        "steps": n_win,
-        "warmup": warm,
-        "ms_per_step": round(wall * 1000.0 / n_win, 6),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": round(rtf / ref_rtf, 2) if N == 50000 else None,
-        "dtype": "f32",
-        "data": "synthetic",
+        "value": round(rtf, 4), "unit": "bio_s/wall_s", "n_gpus": 1, "steps": n_win, "warmup": warm,
+        "ms_per_step": round(wall * 1000.0 / n_win, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(rtf / ref_rtf, 2) if N == 50000 else None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"brunel_exact: event-driven Brunel model A with exact spike times, N {N} "
                                f"(N_E {pops['E'].n}, N_I {pops['I'].n}), {pops['P'].n} Poisson sources at "
                                f"{brunel_exact_theory(N)['rate_ext_Hz']:.1f} Hz, C_E {brunel_exact_theory(N)['C_E']}, "
@@ -539,8 +670,7 @@ def bench_brunel_exact(args) -> int:
                    "parallelism": "single GPU"},
         "vs_baseline_note": "reference: N = 50,000 at 2.94 ms wall per simulated ms on an RTX 4090, captured "
                             "(test/networks/Brunel/net.py:77-78)",
-        "events_per_s": round(events / wall, 1),
-        "spikes_per_window": round(spikes / n_win, 1),
+        "events_per_s": round(events / wall, 1), "spikes_per_window": round(spikes / n_win, 1),
         "events_per_window": round(events / n_win, 1),
         "rates_hz": {name: round(float(v), 3) for name, v in rates.items()},
         "kernel_split_ms_per_window": {"deliver": round(d_ms / n_split, 4), "sort": round(s_ms / n_split, 4)},
@@ -550,79 +680,65 @@ def bench_brunel_exact(args) -> int:
                      "occupancy": occupancy, "algorithmic_bytes_per_launch": round(bytes_win, 1)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
 
 
-def bench_bptt(args) -> int:
-    """Surrogate-gradient BPTT: B=128 Poisson inputs on 700 channels (rates U[0, 100] Hz), T=100 steps of
-    1 ms, 1024 recurrent LIF units, 20 leaky readouts, cross-entropy of the max-over-time readout,
-    checkpoints every 25 steps.  Times forward and backward with HIP events (median of --steps iterations
-    after --warmup), and checks the gradients against the FP64 autograd unroll (same spikes)."""
-    from naer2_amd.rlif import RecurrentLIF, RLIFConfig, max_over_time_ce, poisson_input
-    B, C, H, O, T = 128, 700, 1024, 20, 100
-    g = torch.Generator(device="cuda").manual_seed(args.seed)
-    cfg = RLIFConfig(checkpoint=25)
-    net = RecurrentLIF(C, H, O, cfg, generator=g)
-    x = poisson_input(B, C, T, 1.0, g)
-    labels = torch.randint(0, O, (B,), generator=g, device="cuda")
-    fw, bw = [], []
-    iters = max(1, args.steps)
-    for it in range(args.warmup + iters):
-        net.zero_grad(set_to_none=True)
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record()
-        r, s = net(x)
-        loss = max_over_time_ce(r, labels)
-        e1.record()
-        loss.backward()
-        e2.record()
-        torch.cuda.synchronize()
-        if it >= args.warmup:
-            fw.append(e0.elapsed_time(e1))
-            bw.append(e1.elapsed_time(e2))
-    fwd_ms, bwd_ms = float(np.median(fw)), float(np.median(bw))
-    from oracle import rlif_oracle as ro
-    grads = [p.grad.detach() for p in (net.w_in, net.w_rec, net.w_out)]
-    w64 = [p.detach().double() for p in (net.w_in, net.w_rec, net.w_out)]
-    loss64, g64, _ = ro.grads(x.double(), *w64, labels, alpha=cfg.alpha, beta=cfg.beta, theta=cfg.theta,
-                              slope=cfg.slope, gamma=cfg.gamma, mode=0, forced=s.detach().double())
-    errs = {n: ro.rel_err(a, b) for n, a, b in zip(("w_in", "w_rec", "w_out"), grads, g64)}
-    flops = 2.0 * B * T * (C * H + H * H + H * O)
-    line = {
-        "metric": "BPTT backward/forward cost ratio (surrogate-gradient recurrent LIF)",
-        "value": round(bwd_ms / fwd_ms, 4), "unit": "ratio", "n_gpus": 1, "steps": iters, "warmup": args.warmup,
-        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "B=128, 700 Poisson inputs (0-100 Hz), 1024 recurrent LIF, 20 leaky readouts, "
-                               "T=100 x 1 ms, fast-sigmoid slope 25, checkpoints every 25 steps"},
-        "forward_ms": round(fwd_ms, 4), "backward_ms": round(bwd_ms, 4),
-        "forward_dense_equiv_tflops": round(flops / (fwd_ms * 1e-3) / 1e12, 3),
-        "spike_fraction": round(float(s.mean()), 5),
-        "grad_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
-        "loss": float(loss), "loss_fp64": float(loss64),
-    }
-    print(json.dumps(line), flush=True)
-    return 0
+# ----------------------------------------------------------------------------------------------- main
+
+# rough seconds each configuration takes at its full size (builds included), for the time budget
+COST_S = {"brunel": 40.0, "bptt": 20.0, "pd14grad": 90.0, "sweep": 240.0}
+
+
+def configs(args, cpu: dict) -> dict:
+    """The other configurations at their full sizes, in order of cost, each within the time budget."""
+    out = {}
+    plan = [
+        ("config0_brunel", "brunel", lambda: measure_plain(args, "brunel", 10000, 1000, not args.no_cpu, cpu)),
+        ("config3_bptt", "bptt", lambda: measure_bptt(args, 5, 2)),
+        ("config4_pd14grad", "pd14grad", lambda: measure_pd14grad(args, 5000, 4)),
+        ("config2_sweep", "sweep", lambda: measure_sweep(args, 10000, 1000, 100, 256)),
+    ]
+    for name, key, fn in plan:
+        left = args.budget_s - (time.perf_counter() - T_START)
+        if left < COST_S[key]:
+            out[name] = {"skipped": f"time budget: {left:.0f} s left of --budget-s {args.budget_s:g}, "
+                                    f"about {COST_S[key]:g} s needed"}
+            log(f"{name}: skipped ({out[name]['skipped']})")
+            continue
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as e:  # one configuration failing must not hide the headline line
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: {type(e).__name__}: {e}")
+        out[name]["seconds"] = round(time.perf_counter() - t0, 1)
+        torch.cuda.empty_cache()
+        log(f"{name}: done in {out[name]['seconds']} s")
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 10000; pd14grad: 5000, i.e. 500 ms of bio time)")
+                    help="timed steps (default 10000 = 1 s of bio time; pd14grad: 5000)")
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--window", type=int, default=100, help="steps per captured hipGraph")
+    ap.add_argument("--window", type=int, default=100, help="steps per captured hipGraph of long runs")
     ap.add_argument("--cpu-steps", type=int, default=2000,
-                    help="scales the CPU samples (PD14: 2000 steps on one thread, 10x that on every core)")
+                    help="scales the CPU samples (PD14: 2000 steps on one thread, 1000 per thread on every CPU)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="pd14: leave out the other configurations (`configs` in the line)")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="pd14: wall seconds from start within which the other configurations are run")
     ap.add_argument("--timestep", action="append", metavar="POP=MS",
                     help="pd14/brunel: a population's own timestep, a whole multiple of dt (repeatable)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad", "brunel_exact"],
                     default="pd14",
-                    help="pd14: the headline PD14 real-time factor; brunel: the 2500-neuron Brunel network; sweep: the batched "
-                         "Brunel sweep; bptt: the recurrent-LIF BPTT; pd14grad: the batched differentiable PD14; "
-                         "brunel_exact: the event-driven Brunel model with exact spike times")
+                    help="pd14: the headline PD14 real-time factor (with the other configurations beside it); "
+                         "brunel: the 2500-neuron Brunel network; sweep: the batched Brunel sweep; bptt: the "
+                         "recurrent-LIF BPTT; pd14grad: the batched differentiable PD14; brunel_exact: the "
+                         "event-driven Brunel model with exact spike times")
     ap.add_argument("--neurons", type=int, default=None, help="brunel_exact: N = N_E + N_I (default 50,000)")
     ap.add_argument("--ev-windows", type=int, default=667, help="brunel_exact: timed windows of 1.5 ms")
     ap.add_argument("--ev-warmup", type=int, default=67, help="brunel_exact: untimed windows first")
@@ -646,8 +762,18 @@ def main() -> int:
                     help="only run 5 plain device-to-device copies of MiB each (the reference kernel that scales "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes), then exit")
     args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU: this build measures one MI355X (the spiking-network step would shard by "
+              "target groups, with one all-gather of each step's spike lists; not built)", file=sys.stderr)
+        return 2
     if args.steps is None:
         args.steps = 5000 if args.workload == "pd14grad" else 10000
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be positive and --warmup non-negative", file=sys.stderr)
+        return 2
+    if args.window <= 0 or args.window % 4:
+        print("--window must be a positive multiple of 4 (the spike lists' phase)", file=sys.stderr)
+        return 2
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -657,23 +783,24 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n}))
         return 0
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU: the spiking-network step has no multi-GPU path in this build (it would "
-              "shard by target groups, with one all-gather of the step's spike lists)", file=sys.stderr)
-        return 2
+    cpu = host_cpu()
+    log(f"host: {cpu}")
     if args.workload == "bptt":
-        return bench_bptt(args)
-    if args.workload == "pd14grad":
-        return bench_pd14grad(args)
-    if args.workload == "brunel_exact":
-        return bench_brunel_exact(args)
-    if args.window <= 0 or args.window % 4 or args.steps % args.window or args.warmup % args.window:
-        print("--window must be a multiple of 4, --steps and --warmup multiples of --window", file=sys.stderr)
-        return 2
-    if args.workload == "sweep":
-        return bench_sweep(args)
-
-    return bench_plain(args)
+        line = measure_bptt(args, max(1, args.steps if args.steps < 1000 else 5), min(args.warmup, 2))
+    elif args.workload == "pd14grad":
+        line = measure_pd14grad(args, args.steps, args.instances or 4)
+    elif args.workload == "brunel_exact":
+        line = measure_brunel_exact(args)
+    elif args.workload == "sweep":
+        line = measure_sweep(args, args.steps, max(args.warmup, args.window), args.window, args.instances or 256)
+    else:
+        line = measure_plain(args, args.workload, args.steps, args.warmup, not args.no_cpu, cpu)
+        torch.cuda.empty_cache()
+        if args.workload == "pd14" and not args.no_secondary:
+            line["configs"] = configs(args, cpu)
+        line["bench_wall_s"] = round(time.perf_counter() - T_START, 1)
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

@@ -175,25 +175,33 @@ int n2h_batch_graph_create(n2h_batch* batch, int32_t n_steps, void* stream, n2h_
 
 /* Dense recurrent LIF layer for surrogate-gradient BPTT (no counterpart in the reference, which has no
  * autograd: a surrogate-gradient adjoint beside the plain path).  Model and adjoint: naer2_amd/csrc/rlif.hip.
+ * All arithmetic is fp32; the state is compensated (each of I and V an unevaluated sum hi + lo of two floats).
  * Buffers are fp32, row-major: u [T][B][H] (= x W_in^T), w_rec [H][H], w_rec_t = its transpose,
- * spikes [T][B][H]; shadow: double [2][B][H], the forward's double-precision copy of (I, V);
- * checkpoints: double [ceil(T/K)][2][B][H], the shadow before steps 0, K, 2K, ... */
+ * spikes [T][B][H]; state [4][B][H] = (I hi, I lo, V hi, V lo) after the last step;
+ * checkpoints [ceil(T/K)][4][B][H], the state before steps 0, K, 2K, ... */
 typedef struct n2h_rlif {
     int32_t batch, n_hidden, n_steps, checkpoint;  /* checkpoint: K steps between saved states */
     int32_t mode;                                   /* 0: s = H(V - theta); 1: s = f(V - theta), smooth (tests) */
     float alpha, beta, theta, slope, gamma;         /* synaptic / membrane decay, threshold, surrogate slope, scale */
+    float alpha_lo, beta_lo;                        /* the decays' remainders: alpha = alpha + alpha_lo exactly
+                                                       enough for the compensated state (0 for fp32 decays) */
 } n2h_rlif;
 int n2h_rlif_last_error(char* buf, size_t len);
-int n2h_rlif_forward(const n2h_rlif* layer, const float* u, const float* w_rec_t, float* i_state, float* v_state,
-                     double* shadow, float* spikes, double* checkpoints, void* stream);
+int n2h_rlif_forward(const n2h_rlif* layer, const float* u, const float* w_rec_t, float* state, float* spikes,
+                     float* checkpoints, void* stream);
 int n2h_rlif_readout(int32_t n_steps, int64_t bo, float alpha, float beta, const float* z, float* r, void* stream);
 int n2h_rlif_readout_adjoint(int32_t n_steps, int64_t bo, float alpha, float beta, const float* g_r, float* g_j,
                              void* stream);
+/* The adjoint of the recurrent delivery alone: m [B][H] = g_i [B][H] w_rec [H][H], read through
+ * w_rec_t = w_rec^T (hand-written, on the matrix cores; the product every reverse step of n2h_rlif_backward
+ * computes).  n_hidden must be a multiple of 16. */
+int n2h_rlif_delivery_adjoint(const n2h_rlif* layer, const float* g_i, const float* w_rec_t, float* m, void* stream);
+/* Floats of n2h_rlif_backward's work area: (5 + min(K, T)) * B * H (-1: bad layer). */
+int64_t n2h_rlif_work_floats(const n2h_rlif* layer);
 /* gso = W_out^T gJ per step [T][B][H]; writes gI_t for every step to g_i_all [T][B][H];
- * work: (7 + 2 K) * B * H floats. */
-int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec, const float* w_rec_t,
-                      const float* spikes, const double* checkpoints, const float* gso, float* g_i_all, float* work,
-                      void* stream);
+ * work: n2h_rlif_work_floats(layer) floats of device memory.  n_hidden must be a multiple of 16. */
+int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec_t, const float* spikes,
+                      const float* checkpoints, const float* gso, float* g_i_all, float* work, void* stream);
 
 /* Row intake (the target-major pull): each target's in-links in contract order (ascending canonical link
  * index).  Per group a block of 16-byte cells [row_depth[g]][64]: cell q of lane l (cell index

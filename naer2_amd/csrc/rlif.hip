@@ -13,28 +13,38 @@
 //   gs_t = W_out^T gJ_t + W_rec^T gI_{t+1} - theta gV_{t+1}
 //   gV_t = gs_t ds/dV(V_t) + beta gV_{t+1};   gI_t = gV_t + alpha gI_{t+1}
 //
-// The recurrent delivery W_rec s_{t-1} is event-driven: a workgroup compacts the nonzero entries of
-// s_{t-1}[b] (ascending) and each lane sums the matching rows of W_rec^T in that order.  The dense
-// adjoint product W_rec^T gI_{t+1} is a plain GEMM (rocBLAS) per step, followed by the fused elementwise
-// adjoint.  The forward carries a double-precision shadow of (I, V) beside its fp32 state (which decides the
-// spikes); every K steps the shadow is checkpointed, and the backward pass recomputes the shadow's V inside
-// each segment from its checkpoint and the stored spikes -- the same trajectory whatever K -- for the
-// surrogate derivative.
+// Precision.  The surrogate's relative sensitivity to V reaches 2 slope (50) at the threshold, so the
+// state is carried in compensated fp32: I and V are unevaluated sums hi + lo of two floats (double-float
+// arithmetic with error-free TwoSum / TwoProd transformations; Dekker 1971, Knuth TAOCP 4.2.2), the
+// recurrent sum is a compensated (Kahan-Babuska style) sum, and the decays are passed as hi + lo pairs.
+// That carries about 44 bits of the potential with fp32 arithmetic only; spikes are decided, and the
+// surrogate evaluated, on x = (V_hi - theta) + V_lo.
+//
+// Kernels:
+//   step_kernel<false>   one forward step: compacts the nonzero entries of s_{t-1}[b] (ascending) and sums
+//                        the matching rows of W_rec^T in that order (event-driven recurrent delivery), then
+//                        the LIF update; every K steps the state is checkpointed (4 floats a unit).
+//   step_kernel<true>    the same step in the backward pass's recompute, from a checkpoint and the stored
+//                        spikes (not re-decided); writes the surrogate derivative h_t of every unit.
+//   reverse_kernel       one reverse step: the adjoint of the recurrent delivery m = gI_{t+1} W_rec on the
+//                        matrix cores (v_mfma_f32_16x16x4f32, 16x16 output tiles, K split over the four
+//                        waves of a workgroup and reduced through LDS in a fixed order), fused with the
+//                        elementwise LIF adjoint above.  Deterministic: the same result for any checkpoint
+//                        interval and any launch.
 // There is no counterpart in the reference: autograd and BPTT do not exist in naer2.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
-#include <new>
 
 #include "naer2_hip.h"
 
 namespace {
 
-constexpr int kTile = 256;  // hidden units per workgroup
+constexpr int kTile = 256;     // hidden units per workgroup of the step kernels
+constexpr int kMM = 16;        // output tile edge of the reverse kernel
 
 thread_local char r_err[512] = "";
 
@@ -51,6 +61,41 @@ int rfail(int code, const char* fmt, ...) {
         hipError_t e_ = (call);                                                                  \
         if (e_ != hipSuccess) return rfail(N2H_EHIP, "%s: %s", #call, hipGetErrorString(e_));   \
     } while (0)
+
+// ---- double-float arithmetic (every operation rounds on its own: the library builds with
+//      -ffp-contract=off, and the one fused multiply-add is written out where the method needs it)
+
+struct df {
+    float hi, lo;
+};
+
+__device__ __forceinline__ df two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+
+__device__ __forceinline__ df fast_two_sum(float a, float b) {  // |a| >= |b| or a == 0
+    const float s = a + b;
+    return {s, b - (s - a)};
+}
+
+__device__ __forceinline__ df two_prod(float a, float b) {
+    const float p = a * b;
+    return {p, __builtin_fmaf(a, b, -p)};
+}
+
+__device__ __forceinline__ df df_add(df a, df b) {
+    df s = two_sum(a.hi, b.hi);
+    const float e = s.lo + (a.lo + b.lo);
+    return fast_two_sum(s.hi, e);
+}
+
+__device__ __forceinline__ df df_mul(df a, df b) {
+    df p = two_prod(a.hi, b.hi);
+    const float e = p.lo + (a.hi * b.lo + a.lo * b.hi);
+    return fast_two_sum(p.hi, e);
+}
 
 __device__ __forceinline__ float spike_fn(float x, int mode, float slope) {
     if (mode == 0) return x > 0.0f ? 1.0f : 0.0f;
@@ -95,20 +140,14 @@ __device__ __forceinline__ int compact_row(const float* __restrict__ s_row, int 
     return n_act;
 }
 
-// The double-precision shadow of one step (identical expression in the forward and the recompute).
-__device__ __forceinline__ void shadow_update(const n2h_rlif& p, double rec, float u, float sp, double& I, double& V) {
-    I = static_cast<double>(p.alpha) * I + (static_cast<double>(u) + rec);
-    V = (static_cast<double>(p.beta) * V + I) - static_cast<double>(p.theta) * static_cast<double>(sp);
-}
-
-// One forward step for batch row b and hidden units [tile*256, tile*256 + 256): the fp32 state decides the
-// spikes; a double shadow of (I, V) runs alongside and is what the checkpoints keep, so that the
-// backward's recompute reproduces one continuous double trajectory whatever the checkpoint interval.
-__global__ __launch_bounds__(kTile) void fwd_step_kernel(n2h_rlif p, const float* __restrict__ u_t,
-                                                         const float* __restrict__ w_rec_t,
-                                                         float* __restrict__ i_state, float* __restrict__ v_state,
-                                                         double* __restrict__ shadow, const float* __restrict__ s_prev,
-                                                         float* __restrict__ s_out, double* __restrict__ ckpt) {
+// One step for batch row b and hidden units [tile*256, tile*256 + 256).  state: [4][B][H] floats
+// (I hi, I lo, V hi, V lo).  Forward (kRecompute false): decides s_t into out, checkpoints the state before
+// the step into ckpt when given.  Recompute: the forward's spikes s_{t-1} come in, h_t = ds/dV goes out.
+template <bool kRecompute>
+__global__ __launch_bounds__(kTile) void step_kernel(n2h_rlif p, const float* __restrict__ u_t,
+                                                     const float* __restrict__ w_rec_t, float* __restrict__ state,
+                                                     const float* __restrict__ s_prev, float* __restrict__ out,
+                                                     float* __restrict__ ckpt) {
     extern __shared__ unsigned char smem[];
     int* act_idx = reinterpret_cast<int*>(smem);                    // [H]
     float* act_val = reinterpret_cast<float*>(smem) + p.n_hidden;  // [H]
@@ -120,83 +159,121 @@ __global__ __launch_bounds__(kTile) void fwd_step_kernel(n2h_rlif p, const float
     const int64_t bh = static_cast<int64_t>(p.batch) * H;
     const int n_act = s_prev ? compact_row(s_prev + row, H, act_idx, act_val, wave_base) : 0;
     if (h >= H) return;
-    float rec = 0.0f;
-    double rec64 = 0.0;
-    for (int q = 0; q < n_act; ++q) {
-        const float w = w_rec_t[static_cast<int64_t>(act_idx[q]) * H + h];
-        rec = rec + w * act_val[q];
-        rec64 = rec64 + static_cast<double>(w) * static_cast<double>(act_val[q]);
+    // compensated recurrent sum, in ascending presynaptic order
+    float rs = 0.0f, rc = 0.0f;
+    if (p.mode == 0) {
+        for (int q = 0; q < n_act; ++q) {
+            const df t = two_sum(rs, w_rec_t[static_cast<int64_t>(act_idx[q]) * H + h]);
+            rs = t.hi;
+            rc = rc + t.lo;
+        }
+    } else {
+        for (int q = 0; q < n_act; ++q) {
+            const df pr = two_prod(w_rec_t[static_cast<int64_t>(act_idx[q]) * H + h], act_val[q]);
+            const df t = two_sum(rs, pr.hi);
+            rs = t.hi;
+            rc = rc + (t.lo + pr.lo);
+        }
     }
     const int64_t k = row + h;
-    double I64 = shadow[k], V64 = shadow[bh + k];
-    if (ckpt) {  // the segment's starting state
-        ckpt[k] = I64;
-        ckpt[bh + k] = V64;
+    df I{state[k], state[bh + k]}, V{state[2 * bh + k], state[3 * bh + k]};
+    if (!kRecompute && ckpt) {  // the segment's starting state
+        ckpt[k] = I.hi;
+        ckpt[bh + k] = I.lo;
+        ckpt[2 * bh + k] = V.hi;
+        ckpt[3 * bh + k] = V.lo;
     }
     const float sp = s_prev ? s_prev[k] : 0.0f;
-    const float a = u_t[k] + rec;
-    const float I = p.alpha * i_state[k] + a;
-    const float V = (p.beta * v_state[k] + I) - p.theta * sp;
-    i_state[k] = I;
-    v_state[k] = V;
-    s_out[k] = spike_fn(V - p.theta, p.mode, p.slope);
-    shadow_update(p, rec64, u_t[k], sp, I64, V64);
-    shadow[k] = I64;
-    shadow[bh + k] = V64;
+    const df a = df_add(two_sum(u_t[k], rs), df{rc, 0.0f});
+    I = df_add(df_mul(df{p.alpha, p.alpha_lo}, I), a);
+    V = df_add(df_mul(df{p.beta, p.beta_lo}, V), I);
+    V = df_add(V, two_prod(-p.theta, sp));
+    state[k] = I.hi;
+    state[bh + k] = I.lo;
+    state[2 * bh + k] = V.hi;
+    state[3 * bh + k] = V.lo;
+    const float x = (V.hi - p.theta) + V.lo;
+    out[k] = kRecompute ? surrogate(x, p.slope, p.gamma) : spike_fn(x, p.mode, p.slope);
 }
 
-// Recompute of one step for the backward pass: the double shadow of fwd_step_kernel, with the forward's
-// spikes (not re-decided).  The surrogate derivative's relative sensitivity to V reaches 2 slope (50) at
-// the threshold, so it is evaluated on this trajectory rather than on the fp32 one.
-__global__ __launch_bounds__(kTile) void recompute_step_kernel(n2h_rlif p, const float* __restrict__ u_t,
-                                                               const float* __restrict__ w_rec_t,
-                                                               double* __restrict__ shadow,
-                                                               const float* __restrict__ s_prev,
-                                                               double* __restrict__ v_out) {
-    extern __shared__ unsigned char smem[];
-    int* act_idx = reinterpret_cast<int*>(smem);
-    float* act_val = reinterpret_cast<float*>(smem) + p.n_hidden;
-    __shared__ int wave_base[kTile / 64 + 1];
-    const int H = p.n_hidden;
-    const int b = blockIdx.y;
-    const int h = blockIdx.x * kTile + threadIdx.x;
-    const int64_t row = static_cast<int64_t>(b) * H;
-    const int64_t bh = static_cast<int64_t>(p.batch) * H;
-    const int n_act = s_prev ? compact_row(s_prev + row, H, act_idx, act_val, wave_base) : 0;
-    if (h >= H) return;
-    double rec64 = 0.0;
-    for (int q = 0; q < n_act; ++q)
-        rec64 = rec64 + static_cast<double>(w_rec_t[static_cast<int64_t>(act_idx[q]) * H + h]) *
-                            static_cast<double>(act_val[q]);
-    const int64_t k = row + h;
-    double I64 = shadow[k], V64 = shadow[bh + k];
-    shadow_update(p, rec64, u_t[k], s_prev ? s_prev[k] : 0.0f, I64, V64);
-    shadow[k] = I64;
-    shadow[bh + k] = V64;
-    v_out[k] = V64;
-}
-
-__global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k < n) dst[k] = src[k];
-}
-
-// Elementwise adjoint of one step: gs = gso + m - theta gV_{t+1}; gV = gs s'(V) + beta gV_{t+1};
-// gI = gV + alpha gI_{t+1}.  gv / gi hold step t+1's values on entry and step t's on return.
-__global__ void adj_step_kernel(n2h_rlif p, int64_t n, const double* __restrict__ v_t,
-                                const float* __restrict__ gso_t, const float* __restrict__ m_t,
-                                float* __restrict__ gv, float* __restrict__ gi, float* __restrict__ gi_out) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const float gv1 = gv[k], gi1 = gi[k];
-    const float gs = (gso_t[k] + (m_t ? m_t[k] : 0.0f)) - p.theta * gv1;
-    const double x = v_t[k] - static_cast<double>(p.theta);
-    const double d = 1.0 + static_cast<double>(p.slope) * fabs(x);
-    const float sg = static_cast<float>(static_cast<double>(p.gamma) / (d * d));
-    const float gV = gs * sg + p.beta * gv1;
+// One reverse step, for the 16x16 output tile (rows b0.., columns j0..) of m = gI_{t+1} W_rec and the
+// elementwise adjoint over it.  gi_next = gI_{t+1} [B][H] (null at t = T-1: m = 0, gV_{t+1} = gI_{t+1} = 0);
+// h_t the surrogate derivatives; gso_t = W_out^T gJ_t; gv [B][H] holds gV_{t+1} on entry and gV_t on
+// return; gi_out = gI_t.  epilogue false: writes m itself to gi_out (the delivery adjoint alone).
+// The product reads W_rec through its transpose w_rec_t [H][H] (m[b][j] = sum_k gI[b][k] w_rec_t[j][k]), so
+// that both operands are rows: H is a multiple of 16, split into chunks of 16 k; wave w takes chunks
+// [w nc / 4, (w + 1) nc / 4) of the nc = H / 16.  Lane l (row i = l % 16, quarter g = l / 16) loads the
+// float4 k = 16 c + 4 g .. + 3 of gI row b0 + i and of w_rec_t row j0 + i; MFMA (c, q) then takes k =
+// 16 c + 4 g + q in lane quarter g, as v_mfma_f32_16x16x4f32 wants (A[i][k] from lane i + 16 k', B[k][j]
+// from lane j + 16 k', D[4 (l / 16) + r][l % 16] in accumulator component r).  Eight chunks are loaded
+// before their 32 MFMAs.  The four waves' partial tiles are added in wave order through LDS.
+template <bool kEpilogue>
+__global__ __launch_bounds__(256) void reverse_kernel(n2h_rlif p, const float* __restrict__ gi_next,
+                                                      const float* __restrict__ w_rec_t, const float* __restrict__ h_t,
+                                                      const float* __restrict__ gso_t, float* __restrict__ gv,
+                                                      float* __restrict__ gi_out) {
+    __shared__ float part[4][kMM][kMM + 1];
+    const int H = p.n_hidden, B = p.batch;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j0 = blockIdx.x * kMM, b0 = blockIdx.y * kMM;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (gi_next) {
+        const int i = lane & 15, g = lane >> 4;
+        const int nc = H >> 4;
+        const int c0 = nc * wave / 4, c1 = nc * (wave + 1) / 4;
+        // rows past the batch are clamped: they only feed output rows that are not written
+        const int brow = b0 + i < B ? b0 + i : B - 1;
+        const f4* arow = reinterpret_cast<const f4*>(gi_next + static_cast<int64_t>(brow) * H) + g;
+        const f4* brw = reinterpret_cast<const f4*>(w_rec_t + static_cast<int64_t>(j0 + i) * H) + g;
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            f4 a[8], bv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                a[u] = arow[4 * (c + u)];
+                bv[u] = brw[4 * (c + u)];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // all sixteen loads in flight before the first MFMA waits
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][0], bv[u][0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][1], bv[u][1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][2], bv[u][2], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][3], bv[u][3], acc, 0, 0, 0);
+            }
+        }
+        for (; c < c1; ++c) {
+            const f4 a = arow[4 * c], bv = brw[4 * c];
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bv[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bv[1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bv[2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bv[3], acc, 0, 0, 0);
+        }
+    }
+    {
+        const int col = lane & 15, r0 = 4 * (lane >> 4);
+        part[wave][r0 + 0][col] = acc[0];
+        part[wave][r0 + 1][col] = acc[1];
+        part[wave][r0 + 2][col] = acc[2];
+        part[wave][r0 + 3][col] = acc[3];
+    }
+    __syncthreads();
+    const int ii = threadIdx.x >> 4, jj = threadIdx.x & 15;
+    const int b = b0 + ii, j = j0 + jj;
+    if (b >= B) return;
+    const float m = ((part[0][ii][jj] + part[1][ii][jj]) + part[2][ii][jj]) + part[3][ii][jj];
+    const int64_t k = static_cast<int64_t>(b) * H + j;
+    if (!kEpilogue) {
+        gi_out[k] = m;
+        return;
+    }
+    const float gv1 = gi_next ? gv[k] : 0.0f;
+    const float gi1 = gi_next ? gi_next[k] : 0.0f;
+    const float gs = (gso_t[k] + m) - p.theta * gv1;
+    const float gV = gs * h_t[k] + p.beta * gv1;
     const float gI = gV + p.alpha * gi1;
     gv[k] = gV;
-    gi[k] = gI;
     gi_out[k] = gI;
 }
 
@@ -231,6 +308,11 @@ __global__ void zero_kernel(uint32_t* __restrict__ p, int64_t n) {
     if (k < n) p[k] = 0u;
 }
 
+__global__ void copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
 // Clear n 32-bit words with a kernel rather than hipMemsetAsync, so that a caller may capture these calls
 // into a graph.
 int zero(void* ptr, int64_t n_words, hipStream_t st) {
@@ -244,12 +326,59 @@ int check(const n2h_rlif* p) {
     if (p->batch <= 0 || p->n_hidden <= 0 || p->n_steps <= 0) return rfail(N2H_EINVAL, "sizes must be positive");
     if (p->mode != 0 && p->mode != 1) return rfail(N2H_EINVAL, "mode must be 0 or 1");
     if (p->checkpoint <= 0) return rfail(N2H_EINVAL, "checkpoint interval must be positive");
+    if (p->batch > 65535) return rfail(N2H_EINVAL, "batch %d above 65535", p->batch);
+    if (p->n_hidden > 8192) return rfail(N2H_EINVAL, "n_hidden %d above 8192", p->n_hidden);
     return N2H_OK;
 }
 
-size_t fwd_lds(const n2h_rlif* p) { return sizeof(float) * 2 * static_cast<size_t>(p->n_hidden); }
+size_t step_lds(const n2h_rlif* p) { return sizeof(float) * 2 * static_cast<size_t>(p->n_hidden); }
 
 dim3 step_grid(const n2h_rlif* p) { return dim3((p->n_hidden + kTile - 1) / kTile, p->batch); }
+
+dim3 reverse_grid(const n2h_rlif* p) { return dim3((p->n_hidden + kMM - 1) / kMM, (p->batch + kMM - 1) / kMM); }
+
+int64_t work_floats(const n2h_rlif* p) {
+    const int32_t K = p->checkpoint < p->n_steps ? p->checkpoint : p->n_steps;
+    return (5 + static_cast<int64_t>(K)) * p->batch * p->n_hidden;
+}
+
+// The backward pass over the hidden layer with `work` as its scratch ((5 + K) B H floats: the recomputed
+// state [4][B][H], gV [B][H], the segment's surrogate derivatives [K][B][H]).
+int backward(const n2h_rlif* p, const float* u, const float* w_rec_t, const float* spikes,
+             const float* ckpt, const float* gso, float* g_i_all, float* work, hipStream_t st) {
+    const int H = p->n_hidden, B = p->batch, T = p->n_steps, K = p->checkpoint;
+    const int64_t bh = static_cast<int64_t>(B) * H;
+    float* state = work;
+    float* gv = work + 4 * bh;
+    float* hseg = work + 5 * bh;
+    int rc = zero(gv, bh, st);
+    if (rc) return rc;
+    const int n_seg = (T + K - 1) / K;
+    for (int sgm = n_seg - 1; sgm >= 0; --sgm) {
+        const int t0 = sgm * K, t1 = (t0 + K < T) ? t0 + K : T;
+        // recompute the surrogate derivatives over the segment from its checkpoint and the stored spikes
+        hipLaunchKernelGGL(copy_kernel, dim3((4 * bh + 255) / 256), dim3(256), 0, st, ckpt + sgm * 4 * bh, state, 4 * bh);
+        for (int t = t0; t < t1; ++t)
+            hipLaunchKernelGGL(step_kernel<true>, step_grid(p), dim3(kTile), step_lds(p), st, *p, u + t * bh, w_rec_t,
+                               state, t ? spikes + (t - 1) * bh : nullptr, hseg + (t - t0) * bh, nullptr);
+        for (int t = t1 - 1; t >= t0; --t)
+            hipLaunchKernelGGL(reverse_kernel<true>, reverse_grid(p), dim3(256), 0, st, *p,
+                               t < T - 1 ? g_i_all + (t + 1) * bh : nullptr, w_rec_t, hseg + (t - t0) * bh, gso + t * bh,
+                               gv, g_i_all + t * bh);
+        R_HIP(hipGetLastError());
+    }
+    return N2H_OK;
+}
+
+int check_backward_args(const n2h_rlif* p, const float* u, const float* w_rec_t,
+                        const float* spikes, const float* ckpt, const float* gso, const float* g_i_all) {
+    int rc = check(p);
+    if (rc) return rc;
+    if (!u || !w_rec_t || !spikes || !ckpt || !gso || !g_i_all) return rfail(N2H_EINVAL, "null buffer");
+    if (step_lds(p) > 64 * 1024) return rfail(N2H_EINVAL, "n_hidden %d too large", p->n_hidden);
+    if (p->n_hidden % kMM) return rfail(N2H_EINVAL, "n_hidden %d is not a multiple of %d", p->n_hidden, kMM);
+    return N2H_OK;
+}
 
 }  // namespace
 
@@ -262,22 +391,20 @@ int n2h_rlif_last_error(char* buf, size_t len) {
     return N2H_OK;
 }
 
-int n2h_rlif_forward(const n2h_rlif* p, const float* u, const float* w_rec_t, float* i_state, float* v_state,
-                     double* shadow, float* spikes, double* ckpt, void* stream) {
+int n2h_rlif_forward(const n2h_rlif* p, const float* u, const float* w_rec_t, float* state, float* spikes,
+                     float* ckpt, void* stream) {
     int rc = check(p);
     if (rc) return rc;
-    if (!u || !w_rec_t || !i_state || !v_state || !shadow || !spikes || !ckpt) return rfail(N2H_EINVAL, "null buffer");
-    if (fwd_lds(p) > 64 * 1024) return rfail(N2H_EINVAL, "n_hidden %d too large", p->n_hidden);
+    if (!u || !w_rec_t || !state || !spikes || !ckpt) return rfail(N2H_EINVAL, "null buffer");
+    if (step_lds(p) > 64 * 1024) return rfail(N2H_EINVAL, "n_hidden %d too large", p->n_hidden);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t bh = static_cast<int64_t>(p->batch) * p->n_hidden;
-    rc = zero(i_state, bh, st);
-    if (!rc) rc = zero(v_state, bh, st);
-    if (!rc) rc = zero(shadow, 4 * bh, st);
+    rc = zero(state, 4 * bh, st);
     if (rc) return rc;
     for (int32_t t = 0; t < p->n_steps; ++t) {
-        double* ck = (t % p->checkpoint == 0) ? ckpt + (t / p->checkpoint) * 2 * bh : nullptr;
-        hipLaunchKernelGGL(fwd_step_kernel, step_grid(p), dim3(kTile), fwd_lds(p), st, *p, u + t * bh, w_rec_t, i_state,
-                           v_state, shadow, t ? spikes + (t - 1) * bh : nullptr, spikes + t * bh, ck);
+        float* ck = (t % p->checkpoint == 0) ? ckpt + (t / p->checkpoint) * 4 * bh : nullptr;
+        hipLaunchKernelGGL(step_kernel<false>, step_grid(p), dim3(kTile), step_lds(p), st, *p, u + t * bh, w_rec_t,
+                           state, t ? spikes + (t - 1) * bh : nullptr, spikes + t * bh, ck);
     }
     R_HIP(hipGetLastError());
     return N2H_OK;
@@ -300,64 +427,28 @@ int n2h_rlif_readout_adjoint(int32_t n_steps, int64_t bo, float alpha, float bet
     return N2H_OK;
 }
 
-// Backward through the hidden layer.  gso = W_out^T gJ for every step ([T][B][H]); writes gI_t for every step
-// into g_i_all.  work: (7 + 2 K) * B * H floats: I, V (double), gV, gI, M (float), segment V ([K][B][H] double).
-int n2h_rlif_backward(const n2h_rlif* p, const float* u, const float* w_rec, const float* w_rec_t, const float* spikes,
-                      const double* ckpt, const float* gso, float* g_i_all, float* work, void* stream) {
+int n2h_rlif_delivery_adjoint(const n2h_rlif* p, const float* g_i, const float* w_rec_t, float* m, void* stream) {
     int rc = check(p);
     if (rc) return rc;
-    if (!u || !w_rec || !w_rec_t || !spikes || !ckpt || !gso || !g_i_all || !work)
-        return rfail(N2H_EINVAL, "null buffer");
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const int H = p->n_hidden, B = p->batch, T = p->n_steps, K = p->checkpoint;
-    const int64_t bh = static_cast<int64_t>(B) * H;
-    double* shadow = reinterpret_cast<double*>(work);  // [2][B][H]
-    float* gv = work + 4 * bh;
-    float* gi = work + 5 * bh;
-    float* m = work + 6 * bh;
-    double* vseg = reinterpret_cast<double*>(work + 7 * bh);
-    rocblas_handle handle;
-    if (rocblas_create_handle(&handle) != rocblas_status_success) return rfail(N2H_EHIP, "rocblas_create_handle");
-    rocblas_set_stream(handle, st);
-    const float f_one = 1.0f, f_zero = 0.0f;
-    rc = zero(gv, bh, st);
-    if (!rc) rc = zero(gi, bh, st);
-    if (rc) {
-        rocblas_destroy_handle(handle);
-        return rc;
-    }
-    hipError_t e = hipSuccess;
-    const int n_seg = (T + K - 1) / K;
-    for (int sgm = n_seg - 1; sgm >= 0 && rc == N2H_OK; --sgm) {
-        const int t0 = sgm * K, t1 = (t0 + K < T) ? t0 + K : T;
-        // recompute V over the segment from its checkpoint and the stored spikes
-        hipLaunchKernelGGL(copy_kernel, dim3((2 * bh + 255) / 256), dim3(256), 0, st, ckpt + sgm * 2 * bh, shadow, 2 * bh);
-        for (int t = t0; t < t1; ++t)
-            hipLaunchKernelGGL(recompute_step_kernel, step_grid(p), dim3(kTile), fwd_lds(p), st, *p, u + t * bh, w_rec_t,
-                               shadow, t ? spikes + (t - 1) * bh : nullptr, vseg + (t - t0) * bh);
-        e = hipGetLastError();
-        if (e != hipSuccess) {
-            rc = rfail(N2H_EHIP, "recompute: %s", hipGetErrorString(e));
-            break;
-        }
-        for (int t = t1 - 1; t >= t0 && rc == N2H_OK; --t) {
-            const float* mt = nullptr;
-            if (t < T - 1) {  // m = gI_{t+1} W_rec (row-major), as column-major m^T = W_rec^T gI^T
-                if (rocblas_sgemm(handle, rocblas_operation_none, rocblas_operation_none, H, B, H, &f_one, w_rec, H, gi,
-                                  H, &f_zero, m, H) != rocblas_status_success) {
-                    rc = rfail(N2H_EHIP, "rocblas_sgemm");
-                    break;
-                }
-                mt = m;
-            }
-            hipLaunchKernelGGL(adj_step_kernel, dim3((bh + 255) / 256), dim3(256), 0, st, *p, bh, vseg + (t - t0) * bh,
-                               gso + t * bh, mt, gv, gi, g_i_all + t * bh);
-            e = hipGetLastError();
-            if (e != hipSuccess) rc = rfail(N2H_EHIP, "adj_step_kernel: %s", hipGetErrorString(e));
-        }
-    }
-    rocblas_destroy_handle(handle);
-    return rc;
+    if (!g_i || !w_rec_t || !m) return rfail(N2H_EINVAL, "null buffer");
+    if (p->n_hidden % kMM) return rfail(N2H_EINVAL, "n_hidden %d is not a multiple of %d", p->n_hidden, kMM);
+    hipLaunchKernelGGL(reverse_kernel<false>, reverse_grid(p), dim3(256), 0, static_cast<hipStream_t>(stream), *p, g_i,
+                       w_rec_t, nullptr, nullptr, nullptr, m);
+    R_HIP(hipGetLastError());
+    return N2H_OK;
+}
+
+int64_t n2h_rlif_work_floats(const n2h_rlif* p) {
+    if (check(p)) return -1;
+    return work_floats(p);
+}
+
+int n2h_rlif_backward(const n2h_rlif* p, const float* u, const float* w_rec_t, const float* spikes,
+                      const float* ckpt, const float* gso, float* g_i_all, float* work, void* stream) {
+    int rc = check_backward_args(p, u, w_rec_t, spikes, ckpt, gso, g_i_all);
+    if (rc) return rc;
+    if (!work) return rfail(N2H_EINVAL, "null work buffer");
+    return backward(p, u, w_rec_t, spikes, ckpt, gso, g_i_all, work, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -66,6 +66,25 @@ class FusedRound:
     def step(self) -> int:
         return self.runtimes[0].step
 
+    def prepare(self, n_steps: int, graph_window: int) -> int:
+        """Capture (without running) the graphs launch(n_steps, graph_window) will replay from the current
+        step, so that a timed launch captures nothing; returns the graphs captured."""
+        from .network import Runtime
+
+        lib = _lib.load()
+        sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        made = 0
+        for kind, k, phase in Runtime.schedule(self.step, n_steps, graph_window):
+            if kind != "graph" or (k, phase) in self._graphs:
+                continue
+            if k % 4 or phase != self.step % 4:
+                continue  # launch() replays whole windows from phase 0 only; captured when reached
+            gp = ctypes.c_void_p()
+            _lib.check(lib.n2h_batch_graph_create(self._batch, k, sv, ctypes.byref(gp)), "n2h_batch_graph_create")
+            self._graphs[(k, phase)] = gp
+            made += 1
+        return made
+
     def launch(self, n_steps: int = 1, graph_window: int = 0) -> None:
         """n_steps steps of every covered runtime in batched launches (graph windows of graph_window steps,
         a multiple of 4, where they fit; eager launches for the rest)."""

@@ -419,33 +419,92 @@ class Runtime:
         self.bits, self.bit_slots = new, new_slots
         self._bind_ring()
 
+    @staticmethod
+    def schedule(step: int, n_steps: int, graph_window: int) -> list[tuple[str, int, int]]:
+        """How run(n_steps, graph_window) from `step` launches: a list of ("eager", steps, phase) and
+        ("graph", steps, phase) entries, phase = the step phase (mod 4) the entry starts at.  A graph replays
+        only at the phase it was captured at.  graph_window == n_steps: one graph of exactly n_steps (any
+        length).  Otherwise windows are a multiple of 4 steps: eager steps up to phase 0, whole windows, an
+        eager tail."""
+        if n_steps <= 0:
+            return []
+        if not graph_window:
+            return [("eager", n_steps, step % 4)]
+        if graph_window == n_steps:
+            return [("graph", n_steps, step % 4)]
+        if graph_window % 4:
+            raise ValueError("graph windows must be a multiple of 4 steps (the spike lists' phase) unless the "
+                             "window is the whole run")
+        out, done = [], 0
+        if step % 4 and n_steps >= graph_window + 4:
+            lead = 4 - step % 4  # eager steps up to the captured phase
+            out.append(("eager", lead, step % 4))
+            done = lead
+        while n_steps - done >= graph_window:
+            out.append(("graph", graph_window, (step + done) % 4))
+            done += graph_window
+        if n_steps > done:
+            out.append(("eager", n_steps - done, (step + done) % 4))
+        return out
+
+    def _graph(self, n: int, phase: int, at: int, sv) -> ctypes.c_void_p:
+        """The graph of n steps replayed at `phase`; captured now (the plan's step mirror is at step `at`)
+        when missing."""
+        key = (n, phase)
+        g = self._graphs.get(key)
+        if g is None:
+            lib = _lib.load()
+            if at % 4 != phase:
+                raise RuntimeError(f"capture at phase {phase} from step {at}")
+            gp = ctypes.c_void_p()
+            _lib.check(lib.n2h_graph_create(self._plan, n, sv, ctypes.byref(gp)), "n2h_graph_create")
+            g = self._graphs[key] = gp
+        return g
+
+    def prepare(self, n_steps: int, graph_window: int = 0) -> int:
+        """Capture (without running) every graph that run(n_steps, graph_window) will replay from the current
+        step, so that a timed run captures nothing; returns the graphs captured.  Graphs are captured at the
+        current step's phase: each is keyed by (length, phase), and the plan's step mirror is moved to the
+        entry's phase for its capture and back."""
+        plan = self.schedule(self.step, n_steps, graph_window)
+        self._ensure_slots(n_steps)  # a history grown later would rebind the ring and drop the graphs
+        lib = _lib.load()
+        sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        made = 0
+        for kind, k, phase in plan:
+            if kind != "graph" or (k, phase) in self._graphs:
+                continue
+            # the capture records kernels whose arguments follow the host mirror's phase only
+            at = self.step
+            shift = (phase - at % 4) % 4
+            if shift:
+                _lib.check(lib.n2h_set_step(self._plan, at + shift, sv), "n2h_set_step")
+            try:
+                gp = ctypes.c_void_p()
+                _lib.check(lib.n2h_graph_create(self._plan, k, sv, ctypes.byref(gp)), "n2h_graph_create")
+                self._graphs[(k, phase)] = gp
+                made += 1
+            finally:
+                if shift:
+                    _lib.check(lib.n2h_set_step(self._plan, at, sv), "n2h_set_step")
+        return made
+
     def run(self, n_steps: int, graph_window: int = 0) -> None:
         if n_steps < 0:
             raise ValueError("n_steps must be >= 0")
         if n_steps == 0:
             return
+        plan = self.schedule(self.step, n_steps, graph_window)
         self._ensure_slots(n_steps)
         lib = _lib.load()
         sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        done = 0
-        if graph_window and graph_window % 4:
-            raise ValueError("graph windows must be a multiple of 4 steps (the spike lists' phase)")
-        if graph_window and self.step % 4 and n_steps >= graph_window + 4:
-            lead = 4 - self.step % 4  # eager steps up to the captured phase
-            _lib.check(lib.n2h_step(self._plan, lead, sv), "n2h_step")
-            done = lead
-        if graph_window and n_steps - done >= graph_window:
-            key = (graph_window, (self.step + done) % 4)
-            g = self._graphs.get(key)
-            if g is None:
-                gp = ctypes.c_void_p()
-                _lib.check(lib.n2h_graph_create(self._plan, graph_window, sv, ctypes.byref(gp)), "n2h_graph_create")
-                g = self._graphs[key] = gp
-            while n_steps - done >= graph_window:
-                _lib.check(lib.n2h_graph_launch(g, sv), "n2h_graph_launch")
-                done += graph_window
-        if n_steps > done:
-            _lib.check(lib.n2h_step(self._plan, n_steps - done, sv), "n2h_step")
+        at = self.step
+        for kind, k, phase in plan:
+            if kind == "eager":
+                _lib.check(lib.n2h_step(self._plan, k, sv), "n2h_step")
+            else:
+                _lib.check(lib.n2h_graph_launch(self._graph(k, phase, at, sv), sv), "n2h_graph_launch")
+            at += k
         self.step += n_steps
         if not self.net.record:
             self.record_from = max(self.step - 1, 0)

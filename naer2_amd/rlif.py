@@ -11,9 +11,10 @@ Dynamics (dt = 1 ms; alpha = exp(-dt/tau_syn), beta = exp(-dt/tau_m)):
     s_t = H(V_t - theta), surrogate ds/dV = 1 / (1 + slope |V_t - theta|)^2 (fast sigmoid, slope 25)
     J_t = alpha J_{t-1} + W_out s_t;  R_t = beta R_{t-1} + J_t (leaky readout integrators)
 The reference has no autograd (naer2 is forward-only): a surrogate-gradient adjoint added beside its plain path.
-Plain library GEMMs (the input projection, the readout projection, the per-step W_rec^T product of the
-adjoint and the weight gradients) run through rocBLAS / hipBLASLt; delivery and the LIF update and their
-adjoints are the hand-written kernels.
+Plain library GEMMs (the input projection, the readout projection and the weight gradients) run through
+rocBLAS / hipBLASLt; delivery and the LIF update and their adjoints are the hand-written kernels: the
+event-driven recurrent delivery, the compensated-fp32 LIF update, and per reverse step the delivery adjoint
+W_rec^T gI on the matrix cores fused with the LIF adjoint.
 """
 from __future__ import annotations
 
@@ -21,6 +22,7 @@ import ctypes
 import math
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -30,7 +32,14 @@ class RLIFLayer(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_int32), ("n_hidden", ctypes.c_int32), ("n_steps", ctypes.c_int32),
                 ("checkpoint", ctypes.c_int32), ("mode", ctypes.c_int32), ("alpha", ctypes.c_float),
                 ("beta", ctypes.c_float), ("theta", ctypes.c_float), ("slope", ctypes.c_float),
-                ("gamma", ctypes.c_float)]
+                ("gamma", ctypes.c_float), ("alpha_lo", ctypes.c_float), ("beta_lo", ctypes.c_float)]
+
+
+def layer(B: int, H: int, T: int, cfg: "RLIFConfig") -> RLIFLayer:
+    """The C layer description; the decays as float + float remainder (for the compensated state)."""
+    a_hi, b_hi = float(np.float32(cfg.alpha)), float(np.float32(cfg.beta))
+    return RLIFLayer(B, H, T, cfg.checkpoint, cfg.mode, a_hi, b_hi, cfg.theta, cfg.slope, cfg.gamma,
+                     float(np.float32(cfg.alpha - a_hi)), float(np.float32(cfg.beta - b_hi)))
 
 
 def _bind(lib):
@@ -39,10 +48,12 @@ def _bind(lib):
     vp, i32, i64, f = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
     P = ctypes.POINTER(RLIFLayer)
     lib.n2h_rlif_last_error.argtypes, lib.n2h_rlif_last_error.restype = [ctypes.c_char_p, ctypes.c_size_t], i32
-    lib.n2h_rlif_forward.argtypes, lib.n2h_rlif_forward.restype = [P, vp, vp, vp, vp, vp, vp, vp, vp], i32
+    lib.n2h_rlif_forward.argtypes, lib.n2h_rlif_forward.restype = [P, vp, vp, vp, vp, vp, vp], i32
     lib.n2h_rlif_readout.argtypes, lib.n2h_rlif_readout.restype = [i32, i64, f, f, vp, vp, vp], i32
     lib.n2h_rlif_readout_adjoint.argtypes, lib.n2h_rlif_readout_adjoint.restype = [i32, i64, f, f, vp, vp, vp], i32
-    lib.n2h_rlif_backward.argtypes, lib.n2h_rlif_backward.restype = [P, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32
+    lib.n2h_rlif_backward.argtypes, lib.n2h_rlif_backward.restype = [P, vp, vp, vp, vp, vp, vp, vp, vp], i32
+    lib.n2h_rlif_delivery_adjoint.argtypes, lib.n2h_rlif_delivery_adjoint.restype = [P, vp, vp, vp, vp], i32
+    lib.n2h_rlif_work_floats.argtypes, lib.n2h_rlif_work_floats.restype = [P], i64
     lib._rlif_bound = True
     return lib
 
@@ -92,20 +103,18 @@ class RecurrentLIFFunction(torch.autograd.Function):
         w_rec_t = w_rec.t().contiguous()
         spikes = torch.empty(T, B, H, device=dev)
         n_seg = (T + cfg.checkpoint - 1) // cfg.checkpoint
-        ckpt = torch.empty(n_seg, 2, B, H, device=dev, dtype=torch.float64)
-        i_state = torch.empty(B, H, device=dev)
-        v_state = torch.empty(B, H, device=dev)
-        shadow = torch.empty(2, B, H, device=dev, dtype=torch.float64)
-        layer = RLIFLayer(B, H, T, cfg.checkpoint, cfg.mode, cfg.alpha, cfg.beta, cfg.theta, cfg.slope, cfg.gamma)
+        ckpt = torch.empty(n_seg, 4, B, H, device=dev)
+        state = torch.empty(4, B, H, device=dev)
+        lay = layer(B, H, T, cfg)
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-        _check(lib.n2h_rlif_forward(ctypes.byref(layer), _ptr(u), _ptr(w_rec_t), _ptr(i_state), _ptr(v_state),
-                                    _ptr(shadow), _ptr(spikes), _ptr(ckpt), stream), "n2h_rlif_forward")
+        _check(lib.n2h_rlif_forward(ctypes.byref(lay), _ptr(u), _ptr(w_rec_t), _ptr(state), _ptr(spikes),
+                                    _ptr(ckpt), stream), "n2h_rlif_forward")
         z = torch.matmul(spikes.reshape(T * B, H), w_out.t()).reshape(T, B, O).contiguous()
         r = torch.empty(T, B, O, device=dev)
         _check(lib.n2h_rlif_readout(T, B * O, cfg.alpha, cfg.beta, _ptr(z), _ptr(r), stream), "n2h_rlif_readout")
         ctx.save_for_backward(x, w_rec, w_rec_t, w_out, u, spikes, ckpt)
         ctx.cfg = cfg
-        ctx.layer = layer
+        ctx.layer = lay
         ctx.mark_non_differentiable(spikes)
         return r, spikes
 
@@ -113,7 +122,7 @@ class RecurrentLIFFunction(torch.autograd.Function):
     def backward(ctx, g_r, g_spikes):
         lib = _bind(_lib.load())
         x, w_rec, w_rec_t, w_out, u, spikes, ckpt = ctx.saved_tensors
-        cfg, layer = ctx.cfg, ctx.layer
+        cfg, lay = ctx.cfg, ctx.layer
         T, B, C = x.shape
         H, O = w_rec.shape[0], w_out.shape[0]
         dev = x.device
@@ -127,9 +136,9 @@ class RecurrentLIFFunction(torch.autograd.Function):
         g_w_out = gj2.t() @ s2
         gso = (gj2 @ w_out).reshape(T, B, H).contiguous()
         g_i = torch.empty(T, B, H, device=dev)
-        work = torch.empty((7 + 2 * cfg.checkpoint) * B * H, device=dev)
-        _check(lib.n2h_rlif_backward(ctypes.byref(layer), _ptr(u), _ptr(w_rec.contiguous()), _ptr(w_rec_t),
-                                     _ptr(spikes), _ptr(ckpt), _ptr(gso), _ptr(g_i), _ptr(work), stream),
+        work = torch.empty(int(lib.n2h_rlif_work_floats(ctypes.byref(lay))), device=dev)
+        _check(lib.n2h_rlif_backward(ctypes.byref(lay), _ptr(u), _ptr(w_rec_t), _ptr(spikes), _ptr(ckpt),
+                                     _ptr(gso), _ptr(g_i), _ptr(work), stream),
                "n2h_rlif_backward")
         gi2 = g_i.reshape(T * B, H)
         g_w_rec = g_i[1:].reshape(-1, H).t() @ spikes[:-1].reshape(-1, H)

@@ -26,6 +26,21 @@ def build() -> Path:
     return LIB
 
 
+def use_native() -> Path:
+    """Build the oracle for this host (-O3 -march=native, always rebuilt: a build made on another machine
+    may use instructions this one lacks) and load that build instead of the portable one.  For the CPU
+    baseline's timing only; it must be called before the first load()."""
+    global LIB
+    native = HERE / "_build" / "liboracle_native.so"
+    if LIB == native:
+        return LIB
+    if _lib is not None:
+        raise RuntimeError("use_native() after the portable oracle was loaded")
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), "native"], check=True)
+    LIB = native
+    return LIB
+
+
 def load():
     global _lib
     if _lib is None:

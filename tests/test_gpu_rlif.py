@@ -1,12 +1,14 @@
 """Surrogate-gradient BPTT kernels (naer2_amd.rlif) against the FP64 autograd unroll (oracle/rlif_oracle.py)."""
 from __future__ import annotations
 
+import ctypes
 import math
 import sys
 
 import pytest
 import torch
 
+from naer2_amd import rlif as rl
 from naer2_amd.rlif import RecurrentLIF, RLIFConfig, max_over_time_ce, poisson_input
 from oracle import rlif_oracle as ro
 
@@ -69,8 +71,8 @@ def test_spiking_config_gradients_match_fp64_autograd():
 
 
 def test_checkpoint_interval_does_not_change_gradients():
-    """Spikes, readout and gradients are bit-identical: the recompute restarts the forward's own double
-    shadow from each checkpoint, the same trajectory whatever the interval."""
+    """Spikes, readout and gradients are bit-identical: the recompute restarts the forward's own compensated
+    state from each checkpoint, the same trajectory whatever the interval."""
     _say("case: rlif checkpoints 25 vs 7 vs 100")
     g = torch.Generator(device="cuda").manual_seed(3)
     x = poisson_input(16, 50, 100, 1.0, g)
@@ -84,3 +86,26 @@ def test_checkpoint_interval_does_not_change_gradients():
         assert torch.equal(o[2], out[0][2]) and torch.equal(o[1], out[0][1])
         for a, b in zip(o[3], out[0][3]):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("B,H", [(128, 1024), (5, 96), (33, 48)])
+def test_delivery_adjoint_matches_a_float64_product(B, H):
+    """The hand-written matrix-core product m = gI W_rec (through W_rec^T) against float64 torch, on ragged
+    batch sizes and hidden sizes of 16 k."""
+    _say(f"case: rlif delivery adjoint B={B} H={H}")
+    lib = rl._bind(rl._lib.load())
+    g = torch.Generator(device="cuda").manual_seed(B * 1000 + H)
+    gi = torch.randn(B, H, generator=g, device="cuda")
+    w = torch.randn(H, H, generator=g, device="cuda") / math.sqrt(H)
+    w_t = w.t().contiguous()
+    m = torch.full((B, H), float("nan"), device="cuda")
+    lay = rl.layer(B, H, 1, RLIFConfig())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rl._check(lib.n2h_rlif_delivery_adjoint(ctypes.byref(lay), rl._ptr(gi), rl._ptr(w_t), rl._ptr(m), stream),
+              "n2h_rlif_delivery_adjoint")
+    torch.cuda.synchronize()
+    ref = (gi.double().cpu() @ w.double().cpu())
+    err = float((m.double().cpu() - ref).abs().max() / ref.abs().max())
+    assert err < 1e-5, err
+    lay_bad = rl.layer(B, H + 1, 1, RLIFConfig())
+    assert lib.n2h_rlif_delivery_adjoint(ctypes.byref(lay_bad), rl._ptr(gi), rl._ptr(w_t), rl._ptr(m), stream) != 0

@@ -267,3 +267,17 @@ def test_tables_built_in_chunks_equal_whole(monkeypatch):
     for a, b in ((c1.src_ptr, c2.src_ptr), (c1.tgt, c2.tgt), (c1.w, c2.w), (c1.d, c2.d),
                  (s1.seg_off, s2.seg_off), (s1.links, s2.links)):
         assert torch.equal(a, b)
+
+
+def test_rlif_layer_splits_the_decays_exactly_enough():
+    """The BPTT layer's decays go to the kernels as float + float remainder: hi + lo equals the double-
+    precision decay to ~2^-48, hi alone only to ~2^-24."""
+    import math
+
+    from naer2_amd.rlif import RLIFConfig, layer
+    cfg = RLIFConfig()
+    lay = layer(128, 1024, 100, cfg)
+    for hi, lo, exact in ((lay.alpha, lay.alpha_lo, math.exp(-1 / 5)), (lay.beta, lay.beta_lo, math.exp(-1 / 20))):
+        assert abs(hi - exact) > 1e-10  # the float alone is off in its last bits
+        assert abs((hi + lo) - exact) < 1e-14 * exact
+    assert (lay.batch, lay.n_hidden, lay.n_steps, lay.checkpoint) == (128, 1024, 100, 25)
