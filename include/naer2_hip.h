@@ -202,6 +202,16 @@ int64_t n2h_rlif_work_floats(const n2h_rlif* layer);
  * work: n2h_rlif_work_floats(layer) floats of device memory.  n_hidden must be a multiple of 16. */
 int n2h_rlif_backward(const n2h_rlif* layer, const float* u, const float* w_rec_t, const float* spikes,
                       const float* checkpoints, const float* gso, float* g_i_all, float* work, void* stream);
+/* n2h_rlif_backward without a work area and without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The work area is a static device array of N2H_RLIF_SCRATCH_FLOATS floats (16 MiB),
+ * one per GPU, held by each GPU the library runs on until the process ends; layers whose
+ * n2h_rlif_work_floats exceeds it are refused (N2H_EINVAL).  The call allocates nothing and keeps no state
+ * between calls beyond the array's contents (every value it reads there it first writes).  All calls on one
+ * GPU share that array: calls that overlap on one GPU (two streams, two host threads) race. */
+#define N2H_RLIF_SCRATCH_FLOATS (1 << 22)
+int n2h_rlif_backward_noalloc(const n2h_rlif* layer, const float* u, const float* w_rec_t, const float* spikes,
+                              const float* checkpoints, const float* gso, float* g_i_all, void* stream);
 
 /* Row intake (the target-major pull): each target's in-links in contract order (ascending canonical link
  * index).  Per group a block of 16-byte cells [row_depth[g]][64]: cell q of lane l (cell index

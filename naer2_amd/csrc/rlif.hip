@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "naer2_hip.h"
 
@@ -45,6 +46,7 @@ namespace {
 
 constexpr int kTile = 256;     // hidden units per workgroup of the step kernels
 constexpr int kMM = 16;        // output tile edge of the reverse kernel
+constexpr int kMaxDevices = 64;
 
 thread_local char r_err[512] = "";
 
@@ -380,6 +382,32 @@ int check_backward_args(const n2h_rlif* p, const float* u, const float* w_rec_t,
     return N2H_OK;
 }
 
+// The scratch of n2h_rlif_backward_noalloc: one array per GPU (each GPU the library runs on holds its own
+// copy until the process ends), the size of the largest work area that entry point takes.
+constexpr int64_t kScratchFloats = N2H_RLIF_SCRATCH_FLOATS;
+__device__ __align__(256) float scratch[kScratchFloats];
+
+std::mutex scratch_mu;
+float* scratch_dev[kMaxDevices] = {};
+
+// This GPU's address of `scratch`, looked up on the first call with the device current and kept per device.
+// Host code passes only this address to kernels: the array's own name is a host-side stand-in.
+int scratch_address(float** out) {
+    int dev = 0;
+    R_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return rfail(N2H_EINVAL, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return rfail(N2H_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        scratch_dev[dev] = static_cast<float*>(ptr);
+    }
+    *out = scratch_dev[dev];
+    return N2H_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -448,6 +476,20 @@ int n2h_rlif_backward(const n2h_rlif* p, const float* u, const float* w_rec_t, c
     int rc = check_backward_args(p, u, w_rec_t, spikes, ckpt, gso, g_i_all);
     if (rc) return rc;
     if (!work) return rfail(N2H_EINVAL, "null work buffer");
+    return backward(p, u, w_rec_t, spikes, ckpt, gso, g_i_all, work, static_cast<hipStream_t>(stream));
+}
+
+int n2h_rlif_backward_noalloc(const n2h_rlif* p, const float* u, const float* w_rec_t, const float* spikes,
+                              const float* ckpt, const float* gso, float* g_i_all, void* stream) {
+    int rc = check_backward_args(p, u, w_rec_t, spikes, ckpt, gso, g_i_all);
+    if (rc) return rc;
+    const int64_t need = work_floats(p);
+    if (need > kScratchFloats)
+        return rfail(N2H_EINVAL, "work area of %lld floats ((5 + min(K, T)) * batch * n_hidden) above the %lld of "
+                     "the library's scratch", static_cast<long long>(need), static_cast<long long>(kScratchFloats));
+    float* work = nullptr;
+    rc = scratch_address(&work);
+    if (rc) return rc;
     return backward(p, u, w_rec_t, spikes, ckpt, gso, g_i_all, work, static_cast<hipStream_t>(stream));
 }
 
