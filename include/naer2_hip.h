@@ -238,12 +238,12 @@ int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base,
  * t * long_parts / n == k), each run kept in canonical order and packed {target | (delay_steps - 1) << 17 |
  * target population << 24, base weight bits}; alpha[target pop][source pop] the weight scales the forward
  * ran with (its weights are alpha * base); pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset,
- * dL/ds of one spike of population p, 0}; this step's potentials before the threshold (n2h_set_vtrace) and
+ * dL/ds of one spike of population p, 0}; this step's distances to the threshold (n2h_set_vtrace) and
  * spikes; the adjoint state gV, gI [n]; the ring gx [n][ring_slots + long_delay - 1] of the input adjoints of
  * the following steps (zero at the start; ring_slots > the largest delay; step s in slot s mod ring_slots,
- * the first long_delay - 1 slots mirrored past the end), and g_sum [n][n_pops], where every spiking source
- * j adds sum over its links into population a of w gx_{step+d} (dL/dalpha[a][pop j] = sum over j in pop j
- * of g_sum[j][a]).  The long links' sums come from long_bins [n][8], which n2h_adj_window filled for the
+ * the first long_delay - 1 slots mirrored past the end), and g_sum [n][n_pops][2], where every spiking
+ * source j adds sum over its links into population a of w gx_{step+d} to the compensated pair (hi, lo) of
+ * g_sum[j][a] (dL/dalpha[a][pop j] = sum over j in pop j of hi + lo).  The long links' sums come from long_bins [n][8], which n2h_adj_window filled for the
  * window holding this step (a neuron reads its row only where it is within the surrogate's support or
  * spiked).  The reset and the refractory gate carry no gradient. */
 typedef struct n2h_adj_net {
@@ -267,7 +267,7 @@ typedef struct n2h_adj_net {
     float* long_bins;            /* device [n][8] of this step (n2h_adj_window: [n_win][n][8]) */
     float* g_v;                  /* device [n] */
     float* g_i;                  /* device [n] */
-    double* g_sum;               /* device [n][n_pops] */
+    float* g_sum;                /* device [n][n_pops][2]: a compensated sum, hi + lo */
 } n2h_adj_net;
 /* The surrogate: ds/dV = gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - V_th| / dv <= cutoff, else 0
  * (dv = V_th - V_reset); neurons beyond the cutoff that did not spike skip their out-link gather. */
@@ -298,11 +298,16 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
  * ranges in one launch, the XCDs taking ranges in turn, and adds their sums in range order. */
 int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay, int32_t long_parts);
 
-/* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer, every step s in [step0, step0 + n_steps) writes each neuron's
- * potential as the threshold sees it (before reset) to buffer[(s - step0) * n_neurons + i], or +inf on a
- * refractory step.  Results are unchanged.  Captured graphs and batches keep the setting they were made
- * with. */
-int n2h_set_vtrace(n2h_plan* plan, float* buffer, int64_t step0, int32_t n_steps);
+/* Potential trace for the adjoint (no counterpart in the reference, which is forward-only): with a buffer,
+ * every step s in [step0, step0 + n_steps) writes each neuron's distance to the threshold as the threshold
+ * sees it (V before reset minus V_th) to buffer[(s - step0) * n_neurons + i], or +inf on a refractory step.
+ * v_lo, i_lo (LIF_EXP; both or neither, fp32 [n_neurons] each): while bound, every step also tracks what fp32
+ * rounding dropped from V and I -- each operation of the update with its exact error (TwoSum / TwoProd), so
+ * that V + v_lo and I + i_lo follow exact arithmetic on the same inputs -- and the trace is then the
+ * compensated distance (V - V_th) + v_lo.  V, I, the spikes and every other result are unchanged (bit for bit).
+ * buffer may be NULL with v_lo set (tracking without a trace).  Captured graphs and batches keep the setting
+ * they were made with. */
+int n2h_set_vtrace(n2h_plan* plan, float* buffer, int64_t step0, int32_t n_steps, float* v_lo, float* i_lo);
 
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list

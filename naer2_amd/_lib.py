@@ -128,7 +128,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_batch_destroy": ([vp], i32),
         "n2h_set_trace": ([vp, vp], i32),
         "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
-        "n2h_set_vtrace": ([vp, vp, i64, i32], i32),
+        "n2h_set_vtrace": ([vp, vp, i64, i32, vp, vp], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
         "n2h_ev_last_error": ([ctypes.c_char_p, ctypes.c_size_t], i32),

@@ -6,8 +6,10 @@ for a per-population rate loss, with respect to the weight scales alpha[target p
 
 loss = sum over networks and populations of (rate - target)^2 (rates in Hz over the n_steps).  The forward is
 the plain step (bit-exact, one batched launch per step, FusedRound), with the state saved every `checkpoint`
-steps.  The backward runs the segments in reverse: each is recomputed from its checkpoint with the
-potentials before the threshold traced (n2h_set_vtrace), then reversed in windows of `long_delay` steps:
+steps; the forward also tracks what fp32 rounding drops from each neuron's V and I (n2h_set_vtrace's
+compensation state, checkpointed with the state; V, I and the spikes are unchanged).  The backward runs the
+segments in reverse: each is recomputed from its checkpoint with the compensated distance to the threshold
+traced, then reversed in windows of `long_delay` steps:
 n2h_adj_window gathers the input adjoints of every out-link with a delay of at least long_delay once per
 window (they all lie past it), then n2h_adj_step reverses each step of the window -- one launch for all
 networks, a wavefront per source gathering its short links' input adjoints and adding the window's long
@@ -170,12 +172,19 @@ class RateLossGrad:
         # ---- forward, checkpointed ----
         t_fw = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t_fw[0].record()
+        # the compensation state (what fp32 rounding dropped from V and I), tracked through the forward and
+        # checkpointed with the state, so that the recompute traces compensated distances to the threshold
+        lo = [torch.zeros(2, rt.n, device=self.dev) for rt in rts]
+        for rt, l in zip(rts, lo):
+            _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, l[0].data_ptr(), l[1].data_ptr()), "n2h_set_vtrace")
         fr = FusedRound(rts, self.tuning, reserve_steps=T + 1)
         ckpts = []
         for k0 in range(0, T, K):
-            ckpts.append([rt.snapshot() for rt in rts])
+            ckpts.append([(rt.snapshot(), l.clone()) for rt, l in zip(rts, lo)])
             fr.launch(min(K, T - k0), self.window)
         fr.close()
+        for rt in rts:
+            _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
         t_fw[1].record()
         final = [rt.snapshot() for rt in rts]
         dt_s = rts[0].dt * 1e-3
@@ -199,7 +208,7 @@ class RateLossGrad:
                 "gx": torch.zeros(rt.n, R + W - 1, device=self.dev), "gv": torch.zeros(rt.n, device=self.dev),
                 "lb": torch.zeros(W, rt.n, 8, device=self.dev), "gr": torch.zeros(W, rt.n, device=self.dev),
                 "gi": torch.zeros(rt.n, device=self.dev),
-                "gsum": torch.zeros(rt.n, self.P, dtype=torch.float64, device=self.dev),
+                "gsum": torch.zeros(rt.n, self.P, 2, device=self.dev),  # compensated: hi, lo
                 "vtr": torch.empty(min(K, T), rt.n, device=self.dev),
                 "spk": torch.empty(min(K, T), rt.n, dtype=torch.uint8, device=self.dev),
             })
@@ -220,14 +229,16 @@ class RateLossGrad:
         for k in reversed(range(len(ckpts))):
             start = s0 + k * K
             L = min(K, T - k * K)
-            for rt, ck, sd in zip(rts, ckpts[k], st):
+            for rt, (ck, lk), sd, l in zip(rts, ckpts[k], st, lo):
                 rt.restore(ck)
-                _lib.check(lib.n2h_set_vtrace(rt._plan, sd["vtr"].data_ptr(), start, L), "n2h_set_vtrace")
+                l.copy_(lk)
+                _lib.check(lib.n2h_set_vtrace(rt._plan, sd["vtr"].data_ptr(), start, L, l[0].data_ptr(),
+                                              l[1].data_ptr()), "n2h_set_vtrace")
             fr = FusedRound(rts, self.tuning, reserve_steps=L)
             fr.launch(L, 0)
             fr.close()
             for rt, sd in zip(rts, st):
-                _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0), "n2h_set_vtrace")
+                _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
                 gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
                 sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
             # windows of W steps from the segment's end: [t0, t1)
@@ -247,13 +258,15 @@ class RateLossGrad:
         for rt, sn, rf in zip(rts, final, rec_from):
             rt.restore(sn)
             rt.record_from = rf  # the history still holds every step since (the final snapshot's bits)
-        g = torch.zeros(self.P, self.P, dtype=torch.float64, device=self.dev)
+        # dL/dalpha[a][b] = sum over networks and source neurons j of population b of g_sum[j][a] (hi + lo), in
+        # fp32 tree sums on the device
+        g = torch.zeros(self.P, self.P, device=self.dev)
         for rt, sd in zip(rts, st):
             pf = np.asarray(rt.pop_first, dtype=np.int64)
-            cs = torch.cat([torch.zeros(1, self.P, dtype=torch.float64, device=self.dev), sd["gsum"].cumsum(0)])
             for b in range(self.P):
-                g[:, b] += cs[pf[b + 1]] - cs[pf[b]]
+                blk = sd["gsum"][pf[b]:pf[b + 1]]
+                g[:, b] += blk[:, :, 0].sum(dim=0) + blk[:, :, 1].sum(dim=0)
         torch.cuda.synchronize(self.dev)
         self.last = {"forward_ms": t_fw[0].elapsed_time(t_fw[1]), "backward_ms": t_bw[0].elapsed_time(t_bw[1]),
                      "c_pops": c_pops}
-        return loss, g.cpu().numpy()
+        return loss, g.cpu().numpy().astype(np.float64)

@@ -1,5 +1,7 @@
 // Surrogate-gradient adjoint of the exponential-synapse LIF step (the LIF_EXP model of naer2_hip.hip) for a
-// per-population rate loss, batched over networks: two launches per reverse step, two per window of long links.
+// per-population rate loss, batched over networks: one launch per reverse step (adj_step_kernel: short links
+// and every neuron's update), and per window of long links a clear, a mark, the long-link gather and the
+// combine of its target ranges (or, for range counts that are not a multiple of 8, one gather launch a range).
 //
 // The reference is forward-only; this adjoint is added beside its plain path.  Forward step t of neuron i (naer2_hip.hip, step_group):
 //     I'_t = I_{t-1} + x_t + ext_t
@@ -18,6 +20,11 @@
 //     gVpre  = (1 - s_t) gV + r_t h(V^pre_t) gs
 //     gI'    = r_t P_VI gVpre + P_I gI;   gV <- r_t P_V gVpre;   gI <- gI';   gx_t[i] = gI'
 //     dL/dalpha[a][pop i] += s_t[i] * sum over out-links i -> k in pop a of w gx_{t+d}[k]
+// (accumulated per source in compensated fp32: a hi + lo pair, every addition's exact error kept in lo).
+// The surrogate takes the traced distance to the threshold, V^pre - theta, which the recompute forward
+// computes compensated (n2h_set_vtrace with v_lo / i_lo): the potential as exact arithmetic on the fp32
+// inputs would give it, to ~2^-44, rather than the fp32 potential whose rounding the surrogate's slope at
+// the threshold (kappa / dv per unit of V) would amplify.
 // A wavefront gathers the out-links of one neuron that needs them (lanes stride the list; per-lane sums per
 // target population, then a fixed butterfly): every sum has a fixed order, so results are deterministic.
 //
@@ -40,6 +47,7 @@
 #include <cstring>
 
 #include "naer2_hip.h"
+#include "twofloat.h"
 
 namespace {
 
@@ -91,8 +99,9 @@ __device__ inline int adj_pop(const n2h_adj_net& a, int j) {
     return pj;
 }
 
-// |V - theta| / dv: +inf on refractory steps (the trace holds +inf there)
-__device__ inline float adj_u(float vp, float theta, float dv) { return fabsf(vp - theta) / dv; }
+// |V - theta| / dv from the traced distance to the threshold V - theta (n2h_set_vtrace): +inf on refractory
+// steps (the trace holds +inf there)
+__device__ inline float adj_u(float dist, float dv) { return fabsf(dist) / dv; }
 
 // one link {target | (d - 1) << 17 | target pop << 24, base weight}: the ring offset of gx_{t+d}[target]
 // (pos = t mod R; d < R) and the target population
@@ -206,6 +215,12 @@ __device__ inline int64_t adj_recent_slot(uint32_t x, int pos, int W, int n, uin
 }
 
 // lane per neuron: its window mask (steps where it is within the support or spiked), appended to the lists
+// the list counters to zero before a window's mark kernel (a kernel rather than hipMemsetAsync, so that a
+// caller may capture the reverse pass into a graph)
+__global__ __launch_bounds__(256) void adj_clear_kernel(int32_t* __restrict__ work) {
+    for (int i = threadIdx.x; i < kAdjListHead; i += blockDim.x) work[i] = 0;
+}
+
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBatch b) {
     int q;
     const int base = adj_wave_base(b, q);
@@ -217,10 +232,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
     uint32_t mask = 0;
     if (j < n) {
         const float* C = a.pop_coef + adj_pop(a, j) * 8;
-        const float theta = C[4], dv = C[5];
+        const float dv = C[5];
         for (int w = 0; w < b.n_win; ++w) {
             const bool s = a.spikes[static_cast<int64_t>(w) * n + j] != 0;
-            const bool near = adj_u(a.vtrace[static_cast<int64_t>(w) * n + j], theta, dv) <= b.sur.cutoff;
+            const bool near = adj_u(a.vtrace[static_cast<int64_t>(w) * n + j], dv) <= b.sur.cutoff;
             mask |= (s || near) ? (1u << w) : 0u;
         }
     }
@@ -365,7 +380,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
         const int j = base + lane;
         if (j >= a.n_neurons) return;
         const float* C = a.pop_coef + adj_pop(a, j) * 8;
-        if (a.spikes[j] != 0 || adj_u(a.vtrace[j], C[4], C[5]) <= b.sur.cutoff) return;  // a gatherer
+        if (a.spikes[j] != 0 || adj_u(a.vtrace[j], C[5]) <= b.sur.cutoff) return;  // a gatherer
         const float none[kAdjPops] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         adj_update_neuron(b, a, j, none);
         return;
@@ -418,16 +433,22 @@ __device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a
 
     const int pj = adj_pop(a, j);
     const float* C = a.pop_coef + pj * 8;
-    const float P_V = C[0], P_I = C[1], P_VI = C[2], theta = C[4], dv = C[5], c_loss = C[6];
+    const float P_V = C[0], P_I = C[1], P_VI = C[2], dv = C[5], c_loss = C[6];
     const float vp = a.vtrace[j];
     const bool s = a.spikes[j] != 0;
-    const float u = adj_u(vp, theta, dv);
+    const float u = adj_u(vp, dv);
     const bool near = u <= b.sur.cutoff;  // the surrogate is nonzero
     float coup = 0.0f;
     for (int p = 0; p < a.n_pops; ++p) coup += a.alpha[p * a.n_pops + pj] * mine[p];
-    if (s) {
-        double* gs = a.g_sum + static_cast<int64_t>(j) * a.n_pops;
-        for (int p = 0; p < a.n_pops; ++p) gs[p] += static_cast<double>(mine[p]);
+    if (s) {  // compensated fp32 accumulation (hi, lo) over the reverse steps: the exact error of every addition
+        float2* gs = reinterpret_cast<float2*>(a.g_sum) + static_cast<int64_t>(j) * a.n_pops;
+        for (int p = 0; p < a.n_pops; ++p) {
+            float2 acc = gs[p];
+            const n2h_twofloat::df t = n2h_twofloat::two_sum(acc.x, mine[p]);
+            acc.x = t.hi;
+            acc.y = acc.y + t.lo;
+            gs[p] = acc;
+        }
     }
     const float gv = a.g_v[j], gi = a.g_i[j];
     float gip, gv_new;
@@ -593,8 +614,7 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
     if (n_win < 1 || n_win > W) return fail(N2H_EINVAL, "n_win %d outside [1, long_delay = %d]", n_win, W);
     b.n_win = n_win;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = hipMemsetAsync(work, 0, kAdjListHead * sizeof(int32_t), s);  // the counters
-    if (e != hipSuccess) return fail(N2H_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(adj_clear_kernel, dim3(1), dim3(256), 0, s, work);  // the counters
     switch (W) {
         case 1: launch_window<1>(blocks, s, b); break;
         case 2: launch_window<2>(blocks, s, b); break;
@@ -605,7 +625,7 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
         case 7: launch_window<7>(blocks, s, b); break;
         default: launch_window<8>(blocks, s, b); break;
     }
-    e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(N2H_EHIP, "adjoint window kernels: %s", hipGetErrorString(e));
     return N2H_OK;
 }

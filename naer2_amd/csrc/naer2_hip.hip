@@ -35,6 +35,7 @@
 #include <new>
 
 #include "naer2_hip.h"
+#include "twofloat.h"
 #include "philox.h"
 
 namespace {
@@ -109,9 +110,11 @@ struct StepArgs {
     int32_t row_uniform;        // every link's delay cell (delay - 1) when all delays are equal, else -1
     int32_t dense_min;          // row intake for steps following at least this many spikes
     int32_t acc_base;           // the delay cell held at acc[0]: row_uniform for uniform delays (one cell), else 0
-    float* vtrace;              // null, or [vtrace_steps][n_neurons]: the potential before the threshold of
-    int64_t vtrace_step0;       //   steps vtrace_step0 .. + vtrace_steps - 1 (+inf on refractory steps)
+    float* vtrace;              // null, or [vtrace_steps][n_neurons]: the potential before the threshold minus the
+    int64_t vtrace_step0;       //   threshold, of steps vtrace_step0 .. + vtrace_steps - 1 (+inf on refractory steps)
     int32_t vtrace_steps;
+    float* vlo;                 // LIF_EXP, null or [n_neurons] each: the remainders of V and I that fp32 rounding
+    float* ilo;                 //   dropped (tracked exactly per operation; V and I themselves are unchanged)
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
 };
@@ -685,6 +688,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     const n2h_pop_params& P = *reinterpret_cast<const n2h_pop_params*>(st.param_words);
     bool fired = false;
     float vpre = __builtin_inff();  // the potential the threshold sees (+inf: refractory)
+    float vpre_lo = 0.0f;           // its remainder (compensated LIF_EXP only)
     if (live && upd) {
         if (stride > 1) {
             // the window's earlier slots, oldest first: ((r[s-k+1] + r[s-k+2]) + ... + r[s-1]) + r[s]
@@ -715,7 +719,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                     R = P.ref_steps;
                 }
             }
-        } else {
+        } else if (!a.vlo) {
             I = I + in;
             if (R > 0) {
                 V = P.v_reset;
@@ -734,6 +738,47 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 V = P.v_reset;
                 R = P.ref_steps;
             }
+        } else {
+            // The same operations in the same order, each with its exact rounding error: V, I and the spikes
+            // are bit-identical to the branch above; vl / il carry the remainder of the trajectory that exact
+            // arithmetic on the same inputs would follow (second-order terms rounded once more).
+            using namespace n2h_twofloat;
+            float vl = a.vlo[nid], il = a.ilo[nid];
+            const df jc = two_prod(P.j_ext, static_cast<float>(c));
+            const df sx = two_sum(x, jc.hi);  // sx.hi == in
+            const df i1 = two_sum(I, sx.hi);
+            I = i1.hi;
+            il = i1.lo + (il + (sx.lo + jc.lo));
+            if (R > 0) {
+                V = P.v_reset;
+                vl = 0.0f;
+                R -= 1;
+            } else {
+                const df dv = two_sum(V, -P.v_rest);
+                const float dv_lo = dv.lo + vl;
+                const df pv = two_prod(P.decay, dv.hi);
+                const float pv_lo = pv.lo + P.decay * dv_lo;
+                const df vr = two_sum(P.v_rest, pv.hi);
+                const float vr_lo = vr.lo + pv_lo;
+                const df pvi = two_prod(P.p_vi, I);
+                const float pvi_lo = pvi.lo + P.p_vi * il;
+                const df vn = two_sum(vr.hi, pvi.hi);
+                V = vn.hi;
+                vl = vn.lo + (vr_lo + pvi_lo);
+                vpre = V;
+                vpre_lo = vl;
+            }
+            const df i2 = two_prod(P.p_i, I);
+            I = i2.hi;
+            il = i2.lo + P.p_i * il;
+            if (V > P.v_th) {
+                fired = true;
+                V = P.v_reset;
+                vl = 0.0f;
+                R = P.ref_steps;
+            }
+            a.vlo[nid] = vl;
+            a.ilo[nid] = il;
         }
     }
 
@@ -770,7 +815,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         }
         if (a.vtrace) {
             const int64_t q = s - a.vtrace_step0;
-            if (q >= 0 && q < a.vtrace_steps) a.vtrace[q * N + nid] = vpre;
+            if (q >= 0 && q < a.vtrace_steps) a.vtrace[q * N + nid] = (vpre - P.v_th) + vpre_lo;
         }
         // ring: add each touched cell's partial sum (arrival s - 1 + (b + 1), b < D); the cell's one add.
         // Without an update the delay-1 cell (arrival s) stays in the ring for the window's sum.
@@ -884,6 +929,8 @@ struct n2h_plan {
     float* vtrace;
     int64_t vtrace_step0;
     int32_t vtrace_steps;
+    float* vlo;
+    float* ilo;
     int32_t multirate;      // some population's stride > 1
     size_t lds_bytes;
     int device;
@@ -958,6 +1005,8 @@ StepArgs make_args(const n2h_plan* p) {
     a.vtrace = p->vtrace;
     a.vtrace_step0 = p->vtrace_step0;
     a.vtrace_steps = p->vtrace_steps;
+    a.vlo = p->vlo;
+    a.ilo = p->ilo;
     a.multirate = p->multirate;
     return a;
 }
@@ -1409,14 +1458,19 @@ int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, co
     return N2H_OK;
 }
 
-int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps) {
+int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps, float* v_lo, float* i_lo) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     if (buffer && n_steps <= 0) return fail(N2H_EINVAL, "n_steps must be positive");
-    if (buffer && p->multirate)
+    if ((buffer || v_lo) && p->multirate)
         return fail(N2H_EINVAL, "potential traces need every population updating every step (stride 1)");
+    if ((v_lo == nullptr) != (i_lo == nullptr)) return fail(N2H_EINVAL, "v_lo and i_lo go together");
+    if (v_lo && p->desc.model != N2H_MODEL_LIF_EXP)
+        return fail(N2H_EINVAL, "the compensated state is tracked for the exponential-synapse model only");
     p->vtrace = buffer;
     p->vtrace_step0 = step0;
     p->vtrace_steps = buffer ? n_steps : 0;
+    p->vlo = v_lo;
+    p->ilo = i_lo;
     return N2H_OK;
 }
 

@@ -41,6 +41,7 @@
 #include <mutex>
 
 #include "naer2_hip.h"
+#include "twofloat.h"
 
 namespace {
 
@@ -64,40 +65,11 @@ int rfail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return rfail(N2H_EHIP, "%s: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-// ---- double-float arithmetic (every operation rounds on its own: the library builds with
-//      -ffp-contract=off, and the one fused multiply-add is written out where the method needs it)
-
-struct df {
-    float hi, lo;
-};
-
-__device__ __forceinline__ df two_sum(float a, float b) {
-    const float s = a + b;
-    const float bb = s - a;
-    return {s, (a - (s - bb)) + (b - bb)};
-}
-
-__device__ __forceinline__ df fast_two_sum(float a, float b) {  // |a| >= |b| or a == 0
-    const float s = a + b;
-    return {s, b - (s - a)};
-}
-
-__device__ __forceinline__ df two_prod(float a, float b) {
-    const float p = a * b;
-    return {p, __builtin_fmaf(a, b, -p)};
-}
-
-__device__ __forceinline__ df df_add(df a, df b) {
-    df s = two_sum(a.hi, b.hi);
-    const float e = s.lo + (a.lo + b.lo);
-    return fast_two_sum(s.hi, e);
-}
-
-__device__ __forceinline__ df df_mul(df a, df b) {
-    df p = two_prod(a.hi, b.hi);
-    const float e = p.lo + (a.hi * b.lo + a.lo * b.hi);
-    return fast_two_sum(p.hi, e);
-}
+using n2h_twofloat::df;
+using n2h_twofloat::df_add;
+using n2h_twofloat::df_mul;
+using n2h_twofloat::two_prod;
+using n2h_twofloat::two_sum;
 
 __device__ __forceinline__ float spike_fn(float x, int mode, float slope) {
     if (mode == 0) return x > 0.0f ? 1.0f : 0.0f;

@@ -548,17 +548,27 @@ class Runtime:
         out["first entry -> last exit"] = float((t[:, 7].max() - t[:, 0].min()) / 100.0)
         return out
 
-    def trace_potentials(self, buffer: torch.Tensor | None, step0: int = 0) -> None:
-        """Write each neuron's potential as the threshold sees it (+inf on refractory steps) for steps
-        step0 .. step0 + len(buffer) - 1 into buffer [steps, n] (float32, on the device); None: off.  Results
-        are unchanged; captured windows are dropped so that the next run sees the setting."""
+    def trace_potentials(self, buffer: torch.Tensor | None, step0: int = 0,
+                         compensation: torch.Tensor | None = None) -> None:
+        """Write each neuron's distance to the threshold as the threshold sees it (V before reset minus V_th;
+        +inf on refractory steps) for steps step0 .. step0 + len(buffer) - 1 into buffer [steps, n] (float32, on
+        the device); None: off.  compensation (exponential-synapse networks): a float32 [2, n] device tensor
+        holding what fp32 rounding dropped from V and I (zeros: none yet), updated every step while set; the
+        trace is then the compensated distance.  Results are unchanged; captured windows are dropped so that
+        the next run sees the setting."""
         lib = _lib.load()
+        lo = (None, None)
+        if compensation is not None:
+            if (compensation.dtype != torch.float32 or tuple(compensation.shape) != (2, self.n)
+                    or not compensation.is_contiguous()):
+                raise ValueError("compensation: contiguous float32 [2, n_neurons]")
+            lo = (compensation[0].data_ptr(), compensation[1].data_ptr())
         if buffer is None:
-            _lib.check(lib.n2h_set_vtrace(self._plan, None, 0, 0), "n2h_set_vtrace")
+            _lib.check(lib.n2h_set_vtrace(self._plan, None, 0, 0, *lo), "n2h_set_vtrace")
         else:
             if buffer.dtype != torch.float32 or buffer.dim() != 2 or buffer.shape[1] != self.n or not buffer.is_contiguous():
                 raise ValueError("buffer: contiguous float32 [steps, n_neurons]")
-            _lib.check(lib.n2h_set_vtrace(self._plan, buffer.data_ptr(), int(step0), int(buffer.shape[0])),
+            _lib.check(lib.n2h_set_vtrace(self._plan, buffer.data_ptr(), int(step0), int(buffer.shape[0]), *lo),
                        "n2h_set_vtrace")
         for g in self._graphs.values():
             lib.n2h_graph_destroy(g)
