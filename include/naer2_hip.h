@@ -22,7 +22,10 @@
  *   n2h_graph_create / _launch / _destroy
  *                                       naer2/graph.py:35-136 (WindowGraph: a captured window of rounds)
  *   n2h_set_step / n2h_get_step         naer2/schedule.py:30-36 (write_entry: the run's round and times)
- *   n2h_status                          naer2/runtime.py:724-735 (check: flags raised by the kernel)
+ *   n2h_status                          plan metadata.  The reference's check (naer2/runtime.py:724-735) judges
+ *                                       flags its kernels raise on ring, history or log overflow; the
+ *                                       time-driven step here has nothing that can overflow (its spike lists
+ *                                       hold every neuron, its ring every delay), so it raises none
  *   n2h_last_error                      (error text of the last failed call; the reference raises RuntimeError)
  *
  * Conventions.  Every function returns an int status: N2H_OK (0) or a negative N2H_E* code; the text of

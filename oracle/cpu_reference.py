@@ -61,6 +61,7 @@ def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps
     V = torch.as_tensor(np.asarray(v0, dtype=np.float32)).clone()
     I = torch.as_tensor(np.asarray(i0, dtype=np.float32)).clone() if model == 2 else None
     R = torch.zeros(n, dtype=torch.int32)
+    per_neuron = torch.zeros(n, dtype=torch.int64)
     n_spk = 0
     n_ev = 0
     t0 = time.perf_counter()
@@ -86,5 +87,7 @@ def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps
                 buf[(s + k) % D] += W @ sv
             n_spk += int(idx.numel())
             n_ev += int(out_deg[idx].sum())
+            per_neuron[idx] += 1
     sec = time.perf_counter() - t0
-    return {"seconds": sec, "steps": n_steps, "spikes": n_spk, "events": n_ev, "threads": int(threads)}
+    return {"seconds": sec, "steps": n_steps, "spikes": n_spk, "events": n_ev, "threads": int(threads),
+            "per_neuron": per_neuron.numpy()}
