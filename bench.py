@@ -459,8 +459,8 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
-    fw, bw = grad.last["forward_ms"], grad.last["backward_ms"]
-    log(f"pd14grad: loss {loss:.4g}, forward {fw:.0f} ms, backward {bw:.0f} ms, wall {wall:.1f} s")
+    fw, bw, phases = grad.last["forward_ms"], grad.last["backward_ms"], grad.last["backward_phases_ms"]
+    log(f"pd14grad: loss {loss:.4g}, forward {fw:.0f} ms, backward {bw:.0f} ms ({phases}), wall {wall:.1f} s")
     del grad, rts
     torch.cuda.empty_cache()
     # the check: the same adjoint on a 4-neuron-per-population subsample against the FP64 unroll
@@ -490,6 +490,7 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
                                f"seed {args.seed}",
                    "n_synapses": n_syn},
         "forward_ms": round(fw, 1), "backward_ms": round(bw, 1),
+        "backward_phases_ms": {k: round(v, 1) for k, v in phases.items()},
         "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
         "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
         "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay, "long_parts": args.long_parts,
@@ -749,7 +750,7 @@ def main() -> int:
     ap.add_argument("--long-delay", type=int, default=4,
                     help="pd14grad: out-links with at least this delay (steps) are gathered once per window of "
                          "that many reverse steps")
-    ap.add_argument("--long-parts", type=int, default=24,
+    ap.add_argument("--long-parts", type=int, default=32,
                     help="pd14grad: target ranges the long out-links are walked in (a multiple of 8: one launch, "
                          "the XCDs taking ranges in turn; otherwise one launch a range)")
     ap.add_argument("--cutoff", type=float, default=0.2,

@@ -238,7 +238,8 @@ int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base,
  * One n2h_adj_step call runs reverse step `step` for up to 16 networks in one launch.  Per network: its
  * out-links (canonical CSR by source) split by delay into a short list (delay < long_delay steps) and a long
  * list (delay >= long_delay), the long list further split by target range (range k: targets t with
- * t * long_parts / n == k), each run kept in canonical order and packed {target | (delay_steps - 1) << 17 |
+ * t * long_parts / n == k, or with range_pop the caller's population-pure ranges), each run kept in canonical
+ * order and packed {target | (delay_steps - 1) << 17 |
  * target population << 24, base weight bits}; alpha[target pop][source pop] the weight scales the forward
  * ran with (its weights are alpha * base); pop_coef[p] = {P_V, P_I, P_VI, V_reset, V_th, V_th - V_reset,
  * dL/ds of one spike of population p, 0}; this step's distances to the threshold (n2h_set_vtrace) and
@@ -271,6 +272,9 @@ typedef struct n2h_adj_net {
     float* g_v;                  /* device [n] */
     float* g_i;                  /* device [n] */
     float* g_sum;                /* device [n][n_pops][2]: a compensated sum, hi + lo */
+    const int32_t* range_pop;    /* NULL, or device [long_parts] with long_parts a multiple of 8: the long
+                                    list's ranges are then population-pure (range k holds targets of population
+                                    range_pop[k] only; any split of each population's targets into ranges) */
 } n2h_adj_net;
 /* The surrogate: ds/dV = gamma (kappa/2) / dv / (1 + kappa |u|)^2 for |u| = |V - V_th| / dv <= cutoff, else 0
  * (dv = V_th - V_reset); neurons beyond the cutoff that did not spike skip their out-link gather. */

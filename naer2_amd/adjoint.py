@@ -39,7 +39,7 @@ class AdjNet(ctypes.Structure):
                 ("long_delay", ctypes.c_int32), ("long_parts", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
                     (k, ctypes.c_void_p) for k in (
                     "pop_first", "out_ptr", "out_links", "long_ptr", "long_links", "alpha", "pop_coef", "vtrace",
-                    "spikes", "gx_ring", "gx_recent", "long_bins", "g_v", "g_i", "g_sum")]
+                    "spikes", "gx_ring", "gx_recent", "long_bins", "g_v", "g_i", "g_sum", "range_pop")]
 
 
 class Surrogate(ctypes.Structure):
@@ -70,12 +70,38 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
 
 
+def pure_ranges(pop_first, long_parts: int):
+    """Population-pure target ranges: each non-empty population's targets split into near-equal contiguous
+    ranges, their number proportional to its size (at least one; largest remainders), long_parts in all.
+    (range_first int64 [long_parts + 1], range_pop int32 [long_parts]), or None when long_parts is not a
+    multiple of 8 or is below the number of non-empty populations."""
+    pf = np.asarray(pop_first, dtype=np.int64)
+    sizes = np.diff(pf)
+    live = np.nonzero(sizes > 0)[0]
+    if long_parts % 8 or long_parts < len(live):
+        return None
+    share = sizes[live] / sizes[live].sum() * long_parts
+    cnt = np.maximum(1, np.floor(share).astype(np.int64))
+    while cnt.sum() > long_parts:  # the minimum of one each overshot: take from counts above one
+        cnt[np.argmax(np.where(cnt > 1, cnt - share, -np.inf))] -= 1
+    rem = share - cnt
+    for k in np.argsort(-rem, kind="stable")[:long_parts - int(cnt.sum())]:
+        cnt[k] += 1
+    first, pops = [], []
+    for p, c in zip(live, cnt):
+        edges = pf[p] + (np.arange(c) * sizes[p]) // c
+        first += edges.tolist()
+        pops += [int(p)] * int(c)
+    return np.asarray(first + [int(pf[-1])], dtype=np.int64), np.asarray(pops, dtype=np.int32)
+
+
 def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1):
-    """(short_ptr, short_links, long_ptr, long_links): rt's canonical CSR by source packed for the adjoint,
-    {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight / alpha[pt][ps],
-    split into the links with delay < long_delay (ptr int64 [n + 1]) and the rest, these by (source, target
-    range k = target * long_parts // n) (ptr int64 [n * long_parts + 1]); links int32 [E, 2], each run in
-    canonical order."""
+    """(short_ptr, short_links, long_ptr, long_links, range_pop): rt's canonical CSR by source packed for the
+    adjoint, {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight /
+    alpha[pt][ps], split into the links with delay < long_delay (ptr int64 [n + 1]) and the rest, these by
+    (source, target range) (ptr int64 [n * long_parts + 1]); links int32 [E, 2], each run in canonical order.
+    The target ranges are population-pure (pure_ranges; range_pop int32 [long_parts] on the device) where
+    long_parts allows, else range k = target * long_parts // n (range_pop None)."""
     c = rt.canon
     if c is None:
         raise RuntimeError("the adjoint needs the canonical synapse table (Network.keep_canonical = True)")
@@ -92,9 +118,15 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1)
     base = c.w.to(dev) / scale
     links = torch.stack([x.to(torch.int32), base.contiguous().view(torch.int32)], dim=1).contiguous()
     is_long = c.d.to(dev) >= long_delay
+    pure = pure_ranges(rt.pop_first, long_parts) if long_parts > 1 else None
     out = []
     for keep, parts in ((~is_long, 1), (is_long, long_parts)):
-        key = src[keep] * parts + (tgt[keep] * parts) // rt.n
+        if parts > 1 and pure is not None:
+            rf = torch.as_tensor(pure[0], device=dev)
+            rk = torch.searchsorted(rf, tgt[keep], right=True) - 1
+        else:
+            rk = (tgt[keep] * parts) // rt.n
+        key = src[keep] * parts + rk
         cnt = torch.bincount(key, minlength=rt.n * parts)
         ptr = torch.zeros(rt.n * parts + 1, dtype=torch.int64, device=dev)
         ptr[1:] = cnt.cumsum(0)
@@ -104,13 +136,14 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1)
         else:
             sel = torch.zeros(1, 2, dtype=torch.int32, device=dev)  # never empty
         out += [ptr, sel.contiguous()]
+    out.append(None if pure is None else torch.as_tensor(pure[1], device=dev))
     return tuple(out)
 
 
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
                  cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
-                 long_delay: int = 4, long_parts: int = 24):
+                 long_delay: int = 4, long_parts: int = 32):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -214,11 +247,11 @@ class RateLossGrad:
             })
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         nets = (AdjNet * len(rts))()
-        for i, (rt, sd, (sptr, sl, lptr, ll)) in enumerate(zip(rts, st, self.tables)):
+        for i, (rt, sd, (sptr, sl, lptr, ll, rpop)) in enumerate(zip(rts, st, self.tables)):
             nets[i] = AdjNet(rt.n, self.P, R, W, self.long_parts, 0, self.pop_first[i].data_ptr(), sptr.data_ptr(), sl.data_ptr(),
                              lptr.data_ptr(), ll.data_ptr(), self.alpha.data_ptr(), sd["coef"].data_ptr(), 0, 0,
                              sd["gx"].data_ptr(), sd["gr"].data_ptr(), 0, sd["gv"].data_ptr(), sd["gi"].data_ptr(),
-                             sd["gsum"].data_ptr())
+                             sd["gsum"].data_ptr(), rpop.data_ptr() if rpop is not None else 0)
         sur = ctypes.byref(self.sur)
         sizes = (ctypes.c_int32 * len(rts))(*[rt.n for rt in rts])
         words = lib.n2h_adj_work_words(sizes, len(rts), W, self.long_parts)
@@ -226,6 +259,14 @@ class RateLossGrad:
             raise RuntimeError("n2h_adj_work_words: bad network sizes")
         work = torch.zeros(words, dtype=torch.int32, device=self.dev)
         wp, wn = work.data_ptr(), work.numel()
+        marks = []  # (phase, event) in stream order: the backward's GPU time by phase
+
+        def mark(phase):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            marks.append((phase, e))
+
+        mark("setup")
         for k in reversed(range(len(ckpts))):
             start = s0 + k * K
             L = min(K, T - k * K)
@@ -237,10 +278,12 @@ class RateLossGrad:
             fr = FusedRound(rts, self.tuning, reserve_steps=L)
             fr.launch(L, 0)
             fr.close()
+            mark("recompute")
             for rt, sd in zip(rts, st):
                 _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
                 gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
                 sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
+            mark("spikes")
             # windows of W steps from the segment's end: [t0, t1)
             # row addresses by arithmetic on the base pointers (a tensor view per row costs microseconds of host
             # time a step, as much as the step's kernels): vtrace fp32 [K][n], spikes u8 [K][n], long_bins [W][n][8]
@@ -254,6 +297,7 @@ class RateLossGrad:
                     for i, (vb, sb, lb, n) in enumerate(base):
                         nets[i].vtrace, nets[i].spikes, nets[i].long_bins = vb + 4 * t * n, sb + t * n, lb + 32 * (t - t0) * n
                     _check(lib.n2h_adj_step(nets, len(rts), start + t, t - t0, sur, wp, wn, stream), "n2h_adj_step")
+            mark("reverse")
         t_bw[1].record()
         for rt, sn, rf in zip(rts, final, rec_from):
             rt.restore(sn)
@@ -267,6 +311,10 @@ class RateLossGrad:
                 blk = sd["gsum"][pf[b]:pf[b + 1]]
                 g[:, b] += blk[:, :, 0].sum(dim=0) + blk[:, :, 1].sum(dim=0)
         torch.cuda.synchronize(self.dev)
+        phases = {}
+        for (_, a), (ph, b) in zip(marks[:-1], marks[1:]):
+            phases[ph] = phases.get(ph, 0.0) + a.elapsed_time(b)
+        phases["setup"] = t_bw[0].elapsed_time(marks[0][1])
         self.last = {"forward_ms": t_fw[0].elapsed_time(t_fw[1]), "backward_ms": t_bw[0].elapsed_time(t_bw[1]),
-                     "c_pops": c_pops}
+                     "backward_phases_ms": phases, "c_pops": c_pops}
         return loss, g.cpu().numpy().astype(np.float64)

@@ -103,6 +103,32 @@ __device__ inline int adj_pop(const n2h_adj_net& a, int j) {
 // steps (the trace holds +inf there)
 __device__ inline float adj_u(float dist, float dv) { return fabsf(dist) / dv; }
 
+// A link record, read once per window: a non-temporal load, so that the stream of records does not evict
+// the L2's ring rows of the range being walked, which every gatherer's links read again.
+__device__ __forceinline__ uint2 adj_load_link(const uint2* __restrict__ p) {
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+    const u2v v = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(p));
+    return make_uint2(v[0], v[1]);
+}
+
+// W consecutive floats from a dword-aligned address: 16-byte vector loads (gfx950 takes dword-aligned
+// global_load_dwordx4) for each full four, single loads for the rest -- one request per link and four
+// adjoints instead of four, which the texture path serves one lane address at a time
+template <int W>
+__device__ __forceinline__ void adj_load_window(const float* __restrict__ g, float (&out)[W]) {
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+#pragma unroll
+    for (int w = 0; w + 4 <= W; w += 4) {
+        const f4u v = *reinterpret_cast<const f4u*>(g + w);
+        out[w] = v[0];
+        out[w + 1] = v[1];
+        out[w + 2] = v[2];
+        out[w + 3] = v[3];
+    }
+#pragma unroll
+    for (int w = W & ~3; w < W; ++w) out[w] = g[w];
+}
+
 // one link {target | (d - 1) << 17 | target pop << 24, base weight}: the ring offset of gx_{t+d}[target]
 // (pos = t mod R; d < R) and the target population
 __device__ inline int64_t adj_link_slot(uint32_t x, int pos, int R, int RS, uint32_t& tp) {
@@ -114,7 +140,7 @@ __device__ inline int64_t adj_link_slot(uint32_t x, int pos, int R, int RS, uint
     return static_cast<int64_t>(tgt) * RS + slot;
 }
 
-// Sums v[0..V) (V a power of 2, 8 to 64) over the 64 lanes of the wavefront by recursive halving, a fixed
+// Sums v[0..V) (V a power of 2, 1 to 64) over the 64 lanes of the wavefront by recursive halving, a fixed
 // tree: at the stage of lane offset o, while more than one element is left, a lane keeps one half of its
 // elements (the upper half if its lane bit o is set) and adds its partner's copy of that half.  Lane l ends
 // with element l >> (6 - log2 V) in v[0]; 2 V - 2 + log2 (64 / V) lane exchanges instead of a butterfly's 6 V.
@@ -143,8 +169,8 @@ __device__ inline float adj_reduce_halving(float (&v)[V], int lane) {
     return v[0];
 }
 template <int V>
-constexpr int adj_halving_shift() {
-    return V == 64 ? 0 : V == 32 ? 1 : V == 16 ? 2 : 3;
+constexpr int adj_halving_shift() {  // 6 - log2 V: lane l ends with element l >> shift
+    return V == 64 ? 0 : V == 32 ? 1 : V == 16 ? 2 : V == 8 ? 3 : V == 4 ? 4 : V == 2 ? 5 : 6;
 }
 
 // Work lists.  Few neurons gather (a few percent a step), and they cluster in the more active populations,
@@ -158,7 +184,9 @@ constexpr int adj_halving_shift() {
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
-static_assert(kSplitBlocks % 8 == 0, "the split long kernel deals workgroups to 8 XCDs in turn");
+constexpr int kGroupLanes = 8;       // the population-pure long kernel: lanes a (gatherer, range) task
+constexpr int kGroupBlocks = 1536;   // its workgroups: 6 blocks per CU (74 VGPRs at W = 4: 6 wavefronts per SIMD)
+static_assert(kSplitBlocks % 8 == 0 && kGroupBlocks % 8 == 0, "the split long kernels deal workgroups to 8 XCDs in turn");
 constexpr int kSubLists = 16;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
@@ -184,6 +212,23 @@ __device__ inline AdjList adj_list_counts(const AdjBatch& b, int which) {
         l.total += l.cnt[c];
     }
     return l;
+}
+// (per lane: each lane's own entry i; adj_list_entry below takes a wavefront-uniform i)
+__device__ inline uint32_t adj_list_entry_lane(const AdjBatch& b, const AdjList& l, int which, int i) {
+    int k = i, sel = kSubLists - 1;
+    bool found = false;
+#pragma unroll
+    for (int c = 0; c < kSubLists; ++c) {
+        if (!found) {
+            if (k < l.cnt[c]) {
+                sel = c;
+                found = true;
+            } else {
+                k -= l.cnt[c];
+            }
+        }
+    }
+    return static_cast<uint32_t>(b.work[adj_sublist(b, which, sel) + k]);
 }
 __device__ inline uint32_t adj_list_entry(const AdjBatch& b, const AdjList& l, int which, int i) {
     int k = i, sel = kSubLists - 1;
@@ -267,15 +312,18 @@ constexpr int adj_win_pad() { return W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8; }
 // same order, as the launch-per-range path).
 constexpr int kXcds = 8;
 
-template <int W, bool SPLIT>
+// PURE (with SPLIT; the caller's range_pop given): every range holds targets of one population only, so a
+// task keeps W sums per lane instead of 8 W, without selects, and its partial row is [W'] for that population.
+template <int W, bool SPLIT, bool PURE>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
+    static_assert(SPLIT || !PURE, "population-pure ranges are walked by the split kernel only");
     constexpr int WP = adj_win_pad<W>();
-    constexpr int V = kAdjPops * WP;
+    constexpr int V = PURE ? WP : kAdjPops * WP;
     constexpr int SH = adj_halving_shift<V>();
     const int lane = threadIdx.x & (kWave - 1);
     const AdjList list = adj_list_counts(b, 0);
-    const int idx = lane >> SH;                    // the element this lane ends with: w * 8 + p
-    const int ww = idx >> 3, pp = idx & 7;
+    const int idx = lane >> SH;                    // the element this lane ends with: w * 8 + p (PURE: w)
+    const int ww = PURE ? idx : idx >> 3, pp = PURE ? 0 : idx & 7;
     const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int xcd = SPLIT ? static_cast<int>(blockIdx.x % kXcds) : 0;
@@ -306,16 +354,12 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 #pragma unroll
             for (int r = 0; r < kUnroll; ++r) {
                 const int64_t er = e + r * kWave;
-                l[r] = er < e1 ? links[er] : make_uint2(0u, 0u);  // padding: target 0, delay 1, unused
+                l[r] = er < e1 ? links[er] : make_uint2(0u, 0u);  // padding: target 0, delay 1, weight +0
             }
             float gw[kUnroll][W];
             uint32_t tp[kUnroll];
 #pragma unroll
-            for (int r = 0; r < kUnroll; ++r) {
-                const float* g = a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]);
-#pragma unroll
-                for (int w = 0; w < W; ++w) gw[r][w] = g[w];
-            }
+            for (int r = 0; r < kUnroll; ++r) adj_load_window<W>(a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]), gw[r]);
 #pragma unroll
             for (int r = 0; r < kUnroll; ++r) {
                 const bool live = e + r * kWave < e1;
@@ -323,10 +367,14 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     const float c = wt * gw[r][w];
+                    if constexpr (PURE) {
+                        bins[w] = live ? bins[w] + c : bins[w];
+                    } else {
 #pragma unroll
-                    for (int p = 0; p < kAdjPops; ++p)
-                        bins[w * kAdjPops + p] =
-                            (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c : bins[w * kAdjPops + p];
+                        for (int p = 0; p < kAdjPops; ++p)
+                            bins[w * kAdjPops + p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c
+                                                                                                  : bins[w * kAdjPops + p];
+                    }
                 }
             }
         }
@@ -341,11 +389,85 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
     }
 }
 
+// Population-pure ranges, LANES lanes a task: a wavefront takes 64 / LANES consecutive gatherers of one
+// range at once (their list entries, run bounds and links all in flight together), each walked by its lane
+// group with the group's lanes striding the run (e, e + LANES, ...), kUnroll links a lane per pass, then a
+// fixed butterfly over the group.  The partial row of (entry, range) is [W'], as adj_long_kernel<W, true,
+// true> writes it.  The XCD of workgroup b (b mod kXcds) walks ranges x, x + kXcds, ... in turn, as there.
+template <int W, int LANES>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_group_kernel(AdjBatch b) {
+    constexpr int WP = adj_win_pad<W>();
+    constexpr int G = kWave / LANES;  // tasks a wavefront
+    constexpr int U = 2 * kUnroll;    // links a lane per pass
+    const int lane = threadIdx.x & (kWave - 1);
+    const int sub = lane % LANES, gq = lane / LANES;
+    const AdjList list = adj_list_counts(b, 0);
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int xcd = static_cast<int>(blockIdx.x % kXcds);
+    const int f0 = static_cast<int>(blockIdx.x / kXcds) * kWavesPerBlock + wave;
+    const int fs = static_cast<int>(gridDim.x / kXcds) * kWavesPerBlock;
+    const int P = b.net[0].long_parts;
+    const int chunks = (list.total + G - 1) / G;  // wavefront tasks per range
+    const int64_t F = static_cast<int64_t>(P / kXcds) * chunks;
+    for (int64_t f = f0; f < F; f += fs) {
+        const int m = static_cast<int>(f / chunks);
+        const int i = static_cast<int>(f - static_cast<int64_t>(m) * chunks) * G + gq;
+        const int part = xcd + kXcds * m;
+        const bool task = i < list.total;
+        float bins[WP];
+#pragma unroll
+        for (int k = 0; k < WP; ++k) bins[k] = 0.0f;
+        if (task) {
+            const uint32_t ent = adj_list_entry_lane(b, list, 0, i);
+            const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
+            const n2h_adj_net& a = b.net[q];
+            const int R = a.ring_slots, RS = R + W - 1;
+            const int64_t seg = static_cast<int64_t>(jj) * a.long_parts + part;
+            const int64_t e0 = a.long_ptr[seg], e1 = a.long_ptr[seg + 1];
+            const uint2* links = reinterpret_cast<const uint2*>(a.long_links);
+            for (int64_t e = e0 + sub; e < e1; e += U * LANES) {
+                uint2 l[U];
+#pragma unroll
+                for (int r = 0; r < U; ++r) {
+                    const int64_t er = e + r * LANES;
+                    l[r] = er < e1 ? adj_load_link(links + er) : make_uint2(0u, 0u);  // padding: target 0, delay 1, weight +0
+                }
+                float gw[U][W];
+#pragma unroll
+                for (int r = 0; r < U; ++r) {
+                    uint32_t tp;
+                    adj_load_window<W>(a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp), gw[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < U; ++r) {
+                    const bool live = e + r * LANES < e1;
+                    const float wt = __uint_as_float(l[r].y);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) bins[w] = live ? bins[w] + wt * gw[r][w] : bins[w];
+                }
+            }
+        }
+#pragma unroll
+        for (int o = LANES / 2; o > 0; o /= 2) {  // every lane of the group takes part (task uniform per group)
+#pragma unroll
+            for (int w = 0; w < W; ++w) bins[w] += __shfl_xor(bins[w], o, kWave);
+        }
+        if (task && sub < W) {
+            float v = bins[0];
+#pragma unroll
+            for (int w = 1; w < W; ++w) v = sub == w ? bins[w] : v;
+            b.partial[(static_cast<int64_t>(i) * P + part) * WP + sub] = v;
+        }
+    }
+}
+
 // The window's gatherers, a wavefront each: lane w * 8 + p adds the entry's range sums of (window step w,
-// target population p) in range order into its long_bins row (steps in the entry's mask).
-template <int W>
+// target population p) in range order into its long_bins row (steps in the entry's mask).  PURE: the partial
+// rows are [W'] per range, and population p's sum takes the ranges whose range_pop is p, in range order.
+template <int W, bool PURE>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(AdjBatch b) {
-    constexpr int V = kAdjPops * adj_win_pad<W>();
+    constexpr int WP = adj_win_pad<W>();
+    constexpr int V = PURE ? WP : kAdjPops * WP;
     const int lane = threadIdx.x & (kWave - 1);
     const int n_waves = gridDim.x * kWavesPerBlock;
     const AdjList list = adj_list_counts(b, 0);
@@ -356,10 +478,22 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(Adj
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const uint32_t mk = ent >> 21;
         if (lane >= W * kAdjPops || !((mk >> ww) & 1u)) continue;
-        const float* src = b.partial + static_cast<int64_t>(i) * P * V + lane;
-        float sum = src[0];
-        for (int r = 1; r < P; ++r) sum = sum + src[static_cast<int64_t>(r) * V];
         const n2h_adj_net& a = b.net[q];
+        float sum = 0.0f;
+        if constexpr (PURE) {
+            const float* src = b.partial + static_cast<int64_t>(i) * P * V + ww;
+            bool first = true;
+            for (int r = 0; r < P; ++r) {
+                if (a.range_pop[r] != pp) continue;
+                const float x = src[static_cast<int64_t>(r) * V];
+                sum = first ? x : sum + x;
+                first = false;
+            }
+        } else {
+            const float* src = b.partial + static_cast<int64_t>(i) * P * V + lane;
+            sum = src[0];
+            for (int r = 1; r < P; ++r) sum = sum + src[static_cast<int64_t>(r) * V];
+        }
         a.long_bins[(static_cast<int64_t>(ww) * a.n_neurons + jj) * kAdjPops + pp] = sum;
     }
 }
@@ -512,6 +646,10 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
         if (ring >= 0 && ring != a.ring_slots) return fail(N2H_EINVAL, "every network needs the same ring_slots");
         if (win >= 0 && win != a.long_delay) return fail(N2H_EINVAL, "every network needs the same long_delay");
         if (i > 0 && a.long_parts != nets[0].long_parts) return fail(N2H_EINVAL, "every network needs the same long_parts");
+        if ((a.range_pop == nullptr) != (nets[0].range_pop == nullptr))
+            return fail(N2H_EINVAL, "range_pop given for some networks only");
+        if (a.range_pop && a.long_parts % kXcds != 0)
+            return fail(N2H_EINVAL, "net %d: population-pure ranges need long_parts a multiple of %d", i, kXcds);
         ring = a.ring_slots;
         win = a.long_delay;
         b.net[i] = a;
@@ -558,14 +696,19 @@ template <int W>
 void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
     const dim3 block(kWave * kWavesPerBlock);
     hipLaunchKernelGGL(adj_mark_kernel, dim3(static_cast<unsigned>(blocks)), block, 0, stream, b);
+    if (b.partial && b.net[0].range_pop) {  // population-pure ranges split over the XCDs, then added in order
+        hipLaunchKernelGGL((adj_long_group_kernel<W, kGroupLanes>), dim3(kGroupBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_combine_kernel<W, true>), dim3(kShortBlocks), block, 0, stream, b);
+        return;
+    }
     if (b.partial) {  // the ranges split over the XCDs, then added in order
-        hipLaunchKernelGGL((adj_long_kernel<W, true>), dim3(kSplitBlocks), block, 0, stream, b);
-        hipLaunchKernelGGL(adj_combine_kernel<W>, dim3(kShortBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_long_kernel<W, true, false>), dim3(kSplitBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_combine_kernel<W, false>), dim3(kShortBlocks), block, 0, stream, b);
         return;
     }
     for (int part = 0; part < b.net[0].long_parts; ++part) {  // one target range a launch
         b.part = part;
-        hipLaunchKernelGGL((adj_long_kernel<W, false>), dim3(kLongBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_long_kernel<W, false, false>), dim3(kLongBlocks), block, 0, stream, b);
     }
 }
 

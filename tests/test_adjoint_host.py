@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from naer2_amd import _lib
-from naer2_amd.adjoint import AdjNet, Surrogate, _bind, out_links
+from naer2_amd.adjoint import AdjNet, Surrogate, _bind, out_links, pure_ranges
 from naer2_amd.topology import canonical
 
 N2H_EINVAL = -1  # include/naer2_hip.h: a bad argument
@@ -34,13 +34,20 @@ def _decode(links):
     return x & 0x1FFFF, ((x >> 17) & 127) + 1, x >> 24, links[:, 1].contiguous().view(torch.float32).numpy()
 
 
-@pytest.mark.parametrize("long_delay,parts", [(1, 1), (4, 3), (4, 8), (13, 2)])
+@pytest.mark.parametrize("long_delay,parts", [(1, 1), (4, 3), (4, 8), (13, 2), (2, 16)])
 def test_out_link_tables_split_every_link_once_in_canonical_order(long_delay, parts):
     rt = _fake_runtime()
     c = rt.canon
     alpha = torch.tensor(np.random.default_rng(1).uniform(0.5, 2.0, (3, 3)), dtype=torch.float32)
-    sptr, sl, lptr, ll = out_links(rt, alpha, long_delay, parts)
+    sptr, sl, lptr, ll, rpop = out_links(rt, alpha, long_delay, parts)
     n = rt.n
+    pure = pure_ranges(rt.pop_first, parts) if parts > 1 else None
+    assert (rpop is None) == (pure is None) and (pure is not None) == (parts in (8, 16))
+    if pure is not None:
+        assert np.array_equal(rpop.numpy(), pure[1])
+        range_of = lambda t: np.searchsorted(pure[0], t, side="right") - 1
+    else:
+        range_of = lambda t: (t * parts) // n
     assert sptr.shape == (n + 1,) and lptr.shape == (n * parts + 1,)
     pop = np.repeat(np.arange(3), np.diff(rt.pop_first))
     ptr, tgt, dl, w = c.src_ptr.numpy(), c.tgt.numpy(), c.d.numpy(), c.w.numpy()
@@ -57,7 +64,7 @@ def test_out_link_tables_split_every_link_once_in_canonical_order(long_delay, pa
             for k in range(P):
                 run = range(tptr[j * P + k], tptr[j * P + k + 1])
                 # the run: source j's links into target range k, in canonical order
-                want = mine[(tgt[mine] * P) // n == k]
+                want = mine[(range_of(tgt[mine]) if P > 1 else 0 * tgt[mine]) == k]
                 assert len(run) == len(want)
                 for e, ci in zip(run, want):
                     assert t_tgt[e] == tgt[ci] and t_d[e] == dl[ci] and t_pop[e] == pop[tgt[ci]]
@@ -108,3 +115,19 @@ def test_adjoint_calls_refuse_bad_arguments_before_any_launch():
     bad_sur = Surrogate(0.0, 0.2, 0.01, 0.0)
     assert lib.n2h_adj_window(nets, 1, 0, 4, ctypes.byref(bad_sur), None, 0, None) == N2H_EINVAL
     assert lib.n2h_adj_window(nets, 0, 0, 4, ctypes.byref(sur), None, 0, None) == N2H_EINVAL
+
+
+@pytest.mark.parametrize("sizes,parts", [([20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948], 8),
+                                         ([20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948], 24),
+                                         ([10, 0, 20], 8), ([1, 1, 1, 1, 1, 1, 1, 1], 16), ([5, 7], 40)])
+def test_pure_ranges_cover_every_population_in_order(sizes, parts):
+    pf = np.concatenate([[0], np.cumsum(sizes)])
+    first, pop = pure_ranges(pf, parts)
+    assert len(pop) == parts and first[0] == 0 and first[-1] == pf[-1]
+    assert (np.diff(first) >= 0).all()  # ranges of populations smaller than their share may be empty
+    assert (np.diff(pop) >= 0).all()  # ranges in target order
+    for k in range(parts):  # every range inside one population
+        assert pf[pop[k]] <= first[k] and first[k + 1] <= pf[pop[k] + 1]
+    assert set(pop.tolist()) == {p for p, s in enumerate(sizes) if s > 0}
+    assert pure_ranges(pf, 12) is None  # not a multiple of 8
+    assert pure_ranges(pf, 0) is None or len(sizes) == 0
