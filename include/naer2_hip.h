@@ -316,6 +316,26 @@ int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t lon
  * they were made with. */
 int n2h_set_vtrace(n2h_plan* plan, float* buffer, int64_t step0, int32_t n_steps, float* v_lo, float* i_lo);
 
+/* The delivery log (naer2/runtime.py:747-752, PopulationRuntime.deliveries: (slot, e, connection, source,
+ * target) in log order).  While bound, every link a step delivers appends a record
+ * records[i][0..5) = {arrival step, emission step, connection, source neuron, target neuron} at
+ * i = (*count)++; past `capacity` the count still grows and nothing is written.  A step delivers the spikes of
+ * the step before it, every delay at once (the pull intake), so the records of one step are its emission
+ * step's links; each target's records appear in contract order (ascending canonical link index), targets
+ * interleaved.  link_conn: device int16 [n_links], each link's connection (creation order), parallel to
+ * tables->links; NULL: -1.  Logging steps run a variant without the row intake (results are identical); a
+ * logging plan cannot join a batch.  records NULL: off.  Graphs captured before the call keep the setting
+ * they were captured with. */
+int n2h_bind_log(n2h_plan* plan, const int16_t* link_conn, int32_t* records, int32_t* count, int64_t capacity);
+
+/* Status rows (the per-group status of naer2/transpiler/templates/fused.py:865-883, which the reference
+ * writes for its runs: blocks held, most logged, lost bits, done): with rows, device int32 [n_groups][4], every
+ * step writes row g = {the step, spikes group g emitted, its intake path (0: no spikes came in, 1: sorted
+ * spike list, 2: spike bitmap scan, 3: row intake), spiking sources it took in}.  Nothing here can be lost or
+ * held, so the rows carry the step's account instead.  NULL: off.  Graphs and batches keep the setting they
+ * were made with. */
+int n2h_set_status(n2h_plan* plan, int32_t* rows);
+
 /* In-kernel phase trace (replaces naer2/trace.py's in-kernel stamps, compiled in by TRACE): with a buffer,
  * lane 0 of every group stamps the 100 MHz wall clock at 8 points of each step it runs -- entry, spike list
  * read, sources ranked, segment bounds, links staged, walked, updated, exit -- into

@@ -115,6 +115,11 @@ struct StepArgs {
     int32_t vtrace_steps;
     float* vlo;                 // LIF_EXP, null or [n_neurons] each: the remainders of V and I that fp32 rounding
     float* ilo;                 //   dropped (tracked exactly per operation; V and I themselves are unchanged)
+    int32_t* log_rec;           // delivery log (the logging kernel variant only): [log_cap][5] records, null: off
+    int32_t* log_count;         //   records appended so far (grows past log_cap; nothing written there)
+    const int16_t* log_conn;    //   connection index of each link, parallel to links (null: -1)
+    int64_t log_cap;
+    int32_t* status;            // null, or [n_groups][4] status rows, written by every step
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
 };
@@ -170,11 +175,12 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
 // Shared-memory staging of one workgroup.  BM: 64-bit words of the row intake's spike bitmap (the
 // groups' words, then the sentinel word); kernel variants size it for the groups they serve.
 // CAPS / CAPL: spiking sources and links staged per pass; RU: row cells in flight per lane and batch.
-template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU>
+template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU, bool LOG = false>
 struct Stage {
     static constexpr int kS = CAPS;
     static constexpr int kL = CAPL;
     static constexpr int kRU = RU;
+    static constexpr bool kLog = LOG;
     union {
         struct {                     // segment intake
             int32_t sp[CAPS];            // spiking source slots of this pass, ascending
@@ -184,6 +190,8 @@ struct Stage {
             uint2 sorted[CAPL];          // the staged links regrouped by target, each target's in contract order
             int32_t tcount[kWave];       // staged links per target
             int32_t tstart[kWave];       // first of each target's links in sorted
+            int64_t log_idx[LOG ? CAPL : 1];  // delivery log: each sorted link's index into the link table
+            int32_t log_src[LOG ? CAPL : 1];  //   and its source slot
         };
         struct {                     // row intake
             unsigned long long bm[BM];   // the previous step's spike bitmap, then a zero word
@@ -224,9 +232,20 @@ __device__ __forceinline__ void cell_add(float* acc, int lane, int b, float w, i
 // links, in contract order, adding each weight to its (delay, t) partial sum: the target-lane walk of the
 // reference's _walk (fused.py:939-991) without every lane reading every link.  Touched delays are
 // recorded in the lane's own mask (t0: delays 1..64, t1: 65..128).
+// Logging (St::kLog): each staged link's table index and source slot travel with it through the sort, and
+// every target lane appends a record per link it walks, in walk order (records of this chunk at one base
+// reserved by lane 0): {arrival step, emission step, connection, source neuron, target neuron}.
+struct LogCtx {
+    int64_t s;     // the step: the staged links are the spikes of step s - 1
+    int first;     // the target group's first neuron
+    const int64_t* idx;
+    const int32_t* src;
+};
+
 template <class St>
 __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl, const uint2 (&r)[St::kL / kWave],
-                                           unsigned long long& t0, unsigned long long& t1) {
+                                           unsigned long long& t0, unsigned long long& t1, const StepArgs& a,
+                                           const LogCtx& lc) {
     constexpr int kPer = St::kL / kWave;
     const unsigned long long below = (1ull << lane) - 1ull;
     st.tcount[lane] = 0;
@@ -266,15 +285,41 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = u * kWave + lane;
-        if (e < nl) st.sorted[st.tstart[ln[u].x & 0xffffu] + off[u]] = ln[u];
+        if (e < nl) {
+            const int pos = st.tstart[ln[u].x & 0xffffu] + off[u];
+            st.sorted[pos] = ln[u];
+            if constexpr (St::kLog) {
+                st.log_idx[pos] = lc.idx[u];
+                st.log_src[pos] = lc.src[u];
+            }
+        }
     }
     __syncthreads();
     const int j0 = incl - cnt;
+    int64_t log_base = 0;
+    if constexpr (St::kLog) {
+        int b = 0;
+        if (lane == 0) b = atomicAdd(a.log_count, nl);
+        log_base = __builtin_amdgcn_readfirstlane(b);
+    }
     int cb = -1;
     float cr = 0.0f;
     for (int j = j0; j < incl; ++j) {
         const uint2 l = st.sorted[j];
         cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, __uint_as_float(l.y), cb, cr, t0, t1);
+        if constexpr (St::kLog) {
+            const int64_t at = log_base + j;
+            if (at < a.log_cap) {
+                const int slot = st.log_src[j];
+                const int32_t src = a.group_first[slot >> 6] + (slot & (kWave - 1));
+                int32_t* rec = a.log_rec + at * 5;
+                rec[0] = static_cast<int32_t>(lc.s - 1 + static_cast<int64_t>(l.x >> 16));
+                rec[1] = static_cast<int32_t>(lc.s - 1);
+                rec[2] = a.log_conn ? static_cast<int32_t>(a.log_conn[st.log_idx[j]]) : -1;
+                rec[3] = src;
+                rec[4] = lc.first + lane;
+            }
+        }
     }
     if (cb >= 0) acc[cb * kWave + lane] = cr;
 }
@@ -381,13 +426,13 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
 
 template <class St>
 __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
-                                              unsigned long long& t0, unsigned long long& t1);
+                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first);
 
 // Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
 // contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
 template <class St>
 __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* acc, int g, int lane, int n_sp,
-                                             unsigned long long& t0, unsigned long long& t1) {
+                                             unsigned long long& t0, unsigned long long& t1, int64_t s, int first) {
     const int NG = a.n_groups;
     // segment bounds: every load of the pass issued before the first is used
     int64_t lo_r[St::kS / kWave], hi_r[St::kS / kWave];
@@ -417,14 +462,14 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
     if (lane == 0) st.seg_start[n_sp] = carry;
     __syncthreads();
     N2H_STAMP(3);
-    deliver_links(a, st, acc, lane, n_sp, carry, t0, t1);
+    deliver_links(a, st, acc, lane, n_sp, carry, t0, t1, s, first);
 }
 
 // The L links of the n_sp sources whose segment bounds are in st (seg_lo, seg_start): staged in chunks of
 // the stage's kL in contract order and walked into acc.
 template <class St>
 __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
-                                              unsigned long long& t0, unsigned long long& t1) {
+                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first) {
     for (int e0 = 0; e0 < L; e0 += St::kL) {
         const int e1 = min(L, e0 + St::kL);
         // owner[j]: the source whose segment holds link e0 + j.  Short chunks: a binary search per link;
@@ -479,18 +524,25 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
             __syncthreads();
         }
         uint2 r[kPer];
+        int64_t lidx[St::kLog ? kPer : 1];
+        int32_t lsrc[St::kLog ? kPer : 1];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int e = e0 + u * kWave + lane;
             r[u] = make_uint2(0u, 0u);
             if (e < e1) {
                 const int i = st.owner[e - e0];
-                r[u] = a.links[st.seg_lo[i] + (e - st.seg_start[i])];
+                const int64_t li = st.seg_lo[i] + (e - st.seg_start[i]);
+                r[u] = a.links[li];
+                if constexpr (St::kLog) {
+                    lidx[u] = li;
+                    lsrc[u] = st.sp[i];
+                }
             }
         }
         N2H_STAMP(4);
         const int nl = e1 - e0;
-        walk_chunk(st, acc, lane, nl, r, t0, t1);
+        walk_chunk(st, acc, lane, nl, r, t0, t1, a, LogCtx{s, first, lidx, lsrc});
         __syncthreads();
     }
 }
@@ -565,7 +617,9 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     }
 
     unsigned long long t0 = 0ull, t1 = 0ull;  // delays 1..64 and 65..128 with a partial sum for this lane
+    int path = 0;  // the intake this step took (status rows): 0 none, 1 sorted list, 2 bitmap scan, 3 rows
     if (KW > 0 && a.rows && n_prev >= a.dense_min) {
+        path = 3;
         // row intake (a CSR-by-target pull): every target lane walks its own in-links in
         // contract order against the previous step's spike bitmap held in LDS -- the whole table, so this
         // path pays off when a large share of the sources spiked
@@ -595,6 +649,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         }
         __syncthreads();
     } else if (n_prev <= kWave) {
+        path = n_prev > 0 ? 1 : 0;
         // sparse path: sort the (atomically appended) source slots, then deliver them.  Each lane's
         // segment bounds are loaded before the ranking ballots, which hide their latency, and land in LDS
         // at the lane's rank.
@@ -622,9 +677,10 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             if (lane == 0) st.seg_start[n_prev] = total;
             __syncthreads();
             N2H_STAMP(3);
-            deliver_links(a, st, acc, lane, n_prev, total, t0, t1);
+            deliver_links(a, st, acc, lane, n_prev, total, t0, t1, s, first);
         }
     } else {
+        path = 2;
         // burst path: the spike bitmap of step s-1, lane-contiguous words, ascending source slots
         const int64_t prev_slot = ((s - 1) % a.bit_slots + a.bit_slots) % a.bit_slots;
         const unsigned long long* prev = a.bits + prev_slot * NG;
@@ -672,7 +728,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 }
             }
             __syncthreads();
-            deliver_pass(a, st, acc, g, lane, r1 - r0, t0, t1);
+            deliver_pass(a, st, acc, g, lane, r1 - r0, t0, t1, s, first);
         }
     }
 
@@ -797,6 +853,10 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         }
     }
     if (lane == 0) a.bits[(s % a.bit_slots) * NG + g] = mask;
+    if (a.status && lane == 0) {  // this group's status row: {step, spikes emitted, intake path, sources seen}
+        int4* row = reinterpret_cast<int4*>(a.status) + g;
+        *row = make_int4(static_cast<int32_t>(s), __popcll(mask), path, n_prev);
+    }
     // the list of step s+1 starts empty (its slot was last read during step s-2)
     if (g == 0 && lane == 0) a.spikes[((slot_cur + 1) & (kListSlots - 1)) * a.list_stride] = 0;
 
@@ -845,6 +905,15 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     __shared__ Stage<bm_words(KW)> st;
     extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
     step_group<MODEL, KW>(a, blockIdx.x, k, st, acc);
+}
+
+// The logging variant (n2h_bind_log): the step without the row intake (KW = 0: segment intakes only, which
+// give the same results), every delivered link recorded.
+template <int MODEL>
+__global__ __launch_bounds__(kWave) void step_log_kernel(StepArgs a, int32_t k) {
+    __shared__ Stage<bm_words(0), kCapS, kCapL, kRowU, true> st;
+    extern __shared__ float acc[];
+    step_group<MODEL, 0>(a, blockIdx.x, k, st, acc);
 }
 
 // Batched step: the target groups of many independent networks (plans) in one grid.  Tiles are the
@@ -931,6 +1000,11 @@ struct n2h_plan {
     int32_t vtrace_steps;
     float* vlo;
     float* ilo;
+    int32_t* log_rec;
+    int32_t* log_count;
+    const int16_t* log_conn;
+    int64_t log_cap;
+    int32_t* status;
     int32_t multirate;      // some population's stride > 1
     size_t lds_bytes;
     int device;
@@ -1007,6 +1081,11 @@ StepArgs make_args(const n2h_plan* p) {
     a.vtrace_steps = p->vtrace_steps;
     a.vlo = p->vlo;
     a.ilo = p->ilo;
+    a.log_rec = p->log_rec;
+    a.log_count = p->log_count;
+    a.log_conn = p->log_conn;
+    a.log_cap = p->log_cap;
+    a.status = p->status;
     a.multirate = p->multirate;
     return a;
 }
@@ -1027,8 +1106,16 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
-        if (p->desc.model == N2H_MODEL_LIF_DELTA) launch_step_model<N2H_MODEL_LIF_DELTA>(words, grid, p->lds_bytes, stream, a, k);
-        else launch_step_model<N2H_MODEL_LIF_EXP>(words, grid, p->lds_bytes, stream, a, k);
+        if (p->log_rec) {
+            if (p->desc.model == N2H_MODEL_LIF_DELTA)
+                hipLaunchKernelGGL(step_log_kernel<N2H_MODEL_LIF_DELTA>, grid, block, p->lds_bytes, stream, a, k);
+            else
+                hipLaunchKernelGGL(step_log_kernel<N2H_MODEL_LIF_EXP>, grid, block, p->lds_bytes, stream, a, k);
+        } else if (p->desc.model == N2H_MODEL_LIF_DELTA) {
+            launch_step_model<N2H_MODEL_LIF_DELTA>(words, grid, p->lds_bytes, stream, a, k);
+        } else {
+            launch_step_model<N2H_MODEL_LIF_EXP>(words, grid, p->lds_bytes, stream, a, k);
+        }
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, stream, p->step_ctr, n_steps);
     N2H_HIP(hipGetLastError());
@@ -1305,6 +1392,8 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     if (!plans || !out) return fail(N2H_EINVAL, "null argument");
     *out = nullptr;
     if (n_plans < 1 || n_plans > kMaxBatch) return fail(N2H_EINVAL, "n_plans %d outside [1, %d]", n_plans, kMaxBatch);
+    for (int32_t i = 0; i < n_plans; ++i)
+        if (plans[i] && plans[i]->log_rec) return fail(N2H_EINVAL, "plan %d logs its deliveries: a logging plan steps alone", i);
     n2h_tuning tu{};
     if (tuning) tu = *tuning;
     if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
@@ -1471,6 +1560,22 @@ int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps, f
     p->vtrace_steps = buffer ? n_steps : 0;
     p->vlo = v_lo;
     p->ilo = i_lo;
+    return N2H_OK;
+}
+
+int n2h_bind_log(n2h_plan* p, const int16_t* link_conn, int32_t* records, int32_t* count, int64_t capacity) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (records && (!count || capacity < 0)) return fail(N2H_EINVAL, "a log needs its count and a capacity >= 0");
+    p->log_rec = records;
+    p->log_count = records ? count : nullptr;
+    p->log_conn = records ? link_conn : nullptr;
+    p->log_cap = records ? capacity : 0;
+    return N2H_OK;
+}
+
+int n2h_set_status(n2h_plan* p, int32_t* rows) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    p->status = rows;
     return N2H_OK;
 }
 

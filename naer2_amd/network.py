@@ -220,8 +220,10 @@ class Assembled:
         return int(self.pop_first[-1])
 
 
-def assemble(net: Network, device=None) -> Assembled:
-    """Everything the plan needs, on `device` (default: the network's), without touching the GPU."""
+def assemble(net: Network, device=None, with_connections: bool = False) -> Assembled:
+    """Everything the plan needs, on `device` (default: the network's), without touching the GPU.
+    with_connections: the canonical table also carries each link's connection index (creation order) as
+    its int16 payload (`canon.extra`)."""
     pops = sorted(net.populations, key=lambda p: p._order)
     fams = {p.family for p in pops}
     if None in fams:
@@ -246,18 +248,22 @@ def assemble(net: Network, device=None) -> Assembled:
         par = p.instance.params(k * dt)  # propagators, Poisson mean and refractory updates for its interval
         par.stride = k
         params.append(par)
-    srcs, tgts, ws, ds = [], [], [], []
-    for c in net.connections:
+    srcs, tgts, ws, ds, cs = [], [], [], [], []
+    if with_connections and len(net.connections) > 32767:
+        raise ValueError("at most 32767 connections carry their index")
+    for ci, c in enumerate(net.connections):
         if c._weight is None or c._delay is None:
             raise ValueError(f"{c!r}: weight and delay must be set")
         srcs.append(c.src.to(dev, torch.int64) + first_of[c.source])
         tgts.append(c.tgt.to(dev, torch.int64) + first_of[c.target])
         ws.append(c.weight.to(dev))
         ds.append(delay_steps(c.delay.to(dev), dt))
+        if with_connections:
+            cs.append(torch.full((srcs[-1].numel(),), ci, dtype=torch.int16, device=dev))
     cat = (lambda xs, dt_: torch.cat(xs) if xs else torch.zeros(0, dtype=dt_, device=dev))
     canon = canonical(n, cat(srcs, torch.int64), cat(tgts, torch.int64), cat(ws, torch.float32),
-                      cat(ds, torch.int32))
-    del srcs, tgts, ws, ds
+                      cat(ds, torch.int32), cat(cs, torch.int16) if with_connections else None)
+    del srcs, tgts, ws, ds, cs
     v = torch.empty(n, dtype=torch.float32, device=dev)
     isyn = torch.zeros(n, dtype=torch.float32, device=dev) if model == MODEL_LIF_EXP else None
     ref = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -520,6 +526,58 @@ class Runtime:
         row[0] = len(slots)
         row[1:1 + len(slots)] = torch.from_numpy(slots[::-1].astype(np.int32).copy())  # any order
         self.spike_list[step % 4] = row.to(self.device)
+
+    # -- readback formats: the delivery log and status rows (naer2/runtime.py:747-752, fused.py:865-883)
+    def set_delivery_log(self, capacity: int | None) -> None:
+        """Record every link the following steps deliver (n2h_bind_log), up to `capacity` records; None:
+        off.  Needs the connections' link tables (Network.keep_description).  Results do not change."""
+        lib = _lib.load()
+        if capacity is None:
+            _lib.check(lib.n2h_bind_log(self._plan, None, None, None, 0), "n2h_bind_log")
+            self._log = None
+        else:
+            if not self.net.keep_description:
+                raise RuntimeError("the delivery log needs the connections' links (Network.keep_description)")
+            a = assemble(self.net, self.device, with_connections=True)
+            conn = segments(a.canon, self.layout).extra.contiguous()
+            del a
+            rec = torch.zeros(int(capacity), 5, dtype=torch.int32, device=self.device)
+            count = torch.zeros(1, dtype=torch.int32, device=self.device)
+            self._log = (conn, rec, count)
+            _lib.check(lib.n2h_bind_log(self._plan, conn.data_ptr(), rec.data_ptr() if capacity else count.data_ptr(),
+                                        count.data_ptr(), int(capacity)), "n2h_bind_log")
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
+
+    def deliveries(self) -> list[tuple[int, int, int, int, int]]:
+        """The delivery log in log order: (arrival step, emission step, connection, source, target), network
+        neuron ids; raises if records were lost to the capacity."""
+        conn, rec, count = self._log
+        torch.cuda.synchronize(self.device)
+        n = int(count.item())
+        if n > rec.shape[0]:
+            raise RuntimeError(f"the delivery log overflowed: {n} records, capacity {rec.shape[0]}")
+        return [tuple(r) for r in rec[:n].cpu().tolist()]
+
+    def delivery_count(self) -> int:
+        torch.cuda.synchronize(self.device)
+        return int(self._log[2].item())
+
+    def set_status_rows(self, on: bool = True) -> None:
+        """Have every step write its per-group status rows (n2h_set_status)."""
+        lib = _lib.load()
+        self._status = torch.full((self.n_groups, 4), -1, dtype=torch.int32, device=self.device) if on else None
+        _lib.check(lib.n2h_set_status(self._plan, self._status.data_ptr() if on else None), "n2h_set_status")
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
+
+    def status_rows(self) -> np.ndarray:
+        """int32 [n_groups, 4]: per target group, the last step it ran, the spikes it emitted then, its intake
+        path (0 none, 1 sorted list, 2 bitmap scan, 3 row intake) and the spiking sources it took in."""
+        torch.cuda.synchronize(self.device)
+        return self._status.cpu().numpy()
 
     TRACE_EVENTS = ("entry", "list read", "sources ranked", "segments", "links staged", "walked", "updated",
                     "exit")

@@ -29,6 +29,7 @@ class Canonical:
     tgt: torch.Tensor      # int32 [E]
     w: torch.Tensor        # float32 [E]
     d: torch.Tensor        # int32 [E], delay in steps, >= 1
+    extra: torch.Tensor | None = None  # an optional per-link payload in the same order (e.g. connection ids)
 
     @property
     def n_links(self) -> int:
@@ -85,9 +86,10 @@ def _stable_order_by_key(key: torch.Tensor, n_keys: int, chunk: int | None = Non
     return pos
 
 
-def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: torch.Tensor) -> Canonical:
+def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: torch.Tensor,
+              extra: torch.Tensor | None = None) -> Canonical:
     """The canonical CSR of links given in any source order (a stable sort by source keeps each source's
-    links in their given order)."""
+    links in their given order); extra, a per-link payload, follows the links."""
     src = src.to(torch.int64)
     if src.numel() and (int(src.min()) < 0 or int(src.max()) >= n):
         raise ValueError("link source out of range")
@@ -105,7 +107,11 @@ def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: 
     out_t[pos] = tgt.to(torch.int32)
     out_w[pos] = w.to(torch.float32)
     out_d[pos] = d.to(torch.int32)
-    return Canonical(n, src_ptr, out_t, out_w, out_d)
+    out_x = None
+    if extra is not None:
+        out_x = torch.empty_like(extra)
+        out_x[pos] = extra
+    return Canonical(n, src_ptr, out_t, out_w, out_d, out_x)
 
 
 @dataclass
@@ -143,6 +149,7 @@ def group_layout(pop_sizes: list[int], device="cpu") -> GroupLayout:
 class Segments:
     seg_off: torch.Tensor  # int64 [64 * G * G + 1]
     links: torch.Tensor    # int32 [E, 2]: (delay << 16 | local target, weight bits)
+    extra: torch.Tensor | None = None  # the canonical table's per-link payload, in segment order
 
 
 def slot_of(layout: GroupLayout) -> torch.Tensor:
@@ -172,6 +179,7 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
     del counts
     # (links are grouped by source already: a stable sort by (source slot, group) chunk by chunk of sources)
     links = torch.empty(key.numel(), 2, dtype=torch.int32, device=dev)
+    extra = torch.empty_like(canon.extra) if canon.extra is not None else None
     E = key.numel()
     sp = canon.src_ptr
     a = 0
@@ -188,10 +196,12 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
         x = (canon.d[a:b].to(torch.int64)[order] << 16) | local_of[t]
         links[a:b, 0] = x.to(torch.int32)
         links[a:b, 1] = canon.w[a:b][order].contiguous().view(torch.int32)
+        if extra is not None:
+            extra[a:b] = canon.extra[a:b][order]
         del k, order, t, x
         a = b
     del key
-    return Segments(seg_off, links)
+    return Segments(seg_off, links, extra)
 
 
 ROWS_WIDE, ROWS_COMPACT = 0, 1
