@@ -455,7 +455,8 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
     n_syn = sum(rt.n_links for rt in rts)
     log(f"pd14grad: built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s")
     sur = {"cutoff": args.cutoff, "gamma": args.gamma}
-    grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, long_parts=args.long_parts, **sur)
+    grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, long_parts=args.long_parts,
+                        pure_ranges=args.pure_ranges, **sur)
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
@@ -494,6 +495,7 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
         "forward_rtf_aggregate": round(n_inst * bio / (fw / 1000.0), 3),
         "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
         "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay, "long_parts": args.long_parts,
+        "pure_ranges": bool(args.pure_ranges),
         "grad_rel_err_vs_fp64_subsample": err,
         "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
     }
@@ -750,9 +752,12 @@ def main() -> int:
     ap.add_argument("--long-delay", type=int, default=4,
                     help="pd14grad: out-links with at least this delay (steps) are gathered once per window of "
                          "that many reverse steps")
-    ap.add_argument("--long-parts", type=int, default=32,
+    ap.add_argument("--long-parts", type=int, default=24,
                     help="pd14grad: target ranges the long out-links are walked in (a multiple of 8: one launch, "
                          "the XCDs taking ranges in turn; otherwise one launch a range)")
+    ap.add_argument("--pure-ranges", action="store_true",
+                    help="pd14grad: population-pure target ranges (each population's targets split apart) instead of "
+                         "equal ones across populations")
     ap.add_argument("--cutoff", type=float, default=0.2,
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,

@@ -95,7 +95,7 @@ def pure_ranges(pop_first, long_parts: int):
     return np.asarray(first + [int(pf[-1])], dtype=np.int64), np.asarray(pops, dtype=np.int32)
 
 
-def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1):
+def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1, pure: bool = False):
     """(short_ptr, short_links, long_ptr, long_links, range_pop): rt's canonical CSR by source packed for the
     adjoint, {target | (delay - 1) << 17 | target pop << 24, base weight bits} with base = weight /
     alpha[pt][ps], split into the links with delay < long_delay (ptr int64 [n + 1]) and the rest, these by
@@ -118,7 +118,7 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1)
     base = c.w.to(dev) / scale
     links = torch.stack([x.to(torch.int32), base.contiguous().view(torch.int32)], dim=1).contiguous()
     is_long = c.d.to(dev) >= long_delay
-    pure = pure_ranges(rt.pop_first, long_parts) if long_parts > 1 else None
+    pure = pure_ranges(rt.pop_first, long_parts) if (pure and long_parts > 1) else None
     out = []
     for keep, parts in ((~is_long, 1), (is_long, long_parts)):
         if parts > 1 and pure is not None:
@@ -143,7 +143,7 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1)
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
                  cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
-                 long_delay: int = 4, long_parts: int = 32):
+                 long_delay: int = 4, long_parts: int = 24, pure_ranges: bool = False):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -176,7 +176,7 @@ class RateLossGrad:
         self.sur = Surrogate(float(kappa), float(cutoff), float(gamma), 0.0)
         self.long_delay = int(long_delay)
         self.long_parts = int(long_parts)
-        self.tables = [out_links(rt, self.alpha, self.long_delay, self.long_parts) for rt in runtimes]
+        self.tables = [out_links(rt, self.alpha, self.long_delay, self.long_parts, pure_ranges) for rt in runtimes]
         self.pop_first = [torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int32), device=self.dev) for rt in runtimes]
         self.ring_slots = max(rt.d_slots for rt in runtimes) + 1
         if any(int(rt.canon.d.max()) >= self.ring_slots for rt in runtimes if rt.canon.d.numel()):

@@ -34,15 +34,17 @@ def _decode(links):
     return x & 0x1FFFF, ((x >> 17) & 127) + 1, x >> 24, links[:, 1].contiguous().view(torch.float32).numpy()
 
 
-@pytest.mark.parametrize("long_delay,parts", [(1, 1), (4, 3), (4, 8), (13, 2), (2, 16)])
-def test_out_link_tables_split_every_link_once_in_canonical_order(long_delay, parts):
+@pytest.mark.parametrize("long_delay,parts,want_pure", [(1, 1, False), (4, 3, False), (4, 8, False), (4, 8, True),
+                                                       (13, 2, False), (2, 16, True), (4, 12, True)])
+def test_out_link_tables_split_every_link_once_in_canonical_order(long_delay, parts, want_pure):
     rt = _fake_runtime()
     c = rt.canon
     alpha = torch.tensor(np.random.default_rng(1).uniform(0.5, 2.0, (3, 3)), dtype=torch.float32)
-    sptr, sl, lptr, ll, rpop = out_links(rt, alpha, long_delay, parts)
+    sptr, sl, lptr, ll, rpop = out_links(rt, alpha, long_delay, parts, pure=want_pure)
     n = rt.n
-    pure = pure_ranges(rt.pop_first, parts) if parts > 1 else None
-    assert (rpop is None) == (pure is None) and (pure is not None) == (parts in (8, 16))
+    pure = pure_ranges(rt.pop_first, parts) if (want_pure and parts > 1) else None
+    # population-pure ranges where asked for and possible (a multiple of 8, at least one per population)
+    assert (rpop is None) == (pure is None) and (pure is not None) == (want_pure and parts in (8, 16))
     if pure is not None:
         assert np.array_equal(rpop.numpy(), pure[1])
         range_of = lambda t: np.searchsorted(pure[0], t, side="right") - 1

@@ -114,21 +114,25 @@ def test_gradient_is_independent_of_checkpoints_and_batching():
 
 def test_gradient_agrees_over_window_lengths():
     """The long-link windows and target ranges only regroup each source's sum (short links + long links, the
-    long ones range by range): every window length and range count, with checkpoints that are not a multiple
-    of the window, gives the single-step gradient to fp32 rounding."""
+    long ones range by range): every window length, range count and range layout (equal ranges across
+    populations, or population-pure ones walked eight tasks a wavefront), with checkpoints that are not a
+    multiple of the window, gives the single-step gradient to fp32 rounding."""
     T = 120
     alpha = _alpha(5)
     targets = np.random.default_rng(2).uniform(0.5, 10.0, (2, 8))
     sur = {"cutoff": 0.5, "gamma": 0.5}
     res = {}
-    for W, P in ((1, 1), (3, 5), (4, 8), (8, 16), (4, 1), (4, 24)):
-        _say(f"case: long_delay {W}, long_parts {P}, checkpoint 50, T={T}")
+    cases = ((1, 1, False), (3, 5, False), (4, 8, False), (8, 16, True), (4, 1, False), (4, 24, False), (4, 24, True),
+             (4, 32, True))
+    for W, P, pure in cases:
+        _say(f"case: long_delay {W}, long_parts {P}, pure ranges {pure}, checkpoint 50, T={T}")
         rts = [n.build() for n in _nets(alpha)]
-        res[W, P] = RateLossGrad(rts, alpha, checkpoint=50, long_delay=W, long_parts=P, **sur)(T, targets)
-    g1 = res[1, 1][1]
+        res[W, P, pure] = RateLossGrad(rts, alpha, checkpoint=50, long_delay=W, long_parts=P, pure_ranges=pure,
+                                       **sur)(T, targets)
+    g1 = res[1, 1, False][1]
     assert np.abs(g1).max() > 0
-    for key in ((3, 5), (4, 8), (8, 16), (4, 1), (4, 24)):
-        assert res[key][0] == res[1, 1][0]
+    for key in cases[1:]:
+        assert res[key][0] == res[1, 1, False][0]
         err = np.linalg.norm(res[key][1] - g1) / np.linalg.norm(g1)
         _say(f"long_delay, long_parts {key}: rel diff {err:.3g}")
         assert err <= 1e-5
