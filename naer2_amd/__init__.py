@@ -10,10 +10,11 @@ from .dsl import Connection, ConnectionBuilder, Points, Population, Region, edge
 from .fusion import FusedRound
 from .network import Network, Runtime
 from .event import EventRuntime
-from .nodes import FixedInDegree, FixedTotalNumber, LIFDelta, LIFDeltaExact, LIFExp, PoissonExact, SpikeTrains
+from .nodes import (FixedInDegree, FixedTotalNumber, GainSynapse, LIFDelta, LIFDeltaExact, LIFExp, PlasticSynapse,
+                    PoissonExact, SpikeTrains, StaticSynapse)
 
 __all__ = [
     "Connection", "ConnectionBuilder", "Points", "Population", "Region", "edge", "node", "pattern",
     "FusedRound", "Network", "Runtime", "EventRuntime", "FixedInDegree", "FixedTotalNumber", "LIFDelta", "LIFExp",
-    "LIFDeltaExact", "PoissonExact", "SpikeTrains",
+    "LIFDeltaExact", "PoissonExact", "SpikeTrains", "StaticSynapse", "GainSynapse", "PlasticSynapse",
 ]

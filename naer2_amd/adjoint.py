@@ -149,6 +149,8 @@ class RateLossGrad:
         for rt in runtimes:
             if rt.model != MODEL_LIF_EXP:
                 raise ValueError("RateLossGrad: exponential-synapse LIF networks only")
+            if rt.edges:
+                raise ValueError("RateLossGrad: static edges into the current only (no edge kinds or ports)")
             if len(rt.pops) > 8:
                 raise ValueError("RateLossGrad: at most 8 populations")
             if not rt.net.record:

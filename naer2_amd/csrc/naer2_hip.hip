@@ -120,6 +120,10 @@ struct StepArgs {
     const int16_t* log_conn;    //   connection index of each link, parallel to links (null: -1)
     int64_t log_cap;
     int32_t* status;            // null, or [n_groups][4] status rows, written by every step
+    const float* edge_attr;     // edge plans (the edge kernel variant): each link's second attribute (gain or
+                                //   factor), parallel to links; link word x bits 8-9 the edge kind, bit 10 the port
+    float* ring_v;              //   [d_slots][n]: the inputs of the potential port
+    uint2* links_rw;            //   the link table, written by plastic links (same address as links)
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
 };
@@ -175,12 +179,14 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
 // Shared-memory staging of one workgroup.  BM: 64-bit words of the row intake's spike bitmap (the
 // groups' words, then the sentinel word); kernel variants size it for the groups they serve.
 // CAPS / CAPL: spiking sources and links staged per pass; RU: row cells in flight per lane and batch.
-template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU, bool LOG = false>
+template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU, bool LOG = false, bool EDGE = false>
 struct Stage {
     static constexpr int kS = CAPS;
     static constexpr int kL = CAPL;
     static constexpr int kRU = RU;
     static constexpr bool kLog = LOG;
+    static constexpr bool kEdge = EDGE;
+    static constexpr bool kCarry = LOG || EDGE;  // staged links carry their table index through the sort
     union {
         struct {                     // segment intake
             int32_t sp[CAPS];            // spiking source slots of this pass, ascending
@@ -190,8 +196,8 @@ struct Stage {
             uint2 sorted[CAPL];          // the staged links regrouped by target, each target's in contract order
             int32_t tcount[kWave];       // staged links per target
             int32_t tstart[kWave];       // first of each target's links in sorted
-            int64_t log_idx[LOG ? CAPL : 1];  // delivery log: each sorted link's index into the link table
-            int32_t log_src[LOG ? CAPL : 1];  //   and its source slot
+            int64_t log_idx[(LOG || EDGE) ? CAPL : 1];  // each sorted link's index into the link table
+            int32_t log_src[LOG ? CAPL : 1];            //   and its source slot (delivery log)
         };
         struct {                     // row intake
             unsigned long long bm[BM];   // the previous step's spike bitmap, then a zero word
@@ -240,7 +246,23 @@ struct LogCtx {
     int first;     // the target group's first neuron
     const int64_t* idx;
     const int32_t* src;
+    float* acc_v;                // edge plans: the potential port's partial sums [d_slots][64]
+    unsigned long long* u;       //   and its touched delays (u[0]: 1..64, u[1]: 65..128)
 };
+
+// Edges (St::kEdge; the reference's D-6 receive variants, D-7 per-link attribute blocks and edge forward):
+// the value a link delivers by its kind -- static w; gain w * g; plastic: w += f first (the stored weight
+// too: the update persists), then the new w -- into the partial sums of its port (current or potential).
+__device__ __forceinline__ float edge_value(const StepArgs& a, uint32_t x, float w, int64_t idx) {
+    const uint32_t kind = (x >> 8) & 3u;
+    if (kind == 1u) return w * a.edge_attr[idx];
+    if (kind == 2u) {
+        const float nw = w + a.edge_attr[idx];
+        a.links_rw[idx].y = __float_as_uint(nw);
+        return nw;
+    }
+    return w;
+}
 
 template <class St>
 __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl, const uint2 (&r)[St::kL / kWave],
@@ -262,7 +284,7 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
         int t = kWave;  // invalid lanes: a target value no valid lane has
         if (valid) {
             ln[u] = r[u];
-            t = static_cast<int>(ln[u].x & 0xffffu);
+            t = static_cast<int>(ln[u].x & (St::kEdge ? 0x3fu : 0xffffu));
         }
         unsigned long long same = ~0ull;
 #pragma unroll
@@ -286,12 +308,10 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
     for (int u = 0; u < kPer; ++u) {
         const int e = u * kWave + lane;
         if (e < nl) {
-            const int pos = st.tstart[ln[u].x & 0xffffu] + off[u];
+            const int pos = st.tstart[ln[u].x & (St::kEdge ? 0x3fu : 0xffffu)] + off[u];
             st.sorted[pos] = ln[u];
-            if constexpr (St::kLog) {
-                st.log_idx[pos] = lc.idx[u];
-                st.log_src[pos] = lc.src[u];
-            }
+            if constexpr (St::kCarry) st.log_idx[pos] = lc.idx[u];
+            if constexpr (St::kLog) st.log_src[pos] = lc.src[u];
         }
     }
     __syncthreads();
@@ -302,11 +322,17 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
         if (lane == 0) b = atomicAdd(a.log_count, nl);
         log_base = __builtin_amdgcn_readfirstlane(b);
     }
-    int cb = -1;
-    float cr = 0.0f;
+    int cb = -1, cbv = -1;
+    float cr = 0.0f, crv = 0.0f;
     for (int j = j0; j < incl; ++j) {
         const uint2 l = st.sorted[j];
-        cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, __uint_as_float(l.y), cb, cr, t0, t1);
+        if constexpr (St::kEdge) {
+            const float v = edge_value(a, l.x, __uint_as_float(l.y), st.log_idx[j]);
+            if ((l.x >> 10) & 1u) cell_add(lc.acc_v, lane, static_cast<int>(l.x >> 16) - 1, v, cbv, crv, lc.u[0], lc.u[1]);
+            else cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, v, cb, cr, t0, t1);
+        } else {
+            cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, __uint_as_float(l.y), cb, cr, t0, t1);
+        }
         if constexpr (St::kLog) {
             const int64_t at = log_base + j;
             if (at < a.log_cap) {
@@ -322,6 +348,9 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
         }
     }
     if (cb >= 0) acc[cb * kWave + lane] = cr;
+    if constexpr (St::kEdge) {
+        if (cbv >= 0) lc.acc_v[cbv * kWave + lane] = crv;
+    }
 }
 
 // Row intake of one batch of St::kRU 16-byte cells of this lane's row (kRU * PER entries).  Padding
@@ -426,13 +455,15 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
 
 template <class St>
 __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
-                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first);
+                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first,
+                                              float* acc_v, unsigned long long* u);
 
 // Delivery of the n_sp spiking sources in st.sp (ascending): their segments into group g, staged in
 // contract order, walked by the target lanes into the per-(delay, target) partial sums acc.
 template <class St>
 __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* acc, int g, int lane, int n_sp,
-                                             unsigned long long& t0, unsigned long long& t1, int64_t s, int first) {
+                                             unsigned long long& t0, unsigned long long& t1, int64_t s, int first,
+                                             float* acc_v, unsigned long long* u) {
     const int NG = a.n_groups;
     // segment bounds: every load of the pass issued before the first is used
     int64_t lo_r[St::kS / kWave], hi_r[St::kS / kWave];
@@ -462,14 +493,15 @@ __device__ __forceinline__ void deliver_pass(const StepArgs& a, St& st, float* a
     if (lane == 0) st.seg_start[n_sp] = carry;
     __syncthreads();
     N2H_STAMP(3);
-    deliver_links(a, st, acc, lane, n_sp, carry, t0, t1, s, first);
+    deliver_links(a, st, acc, lane, n_sp, carry, t0, t1, s, first, acc_v, u);
 }
 
 // The L links of the n_sp sources whose segment bounds are in st (seg_lo, seg_start): staged in chunks of
 // the stage's kL in contract order and walked into acc.
 template <class St>
 __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* acc, int lane, int n_sp, int L,
-                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first) {
+                                              unsigned long long& t0, unsigned long long& t1, int64_t s, int first,
+                                              float* acc_v, unsigned long long* u) {
     for (int e0 = 0; e0 < L; e0 += St::kL) {
         const int e1 = min(L, e0 + St::kL);
         // owner[j]: the source whose segment holds link e0 + j.  Short chunks: a binary search per link;
@@ -524,7 +556,7 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
             __syncthreads();
         }
         uint2 r[kPer];
-        int64_t lidx[St::kLog ? kPer : 1];
+        int64_t lidx[St::kCarry ? kPer : 1];
         int32_t lsrc[St::kLog ? kPer : 1];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
@@ -534,15 +566,13 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
                 const int i = st.owner[e - e0];
                 const int64_t li = st.seg_lo[i] + (e - st.seg_start[i]);
                 r[u] = a.links[li];
-                if constexpr (St::kLog) {
-                    lidx[u] = li;
-                    lsrc[u] = st.sp[i];
-                }
+                if constexpr (St::kCarry) lidx[u] = li;
+                if constexpr (St::kLog) lsrc[u] = st.sp[i];
             }
         }
         N2H_STAMP(4);
         const int nl = e1 - e0;
-        walk_chunk(st, acc, lane, nl, r, t0, t1, a, LogCtx{s, first, lidx, lsrc});
+        walk_chunk(st, acc, lane, nl, r, t0, t1, a, LogCtx{s, first, lidx, lsrc, acc_v, u});
         __syncthreads();
     }
 }
@@ -555,6 +585,9 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     // acc[b * 64 + lane]: the partial sum of delay cell b; the LDS holds cells acc_base onwards (one cell for
     // a uniform delay)
     float* acc = acc_lds - a.acc_base * kWave;
+    // edge plans: the potential port's partial sums follow the current port's (acc_base is 0 there)
+    float* acc_v = St::kEdge ? acc_lds + a.d_slots * kWave : nullptr;
+    unsigned long long tv[2] = {0ull, 0ull};
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
     const int D = a.d_slots;
@@ -605,6 +638,10 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (a.multirate) stride = __builtin_amdgcn_readfirstlane(a.pops[pop].stride);
     const bool upd = stride <= 1 || static_cast<uint64_t>(s + 1) % static_cast<uint64_t>(stride) == 0ull;
     if (live && upd) x = a.ring[cur];
+    float xv = 0.0f;  // edge plans: the potential port's due input
+    if constexpr (St::kEdge) {
+        if (live && upd) xv = a.ring_v[cur];
+    }
     // the background draw needs only the neuron and the step: computed here, under the latency of the
     // spike-list load, and kept in a register (the empty asm pins it here)
     uint32_t u_bg = philox_word0(static_cast<uint32_t>(nid), static_cast<uint32_t>(s),
@@ -677,7 +714,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             if (lane == 0) st.seg_start[n_prev] = total;
             __syncthreads();
             N2H_STAMP(3);
-            deliver_links(a, st, acc, lane, n_prev, total, t0, t1, s, first);
+            deliver_links(a, st, acc, lane, n_prev, total, t0, t1, s, first, acc_v, tv);
         }
     } else {
         path = 2;
@@ -728,7 +765,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 }
             }
             __syncthreads();
-            deliver_pass(a, st, acc, g, lane, r1 - r0, t0, t1, s, first);
+            deliver_pass(a, st, acc, g, lane, r1 - r0, t0, t1, s, first, acc_v, tv);
         }
     }
 
@@ -738,6 +775,9 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (lane < kParamWords) st.param_words[lane] = pword;
     // the delay-1 cell's partial sum is the one due now: ring[s][i] + p, read in registers
     if (live && (t0 & 1ull)) x = x + acc[lane];
+    if constexpr (St::kEdge) {
+        if (live && (tv[0] & 1ull)) xv = xv + acc_v[lane];
+    }
 
     // ---- background, LIF update, threshold ----
     __syncthreads();
@@ -786,6 +826,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 const float vr = P.v_rest + pv;
                 const float pvi = P.p_vi * I;
                 V = vr + pvi;
+                if constexpr (St::kEdge) V = V + xv;  // the potential port: a jump after the propagation
                 vpre = V;
             }
             I = P.p_i * I;
@@ -867,6 +908,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             a.ref[nid] = R;
             if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
             a.ring[cur] = 0.0f;
+            if constexpr (St::kEdge) a.ring_v[cur] = 0.0f;
             int q = s_mod;
             for (int j = 1; j < stride; ++j) {  // the window's earlier slots, read above
                 q = (q == 0) ? D - 1 : q - 1;
@@ -895,6 +937,18 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             if (slot >= D) slot -= D;
             unsafeAtomicAdd(&a.ring[static_cast<int64_t>(slot) * N + nid], acc[b * kWave + lane]);
         }
+        if constexpr (St::kEdge) {  // the potential port's cells into its own ring, as above
+            for (int h = 0; h < 2; ++h) {
+                m = h == 0 ? (upd ? (tv[0] & ~1ull) : tv[0]) : tv[1];
+                while (m) {
+                    const int b = __ffsll(static_cast<long long>(m)) - 1 + 64 * h;
+                    m &= m - 1ull;
+                    int slot = s_mod + b;
+                    if (slot >= D) slot -= D;
+                    unsafeAtomicAdd(&a.ring_v[static_cast<int64_t>(slot) * N + nid], acc_v[b * kWave + lane]);
+                }
+            }
+        }
     }
     N2H_STAMP(7);
     if (a.trace && lane < kTraceEvents) a.trace[static_cast<int64_t>(g) * kTraceEvents + lane] = st.stamps[lane];
@@ -905,6 +959,15 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     __shared__ Stage<bm_words(KW)> st;
     extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
     step_group<MODEL, KW>(a, blockIdx.x, k, st, acc);
+}
+
+// The edge variant (n2h_bind_edges): the step without the row intake, links delivering by their edge kind into
+// their port's partial sums and ring.
+template <int MODEL, bool LOG>
+__global__ __launch_bounds__(kWave) void step_edge_kernel(StepArgs a, int32_t k) {
+    __shared__ Stage<bm_words(0), kCapS, kCapL, kRowU, LOG, true> st;
+    extern __shared__ float acc[];
+    step_group<MODEL, 0>(a, blockIdx.x, k, st, acc);
 }
 
 // The logging variant (n2h_bind_log): the step without the row intake (KW = 0: segment intakes only, which
@@ -1005,6 +1068,8 @@ struct n2h_plan {
     const int16_t* log_conn;
     int64_t log_cap;
     int32_t* status;
+    const float* edge_attr;
+    float* ring_v;
     int32_t multirate;      // some population's stride > 1
     size_t lds_bytes;
     int device;
@@ -1086,6 +1151,9 @@ StepArgs make_args(const n2h_plan* p) {
     a.log_conn = p->log_conn;
     a.log_cap = p->log_cap;
     a.status = p->status;
+    a.edge_attr = p->edge_attr;
+    a.ring_v = p->ring_v;
+    a.links_rw = const_cast<uint2*>(reinterpret_cast<const uint2*>(p->tables.links));
     a.multirate = p->multirate;
     return a;
 }
@@ -1106,7 +1174,10 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
-        if (p->log_rec) {
+        if (p->ring_v) {  // edge plans (LIF_EXP only)
+            if (p->log_rec) hipLaunchKernelGGL((step_edge_kernel<N2H_MODEL_LIF_EXP, true>), grid, block, p->lds_bytes, stream, a, k);
+            else hipLaunchKernelGGL((step_edge_kernel<N2H_MODEL_LIF_EXP, false>), grid, block, p->lds_bytes, stream, a, k);
+        } else if (p->log_rec) {
             if (p->desc.model == N2H_MODEL_LIF_DELTA)
                 hipLaunchKernelGGL(step_log_kernel<N2H_MODEL_LIF_DELTA>, grid, block, p->lds_bytes, stream, a, k);
             else
@@ -1394,6 +1465,8 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     if (n_plans < 1 || n_plans > kMaxBatch) return fail(N2H_EINVAL, "n_plans %d outside [1, %d]", n_plans, kMaxBatch);
     for (int32_t i = 0; i < n_plans; ++i)
         if (plans[i] && plans[i]->log_rec) return fail(N2H_EINVAL, "plan %d logs its deliveries: a logging plan steps alone", i);
+    for (int32_t i = 0; i < n_plans; ++i)
+        if (plans[i] && plans[i]->ring_v) return fail(N2H_EINVAL, "plan %d has edge kinds or ports: it steps alone", i);
     n2h_tuning tu{};
     if (tuning) tu = *tuning;
     if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
@@ -1555,6 +1628,7 @@ int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps, f
     if ((v_lo == nullptr) != (i_lo == nullptr)) return fail(N2H_EINVAL, "v_lo and i_lo go together");
     if (v_lo && p->desc.model != N2H_MODEL_LIF_EXP)
         return fail(N2H_EINVAL, "the compensated state is tracked for the exponential-synapse model only");
+    if (v_lo && p->ring_v) return fail(N2H_EINVAL, "edge plans take no compensated trace");
     p->vtrace = buffer;
     p->vtrace_step0 = step0;
     p->vtrace_steps = buffer ? n_steps : 0;
@@ -1570,6 +1644,24 @@ int n2h_bind_log(n2h_plan* p, const int16_t* link_conn, int32_t* records, int32_
     p->log_count = records ? count : nullptr;
     p->log_conn = records ? link_conn : nullptr;
     p->log_cap = records ? capacity : 0;
+    return N2H_OK;
+}
+
+int n2h_bind_edges(n2h_plan* p, const float* edge_attr, float* ring_v) {
+    if (!p) return fail(N2H_EINVAL, "null plan");
+    if (!ring_v) {
+        p->edge_attr = nullptr;
+        p->ring_v = nullptr;
+        p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(p->tables.uniform_delay > 0 ? 1 : p->desc.d_slots);
+        return N2H_OK;
+    }
+    if (p->desc.model != N2H_MODEL_LIF_EXP) return fail(N2H_EINVAL, "edge kinds and ports need the exponential-synapse model");
+    if (p->tables.uniform_delay > 0) return fail(N2H_EINVAL, "edge plans keep every delay cell (uniform_delay 0)");
+    if (p->multirate) return fail(N2H_EINVAL, "edge plans update every population every step");
+    if (p->vlo) return fail(N2H_EINVAL, "edge plans take no compensated trace");
+    p->edge_attr = edge_attr;
+    p->ring_v = ring_v;
+    p->lds_bytes = 2 * sizeof(float) * kWave * static_cast<size_t>(p->desc.d_slots);
     return N2H_OK;
 }
 

@@ -23,7 +23,11 @@ import itertools
 import torch
 
 _NODE_TYPES: dict[type, str | None] = {}
-_EDGE_TYPES: set[type] = set()
+_EDGE_TYPES: dict[type, str] = {}
+# edge kinds the step kernel runs (n2h_bind_edges) and the name of each kind's second per-link attribute
+EDGE_KINDS = {"static": None, "gain": "gain", "plastic": "factor"}
+# receive ports of the exponential-synapse model: the synaptic current or the membrane potential
+PORTS = ("current", "potential")
 _PATTERN_TYPES: set = set()
 _ORDER = itertools.count()
 
@@ -74,16 +78,29 @@ def node(cls=None, *, family: str | None = None):
     return wrap(cls) if cls is not None else wrap
 
 
-def edge(cls=None, **_):
-    """Register an edge (synapse) class (naer2/decorators/edge.py).  Static synapses only: an edge's
-    attributes become per-link tensors; forward/backward plasticity hooks are not run by this backend."""
+def edge(cls=None, *, kind: str | None = None, **_):
+    """Register an edge (synapse) class (naer2/decorators/edge.py).  `kind` (or a class attribute of that
+    name; default "static") names the edge forward the step kernel runs for its links, as `family` names a
+    node's kernel: "static" delivers the weight; "gain" delivers weight * gain; "plastic" first adds factor
+    to the stored weight (the update persists), then delivers the new weight -- the reference's
+    plastic_synapse forward (test/edges/plastic_synapse.py:10-11).  weight and the kind's attribute
+    (gain, factor) become per-link tensors; backward hooks and hooks reading source or target state (the
+    reference's open decisions D-9 to D-11, TODO.md:196-204) are not run."""
 
     def wrap(c):
-        _EDGE_TYPES.add(c)
+        k = kind if kind is not None else getattr(c, "kind", "static")
+        if k not in EDGE_KINDS:
+            raise ValueError(f"edge kind {k!r}: one of {sorted(EDGE_KINDS)}")
+        _EDGE_TYPES[c] = k
         c.__matmul__ = lambda self, other: _PartialConnection(self) @ other
         return c
 
     return wrap(cls) if cls is not None else wrap
+
+
+def edge_kind(instance) -> str:
+    """The edge kind of an edge instance ("static" for none)."""
+    return "static" if instance is None else _EDGE_TYPES.get(type(instance), "static")
 
 
 def pattern(cb):
@@ -174,15 +191,22 @@ class Population:
 # ---------------------------------------------------------------- connections
 
 class _PartialConnection:
-    """`edge @ pattern` before the source is known (naer2/_partial.py)."""
+    """`edge @ "port" @ pattern` in any order, before the source is known (naer2/_partial.py:1-44): a string
+    names the target port, anything else but a mapping is the pattern."""
 
     def __init__(self, edge_instance, pattern_=None):
         self.edge = edge_instance
         self.pattern = pattern_
+        self.port = None
 
     def __matmul__(self, other):
-        if isinstance(other, (str, dict)):
-            raise ValueError("ports and attribute mappings are not supported by this backend")
+        if isinstance(other, str):
+            if self.port is not None:
+                raise ValueError(f"target port already set to {self.port!r}")
+            self.port = other
+            return self
+        if isinstance(other, dict):
+            raise ValueError("attribute mappings rename hook-facing names: this backend runs no hooks")
         if self.pattern is not None:
             raise ValueError(f"pattern already set to {self.pattern!r}")
         self.pattern = other
@@ -215,10 +239,11 @@ class ConnectionBuilder:
         self._source = source
         self._pattern = None
         self._edge = None
+        self._port = None
 
     def __rshift__(self, to):
         if isinstance(to, _PartialConnection):
-            self._edge, self._pattern = to.edge, to.pattern
+            self._edge, self._pattern, self._port = to.edge, to.pattern, to.port
             return self
         if isinstance(to, Population):
             return self._target(to)
@@ -243,7 +268,7 @@ class ConnectionBuilder:
         if src.numel() > 1 and not bool((src[1:] >= src[:-1]).all()):
             order = torch.argsort(src, stable=True)  # the connection's link order: source-major, stable
             src, tgt = src[order], tgt[order]
-        conn = Connection(self._source, target, src, tgt, self._edge)
+        conn = Connection(self._source, target, src, tgt, self._edge, self._port)
         net.connections.append(conn)
         target.incoming.append(conn)
         return conn
@@ -251,20 +276,27 @@ class ConnectionBuilder:
 
 class Connection:
     """A finished connection: per-link source/target (local indices, source-major), weight and delay
-    (naer2/connection.py:221-309).  `conn.weight = x`, `conn.delay = x` (ms), `conn | delay`."""
+    (naer2/connection.py:221-309).  `conn.weight = x`, `conn.delay = x` (ms), `conn | delay`; gain and
+    plastic edges add `conn.gain` / `conn.factor`; `conn.port` is the target port ("current" or
+    "potential"; None: "current")."""
 
     def __init__(self, source: Population, target: Population, src: torch.Tensor, tgt: torch.Tensor,
-                 edge_instance=None):
+                 edge_instance=None, port: str | None = None):
         self.source, self.target = source, target
         # local indices as int32 (populations hold fewer than 2^31 neurons): 8 bytes per link less than int64
         self._src, self._tgt = src.to(torch.int32), tgt.to(torch.int32)
         self._n_links = int(tgt.numel())
         self._released = False
         self.edge = edge_instance
+        self.kind = edge_kind(edge_instance)
+        self._port = None
+        self.port = port
         self._weight = None
         self._delay = None
+        self._attr2 = None
         if edge_instance is not None:
-            for name in ("weight", "delay"):
+            names = ("weight", "delay") + ((EDGE_KINDS[self.kind],) if EDGE_KINDS[self.kind] else ())
+            for name in names:
                 if hasattr(edge_instance, name):
                     setattr(self, name, getattr(edge_instance, name))
 
@@ -291,7 +323,7 @@ class Connection:
 
     def release(self) -> None:
         """Drop the per-link tensors (the built runtime keeps its own tables)."""
-        self._src = self._tgt = self._weight = self._delay = None
+        self._src = self._tgt = self._weight = self._delay = self._attr2 = None
         self._released = True
 
     def _per_link(self, value, name: str) -> torch.Tensor:
@@ -333,6 +365,62 @@ class Connection:
     def __or__(self, value) -> "Connection":
         self.delay = value
         return self
+
+    @property
+    def port(self) -> str | None:
+        return self._port
+
+    @port.setter
+    def port(self, value: str | None) -> None:
+        if value is not None and value not in PORTS:
+            raise ValueError(f"port {value!r}: this backend's targets have the ports {PORTS}")
+        self._port = value
+        self.source.network._invalidate()
+
+    def _second(self, name: str) -> None:
+        if EDGE_KINDS[self.kind] != name:
+            raise AttributeError(f"{name}: only {[k for k, a in EDGE_KINDS.items() if a == name][0]} edges have it "
+                                 f"(this connection's edge is {self.kind})")
+
+    @property
+    def gain(self) -> torch.Tensor:
+        """Per-link gain of a gain edge (delivers weight * gain)."""
+        self._second("gain")
+        self._check_held()
+        if self._attr2 is None:
+            raise ValueError("gain not set")
+        return self._attr2
+
+    @gain.setter
+    def gain(self, value) -> None:
+        self._second("gain")
+        self._attr2 = self._per_link(value, "gain")
+        self.source.network._invalidate()
+
+    @property
+    def factor(self) -> torch.Tensor:
+        """Per-link increment of a plastic edge (weight += factor at every delivery, before it delivers)."""
+        self._second("factor")
+        self._check_held()
+        if self._attr2 is None:
+            raise ValueError("factor not set")
+        return self._attr2
+
+    @factor.setter
+    def factor(self, value) -> None:
+        self._second("factor")
+        self._attr2 = self._per_link(value, "factor")
+        self.source.network._invalidate()
+
+    @property
+    def second_attr(self) -> torch.Tensor | None:
+        """The edge kind's second per-link attribute (gain or factor), or None for static edges."""
+        return None if EDGE_KINDS[self.kind] is None else getattr(self, EDGE_KINDS[self.kind])
+
+    @property
+    def plain(self) -> bool:
+        """A static edge into the current port: what the plain step delivers."""
+        return self.kind == "static" and self._port in (None, "current")
 
     def __repr__(self) -> str:
         return f"Connection({self.source!r} -> {self.target!r}, {self.shape[0]} links)"

@@ -113,6 +113,8 @@ def describe(net) -> EventPlan:
             raise ValueError(f"{c!r}: weight and delay must be set")
         if c.target.family != "event_lif":
             raise ValueError(f"{c!r}: only event_lif populations receive inputs")
+        if not c.plain:
+            raise NotImplementedError(f"{c!r}: event-driven networks take static edges only (no edge kinds or ports)")
         key = (index[c.source], index[c.target])
         if key in pairs:
             raise NotImplementedError("at most one connection per (source, target) population pair")

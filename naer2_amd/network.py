@@ -214,16 +214,27 @@ class Assembled:
     v: torch.Tensor
     isyn: torch.Tensor | None
     ref: torch.Tensor
+    edges: bool = False  # some connection has a non-static edge kind or the potential port
 
     @property
     def n(self) -> int:
         return int(self.pop_first[-1])
 
 
+# the edge bits of a link's packed word (n2h_bind_edges): kind in bits 8-9, port in bit 10
+EDGE_KIND_BITS = {"static": 0, "gain": 1 << 8, "plastic": 2 << 8}
+PORT_POTENTIAL_BIT = 1 << 10
+# the columns of the per-link payload (`canon.extra`, int32 [n_links, 4])
+PAYLOAD_CONN, PAYLOAD_FLAGS, PAYLOAD_ATTR, PAYLOAD_INPUT = range(4)
+
+
 def assemble(net: Network, device=None, with_connections: bool = False) -> Assembled:
     """Everything the plan needs, on `device` (default: the network's), without touching the GPU.
-    with_connections: the canonical table also carries each link's connection index (creation order) as
-    its int16 payload (`canon.extra`)."""
+    With with_connections, or when some connection has a non-static edge kind or the potential port, the
+    canonical table carries an int32 [n_links, 4] payload (`canon.extra`): per link its connection index
+    (creation order), its edge bits (EDGE_KIND_BITS | PORT_POTENTIAL_BIT), the bits of its second attribute
+    (gain or factor; 0.0 for static edges) and its place in the connections' links concatenated in creation
+    order."""
     pops = sorted(net.populations, key=lambda p: p._order)
     fams = {p.family for p in pops}
     if None in fams:
@@ -248,9 +259,12 @@ def assemble(net: Network, device=None, with_connections: bool = False) -> Assem
         par = p.instance.params(k * dt)  # propagators, Poisson mean and refractory updates for its interval
         par.stride = k
         params.append(par)
+    edges = any(not c.plain for c in net.connections)
+    if edges and model != MODEL_LIF_EXP:
+        raise NotImplementedError("edge kinds and ports need the exponential-synapse model (lif_exp)")
+    payload = with_connections or edges
     srcs, tgts, ws, ds, cs = [], [], [], [], []
-    if with_connections and len(net.connections) > 32767:
-        raise ValueError("at most 32767 connections carry their index")
+    base = 0
     for ci, c in enumerate(net.connections):
         if c._weight is None or c._delay is None:
             raise ValueError(f"{c!r}: weight and delay must be set")
@@ -258,11 +272,20 @@ def assemble(net: Network, device=None, with_connections: bool = False) -> Assem
         tgts.append(c.tgt.to(dev, torch.int64) + first_of[c.target])
         ws.append(c.weight.to(dev))
         ds.append(delay_steps(c.delay.to(dev), dt))
-        if with_connections:
-            cs.append(torch.full((srcs[-1].numel(),), ci, dtype=torch.int16, device=dev))
+        if payload:
+            m = srcs[-1].numel()
+            col = torch.zeros(m, 4, dtype=torch.int32, device=dev)
+            col[:, PAYLOAD_CONN] = ci
+            col[:, PAYLOAD_FLAGS] = EDGE_KIND_BITS[c.kind] | (PORT_POTENTIAL_BIT if c.port == "potential" else 0)
+            if c.second_attr is not None:
+                col[:, PAYLOAD_ATTR] = c.second_attr.to(dev, torch.float32).view(torch.int32)
+            col[:, PAYLOAD_INPUT] = torch.arange(base, base + m, dtype=torch.int32, device=dev)
+            base += m
+            cs.append(col)
     cat = (lambda xs, dt_: torch.cat(xs) if xs else torch.zeros(0, dtype=dt_, device=dev))
     canon = canonical(n, cat(srcs, torch.int64), cat(tgts, torch.int64), cat(ws, torch.float32),
-                      cat(ds, torch.int32), cat(cs, torch.int16) if with_connections else None)
+                      cat(ds, torch.int32), (torch.cat(cs) if cs else torch.zeros(0, 4, dtype=torch.int32, device=dev))
+                      if payload else None)
     del srcs, tgts, ws, ds, cs
     v = torch.empty(n, dtype=torch.float32, device=dev)
     isyn = torch.zeros(n, dtype=torch.float32, device=dev) if model == MODEL_LIF_EXP else None
@@ -279,7 +302,7 @@ def assemble(net: Network, device=None, with_connections: bool = False) -> Assem
     if d_slots > _lib.MAX_SLOTS:
         raise ValueError(f"largest delay {canon.max_delay} steps + largest stride {max(strides)} - 1 exceeds "
                          f"{_lib.MAX_SLOTS} ring slots")
-    return Assembled(pops, family, model, dt, pop_first, params, canon, d_slots, v, isyn, ref)
+    return Assembled(pops, family, model, dt, pop_first, params, canon, d_slots, v, isyn, ref, edges)
 
 
 def pop_stride(pop, dt: float) -> int:
@@ -317,8 +340,23 @@ class Runtime:
         self.n_groups = self.layout.n_groups
         seg = segments(self.canon, self.layout)
         self.seg_off, self.links = seg.seg_off, seg.links
+        # edge kinds and ports (n2h_bind_edges): the edge bits ORed into each link's packed word, the second
+        # attribute beside the table, and the potential port's own ring
+        self.edges = a.edges
+        self.edge_attr = self.ring_v = self.link_input = None
+        if self.edges:
+            if net.row_intake is True:
+                raise ValueError("edge kinds and ports step without the row intake")
+            x = seg.extra
+            self.links[:, 0] |= x[:, PAYLOAD_FLAGS]
+            self.edge_attr = x[:, PAYLOAD_ATTR].contiguous().view(torch.float32)
+            self.link_input = x[:, PAYLOAD_INPUT].contiguous()
+            self.ring_v = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
+            self._plastic = bool((x[:, PAYLOAD_FLAGS] & EDGE_KIND_BITS["plastic"]).any())
+            del x
+        self.conn_links = [c.shape[0] for c in net.connections]
         self.rows = None
-        if net.row_intake and self.n_groups <= _lib.DENSE_GROUPS:
+        if net.row_intake and self.n_groups <= _lib.DENSE_GROUPS and not self.edges:
             self.rows = rows(self.canon, self.layout, fmt=net.row_format)
         elif net.row_intake is True:
             raise ValueError(f"row intake needs at most {_lib.DENSE_GROUPS} groups ({self.n_groups} here)")
@@ -331,7 +369,7 @@ class Runtime:
         self.out_degree = self.canon.out_degree()
         # every link's delay in steps when all are equal (one partial sum per target in the step), else 0
         d = self.canon.d
-        self.uniform_delay = int(d[0]) if d.numel() and bool((d == d[0]).all()) else 0
+        self.uniform_delay = int(d[0]) if d.numel() and bool((d == d[0]).all()) and not self.edges else 0
         if not net.keep_canonical:
             self.canon = None
         del seg
@@ -368,6 +406,8 @@ class Runtime:
                                       self.ref.data_ptr()), "n2h_bind_state")
         self._bind_rows()
         self._bind_ring()
+        if self.edges:
+            _lib.check(lib.n2h_bind_edges(plan, self.edge_attr.data_ptr(), self.ring_v.data_ptr()), "n2h_bind_edges")
 
     def _bind_rows(self) -> None:
         lib = _lib.load()
@@ -538,8 +578,10 @@ class Runtime:
         else:
             if not self.net.keep_description:
                 raise RuntimeError("the delivery log needs the connections' links (Network.keep_description)")
+            if len(self.net.connections) > 32767:
+                raise ValueError("at most 32767 connections carry their index in the delivery log")
             a = assemble(self.net, self.device, with_connections=True)
-            conn = segments(a.canon, self.layout).extra.contiguous()
+            conn = segments(a.canon, self.layout).extra[:, PAYLOAD_CONN].to(torch.int16).contiguous()
             del a
             rec = torch.zeros(int(capacity), 5, dtype=torch.int32, device=self.device)
             count = torch.zeros(1, dtype=torch.int32, device=self.device)
@@ -633,14 +675,22 @@ class Runtime:
         self._graphs = {}
 
     def snapshot(self) -> tuple:
-        """A device copy of the dynamic state (step, V, I, refractory counters, ring, spike bits and lists)."""
+        """A device copy of the dynamic state (step, V, I, refractory counters, ring, spike bits and lists;
+        with edges, the potential port's ring and, with plastic edges, the link weights)."""
+        edge = None
+        if self.edges:
+            edge = (self.ring_v.clone(), self.links[:, 1].clone() if self._plastic else None)
         return (self.step, self.v.clone(), None if self.isyn is None else self.isyn.clone(), self.ref.clone(),
-                self.ring.clone(), self.bits.clone(), self.spike_list.clone())
+                self.ring.clone(), self.bits.clone(), self.spike_list.clone(), edge)
 
     def restore(self, snap: tuple) -> None:
         """Back to a snapshot() of this runtime; recording restarts at its step.  A bit history grown since
         is replaced by the snapshot's (not while a FusedRound holds this runtime: it keeps the addresses)."""
-        step, v, isyn, ref, ring, bits, lst = snap
+        step, v, isyn, ref, ring, bits, lst, edge = snap
+        if edge is not None:
+            self.ring_v.copy_(edge[0])
+            if edge[1] is not None:
+                self.links[:, 1] = edge[1]
         self.v.copy_(v)
         if isyn is not None:
             self.isyn.copy_(isyn)
@@ -663,6 +713,21 @@ class Runtime:
         _lib.check(lib.n2h_set_step(self._plan, int(step), ctypes.c_void_p(stream)), "n2h_set_step")
         self.step = int(step)
         self.record_from = int(step)
+
+    def link_weights(self, conn=None) -> torch.Tensor:
+        """The current link weights (plastic edges update them): of the connections' links concatenated in
+        creation order, or of `conn` alone, each in its own link order (float32, on the device)."""
+        w_seg = self.links[:, 1].contiguous().view(torch.float32)
+        if self.link_input is None:
+            raise RuntimeError("link weights in connection order need an edge plan (the weights of static "
+                               "links do not change: Connection.weight holds them)")
+        out = torch.empty_like(w_seg)
+        out[self.link_input.to(torch.int64)] = w_seg
+        if conn is None:
+            return out
+        ci = self.net.connections.index(conn)
+        a = sum(self.conn_links[:ci])
+        return out[a:a + self.conn_links[ci]]
 
     # -- state access for populations
     def _tensor(self, name: str):

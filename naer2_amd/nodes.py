@@ -14,6 +14,11 @@ Node families (one hand-written kernel each, naer2_amd/csrc/naer2_hip.hip):
                                   exact spike times
   SpikeTrains    "event_train"    listed spike times per source (the commanded sources of the reference's
                                   tests, test/nodes/commanded.py)
+Edges (one edge forward each, run by the step kernel's edge variant; exponential-synapse networks):
+  StaticSynapse(weight)           "static"   delivers weight (test/edges/static_synapse.py:5-21)
+  GainSynapse(weight, gain)       "gain"     delivers weight * gain: two per-link attributes (TODO.md:189-192)
+  PlasticSynapse(weight, factor)  "plastic"  weight += factor, stored, then delivers the new weight
+                                             (test/edges/plastic_synapse.py:5-14)
 Patterns:
   FixedInDegree(K)          every target draws K sources uniformly with replacement
                             (test/networks/Brunel/connection.py:7-33, tools/brunel_reference.py:20-21)
@@ -26,7 +31,7 @@ import math
 
 import torch
 
-from .dsl import node, pattern
+from .dsl import edge, node, pattern
 from .params import lif_delta_params, lif_exp_params
 
 
@@ -154,3 +159,31 @@ class FixedTotalNumber:
         n_tgt = len(target_region if target_region is not None else source_region)
         k = self.per_source(n_src, n_tgt)
         return Points(torch.randint(0, max(n_tgt, 1), (k, 1)))
+
+
+@edge(kind="static")
+class StaticSynapse:
+    """A fixed weight (test/edges/static_synapse.py:5-21)."""
+
+    def __init__(self, weight: float):
+        self.weight = weight
+
+
+@edge(kind="gain")
+class GainSynapse:
+    """Delivers weight * gain, both per-link attributes (the reference's two-attribute receive,
+    test/test_fused_generated_gpu.py:373-381)."""
+
+    def __init__(self, weight: float, gain: float = 1.0):
+        self.weight = weight
+        self.gain = gain
+
+
+@edge(kind="plastic")
+class PlasticSynapse:
+    """weight += factor at every delivery, before the weight is delivered; the update persists
+    (test/edges/plastic_synapse.py:5-14)."""
+
+    def __init__(self, weight: float, factor: float):
+        self.weight = weight
+        self.factor = factor

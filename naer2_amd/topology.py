@@ -44,7 +44,7 @@ class Canonical:
 
     def to(self, device) -> "Canonical":
         return Canonical(self.n, self.src_ptr.to(device), self.tgt.to(device), self.w.to(device),
-                         self.d.to(device))
+                         self.d.to(device), None if self.extra is None else self.extra.to(device))
 
 
 def delay_steps(delay_ms: torch.Tensor, dt: float) -> torch.Tensor:

@@ -51,21 +51,27 @@ static int pop_of(int32_t i, int32_t n_pops, const int32_t* pop_first) {
     return p;
 }
 
-int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops,
-           int32_t d_slots, uint64_t seed, uint32_t instance,
-           const int64_t* src_ptr, const int32_t* tgt, const float* w, const int32_t* delay,
-           float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
-           int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events) {
+/* The step with optional edges (or_run_edges): kind/port/attr2 NULL is the plain step.  The cells of the
+ * potential port follow those of the current port in acc / touched (cell + N * D). */
+static int run_core(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops,
+                    int32_t d_slots, uint64_t seed, uint32_t instance,
+                    const int64_t* src_ptr, const int32_t* tgt, float* w, const int32_t* delay,
+                    const uint8_t* kind, const uint8_t* port, const float* attr2, float* ring_v,
+                    float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+                    int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events) {
     if (n_neurons < 0 || n_pops < 1 || d_slots < 1 || n_steps < 0) return -1;
     if (model != OR_LIF_DELTA && model != OR_LIF_EXP) return -1;
     if (model == OR_LIF_EXP && !isyn) return -1;
     const int mr = multirate(n_pops, pops, d_slots);
     if (mr < 0 || (mr && accum_mode != 0)) return -1;
+    const int edges = kind != NULL;
+    if (edges && (!port || !ring_v || model != OR_LIF_EXP || mr || accum_mode != 0)) return -1;
     const int64_t N = n_neurons, D = d_slots;
-    /* per-(slot, target) partial sums of one emission step, with a touched list */
-    float* acc = (float*)malloc(sizeof(float) * (size_t)(N * D + 1));
-    uint8_t* touched = (uint8_t*)calloc((size_t)(N * D + 1), 1);
-    int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N * D + 1));
+    const int64_t n_cells = (edges ? 2 : 1) * N * D;
+    /* per-(port, slot, target) partial sums of one emission step, with a touched list */
+    float* acc = (float*)malloc(sizeof(float) * (size_t)(n_cells + 1));
+    uint8_t* touched = (uint8_t*)calloc((size_t)(n_cells + 1), 1);
+    int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_cells + 1));
     int32_t* pop = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N + 1));
     if (!acc || !touched || !list || !pop) {
         free(acc); free(touched); free(list); free(pop);
@@ -85,22 +91,41 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
             for (int64_t l = src_ptr[src]; l < src_ptr[src + 1]; ++l) {
                 const int32_t d = delay[l];
                 if (d < 1 || d > d_slots) { rc = -1; goto done; }
-                const int64_t cell = ((s - 1 + d) % D) * N + tgt[l];
+                int64_t cell = ((s - 1 + d) % D) * N + tgt[l];
+                float val = w[l];
+                if (edges) {
+                    /* the edge forward before receive (TODO.md D-6/D-7; test_fused_generated_gpu.py:361-398):
+                     * gain w * g; plastic w += f, stored, then the new w; the port picks the cells */
+                    if (kind[l] == 1) {
+                        val = w[l] * attr2[l];
+                    } else if (kind[l] == 2) {
+                        w[l] = w[l] + attr2[l];
+                        val = w[l];
+                    } else if (kind[l] != 0) {
+                        rc = -1;
+                        goto done;
+                    }
+                    if (port[l] == 1) cell += N * D;
+                }
                 if (!touched[cell]) {
                     touched[cell] = 1;
                     list[n_list++] = cell;
-                    acc[cell] = 0.0f + w[l];
+                    acc[cell] = 0.0f + val;
                 } else if (accum_mode == 1) {
-                    acc[cell] = acc[cell] * 0.5f + w[l];
+                    acc[cell] = acc[cell] * 0.5f + val;
                 } else {
-                    acc[cell] = acc[cell] + w[l];
+                    acc[cell] = acc[cell] + val;
                 }
                 ++n_ev;
             }
         }
         for (int64_t j = 0; j < n_list; ++j) {
             const int64_t cell = list[j];
-            ring[cell] = accum_mode == 1 ? ring[cell] * 0.5f + acc[cell] : ring[cell] + acc[cell];
+            if (cell >= N * D) {
+                ring_v[cell - N * D] = ring_v[cell - N * D] + acc[cell];
+            } else {
+                ring[cell] = accum_mode == 1 ? ring[cell] * 0.5f + acc[cell] : ring[cell] + acc[cell];
+            }
             touched[cell] = 0;
         }
         /* 2-4. read and clear the due slot, background, LIF update, threshold */
@@ -118,6 +143,11 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
             }
             float x = ring[slot + i];
             ring[slot + i] = 0.0f;
+            float xv = 0.0f; /* the potential port's due input (edge plans) */
+            if (edges) {
+                xv = ring_v[slot + i];
+                ring_v[slot + i] = 0.0f;
+            }
             if (k_i > 1) {
                 int64_t q = (s - k_i + 1) % D;
                 float xw = ring[q * N + i];
@@ -159,6 +189,7 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
                     const float vr = P->v_rest + pv;
                     const float pvi = P->p_vi * I;
                     v[i] = vr + pvi;
+                    if (edges) v[i] = v[i] + xv; /* the potential port: a jump after the propagation */
                 }
                 isyn[i] = P->p_i * I;
                 /* send(): threshold, reset, refractory (test/fused_reference.py:100-103) */
@@ -176,6 +207,25 @@ done:
     if (events) *events = n_ev;
     free(acc); free(touched); free(list); free(pop);
     return rc;
+}
+
+int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops,
+           int32_t d_slots, uint64_t seed, uint32_t instance,
+           const int64_t* src_ptr, const int32_t* tgt, const float* w, const int32_t* delay,
+           float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk,
+           int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events) {
+    return run_core(model, n_neurons, n_pops, pop_first, pops, d_slots, seed, instance, src_ptr, tgt, (float*)w, delay,
+                    NULL, NULL, NULL, NULL, v, isyn, ref, ring, prev_spk, step0, n_steps, raster, accum_mode, events);
+}
+
+int or_run_edges(int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops, int32_t d_slots,
+                 uint64_t seed, uint32_t instance, const int64_t* src_ptr, const int32_t* tgt, float* w,
+                 const int32_t* delay, const uint8_t* kind, const uint8_t* port, const float* attr2, float* ring_v,
+                 float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk, int64_t step0, int32_t n_steps,
+                 uint8_t* raster, int64_t* events) {
+    if (!kind || !port || !ring_v) return -1;
+    return run_core(OR_LIF_EXP, n_neurons, n_pops, pop_first, pops, d_slots, seed, instance, src_ptr, tgt, w, delay,
+                    kind, port, attr2, ring_v, v, isyn, ref, ring, prev_spk, step0, n_steps, raster, 0, events);
 }
 
 /* ---- the same step, OpenMP-parallel over target ranges (the CPU baseline at all host cores) ----

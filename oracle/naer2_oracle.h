@@ -64,6 +64,19 @@ int or_run(int model, int32_t n_neurons, int32_t n_pops, const int32_t* pop_firs
            int64_t step0, int32_t n_steps, uint8_t* raster, int32_t accum_mode, int64_t* events);
 
 
+/* The exp-LIF step with per-link edge kinds and ports (the reference's open decisions D-6 / D-7 and edge
+ * forward, TODO.md:179-204, test/test_fused_generated_gpu.py:361-398, as libnaer2_hip.so's n2h_bind_edges
+ * defines them): per canonical link, kind 0 static delivers w, 1 gain delivers w * attr2, 2 plastic first does
+ * w += attr2 (stored: w is updated in place) then delivers the new w; port 0 adds to the current's ring, port 1
+ * to ring_v [d_slots][N], whose due input is added to V after the propagation, V = (V_rest + P_V (V - V_rest))
+ * + P_VI I, then V = V + x_v (not on refractory steps; the slot is cleared either way).  Per port the
+ * summation order is the plain step's.  Single-rate networks only. */
+int or_run_edges(int32_t n_neurons, int32_t n_pops, const int32_t* pop_first, const or_pop* pops, int32_t d_slots,
+                 uint64_t seed, uint32_t instance, const int64_t* src_ptr, const int32_t* tgt, float* w,
+                 const int32_t* delay, const uint8_t* kind, const uint8_t* port, const float* attr2, float* ring_v,
+                 float* v, float* isyn, int32_t* ref, float* ring, uint8_t* prev_spk, int64_t step0, int32_t n_steps,
+                 uint8_t* raster, int64_t* events);
+
 /* The same step, OpenMP-parallel over contiguous target ranges: identical results (every per-target sum
  * sees the same links in the same order).  or_mt_create splits the canonical CSR by target range. */
 typedef struct or_mt or_mt;

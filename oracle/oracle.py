@@ -56,6 +56,10 @@ def load():
                                ctypes.c_uint32, P, P, P, P, P, P, P, P, P, ctypes.c_int64, ctypes.c_int32, P,
                                ctypes.c_int32, P]
         lib.or_run.restype = ctypes.c_int
+        lib.or_run_edges.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, ctypes.c_int32, ctypes.c_uint64,
+                                     ctypes.c_uint32, P, P, P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int64,
+                                     ctypes.c_int32, P, P]
+        lib.or_run_edges.restype = ctypes.c_int
         lib.or_mt_create.argtypes = [ctypes.c_int32, P, P, P, P, ctypes.c_int32]
         lib.or_mt_create.restype = P
         lib.or_mt_destroy.argtypes = [P]
@@ -118,6 +122,35 @@ def run(model: int, pop_first, pops_bytes: bytes, d_slots: int, seed: int, insta
                     ctypes.byref(ev))
     if rc != 0:
         raise RuntimeError(f"or_run failed ({rc})")
+    return (raster.astype(bool) if record else None), int(ev.value)
+
+
+def run_edges(pop_first, pops_bytes: bytes, d_slots: int, seed: int, instance: int, src_ptr, tgt, w, delay,
+              kind, port, attr2, state: State, ring_v, step0: int, n_steps: int, record: bool = True):
+    """The exp-LIF step with edge kinds (0 static, 1 gain w * attr2, 2 plastic w += attr2 before delivery) and
+    ports (0 current, 1 potential through ring_v [d_slots, n]), in place on `state`, `w` (plastic links; must be
+    a contiguous float32 array) and `ring_v`; returns (raster or None, events)."""
+    lib = load()
+    pf = np.ascontiguousarray(pop_first, dtype=np.int32)
+    n = int(pf[-1])
+    pops = ctypes.create_string_buffer(pops_bytes, len(pops_bytes))
+    sp = np.ascontiguousarray(src_ptr, dtype=np.int64)
+    tg = np.ascontiguousarray(tgt, dtype=np.int32)
+    dd = np.ascontiguousarray(delay, dtype=np.int32)
+    kk = np.ascontiguousarray(kind, dtype=np.uint8)
+    pp = np.ascontiguousarray(port, dtype=np.uint8)
+    aa = np.ascontiguousarray(attr2, dtype=np.float32)
+    for a in (w, ring_v):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise ValueError("w and ring_v are updated in place: contiguous float32 arrays")
+    raster = np.zeros((n_steps, n), dtype=np.uint8) if record else None
+    ev = ctypes.c_int64(0)
+    rc = lib.or_run_edges(n, len(pf) - 1, _ptr(pf), ctypes.cast(pops, ctypes.c_void_p), d_slots, seed, instance,
+                          _ptr(sp), _ptr(tg), _ptr(w), _ptr(dd), _ptr(kk), _ptr(pp), _ptr(aa), _ptr(ring_v),
+                          _ptr(state.v), _ptr(state.isyn), _ptr(state.ref), _ptr(state.ring), _ptr(state.prev_spk),
+                          step0, n_steps, _ptr(raster), ctypes.byref(ev))
+    if rc != 0:
+        raise RuntimeError(f"or_run_edges failed ({rc})")
     return (raster.astype(bool) if record else None), int(ev.value)
 
 
