@@ -691,6 +691,77 @@ def measure_brunel_exact(args) -> dict:
 COST_S = {"brunel": 40.0, "bptt": 20.0, "pd14grad": 90.0, "sweep": 240.0}
 
 
+def _edge_variant(net, name: str) -> None:
+    """Turn PD14's connections into edges of the step's edge variant: "gain" (gain 1.0: delivers exactly w),
+    "plastic" (factor -0.0: w + -0.0 = w, stored back every delivery), or "mixed" (excitatory connections
+    plastic with factors U(-1e-4, 1e-4) nA, inhibitory ones into the potential port with their weight scaled
+    by R_mem tau_syn / tau_m = 2 mV per nA: the jump that the charge of their current pulse would give)."""
+    g = torch.Generator(device="cpu").manual_seed(7)
+    for c in net.connections:
+        if name == "gain":
+            c.kind = "gain"
+            c.gain = 1.0
+        elif name == "plastic":
+            c.kind = "plastic"
+            c.factor = -0.0
+        elif float(c.weight[0]) >= 0:
+            c.kind = "plastic"
+            c.factor = (torch.rand(c.shape, generator=g) * 2e-4 - 1e-4)
+        else:
+            c.port = "potential"
+            c.weight = c.weight * 2.0
+
+
+def measure_edges(args, steps: int, warmup: int) -> dict:
+    """Full-scale PD14 through the step's edge variant (edge kinds and ports), timed like the headline: the
+    plain step, then every connection a gain edge of gain 1.0 and a plastic edge of factor -0.0 -- both deliver
+    exactly the plain weights, so their rasters must equal the plain run's bit for bit -- and a mixed network
+    (plastic excitatory connections, inhibitory ones into the potential port)."""
+    from naer2_amd.networks import pd14
+
+    out, plain_bits = {}, None
+    for name in ("plain", "gain", "plastic", "mixed"):
+        t0 = time.perf_counter()
+        net, pops = pd14(scale=1.0, seed=args.seed, device="cuda")
+        if name != "plain":
+            _edge_variant(net, name)
+        rt = net.build()
+        net.record = True
+        out_deg = rt.canon.out_degree()
+        build_s = time.perf_counter() - t0
+        if warmup:
+            rt.prepare(warmup, args.window)
+            net.run(warmup, graph_window=args.window)
+        m = timed_run(net, rt, steps, args.window, out_deg, "pd14")
+        rates = net.rates(t_skip_ms=m["from_step"] * rt.dt)
+        bits = rt.raster_bits(0, rt.step)
+        row = {"rtf": round(m["rtf"], 4), "kernel_avg_us": round(m["kernel_avg_us"], 3),
+               "events_per_s": round(m["events_per_s"], 1), "spikes_per_step": round(m["spikes"] / steps, 3),
+               "rates_hz": {k: round(rates[p], 3) for k, p in pops.items()}, "edges": bool(rt.edges),
+               "build_s": round(build_s, 1)}
+        if name == "plain":
+            plain_bits = bits.clone()
+        elif name in ("gain", "plastic"):
+            row["raster_equals_plain"] = bool(torch.equal(bits, plain_bits))
+            if name == "plastic":
+                w = rt.link_weights()
+                w0 = torch.cat([c.weight.to(w.device) for c in net.connections])
+                row["weights_unchanged"] = bool(torch.equal(w.view(torch.int32), w0.view(torch.int32)))
+        out[name] = row
+        log(f"edges {name}: RTF {row['rtf']}, {row['kernel_avg_us']} us per launch")
+        del rt, net, bits
+        torch.cuda.empty_cache()
+    return {
+        "metric": "PD14 real-time factor through the edge variant (plastic edges)",
+        "value": out["plastic"]["rtf"], "unit": "bio_s/wall_s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"pd14 full scale, seed {args.seed}, {steps} timed steps after {warmup}: plain, every "
+                               "connection gain 1.0, every connection plastic with factor -0.0, mixed (plastic "
+                               "excitatory, inhibitory into the potential port)"},
+        "variants": out,
+    }
+
+
 def configs(args, cpu: dict) -> dict:
     """The other configurations at their full sizes, in order of cost, each within the time budget."""
     out = {}
@@ -736,12 +807,14 @@ def main() -> int:
     ap.add_argument("--timestep", action="append", metavar="POP=MS",
                     help="pd14/brunel: a population's own timestep, a whole multiple of dt (repeatable)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad", "brunel_exact"],
+    ap.add_argument("--workload", choices=["pd14", "brunel", "sweep", "bptt", "pd14grad", "brunel_exact",
+                                           "pd14_edges"],
                     default="pd14",
                     help="pd14: the headline PD14 real-time factor (with the other configurations beside it); "
                          "brunel: the 2500-neuron Brunel network; sweep: the batched Brunel sweep; bptt: the "
                          "recurrent-LIF BPTT; pd14grad: the batched differentiable PD14; brunel_exact: the "
-                         "event-driven Brunel model with exact spike times")
+                         "event-driven Brunel model with exact spike times; pd14_edges: PD14 through the step's "
+                         "edge variant (gain, plastic and potential-port edges)")
     ap.add_argument("--neurons", type=int, default=None, help="brunel_exact: N = N_E + N_I (default 50,000)")
     ap.add_argument("--ev-windows", type=int, default=667, help="brunel_exact: timed windows of 1.5 ms")
     ap.add_argument("--ev-warmup", type=int, default=67, help="brunel_exact: untimed windows first")
@@ -799,6 +872,8 @@ def main() -> int:
         line = measure_brunel_exact(args)
     elif args.workload == "sweep":
         line = measure_sweep(args, args.steps, max(args.warmup, args.window), args.window, args.instances or 256)
+    elif args.workload == "pd14_edges":
+        line = measure_edges(args, args.steps, args.warmup)
     else:
         line = measure_plain(args, args.workload, args.steps, args.warmup, not args.no_cpu, cpu)
         torch.cuda.empty_cache()

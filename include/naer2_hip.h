@@ -324,15 +324,15 @@ int n2h_set_vtrace(n2h_plan* plan, float* buffer, int64_t step0, int32_t n_steps
  * step's links; each target's records appear in contract order (ascending canonical link index), targets
  * interleaved.  link_conn: device int16 [n_links], each link's connection (creation order), parallel to
  * tables->links; NULL: -1.  Logging steps run a variant without the row intake (results are identical); a
- * logging plan cannot join a batch.  records NULL: off.  Graphs captured before the call keep the setting
+ * logging plan (at most 2^32 - 1 links) cannot join a batch.  records NULL: off.  Graphs captured before the call keep the setting
  * they were captured with. */
 int n2h_bind_log(n2h_plan* plan, const int16_t* link_conn, int32_t* records, int32_t* count, int64_t capacity);
 
 /* Edge kinds and ports per connection (the reference's open decisions D-6, per-connection receive code, and D-7,
  * per-link attribute blocks, with edge forward: TODO.md:179-204, test/test_fused_generated_gpu.py:361-398).
- * With ring_v bound, every link's packed word x carries its edge kind in bits 8-9 and its port in bit 10
+ * Every link's packed word x carries its edge kind in bits 8-9 and its port in bit 10
  * (x = delay << 16 | port << 10 | kind << 8 | local target), and edge_attr [n_links] (parallel to
- * tables->links, NULL when no link needs it) its second attribute:
+ * tables->links; NULL when every link is static) its second attribute:
  *   kind 0  static: delivers its weight w
  *   kind 1  gain: delivers w * edge_attr
  *   kind 2  plastic: w += edge_attr first, stored back into the link table (the update persists), then
@@ -340,10 +340,14 @@ int n2h_bind_log(n2h_plan* plan, const int16_t* link_conn, int32_t* records, int
  *   port 0  current: adds to I with the other current inputs (the summation contract per port)
  *   port 1  potential: ring_v [d_slots][n_neurons], read and cleared like the ring; at an update that is not
  *           refractory, V = (V_rest + P_V (V - V_rest)) + P_VI I, then V = V + the potential input
- * LIF_EXP plans only, every delay cell kept (tables->uniform_delay 0), every population updating every step;
- * such a plan steps without the row intake (same results), takes no compensated trace and cannot join a
- * batch.  ring_v NULL: off (kinds and ports in x must then be 0). */
-int n2h_bind_edges(n2h_plan* plan, const float* edge_attr, float* ring_v);
+ * v_slots: at least the largest delay (steps) of a potential-port link, at most d_slots; the step keeps that
+ * many extra LDS partial-sum cells per group (4 * 64 * v_slots bytes), and a potential-port link of a longer
+ * delay is not delivered.  ring_v and v_slots > 0 go together (0 and NULL: no potential-port links).
+ * LIF_EXP plans of at most 2^32 - 1 links only, every delay cell kept (tables->uniform_delay 0), every
+ * population updating every step; such a plan steps without the row intake (same results), takes no
+ * compensated trace and cannot join a batch.  edge_attr and ring_v both NULL: off (kinds and ports in x must
+ * then be 0). */
+int n2h_bind_edges(n2h_plan* plan, const float* edge_attr, float* ring_v, int32_t v_slots);
 
 /* Status rows (the per-group status of naer2/transpiler/templates/fused.py:865-883, which the reference
  * writes for its runs: blocks held, most logged, lost bits, done): with rows, device int32 [n_groups][4], every

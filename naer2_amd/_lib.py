@@ -129,7 +129,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_set_trace": ([vp, vp], i32),
         "n2h_bind_log": ([vp, vp, vp, vp, i64], i32),
         "n2h_set_status": ([vp, vp], i32),
-        "n2h_bind_edges": ([vp, vp, vp], i32),
+        "n2h_bind_edges": ([vp, vp, vp, i32], i32),
         "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
         "n2h_set_vtrace": ([vp, vp, i64, i32, vp, vp], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),

@@ -122,7 +122,8 @@ struct StepArgs {
     int32_t* status;            // null, or [n_groups][4] status rows, written by every step
     const float* edge_attr;     // edge plans (the edge kernel variant): each link's second attribute (gain or
                                 //   factor), parallel to links; link word x bits 8-9 the edge kind, bit 10 the port
-    float* ring_v;              //   [d_slots][n]: the inputs of the potential port
+    float* ring_v;              //   [d_slots][n]: the inputs of the potential port (null: no potential-port links)
+    int32_t v_slots;            //   the potential port's LDS cells: its links' largest delay
     uint2* links_rw;            //   the link table, written by plastic links (same address as links)
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
@@ -179,13 +180,15 @@ __device__ __forceinline__ int wave_rank(int key, bool valid, int bits) {
 // Shared-memory staging of one workgroup.  BM: 64-bit words of the row intake's spike bitmap (the
 // groups' words, then the sentinel word); kernel variants size it for the groups they serve.
 // CAPS / CAPL: spiking sources and links staged per pass; RU: row cells in flight per lane and batch.
-template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU, bool LOG = false, bool EDGE = false>
+template <int BM, int CAPS = kCapS, int CAPL = kCapL, int RU = kRowU, bool LOG = false, bool EDGE = false,
+          bool PORTS = false>
 struct Stage {
     static constexpr int kS = CAPS;
     static constexpr int kL = CAPL;
     static constexpr int kRU = RU;
     static constexpr bool kLog = LOG;
     static constexpr bool kEdge = EDGE;
+    static constexpr bool kPorts = PORTS;         // edge plans with potential-port links
     static constexpr bool kCarry = LOG || EDGE;  // staged links carry their table index through the sort
     union {
         struct {                     // segment intake
@@ -196,7 +199,7 @@ struct Stage {
             uint2 sorted[CAPL];          // the staged links regrouped by target, each target's in contract order
             int32_t tcount[kWave];       // staged links per target
             int32_t tstart[kWave];       // first of each target's links in sorted
-            int64_t log_idx[(LOG || EDGE) ? CAPL : 1];  // each sorted link's index into the link table
+            uint32_t log_idx[(LOG || EDGE) ? CAPL : 1];  // each sorted link's index into the link table
             int32_t log_src[LOG ? CAPL : 1];            //   and its source slot (delivery log)
         };
         struct {                     // row intake
@@ -244,16 +247,16 @@ __device__ __forceinline__ void cell_add(float* acc, int lane, int b, float w, i
 struct LogCtx {
     int64_t s;     // the step: the staged links are the spikes of step s - 1
     int first;     // the target group's first neuron
-    const int64_t* idx;
+    const uint32_t* idx;
     const int32_t* src;
-    float* acc_v;                // edge plans: the potential port's partial sums [d_slots][64]
+    float* acc_v;                // edge plans with ports: the potential port's partial sums [v_slots][64]
     unsigned long long* u;       //   and its touched delays (u[0]: 1..64, u[1]: 65..128)
 };
 
 // Edges (St::kEdge; the reference's D-6 receive variants, D-7 per-link attribute blocks and edge forward):
 // the value a link delivers by its kind -- static w; gain w * g; plastic: w += f first (the stored weight
 // too: the update persists), then the new w -- into the partial sums of its port (current or potential).
-__device__ __forceinline__ float edge_value(const StepArgs& a, uint32_t x, float w, int64_t idx) {
+__device__ __forceinline__ float edge_value(const StepArgs& a, uint32_t x, float w, uint32_t idx) {
     const uint32_t kind = (x >> 8) & 3u;
     if (kind == 1u) return w * a.edge_attr[idx];
     if (kind == 2u) {
@@ -328,8 +331,17 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
         const uint2 l = st.sorted[j];
         if constexpr (St::kEdge) {
             const float v = edge_value(a, l.x, __uint_as_float(l.y), st.log_idx[j]);
-            if ((l.x >> 10) & 1u) cell_add(lc.acc_v, lane, static_cast<int>(l.x >> 16) - 1, v, cbv, crv, lc.u[0], lc.u[1]);
-            else cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, v, cb, cr, t0, t1);
+            const int b = static_cast<int>(l.x >> 16) - 1;
+            if constexpr (St::kPorts) {
+                // the potential port's cells stop at v_slots (its links' largest delay, set by the host)
+                if ((l.x >> 10) & 1u) {
+                    if (b < a.v_slots) cell_add(lc.acc_v, lane, b, v, cbv, crv, lc.u[0], lc.u[1]);
+                } else {
+                    cell_add(acc, lane, b, v, cb, cr, t0, t1);
+                }
+            } else {
+                cell_add(acc, lane, b, v, cb, cr, t0, t1);
+            }
         } else {
             cell_add(acc, lane, static_cast<int>(l.x >> 16) - 1, __uint_as_float(l.y), cb, cr, t0, t1);
         }
@@ -348,7 +360,7 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
         }
     }
     if (cb >= 0) acc[cb * kWave + lane] = cr;
-    if constexpr (St::kEdge) {
+    if constexpr (St::kPorts) {
         if (cbv >= 0) lc.acc_v[cbv * kWave + lane] = crv;
     }
 }
@@ -556,7 +568,7 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
             __syncthreads();
         }
         uint2 r[kPer];
-        int64_t lidx[St::kCarry ? kPer : 1];
+        uint32_t lidx[St::kCarry ? kPer : 1];
         int32_t lsrc[St::kLog ? kPer : 1];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
@@ -566,7 +578,7 @@ __device__ __forceinline__ void deliver_links(const StepArgs& a, St& st, float* 
                 const int i = st.owner[e - e0];
                 const int64_t li = st.seg_lo[i] + (e - st.seg_start[i]);
                 r[u] = a.links[li];
-                if constexpr (St::kCarry) lidx[u] = li;
+                if constexpr (St::kCarry) lidx[u] = static_cast<uint32_t>(li);
                 if constexpr (St::kLog) lsrc[u] = st.sp[i];
             }
         }
@@ -586,7 +598,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     // a uniform delay)
     float* acc = acc_lds - a.acc_base * kWave;
     // edge plans: the potential port's partial sums follow the current port's (acc_base is 0 there)
-    float* acc_v = St::kEdge ? acc_lds + a.d_slots * kWave : nullptr;
+    float* acc_v = St::kPorts ? acc_lds + a.d_slots * kWave : nullptr;
     unsigned long long tv[2] = {0ull, 0ull};
     const int NG = a.n_groups;
     const int64_t N = a.n_neurons;
@@ -638,8 +650,8 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (a.multirate) stride = __builtin_amdgcn_readfirstlane(a.pops[pop].stride);
     const bool upd = stride <= 1 || static_cast<uint64_t>(s + 1) % static_cast<uint64_t>(stride) == 0ull;
     if (live && upd) x = a.ring[cur];
-    float xv = 0.0f;  // edge plans: the potential port's due input
-    if constexpr (St::kEdge) {
+    float xv = 0.0f;  // edge plans with ports: the potential port's due input
+    if constexpr (St::kPorts) {
         if (live && upd) xv = a.ring_v[cur];
     }
     // the background draw needs only the neuron and the step: computed here, under the latency of the
@@ -775,7 +787,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
     if (lane < kParamWords) st.param_words[lane] = pword;
     // the delay-1 cell's partial sum is the one due now: ring[s][i] + p, read in registers
     if (live && (t0 & 1ull)) x = x + acc[lane];
-    if constexpr (St::kEdge) {
+    if constexpr (St::kPorts) {
         if (live && (tv[0] & 1ull)) xv = xv + acc_v[lane];
     }
 
@@ -826,7 +838,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                 const float vr = P.v_rest + pv;
                 const float pvi = P.p_vi * I;
                 V = vr + pvi;
-                if constexpr (St::kEdge) V = V + xv;  // the potential port: a jump after the propagation
+                if constexpr (St::kPorts) V = V + xv;  // the potential port: a jump after the propagation
                 vpre = V;
             }
             I = P.p_i * I;
@@ -908,7 +920,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             a.ref[nid] = R;
             if (MODEL == N2H_MODEL_LIF_EXP) a.isyn[nid] = I;
             a.ring[cur] = 0.0f;
-            if constexpr (St::kEdge) a.ring_v[cur] = 0.0f;
+            if constexpr (St::kPorts) a.ring_v[cur] = 0.0f;
             int q = s_mod;
             for (int j = 1; j < stride; ++j) {  // the window's earlier slots, read above
                 q = (q == 0) ? D - 1 : q - 1;
@@ -937,7 +949,7 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             if (slot >= D) slot -= D;
             unsafeAtomicAdd(&a.ring[static_cast<int64_t>(slot) * N + nid], acc[b * kWave + lane]);
         }
-        if constexpr (St::kEdge) {  // the potential port's cells into its own ring, as above
+        if constexpr (St::kPorts) {  // the potential port's cells into its own ring, as above
             for (int h = 0; h < 2; ++h) {
                 m = h == 0 ? (upd ? (tv[0] & ~1ull) : tv[0]) : tv[1];
                 while (m) {
@@ -963,9 +975,12 @@ __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
 
 // The edge variant (n2h_bind_edges): the step without the row intake, links delivering by their edge kind into
 // their port's partial sums and ring.
-template <int MODEL, bool LOG>
+// With ports the LDS holds a second set of partial sums (v_slots cells): the stage then takes the batched
+// kernel's smaller passes (kBatchCapS sources, kBatchCapL links), so that a PD14-sized plan (60 slots, ~30
+// potential-port slots) still fits five groups per CU.
+template <int MODEL, bool LOG, bool PORTS>
 __global__ __launch_bounds__(kWave) void step_edge_kernel(StepArgs a, int32_t k) {
-    __shared__ Stage<bm_words(0), kCapS, kCapL, kRowU, LOG, true> st;
+    __shared__ Stage<bm_words(0), PORTS ? kBatchCapS : kCapS, PORTS ? kBatchCapL : kCapL, kRowU, LOG, true, PORTS> st;
     extern __shared__ float acc[];
     step_group<MODEL, 0>(a, blockIdx.x, k, st, acc);
 }
@@ -1070,6 +1085,8 @@ struct n2h_plan {
     int32_t* status;
     const float* edge_attr;
     float* ring_v;
+    int32_t v_slots;        // edge plans: LDS cells of the potential port (0: no potential-port links)
+    bool edges;             // n2h_bind_edges bound kinds or ports
     int32_t multirate;      // some population's stride > 1
     size_t lds_bytes;
     int device;
@@ -1153,6 +1170,7 @@ StepArgs make_args(const n2h_plan* p) {
     a.status = p->status;
     a.edge_attr = p->edge_attr;
     a.ring_v = p->ring_v;
+    a.v_slots = p->v_slots;
     a.links_rw = const_cast<uint2*>(reinterpret_cast<const uint2*>(p->tables.links));
     a.multirate = p->multirate;
     return a;
@@ -1174,9 +1192,16 @@ int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
-        if (p->ring_v) {  // edge plans (LIF_EXP only)
-            if (p->log_rec) hipLaunchKernelGGL((step_edge_kernel<N2H_MODEL_LIF_EXP, true>), grid, block, p->lds_bytes, stream, a, k);
-            else hipLaunchKernelGGL((step_edge_kernel<N2H_MODEL_LIF_EXP, false>), grid, block, p->lds_bytes, stream, a, k);
+        if (p->edges) {  // edge plans (LIF_EXP only)
+            constexpr int E = N2H_MODEL_LIF_EXP;
+            if (p->log_rec && p->ring_v)
+                hipLaunchKernelGGL((step_edge_kernel<E, true, true>), grid, block, p->lds_bytes, stream, a, k);
+            else if (p->log_rec)
+                hipLaunchKernelGGL((step_edge_kernel<E, true, false>), grid, block, p->lds_bytes, stream, a, k);
+            else if (p->ring_v)
+                hipLaunchKernelGGL((step_edge_kernel<E, false, true>), grid, block, p->lds_bytes, stream, a, k);
+            else
+                hipLaunchKernelGGL((step_edge_kernel<E, false, false>), grid, block, p->lds_bytes, stream, a, k);
         } else if (p->log_rec) {
             if (p->desc.model == N2H_MODEL_LIF_DELTA)
                 hipLaunchKernelGGL(step_log_kernel<N2H_MODEL_LIF_DELTA>, grid, block, p->lds_bytes, stream, a, k);
@@ -1466,7 +1491,7 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     for (int32_t i = 0; i < n_plans; ++i)
         if (plans[i] && plans[i]->log_rec) return fail(N2H_EINVAL, "plan %d logs its deliveries: a logging plan steps alone", i);
     for (int32_t i = 0; i < n_plans; ++i)
-        if (plans[i] && plans[i]->ring_v) return fail(N2H_EINVAL, "plan %d has edge kinds or ports: it steps alone", i);
+        if (plans[i] && plans[i]->edges) return fail(N2H_EINVAL, "plan %d has edge kinds or ports: it steps alone", i);
     n2h_tuning tu{};
     if (tuning) tu = *tuning;
     if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
@@ -1628,7 +1653,7 @@ int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps, f
     if ((v_lo == nullptr) != (i_lo == nullptr)) return fail(N2H_EINVAL, "v_lo and i_lo go together");
     if (v_lo && p->desc.model != N2H_MODEL_LIF_EXP)
         return fail(N2H_EINVAL, "the compensated state is tracked for the exponential-synapse model only");
-    if (v_lo && p->ring_v) return fail(N2H_EINVAL, "edge plans take no compensated trace");
+    if (v_lo && p->edges) return fail(N2H_EINVAL, "edge plans take no compensated trace");
     p->vtrace = buffer;
     p->vtrace_step0 = step0;
     p->vtrace_steps = buffer ? n_steps : 0;
@@ -1640,6 +1665,8 @@ int n2h_set_vtrace(n2h_plan* p, float* buffer, int64_t step0, int32_t n_steps, f
 int n2h_bind_log(n2h_plan* p, const int16_t* link_conn, int32_t* records, int32_t* count, int64_t capacity) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     if (records && (!count || capacity < 0)) return fail(N2H_EINVAL, "a log needs its count and a capacity >= 0");
+    if (records && p->tables.n_links > static_cast<int64_t>(UINT32_MAX))
+        return fail(N2H_EINVAL, "logging plans hold at most 2^32 - 1 links (%lld)", static_cast<long long>(p->tables.n_links));
     p->log_rec = records;
     p->log_count = records ? count : nullptr;
     p->log_conn = records ? link_conn : nullptr;
@@ -1647,21 +1674,30 @@ int n2h_bind_log(n2h_plan* p, const int16_t* link_conn, int32_t* records, int32_
     return N2H_OK;
 }
 
-int n2h_bind_edges(n2h_plan* p, const float* edge_attr, float* ring_v) {
+int n2h_bind_edges(n2h_plan* p, const float* edge_attr, float* ring_v, int32_t v_slots) {
     if (!p) return fail(N2H_EINVAL, "null plan");
-    if (!ring_v) {
+    if (v_slots < 0 || v_slots > p->desc.d_slots) return fail(N2H_EINVAL, "v_slots %d outside [0, d_slots]", v_slots);
+    if ((v_slots > 0) != (ring_v != nullptr)) return fail(N2H_EINVAL, "ring_v and v_slots > 0 go together");
+    const size_t cell = sizeof(float) * kWave;
+    if (!edge_attr && !ring_v) {
+        p->edges = false;
         p->edge_attr = nullptr;
         p->ring_v = nullptr;
-        p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(p->tables.uniform_delay > 0 ? 1 : p->desc.d_slots);
+        p->v_slots = 0;
+        p->lds_bytes = cell * static_cast<size_t>(p->tables.uniform_delay > 0 ? 1 : p->desc.d_slots);
         return N2H_OK;
     }
     if (p->desc.model != N2H_MODEL_LIF_EXP) return fail(N2H_EINVAL, "edge kinds and ports need the exponential-synapse model");
     if (p->tables.uniform_delay > 0) return fail(N2H_EINVAL, "edge plans keep every delay cell (uniform_delay 0)");
     if (p->multirate) return fail(N2H_EINVAL, "edge plans update every population every step");
     if (p->vlo) return fail(N2H_EINVAL, "edge plans take no compensated trace");
+    if (p->tables.n_links > static_cast<int64_t>(UINT32_MAX))
+        return fail(N2H_EINVAL, "edge plans hold at most 2^32 - 1 links (%lld)", static_cast<long long>(p->tables.n_links));
+    p->edges = true;
     p->edge_attr = edge_attr;
     p->ring_v = ring_v;
-    p->lds_bytes = 2 * sizeof(float) * kWave * static_cast<size_t>(p->desc.d_slots);
+    p->v_slots = v_slots;
+    p->lds_bytes = cell * static_cast<size_t>(p->desc.d_slots + v_slots);
     return N2H_OK;
 }
 

@@ -288,7 +288,7 @@ class Connection:
         self._n_links = int(tgt.numel())
         self._released = False
         self.edge = edge_instance
-        self.kind = edge_kind(edge_instance)
+        self._kind = edge_kind(edge_instance)
         self._port = None
         self.port = port
         self._weight = None
@@ -365,6 +365,21 @@ class Connection:
     def __or__(self, value) -> "Connection":
         self.delay = value
         return self
+
+    @property
+    def kind(self) -> str:
+        """The edge kind the step runs for this connection's links (EDGE_KINDS); set from the edge instance,
+        or directly (the kind's second attribute then has to be set again)."""
+        return self._kind
+
+    @kind.setter
+    def kind(self, value: str) -> None:
+        if value not in EDGE_KINDS:
+            raise ValueError(f"edge kind {value!r}: one of {sorted(EDGE_KINDS)}")
+        if value != self._kind:
+            self._kind = value
+            self._attr2 = None
+            self.source.network._invalidate()
 
     @property
     def port(self) -> str | None:

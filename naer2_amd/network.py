@@ -224,14 +224,16 @@ class Assembled:
 # the edge bits of a link's packed word (n2h_bind_edges): kind in bits 8-9, port in bit 10
 EDGE_KIND_BITS = {"static": 0, "gain": 1 << 8, "plastic": 2 << 8}
 PORT_POTENTIAL_BIT = 1 << 10
-# the columns of the per-link payload (`canon.extra`, int32 [n_links, 4])
+# the rows of the per-link payload (`canon.extra`, int32 [4, n_links])
 PAYLOAD_CONN, PAYLOAD_FLAGS, PAYLOAD_ATTR, PAYLOAD_INPUT = range(4)
 
 
 def assemble(net: Network, device=None, with_connections: bool = False) -> Assembled:
     """Everything the plan needs, on `device` (default: the network's), without touching the GPU.
     With with_connections, or when some connection has a non-static edge kind or the potential port, the
-    canonical table carries an int32 [n_links, 4] payload (`canon.extra`): per link its connection index
+    canonical table carries an int32 [4, n_links] payload (`canon.extra`, one row per field, each row indexed
+    as a 1-D tensor: torch's row indexing of a [n_links, 4] table gives wrong rows past about 2 GiB on this
+    ROCm build): per link its connection index
     (creation order), its edge bits (EDGE_KIND_BITS | PORT_POTENTIAL_BIT), the bits of its second attribute
     (gain or factor; 0.0 for static edges) and its place in the connections' links concatenated in creation
     order."""
@@ -274,17 +276,17 @@ def assemble(net: Network, device=None, with_connections: bool = False) -> Assem
         ds.append(delay_steps(c.delay.to(dev), dt))
         if payload:
             m = srcs[-1].numel()
-            col = torch.zeros(m, 4, dtype=torch.int32, device=dev)
-            col[:, PAYLOAD_CONN] = ci
-            col[:, PAYLOAD_FLAGS] = EDGE_KIND_BITS[c.kind] | (PORT_POTENTIAL_BIT if c.port == "potential" else 0)
+            col = torch.zeros(4, m, dtype=torch.int32, device=dev)
+            col[PAYLOAD_CONN] = ci
+            col[PAYLOAD_FLAGS] = EDGE_KIND_BITS[c.kind] | (PORT_POTENTIAL_BIT if c.port == "potential" else 0)
             if c.second_attr is not None:
-                col[:, PAYLOAD_ATTR] = c.second_attr.to(dev, torch.float32).view(torch.int32)
-            col[:, PAYLOAD_INPUT] = torch.arange(base, base + m, dtype=torch.int32, device=dev)
+                col[PAYLOAD_ATTR] = c.second_attr.to(dev, torch.float32).view(torch.int32)
+            col[PAYLOAD_INPUT] = torch.arange(base, base + m, dtype=torch.int32, device=dev)
             base += m
             cs.append(col)
     cat = (lambda xs, dt_: torch.cat(xs) if xs else torch.zeros(0, dtype=dt_, device=dev))
     canon = canonical(n, cat(srcs, torch.int64), cat(tgts, torch.int64), cat(ws, torch.float32),
-                      cat(ds, torch.int32), (torch.cat(cs) if cs else torch.zeros(0, 4, dtype=torch.int32, device=dev))
+                      cat(ds, torch.int32), (torch.cat(cs, dim=1) if cs else torch.zeros(4, 0, dtype=torch.int32, device=dev))
                       if payload else None)
     del srcs, tgts, ws, ds, cs
     v = torch.empty(n, dtype=torch.float32, device=dev)
@@ -344,16 +346,24 @@ class Runtime:
         # attribute beside the table, and the potential port's own ring
         self.edges = a.edges
         self.edge_attr = self.ring_v = self.link_input = None
+        self.v_slots = 0
         if self.edges:
             if net.row_intake is True:
                 raise ValueError("edge kinds and ports step without the row intake")
             x = seg.extra
-            self.links[:, 0] |= x[:, PAYLOAD_FLAGS]
-            self.edge_attr = x[:, PAYLOAD_ATTR].contiguous().view(torch.float32)
-            self.link_input = x[:, PAYLOAD_INPUT].contiguous()
-            self.ring_v = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
-            self._plastic = bool((x[:, PAYLOAD_FLAGS] & EDGE_KIND_BITS["plastic"]).any())
-            del x
+            flags = x[PAYLOAD_FLAGS]
+            self.links[:, 0] |= flags
+            kinds = flags & (3 << 8)
+            if bool((kinds != 0).any()):  # gain or plastic links read their second attribute
+                self.edge_attr = x[PAYLOAD_ATTR].contiguous().view(torch.float32)
+            self.link_input = x[PAYLOAD_INPUT].contiguous()
+            self._plastic = bool((kinds == EDGE_KIND_BITS["plastic"]).any())
+            # the potential port: its own ring, and LDS cells up to its links' largest delay
+            pot = (flags & PORT_POTENTIAL_BIT) != 0
+            self.v_slots = int(((self.links[:, 0] >> 16) * pot).max()) if bool(pot.any()) else 0
+            if self.v_slots:
+                self.ring_v = torch.zeros(self.d_slots, self.n, dtype=torch.float32, device=self.device)
+            del x, flags, kinds, pot
         self.conn_links = [c.shape[0] for c in net.connections]
         self.rows = None
         if net.row_intake and self.n_groups <= _lib.DENSE_GROUPS and not self.edges:
@@ -407,7 +417,9 @@ class Runtime:
         self._bind_rows()
         self._bind_ring()
         if self.edges:
-            _lib.check(lib.n2h_bind_edges(plan, self.edge_attr.data_ptr(), self.ring_v.data_ptr()), "n2h_bind_edges")
+            ptr = (lambda t: t.data_ptr() if t is not None else None)
+            _lib.check(lib.n2h_bind_edges(plan, ptr(self.edge_attr), ptr(self.ring_v), self.v_slots),
+                       "n2h_bind_edges")
 
     def _bind_rows(self) -> None:
         lib = _lib.load()
@@ -581,7 +593,7 @@ class Runtime:
             if len(self.net.connections) > 32767:
                 raise ValueError("at most 32767 connections carry their index in the delivery log")
             a = assemble(self.net, self.device, with_connections=True)
-            conn = segments(a.canon, self.layout).extra[:, PAYLOAD_CONN].to(torch.int16).contiguous()
+            conn = segments(a.canon, self.layout).extra[PAYLOAD_CONN].to(torch.int16).contiguous()
             del a
             rec = torch.zeros(int(capacity), 5, dtype=torch.int32, device=self.device)
             count = torch.zeros(1, dtype=torch.int32, device=self.device)
@@ -679,7 +691,8 @@ class Runtime:
         with edges, the potential port's ring and, with plastic edges, the link weights)."""
         edge = None
         if self.edges:
-            edge = (self.ring_v.clone(), self.links[:, 1].clone() if self._plastic else None)
+            edge = (None if self.ring_v is None else self.ring_v.clone(),
+                    self.links[:, 1].clone() if self._plastic else None)
         return (self.step, self.v.clone(), None if self.isyn is None else self.isyn.clone(), self.ref.clone(),
                 self.ring.clone(), self.bits.clone(), self.spike_list.clone(), edge)
 
@@ -688,7 +701,8 @@ class Runtime:
         is replaced by the snapshot's (not while a FusedRound holds this runtime: it keeps the addresses)."""
         step, v, isyn, ref, ring, bits, lst, edge = snap
         if edge is not None:
-            self.ring_v.copy_(edge[0])
+            if edge[0] is not None:
+                self.ring_v.copy_(edge[0])
             if edge[1] is not None:
                 self.links[:, 1] = edge[1]
         self.v.copy_(v)

@@ -29,7 +29,7 @@ class Canonical:
     tgt: torch.Tensor      # int32 [E]
     w: torch.Tensor        # float32 [E]
     d: torch.Tensor        # int32 [E], delay in steps, >= 1
-    extra: torch.Tensor | None = None  # an optional per-link payload in the same order (e.g. connection ids)
+    extra: torch.Tensor | None = None  # an optional per-link payload in the same order: [E] or [k, E] (k rows)
 
     @property
     def n_links(self) -> int:
@@ -89,7 +89,7 @@ def _stable_order_by_key(key: torch.Tensor, n_keys: int, chunk: int | None = Non
 def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: torch.Tensor,
               extra: torch.Tensor | None = None) -> Canonical:
     """The canonical CSR of links given in any source order (a stable sort by source keeps each source's
-    links in their given order); extra, a per-link payload, follows the links."""
+    links in their given order); extra, a per-link payload ([E], or [k, E]: k rows of E), follows the links."""
     src = src.to(torch.int64)
     if src.numel() and (int(src.min()) < 0 or int(src.max()) >= n):
         raise ValueError("link source out of range")
@@ -110,7 +110,8 @@ def canonical(n: int, src: torch.Tensor, tgt: torch.Tensor, w: torch.Tensor, d: 
     out_x = None
     if extra is not None:
         out_x = torch.empty_like(extra)
-        out_x[pos] = extra
+        for row_out, row in _payload_rows(out_x, extra):
+            row_out[pos] = row
     return Canonical(n, src_ptr, out_t, out_w, out_d, out_x)
 
 
@@ -149,13 +150,20 @@ def group_layout(pop_sizes: list[int], device="cpu") -> GroupLayout:
 class Segments:
     seg_off: torch.Tensor  # int64 [64 * G * G + 1]
     links: torch.Tensor    # int32 [E, 2]: (delay << 16 | local target, weight bits)
-    extra: torch.Tensor | None = None  # the canonical table's per-link payload, in segment order
+    extra: torch.Tensor | None = None  # the canonical table's per-link payload ([E] or [k, E]), in segment order
 
 
 def slot_of(layout: GroupLayout) -> torch.Tensor:
     """int64 [n]: a neuron's source slot, 64 * (its group) + (its index in the group) -- the id the step's
     spike lists and bitmaps carry."""
     return layout.group_of * GROUP + layout.local_of
+
+
+def _payload_rows(out: torch.Tensor, src: torch.Tensor):
+    """(out row, source row) pairs of a per-link payload: itself when 1-D, else its k rows.  Each row is
+    permuted as a 1-D tensor: torch's row gather and scatter on a contiguous [E, 4] int32 table return wrong
+    rows once the table passes about 2 GiB on this ROCm build, where 1-D indexing is exact."""
+    return ((out, src),) if src.dim() == 1 else zip(out, src)
 
 
 def segments(canon: Canonical, layout: GroupLayout) -> Segments:
@@ -197,7 +205,8 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
         links[a:b, 0] = x.to(torch.int32)
         links[a:b, 1] = canon.w[a:b][order].contiguous().view(torch.int32)
         if extra is not None:
-            extra[a:b] = canon.extra[a:b][order]
+            for row_out, row in _payload_rows(extra, canon.extra):
+                row_out[a:b] = row[a:b][order]
         del k, order, t, x
         a = b
     del key

@@ -90,9 +90,9 @@ def oracle_run_edges(rt, st, ring_v, w, step0, n_steps, record=True):
     keeps both: Network.keep_canonical); updates st, ring_v and the canonical weights w in place."""
     from naer2_amd.network import PAYLOAD_ATTR, PAYLOAD_FLAGS
     c = rt.canon.to("cpu")
-    fl = c.extra[:, PAYLOAD_FLAGS].numpy()
+    fl = c.extra[PAYLOAD_FLAGS].numpy()
     kind, port = ((fl >> 8) & 3).astype(np.uint8), ((fl >> 10) & 1).astype(np.uint8)
-    attr = c.extra[:, PAYLOAD_ATTR].contiguous().view(torch.float32).numpy()
+    attr = c.extra[PAYLOAD_ATTR].contiguous().view(torch.float32).numpy()
     return oracle.run_edges(rt.pop_first, params_bytes(rt.params), rt.d_slots, rt.net.seed, rt.net.instance,
                             c.src_ptr.numpy(), c.tgt.numpy(), w, c.d.numpy(), kind, port, attr, st, ring_v, step0,
                             n_steps, record)
@@ -101,7 +101,7 @@ def oracle_run_edges(rt, st, ring_v, w, step0, n_steps, record=True):
 def canonical_weights(rt) -> np.ndarray:
     """The runtime's current link weights in canonical order."""
     from naer2_amd.network import PAYLOAD_INPUT
-    return rt.link_weights().cpu().numpy()[rt.canon.extra[:, PAYLOAD_INPUT].cpu().numpy()]
+    return rt.link_weights().cpu().numpy()[rt.canon.extra[PAYLOAD_INPUT].cpu().numpy()]
 
 
 def bits_equal(a: np.ndarray, b: np.ndarray) -> bool:

@@ -53,6 +53,14 @@ def test_ports_kinds_and_attributes_in_any_order():
     assert c1.gain[-1] == c1.shape[0] - 1
     c3.port = "potential"
     assert not c3.plain
+    # a connection's kind can be changed afterwards; the new kind's attribute is then unset
+    c3.kind = "plastic"
+    with pytest.raises(ValueError, match="factor not set"):
+        _ = c3.factor
+    c3.factor = 0.5
+    assert torch.equal(c3.factor, torch.full(c3.shape, 0.5)) and c3.second_attr is c3.factor
+    with pytest.raises(ValueError, match="edge kind"):
+        c3.kind = "stdp"
 
 
 def test_payload_carries_kind_port_attribute_and_input_place():
@@ -67,22 +75,22 @@ def test_payload_carries_kind_port_attribute_and_input_place():
     a = assemble(_net_of(A), "cpu")
     assert a.edges
     x = a.canon.extra
-    assert x.shape == (a.canon.n_links, 4)
+    assert x.shape == (4, a.canon.n_links)
     # each canonical link's input place names the link it came from
     src = torch.cat([c.src.long() + (0 if c.source is A else 40) for c in (c1, c2, c3)])
     tgt = torch.cat([c.tgt.long() + (0 if c.target is A else 40) for c in (c1, c2, c3)])
     w = torch.cat([c.weight for c in (c1, c2, c3)])
-    inp = x[:, PAYLOAD_INPUT].long()
+    inp = x[PAYLOAD_INPUT].long()
     assert torch.equal(torch.sort(inp).values, torch.arange(a.canon.n_links))
     c_src = torch.repeat_interleave(torch.arange(a.n), a.canon.out_degree())
     assert torch.equal(src[inp], c_src) and torch.equal(tgt[inp], a.canon.tgt.long()) and torch.equal(w[inp], a.canon.w)
     flags = {0: EDGE_KIND_BITS["gain"] | PORT_POTENTIAL_BIT, 1: EDGE_KIND_BITS["plastic"], 2: 0}
     attr = torch.cat([c1.gain, c2.factor, torch.zeros(c3.shape)])
     for ci in range(3):
-        m = x[:, PAYLOAD_CONN] == ci
+        m = x[PAYLOAD_CONN] == ci
         assert int(m.sum()) == (c1, c2, c3)[ci].shape[0]
-        assert (x[m, PAYLOAD_FLAGS] == flags[ci]).all()
-    assert torch.equal(x[:, PAYLOAD_ATTR].contiguous().view(torch.float32), attr[inp])
+        assert (x[PAYLOAD_FLAGS][m] == flags[ci]).all()
+    assert torch.equal(x[PAYLOAD_ATTR].contiguous().view(torch.float32), attr[inp])
     # a network of static current edges carries no payload
     P = LIFExp() * 10
     (P >> FixedInDegree(2) >> P).delay = 0.1
@@ -106,9 +114,9 @@ def _mixed(seed=1, kinds=("static", "gain", "plastic", "static"), ports=("curren
 
 def _edge_arrays(canon):
     x = canon.extra
-    fl = x[:, PAYLOAD_FLAGS].numpy()
+    fl = x[PAYLOAD_FLAGS].numpy()
     return (((fl >> 8) & 3).astype(np.uint8), ((fl >> 10) & 1).astype(np.uint8),
-            x[:, PAYLOAD_ATTR].contiguous().view(torch.float32).numpy())
+            x[PAYLOAD_ATTR].contiguous().view(torch.float32).numpy())
 
 
 def _state(a):
@@ -230,7 +238,7 @@ def test_oracle_potential_port_closed_form():
     assert abs(float(st.v[1]) - v) <= 2e-4, (float(st.v[1]), v)
     assert float(st.isyn[1]) == 0.0
     delivered = int((spikes <= n_steps - 2).sum())
-    assert w_after[list(a.canon.extra[:, PAYLOAD_CONN]).index(0)] == np.float32(0.3 * delivered)
+    assert w_after[list(a.canon.extra[PAYLOAD_CONN]).index(0)] == np.float32(0.3 * delivered)
     # an arrival one step late (or early) would miss by far more than the tolerance
     late = sum(0.3 * k * math.exp(-(n_steps - 1 - s - 4) * dt / tau) for k, s in enumerate(spikes, 1) if s + 4 <= n_steps - 1)
     on_time = sum(0.3 * k * math.exp(-(n_steps - 1 - s - 3) * dt / tau) for k, s in enumerate(spikes, 1) if s + 3 <= n_steps - 1)

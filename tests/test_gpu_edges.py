@@ -19,10 +19,14 @@ from tests.gpu_util import (assert_state_equal, bits_equal, canonical_weights, m
 
 pytestmark = pytest.mark.gpu
 
+# kinds and ports of E->E, E->I, I->E, I->I, and the largest delay (steps) of potential-port links (None: 8, as
+# every other link)
 CASES = {
-    "mixed": (("static", "gain", "plastic", "static"), ("current", "potential", "current", "potential")),
-    "plastic": (("plastic",) * 4, ("potential", "current", "potential", "current")),
-    "gain-potential": (("gain", "static", "gain", "plastic"), ("potential",) * 4),
+    "mixed": (("static", "gain", "plastic", "static"), ("current", "potential", "current", "potential"), None),
+    "plastic": (("plastic",) * 4, ("potential", "current", "potential", "current"), None),
+    "gain-potential": (("gain", "static", "gain", "plastic"), ("potential",) * 4, None),
+    "kinds-only": (("plastic", "gain", "static", "plastic"), ("current",) * 4, None),
+    "short-potential": (("static", "plastic", "gain", "static"), ("current", "potential", "current", "potential"), 3),
 }
 
 
@@ -32,8 +36,13 @@ def _say(msg):
 
 
 def _build(case, seed, device="cuda", sizes=(1600, 400), in_degree=(80, 20)):
-    kinds, ports = CASES[case]
+    kinds, ports, pot_max = CASES[case]
     net, conns = mixed_edges(seed, kinds, ports, sizes=sizes, in_degree=in_degree, device=device)
+    if pot_max is not None:
+        rng = np.random.default_rng(seed + 100)
+        for c in conns:
+            if c.port == "potential":
+                c.delay = torch.as_tensor(rng.integers(1, pot_max + 1, c.shape[0]) * 0.1, dtype=torch.float32)
     net.keep_canonical = True
     net.record = True
     return net, conns
@@ -47,8 +56,14 @@ def test_edge_plans_match_the_oracle(case, graph_window):
     net, _ = _build(case, seed=11)
     rt = net.build()
     assert rt.edges and rt.rows is None and rt.uniform_delay == 0
+    kinds, ports, pot_max = CASES[case]
+    want_v = 0 if "potential" not in ports else (pot_max or 8)
+    assert rt.v_slots == want_v and (rt.ring_v is None) == (want_v == 0)
+    if pot_max is not None:
+        assert rt.v_slots < rt.d_slots
     st = oracle_state(rt)
-    ring_v = rt.ring_v.cpu().numpy().copy()
+    ring_v = (rt.ring_v.cpu().numpy().copy() if rt.ring_v is not None
+              else np.zeros((rt.d_slots, rt.n), np.float32))
     w = canonical_weights(rt).copy()
     w0 = w.copy()
     net.run(T, graph_window=graph_window)
@@ -60,9 +75,12 @@ def test_edge_plans_match_the_oracle(case, graph_window):
     assert per_step.max() > 64 and ((per_step > 0) & (per_step <= 64)).any()  # both intakes
     assert np.array_equal(raster, raster_o)
     assert_state_equal(rt, st)
-    assert bits_equal(rt.ring_v.cpu().numpy(), ring_v)
+    if rt.ring_v is not None:
+        assert bits_equal(rt.ring_v.cpu().numpy(), ring_v)
+    else:
+        assert not ring_v.any()
     assert bits_equal(canonical_weights(rt), w)
-    if "plastic" in CASES[case][0]:
+    if "plastic" in kinds:
         assert not np.array_equal(w, w0)
 
 
@@ -82,7 +100,7 @@ def test_edge_plan_delivery_log_and_status_rows():
     raster = rt.raster(99, 159)
     c = rt.canon.to("cpu")
     ptr, tgt, d = c.src_ptr.numpy(), c.tgt.numpy(), c.d.numpy()
-    conn = c.extra[:, 0].numpy()
+    conn = c.extra[0].numpy()
     exp = []
     for i, s in enumerate(range(99, 159)):
         for j in np.nonzero(raster[i])[0]:
@@ -121,3 +139,38 @@ def test_edge_plans_step_alone():
         RateLossGrad([rt], np.ones((2, 2), np.float32))
     with pytest.raises(RuntimeError, match="compensated"):
         rt.trace_potentials(torch.empty(4, rt.n, device="cuda"), 0, torch.zeros(2, rt.n, device="cuda"))
+
+
+def test_full_scale_pd14_edges_deliver_the_plain_weights():
+    """Full-scale PD14 (298.8 M links) with every connection a gain edge of gain 1.0, then a plastic edge of
+    factor -0.0: both deliver exactly the plain weights, so 300 steps give the plain run's raster bit for
+    bit, and the weights read back in connection order are the connections' own (the payload's input places
+    at this size)."""
+    from naer2_amd.networks import pd14
+    T = 300
+    _say(f"case: full-scale pd14, plain vs gain 1.0 vs plastic -0.0 edges, {T} steps")
+    net, _ = pd14(scale=1.0, seed=0, device="cuda")
+    net.record = True
+    rt = net.build()
+    net.run(T, graph_window=100)
+    plain = rt.raster_bits(0, T).clone()
+    del rt, net
+    torch.cuda.empty_cache()
+    for kind, attr in (("gain", 1.0), ("plastic", -0.0)):
+        net, _ = pd14(scale=1.0, seed=0, device="cuda")
+        net.record = True
+        net.keep_canonical = False
+        for c in net.connections:
+            c.kind = kind
+            setattr(c, "gain" if kind == "gain" else "factor", attr)
+        rt = net.build()
+        assert rt.edges and rt.v_slots == 0 and rt.n_links == 298822986
+        w0 = torch.cat([c.weight for c in net.connections])
+        assert torch.equal(rt.link_weights().view(torch.int32), w0.view(torch.int32))
+        net.run(T, graph_window=100)
+        torch.cuda.synchronize()
+        assert torch.equal(rt.raster_bits(0, T), plain), kind
+        assert torch.equal(rt.link_weights().view(torch.int32), w0.view(torch.int32)), kind
+        _say(f"{kind}: raster and weights as the plain run's")
+        del rt, net, w0
+        torch.cuda.empty_cache()
