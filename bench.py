@@ -289,6 +289,8 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
     settle, full_n = 1000, int(round(1000.0 / rt.dt))
     if m["from_step"] >= settle and steps >= full_n:
         full, full_rates = m, timed_rates
+    elif args.no_full:
+        full = None
     else:
         if rt.step < settle:
             net.run(settle - rt.step, graph_window=0)
@@ -334,7 +336,7 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
         "events_per_step": round(m["events"] / steps, 1),
         "rates_hz": timed_rates,
         "roofline": _roofline(m, traffic, occupancy, kernel),
-        "full_config": {
+        "full_config": None if full is None else {
             "what": "1 s of bio time (10,000 steps) after 1,000 settling steps, graph windows of "
                     f"{full['graph_window']} steps",
             "rtf": round(full["rtf"], 4), "ms_per_step": round(full["ms_per_step"], 6),
@@ -802,6 +804,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="pd14: leave out the other configurations (`configs` in the line)")
+    ap.add_argument("--no-full", action="store_true",
+                    help="pd14/brunel: leave out the configuration's own 10,000-step run when the timed run is shorter "
+                         "(`full_config` null; for counter-collecting profiler runs, which slow every launch)")
     ap.add_argument("--budget-s", type=float, default=420.0,
                     help="pd14: wall seconds from start within which the other configurations are run")
     ap.add_argument("--timestep", action="append", metavar="POP=MS",
