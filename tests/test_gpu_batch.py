@@ -12,7 +12,7 @@ import torch
 from naer2_amd import FusedRound
 from naer2_amd.autotune import CANDIDATES, measure_work, tune, tune_intake
 from naer2_amd.networks import brunel_sweep, sweep_params
-from tests.gpu_util import bits_equal
+from tests.gpu_util import assert_state_equal, bits_equal, oracle_run, oracle_state
 
 pytestmark = pytest.mark.gpu
 
@@ -142,15 +142,15 @@ def test_batch_of_large_networks_equals_plain_runs():
 def test_batch_of_both_models_equals_plain_runs():
     """A heterogeneous round (naer2/fusion.py:61-116 over runtimes of different node types): delta-synapse
     Brunel networks and exponential-synapse PD14 microcircuits interleaved in one launch, each group running
-    its own network's model, give the same rasters, potentials, currents, refractory counters and rings as
-    each network alone, under two launch mappings."""
+    each network alone, under two launch mappings, and the first 96 batched steps of every network equal
+    the oracle's (delta-synapse and exponential-synapse steps alike)."""
     from naer2_amd._lib import MODEL_LIF_EXP
     from naer2_amd.networks import pd14
 
     params = sweep_params(n_instances=3, seed=5, n_min=500, n_max=4000)
 
     def build():
-        rts = [n.runtime for n in brunel_sweep(params, device="cuda", record=True)]
+        rts = [n.runtime for n in brunel_sweep(params, device="cuda", record=True, keep_canonical=True)]
         for i, scale in enumerate((0.02, 0.03)):
             net, _ = pd14(scale=scale, seed=7 + i, device="cuda", instance=i)
             net.record = True
@@ -160,6 +160,7 @@ def test_batch_of_both_models_equals_plain_runs():
     batched, plain = build(), build()
     assert {rt.model for rt in batched} == {MODEL_LIF_EXP, batched[0].model} and batched[0].model != MODEL_LIF_EXP
     _say(f"case: mixed batch, sizes {[rt.n for rt in batched]}")
+    states = [oracle_state(rt) for rt in batched]
     for cand in ({"order": 0, "xcd": 0}, {"order": 1, "xcd": 8}):
         _say(f"case: mapping {cand}")
         fr = FusedRound(batched, cand, reserve_steps=96)
@@ -176,4 +177,10 @@ def test_batch_of_both_models_equals_plain_runs():
             assert bits_equal(a.ring.cpu().numpy(), b.ring.cpu().numpy())
             if a.model == MODEL_LIF_EXP:
                 assert bits_equal(a.isyn.cpu().numpy(), b.isyn.cpu().numpy())
+        if states is not None:  # after the first mapping's 96 steps: every network against the oracle
+            for rt, st in zip(batched, states):
+                raster, _ = oracle_run(rt, st, 0, 96)
+                assert np.array_equal(rt.raster(0, 96), raster), f"network of {rt.n} neurons"
+                assert_state_equal(rt, st)
+            states = None
     assert all(int(rt.raster().sum()) > 0 for rt in plain)
