@@ -222,6 +222,11 @@ int n2h_rlif_backward_noalloc(const n2h_rlif* layer, const float* u, const float
  *   N2H_ROWS_WIDE    (per = 2): {source slot << 7 | (delay_steps - 1), fp32 weight bits}
  *   N2H_ROWS_COMPACT (per = 4): source slot << 15 | (delay_steps - 1) << 8 | weight index into wtab[256]
  *                               (at most 256 distinct weights)
+ *   N2H_ROWS_SLOT16  (per = 8): the 16-bit source slot alone, entry 2k in the low half of word k; every link
+ *                               has tables->uniform_delay, and the weight is wtab[0] for source slots below
+ *                               split, wtab[1] from it on, split = the bits of wtab[2] as uint32 (networks
+ *                               whose weights are set by two source populations, Brunel's; 64 n_groups
+ *                               <= 65535)
  * Entries past a target's in-links name the sentinel source slot 64 * n_groups (never spiking).
  * With tables->uniform_delay set, the walk is branch-free.
  * Steps that follow at least dense_min_spikes spikes walk every in-link against the previous step's spike
@@ -230,6 +235,7 @@ int n2h_rlif_backward_noalloc(const n2h_rlif* layer, const float* u, const float
  * Replaces: the reference's pull, fused.py:1626-1965 (here target-major, as receives.py names it). */
 #define N2H_ROWS_WIDE 0
 #define N2H_ROWS_COMPACT 1
+#define N2H_ROWS_SLOT16 2
 int n2h_bind_rows(n2h_plan* plan, const uint32_t* rows, const int64_t* row_base, const int32_t* row_depth,
                   int32_t format, const float* wtab, int32_t dense_min_spikes);
 

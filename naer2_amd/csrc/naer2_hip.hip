@@ -105,8 +105,8 @@ struct StepArgs {
     const uint4* rows;          // row intake (null: off): each target's in-links in contract order
     const int64_t* row_base;    // [n_groups]: first 16-byte cell of group g's block (cell q of lane l: + 64 q + l)
     const int32_t* row_depth;   // [n_groups]: cells per lane
-    const float* row_wtab;      // compact rows: the weight table [256]
-    int32_t row_format;         // N2H_ROWS_WIDE / N2H_ROWS_COMPACT
+    const float* row_wtab;      // compact rows: the weight table [256]; slot16: {w_lo, w_hi, split slot bits}
+    int32_t row_format;         // N2H_ROWS_WIDE / N2H_ROWS_COMPACT / N2H_ROWS_SLOT16
     int32_t row_uniform;        // every link's delay cell (delay - 1) when all delays are equal, else -1
     int32_t dense_min;          // row intake for steps following at least this many spikes
     int32_t acc_base;           // the delay cell held at acc[0]: row_uniform for uniform delays (one cell), else 0
@@ -372,9 +372,39 @@ __device__ __forceinline__ void walk_chunk(St& st, float* acc, int lane, int nl,
 //   either the weight or -0.0f, the exact identity of fp32 addition, into the register sum cr, so the sum
 //   sees exactly the spiking sources' weights in row order; `any` records whether there was one.
 //   Otherwise: each hit goes to its own delay cell through cell_add.
+// The slot16 format's weights: w_lo for source slots below `split`, w_hi from it on (fp32 bits).
+struct SlotWeights {
+    uint32_t split, w_lo, w_hi;
+};
+
 template <int PER, bool UNIFORM, class St>
 __device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, const uint4 (&c)[St::kRU], int& cb,
-                                          float& cr, uint32_t& any, unsigned long long& t0, unsigned long long& t1) {
+                                          float& cr, uint32_t& any, unsigned long long& t0, unsigned long long& t1,
+                                          const SlotWeights& sw) {
+    if constexpr (PER == 8) {
+        // slot16 (uniform delay only): a cell's eight 16-bit source slots, low half of each word first; the
+        // weight is the slot's side of the split; branch-free as the UNIFORM walk below
+        static_assert(UNIFORM, "slot16 rows hold no delay: every link has the uniform delay");
+        const uint32_t* bm32 = reinterpret_cast<const uint32_t*>(st.bm);
+#pragma unroll
+        for (int u = 0; u < St::kRU; ++u) {
+            const uint32_t cw[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+            uint32_t src[8], bw[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                src[e] = (cw[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+                bw[e] = bm32[src[e] >> 5];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bw[e]), src[e], 1u));
+                any |= m;
+                const uint32_t wb = src[e] >= sw.split ? sw.w_hi : sw.w_lo;
+                cr = cr + __uint_as_float((wb & m) | (0x80000000u & ~m));
+            }
+        }
+        return;
+    }
     constexpr int kHalf = St::kRU / 2;  // cells per LDS round (bounds the registers the decoded entries hold)
     constexpr int E = kHalf * PER;
     const uint32_t* bm32 = reinterpret_cast<const uint32_t*>(st.bm);
@@ -419,7 +449,7 @@ __device__ __forceinline__ void row_batch(const St& st, float* acc, int lane, co
 template <int PER, bool UNIFORM, class St, typename Ready>
 __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, const uint4* row, int depth, int b0,
                                          uint32_t sentinel, unsigned long long& t0, unsigned long long& t1,
-                                         Ready ready) {
+                                         Ready ready, const SlotWeights& sw = SlotWeights{0u, 0u, 0u}) {
     constexpr int RU = St::kRU;
     if (depth <= 0) {
         ready();
@@ -428,7 +458,7 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
     const int last = depth - 1;
     // a batch past the end of the block: its clamped loads repeat the last cell, so those cells are
     // replaced by sentinel entries (uniform branch, once per walk)
-    const uint32_t sx = (PER == 4) ? (sentinel << 15) : (sentinel << 7);
+    const uint32_t sx = (PER == 8) ? (sentinel | (sentinel << 16)) : (PER == 4) ? (sentinel << 15) : (sentinel << 7);
     auto trim = [&](uint4 (&c)[RU], int q_first) {
         if (q_first + RU > depth) {
 #pragma unroll
@@ -447,12 +477,12 @@ __device__ __forceinline__ void row_walk(const St& st, float* acc, int lane, con
 #pragma unroll
         for (int u = 0; u < RU; ++u) B[u] = row[static_cast<int64_t>(min(q0 + RU + u, last)) * kWave];
         trim(A, q0);
-        row_batch<PER, UNIFORM>(st, acc, lane, A, cb, cr, any, t0, t1);
+        row_batch<PER, UNIFORM>(st, acc, lane, A, cb, cr, any, t0, t1, sw);
         if (q0 + RU >= depth) break;
 #pragma unroll
         for (int u = 0; u < RU; ++u) A[u] = row[static_cast<int64_t>(min(q0 + 2 * RU + u, last)) * kWave];
         trim(B, q0 + RU);
-        row_batch<PER, UNIFORM>(st, acc, lane, B, cb, cr, any, t0, t1);
+        row_batch<PER, UNIFORM>(st, acc, lane, B, cb, cr, any, t0, t1, sw);
     }
     if (UNIFORM) {
         if (any) {  // the row intake is the step's only intake: the cell starts untouched
@@ -689,7 +719,11 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
             N2H_STAMP(2);
         };
         const uint32_t sentinel = static_cast<uint32_t>(kWave) * static_cast<uint32_t>(NG);
-        if (compact) {
+        if (a.row_format == N2H_ROWS_SLOT16) {  // uniform delay (checked when bound)
+            const SlotWeights sw{__float_as_uint(a.row_wtab[2]), __float_as_uint(a.row_wtab[0]),
+                                 __float_as_uint(a.row_wtab[1])};
+            row_walk<8, true>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready, sw);
+        } else if (compact) {
             if (b0 >= 0) row_walk<4, true>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
             else row_walk<4, false>(st, acc, lane, row, depth, b0, sentinel, t0, t1, ready);
         } else {
@@ -1630,8 +1664,14 @@ int n2h_bind_rows(n2h_plan* p, const uint32_t* rows, const int64_t* row_base, co
     if (rows && (!row_base || !row_depth)) return fail(N2H_EINVAL, "null row table");
     if (rows && p->tables.n_groups > kDenseWords - 1)
         return fail(N2H_EINVAL, "row intake holds the spike bitmap in LDS: at most %d groups", kDenseWords - 1);
-    if (rows && format != N2H_ROWS_WIDE && format != N2H_ROWS_COMPACT)
+    if (rows && format != N2H_ROWS_WIDE && format != N2H_ROWS_COMPACT && format != N2H_ROWS_SLOT16)
         return fail(N2H_EINVAL, "unknown row format %d", format);
+    if (rows && format == N2H_ROWS_SLOT16) {
+        if (!wtab) return fail(N2H_EINVAL, "slot16 rows need their weights and split (wtab[3])");
+        if (p->tables.uniform_delay <= 0) return fail(N2H_EINVAL, "slot16 rows need one delay for every link");
+        if (static_cast<int64_t>(kWave) * p->tables.n_groups > 0xffff)
+            return fail(N2H_EINVAL, "slot16 rows name at most 65,535 source slots (the sentinel included)");
+    }
     if (rows && format == N2H_ROWS_COMPACT && !wtab) return fail(N2H_EINVAL, "compact rows need a weight table");
     if (rows && format == N2H_ROWS_COMPACT && static_cast<int64_t>(kWave) * p->tables.n_groups > (1 << 17))
         return fail(N2H_EINVAL, "compact rows name at most 2^17 source slots");

@@ -43,7 +43,7 @@ class Network:
         # row intake (target-major pull over every in-link, for steps after many spikes): "auto" builds its
         # table where the spike bitmap fits in LDS (<= 1535 groups); dense_min_spikes None: n_neurons // 20
         self.row_intake = "auto"
-        self.row_format = "auto"          # "auto" (compact where it fits), "wide" or "compact"
+        self.row_format = "auto"          # "auto" (slot16, else compact where it fits), "wide", "compact", "slot16"
         self.dense_min_spikes: int | None = None
         # keep the connections' per-link tensors after the build (False: release them -- the runtime holds
         # its own tables; connection weights and delays can then no longer be read or changed)

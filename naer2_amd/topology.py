@@ -213,7 +213,7 @@ def segments(canon: Canonical, layout: GroupLayout) -> Segments:
     return Segments(seg_off, links, extra)
 
 
-ROWS_WIDE, ROWS_COMPACT = 0, 1
+ROWS_WIDE, ROWS_COMPACT, ROWS_SLOT16 = 0, 1, 2
 COMPACT_WEIGHTS = 256      # distinct weights a compact row table can index
 COMPACT_SLOTS = 1 << 17    # source slots a compact entry can name
 
@@ -229,6 +229,11 @@ class Rows:
       compact (per = 4): one int32 word  source slot << 15 | (delay - 1) << 8 | weight index, the weight
                          being wtab[index] -- for networks with at most 256 distinct weights and
                          2^17 source slots (Brunel-type networks: half the bytes).
+      slot16  (per = 8): the 16-bit source slot alone (entry 2k in the low half of word k), for networks
+                         with one delay for every link whose weights are set by the source's side of one
+                         slot split (two source populations: Brunel-type networks) and 64 G <= 65535; wtab
+                         holds {w_lo, w_hi, split slot bits}.  Half the compact entry: the row intake of a
+                         bursting network streams half the bytes.
     Entries past a target's in-links name the sentinel source slot 64 G, which never spikes (the kernel
     keeps a zero bitmap word for it), so the walk needs no bounds test.
     """
@@ -236,7 +241,7 @@ class Rows:
     row_base: torch.Tensor   # int64 [G]: first cell of each group
     row_len: torch.Tensor    # int32 [64 G]: in-links per target slot
     row_depth: torch.Tensor  # int32 [G]: cells per lane
-    fmt: int                 # ROWS_WIDE or ROWS_COMPACT
+    fmt: int                 # ROWS_WIDE, ROWS_COMPACT or ROWS_SLOT16
     wtab: torch.Tensor | None  # float32 [COMPACT_WEIGHTS] (compact)
     uniform_delay: int       # the delay in steps of every link when all are equal, else 0
 
@@ -246,10 +251,39 @@ def _as_u32_bits(v: torch.Tensor) -> torch.Tensor:
     return torch.where(v >= 2 ** 31, v - 2 ** 32, v).to(torch.int32)
 
 
+def _slot_split(canon: Canonical, layout: GroupLayout):
+    """(split slot, w_lo bits, w_hi bits) when every link's weight is set by its source slot's side of one
+    split (w_lo below, w_hi from it on) and every link has one delay, else None."""
+    E = canon.n_links
+    if E == 0 or not bool((canon.d == canon.d[0]).all()):
+        return None
+    slot = torch.repeat_interleave(slot_of(layout).to(canon.tgt.device), canon.out_degree())
+    wbits = canon.w.contiguous().view(torch.int32)
+    uniq = torch.unique(wbits)
+    if uniq.numel() > 2:
+        return None
+    first = int(torch.argmin(slot))
+    w_lo = int(wbits[first])
+    other = wbits != w_lo
+    if not bool(other.any()):
+        return 0, w_lo, w_lo
+    split = int(slot[other].min())
+    if int(slot[~other].max()) >= split:
+        return None
+    return split, w_lo, int(uniq[uniq != w_lo][0])
+
+
 def rows(canon: Canonical, layout: GroupLayout, fmt: str = "auto") -> Rows:
-    """fmt: "auto" (compact where it fits), "wide" or "compact"."""
+    """fmt: "auto" (slot16 where it applies, else compact where it fits), "wide", "compact" or "slot16"."""
     dev = canon.tgt.device
     n, G = canon.n, layout.n_groups
+    if fmt in ("auto", "slot16"):
+        sp = _slot_split(canon, layout) if GROUP * G <= 0xffff else None
+        if sp is not None:
+            return _rows_slot16(canon, layout, *sp)
+        if fmt == "slot16":
+            raise ValueError("slot16 rows need one delay for every link, weights set by one source-slot split "
+                             "and at most 65,535 source slots")
     wbits = canon.w.contiguous().view(torch.int32)
     uniq = None
     if fmt in ("auto", "compact") and GROUP * G <= COMPACT_SLOTS:
@@ -306,3 +340,39 @@ def rows(canon: Canonical, layout: GroupLayout, fmt: str = "auto") -> Rows:
         uniform = int(canon.d[0])
     return Rows(out, row_base, row_len.to(torch.int32), depth.to(torch.int32),
                 ROWS_COMPACT if compact else ROWS_WIDE, wtab, uniform)
+
+
+def _rows_slot16(canon: Canonical, layout: GroupLayout, split: int, w_lo: int, w_hi: int) -> Rows:
+    """The slot16 row table (see Rows): eight 16-bit source slots a cell, in contract order per target."""
+    dev = canon.tgt.device
+    n, G = canon.n, layout.n_groups
+    per = 8
+    slot = slot_of(layout).to(dev)
+    group_of = layout.group_of.to(dev)
+    local_of = layout.local_of.to(dev)
+    tg = canon.tgt.to(torch.int64)
+    E = int(tg.numel())
+    deg = torch.bincount(tg, minlength=n)
+    row_len = torch.zeros(GROUP * G, dtype=torch.int64, device=dev)
+    row_len[slot] = deg
+    depth = (row_len.view(G, GROUP).max(dim=1).values + per - 1) // per
+    row_base = torch.zeros(G, dtype=torch.int64, device=dev)
+    if G > 1:
+        row_base[1:] = torch.cumsum(depth * GROUP, 0)[:-1]
+    cells = int((depth * GROUP).sum())
+    order = torch.argsort(tg, stable=True)  # by target, canonical order within each target
+    t_sorted = tg[order]
+    row_start = torch.cumsum(deg, 0) - deg
+    rank = torch.arange(E, device=dev) - row_start[t_sorted]
+    cell = row_base[group_of[t_sorted]] + (rank // per) * GROUP + local_of[t_sorted]
+    half = cell * per + rank % per  # the entry's 16-bit position: entry 2k is the low half of word k
+    del rank, row_start, cell, t_sorted
+    src = torch.repeat_interleave(slot, canon.out_degree())[order]
+    sentinel = GROUP * G
+    out16 = torch.full((cells * per,), sentinel - 65536 if sentinel >= 32768 else sentinel, dtype=torch.int16,
+                       device=dev)
+    out16[half] = torch.where(src >= 32768, src - 65536, src).to(torch.int16)
+    del src, order, half
+    wtab = torch.tensor([w_lo, w_hi, split], dtype=torch.int32, device=dev).view(torch.float32)
+    return Rows(out16.view(torch.int32), row_base, row_len.to(torch.int32), depth.to(torch.int32), ROWS_SLOT16,
+                wtab, int(canon.d[0]))

@@ -75,20 +75,25 @@ def test_forced_spikes_deliver_like_the_oracle(n, n_links, max_delay, p_spike):
 
 
 @pytest.mark.parametrize("dense_min,fmt,uniform", [(0, "wide", False), (10**9, "wide", False), (0, "compact", False),
-                                                   (0, "wide", True), (0, "compact", True)])
+                                                   (0, "wide", True), (0, "compact", True), (0, "slot16", True),
+                                                   (10**9, "slot16", True)])
 def test_row_intake_and_segment_intake_agree_with_the_oracle(dense_min, fmt, uniform):
-    """The same forced spike patterns through the row intake (every step; wide and compact row entries,
-    mixed delays and one delay for all links) and the segment intake (never)."""
+    """The same forced spike patterns through the row intake (every step; wide, compact and slot16 row
+    entries, mixed delays and one delay for all links) and the segment intake (never)."""
     n, n_links, max_delay = 700, 30000, 12
     _say(f"case: row intake dense_min={dense_min} fmt={fmt} uniform={uniform} n={n}")
     src_ptr, tgt, w, d = tiny_topology(n, n_links, max_delay, seed=17)
     if fmt == "compact":  # 200 distinct weights of varied magnitude and sign, -0.0 among them
         w = np.concatenate([w[:199], np.array([-0.0], np.float32)])[np.random.default_rng(5).integers(0, 200, n_links)]
+    if fmt == "slot16":  # the weight set by the source's side of slot 384 (one population: slot = neuron)
+        w = np.where(np.repeat(np.arange(n), np.diff(src_ptr)) < 384, np.float32(0.2), np.float32(-1.0))
     if uniform:
         d = np.full_like(d, 7)
     net, pop = network_from_topology(ProbeNode(), n, (src_ptr, tgt, w, d))
+    if fmt != "slot16":
+        net.row_format = fmt
     rt = net.build()
-    assert rt.rows is not None and rt.rows.fmt == (1 if fmt == "compact" else 0)
+    assert rt.rows is not None and rt.rows.fmt == {"wide": 0, "compact": 1, "slot16": 2}[fmt]
     assert rt.rows.uniform_delay == (7 if uniform else 0)
     rt.set_dense_min(dense_min)
     st = oracle_state(rt)
@@ -111,6 +116,7 @@ def test_brunel_bursts_row_intake_bit_exact():
     net, _ = brunel(N_E=4000, N_I=1000, C_E=400, C_I=100, g=3.0, eta=3.0, device="cuda", seed=4)
     net.dense_min_spikes = 200
     rt = net.build()
+    assert rt.rows.fmt == 2  # Brunel: one delay, weights set by the source population (slot16 rows)
     st = oracle_state(rt)
     net.run(600)
     torch.cuda.synchronize()

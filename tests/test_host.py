@@ -17,7 +17,8 @@ from naer2_amd import _lib
 from naer2_amd.network import assemble, bits_to_raster, raster_to_bits
 from naer2_amd.networks import PD14, brunel, pd14
 from naer2_amd.nodes import FixedInDegree, FixedTotalNumber
-from naer2_amd.topology import ROWS_COMPACT, ROWS_WIDE, canonical, delay_steps, group_layout, rows, segments, slot_of
+from naer2_amd.topology import (ROWS_COMPACT, ROWS_SLOT16, ROWS_WIDE, canonical, delay_steps, group_layout, rows,
+                                segments, slot_of)
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -243,8 +244,68 @@ def test_rows_flag_a_uniform_delay():
     tgt = torch.from_numpy(rng.integers(0, n, 900))
     w = torch.full((900,), 0.25)
     d = torch.full((900,), 15, dtype=torch.int32)
-    r = rows(canonical(n, src, tgt, w, d), group_layout([n]))
+    can = canonical(n, src, tgt, w, d)
+    r = rows(can, group_layout([n]), fmt="compact")
     assert r.uniform_delay == 15 and r.fmt == ROWS_COMPACT
+    r = rows(can, group_layout([n]))  # one weight, one delay: the 16-bit format, split 0
+    assert r.uniform_delay == 15 and r.fmt == ROWS_SLOT16
+    assert r.wtab.numpy().view(np.uint32).tolist() == [np.float32(0.25).view(np.uint32)] * 2 + [0]
+
+
+def _brunel_like(seed, n_e=150, n_i=40, m=6000):
+    """Links from E (weight 0.2) and I (weight -1.0) sources, one delay of 15 steps."""
+    rng = np.random.default_rng(seed)
+    n = n_e + n_i
+    src = rng.integers(0, n, m)
+    tgt = rng.integers(0, n, m)
+    w = np.where(src < n_e, np.float32(0.2), np.float32(-1.0)).astype(np.float32)
+    return n, torch.from_numpy(src), torch.from_numpy(tgt), torch.from_numpy(w), torch.full((m,), 15, dtype=torch.int32)
+
+
+def test_slot16_rows_hold_each_targets_source_slots_in_canonical_order():
+    n, src, tgt, w, d = _brunel_like(5)
+    can = canonical(n, src, tgt, w, d)
+    lay = group_layout([150, 40])
+    r = rows(can, lay)
+    assert r.fmt == ROWS_SLOT16 and r.uniform_delay == 15
+    split = 64 * 3  # the I population's first group
+    wt = r.wtab.numpy().view(np.uint32)
+    assert wt.tolist() == [np.float32(0.2).view(np.uint32), np.float32(-1.0).view(np.uint32), split]
+    slot = (lay.group_of * 64 + lay.local_of).numpy()
+    src_of = np.repeat(np.arange(n), np.diff(can.src_ptr.numpy()))
+    half = r.rows.numpy().view(np.uint16)  # little-endian: entry 2k is the low half of word k
+    sentinel = 64 * lay.n_groups
+    for i in range(n):
+        g, lane = int(lay.group_of[i]), int(lay.local_of[i])
+        expect = np.nonzero(can.tgt.numpy() == i)[0]
+        assert int(r.row_len[g * 64 + lane]) == len(expect)
+        got = [int(half[(int(r.row_base[g]) + 64 * (k // 8) + lane) * 8 + k % 8]) for k in range(int(r.row_depth[g]) * 8)]
+        assert got[:len(expect)] == [int(slot[src_of[l]]) for l in expect]
+        assert all(x == sentinel for x in got[len(expect):])
+        # the weight the walk takes for each entry is the link's
+        for x, l in zip(got, expect):
+            assert (wt[1] if x >= wt[2] else wt[0]) == can.w[l:l + 1].numpy().view(np.uint32)[0]
+
+
+def test_slot16_rows_only_where_they_apply():
+    n, src, tgt, w, d = _brunel_like(6)
+    lay = group_layout([150, 40])
+    bad_w = w.clone()
+    bad_w[0] = 0.3  # a third weight
+    mixed = w.clone()
+    mixed[src == 0] = -1.0  # an E source with the I weight: no single split
+    bad_d = d.clone()
+    bad_d[0] = 3  # a second delay
+    for ww, dd in ((bad_w, d), (mixed, d), (w, bad_d)):
+        can = canonical(n, src, tgt, ww, dd)
+        assert rows(can, lay).fmt == ROWS_COMPACT
+        with pytest.raises(ValueError, match="slot16"):
+            rows(can, lay, fmt="slot16")
+    # the sentinel slot must fit 16 bits: 1,024 groups are too many
+    big = group_layout([64 * 1024])
+    m = 2000
+    can = canonical(64 * 1024, torch.arange(m), torch.arange(m), torch.full((m,), 0.5), torch.full((m,), 2, dtype=torch.int32))
+    assert rows(can, big).fmt == ROWS_COMPACT
 
 
 def test_tables_built_in_chunks_equal_whole(monkeypatch):
