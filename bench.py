@@ -458,7 +458,7 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
     log(f"pd14grad: built {n_inst} PD14 instances ({n_syn} synapses) in {time.perf_counter() - t0:.0f} s")
     sur = {"cutoff": args.cutoff, "gamma": args.gamma}
     grad = RateLossGrad(rts, alpha, checkpoint=K, long_delay=args.long_delay, long_parts=args.long_parts,
-                        pure_ranges=args.pure_ranges, **sur)
+                        pure_ranges=args.pure_ranges, overlap=not args.serial_backward, **sur)
     t1 = time.perf_counter()
     loss, g = grad(T, targets)
     wall = time.perf_counter() - t1
@@ -498,6 +498,7 @@ def measure_pd14grad(args, steps: int, instances: int) -> dict:
         "loss": loss, "grad_norm": float(np.linalg.norm(g)), "grad_finite": bool(np.isfinite(g).all()),
         "surrogate": {"kappa": 25.0, **sur}, "long_delay": args.long_delay, "long_parts": args.long_parts,
         "pure_ranges": bool(args.pure_ranges),
+        "recompute_overlapped": not args.serial_backward,
         "grad_rel_err_vs_fp64_subsample": err,
         "subsample": f"4 neurons per population (32), PD14 probabilities x10, {Tc} steps, same spike trains",
     }
@@ -836,6 +837,9 @@ def main() -> int:
     ap.add_argument("--pure-ranges", action="store_true",
                     help="pd14grad: population-pure target ranges (each population's targets split apart) instead of "
                          "equal ones across populations")
+    ap.add_argument("--serial-backward", action="store_true",
+                    help="pd14grad: recompute each checkpoint segment before its reverse steps on one stream, instead "
+                         "of recomputing the next segment on a second stream while a segment is reversed")
     ap.add_argument("--cutoff", type=float, default=0.2,
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,

@@ -174,6 +174,13 @@ int n2h_graph_destroy(n2h_graph* graph);
 int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* tuning, n2h_batch** out);
 int n2h_batch_destroy(n2h_batch* batch);
 int n2h_batch_step(n2h_batch* batch, int32_t n_steps, void* stream);
+/* Bring a batch in line with its plans again (no counterpart in the reference: FusedRound is rebuilt there):
+ * the step they are at, all equal, after n2h_set_step moved them (a restored checkpoint), and every plan's
+ * launch arguments as n2h_batch_create copied them, so that bindings changed since (n2h_set_vtrace, the ring
+ * or the state moved) take effect.  Enqueued on `stream` as one small kernel per 8 plans; allocates nothing.
+ * A plan bound to a delivery log or to edges since is refused, as at creation.  Captured graphs of the
+ * batch replay with the new arguments but keep the step phase they were captured at. */
+int n2h_batch_sync(n2h_batch* batch, void* stream);
 int n2h_batch_graph_create(n2h_batch* batch, int32_t n_steps, void* stream, n2h_graph** out);
 
 /* Dense recurrent LIF layer for surrogate-gradient BPTT (no counterpart in the reference, which has no

@@ -26,7 +26,7 @@ DENSE_GROUPS = 1535  # row intake: spike bitmaps of at most this many groups fit
 EXPORTS = (
     "n2h_version", "n2h_last_error", "n2h_plan_create", "n2h_plan_destroy", "n2h_bind_state",
     "n2h_bind_ring", "n2h_set_step", "n2h_get_step", "n2h_step", "n2h_graph_create", "n2h_graph_launch",
-    "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step",
+    "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step", "n2h_batch_sync",
     "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_vtrace", "n2h_set_trace", "n2h_adj_last_error", "n2h_adj_step", "n2h_adj_window", "n2h_adj_work_words", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward", "n2h_rlif_delivery_adjoint", "n2h_rlif_work_floats",
     "n2h_rlif_backward_noalloc", "n2h_bind_log", "n2h_set_status", "n2h_bind_edges",
@@ -133,6 +133,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
         "n2h_set_vtrace": ([vp, vp, i64, i32, vp, vp], i32),
         "n2h_batch_step": ([vp, i32, vp], i32),
+    "n2h_batch_sync": ([vp, vp], i32),
         "n2h_batch_graph_create": ([vp, i32, vp, ctypes.POINTER(vp)], i32),
         "n2h_ev_last_error": ([ctypes.c_char_p, ctypes.c_size_t], i32),
         "n2h_ev_create": ([ctypes.POINTER(EvDesc), ctypes.POINTER(vp)], i32),

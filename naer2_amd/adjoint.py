@@ -143,7 +143,7 @@ def out_links(rt, alpha: torch.Tensor, long_delay: int = 1, long_parts: int = 1,
 class RateLossGrad:
     def __init__(self, runtimes: list, alpha, checkpoint: int = 500, kappa: float = 25.0,
                  cutoff: float = float("inf"), gamma: float = 1.0, graph_window: int = 100, tuning=None,
-                 long_delay: int = 4, long_parts: int = 24, pure_ranges: bool = False):
+                 long_delay: int = 4, long_parts: int = 24, pure_ranges: bool = False, overlap: bool = True):
         if not runtimes or len(runtimes) > 16:
             raise ValueError("RateLossGrad: 1 to 16 networks")
         for rt in runtimes:
@@ -178,6 +178,7 @@ class RateLossGrad:
         self.sur = Surrogate(float(kappa), float(cutoff), float(gamma), 0.0)
         self.long_delay = int(long_delay)
         self.long_parts = int(long_parts)
+        self.overlap = bool(overlap)
         self.tables = [out_links(rt, self.alpha, self.long_delay, self.long_parts, pure_ranges) for rt in runtimes]
         self.pop_first = [torch.as_tensor(np.asarray(rt.pop_first, dtype=np.int32), device=self.dev) for rt in runtimes]
         self.ring_slots = max(rt.d_slots for rt in runtimes) + 1
@@ -244,8 +245,6 @@ class RateLossGrad:
                 "lb": torch.zeros(W, rt.n, 8, device=self.dev), "gr": torch.zeros(W, rt.n, device=self.dev),
                 "gi": torch.zeros(rt.n, device=self.dev),
                 "gsum": torch.zeros(rt.n, self.P, 2, device=self.dev),  # compensated: hi, lo
-                "vtr": torch.empty(min(K, T), rt.n, device=self.dev),
-                "spk": torch.empty(min(K, T), rt.n, dtype=torch.uint8, device=self.dev),
             })
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         nets = (AdjNet * len(rts))()
@@ -269,27 +268,53 @@ class RateLossGrad:
             marks.append((phase, e))
 
         mark("setup")
-        for k in reversed(range(len(ckpts))):
-            start = s0 + k * K
-            L = min(K, T - k * K)
-            for rt, (ck, lk), sd, l in zip(rts, ckpts[k], st, lo):
-                rt.restore(ck)
-                l.copy_(lk)
-                _lib.check(lib.n2h_set_vtrace(rt._plan, sd["vtr"].data_ptr(), start, L, l[0].data_ptr(),
-                                              l[1].data_ptr()), "n2h_set_vtrace")
-            fr = FusedRound(rts, self.tuning, reserve_steps=L)
-            fr.launch(L, 0)
-            fr.close()
-            mark("recompute")
-            for rt, sd in zip(rts, st):
-                _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
-                gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
-                sd["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
-            mark("spikes")
+        # The recompute of segment k - 1 (its checkpoint restored, its steps traced, its spikes extracted) runs
+        # on a stream of its own while segment k's reverse steps run on this one: the two touch disjoint
+        # buffers (the runtimes' state against the adjoint's), and two trace/spike buffers alternate between
+        # segments.  One batch recomputes every segment: n2h_batch_sync brings its step and its copy of the
+        # plans' trace bindings in line after each restore, on the recompute's stream.
+        rev_stream = torch.cuda.current_stream(self.dev)
+        fr_rec = FusedRound(rts, self.tuning)
+        rec_stream = torch.cuda.Stream(device=self.dev) if self.overlap else rev_stream
+        rec_stream.wait_stream(rev_stream)
+        L0 = min(K, T)
+        bufs = [[{"vtr": torch.empty(L0, rt.n, device=self.dev),
+                  "spk": torch.empty(L0, rt.n, dtype=torch.uint8, device=self.dev)} for rt in rts] for _ in range(2)]
+        n_seg = len(ckpts)
+        ev_rec, ev_rev, t_rec, t_rev = [None] * n_seg, [None] * n_seg, [], []
+
+        def timed_event(stream):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(stream)
+            return e
+
+        def recompute(k):
+            start, L = s0 + k * K, min(K, T - k * K)
+            with torch.cuda.stream(rec_stream):
+                e0 = timed_event(rec_stream)
+                for rt, (ck, lk), b, l in zip(rts, ckpts[k], bufs[k % 2], lo):
+                    rt.restore(ck)
+                    l.copy_(lk)
+                    _lib.check(lib.n2h_set_vtrace(rt._plan, b["vtr"].data_ptr(), start, L, l[0].data_ptr(),
+                                                  l[1].data_ptr()), "n2h_set_vtrace")
+                fr_rec.sync()
+                fr_rec.launch(L, 0)
+                for rt, b in zip(rts, bufs[k % 2]):
+                    _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
+                    gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
+                    b["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
+                ev_rec[k] = timed_event(rec_stream)
+            t_rec.append((e0, ev_rec[k]))
+
+        def reverse(k):
+            start, L = s0 + k * K, min(K, T - k * K)
+            rev_stream.wait_event(ev_rec[k])
+            e0 = timed_event(rev_stream)
             # windows of W steps from the segment's end: [t0, t1)
             # row addresses by arithmetic on the base pointers (a tensor view per row costs microseconds of host
             # time a step, as much as the step's kernels): vtrace fp32 [K][n], spikes u8 [K][n], long_bins [W][n][8]
-            base = [(sd["vtr"].data_ptr(), sd["spk"].data_ptr(), sd["lb"].data_ptr(), rt.n) for sd, rt in zip(st, rts)]
+            base = [(b["vtr"].data_ptr(), b["spk"].data_ptr(), sd["lb"].data_ptr(), rt.n)
+                    for b, sd, rt in zip(bufs[k % 2], st, rts)]
             for t1 in range(L, 0, -W):
                 t0 = max(0, t1 - W)
                 for i, (vb, sb, lb, n) in enumerate(base):
@@ -299,8 +324,19 @@ class RateLossGrad:
                     for i, (vb, sb, lb, n) in enumerate(base):
                         nets[i].vtrace, nets[i].spikes, nets[i].long_bins = vb + 4 * t * n, sb + t * n, lb + 32 * (t - t0) * n
                     _check(lib.n2h_adj_step(nets, len(rts), start + t, t - t0, sur, wp, wn, stream), "n2h_adj_step")
-            mark("reverse")
+            ev_rev[k] = timed_event(rev_stream)
+            t_rev.append((e0, ev_rev[k]))
+
+        recompute(n_seg - 1)
+        for k in reversed(range(n_seg)):
+            if k >= 1:
+                if k + 1 < n_seg:
+                    rec_stream.wait_event(ev_rev[k + 1])  # segment k + 1's buffers become segment k - 1's
+                recompute(k - 1)
+            reverse(k)
+        rev_stream.wait_stream(rec_stream)
         t_bw[1].record()
+        fr_rec.close()
         for rt, sn, rf in zip(rts, final, rec_from):
             rt.restore(sn)
             rt.record_from = rf  # the history still holds every step since (the final snapshot's bits)
@@ -313,10 +349,10 @@ class RateLossGrad:
                 blk = sd["gsum"][pf[b]:pf[b + 1]]
                 g[:, b] += blk[:, :, 0].sum(dim=0) + blk[:, :, 1].sum(dim=0)
         torch.cuda.synchronize(self.dev)
-        phases = {}
-        for (_, a), (ph, b) in zip(marks[:-1], marks[1:]):
-            phases[ph] = phases.get(ph, 0.0) + a.elapsed_time(b)
-        phases["setup"] = t_bw[0].elapsed_time(marks[0][1])
+        phases = {"setup": t_bw[0].elapsed_time(marks[0][1]),
+                  "recompute": sum(a.elapsed_time(b) for a, b in t_rec),   # on its own stream when overlapped
+                  "reverse": sum(a.elapsed_time(b) for a, b in t_rev),
+                  "first_recompute": t_rec[0][0].elapsed_time(t_rec[0][1])}  # the one nothing overlaps
         self.last = {"forward_ms": t_fw[0].elapsed_time(t_fw[1]), "backward_ms": t_bw[0].elapsed_time(t_bw[1]),
                      "backward_phases_ms": phases, "c_pops": c_pops}
         return loss, g.cpu().numpy().astype(np.float64)

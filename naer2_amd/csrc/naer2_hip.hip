@@ -1078,6 +1078,23 @@ __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
 }
 
 __global__ void advance_kernel(int64_t* ctr, int32_t n) { *ctr += n; }
+
+// n2h_batch_sync: up to kSyncChunk plans' launch arguments by value (kernel arguments, so nothing on the host
+// has to outlive the call) written over the batch's device copy, and the batch's step counter.
+constexpr int kSyncChunk = 8;
+struct SyncChunk {
+    StepArgs args[kSyncChunk];
+    int64_t* step_ctr;  // null: leave it
+    int64_t step;
+    int32_t first;
+    int32_t n;
+};
+static_assert(sizeof(SyncChunk) <= 4000, "kernel arguments are limited to 4 KB");
+__global__ void batch_sync_kernel(StepArgs* dst, SyncChunk c) {
+    const int i = static_cast<int>(threadIdx.x);
+    if (i < c.n) dst[c.first + i] = c.args[i];
+    if (i == 0 && c.step_ctr) *c.step_ctr = c.step;
+}
 __global__ void advance_many_kernel(int64_t* const* ctrs, int32_t n_ctrs, int32_t n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_ctrs) *ctrs[i] += n;
@@ -1611,6 +1628,35 @@ int n2h_batch_destroy(n2h_batch* bt) {
     if (bt->step_ctr) (void)hipFree(bt->step_ctr);
     delete[] bt->plans;
     delete bt;
+    return N2H_OK;
+}
+
+int n2h_batch_sync(n2h_batch* bt, void* stream) {
+    if (!bt) return fail(N2H_EINVAL, "null batch");
+    const int64_t at = bt->plans[0]->host_step;
+    size_t lds = 0;
+    for (int32_t i = 0; i < bt->n_plans; ++i) {
+        const n2h_plan* p = bt->plans[i];
+        if (p->host_step != at) return fail(N2H_ESTATE, "plan %d is at step %lld, plan 0 at %lld", i,
+                                            static_cast<long long>(p->host_step), static_cast<long long>(at));
+        if (p->log_rec) return fail(N2H_EINVAL, "plan %d logs its deliveries: a logging plan steps alone", i);
+        if (p->edges) return fail(N2H_EINVAL, "plan %d has edge kinds or ports: it steps alone", i);
+        if (p->lds_bytes > lds) lds = p->lds_bytes;
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int32_t i0 = 0; i0 < bt->n_plans; i0 += kSyncChunk) {
+        SyncChunk c;
+        memset(&c, 0, sizeof(c));
+        c.first = i0;
+        c.n = bt->n_plans - i0 < kSyncChunk ? bt->n_plans - i0 : kSyncChunk;
+        for (int32_t j = 0; j < c.n; ++j) c.args[j] = make_args(bt->plans[i0 + j]);
+        c.step_ctr = i0 == 0 ? bt->step_ctr : nullptr;
+        c.step = at;
+        hipLaunchKernelGGL(batch_sync_kernel, dim3(1), dim3(kSyncChunk), 0, st, bt->args_dev, c);
+    }
+    N2H_HIP(hipGetLastError());
+    bt->host_step = at;
+    bt->lds_bytes = lds;
     return N2H_OK;
 }
 

@@ -122,6 +122,16 @@ class FusedRound:
             self._advance(n_steps - done)
         self.launches += 1
 
+    def sync(self) -> None:
+        """Follow the runtimes again after they were moved (restore, set_step) or rebound (trace_potentials):
+        the batch's step and every plan's launch arguments, refreshed on the current stream (n2h_batch_sync)."""
+        steps = {rt.step for rt in self.runtimes}
+        if len(steps) != 1:
+            raise ValueError(f"FusedRound: the runtimes are at different steps ({sorted(steps)})")
+        lib = _lib.load()
+        sv = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check(lib.n2h_batch_sync(self._batch, sv), "n2h_batch_sync")
+
     def _advance(self, n: int) -> None:
         for rt in self.runtimes:
             rt.step += n

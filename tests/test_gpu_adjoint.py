@@ -96,12 +96,14 @@ def test_gradient_is_independent_of_checkpoints_and_batching():
     T = 160
     alpha = _alpha(4)
     targets = np.random.default_rng(1).uniform(0.5, 10.0, (2, 8))
-    _say(f"case: checkpoint 32 vs 160, batched vs alone, T={T}")
+    _say(f"case: checkpoint 32 vs 160, recompute overlapped or serial, batched vs alone, T={T}")
     res = []
-    for K in (32, 160):
+    for K, overlap in ((32, True), (160, True), (32, False)):
         rts = [n.build() for n in _nets(alpha)]
-        res.append(RateLossGrad(rts, alpha, checkpoint=K)(T, targets))
-    assert res[0][0] == res[1][0] and np.array_equal(res[0][1], res[1][1])
+        res.append(RateLossGrad(rts, alpha, checkpoint=K, overlap=overlap)(T, targets))
+    # checkpoint intervals, and the recompute overlapped on its own stream or not, change nothing
+    for r in res[1:]:
+        assert r[0] == res[0][0] and np.array_equal(r[1], res[0][1])
     alone = []
     for i in range(2):
         rt = _nets(alpha)[i].build()

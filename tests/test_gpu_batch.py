@@ -184,3 +184,31 @@ def test_batch_of_both_models_equals_plain_runs():
                 assert_state_equal(rt, st)
             states = None
     assert all(int(rt.raster().sum()) > 0 for rt in plain)
+
+
+def test_batch_sync_follows_restored_runtimes():
+    """n2h_batch_sync: after the runtimes are restored to an earlier step, the same batch (synced, no new
+    allocation) replays those steps bit for bit; a plan bound to a delivery log since is refused."""
+    params = sweep_params(n_instances=3, seed=9, n_min=800, n_max=3000)
+    rts = [n.runtime for n in brunel_sweep(params, device="cuda", record=True, keep_canonical=True)]
+    _say(f"case: batch sync after restore, sizes {[rt.n for rt in rts]}")
+    fr = FusedRound(rts, {"order": 0, "xcd": 0}, reserve_steps=200)
+    fr.launch(40, graph_window=0)
+    snaps = [rt.snapshot() for rt in rts]
+    fr.launch(60, graph_window=0)
+    torch.cuda.synchronize()
+    first = [(rt.raster(40, 100), rt.v.clone(), rt.ring.clone()) for rt in rts]
+    for rt, sn in zip(rts, snaps):
+        rt.restore(sn)
+    fr.sync()
+    fr.launch(60, graph_window=0)
+    torch.cuda.synchronize()
+    for rt, (r, v, ring) in zip(rts, first):
+        assert rt.step == 100 and np.array_equal(rt.raster(40, 100), r)
+        assert torch.equal(rt.v.view(torch.int32), v.view(torch.int32))
+        assert torch.equal(rt.ring.view(torch.int32), ring.view(torch.int32))
+    rts[1].set_delivery_log(1000)
+    with pytest.raises(RuntimeError, match="log"):
+        fr.sync()
+    rts[1].set_delivery_log(None)
+    fr.close()
