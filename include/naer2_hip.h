@@ -127,7 +127,10 @@ typedef struct n2h_tuning {
     int32_t order;    /* 0: plans in the given order; 1: plans with the most links first */
     int32_t xcd;      /* 0 or 1: block b runs tile b; k > 1: block b runs tile (b % k) * ceil(T / k) + b / k, so
                          one plan's consecutive groups share an XCD (blocks are dealt to the k XCDs in turn) */
-    int32_t reserved[2];
+    int32_t stage;    /* 0: the lean batched stage (128 sources and 384 links a pass, 4 row cells in flight; less
+                         LDS and fewer registers a group, more groups a CU); 1: the single-plan stage (256, 512,
+                         8), for single-model batches of plans of at most 1,024 groups (otherwise 0 is used) */
+    int32_t reserved;
 } n2h_tuning;
 
 typedef struct n2h_plan n2h_plan;

@@ -69,10 +69,11 @@ class Tables(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("order", ctypes.c_int32), ("xcd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+    _fields_ = [("order", ctypes.c_int32), ("xcd", ctypes.c_int32), ("stage", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
-        return {"order": int(self.order), "xcd": int(self.xcd)}
+        return {"order": int(self.order), "xcd": int(self.xcd), "stage": int(self.stage)}
 
 
 class EvPop(ctypes.Structure):

@@ -1,4 +1,4 @@
-"""Launch-mapping autotuner of batched sweeps (FusedRound): plan order and XCD-aware block mapping.
+"""Launch-mapping autotuner of batched sweeps (FusedRound): plan order, XCD-aware block mapping and stage size.
 
 Every candidate mapping computes exactly the same results (each target group's work does not depend on
 where or when it runs), so the tuner times the candidates on the batch's own first steps -- the warmup --
@@ -8,7 +8,9 @@ and keeps the fastest.  Choices are cached per device and batch signature in a J
 Candidates: order 0 (plans as given), 1 (most links first, so the heaviest groups start first and light
 ones fill the tail) or 2 (most delivered links per group first, measured on each network alone:
 measure_work); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8 (a
-plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments).  Every
+plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments); stage 0 (the
+lean batched stage: 128 sources and 384 links a pass, 4 row cells in flight, so more groups fit a CU) or 1
+(the single-plan stage: 256, 512 and 8 -- fewer passes and more row cells in flight a group).  Every
 candidate runs from the same snapshot of the state, which is restored afterwards.
 
 tune_intake picks, per network, the spike count from which a step takes the row intake (every in-link
@@ -29,7 +31,7 @@ import torch
 from ._lib import Tuning
 from .fusion import FusedRound
 
-CANDIDATES = [{"order": o, "xcd": x} for o in (0, 1, 2) for x in (0, 8)]
+CANDIDATES = [{"order": o, "xcd": x, "stage": g} for o in (0, 1, 2) for x in (0, 8) for g in (0, 1)]
 
 
 def cache_path() -> Path:
@@ -40,7 +42,8 @@ def signature(runtimes) -> str:
     h = hashlib.sha1()
     for rt in runtimes:
         h.update(f"{rt.n}:{rt.n_groups}:{rt.n_links}:{rt.d_slots};".encode())
-    return f"{torch.cuda.get_device_name(runtimes[0].device)}|{len(runtimes)}|{h.hexdigest()[:16]}"
+    # "2": the candidate set with stage sizes (entries cached before it lack that dimension)
+    return f"2|{torch.cuda.get_device_name(runtimes[0].device)}|{len(runtimes)}|{h.hexdigest()[:16]}"
 
 
 def _load() -> dict:
@@ -94,8 +97,8 @@ def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int
             continue
         for rt, sn in zip(runtimes, snaps):
             rt.restore(sn)
-        fr = FusedRound(runtimes, Tuning(order=cand["order"], xcd=cand["xcd"]), reserve_steps=reserve_steps,
-                        weights=weights)
+        tuning = Tuning(order=cand["order"], xcd=cand["xcd"], stage=cand.get("stage", 0))
+        fr = FusedRound(runtimes, tuning, reserve_steps=reserve_steps, weights=weights)
         fr.launch(graph_window, graph_window)  # capture + first replay, untimed
         torch.cuda.synchronize()
         t0 = time.perf_counter()

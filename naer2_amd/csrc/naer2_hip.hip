@@ -1048,9 +1048,12 @@ struct BatchArgs {
 // the workgroup.
 constexpr int kModelMixed = -1;
 
-template <int MODEL, int KW>
+// WIDE (n2h_tuning.stage 1): the single-plan kernel's stage (kCapS sources, kCapL links a pass, kRowU row cells
+// in flight) instead of the lean batched one -- fewer passes and more row cells in flight a group, more LDS
+// and registers a group.
+template <int MODEL, int KW, bool WIDE = false>
 __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
-    __shared__ Stage<bm_words(KW), kBatchCapS, kBatchCapL, kBatchRowU> st;
+    __shared__ Stage<bm_words(KW), WIDE ? kCapS : kBatchCapS, WIDE ? kCapL : kBatchCapL, WIDE ? kRowU : kBatchRowU> st;
     extern __shared__ float acc[];
     const int b = blockIdx.x;
     int tile = b;
@@ -1312,6 +1315,15 @@ int batch_in_step(const n2h_batch* bt) {
 template <int MODEL>
 void launch_batch_model(const n2h_batch* bt, const BatchArgs& ba, dim3 grid, hipStream_t stream, int32_t k) {
     const dim3 block(kWave);
+    if (MODEL != kModelMixed && bt->tuning.stage == 1 && bt->words <= 16) {  // the wide stage (single-model batches)
+        if (bt->words <= 8)
+            hipLaunchKernelGGL((batch_kernel<MODEL, 8, true>), grid, block, bt->lds_bytes, stream, ba, k);
+        else if (bt->words <= 13)
+            hipLaunchKernelGGL((batch_kernel<MODEL, 13, true>), grid, block, bt->lds_bytes, stream, ba, k);
+        else
+            hipLaunchKernelGGL((batch_kernel<MODEL, 16, true>), grid, block, bt->lds_bytes, stream, ba, k);
+        return;
+    }
     if (bt->words <= 8)
         hipLaunchKernelGGL((batch_kernel<MODEL, 8>), grid, block, bt->lds_bytes, stream, ba, k);
     else if (bt->words <= 13)  // up to 832 groups a plan (Brunel networks up to 53k neurons) at 20 groups per CU
@@ -1547,6 +1559,7 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     if (tuning) tu = *tuning;
     if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
     if (tu.xcd < 0 || tu.xcd > 64) return fail(N2H_EINVAL, "tuning.xcd must be in [0, 64]");
+    if (tu.stage < 0 || tu.stage > 1) return fail(N2H_EINVAL, "tuning.stage must be 0 or 1");
     for (int32_t i = 0; i < n_plans; ++i) {
         int rc = check_ready(plans[i]);
         if (rc) return rc;
