@@ -380,6 +380,15 @@ int n2h_set_status(n2h_plan* plan, int32_t* rows);
  * NULL turns it off.  Graphs captured before the call keep the setting they were captured with. */
 int n2h_set_trace(n2h_plan* plan, uint64_t* buffer);
 
+/* Launch mapping of the plan's own step kernels (replaces the reference's per-device launch choice,
+ * naer2/runtime.py:88-95 _warp_cap, and its autotuning backlog TODO.md:107-109): tuning->xcd = 0 or 1: block b
+ * runs target group b; k > 1: block b runs group (b % k) * ceil(n_groups / k) + b / k, so that the blocks
+ * the hardware deals to one of k XCDs run consecutive groups, which read neighbouring entries of a spiking
+ * source's offset row and neighbouring link segments through one L2.  The default is k = 8 (MI355X's XCDs).
+ * Results do not change.  Graphs captured before the call keep the mapping they were captured with; batches
+ * take theirs from n2h_batch_create. */
+int n2h_set_tuning(n2h_plan* plan, const n2h_tuning* tuning);
+
 /* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */
 int n2h_status(n2h_plan* plan, int32_t* info, int32_t n_info);
 

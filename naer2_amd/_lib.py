@@ -29,7 +29,7 @@ EXPORTS = (
     "n2h_graph_destroy", "n2h_status", "n2h_batch_create", "n2h_batch_destroy", "n2h_batch_step", "n2h_batch_sync",
     "n2h_batch_graph_create", "n2h_bind_rows", "n2h_set_vtrace", "n2h_set_trace", "n2h_adj_last_error", "n2h_adj_step", "n2h_adj_window", "n2h_adj_work_words", "n2h_rlif_last_error", "n2h_rlif_forward", "n2h_rlif_readout",
     "n2h_rlif_readout_adjoint", "n2h_rlif_backward", "n2h_rlif_delivery_adjoint", "n2h_rlif_work_floats",
-    "n2h_rlif_backward_noalloc", "n2h_bind_log", "n2h_set_status", "n2h_bind_edges",
+    "n2h_rlif_backward_noalloc", "n2h_bind_log", "n2h_set_status", "n2h_bind_edges", "n2h_set_tuning",
     "n2h_ev_last_error", "n2h_ev_create", "n2h_ev_destroy", "n2h_ev_bind_state", "n2h_ev_bind_record",
     "n2h_ev_set_window", "n2h_ev_get_window", "n2h_ev_run", "n2h_ev_graph_create", "n2h_ev_graph_launch",
     "n2h_ev_graph_destroy", "n2h_ev_status", "n2h_ev_bind_log", "n2h_ev_set_tuning", "n2h_ev_set_trace", "n2h_ev_counters", "n2h_ev_run_timed", "n2h_ev_math",
@@ -130,6 +130,7 @@ def load(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         "n2h_set_trace": ([vp, vp], i32),
         "n2h_bind_log": ([vp, vp, vp, vp, i64], i32),
         "n2h_set_status": ([vp, vp], i32),
+        "n2h_set_tuning": ([vp, ctypes.POINTER(Tuning)], i32),
         "n2h_bind_edges": ([vp, vp, vp, i32], i32),
         "n2h_bind_rows": ([vp, vp, vp, vp, i32, vp, i32], i32),
         "n2h_set_vtrace": ([vp, vp, i64, i32, vp, vp], i32),

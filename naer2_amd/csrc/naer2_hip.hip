@@ -53,6 +53,10 @@ constexpr int kListSlots = 4;
 constexpr int kMaxBatch = 256;  // plans in one batched launch
 constexpr int kDenseWords = 1536;  // row intake: spike bitmaps of up to 1535 groups (98,240 neurons) in LDS, plus
                                    // the sentinel word (source slots 64 n_groups .. +63: never spiking)
+// Single-plan launches deal consecutive target groups to one XCD by default (n2h_set_tuning): neighbouring
+// groups read neighbouring entries of a spiking source's offset row and neighbouring link segments, so
+// their cache lines are fetched once into one L2 instead of once into each of the 8.
+constexpr int kDefaultXcds = 8;
 constexpr int kTraceEvents = 8;  // entry, list read, sources ranked, segments, staged, walked, updated, exit
 
 // In-kernel phase stamps (100 MHz wall clock), lane 0 of each group, when tracing is on: kept in LDS and
@@ -127,7 +131,17 @@ struct StepArgs {
     uint2* links_rw;            //   the link table, written by plastic links (same address as links)
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
+    int32_t xcd;                // single-plan launches: 0, or k > 1 XCDs dealt blocks in turn: block b runs group
+    int32_t xcd_per;            //   (b % k) * xcd_per + b / k, xcd_per = ceil(n_groups / k)
 };
+
+// The target group of this block in a single-plan launch (-1: a padding block of the XCD mapping).
+__device__ __forceinline__ int plan_group(const StepArgs& a) {
+    if (a.xcd <= 1) return static_cast<int>(blockIdx.x);
+    const int b = static_cast<int>(blockIdx.x);
+    const int g = (b % a.xcd) * a.xcd_per + b / a.xcd;
+    return g < a.n_groups ? g : -1;
+}
 
 // Inclusive prefix sum over the 64 lanes of the wavefront, in DPP lane moves (no LDS round trip): a
 // Hillis-Steele scan inside each row of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row
@@ -1004,7 +1018,9 @@ template <int MODEL, int KW>
 __global__ __launch_bounds__(kWave) void step_kernel(StepArgs a, int32_t k) {
     __shared__ Stage<bm_words(KW)> st;
     extern __shared__ float acc[];  // [d_slots][64] partial sums of this emission step
-    step_group<MODEL, KW>(a, blockIdx.x, k, st, acc);
+    const int g = plan_group(a);
+    if (g < 0) return;
+    step_group<MODEL, KW>(a, g, k, st, acc);
 }
 
 // The edge variant (n2h_bind_edges): the step without the row intake, links delivering by their edge kind into
@@ -1016,7 +1032,9 @@ template <int MODEL, bool LOG, bool PORTS>
 __global__ __launch_bounds__(kWave) void step_edge_kernel(StepArgs a, int32_t k) {
     __shared__ Stage<bm_words(0), PORTS ? kBatchCapS : kCapS, PORTS ? kBatchCapL : kCapL, kRowU, LOG, true, PORTS> st;
     extern __shared__ float acc[];
-    step_group<MODEL, 0>(a, blockIdx.x, k, st, acc);
+    const int g = plan_group(a);
+    if (g < 0) return;
+    step_group<MODEL, 0>(a, g, k, st, acc);
 }
 
 // The logging variant (n2h_bind_log): the step without the row intake (KW = 0: segment intakes only, which
@@ -1025,7 +1043,9 @@ template <int MODEL>
 __global__ __launch_bounds__(kWave) void step_log_kernel(StepArgs a, int32_t k) {
     __shared__ Stage<bm_words(0), kCapS, kCapL, kRowU, true> st;
     extern __shared__ float acc[];
-    step_group<MODEL, 0>(a, blockIdx.x, k, st, acc);
+    const int g = plan_group(a);
+    if (g < 0) return;
+    step_group<MODEL, 0>(a, g, k, st, acc);
 }
 
 // Batched step: the target groups of many independent networks (plans) in one grid.  Tiles are the
@@ -1142,6 +1162,7 @@ struct n2h_plan {
     int32_t v_slots;        // edge plans: LDS cells of the potential port (0: no potential-port links)
     bool edges;             // n2h_bind_edges bound kinds or ports
     int32_t multirate;      // some population's stride > 1
+    int32_t xcd;            // launch mapping (n2h_set_tuning): 0 or 1 block b runs group b; k > 1 see plan_group
     size_t lds_bytes;
     int device;
 };
@@ -1227,6 +1248,10 @@ StepArgs make_args(const n2h_plan* p) {
     a.v_slots = p->v_slots;
     a.links_rw = const_cast<uint2*>(reinterpret_cast<const uint2*>(p->tables.links));
     a.multirate = p->multirate;
+    if (p->xcd > 1) {
+        a.xcd = p->xcd;
+        a.xcd_per = (a.n_groups + p->xcd - 1) / p->xcd;
+    }
     return a;
 }
 
@@ -1243,7 +1268,7 @@ void launch_step_model(int words, dim3 grid, size_t lds, hipStream_t stream, con
 int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     StepArgs a = make_args(p);
     a.step_ctr = graph ? p->step_ctr : nullptr;
-    const dim3 grid(static_cast<unsigned>(a.n_groups)), block(kWave);
+    const dim3 grid(static_cast<unsigned>(a.xcd > 1 ? a.xcd * a.xcd_per : a.n_groups)), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
         if (p->edges) {  // edge plans (LIF_EXP only)
@@ -1435,6 +1460,7 @@ int n2h_plan_create(const n2h_model_desc* desc, const n2h_tables* tables, n2h_pl
     for (int q = 0; q < desc->n_pops; ++q) p->multirate |= desc->pops[q].stride > 1 ? 1 : 0;
     // partial sums: one cell per delay, or the one cell of a uniform delay
     p->lds_bytes = sizeof(float) * kWave * static_cast<size_t>(tables->uniform_delay > 0 ? 1 : desc->d_slots);
+    p->xcd = kDefaultXcds;
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->pops_dev, sizeof(n2h_pop_params) * desc->n_pops);
     if (e == hipSuccess) e = hipMalloc(&p->step_ctr, sizeof(int64_t));
@@ -1803,6 +1829,13 @@ int n2h_bind_edges(n2h_plan* p, const float* edge_attr, float* ring_v, int32_t v
 int n2h_set_status(n2h_plan* p, int32_t* rows) {
     if (!p) return fail(N2H_EINVAL, "null plan");
     p->status = rows;
+    return N2H_OK;
+}
+
+int n2h_set_tuning(n2h_plan* p, const n2h_tuning* t) {
+    if (!p || !t) return fail(N2H_EINVAL, "null argument");
+    if (t->xcd < 0 || t->xcd > 64) return fail(N2H_EINVAL, "tuning.xcd must be in [0, 64]");
+    p->xcd = t->xcd;
     return N2H_OK;
 }
 

@@ -627,6 +627,17 @@ class Runtime:
             lib.n2h_graph_destroy(g)
         self._graphs = {}
 
+    def set_launch_mapping(self, xcd: int) -> None:
+        """Launch mapping of this runtime's own step kernels (n2h_set_tuning): xcd = 0 runs group b in block b;
+        k > 1 gives the blocks the hardware deals to each of k XCDs consecutive groups (default 8).  Results do
+        not change; captured graphs are dropped so that the next run captures with the new mapping."""
+        lib = _lib.load()
+        t = _lib.Tuning(order=0, xcd=int(xcd), stage=0, reserved=0)
+        _lib.check(lib.n2h_set_tuning(self._plan, ctypes.byref(t)), "n2h_set_tuning")
+        for g in self._graphs.values():
+            lib.n2h_graph_destroy(g)
+        self._graphs = {}
+
     def status_rows(self) -> np.ndarray:
         """int32 [n_groups, 4]: per target group, the last step it ran, the spikes it emitted then, its intake
         path (0 none, 1 sorted list, 2 bitmap scan, 3 row intake) and the spiking sources it took in."""

@@ -180,3 +180,22 @@ def test_dsl_state_views_write_through():
     assert float(rt.v[:300].min()) == 5.0 and float(rt.v[:300].max()) == 5.0
     net.step(1.0)
     assert net.step_index == 10 and abs(net.t - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("xcd", [0, 3, 8])
+def test_launch_mappings_equal_the_oracle(xcd):
+    """Every block-to-group mapping of the single-plan step (n2h_set_tuning) gives the oracle's results,
+    eager and through captured graphs (groups not a multiple of the XCD count: padding blocks)."""
+    _say(f"case: pd14 scale=0.05 seed=4, launch mapping xcd={xcd}, 400 steps")
+    net, _ = pd14(scale=0.05, seed=4, device="cuda")
+    rt = net.build()
+    rt.set_launch_mapping(xcd)
+    assert rt.n_groups % 8 != 0
+    st = oracle_state(rt)
+    net.run(200, graph_window=0)
+    net.run(200, graph_window=100)
+    torch.cuda.synchronize()
+    raster, _ = oracle_run(rt, st, 0, 400)
+    assert raster.sum() > 50
+    assert np.array_equal(rt.raster(), raster)
+    assert_state_equal(rt, st)
