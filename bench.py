@@ -268,7 +268,7 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
             raise SystemExit(f"--timestep: no population {name!r} (have {sorted(pops)})")
         pops[name].timestep = timesteps[name] = float(ms)
     rt = net.build()
-    rt.set_launch_mapping(args.xcd)
+    rt.set_launch_mapping(args.xcd, args.xcd_run)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
     log(f"built {kind}: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, {rt.d_slots} ring "
@@ -331,7 +331,7 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
                                f"{warmup} warmup steps",
                    "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
                    **({"timesteps_ms": timesteps} if timesteps else {}),
-                   "graph_window": m["graph_window"], "launch_mapping": {"xcd": args.xcd},
+                   "graph_window": m["graph_window"], "launch_mapping": {"xcd": args.xcd, "run": args.xcd_run},
                    "parallelism": "single GPU"},
         "events_per_s": round(m["events_per_s"], 1),
         "spikes_per_step": round(m["spikes"] / steps, 3),
@@ -849,6 +849,9 @@ def main() -> int:
     ap.add_argument("--xcd", type=int, default=8,
                     help="pd14/brunel: launch mapping of the step (n2h_set_tuning): 0 block b runs group b; k > 1 "
                          "consecutive groups on each of k XCDs")
+    ap.add_argument("--xcd-run", type=int, default=0,
+                    help="pd14/brunel: with --xcd k > 1, runs of this many consecutive groups dealt to the XCDs in "
+                         "turn (0: one contiguous chunk per XCD)")
     ap.add_argument("--tune-intake", action="store_true",
                     help="sweep: time each network's row-intake thresholds first (naer2_amd.autotune.tune_intake)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",

@@ -125,12 +125,15 @@ typedef struct n2h_tables {
 /* Launch mapping of a batch (chosen by naer2_amd/autotune.py). */
 typedef struct n2h_tuning {
     int32_t order;    /* 0: plans in the given order; 1: plans with the most links first */
-    int32_t xcd;      /* 0 or 1: block b runs tile b; k > 1: block b runs tile (b % k) * ceil(T / k) + b / k, so
-                         one plan's consecutive groups share an XCD (blocks are dealt to the k XCDs in turn) */
+    int32_t xcd;      /* 0 or 1: block b runs tile b.  k > 1: the blocks the hardware deals to each of k XCDs (block
+                         b to XCD b mod k) run consecutive tiles, neighbouring groups that share cache lines: */
     int32_t stage;    /* 0: the lean batched stage (128 sources and 384 links a pass, 4 row cells in flight; less
                          LDS and fewer registers a group, more groups a CU); 1: the single-plan stage (256, 512,
                          8), for single-model batches of plans of at most 1,024 groups (otherwise 0 is used) */
-    int32_t reserved;
+    int32_t run;      /* with xcd = k > 1: 0, each XCD one contiguous chunk of ceil(T / k) tiles (block b runs tile
+                         (b % k) * ceil(T / k) + b / k); r in [1, 4096], runs of r consecutive tiles dealt to the
+                         k XCDs in turn (block b runs tile ((b / k / r) * k + b % k) * r + (b / k) % r), so the
+                         XCDs' shares interleave over the whole tile order.  Ignored by n2h_ev_set_tuning. */
 } n2h_tuning;
 
 typedef struct n2h_plan n2h_plan;
@@ -381,12 +384,12 @@ int n2h_set_status(n2h_plan* plan, int32_t* rows);
 int n2h_set_trace(n2h_plan* plan, uint64_t* buffer);
 
 /* Launch mapping of the plan's own step kernels (replaces the reference's per-device launch choice,
- * naer2/runtime.py:88-95 _warp_cap, and its autotuning backlog TODO.md:107-109): tuning->xcd = 0 or 1: block b
- * runs target group b; k > 1: block b runs group (b % k) * ceil(n_groups / k) + b / k, so that the blocks
- * the hardware deals to one of k XCDs run consecutive groups, which read neighbouring entries of a spiking
- * source's offset row and neighbouring link segments through one L2.  The default is k = 8 (MI355X's XCDs).
- * Results do not change.  Graphs captured before the call keep the mapping they were captured with; batches
- * take theirs from n2h_batch_create. */
+ * naer2/runtime.py:88-95 _warp_cap, and its autotuning backlog TODO.md:107-109): tuning->xcd and ->run as in
+ * n2h_tuning, tiles being the plan's target groups.  With k > 1 the blocks the hardware deals to one XCD run
+ * consecutive groups, which read neighbouring entries of a spiking source's offset row and neighbouring
+ * link segments through one L2.  The default is k = 8 (MI355X's XCDs), run 0.  Results do not change.
+ * Graphs captured before the call keep the mapping they were captured with; batches take theirs from
+ * n2h_batch_create. */
 int n2h_set_tuning(n2h_plan* plan, const n2h_tuning* tuning);
 
 /* info[0] = n_groups, info[1] = d_slots, info[2] = dynamic LDS bytes per workgroup, info[3] = model. */

@@ -70,10 +70,10 @@ class Tables(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("order", ctypes.c_int32), ("xcd", ctypes.c_int32), ("stage", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("run", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
-        return {"order": int(self.order), "xcd": int(self.xcd), "stage": int(self.stage)}
+        return {"order": int(self.order), "xcd": int(self.xcd), "stage": int(self.stage), "run": int(self.run)}
 
 
 class EvPop(ctypes.Structure):

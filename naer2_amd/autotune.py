@@ -7,8 +7,11 @@ and keeps the fastest.  Choices are cached per device and batch signature in a J
 
 Candidates: order 0 (plans as given), 1 (most links first, so the heaviest groups start first and light
 ones fill the tail) or 2 (most delivered links per group first, measured on each network alone:
-measure_work); xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs) or 8 (a
-plan's consecutive groups on one XCD, sharing its L2: spike lists, offset rows, link segments); stage 0 (the
+measure_work); XCD mapping xcd 0 (blocks in tile order, a plan's groups dealt round-robin over the 8 XCDs),
+xcd 8 run 0 (each XCD one contiguous eighth of the tiles: a plan's consecutive groups share an L2 -- spike
+lists, offset rows, link segments -- but whole networks' uneven work lands on one XCD) or xcd 8 run 16
+(runs of 16 neighbouring groups dealt to the XCDs in turn: the sharing within a run, the XCDs' shares
+interleaved over the whole batch); stage 0 (the
 lean batched stage: 128 sources and 384 links a pass, 4 row cells in flight, so more groups fit a CU) or 1
 (the single-plan stage: 256, 512 and 8 -- fewer passes and more row cells in flight a group).  Every
 candidate runs from the same snapshot of the state, which is restored afterwards.
@@ -31,7 +34,10 @@ import torch
 from ._lib import Tuning
 from .fusion import FusedRound
 
-CANDIDATES = [{"order": o, "xcd": x, "stage": g} for o in (0, 1, 2) for x in (0, 8) for g in (0, 1)]
+# XCD mappings (xcd, run): tile order; one chunk of the batch per XCD; runs of 16 neighbouring groups dealt to
+# the XCDs in turn
+MAPPINGS = ((0, 0), (8, 0), (8, 16))
+CANDIDATES = [{"order": o, "xcd": x, "run": r, "stage": g} for o in (0, 1, 2) for x, r in MAPPINGS for g in (0, 1)]
 
 
 def cache_path() -> Path:
@@ -42,8 +48,8 @@ def signature(runtimes) -> str:
     h = hashlib.sha1()
     for rt in runtimes:
         h.update(f"{rt.n}:{rt.n_groups}:{rt.n_links}:{rt.d_slots};".encode())
-    # "2": the candidate set with stage sizes (entries cached before it lack that dimension)
-    return f"2|{torch.cuda.get_device_name(runtimes[0].device)}|{len(runtimes)}|{h.hexdigest()[:16]}"
+    # "3": the candidate set with stage sizes and XCD runs (entries cached before it lack those dimensions)
+    return f"3|{torch.cuda.get_device_name(runtimes[0].device)}|{len(runtimes)}|{h.hexdigest()[:16]}"
 
 
 def _load() -> dict:
@@ -97,7 +103,7 @@ def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int
             continue
         for rt, sn in zip(runtimes, snaps):
             rt.restore(sn)
-        tuning = Tuning(order=cand["order"], xcd=cand["xcd"], stage=cand.get("stage", 0))
+        tuning = Tuning(order=cand["order"], xcd=cand["xcd"], stage=cand.get("stage", 0), run=cand.get("run", 0))
         fr = FusedRound(runtimes, tuning, reserve_steps=reserve_steps, weights=weights)
         fr.launch(graph_window, graph_window)  # capture + first replay, untimed
         torch.cuda.synchronize()

@@ -131,16 +131,36 @@ struct StepArgs {
     uint2* links_rw;            //   the link table, written by plastic links (same address as links)
     int32_t multirate;          // some population updates every stride > 1 steps (its parameter block's stride)
     int32_t model;              // N2H_MODEL_*: the network's neuron model (mixed batches branch on it per tile)
-    int32_t xcd;                // single-plan launches: 0, or k > 1 XCDs dealt blocks in turn: block b runs group
-    int32_t xcd_per;            //   (b % k) * xcd_per + b / k, xcd_per = ceil(n_groups / k)
+    int32_t xcd;                // single-plan launches: the XCD mapping (xcd_tile) of blocks to target groups
+    int32_t xcd_run;
 };
+
+// The tile that block b runs under an XCD mapping (k, run) of n_tiles tiles (-1: a padding block).  The
+// hardware deals a launch's blocks to its k XCDs in turn (block b to XCD b mod k).  k <= 1: block b runs
+// tile b.  run <= 0: each XCD runs one contiguous chunk of ceil(n_tiles / k) tiles.  run > 0: runs of `run`
+// consecutive tiles are dealt to the k XCDs in turn, so that each XCD's tiles are runs of neighbours and
+// the XCDs' shares interleave over the whole tile order.
+__host__ __device__ inline int64_t xcd_blocks(int64_t n_tiles, int k, int run) {
+    if (k <= 1) return n_tiles;
+    if (run <= 0) return static_cast<int64_t>(k) * ((n_tiles + k - 1) / k);
+    const int64_t span = static_cast<int64_t>(k) * run;
+    return span * ((n_tiles + span - 1) / span);
+}
+__device__ __forceinline__ int xcd_tile(int b, int n_tiles, int k, int run) {
+    if (k <= 1) return b;
+    int t;
+    if (run <= 0) {
+        t = (b % k) * ((n_tiles + k - 1) / k) + b / k;
+    } else {
+        const int i = b / k;
+        t = ((i / run) * k + b % k) * run + i % run;
+    }
+    return t < n_tiles ? t : -1;
+}
 
 // The target group of this block in a single-plan launch (-1: a padding block of the XCD mapping).
 __device__ __forceinline__ int plan_group(const StepArgs& a) {
-    if (a.xcd <= 1) return static_cast<int>(blockIdx.x);
-    const int b = static_cast<int>(blockIdx.x);
-    const int g = (b % a.xcd) * a.xcd_per + b / a.xcd;
-    return g < a.n_groups ? g : -1;
+    return xcd_tile(static_cast<int>(blockIdx.x), a.n_groups, a.xcd, a.xcd_run);
 }
 
 // Inclusive prefix sum over the 64 lanes of the wavefront, in DPP lane moves (no LDS round trip): a
@@ -1049,9 +1069,8 @@ __global__ __launch_bounds__(kWave) void step_log_kernel(StepArgs a, int32_t k) 
 }
 
 // Batched step: the target groups of many independent networks (plans) in one grid.  Tiles are the
-// plans' groups in plan order; with xcd = k > 1, block b runs tile (b % k) * ceil(T / k) + b / k, so that
-// consecutive tiles -- one plan's groups -- land on the same XCD (blocks are dealt to the k XCDs in turn)
-// and share its L2.
+// plans' groups in plan order; block b runs tile xcd_tile(b, T, xcd, run): with xcd = k > 1 consecutive
+// tiles -- neighbouring groups of one plan -- land on the same XCD and share its L2.
 struct BatchArgs {
     const StepArgs* plans;     // device [n_plans]
     const int64_t* step_ctr;   // graph launches: the window's first step; eager launches: null
@@ -1060,6 +1079,7 @@ struct BatchArgs {
     int32_t n_plans;
     int32_t n_tiles;
     int32_t xcd;
+    int32_t run;
     int32_t first[kMaxBatch + 1];  // first tile of each plan
 };
 
@@ -1075,13 +1095,8 @@ template <int MODEL, int KW, bool WIDE = false>
 __global__ __launch_bounds__(kWave) void batch_kernel(BatchArgs ba, int32_t k) {
     __shared__ Stage<bm_words(KW), WIDE ? kCapS : kBatchCapS, WIDE ? kCapL : kBatchCapL, WIDE ? kRowU : kBatchRowU> st;
     extern __shared__ float acc[];
-    const int b = blockIdx.x;
-    int tile = b;
-    if (ba.xcd > 1) {
-        const int per = (ba.n_tiles + ba.xcd - 1) / ba.xcd;
-        tile = (b % ba.xcd) * per + b / ba.xcd;
-    }
-    if (tile >= ba.n_tiles) return;
+    const int tile = xcd_tile(static_cast<int>(blockIdx.x), ba.n_tiles, ba.xcd, ba.run);
+    if (tile < 0) return;
     int lo = 0, hi = ba.n_plans;  // first[lo] <= tile < first[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -1162,7 +1177,8 @@ struct n2h_plan {
     int32_t v_slots;        // edge plans: LDS cells of the potential port (0: no potential-port links)
     bool edges;             // n2h_bind_edges bound kinds or ports
     int32_t multirate;      // some population's stride > 1
-    int32_t xcd;            // launch mapping (n2h_set_tuning): 0 or 1 block b runs group b; k > 1 see plan_group
+    int32_t xcd;            // launch mapping (n2h_set_tuning): xcd_tile's k and run
+    int32_t xcd_run;
     size_t lds_bytes;
     int device;
 };
@@ -1248,10 +1264,8 @@ StepArgs make_args(const n2h_plan* p) {
     a.v_slots = p->v_slots;
     a.links_rw = const_cast<uint2*>(reinterpret_cast<const uint2*>(p->tables.links));
     a.multirate = p->multirate;
-    if (p->xcd > 1) {
-        a.xcd = p->xcd;
-        a.xcd_per = (a.n_groups + p->xcd - 1) / p->xcd;
-    }
+    a.xcd = p->xcd;
+    a.xcd_run = p->xcd_run;
     return a;
 }
 
@@ -1268,7 +1282,7 @@ void launch_step_model(int words, dim3 grid, size_t lds, hipStream_t stream, con
 int launch_steps(n2h_plan* p, int32_t n_steps, hipStream_t stream, bool graph) {
     StepArgs a = make_args(p);
     a.step_ctr = graph ? p->step_ctr : nullptr;
-    const dim3 grid(static_cast<unsigned>(a.xcd > 1 ? a.xcd * a.xcd_per : a.n_groups)), block(kWave);
+    const dim3 grid(static_cast<unsigned>(xcd_blocks(a.n_groups, a.xcd, a.xcd_run))), block(kWave);
     const int words = (a.n_groups + kWave - 1) / kWave;  // bitmap words per lane on the burst path
     for (int32_t k = 0; k < n_steps; ++k) {
         if (p->edges) {  // edge plans (LIF_EXP only)
@@ -1372,10 +1386,9 @@ int launch_batch(n2h_batch* bt, int32_t n_steps, hipStream_t stream, bool graph)
     ba.n_plans = bt->n_plans;
     ba.n_tiles = bt->n_tiles;
     ba.xcd = bt->tuning.xcd;
+    ba.run = bt->tuning.run;
     for (int32_t i = 0; i <= bt->n_plans; ++i) ba.first[i] = bt->first[i];
-    int64_t blocks = bt->n_tiles;
-    if (ba.xcd > 1) blocks = static_cast<int64_t>(ba.xcd) * ((bt->n_tiles + ba.xcd - 1) / ba.xcd);
-    const dim3 grid(static_cast<unsigned>(blocks));
+    const dim3 grid(static_cast<unsigned>(xcd_blocks(bt->n_tiles, ba.xcd, ba.run)));
     for (int32_t k = 0; k < n_steps; ++k) {
         if (bt->model == N2H_MODEL_LIF_DELTA) launch_batch_model<N2H_MODEL_LIF_DELTA>(bt, ba, grid, stream, k);
         else if (bt->model == N2H_MODEL_LIF_EXP) launch_batch_model<N2H_MODEL_LIF_EXP>(bt, ba, grid, stream, k);
@@ -1586,6 +1599,7 @@ int n2h_batch_create(n2h_plan* const* plans, int32_t n_plans, const n2h_tuning* 
     if (tu.order < 0 || tu.order > 1) return fail(N2H_EINVAL, "tuning.order must be 0 or 1");
     if (tu.xcd < 0 || tu.xcd > 64) return fail(N2H_EINVAL, "tuning.xcd must be in [0, 64]");
     if (tu.stage < 0 || tu.stage > 1) return fail(N2H_EINVAL, "tuning.stage must be 0 or 1");
+    if (tu.run < 0 || tu.run > 4096) return fail(N2H_EINVAL, "tuning.run must be in [0, 4096]");
     for (int32_t i = 0; i < n_plans; ++i) {
         int rc = check_ready(plans[i]);
         if (rc) return rc;
@@ -1835,7 +1849,9 @@ int n2h_set_status(n2h_plan* p, int32_t* rows) {
 int n2h_set_tuning(n2h_plan* p, const n2h_tuning* t) {
     if (!p || !t) return fail(N2H_EINVAL, "null argument");
     if (t->xcd < 0 || t->xcd > 64) return fail(N2H_EINVAL, "tuning.xcd must be in [0, 64]");
+    if (t->run < 0 || t->run > 4096) return fail(N2H_EINVAL, "tuning.run must be in [0, 4096]");
     p->xcd = t->xcd;
+    p->xcd_run = t->run;
     return N2H_OK;
 }
 

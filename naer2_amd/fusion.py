@@ -36,7 +36,7 @@ class FusedRound:
             raise ValueError(f"FusedRound: the runtimes are at different steps ({sorted(steps)})")
         if isinstance(tuning, dict):
             tuning = Tuning(order=int(tuning.get("order", 0)), xcd=int(tuning.get("xcd", 0)),
-                            stage=int(tuning.get("stage", 0)))
+                            stage=int(tuning.get("stage", 0)), run=int(tuning.get("run", 0)))
         self.tuning = tuning if tuning is not None else Tuning(order=0, xcd=0, stage=0)
         self.runtimes = list(runtimes)
         launch_tuning = self.tuning
@@ -45,7 +45,7 @@ class FusedRound:
                 raise ValueError("FusedRound: order 2 needs one weight per runtime (autotune.measure_work)")
             perm = sorted(range(len(self.runtimes)), key=lambda i: -float(weights[i]))
             self.runtimes = [self.runtimes[i] for i in perm]
-            launch_tuning = Tuning(order=0, xcd=self.tuning.xcd, stage=self.tuning.stage)
+            launch_tuning = Tuning(order=0, xcd=self.tuning.xcd, stage=self.tuning.stage, run=self.tuning.run)
         for rt in self.runtimes:  # the bit histories must not move while the batch holds their addresses
             rt._ensure_slots(reserve_steps)
             rt._batch_capacity = rt.step + reserve_steps if rt.net.record else None

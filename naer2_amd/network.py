@@ -627,12 +627,13 @@ class Runtime:
             lib.n2h_graph_destroy(g)
         self._graphs = {}
 
-    def set_launch_mapping(self, xcd: int) -> None:
+    def set_launch_mapping(self, xcd: int, run: int = 0) -> None:
         """Launch mapping of this runtime's own step kernels (n2h_set_tuning): xcd = 0 runs group b in block b;
-        k > 1 gives the blocks the hardware deals to each of k XCDs consecutive groups (default 8).  Results do
-        not change; captured graphs are dropped so that the next run captures with the new mapping."""
+        k > 1 gives the blocks the hardware deals to each of k XCDs consecutive groups: one chunk each (run 0,
+        the default with k = 8) or runs of `run` groups dealt to the XCDs in turn.  Results do not change;
+        captured graphs are dropped so that the next run captures with the new mapping."""
         lib = _lib.load()
-        t = _lib.Tuning(order=0, xcd=int(xcd), stage=0, reserved=0)
+        t = _lib.Tuning(order=0, xcd=int(xcd), stage=0, run=int(run))
         _lib.check(lib.n2h_set_tuning(self._plan, ctypes.byref(t)), "n2h_set_tuning")
         for g in self._graphs.values():
             lib.n2h_graph_destroy(g)

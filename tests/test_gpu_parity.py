@@ -182,14 +182,15 @@ def test_dsl_state_views_write_through():
     assert net.step_index == 10 and abs(net.t - 1.0) < 1e-9
 
 
-@pytest.mark.parametrize("xcd", [0, 3, 8])
-def test_launch_mappings_equal_the_oracle(xcd):
+@pytest.mark.parametrize("xcd,run", [(0, 0), (3, 0), (8, 0), (8, 4), (5, 7)])
+def test_launch_mappings_equal_the_oracle(xcd, run):
     """Every block-to-group mapping of the single-plan step (n2h_set_tuning) gives the oracle's results,
-    eager and through captured graphs (groups not a multiple of the XCD count: padding blocks)."""
-    _say(f"case: pd14 scale=0.05 seed=4, launch mapping xcd={xcd}, 400 steps")
+    eager and through captured graphs (groups not a multiple of the XCD count or of its runs: padding
+    blocks)."""
+    _say(f"case: pd14 scale=0.05 seed=4, launch mapping xcd={xcd} run={run}, 400 steps")
     net, _ = pd14(scale=0.05, seed=4, device="cuda")
     rt = net.build()
-    rt.set_launch_mapping(xcd)
+    rt.set_launch_mapping(xcd, run)
     assert rt.n_groups % 8 != 0
     st = oracle_state(rt)
     net.run(200, graph_window=0)

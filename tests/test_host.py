@@ -60,7 +60,7 @@ def test_plan_create_rejects_bad_arguments():
     assert "poisson_thr" in _lib.last_error()
     assert lib.n2h_step(None, 1, None) == -1
     assert lib.n2h_bind_ring(None, None, None, 2, None) == -1
-    t = _lib.Tuning(order=0, xcd=8, stage=0, reserved=0)
+    t = _lib.Tuning(order=0, xcd=8, stage=0, run=0)
     assert lib.n2h_set_tuning(None, ctypes.byref(t)) == -1 and "null" in _lib.last_error()
 
 
