@@ -45,6 +45,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "naer2_hip.h"
 #include "twofloat.h"
@@ -184,9 +185,11 @@ constexpr int adj_halving_shift() {  // 6 - log2 V: lane l ends with element l >
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
+constexpr int kPipeBlocks = 1024;    // its pipelined form: 4 blocks per CU (117 VGPRs: 4 wavefronts a SIMD)
 constexpr int kGroupLanes = 8;       // the population-pure long kernel: lanes a (gatherer, range) task
 constexpr int kGroupBlocks = 1536;   // its workgroups: 6 blocks per CU (74 VGPRs at W = 4: 6 wavefronts per SIMD)
-static_assert(kSplitBlocks % 8 == 0 && kGroupBlocks % 8 == 0, "the split long kernels deal workgroups to 8 XCDs in turn");
+static_assert(kSplitBlocks % 8 == 0 && kGroupBlocks % 8 == 0 && kPipeBlocks % 8 == 0,
+              "the split long kernels deal workgroups to 8 XCDs in turn");
 constexpr int kSubLists = 16;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
@@ -385,6 +388,127 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
             // the rows of the steps in the mask: target ranges added in launch order (the first one stores)
             float* row = a.long_bins + (static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp;
             *row = part == 0 ? v : *row + v;
+        }
+    }
+}
+
+// The split long kernel (all ranges in one launch over the XCDs, equal ranges), with its items pipelined in
+// each wavefront: the wavefront's next 64 items' list entries and run bounds are loaded at once (an item a
+// lane, read back uniform), and the first pass of the next item's link records is issued right after the
+// current item's ring-row loads, so that its HBM round trip runs under the current item's sums and
+// reduction.  Every item's sums are those of adj_long_kernel<W, true, false>, in the same order.
+template <int W>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipe_kernel(AdjBatch b) {
+    constexpr int WP = adj_win_pad<W>();
+    constexpr int V = kAdjPops * WP;
+    constexpr int SH = adj_halving_shift<V>();
+    const int lane = threadIdx.x & (kWave - 1);
+    const AdjList list = adj_list_counts(b, 0);
+    const int idx = lane >> SH;  // the element this lane ends with: w * 8 + p
+    const int ww = idx >> 3;
+    const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int xcd = static_cast<int>(blockIdx.x % kXcds);
+    const int64_t f0 = static_cast<int64_t>(blockIdx.x / kXcds) * kWavesPerBlock + wave;
+    const int64_t fs = static_cast<int64_t>(gridDim.x / kXcds) * kWavesPerBlock;
+    const int P = b.net[0].long_parts;
+    const int64_t total = list.total;
+    const int64_t F = static_cast<int64_t>(P / kXcds) * total;
+    for (int64_t fb = f0; fb < F; fb += kWave * fs) {
+        // lane l: item fb + l fs -- its list index, range, network, source and run bounds
+        const int64_t fl = fb + static_cast<int64_t>(lane) * fs;
+        int i_l = 0, part_l = 0, q_l = 0, jj_l = 0;
+        int64_t e0_l = 0, e1_l = 0;
+        if (fl < F) {
+            const int m = static_cast<int>(fl / total);
+            i_l = static_cast<int>(fl - static_cast<int64_t>(m) * total);
+            part_l = xcd + kXcds * m;
+            const uint32_t ent = adj_list_entry_lane(b, list, 0, i_l);
+            q_l = static_cast<int>((ent >> 17) & 15u);
+            jj_l = static_cast<int>(ent & 0x1ffffu);
+            const int64_t seg = static_cast<int64_t>(jj_l) * b.net[q_l].long_parts + part_l;
+            e0_l = b.net[q_l].long_ptr[seg];
+            e1_l = b.net[q_l].long_ptr[seg + 1];
+        }
+        const int n_items = static_cast<int>(min(static_cast<int64_t>(kWave), (F - fb + fs - 1) / fs));
+        auto rd64 = [](int64_t v, int k) {
+            const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), k));
+            const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), k));
+            return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+        };
+        // item k's first pass of link records, kUnroll a lane, as loaded: the loads are unconditional -- a lane
+        // past the run reads the network's first run bound, 8 valid bytes -- so that no branch hides from the
+        // compiler how many loads are in flight, and nothing uses them before `pass` (which pads the records
+        // past the run) so that they stay in flight meanwhile.
+        auto first_pass = [&](int k, uint2 (&l)[kUnroll]) {
+            const int q = __builtin_amdgcn_readlane(q_l, k);
+            const int64_t e0 = rd64(e0_l, k), e1 = rd64(e1_l, k);
+            const uint2* links = reinterpret_cast<const uint2*>(b.net[q].long_links);
+            const uint2* valid = reinterpret_cast<const uint2*>(b.net[q].long_ptr);
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) {
+                const int64_t er = e0 + lane + r * kWave;
+                l[r] = adj_load_link(er < e1 ? links + er : valid);
+            }
+        };
+        uint2 cur[kUnroll], nxt[kUnroll];
+        first_pass(0, cur);
+        for (int k = 0; k < n_items; ++k) {
+            const int q = __builtin_amdgcn_readlane(q_l, k);
+            const int i = __builtin_amdgcn_readlane(i_l, k);
+            const int part = __builtin_amdgcn_readlane(part_l, k);
+            const int64_t e0 = rd64(e0_l, k), e1 = rd64(e1_l, k);
+            const n2h_adj_net& a = b.net[q];
+            const int R = a.ring_slots, RS = R + W - 1;
+            const uint2* links = reinterpret_cast<const uint2*>(a.long_links);
+            float bins[V];
+#pragma unroll
+            for (int t = 0; t < V; ++t) bins[t] = 0.0f;
+            // one pass of kUnroll records a lane: ring rows, then the sums (records at or past e1 become the
+            // padding {0, 0}: target 0, delay 1, weight +0 -- an in-bounds row, never added)
+            auto pass = [&](int64_t e, const uint2 (&raw)[kUnroll], auto prefetch) {
+                uint2 l[kUnroll];
+#pragma unroll
+                for (int r = 0; r < kUnroll; ++r) l[r] = e + r * kWave < e1 ? raw[r] : make_uint2(0u, 0u);
+                float gw[kUnroll][W];
+                uint32_t tp[kUnroll];
+#pragma unroll
+                for (int r = 0; r < kUnroll; ++r) adj_load_window<W>(a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]), gw[r]);
+                // the next item's first pass, issued behind these ring-row loads by every lane and on every
+                // path (the last item loads its own again), so that the compiler counts it in flight
+                if constexpr (decltype(prefetch)::value) {
+                    first_pass(min(k + 1, n_items - 1), nxt);
+                    __builtin_amdgcn_sched_barrier(0);  // the scheduler would sink these loads below the sums
+                }
+#pragma unroll
+                for (int r = 0; r < kUnroll; ++r) {
+                    const bool live = e + r * kWave < e1;
+                    const float wt = __uint_as_float(l[r].y);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const float c = wt * gw[r][w];
+#pragma unroll
+                        for (int p = 0; p < kAdjPops; ++p)
+                            bins[w * kAdjPops + p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c
+                                                                                                  : bins[w * kAdjPops + p];
+                    }
+                }
+            };
+            pass(e0 + lane, cur, std::true_type{});  // the first pass, every lane, its records loaded ahead
+            // later passes: lanes with records left, loaded here
+            for (int64_t e = e0 + lane + kUnroll * kWave; e < e1; e += kUnroll * kWave) {
+                uint2 l[kUnroll];
+#pragma unroll
+                for (int r = 0; r < kUnroll; ++r) {
+                    const int64_t er = e + r * kWave;
+                    l[r] = er < e1 ? adj_load_link(links + er) : make_uint2(0u, 0u);
+                }
+                pass(e, l, std::false_type{});
+            }
+            const float v = (e1 > e0) ? adj_reduce_halving<V>(bins, lane) : 0.0f;  // uniform branch
+            if (writer) b.partial[(static_cast<int64_t>(i) * P + part) * V + idx] = v;
+#pragma unroll
+            for (int r = 0; r < kUnroll; ++r) cur[r] = nxt[r];
         }
     }
 }
@@ -702,7 +826,7 @@ void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
         return;
     }
     if (b.partial) {  // the ranges split over the XCDs, then added in order
-        hipLaunchKernelGGL((adj_long_kernel<W, true, false>), dim3(kSplitBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_long_pipe_kernel<W>), dim3(kPipeBlocks), block, 0, stream, b);
         hipLaunchKernelGGL((adj_combine_kernel<W, false>), dim3(kShortBlocks), block, 0, stream, b);
         return;
     }
