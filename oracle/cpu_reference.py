@@ -20,9 +20,11 @@ import numpy as np
 import torch
 
 
-def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps: int, threads: int) -> dict:
+def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps: int, threads: int,
+        record: bool = False) -> dict:
     """Time n_steps steps. model 1: delta LIF, 2: exponential synapse. params: the populations'
-    n2h_pop_params; lam: Poisson mean per population and step; canon: the canonical CSR (CPU)."""
+    n2h_pop_params; lam: Poisson mean per population and step; canon: the canonical CSR (CPU).  record: also
+    return every spike as (step, neuron) in "spike_steps" / "spike_ids" (statistics such as CV-ISI)."""
     torch.set_num_threads(int(threads))
     f32 = torch.float32
     pf = np.asarray(pop_first, dtype=np.int64)
@@ -62,6 +64,7 @@ def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps
     I = torch.as_tensor(np.asarray(i0, dtype=np.float32)).clone() if model == 2 else None
     R = torch.zeros(n, dtype=torch.int32)
     per_neuron = torch.zeros(n, dtype=torch.int64)
+    rec_steps, rec_ids = [], []
     n_spk = 0
     n_ev = 0
     t0 = time.perf_counter()
@@ -88,6 +91,13 @@ def run(model: int, pop_first, params, lam, canon, d_slots: int, v0, i0, n_steps
             n_spk += int(idx.numel())
             n_ev += int(out_deg[idx].sum())
             per_neuron[idx] += 1
+            if record:
+                rec_steps.append(np.full(idx.numel(), s, dtype=np.int64))
+                rec_ids.append(idx.numpy().astype(np.int64))
     sec = time.perf_counter() - t0
-    return {"seconds": sec, "steps": n_steps, "spikes": n_spk, "events": n_ev, "threads": int(threads),
-            "per_neuron": per_neuron.numpy()}
+    out = {"seconds": sec, "steps": n_steps, "spikes": n_spk, "events": n_ev, "threads": int(threads),
+           "per_neuron": per_neuron.numpy()}
+    if record:
+        out["spike_steps"] = np.concatenate(rec_steps) if rec_steps else np.zeros(0, np.int64)
+        out["spike_ids"] = np.concatenate(rec_ids) if rec_ids else np.zeros(0, np.int64)
+    return out
