@@ -129,7 +129,10 @@ def _lambdas(rt) -> list:
 def timed_run(net, rt, n_steps: int, window: int, out_deg: torch.Tensor, kind: str) -> dict:
     """Exactly n_steps steps from the current state, every graph captured beforehand: wall time between two
     device synchronizations, HIP events on the launch stream, delivered links and spikes, the roofline."""
-    gw = n_steps if n_steps <= 2000 else window  # short runs: one graph of exactly n_steps
+    # the runtime's own schedule (Network.run): graph windows of `window` steps, eager steps before the first
+    # window's phase and after the last; a run shorter than a window is all eager (a graph launch costs more
+    # host time than the few launches it replaces); window 0: eager launches only
+    gw = window
     rt.prepare(n_steps, gw)
     s0 = rt.step
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -867,8 +870,9 @@ def main() -> int:
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be positive and --warmup non-negative", file=sys.stderr)
         return 2
-    if args.window <= 0 or args.window % 4:
-        print("--window must be a positive multiple of 4 (the spike lists' phase)", file=sys.stderr)
+    if args.window < 0 or args.window % 4:
+        print("--window must be 0 (eager launches) or a positive multiple of 4 (the spike lists' phase)",
+              file=sys.stderr)
         return 2
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
