@@ -624,6 +624,49 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(Adj
 
 __device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a, int j, const float (&mine)[kAdjPops]);
 
+// The update of a gatherer j (adj_update_neuron's arithmetic, in its order) from inputs its wavefront loaded
+// ahead: lane p < 8 holds its population's total tot_l (short + long sums), lane c < 8 the coefficient C[c]
+// of j's population pj, lane p < n_pops alpha[p][pj] and j's compensated dL/dalpha pair; vp, spk, gv, gi
+// are uniform.  The coupling sum runs over p in order, as there; lane p updates its dL/dalpha pair; lane 0
+// writes the state and the adjoint rings.
+__device__ inline void adj_update_gatherer(const AdjBatch& b, const n2h_adj_net& a, int j, float tot_l, float coef,
+                                           float alpha_l, float2 gs_l, float vp, bool spk, float gv, float gi,
+                                           int lane) {
+    const int np = a.n_pops;
+    float coup = 0.0f;
+    for (int p = 0; p < np; ++p) coup += __shfl(alpha_l, p, kWave) * __shfl(tot_l, p, kWave);
+    if (spk && lane < np) {  // compensated fp32 accumulation (hi, lo) over the reverse steps
+        const n2h_twofloat::df t = n2h_twofloat::two_sum(gs_l.x, tot_l);
+        gs_l.x = t.hi;
+        gs_l.y = gs_l.y + t.lo;
+        reinterpret_cast<float2*>(a.g_sum)[static_cast<int64_t>(j) * np + lane] = gs_l;
+    }
+    const float P_V = __shfl(coef, 0, kWave), P_I = __shfl(coef, 1, kWave), P_VI = __shfl(coef, 2, kWave);
+    const float dv = __shfl(coef, 5, kWave), c_loss = __shfl(coef, 6, kWave);
+    if (lane != 0) return;
+    const float u = adj_u(vp, dv);
+    const bool near = u <= b.sur.cutoff;
+    float gip, gv_new;
+    if (isfinite(vp)) {
+        const float gsum = c_loss + coup;
+        const float den = 1.0f + b.sur.kappa * u;
+        const float h = near ? (b.sur.gamma * 0.5f * b.sur.kappa / dv) / (den * den) : 0.0f;
+        const float gvpre = (spk ? 0.0f : gv) + h * gsum;
+        gip = P_VI * gvpre + P_I * gi;
+        gv_new = P_V * gvpre;
+    } else {
+        gip = P_I * gi;
+        gv_new = 0.0f;
+    }
+    const int R = a.ring_slots, RS = R + a.long_delay - 1;
+    a.g_v[j] = gv_new;
+    a.g_i[j] = gip;
+    float* row = a.gx_ring + static_cast<int64_t>(j) * RS;
+    row[b.ring_pos] = gip;
+    if (b.ring_pos < a.long_delay - 1) row[b.ring_pos + R] = gip;  // the mirror
+    a.gx_recent[static_cast<int64_t>(b.recent_pos) * a.n_neurons + j] = gip;
+}
+
 // One reverse step in one launch.  Workgroups [0, kShortBlocks) take this step's gatherers, a wavefront
 // each: their short links (lanes stride the list, per-lane sums per target population, then a fixed
 // reduction), plus the window's long sums (short + long), then lane 0 updates the neuron.  The other
@@ -649,11 +692,26 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
         const uint32_t ent = adj_list_entry(b, list, 1 + b.win_step, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const n2h_adj_net& a = b.net[q];
-        const int W = a.long_delay, n = a.n_neurons;
+        const int W = a.long_delay, n = a.n_neurons, np = a.n_pops;
         float bins[kAdjPops];
 #pragma unroll
         for (int p = 0; p < kAdjPops; ++p) bins[p] = 0.0f;
+        // The run bounds and, in the same round, every input of the neuron's update, spread over the lanes,
+        // so that after the sums only arithmetic and stores are left: lane p's population start (the
+        // population is the count of starts <= jj), then the population's coefficients (lane c < 8) and
+        // its alpha column (lane p: alpha[p][pj]); the long sums' row, the compensated dL/dalpha pair
+        // (lane p), and the trace, spike flag and adjoint state.
         const int64_t e0 = a.out_ptr[jj], e1 = a.out_ptr[jj + 1];
+        const bool starts_before = lane >= 1 && lane < np && a.pop_first[lane] <= jj;
+        const float lb = a.long_bins[static_cast<int64_t>(jj) * kAdjPops + (lane & 7)];
+        const float vp = a.vtrace[jj];
+        const bool spk = a.spikes[jj] != 0;
+        const float gv = a.g_v[jj], gi = a.g_i[jj];
+        float2 gs_l = make_float2(0.0f, 0.0f);
+        if (lane < np) gs_l = reinterpret_cast<const float2*>(a.g_sum)[static_cast<int64_t>(jj) * np + lane];
+        const int pj = __popcll(__ballot(starts_before));
+        const float coef = a.pop_coef[pj * 8 + (lane & 7)];
+        const float alpha_l = lane < np ? a.alpha[lane * np + pj] : 0.0f;
         const uint2* links = reinterpret_cast<const uint2*>(a.out_links);
         for (int64_t e = e0 + lane; e < e1; e += kUnroll * kWave) {  // as in adj_long_kernel
             uint2 l[kUnroll];
@@ -676,11 +734,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
             }
         }
         const float v = (e1 > e0) ? adj_reduce_halving<kAdjPops>(bins, lane) : 0.0f;  // lane l: population l >> 3
-        const float* row = a.long_bins + static_cast<int64_t>(jj) * kAdjPops;
-        float tot[kAdjPops];
-#pragma unroll
-        for (int p = 0; p < kAdjPops; ++p) tot[p] = __shfl(v, p * 8, kWave) + row[p];  // short + long
-        if (lane == 0) adj_update_neuron(b, a, jj, tot);
+        // lane p < 8: its population's total, short + long (the long row prefetched)
+        const float tot_l = __shfl(v, (lane & 7) * 8, kWave) + lb;
+        adj_update_gatherer(b, a, jj, tot_l, coef, alpha_l, gs_l, vp, spk, gv, gi, lane);
     }
 }
 
