@@ -271,9 +271,15 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
             raise SystemExit(f"--timestep: no population {name!r} (have {sorted(pops)})")
         pops[name].timestep = timesteps[name] = float(ms)
     rt = net.build()
-    rt.set_launch_mapping(args.xcd, args.xcd_run)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
+    if args.xcd is None:  # the launch mapping from the autotuner (untimed, the state restored afterwards)
+        from naer2_amd.autotune import tune_plan
+        mapping, map_times = tune_plan(rt, steps=200, graph_window=args.window)
+        log(f"launch mapping (autotuned): {mapping} from {map_times}")
+    else:
+        rt.set_launch_mapping(args.xcd, args.xcd_run)
+        mapping, map_times = {"xcd": args.xcd, "run": args.xcd_run}, None
     log(f"built {kind}: {rt.n} neurons, {rt.canon.n_links} synapses, {rt.n_groups} groups, {rt.d_slots} ring "
         f"slots in {t_build:.1f} s")
     net.record = True
@@ -334,7 +340,8 @@ def measure_plain(args, kind: str, steps: int, warmup: int, cpu_leg: bool, cpu: 
                                f"{warmup} warmup steps",
                    "n_neurons": rt.n, "n_synapses": rt.canon.n_links, "dt_ms": rt.dt,
                    **({"timesteps_ms": timesteps} if timesteps else {}),
-                   "graph_window": m["graph_window"], "launch_mapping": {"xcd": args.xcd, "run": args.xcd_run},
+                   "graph_window": m["graph_window"],
+                   "launch_mapping": {**mapping, **({"autotune_s": map_times} if map_times else {})},
                    "parallelism": "single GPU"},
         "events_per_s": round(m["events_per_s"], 1),
         "spikes_per_step": round(m["spikes"] / steps, 3),
@@ -849,9 +856,9 @@ def main() -> int:
                     help="pd14grad: surrogate support |V - theta| / (theta - V_reset) (sparse gradient)")
     ap.add_argument("--gamma", type=float, default=0.01,
                     help="pd14grad: surrogate scale (0.1 and above diverge over 500 ms at full scale)")
-    ap.add_argument("--xcd", type=int, default=8,
+    ap.add_argument("--xcd", type=int, default=None,
                     help="pd14/brunel: launch mapping of the step (n2h_set_tuning): 0 block b runs group b; k > 1 "
-                         "consecutive groups on each of k XCDs")
+                         "consecutive groups on each of k XCDs; default: the autotuner's choice (autotune.tune_plan)")
     ap.add_argument("--xcd-run", type=int, default=0,
                     help="pd14/brunel: with --xcd k > 1, runs of this many consecutive groups dealt to the XCDs in "
                          "turn (0: one contiguous chunk per XCD)")

@@ -16,6 +16,9 @@ lean batched stage: 128 sources and 384 links a pass, 4 row cells in flight, so 
 (the single-plan stage: 256, 512 and 8 -- fewer passes and more row cells in flight a group).  Every
 candidate runs from the same snapshot of the state, which is restored afterwards.
 
+tune_plan picks a single network's step launch mapping (tile order, one chunk per XCD or runs of 16 groups
+per XCD) by timing each on the network's own steps from one snapshot of the state.
+
 tune_intake picks, per network, the spike count from which a step takes the row intake (every in-link
 walked against the spike bitmap) instead of the segment intake (the spiking sources' segments): both give
 bit-identical results, and which is faster depends on the in-degree and the activity regime, so each
@@ -118,6 +121,33 @@ def tune(runtimes, steps: int = 200, graph_window: int = 100, reserve_steps: int
     if use_cache:
         table[key] = {"best": best, "times": times}
         _save(table)
+    return best, times
+
+
+# launch mappings (xcd, run) of a single network's step kernels (n2h_set_tuning): blocks in group order;
+# one contiguous chunk of the groups per XCD (the runtime's default); runs of 16 groups dealt to the XCDs
+PLAN_MAPPINGS = ((0, 0), (8, 0), (8, 16))
+
+
+def tune_plan(rt, steps: int = 200, graph_window: int = 100, mappings=PLAN_MAPPINGS):
+    """Set rt's step launch mapping (Runtime.set_launch_mapping) to the fastest of `mappings`, each timed
+    over `steps` steps from the same snapshot of the state, which is restored afterwards (recording
+    restarts there); every mapping gives the same results.  Returns ({"xcd", "run"}, {mapping: seconds})."""
+    rt._ensure_slots(max(steps, graph_window) + 4)  # the recorded history must not grow under the snapshot
+    snap = rt.snapshot()
+    times = {}
+    for xcd, run in mappings:
+        rt.set_launch_mapping(xcd, run)
+        rt.restore(snap)
+        rt.prepare(steps, graph_window)  # graphs captured untimed
+        torch.cuda.synchronize(rt.device)
+        t0 = time.perf_counter()
+        rt.run(steps, graph_window)
+        torch.cuda.synchronize(rt.device)
+        times[json.dumps({"xcd": xcd, "run": run}, sort_keys=True)] = time.perf_counter() - t0
+    best = json.loads(min(times, key=times.get))
+    rt.set_launch_mapping(best["xcd"], best["run"])
+    rt.restore(snap)
     return best, times
 
 

@@ -11,7 +11,7 @@ import torch
 
 from naer2_amd import FusedRound
 from naer2_amd.autotune import CANDIDATES, measure_work, tune, tune_intake
-from naer2_amd.networks import brunel_sweep, sweep_params
+from naer2_amd.networks import brunel_sweep, pd14, sweep_params
 from tests.gpu_util import assert_state_equal, bits_equal, oracle_run, oracle_state
 
 pytestmark = pytest.mark.gpu
@@ -113,6 +113,29 @@ def test_intake_tuning_restores_the_state_and_changes_no_result():
     assert np.array_equal(ra.raster(40, 240), rb.raster(40, 240))
     assert ra.raster(40, 240).sum(axis=1).max() >= 200  # bursts: the thresholds mattered
     assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert np.array_equal(ra.ref.cpu().numpy(), rb.ref.cpu().numpy())
+    assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
+
+
+def test_plan_mapping_tuning_restores_the_state_and_changes_no_result():
+    """tune_plan times the single-network launch mappings from one snapshot and restores it: the run that
+    follows equals an untuned run bit for bit (every mapping gives identical results)."""
+    from naer2_amd.autotune import PLAN_MAPPINGS, tune_plan
+    _say("case: launch-mapping tuning, pd14 scale=0.05")
+    a, _ = pd14(scale=0.05, seed=6, device="cuda")
+    b, _ = pd14(scale=0.05, seed=6, device="cuda")
+    a.run(40)
+    b.run(40)
+    best, times = tune_plan(a.runtime, steps=60, graph_window=20)
+    assert (best["xcd"], best["run"]) in PLAN_MAPPINGS and len(times) == len(PLAN_MAPPINGS)
+    assert a.runtime.step == b.runtime.step == 40
+    a.run(200)
+    b.run(200)
+    torch.cuda.synchronize()
+    ra, rb = a.runtime, b.runtime
+    assert np.array_equal(ra.raster(40, 240), rb.raster(40, 240)) and ra.raster(40, 240).sum() > 0
+    assert bits_equal(ra.v.cpu().numpy(), rb.v.cpu().numpy())
+    assert bits_equal(ra.isyn.cpu().numpy(), rb.isyn.cpu().numpy())
     assert np.array_equal(ra.ref.cpu().numpy(), rb.ref.cpu().numpy())
     assert bits_equal(ra.ring.cpu().numpy(), rb.ring.cpu().numpy())
 
