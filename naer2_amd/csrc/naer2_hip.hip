@@ -820,20 +820,14 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
         for (int r0 = 0; r0 < total; r0 += St::kS) {
             const int r1 = min(total, r0 + St::kS);
             if (excl < r1 && excl + cnt > r0) {
+                // this lane's words in order: each word's set bits ranked from the lane's scan offset, the
+                // pass's [r0, r1) stored
                 int rank = excl;
-                for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) {
-                    unsigned long long m;
-                    if (KW > 0) {
-                        m = 0ull;
-#pragma unroll
-                        for (int z = 0; z < (KW > 0 ? KW : 1); ++z) m = (z == q) ? wd[z] : m;
-                    } else {
-                        m = prev[w_lo + q];
-                    }
+                auto take = [&](unsigned long long m, int q) {
                     const int pc = __popcll(m);
                     if (rank + pc <= r0) {
                         rank += pc;
-                        continue;
+                        return;
                     }
                     const int base = (w_lo + q) * kWave;
                     while (m) {
@@ -842,6 +836,21 @@ __device__ __forceinline__ void step_group(const StepArgs& a, const int g, const
                         ++rank;
                         m &= m - 1ull;
                     }
+                };
+                if constexpr (KW > 0 && KW <= 16) {
+                    // q is wave-uniform: the word is read by indexed register moves, no per-word select
+#pragma unroll 1
+                    for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) take(wd[q], q);
+                } else if constexpr (KW > 0) {
+                    // wider arrays would go to scratch memory when indexed: the word is selected
+                    for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) {
+                        unsigned long long m = 0ull;
+#pragma unroll
+                        for (int z = 0; z < KW; ++z) m = (z == q) ? wd[z] : m;
+                        take(m, q);
+                    }
+                } else {
+                    for (int q = 0; q < w_hi - w_lo && rank < r1; ++q) take(prev[w_lo + q], q);
                 }
             }
             __syncthreads();
