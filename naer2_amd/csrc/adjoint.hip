@@ -180,17 +180,23 @@ constexpr int adj_halving_shift() {  // 6 - log2 V: lane l ends with element l >
 // neuron, and each window step's, net << 17 | neuron), and the gather kernels give each list entry a whole
 // wavefront in grid-stride order.  List order depends on the schedule, but every entry's sums are its own,
 // in a fixed order, so results do not.  Each list (which = 0: the window's, 1 + w: step w's) is kept as
-// kSubLists sub-lists, a wavefront appending to sub-list (its index mod kSubLists), each counter on a cache
-// line of its own: appends to one counter serialize in the L2.
+// kSubLists sub-lists, each counter on a cache line of its own: appends to one counter serialize in the L2.
+// The wavefronts of mark workgroup b append to sub-lists b mod 8 and 8 + b mod 8 (by their index's parity),
+// and the step kernel's workgroups on XCD x walk sub-lists x and 8 + x: a neuron is then gathered on the
+// same XCD in every step, the XCD that also updates it when it does not gather, so that its short links
+// and state, read again in the steps it stays near the threshold, are in that XCD's L2.  (Workgroup b runs
+// on XCD b mod 8: the hardware deals a launch's workgroups to the 8 XCDs in turn.)
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
 constexpr int kPipeBlocks = 1024;    // its pipelined form: 4 blocks per CU (117 VGPRs: 4 wavefronts a SIMD)
 constexpr int kGroupLanes = 8;       // the population-pure long kernel: lanes a (gatherer, range) task
 constexpr int kGroupBlocks = 1536;   // its workgroups: 6 blocks per CU (74 VGPRs at W = 4: 6 wavefronts per SIMD)
-static_assert(kSplitBlocks % 8 == 0 && kGroupBlocks % 8 == 0 && kPipeBlocks % 8 == 0,
-              "the split long kernels deal workgroups to 8 XCDs in turn");
-constexpr int kSubLists = 16;
+constexpr int kXcds = 8;
+static_assert(kSplitBlocks % kXcds == 0 && kGroupBlocks % kXcds == 0 && kPipeBlocks % kXcds == 0 &&
+                  kShortBlocks % kXcds == 0,
+              "the split long kernels and the step kernel deal workgroups to the XCDs in turn");
+constexpr int kSubLists = 2 * kXcds;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
 
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
         }
     }
     const uint64_t below = (1ull << lane) - 1ull;
-    const int sub = static_cast<int>((blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kSubLists);
+    const int sub = static_cast<int>(blockIdx.x % kXcds) + kXcds * static_cast<int>((threadIdx.x >> 6) & 1u);
     for (int which = 0; which <= b.n_win; ++which) {
         const bool in = which == 0 ? mask != 0 : ((mask >> (which - 1)) & 1u) != 0;
         const uint64_t bal = __ballot(in);
@@ -313,7 +319,6 @@ constexpr int adj_win_pad() { return W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8; }
 // that order, so each XCD's L2 holds the ring rows of one range at a time and no two L2s hold the same
 // rows; the range sums go to `partial`, which adj_combine_kernel adds in range order (the same sums, in the
 // same order, as the launch-per-range path).
-constexpr int kXcds = 8;
 
 // PURE (with SPLIT; the caller's range_pop given): every range holds targets of one population only, so a
 // task keeps W sums per lane instead of 8 W, without selects, and its partial row is [W'] for that population.
@@ -686,10 +691,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
         adj_update_neuron(b, a, j, none);
         return;
     }
-    const int n_waves = kShortBlocks * kWavesPerBlock;
-    const AdjList list = adj_list_counts(b, 1 + b.win_step);
-    for (int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6); i < list.total; i += n_waves) {
-        const uint32_t ent = adj_list_entry(b, list, 1 + b.win_step, i);
+    // this XCD's sub-lists x and 8 + x of the step's list, in that order
+    const int x = static_cast<int>(blockIdx.x % kXcds);
+    const int n_waves = kShortBlocks / kXcds * kWavesPerBlock;
+    const int which = 1 + b.win_step;
+    const int c0 = b.work[adj_counter(which, x)], c1 = b.work[adj_counter(which, x + kXcds)];
+    for (int i = static_cast<int>(blockIdx.x / kXcds) * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6);
+         i < c0 + c1; i += n_waves) {
+        const uint32_t ent = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(
+            i < c0 ? b.work[adj_sublist(b, which, x) + i] : b.work[adj_sublist(b, which, x + kXcds) + (i - c0)]));
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const n2h_adj_net& a = b.net[q];
         const int W = a.long_delay, n = a.n_neurons, np = a.n_pops;
