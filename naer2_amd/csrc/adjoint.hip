@@ -189,7 +189,9 @@ constexpr int adj_halving_shift() {  // 6 - log2 V: lane l ends with element l >
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
-constexpr int kPipeBlocks = 1024;    // its pipelined form: 4 blocks per CU (117 VGPRs: 4 wavefronts a SIMD)
+constexpr int kPipeBlocks = 768;     // its pipelined form: 3 blocks per CU (108 VGPRs at W = 4 would fit
+                                     // 4; with 3 fewer items' ring rows share each L2, measured faster)
+constexpr int kPipeBins = 4;         // its population bins a lane per window step
 constexpr int kGroupLanes = 8;       // the population-pure long kernel: lanes a (gatherer, range) task
 constexpr int kGroupBlocks = 1536;   // its workgroups: 6 blocks per CU (74 VGPRs at W = 4: 6 wavefronts per SIMD)
 constexpr int kXcds = 8;
@@ -398,20 +400,25 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 }
 
 // The split long kernel (all ranges in one launch over the XCDs, equal ranges), with its items pipelined in
-// each wavefront: the wavefront's next 64 items' list entries and run bounds are loaded at once (an item a
-// lane, read back uniform), and the first pass of the next item's link records is issued right after the
-// current item's ring-row loads, so that its HBM round trip runs under the current item's sums and
-// reduction.  Every item's sums are those of adj_long_kernel<W, true, false>, in the same order.
-template <int W>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipe_kernel(AdjBatch b) {
+// each wavefront: the wavefront's next 64 items' list entries, run bounds and populations are loaded at once
+// (an item a lane, read back uniform), and the first pass of the next item's link records is issued right
+// after the current item's ring-row loads, so that its HBM round trip runs under the current item's sums and
+// reduction.  A lane keeps K population bins per window step instead of 8: a target range (range k of a
+// network: targets t with t * long_parts / n = k) spans few populations, populations being contiguous in the
+// neuron order, so an item sums into the bins of populations [g0, g0 + K) of its range's span, one sweep over
+// its links for each K populations of the span (one sweep for every range of PD14 at 24 ranges).  A bin's
+// sum takes its population's links in link order, and the partial row holds +0 for populations outside the
+// range, as adj_long_kernel<W, true, false> computes them.
+template <int W, int K>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipek_kernel(AdjBatch b) {
     constexpr int WP = adj_win_pad<W>();
-    constexpr int V = kAdjPops * WP;
+    constexpr int V = K * WP;
     constexpr int SH = adj_halving_shift<V>();
     const int lane = threadIdx.x & (kWave - 1);
     const AdjList list = adj_list_counts(b, 0);
-    const int idx = lane >> SH;  // the element this lane ends with: w * 8 + p
-    const int ww = idx >> 3;
-    const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
+    // this lane's slot of a partial row: window step ws, population ps (lanes below 8 W write)
+    const int ws = lane >> 3, ps = lane & 7;
+    const bool writer = ws < W;
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int xcd = static_cast<int>(blockIdx.x % kXcds);
     const int64_t f0 = static_cast<int64_t>(blockIdx.x / kXcds) * kWavesPerBlock + wave;
@@ -420,9 +427,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipe_kernel(A
     const int64_t total = list.total;
     const int64_t F = static_cast<int64_t>(P / kXcds) * total;
     for (int64_t fb = f0; fb < F; fb += kWave * fs) {
-        // lane l: item fb + l fs -- its list index, range, network, source and run bounds
+        // lane l: item fb + l fs -- its list index, range, network, source, run bounds and the range's
+        // populations [plo_l, phi_l]
         const int64_t fl = fb + static_cast<int64_t>(lane) * fs;
-        int i_l = 0, part_l = 0, q_l = 0, jj_l = 0;
+        int i_l = 0, part_l = 0, q_l = 0, plo_l = 0, phi_l = 0;
         int64_t e0_l = 0, e1_l = 0;
         if (fl < F) {
             const int m = static_cast<int>(fl / total);
@@ -430,10 +438,23 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipe_kernel(A
             part_l = xcd + kXcds * m;
             const uint32_t ent = adj_list_entry_lane(b, list, 0, i_l);
             q_l = static_cast<int>((ent >> 17) & 15u);
-            jj_l = static_cast<int>(ent & 0x1ffffu);
-            const int64_t seg = static_cast<int64_t>(jj_l) * b.net[q_l].long_parts + part_l;
-            e0_l = b.net[q_l].long_ptr[seg];
-            e1_l = b.net[q_l].long_ptr[seg + 1];
+            const int jj = static_cast<int>(ent & 0x1ffffu);
+            const n2h_adj_net& aq = b.net[q_l];
+            const int64_t seg = static_cast<int64_t>(jj) * aq.long_parts + part_l;
+            e0_l = aq.long_ptr[seg];
+            e1_l = aq.long_ptr[seg + 1];
+            const int64_t n = aq.n_neurons;
+            const int t0 = static_cast<int>((part_l * n + P - 1) / P);
+            const int t1 = static_cast<int>(((part_l + 1) * n + P - 1) / P) - 1;  // last target (t1 < t0: none)
+#pragma unroll
+            for (int p = 1; p < kAdjPops; ++p) {
+                if (p < aq.n_pops) {
+                    const int first = aq.pop_first[p];
+                    plo_l += first <= t0 ? 1 : 0;
+                    phi_l += first <= t1 ? 1 : 0;
+                }
+            }
+            phi_l = max(phi_l, plo_l);
         }
         const int n_items = static_cast<int>(min(static_cast<int64_t>(kWave), (F - fb + fs - 1) / fs));
         auto rd64 = [](int64_t v, int k) {
@@ -462,56 +483,78 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipe_kernel(A
             const int q = __builtin_amdgcn_readlane(q_l, k);
             const int i = __builtin_amdgcn_readlane(i_l, k);
             const int part = __builtin_amdgcn_readlane(part_l, k);
+            const int plo = __builtin_amdgcn_readlane(plo_l, k), phi = __builtin_amdgcn_readlane(phi_l, k);
             const int64_t e0 = rd64(e0_l, k), e1 = rd64(e1_l, k);
             const n2h_adj_net& a = b.net[q];
             const int R = a.ring_slots, RS = R + W - 1;
             const uint2* links = reinterpret_cast<const uint2*>(a.long_links);
-            float bins[V];
+            float out = 0.0f;  // this lane's slot (ws, ps); +0 for populations outside the range
+            // one sweep over the item's links, the sums of populations [g0, g0 + K); FIRST: the first sweep,
+            // whose first pass takes the records loaded ahead and issues the next item's
+            auto sweep = [&](int g0, auto first) {
+                float bins[V];
 #pragma unroll
-            for (int t = 0; t < V; ++t) bins[t] = 0.0f;
-            // one pass of kUnroll records a lane: ring rows, then the sums (records at or past e1 become the
-            // padding {0, 0}: target 0, delay 1, weight +0 -- an in-bounds row, never added)
-            auto pass = [&](int64_t e, const uint2 (&raw)[kUnroll], auto prefetch) {
-                uint2 l[kUnroll];
+                for (int t = 0; t < V; ++t) bins[t] = 0.0f;
+                // one pass of kUnroll records a lane: ring rows, then the sums (records at or past e1 become
+                // the padding {0, 0}, never added)
+                auto pass = [&](int64_t e, const uint2 (&raw)[kUnroll], auto prefetch) {
+                    uint2 l[kUnroll];
 #pragma unroll
-                for (int r = 0; r < kUnroll; ++r) l[r] = e + r * kWave < e1 ? raw[r] : make_uint2(0u, 0u);
-                float gw[kUnroll][W];
-                uint32_t tp[kUnroll];
+                    for (int r = 0; r < kUnroll; ++r) l[r] = e + r * kWave < e1 ? raw[r] : make_uint2(0u, 0u);
+                    float gw[kUnroll][W];
+                    uint32_t tp[kUnroll];
 #pragma unroll
-                for (int r = 0; r < kUnroll; ++r) adj_load_window<W>(a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]), gw[r]);
-                // the next item's first pass, issued behind these ring-row loads by every lane and on every
-                // path (the last item loads its own again), so that the compiler counts it in flight
-                if constexpr (decltype(prefetch)::value) {
-                    first_pass(min(k + 1, n_items - 1), nxt);
-                    __builtin_amdgcn_sched_barrier(0);  // the scheduler would sink these loads below the sums
-                }
-#pragma unroll
-                for (int r = 0; r < kUnroll; ++r) {
-                    const bool live = e + r * kWave < e1;
-                    const float wt = __uint_as_float(l[r].y);
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        const float c = wt * gw[r][w];
-#pragma unroll
-                        for (int p = 0; p < kAdjPops; ++p)
-                            bins[w * kAdjPops + p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c
-                                                                                                  : bins[w * kAdjPops + p];
+                    for (int r = 0; r < kUnroll; ++r)
+                        adj_load_window<W>(a.gx_ring + adj_link_slot(l[r].x, b.ring_pos, R, RS, tp[r]), gw[r]);
+                    // the next item's first pass, issued behind these ring-row loads by every lane and on
+                    // every path (the last item loads its own again), so that the compiler counts it in flight
+                    if constexpr (decltype(prefetch)::value) {
+                        first_pass(min(k + 1, n_items - 1), nxt);
+                        __builtin_amdgcn_sched_barrier(0);  // the scheduler would sink these loads below the sums
                     }
-                }
-            };
-            pass(e0 + lane, cur, std::true_type{});  // the first pass, every lane, its records loaded ahead
-            // later passes: lanes with records left, loaded here
-            for (int64_t e = e0 + lane + kUnroll * kWave; e < e1; e += kUnroll * kWave) {
-                uint2 l[kUnroll];
 #pragma unroll
-                for (int r = 0; r < kUnroll; ++r) {
-                    const int64_t er = e + r * kWave;
-                    l[r] = er < e1 ? adj_load_link(links + er) : make_uint2(0u, 0u);
+                    for (int r = 0; r < kUnroll; ++r) {
+                        const bool live = e + r * kWave < e1;
+                        const uint32_t kk = tp[r] - static_cast<uint32_t>(g0);  // >= K: another sweep's
+                        const float wt = __uint_as_float(l[r].y);
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            const float c = wt * gw[r][w];
+#pragma unroll
+                            for (int p = 0; p < K; ++p)
+                                bins[w * K + p] = (live && kk == static_cast<uint32_t>(p)) ? bins[w * K + p] + c
+                                                                                          : bins[w * K + p];
+                        }
+                    }
+                };
+                if constexpr (decltype(first)::value) {
+                    pass(e0 + lane, cur, std::true_type{});
+                } else {
+                    uint2 l[kUnroll];
+#pragma unroll
+                    for (int r = 0; r < kUnroll; ++r) {
+                        const int64_t er = e0 + lane + r * kWave;
+                        l[r] = er < e1 ? adj_load_link(links + er) : make_uint2(0u, 0u);
+                    }
+                    pass(e0 + lane, l, std::false_type{});
                 }
-                pass(e, l, std::false_type{});
-            }
-            const float v = (e1 > e0) ? adj_reduce_halving<V>(bins, lane) : 0.0f;  // uniform branch
-            if (writer) b.partial[(static_cast<int64_t>(i) * P + part) * V + idx] = v;
+                for (int64_t e = e0 + lane + kUnroll * kWave; e < e1; e += kUnroll * kWave) {
+                    uint2 l[kUnroll];
+#pragma unroll
+                    for (int r = 0; r < kUnroll; ++r) {
+                        const int64_t er = e + r * kWave;
+                        l[r] = er < e1 ? adj_load_link(links + er) : make_uint2(0u, 0u);
+                    }
+                    pass(e, l, std::false_type{});
+                }
+                const float v = adj_reduce_halving<V>(bins, lane);  // lane l: element l >> SH = w K + p
+                const int kp = ps - g0;
+                const float mine = __shfl(v, ((ws * K + min(max(kp, 0), K - 1)) << SH) & (kWave - 1), kWave);
+                if (kp >= 0 && kp < K) out = mine;
+            };
+            sweep(plo, std::true_type{});
+            for (int g0 = plo + K; g0 <= phi; g0 += K) sweep(g0, std::false_type{});  // spans wider than K
+            if (writer) b.partial[(static_cast<int64_t>(i) * P + part) * (kAdjPops * WP) + lane] = out;
 #pragma unroll
             for (int r = 0; r < kUnroll; ++r) cur[r] = nxt[r];
         }
@@ -892,7 +935,7 @@ void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
         return;
     }
     if (b.partial) {  // the ranges split over the XCDs, then added in order
-        hipLaunchKernelGGL((adj_long_pipe_kernel<W>), dim3(kPipeBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_long_pipek_kernel<W, kPipeBins>), dim3(kPipeBlocks), block, 0, stream, b);
         hipLaunchKernelGGL((adj_combine_kernel<W, false>), dim3(kShortBlocks), block, 0, stream, b);
         return;
     }

@@ -140,6 +140,37 @@ def test_gradient_agrees_over_window_lengths():
         assert err <= 1e-5
 
 
+def test_ranges_spanning_many_populations_agree():
+    """A target range whose targets span more populations than the long kernel keeps bins for (4) is walked
+    in several sweeps, each summing 4 of its populations: with six two-neuron populations between
+    populations of 90 and 110, range 3 of 8 holds targets of all 8 populations (two sweeps) and range 7 of 16
+    of populations 2-7.  The gradient, the sums of the second sweeps included, is the single-step one to
+    fp32 rounding."""
+    T = 120
+    alpha = _alpha(6)
+    sizes = [90, 2, 2, 2, 2, 2, 2, 110]
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(first[-1])
+    span = [int(np.searchsorted(first, t, side="right")) - 1
+            for t in (-(-3 * n // 8), -(-4 * n // 8) - 1)]  # range 3: its first and last target's population
+    assert span == [0, 7]
+    targets = np.random.default_rng(3).uniform(0.5, 10.0, (1, 8))
+    sur = {"cutoff": 0.5, "gamma": 0.5}
+    res = {}
+    for W, P in ((1, 1), (4, 8), (4, 16)):
+        _say(f"case: populations {sizes}, long_delay {W}, long_parts {P}, checkpoint 50, T={T}")
+        net, _ = pd14(sizes=sizes, prob_scale=10.0, seed=5, device="cuda", weight_scale=alpha, instance=0)
+        net.record = True
+        res[W, P] = RateLossGrad([net.build()], alpha, checkpoint=50, long_delay=W, long_parts=P, **sur)(T, targets)
+    g1 = res[1, 1][1]
+    assert np.abs(g1[4:]).max() > 0  # links into populations 4-7 carry gradient
+    for key in ((4, 8), (4, 16)):
+        assert res[key][0] == res[1, 1][0]
+        err = np.linalg.norm(res[key][1] - g1) / np.linalg.norm(g1)
+        _say(f"long_delay, long_parts {key}: rel diff {err:.3g}")
+        assert err <= 1e-5
+
+
 def test_compensated_trace_changes_nothing_and_tracks_fp64_potentials():
     """With the compensation state bound, the step's potentials, currents, refractory counters, ring and
     spikes stay bit-identical to the plain run, and the traced distances to the threshold match the FP64
