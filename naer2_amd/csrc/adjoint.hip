@@ -201,6 +201,12 @@ static_assert(kSplitBlocks % kXcds == 0 && kGroupBlocks % kXcds == 0 && kPipeBlo
 constexpr int kSubLists = 2 * kXcds;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
+// entries a sub-list holds: sub-list c takes the appends of the wavefronts of parity c / kXcds (two a
+// workgroup) in the workgroups b with b mod kXcds = c mod kXcds, at most 64 each
+static_assert(kWavesPerBlock % 2 == 0, "mark workgroups append to two sub-lists by wavefront parity");
+inline int64_t adj_sub_cap(int64_t blocks) {
+    return static_cast<int64_t>(kWave) * (kWavesPerBlock / 2) * ((blocks + kXcds - 1) / kXcds);
+}
 
 __device__ inline int64_t adj_counter(int which, int c) {
     return static_cast<int64_t>(which * kSubLists + c) * kCounterStride;
@@ -892,15 +898,14 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
     b.first_block[n_nets] = static_cast<int32_t>(blocks);
     b.ring_pos = static_cast<int32_t>(step % ring);
     b.recent_pos = static_cast<int32_t>(step % win);
-    b.sub_cap = static_cast<int32_t>(kWave * ((blocks * kWavesPerBlock + kSubLists - 1) / kSubLists));
+    b.sub_cap = static_cast<int32_t>(adj_sub_cap(blocks));
     return N2H_OK;
 }
 
 // the work buffer: the counters, then 1 + long_delay lists of kSubLists sub-lists of sub_cap entries, then
 // (long_parts a multiple of kXcds) the range sums of every possible window entry
 int64_t adj_lists_words(int64_t blocks, int32_t long_delay) {
-    const int64_t sub_cap = kWave * ((blocks * kWavesPerBlock + kSubLists - 1) / kSubLists);
-    return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * sub_cap;
+    return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * adj_sub_cap(blocks);
 }
 int64_t adj_partial_floats(int64_t n_total, int32_t long_delay, int32_t long_parts) {
     if (long_parts % kXcds != 0) return 0;

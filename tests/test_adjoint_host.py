@@ -83,8 +83,11 @@ def test_work_words_follow_the_list_layout():
     sizes = [77169, 77169, 77169, 77169]
     arr = (ctypes.c_int32 * 4)(*sizes)
     blocks = sum((s + 255) // 256 for s in sizes)
-    sub_cap = 64 * ((blocks * 4 + 15) // 16)
+    # a sub-list takes the appends of two wavefronts (one parity) of every 8th mark workgroup, 64 each
+    sub_cap = 64 * 2 * ((blocks + 7) // 8)
     lists = {4: 9 * 16 * 32 + 5 * 16 * sub_cap, 8: 9 * 16 * 32 + 9 * 16 * sub_cap, 3: 9 * 16 * 32 + 4 * 16 * sub_cap}
+    one = (ctypes.c_int32 * 1)(212)  # one workgroup: its two wavefronts of a parity fill one sub-list
+    assert lib.n2h_adj_work_words(one, 1, 4, 1) == 9 * 16 * 32 + 5 * 16 * 128
     assert lib.n2h_adj_work_words(arr, 4, 4, 1) == lists[4]  # one launch a range: the lists only
     assert lib.n2h_adj_work_words(arr, 4, 8, 5) == lists[8]
     n = sum(sizes)
