@@ -658,19 +658,37 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(Adj
         if (lane >= W * kAdjPops || !((mk >> ww) & 1u)) continue;
         const n2h_adj_net& a = b.net[q];
         float sum = 0.0f;
+        // the ranges' sums loaded kBatch at a time, all in flight together, then added in range order (a load
+        // and its wait per range would make P dependent round trips)
+        constexpr int kBatch = 32;
         if constexpr (PURE) {
             const float* src = b.partial + static_cast<int64_t>(i) * P * V + ww;
             bool first = true;
-            for (int r = 0; r < P; ++r) {
-                if (a.range_pop[r] != pp) continue;
-                const float x = src[static_cast<int64_t>(r) * V];
-                sum = first ? x : sum + x;
-                first = false;
+            for (int r0 = 0; r0 < P; r0 += kBatch) {
+                float x[kBatch];
+                bool mine[kBatch];
+#pragma unroll
+                for (int k = 0; k < kBatch; ++k) {
+                    mine[k] = r0 + k < P && a.range_pop[r0 + k] == pp;
+                    x[k] = mine[k] ? src[static_cast<int64_t>(r0 + k) * V] : 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < kBatch; ++k) {
+                    if (!mine[k]) continue;
+                    sum = first ? x[k] : sum + x[k];
+                    first = false;
+                }
             }
         } else {
             const float* src = b.partial + static_cast<int64_t>(i) * P * V + lane;
-            sum = src[0];
-            for (int r = 1; r < P; ++r) sum = sum + src[static_cast<int64_t>(r) * V];
+            for (int r0 = 0; r0 < P; r0 += kBatch) {
+                float x[kBatch];
+#pragma unroll
+                for (int k = 0; k < kBatch; ++k) x[k] = r0 + k < P ? src[static_cast<int64_t>(r0 + k) * V] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kBatch; ++k)
+                    if (r0 + k < P) sum = r0 + k == 0 ? x[k] : sum + x[k];
+            }
         }
         a.long_bins[(static_cast<int64_t>(ww) * a.n_neurons + jj) * kAdjPops + pp] = sum;
     }
