@@ -94,9 +94,12 @@ __device__ inline int adj_wave_base_at(const AdjBatch& b, int blk, int& q) {
 }
 __device__ inline int adj_wave_base(const AdjBatch& b, int& q) { return adj_wave_base_at(b, blockIdx.x, q); }
 
+// j's population: the count of population starts at or below j, the starts loaded all at once (past the last
+// population the end, n > j) rather than one after another
 __device__ inline int adj_pop(const n2h_adj_net& a, int j) {
     int pj = 0;
-    while (pj + 1 < a.n_pops && a.pop_first[pj + 1] <= j) ++pj;
+#pragma unroll
+    for (int p = 1; p < kAdjPops; ++p) pj += a.pop_first[min(p, a.n_pops)] <= j ? 1 : 0;
     return pj;
 }
 
@@ -295,10 +298,18 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
     if (j < n) {
         const float* C = a.pop_coef + adj_pop(a, j) * 8;
         const float dv = C[5];
-        for (int w = 0; w < b.n_win; ++w) {
-            const bool s = a.spikes[static_cast<int64_t>(w) * n + j] != 0;
-            const bool near = adj_u(a.vtrace[static_cast<int64_t>(w) * n + j], dv) <= b.sur.cutoff;
-            mask |= (s || near) ? (1u << w) : 0u;
+        // the window's traces and spike flags loaded all at once (unrolled to the longest window)
+        uint8_t s[kMaxWindow];
+        float vt[kMaxWindow];
+#pragma unroll
+        for (int w = 0; w < kMaxWindow; ++w) {
+            s[w] = w < b.n_win ? a.spikes[static_cast<int64_t>(w) * n + j] : uint8_t{0};
+            vt[w] = w < b.n_win ? a.vtrace[static_cast<int64_t>(w) * n + j] : __builtin_inff();
+        }
+#pragma unroll
+        for (int w = 0; w < kMaxWindow; ++w) {
+            const bool near = adj_u(vt[w], dv) <= b.sur.cutoff;
+            mask |= (w < b.n_win && (s[w] != 0 || near)) ? (1u << w) : 0u;
         }
     }
     const uint64_t below = (1ull << lane) - 1ull;
@@ -694,13 +705,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_combine_kernel(Adj
     }
 }
 
-__device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a, int j, const float (&mine)[kAdjPops]);
+__device__ inline void adj_update_quiet(const AdjBatch& b, const n2h_adj_net& a, int j);
 
-// The update of a gatherer j (adj_update_neuron's arithmetic, in its order) from inputs its wavefront loaded
-// ahead: lane p < 8 holds its population's total tot_l (short + long sums), lane c < 8 the coefficient C[c]
-// of j's population pj, lane p < n_pops alpha[p][pj] and j's compensated dL/dalpha pair; vp, spk, gv, gi
-// are uniform.  The coupling sum runs over p in order, as there; lane p updates its dL/dalpha pair; lane 0
-// writes the state and the adjoint rings.
+// The update of a gatherer j (the reverse step's update above: gs = c + sum over p of alpha[p][pj] bins[p],
+// gVpre = (1 - s) gV + h gs, ...) from inputs its wavefront loaded ahead: lane p < 8 holds its population's
+// total tot_l (short + long sums), lane c < 8 the coefficient C[c] of j's population pj, lane p < n_pops
+// alpha[p][pj] and j's compensated dL/dalpha pair; vp, spk, gv, gi are uniform.  The coupling sum runs over
+// p in order; lane p updates its dL/dalpha pair; lane 0 writes the state and the adjoint rings.
 __device__ inline void adj_update_gatherer(const AdjBatch& b, const n2h_adj_net& a, int j, float tot_l, float coef,
                                            float alpha_l, float2 gs_l, float vp, bool spk, float gv, float gi,
                                            int lane) {
@@ -752,10 +763,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
         const n2h_adj_net& a = b.net[q];
         const int j = base + lane;
         if (j >= a.n_neurons) return;
-        const float* C = a.pop_coef + adj_pop(a, j) * 8;
-        if (a.spikes[j] != 0 || adj_u(a.vtrace[j], C[5]) <= b.sur.cutoff) return;  // a gatherer
-        const float none[kAdjPops] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        adj_update_neuron(b, a, j, none);
+        adj_update_quiet(b, a, j);
         return;
     }
     // this XCD's sub-lists x and 8 + x of the step's list, in that order
@@ -817,37 +825,22 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBat
     }
 }
 
-// The update of neuron j in this reverse step from its summed out-link adjoints (zero when it did not
-// gather).
-__device__ inline void adj_update_neuron(const AdjBatch& b, const n2h_adj_net& a, int j, const float (&mine)[kAdjPops]) {
+// The update of a quiet neuron j in this reverse step: it did not spike and lies outside the surrogate's
+// support (h = 0), so it gathered nothing.  This is the update of adj_update_gatherer with those values --
+// gs = c + sum over a of alpha[a][pj] * 0 = c + 0 (the sum is +0 for finite alpha), gVpre = gV + 0 gs -- from
+// inputs loaded in one round (the population starts, then its coefficients; trace, spike flag and state).
+__device__ inline void adj_update_quiet(const AdjBatch& b, const n2h_adj_net& a, int j) {
     const int R = a.ring_slots, RS = R + a.long_delay - 1;
-
-    const int pj = adj_pop(a, j);
-    const float* C = a.pop_coef + pj * 8;
+    const float* C = a.pop_coef + adj_pop(a, j) * 8;
     const float P_V = C[0], P_I = C[1], P_VI = C[2], dv = C[5], c_loss = C[6];
     const float vp = a.vtrace[j];
     const bool s = a.spikes[j] != 0;
-    const float u = adj_u(vp, dv);
-    const bool near = u <= b.sur.cutoff;  // the surrogate is nonzero
-    float coup = 0.0f;
-    for (int p = 0; p < a.n_pops; ++p) coup += a.alpha[p * a.n_pops + pj] * mine[p];
-    if (s) {  // compensated fp32 accumulation (hi, lo) over the reverse steps: the exact error of every addition
-        float2* gs = reinterpret_cast<float2*>(a.g_sum) + static_cast<int64_t>(j) * a.n_pops;
-        for (int p = 0; p < a.n_pops; ++p) {
-            float2 acc = gs[p];
-            const n2h_twofloat::df t = n2h_twofloat::two_sum(acc.x, mine[p]);
-            acc.x = t.hi;
-            acc.y = acc.y + t.lo;
-            gs[p] = acc;
-        }
-    }
     const float gv = a.g_v[j], gi = a.g_i[j];
+    if (s || adj_u(vp, dv) <= b.sur.cutoff) return;  // a gatherer: a wavefront of the gather half updates it
     float gip, gv_new;
     if (isfinite(vp)) {
-        const float gsum = c_loss + coup;
-        const float den = 1.0f + b.sur.kappa * u;
-        const float h = near ? (b.sur.gamma * 0.5f * b.sur.kappa / dv) / (den * den) : 0.0f;
-        const float gvpre = (s ? 0.0f : gv) + h * gsum;
+        const float gsum = c_loss + 0.0f;
+        const float gvpre = gv + 0.0f * gsum;
         gip = P_VI * gvpre + P_I * gi;
         gv_new = P_V * gvpre;
     } else {
