@@ -1,7 +1,8 @@
 """Degenerate networks against the oracle, bit for bit (raster, V, I, refractory counters, ring), through the
 segment intake and the row intake taken at every step: a network without a single link, a one-neuron
 network, and ragged populations (1, 63, 65 and 130 neurons: groups of one lane, a full group, a group and
-one lane) joined by connections of which one holds no links."""
+one lane) joined by connections of which one holds no links; and the three in one batched launch under
+every launch mapping."""
 from __future__ import annotations
 
 import sys
@@ -72,3 +73,27 @@ def test_degenerate_networks_equal_the_oracle(make, rows):
     assert raster.sum() > 0
     assert np.array_equal(rt.raster(), raster)
     assert_state_equal(rt, st)
+
+
+def test_degenerate_networks_batched_equal_the_oracle():
+    """The three networks in one batched launch (FusedRound), 40 steps under every launch mapping the
+    autotuner can choose, then each network against the oracle over all the steps."""
+    from naer2_amd import FusedRound
+    from naer2_amd.autotune import CANDIDATES
+    nets = [_unconnected(), _one_neuron(), _ragged()]
+    rts = [n.build() for n in nets]
+    _say(f"case: batch of degenerate networks, sizes {[rt.n for rt in rts]}, links {[rt.n_links for rt in rts]}, "
+         f"{len(CANDIDATES)} mappings x 40 steps")
+    states = [oracle_state(rt) for rt in rts]
+    for cand in CANDIDATES:
+        fr = FusedRound(rts, cand, reserve_steps=40, weights=[1.0, 2.0, 3.0])
+        fr.launch(40, graph_window=0)
+        fr.close()
+    torch.cuda.synchronize()
+    steps = 40 * len(CANDIDATES)
+    for rt, st in zip(rts, states):
+        assert rt.step == steps
+        raster, _ = oracle_run(rt, st, 0, steps)
+        assert np.array_equal(rt.raster(), raster)
+        assert_state_equal(rt, st)
+    assert sum(int(rt.raster().sum()) for rt in rts) > 0
