@@ -33,9 +33,9 @@
 // adj_long_kernel reads each gathering source's long links once per window and gathers a link's W values
 // in one contiguous read: the ring is target-major, gx[k][slot], step s in slot s mod R, with the first W - 1
 // slots mirrored past slot R - 1 so that W consecutive steps never wrap.  The long links are walked in
-// long_parts target ranges so that the L2s hold the rows being read (adj_long_kernel<W, true>: all ranges in one
-// launch over the XCDs, adj_combine_kernel adding their sums in order; adj_long_kernel<W, false>: one
-// launch a range).  Each step, adj_step_kernel walks only the short links, adds the window's long sums (bins[a] =
+// long_parts target ranges so that the L2s hold the rows being read (adj_long_pipek_kernel: all ranges in one
+// launch over the XCDs, adj_combine_kernel adding their sums in order; adj_long_kernel: one launch a
+// range).  Each step, adj_step_kernel walks only the short links, adds the window's long sums (bins[a] =
 // short sum + long sum) and updates every neuron.  The split depends on the link alone, so the sums do not
 // depend on where windows, checkpoints or batches start.
 #include <hip/hip_runtime.h>
@@ -326,43 +326,33 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
     }
 }
 
-// The window's gatherers, a wavefront each: their long links, a link's W input adjoints in one contiguous
-// read; lanes stride the list with per-lane sums per (target population, window step), then a fixed
-// butterfly; long_bins rows of the steps in the entry's mask.
 template <int W>
 constexpr int adj_win_pad() { return W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8; }  // W padded to a power of 2
 
-// SPLIT = false: one launch per target range (b.part), each adding its sums to the long_bins rows in launch
-// order.  SPLIT = true (long_parts a multiple of kXcds): one launch for all ranges; workgroup b runs on XCD
-// b mod kXcds (workgroups are dealt to the XCDs in turn), whose wavefronts walk ranges x, x + kXcds, ... in
-// that order, so each XCD's L2 holds the ring rows of one range at a time and no two L2s hold the same
-// rows; the range sums go to `partial`, which adj_combine_kernel adds in range order (the same sums, in the
-// same order, as the launch-per-range path).
-
-// PURE (with SPLIT; the caller's range_pop given): every range holds targets of one population only, so a
-// task keeps W sums per lane instead of 8 W, without selects, and its partial row is [W'] for that population.
-template <int W, bool SPLIT, bool PURE>
+// The window's gatherers' long links into one target range (b.part), for range counts that are not a
+// multiple of 8 (one launch a range): a wavefront a gatherer, a link's W input adjoints in one contiguous
+// read; lanes stride the list with per-lane sums per (target population, window step), then a fixed
+// reduction; each launch adds its sums to the long_bins rows of the steps in the entry's mask, in launch
+// (range) order.  Range counts that are a multiple of 8 take one launch for all ranges instead
+// (adj_long_pipek_kernel, or adj_long_group_kernel for population-pure ranges): workgroup b runs on XCD
+// b mod kXcds (the hardware deals workgroups to the XCDs in turn), whose wavefronts walk ranges x,
+// x + kXcds, ... in that order, so each XCD's L2 holds the ring rows of one range at a time and no two L2s
+// hold the same rows; the range sums go to `partial`, which adj_combine_kernel adds in range order (the
+// same sums, in the same order, as here).
+template <int W>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBatch b) {
-    static_assert(SPLIT || !PURE, "population-pure ranges are walked by the split kernel only");
     constexpr int WP = adj_win_pad<W>();
-    constexpr int V = PURE ? WP : kAdjPops * WP;
+    constexpr int V = kAdjPops * WP;
     constexpr int SH = adj_halving_shift<V>();
     const int lane = threadIdx.x & (kWave - 1);
     const AdjList list = adj_list_counts(b, 0);
-    const int idx = lane >> SH;                    // the element this lane ends with: w * 8 + p (PURE: w)
-    const int ww = PURE ? idx : idx >> 3, pp = PURE ? 0 : idx & 7;
+    const int idx = lane >> SH;  // the element this lane ends with: w * 8 + p
+    const int ww = idx >> 3, pp = idx & 7;
     const bool writer = (lane & ((1 << SH) - 1)) == 0 && ww < W;
     const int wave = static_cast<int>(threadIdx.x >> 6);
-    const int xcd = SPLIT ? static_cast<int>(blockIdx.x % kXcds) : 0;
-    const int f0 = SPLIT ? static_cast<int>(blockIdx.x / kXcds) * kWavesPerBlock + wave
-                         : static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
-    const int fs = SPLIT ? static_cast<int>(gridDim.x / kXcds) * kWavesPerBlock : static_cast<int>(gridDim.x) * kWavesPerBlock;
-    const int P = b.net[0].long_parts;
-    const int64_t F = SPLIT ? static_cast<int64_t>(P / kXcds) * list.total : list.total;
-    for (int64_t f = f0; f < F; f += fs) {
-        const int m = SPLIT ? static_cast<int>(f / list.total) : 0;  // this XCD's m-th range
-        const int i = static_cast<int>(f - static_cast<int64_t>(m) * list.total);
-        const int part = SPLIT ? xcd + kXcds * m : b.part;
+    const int part = b.part;
+    for (int i = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave; i < list.total;
+         i += static_cast<int>(gridDim.x) * kWavesPerBlock) {
         const uint32_t ent = adj_list_entry(b, list, 0, i);
         const int q = static_cast<int>((ent >> 17) & 15u), jj = static_cast<int>(ent & 0x1ffffu);
         const uint32_t mk = ent >> 21;
@@ -394,21 +384,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     const float c = wt * gw[r][w];
-                    if constexpr (PURE) {
-                        bins[w] = live ? bins[w] + c : bins[w];
-                    } else {
 #pragma unroll
-                        for (int p = 0; p < kAdjPops; ++p)
-                            bins[w * kAdjPops + p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c
-                                                                                                  : bins[w * kAdjPops + p];
-                    }
+                    for (int p = 0; p < kAdjPops; ++p)
+                        bins[w * kAdjPops + p] = (live && tp[r] == static_cast<uint32_t>(p)) ? bins[w * kAdjPops + p] + c
+                                                                                              : bins[w * kAdjPops + p];
                 }
             }
         }
         const float v = (e1 > e0) ? adj_reduce_halving<V>(bins, lane) : 0.0f;  // uniform branch
-        if constexpr (SPLIT) {
-            if (writer) b.partial[(static_cast<int64_t>(i) * P + part) * V + idx] = v;
-        } else if (writer && ((mk >> ww) & 1u)) {
+        if (writer && ((mk >> ww) & 1u)) {
             // the rows of the steps in the mask: target ranges added in launch order (the first one stores)
             float* row = a.long_bins + (static_cast<int64_t>(ww) * n + jj) * kAdjPops + pp;
             *row = part == 0 ? v : *row + v;
@@ -425,7 +409,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_kernel(AdjBat
 // neuron order, so an item sums into the bins of populations [g0, g0 + K) of its range's span, one sweep over
 // its links for each K populations of the span (one sweep for every range of PD14 at 24 ranges).  A bin's
 // sum takes its population's links in link order, and the partial row holds +0 for populations outside the
-// range, as adj_long_kernel<W, true, false> computes them.
+// range, as the launch-per-range adj_long_kernel's sums are.
 template <int W, int K>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipek_kernel(AdjBatch b) {
     constexpr int WP = adj_win_pad<W>();
@@ -581,8 +565,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipek_kernel(
 // Population-pure ranges, LANES lanes a task: a wavefront takes 64 / LANES consecutive gatherers of one
 // range at once (their list entries, run bounds and links all in flight together), each walked by its lane
 // group with the group's lanes striding the run (e, e + LANES, ...), kUnroll links a lane per pass, then a
-// fixed butterfly over the group.  The partial row of (entry, range) is [W'], as adj_long_kernel<W, true,
-// true> writes it.  The XCD of workgroup b (b mod kXcds) walks ranges x, x + kXcds, ... in turn, as there.
+// fixed butterfly over the group.  The partial row of (entry, range) is [W']: the range's one population
+// only, which adj_combine_kernel<W, true> adds over that population's ranges.  The XCD of workgroup b
+// (b mod kXcds) walks ranges x, x + kXcds, ... in turn, as in adj_long_pipek_kernel.
 template <int W, int LANES>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_group_kernel(AdjBatch b) {
     constexpr int WP = adj_win_pad<W>();
@@ -957,7 +942,7 @@ void launch_window(int64_t blocks, hipStream_t stream, AdjBatch& b) {
     }
     for (int part = 0; part < b.net[0].long_parts; ++part) {  // one target range a launch
         b.part = part;
-        hipLaunchKernelGGL((adj_long_kernel<W, false, false>), dim3(kLongBlocks), block, 0, stream, b);
+        hipLaunchKernelGGL((adj_long_kernel<W>), dim3(kLongBlocks), block, 0, stream, b);
     }
 }
 
