@@ -7,9 +7,10 @@ for a per-population rate loss, with respect to the weight scales alpha[target p
 loss = sum over networks and populations of (rate - target)^2 (rates in Hz over the n_steps).  The forward is
 the plain step (bit-exact, one batched launch per step, FusedRound), with the state saved every `checkpoint`
 steps; the forward also tracks what fp32 rounding drops from each neuron's V and I (n2h_set_vtrace's
-compensation state, checkpointed with the state; V, I and the spikes are unchanged).  The backward runs the
-segments in reverse: each is recomputed from its checkpoint with the compensated distance to the threshold
-traced, then reversed in windows of `long_delay` steps:
+compensation state, checkpointed with the state; V, I and the spikes are unchanged), and traces the
+compensated distance to the threshold through its last segment.  The backward runs the segments in reverse:
+each before the last is recomputed from its checkpoint with that distance traced (beside the reverse of the
+segment after it, on a stream of its own), then reversed in windows of `long_delay` steps:
 n2h_adj_window gathers the input adjoints of every out-link with a delay of at least long_delay once per
 window (they all lie past it), then n2h_adj_step reverses each step of the window -- one launch for all
 networks, a wavefront per source gathering its short links' input adjoints and adding the window's long
@@ -214,9 +215,24 @@ class RateLossGrad:
         for rt, l in zip(rts, lo):
             _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, l[0].data_ptr(), l[1].data_ptr()), "n2h_set_vtrace")
         fr = FusedRound(rts, self.tuning, reserve_steps=T + 1)
+        # the traced distances to the threshold and the spike flags of a segment, two sets that alternate
+        # between segments in the backward
+        L0 = min(K, T)
+        bufs = [[{"vtr": torch.empty(L0, rt.n, device=self.dev),
+                  "spk": torch.empty(L0, rt.n, dtype=torch.uint8, device=self.dev)} for rt in rts] for _ in range(2)]
+        n_seg = (T + K - 1) // K
         ckpts = []
-        for k0 in range(0, T, K):
-            ckpts.append([(rt.snapshot(), l.clone()) for rt, l in zip(rts, lo)])
+        for k, k0 in enumerate(range(0, T, K)):
+            if k < n_seg - 1:
+                ckpts.append([(rt.snapshot(), l.clone()) for rt, l in zip(rts, lo)])
+            else:
+                # the last segment is traced as it runs: the backward reverses it first, and need not recompute
+                # it from its checkpoint (the trace does not change the step's results)
+                ckpts.append(None)
+                for rt, b, l in zip(rts, bufs[k % 2], lo):
+                    _lib.check(lib.n2h_set_vtrace(rt._plan, b["vtr"].data_ptr(), s0 + k0, T - k0, l[0].data_ptr(),
+                                                  l[1].data_ptr()), "n2h_set_vtrace")
+                fr.sync()
             fr.launch(min(K, T - k0), self.window)
         fr.close()
         for rt in rts:
@@ -277,16 +293,18 @@ class RateLossGrad:
         fr_rec = FusedRound(rts, self.tuning)
         rec_stream = torch.cuda.Stream(device=self.dev) if self.overlap else rev_stream
         rec_stream.wait_stream(rev_stream)
-        L0 = min(K, T)
-        bufs = [[{"vtr": torch.empty(L0, rt.n, device=self.dev),
-                  "spk": torch.empty(L0, rt.n, dtype=torch.uint8, device=self.dev)} for rt in rts] for _ in range(2)]
-        n_seg = len(ckpts)
         ev_rec, ev_rev, t_rec, t_rev = [None] * n_seg, [None] * n_seg, [], []
 
         def timed_event(stream):
             e = torch.cuda.Event(enable_timing=True)
             e.record(stream)
             return e
+
+        def spike_flags(k):
+            start, L = s0 + k * K, min(K, T - k * K)
+            for rt, b in zip(rts, bufs[k % 2]):
+                gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
+                b["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
 
         def recompute(k):
             start, L = s0 + k * K, min(K, T - k * K)
@@ -299,10 +317,9 @@ class RateLossGrad:
                                                   l[1].data_ptr()), "n2h_set_vtrace")
                 fr_rec.sync()
                 fr_rec.launch(L, 0)
-                for rt, b in zip(rts, bufs[k % 2]):
+                for rt in rts:
                     _lib.check(lib.n2h_set_vtrace(rt._plan, None, 0, 0, None, None), "n2h_set_vtrace")
-                    gof, lof = rt.layout.group_of.to(self.dev), rt.layout.local_of.to(self.dev)
-                    b["spk"][:L] = ((rt.raster_bits(start, start + L)[:, gof] >> lof) & 1).to(torch.uint8)
+                spike_flags(k)
                 ev_rec[k] = timed_event(rec_stream)
             t_rec.append((e0, ev_rec[k]))
 
@@ -327,7 +344,13 @@ class RateLossGrad:
             ev_rev[k] = timed_event(rev_stream)
             t_rev.append((e0, ev_rev[k]))
 
-        recompute(n_seg - 1)
+        # the last segment was traced by the forward: only its spike flags are extracted before its reverse,
+        # and before the recompute stream's first restore replaces the bit history they are read from
+        e0 = timed_event(rev_stream)
+        spike_flags(n_seg - 1)
+        ev_rec[n_seg - 1] = timed_event(rev_stream)
+        t_rec.append((e0, ev_rec[n_seg - 1]))
+        rec_stream.wait_event(ev_rec[n_seg - 1])
         for k in reversed(range(n_seg)):
             if k >= 1:
                 if k + 1 < n_seg:
@@ -352,7 +375,7 @@ class RateLossGrad:
         phases = {"setup": t_bw[0].elapsed_time(marks[0][1]),
                   "recompute": sum(a.elapsed_time(b) for a, b in t_rec),   # on its own stream when overlapped
                   "reverse": sum(a.elapsed_time(b) for a, b in t_rev),
-                  "first_recompute": t_rec[0][0].elapsed_time(t_rec[0][1])}  # the one nothing overlaps
+                  "first_segment": t_rec[0][0].elapsed_time(t_rec[0][1])}  # its spike flags: nothing overlaps them
         self.last = {"forward_ms": t_fw[0].elapsed_time(t_fw[1]), "backward_ms": t_bw[0].elapsed_time(t_bw[1]),
                      "backward_phases_ms": phases, "c_pops": c_pops}
         return loss, g.cpu().numpy().astype(np.float64)
