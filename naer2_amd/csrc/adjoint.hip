@@ -314,15 +314,25 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
     }
     const uint64_t below = (1ull << lane) - 1ull;
     const int sub = static_cast<int>(blockIdx.x % kXcds) + kXcds * static_cast<int>((threadIdx.x >> 6) & 1u);
-    for (int which = 0; which <= b.n_win; ++which) {
-        const bool in = which == 0 ? mask != 0 : ((mask >> (which - 1)) & 1u) != 0;
-        const uint64_t bal = __ballot(in);
-        if (bal == 0) continue;  // uniform
-        int first = 0;
-        if (lane == 0) first = atomicAdd(b.work + adj_counter(which, sub), __popcll(bal));
-        first = __shfl(first, 0, kWave);
+    // every list's ballot, then the entries of all lists reserved at once: lane `which` adds list which's
+    // count to its counter (one returning atomic a lane, one round trip for all lists), then the entries
+    uint64_t bal[1 + kMaxWindow];
+    int cnt_l = 0;  // lane which: list which's entries from this wavefront
+#pragma unroll
+    for (int which = 0; which <= kMaxWindow; ++which) {
+        const bool in = which <= b.n_win && (which == 0 ? mask != 0 : ((mask >> (which - 1)) & 1u) != 0);
+        bal[which] = __ballot(in);
+        cnt_l = lane == which ? __popcll(bal[which]) : cnt_l;
+    }
+    int first_l = 0;
+    if (lane <= b.n_win && cnt_l > 0) first_l = atomicAdd(b.work + adj_counter(lane, sub), cnt_l);
+#pragma unroll
+    for (int which = 0; which <= kMaxWindow; ++which) {
+        if (bal[which] == 0) continue;  // uniform (and every list past the window's steps)
+        const int first = __shfl(first_l, which, kWave);
         const uint32_t ent = (which == 0 ? mask << 21 : 0u) | (static_cast<uint32_t>(q) << 17) | static_cast<uint32_t>(j);
-        if (in) b.work[adj_sublist(b, which, sub) + first + __popcll(bal & below)] = static_cast<int32_t>(ent);
+        if ((bal[which] >> lane) & 1ull)
+            b.work[adj_sublist(b, which, sub) + first + __popcll(bal[which] & below)] = static_cast<int32_t>(ent);
     }
 }
 
