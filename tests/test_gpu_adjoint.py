@@ -117,15 +117,16 @@ def test_gradient_is_independent_of_checkpoints_and_batching():
 def test_gradient_agrees_over_window_lengths():
     """The long-link windows and target ranges only regroup each source's sum (short links + long links, the
     long ones range by range): every window length, range count and range layout (equal ranges across
-    populations, or population-pure ones walked eight tasks a wavefront), with checkpoints that are not a
-    multiple of the window, gives the single-step gradient to fp32 rounding."""
+    populations, walked with 4 population bins a lane at every window length from 2 to 8, or
+    population-pure ones walked eight tasks a wavefront), with checkpoints that are not a multiple of the
+    window, gives the single-step gradient to fp32 rounding."""
     T = 120
     alpha = _alpha(5)
     targets = np.random.default_rng(2).uniform(0.5, 10.0, (2, 8))
     sur = {"cutoff": 0.5, "gamma": 0.5}
     res = {}
     cases = ((1, 1, False), (3, 5, False), (4, 8, False), (8, 16, True), (4, 1, False), (4, 24, False), (4, 24, True),
-             (4, 32, True))
+             (4, 32, True), (2, 8, False), (3, 8, False), (8, 16, False))
     for W, P, pure in cases:
         _say(f"case: long_delay {W}, long_parts {P}, pure ranges {pure}, checkpoint 50, T={T}")
         rts = [n.build() for n in _nets(alpha)]
