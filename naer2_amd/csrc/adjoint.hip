@@ -184,11 +184,14 @@ constexpr int adj_halving_shift() {  // 6 - log2 V: lane l ends with element l >
 // wavefront in grid-stride order.  List order depends on the schedule, but every entry's sums are its own,
 // in a fixed order, so results do not.  Each list (which = 0: the window's, 1 + w: step w's) is kept as
 // kSubLists sub-lists, each counter on a cache line of its own: appends to one counter serialize in the L2.
-// The wavefronts of mark workgroup b append to sub-lists b mod 8 and 8 + b mod 8 (by their index's parity),
-// and the step kernel's workgroups on XCD x walk sub-lists x and 8 + x: a neuron is then gathered on the
-// same XCD in every step, the XCD that also updates it when it does not gather, so that its short links
-// and state, read again in the steps it stays near the threshold, are in that XCD's L2.  (Workgroup b runs
-// on XCD b mod 8: the hardware deals a launch's workgroups to the 8 XCDs in turn.)
+// Every neuron has an XCD of its own: network q's neurons are dealt, by mark workgroup, over the XCDs of
+// network q (with at most 8 networks a contiguous share of the 8, else XCD q mod 8).  The wavefronts of a
+// mark workgroup append to sub-lists x and 8 + x of its XCD x (by their index's parity), and the step
+// kernel's workgroups on XCD x walk sub-lists x and 8 + x.  A neuron is then gathered on the same XCD in
+// every step, so that its short links, read again in the steps it stays near the threshold, are in that
+// XCD's L2, and an XCD's gatherers read the short ring of its own networks only (one of 4 PD14s: 0.9 MB
+// of adjoints a step instead of 3.7).  (Workgroup b runs on XCD b mod 8: the hardware deals a launch's
+// workgroups to the 8 XCDs in turn.)
 constexpr int kLongBlocks = 1536;    // 6 blocks of 4 wavefronts per CU (82 VGPRs: 6 wavefronts per SIMD fit)
 constexpr int kShortBlocks = 2048;   // 8 blocks of 4 wavefronts per CU
 constexpr int kSplitBlocks = 1280;   // the split long kernel: 5 blocks per CU (93 VGPRs)
@@ -204,11 +207,25 @@ static_assert(kSplitBlocks % kXcds == 0 && kGroupBlocks % kXcds == 0 && kPipeBlo
 constexpr int kSubLists = 2 * kXcds;
 constexpr int kCounterStride = 32;   // int32 words: 128 B
 constexpr int kAdjListHead = (1 + kMaxWindow) * kSubLists * kCounterStride;
+// network q's XCDs: the first, and their count
+__host__ __device__ inline int adj_net_xcd0(int q, int n_nets) { return n_nets <= kXcds ? q * kXcds / n_nets : q % kXcds; }
+__host__ __device__ inline int adj_net_xcds(int q, int n_nets) {
+    return n_nets <= kXcds ? (q + 1) * kXcds / n_nets - q * kXcds / n_nets : 1;
+}
 // entries a sub-list holds: sub-list c takes the appends of the wavefronts of parity c / kXcds (two a
-// workgroup) in the workgroups b with b mod kXcds = c mod kXcds, at most 64 each
+// workgroup) of the mark workgroups dealt to XCD c mod kXcds (workgroup bq of network q to XCD
+// adj_net_xcd0(q) + bq mod adj_net_xcds(q)), at most 64 each: sized for the XCD dealt the most
 static_assert(kWavesPerBlock % 2 == 0, "mark workgroups append to two sub-lists by wavefront parity");
-inline int64_t adj_sub_cap(int64_t blocks) {
-    return static_cast<int64_t>(kWave) * (kWavesPerBlock / 2) * ((blocks + kXcds - 1) / kXcds);
+inline int64_t adj_sub_cap(const int32_t* n_neurons, int n_nets) {
+    int64_t per[kXcds] = {};
+    for (int q = 0; q < n_nets; ++q) {
+        const int64_t nb = (n_neurons[q] + kWave * kWavesPerBlock - 1) / (kWave * kWavesPerBlock);
+        const int x0 = adj_net_xcd0(q, n_nets), m = adj_net_xcds(q, n_nets);
+        for (int k = 0; k < m; ++k) per[x0 + k] += (nb + m - 1) / m;
+    }
+    int64_t most = 0;
+    for (int x = 0; x < kXcds; ++x) most = per[x] > most ? per[x] : most;
+    return static_cast<int64_t>(kWave) * (kWavesPerBlock / 2) * most;
 }
 
 __device__ inline int64_t adj_counter(int which, int c) {
@@ -313,7 +330,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_mark_kernel(AdjBat
         }
     }
     const uint64_t below = (1ull << lane) - 1ull;
-    const int sub = static_cast<int>(blockIdx.x % kXcds) + kXcds * static_cast<int>((threadIdx.x >> 6) & 1u);
+    const int bq = static_cast<int>(blockIdx.x) - b.first_block[q];  // this workgroup's XCD among its network's
+    const int x = adj_net_xcd0(q, b.n_nets) + bq % adj_net_xcds(q, b.n_nets);
+    const int sub = x + kXcds * static_cast<int>((threadIdx.x >> 6) & 1u);
     // every list's ballot, then the entries of all lists reserved at once: lane `which` adds list which's
     // count to its counter (one returning atomic a lane, one round trip for all lists), then the entries
     uint64_t bal[1 + kMaxWindow];
@@ -904,35 +923,36 @@ int adj_prepare(const n2h_adj_net* nets, int32_t n_nets, int64_t step, const n2h
     b.first_block[n_nets] = static_cast<int32_t>(blocks);
     b.ring_pos = static_cast<int32_t>(step % ring);
     b.recent_pos = static_cast<int32_t>(step % win);
-    b.sub_cap = static_cast<int32_t>(adj_sub_cap(blocks));
+    int32_t sizes[kAdjNets];
+    for (int i = 0; i < n_nets; ++i) sizes[i] = nets[i].n_neurons;
+    b.sub_cap = static_cast<int32_t>(adj_sub_cap(sizes, n_nets));
     return N2H_OK;
 }
 
 // the work buffer: the counters, then 1 + long_delay lists of kSubLists sub-lists of sub_cap entries, then
 // (long_parts a multiple of kXcds) the range sums of every possible window entry
-int64_t adj_lists_words(int64_t blocks, int32_t long_delay) {
-    return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * adj_sub_cap(blocks);
+int64_t adj_lists_words(int64_t sub_cap, int32_t long_delay) {
+    return kAdjListHead + (1 + static_cast<int64_t>(long_delay)) * kSubLists * sub_cap;
 }
 int64_t adj_partial_floats(int64_t n_total, int32_t long_delay, int32_t long_parts) {
     if (long_parts % kXcds != 0) return 0;
     const int wp = long_delay <= 1 ? 1 : long_delay <= 2 ? 2 : long_delay <= 4 ? 4 : 8;
     return n_total * long_parts * kAdjPops * wp;
 }
-int64_t adj_work_words(int64_t blocks, int64_t n_total, int32_t long_delay, int32_t long_parts) {
-    return adj_lists_words(blocks, long_delay) + adj_partial_floats(n_total, long_delay, long_parts);
+int64_t adj_work_words(int64_t sub_cap, int64_t n_total, int32_t long_delay, int32_t long_parts) {
+    return adj_lists_words(sub_cap, long_delay) + adj_partial_floats(n_total, long_delay, long_parts);
 }
 
-int adj_bind_work(AdjBatch& b, const n2h_adj_net* nets, int32_t n_nets, int64_t blocks, int32_t* work,
-                  int64_t work_words) {
+int adj_bind_work(AdjBatch& b, const n2h_adj_net* nets, int32_t n_nets, int32_t* work, int64_t work_words) {
     int64_t n_total = 0;
     for (int i = 0; i < n_nets; ++i) n_total += nets[i].n_neurons;
     const int32_t W = nets[0].long_delay, P = nets[0].long_parts;
-    const int64_t need = adj_work_words(blocks, n_total, W, P);
+    const int64_t need = adj_work_words(b.sub_cap, n_total, W, P);
     if (!work || work_words < need)
         return fail(N2H_EINVAL, "work: %lld int32 words needed (n2h_adj_work_words), %lld given",
                     static_cast<long long>(need), static_cast<long long>(work_words));
     b.work = work;
-    b.partial = adj_partial_floats(n_total, W, P) ? reinterpret_cast<float*>(work + adj_lists_words(blocks, W)) : nullptr;
+    b.partial = adj_partial_floats(n_total, W, P) ? reinterpret_cast<float*>(work + adj_lists_words(b.sub_cap, W)) : nullptr;
     return N2H_OK;
 }
 
@@ -963,13 +983,12 @@ extern "C" {
 int64_t n2h_adj_work_words(const int32_t* n_neurons, int32_t n_nets, int32_t long_delay, int32_t long_parts) {
     if (!n_neurons || n_nets < 1 || n_nets > kAdjNets || long_delay < 1 || long_delay > kMaxWindow) return -1;
     if (long_parts < 1 || long_parts > kMaxParts) return -1;
-    int64_t blocks = 0, n_total = 0;
+    int64_t n_total = 0;
     for (int i = 0; i < n_nets; ++i) {
         if (n_neurons[i] < 1) return -1;
-        blocks += (n_neurons[i] + kWave * kWavesPerBlock - 1) / (kWave * kWavesPerBlock);
         n_total += n_neurons[i];
     }
-    return adj_work_words(blocks, n_total, long_delay, long_parts);
+    return adj_work_words(adj_sub_cap(n_neurons, n_nets), n_total, long_delay, long_parts);
 }
 
 int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t win_step, const n2h_surrogate* sur,
@@ -978,7 +997,7 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, nets, n_nets, blocks, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
     if (win_step < 0 || win_step >= nets[0].long_delay)
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
@@ -996,7 +1015,7 @@ int n2h_adj_window(const n2h_adj_net* nets, int32_t n_nets, int64_t step0, int32
     int64_t blocks;
     int rc = adj_prepare(nets, n_nets, step0, sur, b, blocks);
     if (rc != N2H_OK) return rc;
-    if ((rc = adj_bind_work(b, nets, n_nets, blocks, work, work_words)) != N2H_OK) return rc;
+    if ((rc = adj_bind_work(b, nets, n_nets, work, work_words)) != N2H_OK) return rc;
     const int W = nets[0].long_delay;
     if (n_win < 1 || n_win > W) return fail(N2H_EINVAL, "n_win %d outside [1, long_delay = %d]", n_win, W);
     b.n_win = n_win;

@@ -83,11 +83,18 @@ def test_work_words_follow_the_list_layout():
     sizes = [77169, 77169, 77169, 77169]
     arr = (ctypes.c_int32 * 4)(*sizes)
     blocks = sum((s + 255) // 256 for s in sizes)
-    # a sub-list takes the appends of two wavefronts (one parity) of every 8th mark workgroup, 64 each
-    sub_cap = 64 * 2 * ((blocks + 7) // 8)
+    # a sub-list takes the appends of two wavefronts (one parity) of the mark workgroups dealt to its XCD, 64
+    # each: with 4 networks, each network's workgroups over 2 XCDs
+    sub_cap = 64 * 2 * ((blocks // 4 + 1) // 2)
     lists = {4: 9 * 16 * 32 + 5 * 16 * sub_cap, 8: 9 * 16 * 32 + 9 * 16 * sub_cap, 3: 9 * 16 * 32 + 4 * 16 * sub_cap}
     one = (ctypes.c_int32 * 1)(212)  # one workgroup: its two wavefronts of a parity fill one sub-list
     assert lib.n2h_adj_work_words(one, 1, 4, 1) == 9 * 16 * 32 + 5 * 16 * 128
+    # uneven networks: 391 workgroups over XCDs 0-3 (98 each at most), 4 over XCDs 4-7
+    uneven = (ctypes.c_int32 * 2)(100000, 1000)
+    assert lib.n2h_adj_work_words(uneven, 2, 4, 1) == 9 * 16 * 32 + 5 * 16 * 128 * 98
+    # more networks than XCDs: network q on XCD q mod 8 (XCDs 0 and 1 take two networks of 2 workgroups)
+    ten = (ctypes.c_int32 * 10)(*[300] * 10)
+    assert lib.n2h_adj_work_words(ten, 10, 4, 1) == 9 * 16 * 32 + 5 * 16 * 128 * 4
     assert lib.n2h_adj_work_words(arr, 4, 4, 1) == lists[4]  # one launch a range: the lists only
     assert lib.n2h_adj_work_words(arr, 4, 8, 5) == lists[8]
     n = sum(sizes)
