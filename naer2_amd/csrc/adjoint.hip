@@ -456,6 +456,28 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipek_kernel(
     const int P = b.net[0].long_parts;
     const int64_t total = list.total;
     const int64_t F = static_cast<int64_t>(P / kXcds) * total;
+    // the items of a range are walked network by network: the window list's sub-lists in the order of their
+    // XCDs' networks (each network's XCDs' sub-lists together), an entry keeping its place in the whole list
+    int ord[kSubLists], ocum[kSubLists + 1], ooff[kSubLists];
+    {
+        int k = 0;
+#pragma unroll
+        for (int x = 0; x < kXcds; ++x) {  // XCD x's two sub-lists; XCDs of one network are adjacent
+            ord[k++] = x;
+            ord[k++] = x + kXcds;
+        }
+        int acc = 0;
+        ocum[0] = 0;
+#pragma unroll
+        for (int kk = 0; kk < kSubLists; ++kk) {
+            int off = 0;
+#pragma unroll
+            for (int c = 0; c < kSubLists; ++c) off += c < ord[kk] ? list.cnt[c] : 0;
+            ooff[kk] = off;
+            acc += list.cnt[ord[kk]];
+            ocum[kk + 1] = acc;
+        }
+    }
     for (int64_t fb = f0; fb < F; fb += kWave * fs) {
         // lane l: item fb + l fs -- its list index, range, network, source, run bounds and the range's
         // populations [plo_l, phi_l]
@@ -464,9 +486,20 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_long_pipek_kernel(
         int64_t e0_l = 0, e1_l = 0;
         if (fl < F) {
             const int m = static_cast<int>(fl / total);
-            i_l = static_cast<int>(fl - static_cast<int64_t>(m) * total);
+            const int e = static_cast<int>(fl - static_cast<int64_t>(m) * total);
             part_l = xcd + kXcds * m;
-            const uint32_t ent = adj_list_entry_lane(b, list, 0, i_l);
+            int k = 0;
+#pragma unroll
+            for (int kk = 1; kk < kSubLists; ++kk) k += ocum[kk] <= e ? 1 : 0;
+            int sub = 0, cum = 0, off = 0;
+#pragma unroll
+            for (int kk = 0; kk < kSubLists; ++kk) {
+                sub = kk == k ? ord[kk] : sub;
+                cum = kk == k ? ocum[kk] : cum;
+                off = kk == k ? ooff[kk] : off;
+            }
+            i_l = off + (e - cum);
+            const uint32_t ent = static_cast<uint32_t>(b.work[adj_sublist(b, 0, sub) + (e - cum)]);
             q_l = static_cast<int>((ent >> 17) & 15u);
             const int jj = static_cast<int>(ent & 0x1ffffu);
             const n2h_adj_net& aq = b.net[q_l];
