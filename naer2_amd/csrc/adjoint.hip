@@ -40,6 +40,7 @@
 // depend on where windows, checkpoints or batches start.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -805,8 +806,21 @@ __device__ inline void adj_update_gatherer(const AdjBatch& b, const n2h_adj_net&
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void adj_step_kernel(AdjBatch b) {
     const int lane = threadIdx.x & (kWave - 1);
     if (static_cast<int>(blockIdx.x) >= kShortBlocks) {
-        int q;
-        const int base = adj_wave_base_at(b, static_cast<int>(blockIdx.x) - kShortBlocks, q);
+        // the quiet neurons, a lane each: with at most 8 networks, workgroup u (on XCD x = u mod 8) takes the
+        // (u / 8)-th neuron block that the mark kernel deals to XCD x, so that a neuron is updated on the XCD
+        // that gathers it (its adjoint, written here, is read there in the next steps)
+        const int u = static_cast<int>(blockIdx.x) - kShortBlocks;
+        int q, base;
+        if (b.n_nets <= kXcds) {
+            const int x = u % kXcds, k = u / kXcds;
+            q = 0;
+            while (q + 1 < b.n_nets && x >= adj_net_xcd0(q + 1, b.n_nets)) ++q;
+            const int bq = (x - adj_net_xcd0(q, b.n_nets)) + adj_net_xcds(q, b.n_nets) * k;
+            if (bq >= b.first_block[q + 1] - b.first_block[q]) return;
+            base = (bq * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6)) * kWave;
+        } else {
+            base = adj_wave_base_at(b, u, q);
+        }
         const n2h_adj_net& a = b.net[q];
         const int j = base + lane;
         if (j >= a.n_neurons) return;
@@ -1035,7 +1049,18 @@ int n2h_adj_step(const n2h_adj_net* nets, int32_t n_nets, int64_t step, int32_t 
         return fail(N2H_EINVAL, "win_step %d outside [0, long_delay = %d)", win_step, nets[0].long_delay);
     b.win_step = win_step;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(adj_step_kernel, dim3(static_cast<unsigned>(kShortBlocks + blocks)), dim3(kWave * kWavesPerBlock), 0,
+    // the quiet neurons' workgroups: with at most 8 networks, 8 times the most neuron blocks an XCD is dealt
+    int64_t quiet = blocks;
+    if (n_nets <= kXcds) {
+        int64_t most = 0;
+        for (int q = 0; q < n_nets; ++q) {
+            const int64_t nb = b.first_block[q + 1] - b.first_block[q];
+            const int m = adj_net_xcds(q, n_nets);
+            most = std::max(most, (nb + m - 1) / m);
+        }
+        quiet = kXcds * most;
+    }
+    hipLaunchKernelGGL(adj_step_kernel, dim3(static_cast<unsigned>(kShortBlocks + quiet)), dim3(kWave * kWavesPerBlock), 0,
                        s, b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(N2H_EHIP, "adjoint step kernels: %s", hipGetErrorString(e));
