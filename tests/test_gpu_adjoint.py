@@ -211,3 +211,42 @@ def test_compensated_trace_changes_nothing_and_tracks_fp64_potentials():
     e_plain = np.abs(plain_t - ref)[live].max() / 15.0
     _say(f"max |error| / (V_th - V_reset): compensated {e_comp:.3g}, fp32 {e_plain:.3g}")
     assert e_comp <= 3e-7 and e_plain > 3 * e_comp
+
+
+@pytest.mark.parametrize("kind", ["uneven3", "ten"])
+def test_batches_of_many_and_uneven_networks_equal_their_networks_alone(kind):
+    """The reverse step deals each network's gatherers to XCDs of its own (a contiguous share of the 8 with at
+    most 8 networks, XCD q mod 8 beyond) and sizes its work lists for the XCD dealt the most: three networks
+    of uneven sizes (2 % PD14, the 32-neuron subsample, 5 % PD14: XCD shares of 2, 3 and 3) and ten
+    subsamples (two networks on XCDs 0 and 1).  The batch's gradient equals the networks' own, added in
+    batch order, bit for bit."""
+    T = 120
+    alpha = _alpha(7)
+
+    def nets():
+        if kind == "uneven3":
+            made = [pd14(scale=0.02, seed=1, device="cuda", weight_scale=alpha, instance=0)[0],
+                    pd14(sizes=[4] * 8, prob_scale=10.0, seed=3, device="cuda", weight_scale=alpha, instance=1)[0],
+                    pd14(scale=0.05, seed=2, device="cuda", weight_scale=alpha, instance=2)[0]]
+        else:
+            made = [pd14(sizes=[4] * 8, prob_scale=10.0, seed=10 + i, device="cuda", weight_scale=alpha, instance=i)[0]
+                    for i in range(10)]
+        for n in made:
+            n.record = True
+            n.keep_canonical = True
+        return made
+
+    first = nets()
+    targets = np.random.default_rng(4).uniform(0.5, 10.0, (len(first), 8))
+    rts = [n.build() for n in first]
+    _say(f"case: {kind}, batch of {len(rts)} networks, sizes {[rt.n for rt in rts]}, T={T}")
+    loss, g = RateLossGrad(rts, alpha, checkpoint=40)(T, targets)
+    g_alone = np.zeros((8, 8), dtype=np.float32)
+    loss_alone = 0.0
+    for i, n in enumerate(nets()):
+        li, gi = RateLossGrad([n.build()], alpha, checkpoint=40)(T, targets[i:i + 1])
+        g_alone = g_alone + gi.astype(np.float32)
+        loss_alone += li
+    assert np.abs(g).max() > 0
+    assert np.array_equal(g_alone.astype(np.float64), g)
+    assert abs(loss - loss_alone) <= 1e-9 * max(1.0, abs(loss))
